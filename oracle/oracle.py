@@ -1,0 +1,336 @@
+"""ctypes front end of the FP64 CPU oracle (``liboracle.so``).
+
+TEST INFRASTRUCTURE ONLY: imported by ``tests/``, by ``__graft_entry__.smoke()`` and by the
+CPU-baseline leg of ``bench.py``. The GPU library never imports, calls or falls back to it.
+
+The oracle restates am1-rs 0.2.3's ``run_am1`` (src/scf.rs:294-468) and its callees in plain
+C++ double precision; see ``am1_oracle.cpp`` for the file:line each routine follows.
+"""
+from __future__ import annotations
+
+import ctypes
+import glob
+import math
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+METHODS = {"am1": 0, "rm1": 1}
+REFERENCES = {"auto": 0, "rhf": 1, "uhf": 2}
+ACCELERATORS = {"none": 0, "cdiis": 1, "adiis_cdiis": 2}
+ANGSTROM_TO_BOHR = 1.0 / 0.529167  # constants.rs:18
+
+
+class _Request(ctypes.Structure):
+    _fields_ = [
+        ("n_atoms", ctypes.c_int),
+        ("z", ctypes.POINTER(ctypes.c_int)),
+        ("xyz_bohr", ctypes.POINTER(ctypes.c_double)),
+        ("charge", ctypes.c_double),
+        ("multiplicity", ctypes.c_int),
+        ("method", ctypes.c_int),
+        ("reference", ctypes.c_int),
+        ("accelerator", ctypes.c_int),
+        ("max_scf", ctypes.c_int),
+        ("e_tol", ctypes.c_double),
+        ("p_tol", ctypes.c_double),
+        ("adiis_switch", ctypes.c_double),
+        ("use_diis", ctypes.c_int),
+    ]
+
+
+class _Result(ctypes.Structure):
+    _fields_ = [
+        ("status", ctypes.c_int),
+        ("nao", ctypes.c_int),
+        ("n_occ", ctypes.c_int),
+        ("n_occ_beta", ctypes.c_int),
+        ("iterations", ctypes.c_int),
+        ("converged", ctypes.c_int),
+        ("unrestricted", ctypes.c_int),
+        ("electronic_ev", ctypes.c_double),
+        ("core_ev", ctypes.c_double),
+        ("total_ev", ctypes.c_double),
+        ("heat_of_formation_kcal", ctypes.c_double),
+        ("homo_ev", ctypes.c_double),
+        ("lumo_ev", ctypes.c_double),
+        ("homo_beta_ev", ctypes.c_double),
+        ("lumo_beta_ev", ctypes.c_double),
+        ("dipole_debye", ctypes.c_double * 3),
+        ("residual", ctypes.c_double),
+        ("charges", ctypes.POINTER(ctypes.c_double)),
+        ("mo_energies", ctypes.POINTER(ctypes.c_double)),
+        ("density", ctypes.POINTER(ctypes.c_double)),
+        ("mo_coeff", ctypes.POINTER(ctypes.c_double)),
+        ("mo_energies_beta", ctypes.POINTER(ctypes.c_double)),
+        ("message", ctypes.c_char * 256),
+    ]
+
+
+_lib = None
+
+
+def build() -> str:
+    """Compile the oracle with its Makefile (gcc, OpenMP)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+def _find_openblas() -> str | None:
+    import numpy
+
+    base = os.path.dirname(numpy.__file__)
+    cands = glob.glob(os.path.join(base, "..", "numpy.libs", "libscipy_openblas*.so")) + glob.glob(
+        os.path.join(base, ".libs", "libscipy_openblas*.so")
+    )
+    return os.path.abspath(cands[0]) if cands else None
+
+
+def lib(use_lapack: bool = True, threads: int = 0):
+    """Load ``liboracle.so`` (building it if absent); optionally bind numpy's OpenBLAS LAPACK."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        L.oracle_scf.argtypes = [ctypes.POINTER(_Request), ctypes.POINTER(_Result)]
+        L.oracle_scf_many.argtypes = [ctypes.c_int, ctypes.POINTER(_Request), ctypes.POINTER(_Result), ctypes.c_int]
+        L.oracle_set_lapack.argtypes = [ctypes.c_char_p, ctypes.c_int]
+        L.oracle_element.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+        L.oracle_bintgs.argtypes = [ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
+        L.oracle_aintgs.argtypes = [ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
+        dp = ctypes.POINTER(ctypes.c_double)
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.oracle_overlap.argtypes = [ctypes.c_int, dp, ctypes.c_int, dp, ctypes.c_int, dp]
+        L.oracle_pair.argtypes = [ctypes.c_int, ctypes.c_int, dp, ctypes.c_int, dp, dp, dp, ip]
+        L.oracle_core.argtypes = [ctypes.c_int, ip, dp, ctypes.c_int, dp, dp, ip]
+        L.oracle_fock.argtypes = [ctypes.c_int, ip, dp, ctypes.c_int, dp, dp, ctypes.c_double, dp]
+        L.oracle_eigen.argtypes = [ctypes.c_int, dp, dp, dp]
+        L.oracle_diis_solve.argtypes = [ctypes.c_int, dp, dp]
+        L.oracle_adiis_simplex.argtypes = [ctypes.c_int, dp, dp, dp]
+        _lib = L
+        if use_lapack:
+            path = _find_openblas()
+            if path:
+                L.oracle_set_lapack(path.encode(), threads)
+    return _lib
+
+
+def lapack_active() -> bool:
+    return bool(lib().oracle_lapack_active())
+
+
+def set_lapack(enabled: bool, threads: int = 0) -> bool:
+    L = lib()
+    path = _find_openblas() if enabled else None
+    L.oracle_set_lapack(path.encode() if path else b"", threads)
+    return bool(L.oracle_lapack_active())
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+def _make_request(z, xyz_bohr, charge=0.0, multiplicity=1, method="am1", reference="auto",
+                  accelerator="adiis_cdiis", max_scf=200, e_tol=1e-8, p_tol=1e-7, adiis_switch=0.1,
+                  use_diis=True, keep=None):
+    z = np.ascontiguousarray(z, dtype=np.int32)
+    xyz = np.ascontiguousarray(xyz_bohr, dtype=np.float64).reshape(-1, 3)
+    if keep is not None:
+        keep.extend([z, xyz])
+    rq = _Request()
+    rq.n_atoms = len(z)
+    rq.z = _ip(z)
+    rq.xyz_bohr = _dp(xyz)
+    rq.charge = float(charge)
+    rq.multiplicity = int(multiplicity)
+    rq.method = METHODS[method] if isinstance(method, str) else int(method)
+    rq.reference = REFERENCES[reference] if isinstance(reference, str) else int(reference)
+    rq.accelerator = ACCELERATORS[accelerator] if isinstance(accelerator, str) else int(accelerator)
+    rq.max_scf = int(max_scf)
+    rq.e_tol = float(e_tol)
+    rq.p_tol = float(p_tol)
+    rq.adiis_switch = float(adiis_switch)
+    rq.use_diis = 1 if use_diis else 0
+    return rq
+
+
+def _result_dict(rs, extra):
+    out = {
+        "status": rs.status,
+        "message": rs.message.decode(errors="replace"),
+        "nao": rs.nao,
+        "n_occ": rs.n_occ,
+        "n_occ_beta": rs.n_occ_beta,
+        "iterations": rs.iterations,
+        "converged": bool(rs.converged),
+        "unrestricted": bool(rs.unrestricted),
+        "electronic_ev": rs.electronic_ev,
+        "core_ev": rs.core_ev,
+        "total_ev": rs.total_ev,
+        "heat_of_formation_kcal": rs.heat_of_formation_kcal,
+        "homo_ev": None if math.isnan(rs.homo_ev) else rs.homo_ev,
+        "lumo_ev": None if math.isnan(rs.lumo_ev) else rs.lumo_ev,
+        "homo_beta_ev": None if math.isnan(rs.homo_beta_ev) else rs.homo_beta_ev,
+        "lumo_beta_ev": None if math.isnan(rs.lumo_beta_ev) else rs.lumo_beta_ev,
+        "dipole_debye": np.array(list(rs.dipole_debye)),
+        "residual": rs.residual,
+    }
+    out.update(extra)
+    return out
+
+
+def scf(z, xyz_bohr, full=True, **kw):
+    """One SCF (``run_am1``); returns a dict of observables (arrays when ``full``)."""
+    L = lib()
+    keep = []
+    rq = _make_request(z, xyz_bohr, keep=keep, **kw)
+    n = len(z)
+    nao_max = 4 * n
+    rs = _Result()
+    charges = np.zeros(n)
+    rs.charges = _dp(charges)
+    extra = {"charges": charges}
+    if full:
+        mo = np.zeros(nao_max)
+        mob = np.zeros(nao_max)
+        dens = np.zeros(nao_max * nao_max)
+        coeff = np.zeros(nao_max * nao_max)
+        rs.mo_energies = _dp(mo)
+        rs.mo_energies_beta = _dp(mob)
+        rs.density = _dp(dens)
+        rs.mo_coeff = _dp(coeff)
+    L.oracle_scf(ctypes.byref(rq), ctypes.byref(rs))
+    if full:
+        nao = rs.nao
+        extra.update(
+            mo_energies=mo[:nao].copy(),
+            mo_energies_beta=mob[:nao].copy() if rs.unrestricted else None,
+            density=dens[: nao * nao].reshape(nao, nao).copy(),
+            mo_coeff=coeff[: nao * nao].reshape(nao, nao).copy(),
+        )
+    return _result_dict(rs, extra)
+
+
+def scf_many(molecules, threads=0, **kw):
+    """Independent SCFs, OpenMP over molecules. ``molecules`` = list of (z, xyz_bohr)."""
+    L = lib()
+    keep = []
+    n = len(molecules)
+    rqs = (_Request * n)()
+    rss = (_Result * n)()
+    charges = []
+    for i, (z, xyz) in enumerate(molecules):
+        rqs[i] = _make_request(z, xyz, keep=keep, **kw)
+        q = np.zeros(len(z))
+        charges.append(q)
+        rss[i].charges = _dp(q)
+    L.oracle_scf_many(n, rqs, rss, int(threads))
+    return [_result_dict(rss[i], {"charges": charges[i]}) for i in range(n)]
+
+
+def element(z, method="am1"):
+    out = np.zeros(23)
+    st = lib().oracle_element(int(z), METHODS[method], _dp(out))
+    if st:
+        return None
+    keys = ["n", "norb", "uss", "upp", "zs", "zp", "bs", "bp", "gss", "gsp", "gpp", "gp2", "hsp", "alpha",
+            "core", "eheat_ev", "e_isol", "dd", "qq", "rho0", "rho1", "rho2", "ng"]
+    return dict(zip(keys, out.tolist()))
+
+
+def bintgs(x):
+    out = np.zeros(13)
+    lib().oracle_bintgs(float(x), _dp(out))
+    return out
+
+
+def aintgs(x):
+    out = np.zeros(13)
+    lib().oracle_aintgs(float(x), _dp(out))
+    return out
+
+
+def overlap(zi, pi, zj, pj, method="am1"):
+    out = np.zeros(16)
+    pi = np.ascontiguousarray(pi, dtype=np.float64)
+    pj = np.ascontiguousarray(pj, dtype=np.float64)
+    st = lib().oracle_overlap(int(zi), _dp(pi), int(zj), _dp(pj), METHODS[method], _dp(out))
+    if st:
+        raise ValueError("overlap not available for these elements")
+    return out.reshape(4, 4)
+
+
+def pair(za, zb, dvec, method="am1"):
+    """Two-centre integrals of the ordered pair (a at origin, b at dvec, Bohr)."""
+    w = np.zeros(100)
+    e1b = np.zeros(16)
+    e2a = np.zeros(16)
+    dims = np.zeros(3, dtype=np.int32)
+    d = np.ascontiguousarray(dvec, dtype=np.float64)
+    lib().oracle_pair(int(za), int(zb), _dp(d), METHODS[method], _dp(w), _dp(e1b), _dp(e2a), _ip(dims))
+    na, nb, wcols = dims.tolist()
+    rows = 10 if na == 4 else 1
+    return {"w": w[: rows * wcols].reshape(rows, wcols), "e1b": e1b.reshape(4, 4), "e2a": e2a.reshape(4, 4),
+            "na": na, "nb": nb}
+
+
+def core(z, xyz_bohr, method="am1"):
+    z = np.ascontiguousarray(z, dtype=np.int32)
+    xyz = np.ascontiguousarray(xyz_bohr, dtype=np.float64)
+    nao_max = 4 * len(z)
+    H = np.zeros(nao_max * nao_max)
+    e = np.zeros(1)
+    nao = np.zeros(1, dtype=np.int32)
+    st = lib().oracle_core(len(z), _ip(z), _dp(xyz), METHODS[method], _dp(H), _dp(e), _ip(nao))
+    if st:
+        raise ValueError(f"oracle_core status {st}")
+    n = int(nao[0])
+    return H[: n * n].reshape(n, n).copy(), float(e[0])
+
+
+def fock(z, xyz_bohr, p_tot, p_spin=None, spin_scale=0.5, method="am1"):
+    z = np.ascontiguousarray(z, dtype=np.int32)
+    xyz = np.ascontiguousarray(xyz_bohr, dtype=np.float64)
+    pt = np.ascontiguousarray(p_tot, dtype=np.float64)
+    ps = pt if p_spin is None else np.ascontiguousarray(p_spin, dtype=np.float64)
+    F = np.zeros_like(pt)
+    st = lib().oracle_fock(len(z), _ip(z), _dp(xyz), METHODS[method], _dp(pt), _dp(ps), float(spin_scale), _dp(F))
+    if st:
+        raise ValueError(f"oracle_fock status {st}")
+    return F
+
+
+def eigen(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    n = a.shape[0]
+    w = np.zeros(n)
+    v = np.zeros((n, n))
+    st = lib().oracle_eigen(n, _dp(a), _dp(w), _dp(v))
+    if st:
+        raise ValueError("eigensolver rejected the input")
+    return w, v
+
+
+def diis_solve(B):
+    B = np.ascontiguousarray(B, dtype=np.float64)
+    m = B.shape[0]
+    c = np.zeros(m + 1)
+    st = lib().oracle_diis_solve(m, _dp(B), _dp(c))
+    return None if st else c
+
+
+def adiis_simplex(d, s):
+    d = np.ascontiguousarray(d, dtype=np.float64)
+    s = np.ascontiguousarray(s, dtype=np.float64)
+    c = np.zeros(len(d))
+    lib().oracle_adiis_simplex(len(d), _dp(d), _dp(s), _dp(c))
+    return c
