@@ -1,0 +1,317 @@
+"""am1g — MI355X-native AM1/RM1 SCF engine (Python front end over the C ABI in include/am1g.h).
+
+``single_point`` mirrors am1-rs's Python entry point ``am1_rs.native.single_point``
+(python/am1_rs/native.py:52-97 -> src/python.rs:147-199): coordinates in Angstrom, result dict
+in eV and Hartree, every error raised as ``ValueError`` with the engine's message
+(python.rs:27-29). ``single_point_batch`` runs many molecules in one persistent GPU launch.
+
+The engine is ``libam1g.so`` built from ``csrc/`` for gfx950. There is no CPU fallback: when the
+library or a GPU is missing, loading fails with an error that says so.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libam1g.so")
+CSRC = os.path.join(os.path.dirname(HERE), "csrc")
+
+AM1_EV = 27.21            # constants.rs:13
+ANGSTROM_TO_BOHR = 1.0 / 0.529167  # constants.rs:18
+
+METHODS = {"am1": 0, "rm1": 1}
+REFERENCES = {"auto": 0, "rhf": 1, "uhf": 2, "restricted": 1, "unrestricted": 2}
+ACCELERATORS = {"none": 0, "cdiis": 1, "adiis_cdiis": 2, "adiis": 2}
+PRECISIONS = {"fp64": 0, "mixed": 1}
+STATUS_NAMES = {0: "Ok", 1: "InvalidInput", 2: "ElementNotParameterized", 3: "LinearAlgebra",
+                4: "ScfNotConverged", 5: "ScfDiverged", 6: "DeviceError"}
+SYMBOLS = {"H": 1, "Be": 4, "B": 5, "C": 6, "N": 7, "O": 8, "F": 9, "Al": 13, "Si": 14, "P": 15, "S": 16,
+           "Cl": 17, "Zn": 30, "Ge": 32, "As": 33, "Se": 34, "Br": 35, "Sb": 51, "Te": 52, "I": 53, "Hg": 80}
+
+
+class Request(ctypes.Structure):
+    _fields_ = [
+        ("n_atoms", ctypes.c_int),
+        ("z", ctypes.POINTER(ctypes.c_uint8)),
+        ("xyz_bohr", ctypes.POINTER(ctypes.c_double)),
+        ("charge", ctypes.c_double),
+        ("multiplicity", ctypes.c_int),
+        ("method", ctypes.c_int),
+        ("reference", ctypes.c_int),
+        ("accelerator", ctypes.c_int),
+        ("use_diis", ctypes.c_int),
+        ("max_scf", ctypes.c_int),
+        ("e_tol", ctypes.c_double),
+        ("p_tol", ctypes.c_double),
+        ("adiis_switch", ctypes.c_double),
+        ("precision", ctypes.c_int),
+    ]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [
+        ("status", ctypes.c_int),
+        ("nao", ctypes.c_int),
+        ("n_occ", ctypes.c_int),
+        ("n_occ_beta", ctypes.c_int),
+        ("iterations", ctypes.c_int),
+        ("converged", ctypes.c_int),
+        ("unrestricted", ctypes.c_int),
+        ("has_homo", ctypes.c_int),
+        ("has_lumo", ctypes.c_int),
+        ("has_homo_beta", ctypes.c_int),
+        ("has_lumo_beta", ctypes.c_int),
+        ("electronic_ev", ctypes.c_double),
+        ("core_ev", ctypes.c_double),
+        ("total_ev", ctypes.c_double),
+        ("heat_of_formation_kcal", ctypes.c_double),
+        ("homo_ev", ctypes.c_double),
+        ("lumo_ev", ctypes.c_double),
+        ("homo_beta_ev", ctypes.c_double),
+        ("lumo_beta_ev", ctypes.c_double),
+        ("dipole_debye", ctypes.c_double * 3),
+        ("dipole_magnitude", ctypes.c_double),
+        ("residual", ctypes.c_double),
+        ("fp32_iterations", ctypes.c_int),
+        ("charges", ctypes.POINTER(ctypes.c_double)),
+        ("mo_energies", ctypes.POINTER(ctypes.c_double)),
+        ("mo_energies_beta", ctypes.POINTER(ctypes.c_double)),
+        ("density", ctypes.POINTER(ctypes.c_double)),
+        ("mo_coeff", ctypes.POINTER(ctypes.c_double)),
+    ]
+
+
+# Every symbol include/am1g.h declares (checked by the CPU test-suite against the built library).
+EXPORTS = ["am1g_request_defaults", "am1g_create", "am1g_destroy", "am1g_scf", "am1g_scf_batch",
+           "am1g_last_error", "am1g_element_info", "am1g_last_kernel_ms"]
+
+_lib = None
+
+
+def build(verbose: bool = False) -> str:
+    """Compile ``libam1g.so`` for gfx950 with the Makefile in ``csrc/``."""
+    subprocess.run(["make", "-C", CSRC] + ([] if verbose else ["-s"]), check=True)
+    return LIB_PATH
+
+
+def load():
+    """Load ``libam1g.so``. Raises ``RuntimeError`` when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"am1g: {LIB_PATH} is missing; build it with __graft_entry__.build() "
+                           "(there is no CPU fallback)")
+    L = ctypes.CDLL(LIB_PATH)
+    L.am1g_create.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+    L.am1g_destroy.argtypes = [ctypes.c_void_p]
+    L.am1g_scf.argtypes = [ctypes.c_void_p, ctypes.POINTER(Request), ctypes.POINTER(Result)]
+    L.am1g_scf_batch.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(Request), ctypes.POINTER(Result)]
+    L.am1g_last_error.argtypes = [ctypes.c_void_p]
+    L.am1g_last_error.restype = ctypes.c_char_p
+    L.am1g_request_defaults.argtypes = [ctypes.POINTER(Request)]
+    L.am1g_element_info.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    L.am1g_last_kernel_ms.argtypes = [ctypes.c_void_p]
+    L.am1g_last_kernel_ms.restype = ctypes.c_double
+    _lib = L
+    return L
+
+
+def element_info(z: int, method: str = "am1"):
+    out = (ctypes.c_double * 23)()
+    st = load().am1g_element_info(int(z), METHODS[method], out)
+    if st:
+        return None
+    keys = ["n", "norb", "uss", "upp", "zs", "zp", "bs", "bp", "gss", "gsp", "gpp", "gp2", "hsp", "alpha",
+            "core", "eheat_ev", "e_isol", "dd", "qq", "rho0", "rho1", "rho2", "ng"]
+    return dict(zip(keys, list(out)))
+
+
+class Engine:
+    """One am1g context on one GPU (owns the device pools and stream)."""
+
+    def __init__(self, device: int = 0):
+        self._L = load()
+        self._ctx = ctypes.c_void_p()
+        st = self._L.am1g_create(int(device), ctypes.byref(self._ctx))
+        if st != 0:
+            msg = self._L.am1g_last_error(None)
+            raise RuntimeError(f"am1g_create failed ({STATUS_NAMES.get(st, st)}): {msg.decode() if msg else ''}")
+
+    def close(self):
+        if self._ctx:
+            self._L.am1g_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def last_error(self) -> str:
+        m = self._L.am1g_last_error(self._ctx)
+        return m.decode() if m else ""
+
+    def last_kernel_ms(self) -> float:
+        return float(self._L.am1g_last_kernel_ms(self._ctx))
+
+    # -- request marshalling -------------------------------------------------------------
+    def _request(self, z, xyz_bohr, keep, charge=0.0, multiplicity=1, method="am1", reference="auto",
+                 accelerator="adiis_cdiis", use_diis=True, max_scf=200, e_tol=1e-8, p_tol=1e-7,
+                 adiis_switch=0.1, precision="fp64"):
+        rq = Request()
+        self._L.am1g_request_defaults(ctypes.byref(rq))
+        za = np.ascontiguousarray(z, dtype=np.uint8)
+        xa = np.ascontiguousarray(xyz_bohr, dtype=np.float64).reshape(-1, 3)
+        keep.extend([za, xa])
+        rq.n_atoms = len(za)
+        rq.z = za.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+        rq.xyz_bohr = xa.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        rq.charge = float(charge)
+        rq.multiplicity = int(multiplicity)
+        rq.method = METHODS[method] if isinstance(method, str) else int(method)
+        rq.reference = REFERENCES[reference] if isinstance(reference, str) else int(reference)
+        rq.accelerator = ACCELERATORS[accelerator] if isinstance(accelerator, str) else int(accelerator)
+        rq.use_diis = 1 if use_diis else 0
+        rq.max_scf = int(max_scf)
+        rq.e_tol = float(e_tol)
+        rq.p_tol = float(p_tol)
+        rq.adiis_switch = float(adiis_switch)
+        rq.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        return rq
+
+    @staticmethod
+    def _attach(rs, n_atoms, full, keep):
+        q = np.zeros(n_atoms)
+        keep.append(q)
+        rs.charges = q.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        arrays = {"charges": q}
+        if full:
+            nao_max = 4 * n_atoms
+            mo = np.zeros(nao_max)
+            mob = np.zeros(nao_max)
+            dens = np.zeros(nao_max * nao_max)
+            coeff = np.zeros(nao_max * nao_max)
+            keep.extend([mo, mob, dens, coeff])
+            rs.mo_energies = mo.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+            rs.mo_energies_beta = mob.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+            rs.density = dens.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+            rs.mo_coeff = coeff.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+            arrays.update(mo=mo, mob=mob, dens=dens, coeff=coeff)
+        return arrays
+
+    @staticmethod
+    def _to_dict(rs, arrays, full):
+        d = {
+            "status": rs.status,
+            "status_name": STATUS_NAMES.get(rs.status, str(rs.status)),
+            "nao": rs.nao,
+            "n_occ": rs.n_occ,
+            "n_occ_beta": rs.n_occ_beta,
+            "iterations": rs.iterations,
+            "converged": bool(rs.converged),
+            "unrestricted": bool(rs.unrestricted),
+            "electronic_ev": rs.electronic_ev,
+            "core_ev": rs.core_ev,
+            "total_ev": rs.total_ev,
+            "heat_of_formation_kcal": rs.heat_of_formation_kcal,
+            "homo_ev": rs.homo_ev if rs.has_homo else None,
+            "lumo_ev": rs.lumo_ev if rs.has_lumo else None,
+            "homo_beta_ev": rs.homo_beta_ev if rs.has_homo_beta else None,
+            "lumo_beta_ev": rs.lumo_beta_ev if rs.has_lumo_beta else None,
+            "dipole_debye": np.array(list(rs.dipole_debye)),
+            "dipole_magnitude": rs.dipole_magnitude,
+            "residual": rs.residual,
+            "fp32_iterations": rs.fp32_iterations,
+            "charges": arrays["charges"].copy(),
+        }
+        if full and rs.status in (0, 4):
+            n = rs.nao
+            d["mo_energies"] = arrays["mo"][:n].copy()
+            d["mo_energies_beta"] = arrays["mob"][:n].copy() if rs.unrestricted else None
+            d["density"] = arrays["dens"][: n * n].reshape(n, n).copy()
+            d["mo_coeff"] = arrays["coeff"][: n * n].reshape(n, n).copy()
+        return d
+
+    def scf(self, z, xyz_bohr, full: bool = True, **kw) -> dict:
+        """One SCF on the plain path (``am1g_scf``)."""
+        keep = []
+        rq = self._request(z, xyz_bohr, keep, **kw)
+        rs = Result()
+        arrays = self._attach(rs, rq.n_atoms, full, keep)
+        self._L.am1g_scf(self._ctx, ctypes.byref(rq), ctypes.byref(rs))
+        d = self._to_dict(rs, arrays, full)
+        if rs.status not in (0, 4):
+            d["message"] = self.last_error()
+        return d
+
+    def scf_batch(self, molecules, full: bool = False, **kw) -> list:
+        """Many SCFs in one persistent launch (``am1g_scf_batch``). ``molecules`` is a list of
+        ``(z, xyz_bohr)`` or ``(z, xyz_bohr, per_molecule_kwargs)``."""
+        keep = []
+        n = len(molecules)
+        rqs = (Request * n)()
+        rss = (Result * n)()
+        arrays = []
+        for i, mol in enumerate(molecules):
+            z, xyz = mol[0], mol[1]
+            opts = dict(kw)
+            if len(mol) > 2 and mol[2]:
+                opts.update(mol[2])
+            rqs[i] = self._request(z, xyz, keep, **opts)
+            arrays.append(self._attach(rss[i], len(z), full, keep))
+        st = self._L.am1g_scf_batch(self._ctx, n, rqs, rss)
+        if st != 0:
+            raise RuntimeError(f"am1g_scf_batch failed ({STATUS_NAMES.get(st, st)}): {self.last_error()}")
+        return [self._to_dict(rss[i], arrays[i], full) for i in range(n)]
+
+
+_default_engine = None
+
+
+def engine() -> Engine:
+    global _default_engine
+    if _default_engine is None:
+        _default_engine = Engine(0)
+    return _default_engine
+
+
+def single_point(symbols_or_z, positions_angstrom, charge=0.0, multiplicity=1, method="am1", max_scf=200,
+                 e_tol=1e-8, p_tol=1e-7, use_diis=True, accelerator="adiis_cdiis", adiis_switch=0.1,
+                 reference="auto", precision="fp64"):
+    """``am1_rs.native.single_point`` counterpart: Angstrom in, the reference's result-dict keys out
+    (python.rs:166-198; energies in eV and Hartree). Errors raise ``ValueError``."""
+    z = [SYMBOLS[s] if isinstance(s, str) else int(s) for s in symbols_or_z]
+    xyz = np.asarray(positions_angstrom, dtype=np.float64).reshape(-1, 3) * ANGSTROM_TO_BOHR
+    r = engine().scf(z, xyz, full=True, charge=charge, multiplicity=multiplicity, method=method,
+                     max_scf=max_scf, e_tol=e_tol, p_tol=p_tol, use_diis=use_diis, accelerator=accelerator,
+                     adiis_switch=adiis_switch, reference=reference, precision=precision)
+    if r["status"] == 4:
+        raise ValueError(f"SCF did not converge in {r['iterations']} iterations "
+                         f"(residual {r['residual']:.3e})")
+    if r["status"] != 0:
+        raise ValueError(r.get("message", r["status_name"]))
+    return {
+        "energy": r["total_ev"] / AM1_EV,
+        "energy_ev": r["total_ev"],
+        "electronic_energy": r["electronic_ev"] / AM1_EV,
+        "electronic_energy_ev": r["electronic_ev"],
+        "core_repulsion": r["core_ev"] / AM1_EV,
+        "core_repulsion_ev": r["core_ev"],
+        "heat_of_formation_kcal": r["heat_of_formation_kcal"],
+        "charges": r["charges"].tolist(),
+        "dipole_debye": r["dipole_debye"].tolist(),
+        "dipole_magnitude_debye": r["dipole_magnitude"],
+        "homo_ev": r["homo_ev"],
+        "lumo_ev": r["lumo_ev"],
+        "mo_energies_ev": r["mo_energies"].tolist(),
+        "iterations": r["iterations"],
+        "converged": r["converged"],
+        "unrestricted": r["unrestricted"],
+    }

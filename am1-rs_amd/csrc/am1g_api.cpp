@@ -1,0 +1,431 @@
+// am1g host runtime: context, device pools, request validation and the C ABI of include/am1g.h.
+//
+// Validation and result assembly follow am1-rs run_am1 (src/scf.rs:294-468): electron count and
+// multiplicity parity, RHF/UHF selection, ScfNotConverged reported after every output is filled.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/am1g.h"
+#include "am1g_common.hpp"
+#include "am1g_launch.hpp"
+
+using namespace am1g;
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    hipError_t reserve(size_t need) {
+        if (need <= bytes) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        size_t want = std::max(need, (size_t)4096);
+        hipError_t e = hipMalloc(&p, want);
+        if (e == hipSuccess) bytes = want;
+        return e;
+    }
+    template <typename T>
+    T* as() const {
+        return static_cast<T*>(p);
+    }
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+};
+
+struct am1g_ctx {
+    int device = 0;
+    int cus = 256;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0.0;
+    std::mutex mu;
+    std::string err;
+    DevElem host_tab[kMethods * kMaxZ];
+    DevBuf elems, mols, z, xyz, mats, wpool, hist, mo_e, mo_eb, charges, results, queue, order;
+};
+
+namespace {
+
+thread_local std::string g_tls_error;
+
+int set_error(am1g_ctx* ctx, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (ctx) ctx->err = buf;
+    g_tls_error = buf;
+    return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                                              \
+    do {                                                                                                \
+        hipError_t e_ = (expr);                                                                         \
+        if (e_ != hipSuccess)                                                                           \
+            return set_error(ctx, AM1G_DEVICE_ERROR, "HIP error %s at %s", hipGetErrorString(e_), #expr); \
+    } while (0)
+
+struct Prepared {
+    MolDesc md;
+    int nat;
+    bool uhf;
+};
+
+// run_am1's input checks (scf.rs:300-366) plus the element checks of Basis::build.
+int prepare(am1g_ctx* ctx, const am1g_request& rq, Prepared& out, std::string& msg) {
+    char buf[256];
+    if (rq.n_atoms < 1 || !rq.z || !rq.xyz_bohr) {
+        msg = "molecule has no atoms";
+        return AM1G_INVALID_INPUT;
+    }
+    if (rq.n_atoms > kMaxAtoms) {
+        std::snprintf(buf, sizeof(buf), "molecule has %d atoms; this entry point takes at most %d", rq.n_atoms,
+                      kMaxAtoms);
+        msg = buf;
+        return AM1G_INVALID_INPUT;
+    }
+    if (rq.multiplicity < 1) {
+        msg = "multiplicity must be >= 1";
+        return AM1G_INVALID_INPUT;
+    }
+    if (rq.method != 0 && rq.method != 1) {
+        msg = "unknown method";
+        return AM1G_INVALID_INPUT;
+    }
+    const DevElem* tab = ctx->host_tab + rq.method * kMaxZ;
+    int nao = 0, nh = 0, nH = 0;
+    double n_elec = 0.0;
+    for (int i = 0; i < rq.n_atoms; ++i) {
+        int zz = rq.z[i];
+        if (zz <= 0 || zz >= kMaxZ || !tab[zz].valid) {
+            std::snprintf(buf, sizeof(buf), "%s has no parameters for element Z=%d",
+                          rq.method == 0 ? "AM1" : "RM1", zz);
+            msg = buf;
+            return AM1G_ELEMENT_NOT_PARAMETERIZED;
+        }
+        if (tab[zz].n > 3) {
+            std::snprintf(buf, sizeof(buf), "element Z=%d needs the numeric n>=4 Slater overlap, not provided", zz);
+            msg = buf;
+            return AM1G_ELEMENT_NOT_PARAMETERIZED;
+        }
+        nao += tab[zz].norb;
+        (zz == 1 ? nH : nh)++;
+        n_elec += tab[zz].core;
+        for (int k = 0; k < 3; ++k)
+            if (!std::isfinite(rq.xyz_bohr[3 * i + k])) {
+                msg = "non-finite coordinate";
+                return AM1G_INVALID_INPUT;
+            }
+    }
+    n_elec -= rq.charge;
+    long long ne = std::llround(n_elec);
+    if (std::fabs(n_elec - (double)ne) > 1.0e-6 || ne < 0) {
+        std::snprintf(buf, sizeof(buf), "invalid electron count %g", n_elec);
+        msg = buf;
+        return AM1G_INVALID_INPUT;
+    }
+    long long nun = rq.multiplicity - 1;
+    if (ne - nun < 0 || (ne - nun) % 2 != 0) {
+        std::snprintf(buf, sizeof(buf),
+                      "electron count %lld is incompatible with multiplicity %d (need same parity)", ne,
+                      rq.multiplicity);
+        msg = buf;
+        return AM1G_INVALID_INPUT;
+    }
+    int na = (int)((ne + nun) / 2), nb = (int)((ne - nun) / 2);
+    bool closed = na == nb;
+    bool uhf;
+    if (rq.reference == AM1G_REF_AUTO) uhf = !closed;
+    else if (rq.reference == AM1G_REF_UHF) uhf = true;
+    else {
+        if (!closed) {
+            std::snprintf(buf, sizeof(buf),
+                          "restricted (RHF) reference requires a closed shell, but multiplicity %d gives %d alpha "
+                          "and %d beta electrons; use the unrestricted (UHF) reference for open-shell systems",
+                          rq.multiplicity, na, nb);
+            msg = buf;
+            return AM1G_INVALID_INPUT;
+        }
+        uhf = false;
+    }
+    if (nao > kMaxBatchAo) {
+        std::snprintf(buf, sizeof(buf), "%d basis functions; the per-molecule SCF kernel takes at most %d", nao,
+                      kMaxBatchAo);
+        msg = buf;
+        return AM1G_INVALID_INPUT;
+    }
+    if (na > nao || nb > nao) {
+        msg = "more electrons than the basis can hold";
+        return AM1G_INVALID_INPUT;
+    }
+    MolDesc& md = out.md;
+    std::memset(&md, 0, sizeof(md));
+    md.nat = rq.n_atoms;
+    md.nao = nao;
+    md.n_alpha = na;
+    md.n_beta = nb;
+    md.method = rq.method;
+    md.uhf = uhf ? 1 : 0;
+    md.accel = rq.use_diis ? rq.accelerator : 0;
+    md.max_scf = rq.max_scf;
+    md.precision = rq.precision;
+    md.n_heavy = nh;
+    md.n_h = nH;
+    md.e_tol = rq.e_tol;
+    md.p_tol = rq.p_tol;
+    md.adiis_switch = rq.adiis_switch;
+    out.nat = rq.n_atoms;
+    out.uhf = uhf;
+    return AM1G_OK;
+}
+
+void fill_result(const MolResult& r, const MolDesc& md, am1g_result& out) {
+    out.status = r.status;
+    out.nao = md.nao;
+    out.n_occ = md.n_alpha;
+    out.n_occ_beta = md.n_beta;
+    out.iterations = r.iterations;
+    out.converged = r.converged;
+    out.unrestricted = md.uhf;
+    out.electronic_ev = r.electronic_ev;
+    out.core_ev = r.core_ev;
+    out.total_ev = r.total_ev;
+    out.heat_of_formation_kcal = r.heat_of_formation_kcal;
+    out.homo_ev = r.homo_ev;
+    out.lumo_ev = r.lumo_ev;
+    out.homo_beta_ev = r.homo_beta_ev;
+    out.lumo_beta_ev = r.lumo_beta_ev;
+    out.has_homo = std::isfinite(r.homo_ev);
+    out.has_lumo = std::isfinite(r.lumo_ev);
+    out.has_homo_beta = md.uhf && std::isfinite(r.homo_beta_ev);
+    out.has_lumo_beta = md.uhf && std::isfinite(r.lumo_beta_ev);
+    for (int k = 0; k < 3; ++k) out.dipole_debye[k] = r.dipole[k];
+    out.dipole_magnitude = std::sqrt(r.dipole[0] * r.dipole[0] + r.dipole[1] * r.dipole[1] + r.dipole[2] * r.dipole[2]);
+    out.residual = r.residual;
+    out.fp32_iterations = (int)r.fp32_iterations;
+}
+
+}  // namespace
+
+extern "C" {
+
+void am1g_request_defaults(am1g_request* rq) {
+    std::memset(rq, 0, sizeof(*rq));
+    rq->multiplicity = 1;
+    rq->method = AM1G_METHOD_AM1;
+    rq->reference = AM1G_REF_AUTO;
+    rq->accelerator = AM1G_ACCEL_ADIIS_CDIIS;
+    rq->use_diis = 1;
+    rq->max_scf = 200;
+    rq->e_tol = 1.0e-8;
+    rq->p_tol = 1.0e-7;
+    rq->adiis_switch = 0.1;
+    rq->precision = AM1G_PRECISION_FP64;
+}
+
+int am1g_create(int device, am1g_ctx** out) {
+    if (!out) return set_error(nullptr, AM1G_INVALID_INPUT, "null output pointer");
+    *out = nullptr;
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0)
+        return set_error(nullptr, AM1G_DEVICE_ERROR, "no HIP device available (%s)", hipGetErrorString(e));
+    if (device < 0 || device >= count)
+        return set_error(nullptr, AM1G_INVALID_INPUT, "device %d out of range (%d devices)", device, count);
+    am1g_ctx* ctx = new am1g_ctx();
+    ctx->device = device;
+    HIP_TRY(ctx, hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HIP_TRY(ctx, hipGetDeviceProperties(&prop, device));
+    ctx->cus = prop.multiProcessorCount;
+    HIP_TRY(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    HIP_TRY(ctx, hipEventCreate(&ctx->ev0));
+    HIP_TRY(ctx, hipEventCreate(&ctx->ev1));
+    build_element_tables(ctx->host_tab);
+    HIP_TRY(ctx, ctx->elems.reserve(sizeof(ctx->host_tab)));
+    HIP_TRY(ctx, hipMemcpy(ctx->elems.p, ctx->host_tab, sizeof(ctx->host_tab), hipMemcpyHostToDevice));
+    *out = ctx;
+    return AM1G_OK;
+}
+
+void am1g_destroy(am1g_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char* am1g_last_error(const am1g_ctx* ctx) { return ctx ? ctx->err.c_str() : g_tls_error.c_str(); }
+
+double am1g_last_kernel_ms(const am1g_ctx* ctx) { return ctx ? ctx->last_ms : 0.0; }
+
+int am1g_element_info(int z, int method, double* out) {
+    static DevElem tab[kMethods * kMaxZ];
+    static std::once_flag once;
+    std::call_once(once, [] { build_element_tables(tab); });
+    if (method < 0 || method >= kMethods || z <= 0 || z >= kMaxZ || !tab[method * kMaxZ + z].valid)
+        return AM1G_ELEMENT_NOT_PARAMETERIZED;
+    const DevElem& e = tab[method * kMaxZ + z];
+    double v[23] = {(double)e.n, (double)e.norb, e.uss, e.upp, e.zs, e.zp, e.bs, e.bp, e.gss, e.gsp, e.gpp, e.gp2,
+                    e.hsp, e.alpha, e.core, e.eheat_ev, e.e_isol, e.dd, e.qq, e.rho0, e.rho1, e.rho2, (double)e.ng};
+    std::memcpy(out, v, sizeof(v));
+    return AM1G_OK;
+}
+
+int am1g_scf_batch(am1g_ctx* ctx, int n, const am1g_request* rqs, am1g_result* results) {
+    if (!ctx) return set_error(nullptr, AM1G_INVALID_INPUT, "null context");
+    if (n < 0 || (n > 0 && (!rqs || !results))) return set_error(ctx, AM1G_INVALID_INPUT, "bad batch arguments");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    std::vector<Prepared> prep(n);
+    std::vector<int> ok_idx;
+    ok_idx.reserve(n);
+    for (int i = 0; i < n; ++i) {
+        std::string msg;
+        int st = prepare(ctx, rqs[i], prep[i], msg);
+        am1g_result& r = results[i];
+        if (st != AM1G_OK) {
+            r.status = st;
+            r.iterations = 0;
+            r.converged = 0;
+            set_error(ctx, st, "%s", msg.c_str());
+            continue;
+        }
+        if (prep[i].uhf) {
+            r.status = AM1G_INVALID_INPUT;
+            set_error(ctx, AM1G_INVALID_INPUT, "unrestricted SCF is not available on this path yet");
+            continue;
+        }
+        ok_idx.push_back(i);
+    }
+    const int m = (int)ok_idx.size();
+    if (m == 0) return AM1G_OK;
+    // pool layout
+    std::vector<MolDesc> descs(m);
+    std::vector<int> zs;
+    std::vector<double> xyz;
+    int64_t atom_off = 0, ao_off = 0, mat_off = 0, w_off = 0, hist_off = 0;
+    int max_nao = 0;
+    for (int j = 0; j < m; ++j) {
+        const am1g_request& rq = rqs[ok_idx[j]];
+        MolDesc md = prep[ok_idx[j]].md;
+        md.atom_off = atom_off;
+        md.ao_off = ao_off;
+        md.mat_off = mat_off;
+        md.w_off = w_off;
+        md.hist_off = hist_off;
+        for (int a = 0; a < rq.n_atoms; ++a) {
+            zs.push_back(rq.z[a]);
+            for (int k = 0; k < 3; ++k) xyz.push_back(rq.xyz_bohr[3 * a + k]);
+        }
+        atom_off += rq.n_atoms;
+        ao_off += md.nao;
+        mat_off += (int64_t)(md.uhf ? kMatSlotsUhf : kMatSlotsRhf) * md.nao * md.nao;
+        w_off += w_pool_size(md.n_heavy, md.n_h);
+        hist_off += hist_size(md.nao, md.uhf);
+        max_nao = std::max(max_nao, md.nao);
+        descs[j] = md;
+    }
+    // largest molecules first so the tail of the persistent launch is short
+    std::vector<int> order(m);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return descs[a].nao > descs[b].nao; });
+
+    HIP_TRY(ctx, ctx->mols.reserve(sizeof(MolDesc) * m));
+    HIP_TRY(ctx, ctx->z.reserve(sizeof(int) * zs.size()));
+    HIP_TRY(ctx, ctx->xyz.reserve(sizeof(double) * xyz.size()));
+    HIP_TRY(ctx, ctx->mats.reserve(sizeof(double) * mat_off));
+    HIP_TRY(ctx, ctx->wpool.reserve(sizeof(double) * std::max<int64_t>(w_off, 1)));
+    HIP_TRY(ctx, ctx->hist.reserve(sizeof(double) * hist_off));
+    HIP_TRY(ctx, ctx->mo_e.reserve(sizeof(double) * ao_off));
+    HIP_TRY(ctx, ctx->mo_eb.reserve(sizeof(double) * ao_off));
+    HIP_TRY(ctx, ctx->charges.reserve(sizeof(double) * atom_off));
+    HIP_TRY(ctx, ctx->results.reserve(sizeof(MolResult) * m));
+    HIP_TRY(ctx, ctx->queue.reserve(sizeof(int) * 4));
+    HIP_TRY(ctx, ctx->order.reserve(sizeof(int) * m));
+    hipStream_t st = ctx->stream;
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->mols.p, descs.data(), sizeof(MolDesc) * m, hipMemcpyHostToDevice, st));
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->z.p, zs.data(), sizeof(int) * zs.size(), hipMemcpyHostToDevice, st));
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->xyz.p, xyz.data(), sizeof(double) * xyz.size(), hipMemcpyHostToDevice, st));
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->order.p, order.data(), sizeof(int) * m, hipMemcpyHostToDevice, st));
+    HIP_TRY(ctx, hipMemsetAsync(ctx->queue.p, 0, sizeof(int) * 4, st));
+
+    BatchArgs a;
+    a.mols = ctx->mols.as<MolDesc>();
+    a.n_mols = m;
+    a.lds_n = std::min(max_nao, 128);
+    a.z = ctx->z.as<int>();
+    a.xyz = ctx->xyz.as<double>();
+    a.elems = ctx->elems.as<DevElem>();
+    a.mats = ctx->mats.as<double>();
+    a.wpool = ctx->wpool.as<double>();
+    a.hist = ctx->hist.as<double>();
+    a.mo_e = ctx->mo_e.as<double>();
+    a.mo_e_beta = ctx->mo_eb.as<double>();
+    a.charges = ctx->charges.as<double>();
+    a.results = ctx->results.as<MolResult>();
+    a.queue = ctx->queue.as<int>();
+    a.order = ctx->order.as<int>();
+    size_t lds = std::max<size_t>((size_t)a.lds_n * a.lds_n * sizeof(double), 1024 * sizeof(double));
+    int precision = 0;
+    for (int j = 0; j < m; ++j) precision |= descs[j].precision;
+    int grid = std::min(m, ctx->cus * 2);
+    HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
+    HIP_TRY(ctx, launch_scf_batch(a, precision, grid, lds, st));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
+
+    std::vector<MolResult> res(m);
+    std::vector<double> mo(ao_off), q(atom_off);
+    HIP_TRY(ctx, hipMemcpyAsync(res.data(), ctx->results.p, sizeof(MolResult) * m, hipMemcpyDeviceToHost, st));
+    HIP_TRY(ctx, hipMemcpyAsync(mo.data(), ctx->mo_e.p, sizeof(double) * ao_off, hipMemcpyDeviceToHost, st));
+    HIP_TRY(ctx, hipMemcpyAsync(q.data(), ctx->charges.p, sizeof(double) * atom_off, hipMemcpyDeviceToHost, st));
+    HIP_TRY(ctx, hipStreamSynchronize(st));
+    float ms = 0.f;
+    HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ctx->last_ms = ms;
+    std::vector<double> mat;
+    for (int j = 0; j < m; ++j) {
+        const int i = ok_idx[j];
+        const MolDesc& md = descs[j];
+        am1g_result& r = results[i];
+        fill_result(res[j], md, r);
+        if (r.charges) std::memcpy(r.charges, q.data() + md.atom_off, sizeof(double) * md.nat);
+        if (r.mo_energies) std::memcpy(r.mo_energies, mo.data() + md.ao_off, sizeof(double) * md.nao);
+        const size_t nn = (size_t)md.nao * md.nao;
+        if (r.density) {
+            HIP_TRY(ctx, hipMemcpy(r.density, ctx->mats.as<double>() + md.mat_off + kP * nn, nn * sizeof(double),
+                                   hipMemcpyDeviceToHost));
+        }
+        if (r.mo_coeff) {
+            mat.resize(nn);
+            HIP_TRY(ctx, hipMemcpy(mat.data(), ctx->mats.as<double>() + md.mat_off + kC * nn, nn * sizeof(double),
+                                   hipMemcpyDeviceToHost));
+            for (int mu = 0; mu < md.nao; ++mu)
+                for (int k = 0; k < md.nao; ++k) r.mo_coeff[(size_t)mu * md.nao + k] = mat[(size_t)k * md.nao + mu];
+        }
+    }
+    return AM1G_OK;
+}
+
+int am1g_scf(am1g_ctx* ctx, const am1g_request* rq, am1g_result* res) {
+    if (!ctx || !rq || !res) return set_error(ctx, AM1G_INVALID_INPUT, "null argument");
+    int st = am1g_scf_batch(ctx, 1, rq, res);
+    if (st != AM1G_OK) return st;
+    return res->status;
+}
+
+}  // extern "C"

@@ -1,0 +1,717 @@
+// am1g SCF kernels for gfx950.
+//
+// k_scf_batch: one persistent launch solves a whole batch of molecules of uneven size. Each
+// workgroup takes the next molecule from a device work counter (largest first) and runs its
+// entire SCF — pair integrals, core Hamiltonian, every Fock/DIIS/eigen/density iteration and
+// the result epilogue — without returning to the host. All of a molecule's sums are reduced in
+// a fixed order inside its own workgroup, so results do not depend on the rest of the batch.
+//
+// Reference behaviour: am1-rs 0.2.3 run_am1 / rhf_loop / uhf_loop (src/scf.rs:294-858),
+// build_core_with_neighbors (src/hamiltonian.rs:211-366), build_fock_spin_with
+// (src/fock.rs:249-511), packed CDIIS / A-DIIS (src/scf.rs:876-1204).
+#include <hip/hip_runtime.h>
+
+#include "am1g_common.hpp"
+#include "am1g_pair.hpp"
+#include "am1g_launch.hpp"
+#include "am1g_wg.hpp"
+
+namespace am1g {
+
+
+
+struct MolShared {
+    int z[kMaxAtoms];
+    int off[kMaxAtoms];
+    int cls[kMaxAtoms];    // index inside the heavy list or the H list
+    int heavy[kMaxAtoms];  // atom indices of heavy atoms, ascending
+    int hyd[kMaxAtoms];    // atom indices of hydrogens, ascending
+    double B[kDiisDepth * kDiisDepth];   // CDIIS Gram of the error history (logical order)
+    double G[kDiisDepth * kDiisDepth];   // A-DIIS Gram <D_i, F_j>
+    double coef[kDiisDepth + 1];
+    int ring[kDiisDepth];  // logical -> physical history slot
+    int nh, nH, hcount, head;
+    int flags[8];
+};
+
+// Packed index k (0..9) of a 4-orbital block -> orbital pair (row >= col).
+__device__ __constant__ int kPackRow[10] = {0, 1, 1, 2, 2, 2, 3, 3, 3, 3};
+__device__ __constant__ int kPackCol[10] = {0, 0, 1, 0, 1, 2, 0, 1, 2, 3};
+
+// Where the integral block of atoms x and y lives, and whether x is its bra (row) atom.
+struct BlockRef {
+    const double* w;
+    int wcols;
+    bool x_is_bra;
+};
+
+__device__ __forceinline__ BlockRef block_of(const MolShared& ms, const double* wpool, int x, int y) {
+    const int nh = ms.nh, nH = ms.nH;
+    const int64_t hh = (int64_t)nh * (nh - 1) / 2, hH = (int64_t)nh * nH;
+    const bool hx = ms.z[x] != 1, hy = ms.z[y] != 1;
+    BlockRef r;
+    if (hx && hy) {
+        int i = ms.cls[x], j = ms.cls[y];
+        bool xa = i < j;
+        r.w = wpool + tri_index(min(i, j), max(i, j), nh) * 100;
+        r.wcols = 10;
+        r.x_is_bra = xa;
+    } else if (hx != hy) {
+        int hi = hx ? ms.cls[x] : ms.cls[y];
+        int mi = hx ? ms.cls[y] : ms.cls[x];
+        r.w = wpool + hh * 100 + ((int64_t)hi * nH + mi) * 10;
+        r.wcols = 1;
+        r.x_is_bra = hx;
+    } else {
+        int i = ms.cls[x], j = ms.cls[y];
+        r.w = wpool + hh * 100 + hH * 10 + tri_index(min(i, j), max(i, j), nH);
+        r.wcols = 1;
+        r.x_is_bra = i < j;
+    }
+    return r;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Setup: pair integrals, resonance blocks, core-core energy, H_core diagonal blocks.
+__device__ double mol_setup(const MolDesc& md, const BatchArgs& g, MolShared& ms, WgScratch& s, double* H,
+                            double* wpool) {
+    const int t = threadIdx.x;
+    const int nat = md.nat, nao = md.nao;
+    const DevElem* el = g.elems + md.method * kMaxZ;
+    const double* xyz = g.xyz + md.atom_off * 3;
+    if (t == 0) {
+        int off = 0, nh = 0, nH = 0;
+        for (int a = 0; a < nat; ++a) {
+            int zz = g.z[md.atom_off + a];
+            ms.z[a] = zz;
+            ms.off[a] = off;
+            off += el[zz].norb;
+            if (zz != 1) {
+                ms.cls[a] = nh;
+                ms.heavy[nh++] = a;
+            } else {
+                ms.cls[a] = nH;
+                ms.hyd[nH++] = a;
+            }
+        }
+        ms.nh = nh;
+        ms.nH = nH;
+    }
+    __syncthreads();
+    const int nh = ms.nh, nH = ms.nH;
+    const int64_t n_hh = (int64_t)nh * (nh - 1) / 2, n_hH = (int64_t)nh * nH, n_HH = (int64_t)nH * (nH - 1) / 2;
+    double ecore = 0.0;
+    // heavy-heavy pairs
+    for (int64_t k = t; k < n_hh; k += kWg) {
+        int i, j;
+        tri_decode(k, nh, i, j);
+        int a = ms.heavy[i], b = ms.heavy[j];
+        const DevElem& ea = el[ms.z[a]];
+        const DevElem& eb = el[ms.z[b]];
+        double d[3] = {xyz[3 * b] - xyz[3 * a], xyz[3 * b + 1] - xyz[3 * a + 1], xyz[3 * b + 2] - xyz[3 * a + 2]};
+        heavy_heavy_block(ea, eb, d, wpool + k * 100);
+        double S[16];
+        overlap_block(ea, eb, d, S);
+        int oa = ms.off[a], ob = ms.off[b];
+#pragma unroll
+        for (int i2 = 0; i2 < 4; ++i2)
+#pragma unroll
+            for (int j2 = 0; j2 < 4; ++j2) {
+                double bi = i2 == 0 ? ea.bs : ea.bp, bj = j2 == 0 ? eb.bs : eb.bp;
+                double v = 0.5 * (bi + bj) * S[i2 * 4 + j2];
+                H[(size_t)(oa + i2) * nao + ob + j2] = v;
+                H[(size_t)(ob + j2) * nao + oa + i2] = v;
+            }
+        ecore += core_core_pair(ea, eb, ms.z[a], ms.z[b], sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]));
+    }
+    // heavy-H pairs (a heavy, b hydrogen; displacement R_b - R_a)
+    for (int64_t k = t; k < n_hH; k += kWg) {
+        int i = (int)(k / nH), m = (int)(k - (int64_t)i * nH);
+        int a = ms.heavy[i], b = ms.hyd[m];
+        const DevElem& ea = el[ms.z[a]];
+        const DevElem& eb = el[1];
+        double d[3] = {xyz[3 * b] - xyz[3 * a], xyz[3 * b + 1] - xyz[3 * a + 1], xyz[3 * b + 2] - xyz[3 * a + 2]};
+        double w[10];
+        heavy_h_block(ea, eb, d, w);
+        double* dst = wpool + n_hh * 100 + k * 10;
+#pragma unroll
+        for (int q = 0; q < 10; ++q) dst[q] = w[q];
+        double S[16];
+        overlap_block(ea, eb, d, S);
+        int oa = ms.off[a], ob = ms.off[b];
+#pragma unroll
+        for (int i2 = 0; i2 < 4; ++i2) {
+            double v = 0.5 * ((i2 == 0 ? ea.bs : ea.bp) + eb.bs) * S[i2 * 4];
+            H[(size_t)(oa + i2) * nao + ob] = v;
+            H[(size_t)ob * nao + oa + i2] = v;
+        }
+        // core-core energy is symmetric in the pair; the reference evaluates it in i<j order
+        int lo = min(a, b), hi = max(a, b);
+        ecore += core_core_pair(el[ms.z[lo]], el[ms.z[hi]], ms.z[lo], ms.z[hi],
+                                sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]));
+    }
+    // H-H pairs
+    for (int64_t k = t; k < n_HH; k += kWg) {
+        int i, j;
+        tri_decode(k, nH, i, j);
+        int a = ms.hyd[i], b = ms.hyd[j];
+        const DevElem& e1 = el[1];
+        double d[3] = {xyz[3 * b] - xyz[3 * a], xyz[3 * b + 1] - xyz[3 * a + 1], xyz[3 * b + 2] - xyz[3 * a + 2]};
+        double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        wpool[n_hh * 100 + n_hH * 10 + k] = hh_integral(e1, e1, r);
+        double S[16];
+        overlap_block(e1, e1, d, S);
+        double v = 0.5 * (e1.bs + e1.bs) * S[0];
+        int oa = ms.off[a], ob = ms.off[b];
+        H[(size_t)oa * nao + ob] = v;
+        H[(size_t)ob * nao + oa] = v;
+        ecore += core_core_pair(e1, e1, 1, 1, r);
+    }
+    __syncthreads();
+    // diagonal blocks: U + sum over partners of (-Z_y) (mu nu | s_y s_y)
+    const int n_items = nh * 10 + nH;
+    for (int it = t; it < n_items; it += kWg) {
+        int x, k;
+        if (it < nh * 10) {
+            x = ms.heavy[it / 10];
+            k = it % 10;
+        } else {
+            x = ms.hyd[it - nh * 10];
+            k = 0;
+        }
+        const DevElem& ex = el[ms.z[x]];
+        int mu = kPackRow[k], nu = kPackCol[k];
+        double acc = 0.0;
+        for (int y = 0; y < nat; ++y) {
+            if (y == x) continue;
+            BlockRef br = block_of(ms, wpool, x, y);
+            double wv = br.x_is_bra ? br.w[k * br.wcols] : br.w[k];
+            acc += -el[ms.z[y]].core * wv;
+        }
+        double u = (mu == nu) ? (mu == 0 ? ex.uss : ex.upp) : 0.0;
+        int o = ms.off[x];
+        H[(size_t)(o + mu) * nao + o + nu] = u + acc;
+        H[(size_t)(o + nu) * nao + o + mu] = u + acc;
+    }
+    ecore = wg_sum(ecore, s);
+    return ecore;
+}
+
+// F = H + one-centre(P, P_spin) + J(P) - K(scale * P_spin) for one spin channel
+// (fock.rs:249-511). Diagonal atom blocks: one thread per (atom, packed orbital pair), summing
+// over partner atoms in index order. Off-diagonal blocks: one thread per pair.
+__device__ void mol_fock(const MolDesc& md, const DevElem* el, const MolShared& ms, const double* wpool,
+                         const double* H, const double* Pt, const double* Ps, double scale, double* F) {
+    const int t = threadIdx.x;
+    const int nat = md.nat, nao = md.nao;
+    const int nh = ms.nh, nH = ms.nH;
+    const int n_items = nh * 10 + nH;
+    for (int it = t; it < n_items; it += kWg) {
+        int x, k;
+        if (it < nh * 10) {
+            x = ms.heavy[it / 10];
+            k = it % 10;
+        } else {
+            x = ms.hyd[it - nh * 10];
+            k = 0;
+        }
+        const DevElem& ex = el[ms.z[x]];
+        const int ox = ms.off[x], nx = ex.norb;
+        const int mu = kPackRow[k], nu = kPackCol[k];
+        double acc = H[(size_t)(ox + mu) * nao + ox + nu];
+        // one-centre
+        for (int la = 0; la < nx; ++la)
+            for (int si = 0; si < nx; ++si) {
+                acc += Pt[(size_t)(ox + la) * nao + ox + si] * one_centre(mu, nu, la, si, ex);
+                acc -= scale * Ps[(size_t)(ox + la) * nao + ox + si] * one_centre(mu, la, nu, si, ex);
+            }
+        // two-centre Coulomb from every partner's diagonal density block
+        for (int y = 0; y < nat; ++y) {
+            if (y == x) continue;
+            BlockRef br = block_of(ms, wpool, x, y);
+            const int oy = ms.off[y];
+            if (ms.z[y] == 1) {
+                double wv = br.x_is_bra ? br.w[k * br.wcols] : br.w[k];
+                acc += Pt[(size_t)oy * nao + oy] * wv;
+            } else {
+#pragma unroll
+                for (int q = 0; q < 10; ++q) {
+                    int l = kPackRow[q], c = kPackCol[q];
+                    double pv = Pt[(size_t)(oy + l) * nao + oy + c] * (l == c ? 1.0 : 2.0);
+                    double wv = br.x_is_bra ? br.w[k * br.wcols + q] : br.w[q * br.wcols + k];
+                    acc += pv * wv;
+                }
+            }
+        }
+        F[(size_t)(ox + mu) * nao + ox + nu] = acc;
+        F[(size_t)(ox + nu) * nao + ox + mu] = acc;
+    }
+    // exchange blocks, one pair per thread, grouped by class
+    const int64_t n_hh = (int64_t)nh * (nh - 1) / 2, n_hH = (int64_t)nh * nH, n_HH = (int64_t)nH * (nH - 1) / 2;
+    for (int64_t k = t; k < n_hh; k += kWg) {
+        int i, j;
+        tri_decode(k, nh, i, j);
+        int a = ms.heavy[i], b = ms.heavy[j];
+        int oa = ms.off[a], ob = ms.off[b];
+        const double* w = wpool + k * 100;
+        double pab[16];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) pab[u * 4 + v] = scale * Ps[(size_t)(oa + u) * nao + ob + v];
+#pragma unroll
+        for (int mu = 0; mu < 4; ++mu)
+#pragma unroll
+            for (int la = 0; la < 4; ++la) {
+                double acc = 0.0;
+#pragma unroll
+                for (int nu = 0; nu < 4; ++nu)
+#pragma unroll
+                    for (int si = 0; si < 4; ++si) acc += pab[nu * 4 + si] * w[pack4(mu, nu) * 10 + pack4(la, si)];
+                F[(size_t)(oa + mu) * nao + ob + la] = H[(size_t)(oa + mu) * nao + ob + la] - acc;
+                F[(size_t)(ob + la) * nao + oa + mu] = H[(size_t)(ob + la) * nao + oa + mu] - acc;
+            }
+    }
+    for (int64_t k = t; k < n_hH; k += kWg) {
+        int i = (int)(k / nH), m = (int)(k - (int64_t)i * nH);
+        int a = ms.heavy[i], b = ms.hyd[m];
+        int oa = ms.off[a], ob = ms.off[b];
+        const double* w = wpool + n_hh * 100 + k * 10;
+        double pab[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) pab[u] = scale * Ps[(size_t)(oa + u) * nao + ob];
+#pragma unroll
+        for (int mu = 0; mu < 4; ++mu) {
+            double acc = 0.0;
+#pragma unroll
+            for (int nu = 0; nu < 4; ++nu) acc += pab[nu] * w[pack4(mu, nu)];
+            F[(size_t)(oa + mu) * nao + ob] = H[(size_t)(oa + mu) * nao + ob] - acc;
+            F[(size_t)ob * nao + oa + mu] = H[(size_t)ob * nao + oa + mu] - acc;
+        }
+    }
+    for (int64_t k = t; k < n_HH; k += kWg) {
+        int i, j;
+        tri_decode(k, nH, i, j);
+        int a = ms.hyd[i], b = ms.hyd[j];
+        int oa = ms.off[a], ob = ms.off[b];
+        double acc = scale * Ps[(size_t)oa * nao + ob] * wpool[n_hh * 100 + n_hH * 10 + k];
+        F[(size_t)oa * nao + ob] = H[(size_t)oa * nao + ob] - acc;
+        F[(size_t)ob * nao + oa] = H[(size_t)ob * nao + oa] - acc;
+    }
+}
+
+// P = w * C_occ C_occ^T from transposed eigenvectors Ct (row k = MO k): P = w Ct_occ^T Ct_occ.
+template <typename T>
+__device__ void mol_density(int nao, int nocc, double w, const double* Ct, double* P) {
+    if (nocc == 0) {
+        for (int idx = threadIdx.x; idx < nao * nao; idx += kWg) P[idx] = 0.0;
+        return;
+    }
+    wg_gemm<T, true, false>(nao, nao, nocc, w, Ct, nao, Ct, nao, P, nao);
+}
+
+// Sort eigenvalues ascending (stable, by rank) and permute the rows of Ct into Cs.
+__device__ void mol_sort_eigen(int n, const double* A, int lda, const double* Ct, double* Cs, double* eps) {
+    for (int i = threadIdx.x; i < n; i += kWg) {
+        double ei = A[(size_t)i * lda + i];
+        int rank = 0;
+        for (int j = 0; j < n; ++j) {
+            double ej = A[(size_t)j * lda + j];
+            rank += (ej < ei) || (ej == ei && j < i);
+        }
+        eps[rank] = ei;
+        for (int c = 0; c < n; ++c) Cs[(size_t)rank * n + c] = Ct[(size_t)i * n + c];
+    }
+}
+
+// One spin channel's eigen-step: A = Ct F Ct^T (warm start in the previous orbitals, or F itself
+// on the first pass), Jacobi, sort. Ct is updated in place; eps receives sorted eigenvalues.
+template <typename T>
+__device__ int mol_eigen(int nao, const double* Fuse, double* Ct, double* T0, double* T1, double* lds, int lds_n,
+                         bool warm, double* eps, WgScratch& s) {
+    const int t = threadIdx.x;
+    double* A;
+    int lda = nao;
+    A = (nao <= lds_n) ? lds : T1;
+    if (warm) {
+        // T0 = Ct F  (row k: sum_nu Ct[k][nu] F[nu][mu]); A = T0 Ct^T
+        wg_gemm<T, false, false>(nao, nao, nao, 1.0, Ct, nao, Fuse, nao, T0, nao);
+        __syncthreads();
+        wg_gemm<T, false, true>(nao, nao, nao, 1.0, T0, nao, Ct, nao, A, lda);
+        __syncthreads();
+        // symmetrize the rotated matrix (the products are symmetric up to rounding)
+        for (int idx = t; idx < nao * nao; idx += kWg) {
+            int i = idx / nao, j = idx - i * nao;
+            if (i < j) {
+                double v = 0.5 * (A[(size_t)i * lda + j] + A[(size_t)j * lda + i]);
+                A[(size_t)i * lda + j] = v;
+                A[(size_t)j * lda + i] = v;
+            }
+        }
+    } else {
+        for (int idx = t; idx < nao * nao; idx += kWg) {
+            int i = idx / nao, j = idx - i * nao;
+            A[(size_t)i * lda + j] = Fuse[idx];
+            Ct[idx] = (i == j) ? 1.0 : 0.0;
+        }
+    }
+    __syncthreads();
+    int sweeps = wg_jacobi(nao, A, lda, Ct, nao, s, 60);
+    __syncthreads();
+    mol_sort_eigen(nao, A, lda, Ct, T0, eps);
+    __syncthreads();
+    for (int idx = t; idx < nao * nao; idx += kWg) Ct[idx] = T0[idx];
+    __syncthreads();
+    return sweeps;
+}
+
+__device__ __forceinline__ int64_t tri_pos(int i, int j, int n) {  // upper incl. diagonal, i <= j
+    return (int64_t)i * n - (int64_t)i * (i - 1) / 2 + (j - i);
+}
+
+// ---------------------------------------------------------------------------------------------
+// DIIS bookkeeping for the RHF loop. History slots hold packed F (n(n+1)/2), packed error
+// (strict upper triangle, n(n-1)/2) and packed D (n(n+1)/2).
+struct DiisView {
+    double* f;
+    double* e;
+    double* d;
+    int64_t len_s, len_a;
+};
+
+__device__ __forceinline__ DiisView diis_view(double* hist, int nao, int channel) {
+    DiisView v;
+    v.len_s = tri_len(nao);
+    v.len_a = (int64_t)nao * (nao - 1) / 2;
+    double* base = hist + (int64_t)channel * kDiisDepth * 3 * v.len_s;
+    v.f = base;
+    v.e = base + kDiisDepth * v.len_s;
+    v.d = base + 2 * kDiisDepth * v.len_s;
+    return v;
+}
+
+// ---------------------------------------------------------------------------------------------
+template <int PREC>
+__device__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolShared& ms, WgScratch& s, double* lds,
+                        double ecore, MolResult& res) {
+    const int t = threadIdx.x;
+    const int nao = md.nao, nat = md.nat, nocc = md.n_alpha;
+    const DevElem* el = g.elems + md.method * kMaxZ;
+    double* M = g.mats + md.mat_off;
+    const size_t nn = (size_t)nao * nao;
+    double* H = M + kH * nn;
+    double* F = M + kF * nn;
+    double* P = M + kP * nn;
+    double* Ct = M + kC * nn;
+    double* T0 = M + kT0 * nn;
+    double* T1 = M + kT1 * nn;
+    double* wpool = g.wpool + md.w_off;
+    double* eps = g.mo_e + md.ao_off;
+    DiisView dv = diis_view(g.hist + md.hist_off, nao, 0);
+    const int accel = md.accel;
+    const bool need_d = accel == 2;
+
+    // SAD guess (scf.rs:475-494)
+    for (size_t idx = t; idx < nn; idx += kWg) P[idx] = 0.0;
+    __syncthreads();
+    for (int a = t; a < nat; a += kWg) {
+        const DevElem& e = el[ms.z[a]];
+        double ns = fmin(e.core, 2.0), np = fmax(e.core - ns, 0.0);
+        int o = ms.off[a];
+        P[(size_t)o * nao + o] = ns;
+        if (e.norb == 4)
+            for (int k = 1; k < 4; ++k) P[(size_t)(o + k) * nao + o + k] = np / 3.0;
+    }
+    if (t == 0) {
+        ms.hcount = 0;
+        ms.head = 0;
+    }
+    __syncthreads();
+
+    double e_old = 0.0, residual = INFINITY;
+    int iterations = 0, converged = 0, status = kOk;
+    bool have_c = false;
+    int fp32_iters = 0;
+    for (int iter = 0; iter < md.max_scf; ++iter) {
+        iterations = iter + 1;
+        mol_fock(md, el, ms, wpool, H, P, P, 0.5, F);
+        __syncthreads();
+        // E = 1/2 sum P (H + F)
+        double pe = 0.0;
+        for (size_t idx = t; idx < nn; idx += kWg) pe += P[idx] * (H[idx] + F[idx]);
+        double e_elec = 0.5 * wg_sum(pe, s);
+        // error = FP - (FP)^T; F and P are symmetric so FP = F^T P
+        bool low = (PREC == 1) && (residual > 1.0e-4);
+        if (low) {
+            wg_gemm<float, true, false>(nao, nao, nao, 1.0, F, nao, P, nao, T0, nao);
+            ++fp32_iters;
+        } else {
+            wg_gemm<double, true, false>(nao, nao, nao, 1.0, F, nao, P, nao, T0, nao);
+        }
+        __syncthreads();
+        // push history: logical slot = count (or overwrite oldest)
+        int slot;
+        if (ms.hcount < kDiisDepth) slot = (ms.head + ms.hcount) % kDiisDepth;
+        else slot = ms.head;  // oldest is replaced
+        double* hf = dv.f + (int64_t)slot * dv.len_s;
+        double* he = dv.e + (int64_t)slot * dv.len_a;
+        double* hd = dv.d + (int64_t)slot * dv.len_s;
+        double en = 0.0;
+        for (int64_t idx = t; idx < (int64_t)nn; idx += kWg) {
+            int i = (int)(idx / nao), j = (int)(idx - (int64_t)i * nao);
+            if (j < i) continue;
+            hf[tri_pos(i, j, nao)] = F[idx];
+            if (need_d) hd[tri_pos(i, j, nao)] = P[idx];
+            if (j > i) {
+                double e = T0[idx] - T0[(size_t)j * nao + i];
+                he[(int64_t)i * nao - (int64_t)i * (i + 1) / 2 + (j - i - 1)] = e;
+                en += e * e;
+            }
+        }
+        double err = sqrt(2.0 * wg_sum(en, s));
+        __syncthreads();
+        if (t == 0) {
+            if (ms.hcount < kDiisDepth) {
+                ms.hcount++;
+            } else {
+                ms.head = (ms.head + 1) % kDiisDepth;
+                // shift the cached Gram matrices one step (drop the oldest row and column)
+                for (int i = 0; i < kDiisDepth - 1; ++i)
+                    for (int j = 0; j < kDiisDepth - 1; ++j) {
+                        ms.B[i * kDiisDepth + j] = ms.B[(i + 1) * kDiisDepth + j + 1];
+                        ms.G[i * kDiisDepth + j] = ms.G[(i + 1) * kDiisDepth + j + 1];
+                    }
+            }
+            for (int i = 0; i < ms.hcount; ++i) ms.ring[i] = (ms.head + i) % kDiisDepth;
+        }
+        __syncthreads();
+        const int m = ms.hcount;
+        const int newest = m - 1;
+        // new Gram entries: <e_new, e_i> for all i; A-DIIS <D_new, F_j>, <D_i, F_new>
+        {
+            double part[3 * kDiisDepth];
+#pragma unroll
+            for (int q = 0; q < 3 * kDiisDepth; ++q) part[q] = 0.0;
+            const double* en_p = dv.e + (int64_t)ms.ring[newest] * dv.len_a;
+            for (int64_t idx = t; idx < dv.len_a; idx += kWg) {
+                double x = en_p[idx];
+                for (int i = 0; i < m; ++i) part[i] += x * dv.e[(int64_t)ms.ring[i] * dv.len_a + idx];
+            }
+            if (need_d) {
+                const double* dn = dv.d + (int64_t)ms.ring[newest] * dv.len_s;
+                const double* fn = dv.f + (int64_t)ms.ring[newest] * dv.len_s;
+                for (int64_t idx = t; idx < dv.len_s; idx += kWg) {
+                    // diagonal entries of the packed triangle count once, the rest twice
+                    int i = 0;
+                    {
+                        // row of idx in the upper triangle: the row start is tri_pos(i, i)
+                        double nn2 = (double)(2 * nao + 1);
+                        i = (int)floor((nn2 - sqrt(nn2 * nn2 - 8.0 * (double)idx)) * 0.5);
+                        if (i < 0) i = 0;
+                        while (i > 0 && tri_pos(i, i, nao) > idx) --i;
+                        while (i + 1 < nao && tri_pos(i + 1, i + 1, nao) <= idx) ++i;
+                    }
+                    double wgt = (tri_pos(i, i, nao) == idx) ? 1.0 : 2.0;
+                    double dnv = dn[idx] * wgt, fnv = fn[idx] * wgt;
+                    for (int j = 0; j < m; ++j) {
+                        part[kDiisDepth + j] += dnv * dv.f[(int64_t)ms.ring[j] * dv.len_s + idx];
+                        part[2 * kDiisDepth + j] += fnv * dv.d[(int64_t)ms.ring[j] * dv.len_s + idx];
+                    }
+                }
+            }
+            double tot[3 * kDiisDepth];
+            wg_sum_many(part, 3 * kDiisDepth, lds, tot);  // LDS is free outside the eigensolver
+            if (t == 0) {
+                for (int i = 0; i < m; ++i) {
+                    ms.B[newest * kDiisDepth + i] = 2.0 * tot[i];
+                    ms.B[i * kDiisDepth + newest] = 2.0 * tot[i];
+                }
+                if (need_d)
+                    for (int j = 0; j < m; ++j) {
+                        ms.G[newest * kDiisDepth + j] = tot[kDiisDepth + j];  // <D_new, F_j>
+                        ms.G[j * kDiisDepth + newest] = tot[2 * kDiisDepth + j];  // <D_j, F_new>
+                    }
+            }
+            __syncthreads();
+        }
+        if (!isfinite(e_elec) || !isfinite(err)) {
+            status = kScfDiverged;
+            break;
+        }
+        // extrapolation coefficients (thread 0), scf.rs:627-653
+        if (t == 0) {
+            int use = 0;
+            if (accel == 1 || (accel == 2 && err <= md.adiis_switch)) {
+                if (m >= 2) {
+                    double Bm[(kDiisDepth + 1) * (kDiisDepth + 1)], rhs[kDiisDepth + 1], x[kDiisDepth + 1];
+                    int dim = m + 1;
+                    for (int i = 0; i < m; ++i) {
+                        for (int j = 0; j < m; ++j) Bm[i * dim + j] = ms.B[i * kDiisDepth + j];
+                        Bm[i * dim + m] = -1.0;
+                        Bm[m * dim + i] = -1.0;
+                        rhs[i] = 0.0;
+                    }
+                    Bm[m * dim + m] = 0.0;
+                    rhs[m] = -1.0;
+                    if (solve_bordered(Bm, rhs, dim, x)) {
+                        double wsum = 0.0;
+                        for (int i = 0; i < m; ++i) wsum += fabs(x[i]);
+                        if (isfinite(wsum)) {
+                            use = 1;
+                            for (int i = 0; i < m; ++i) ms.coef[i] = x[i];
+                        }
+                    }
+                }
+            } else if (accel == 2) {
+                if (m >= 2) {
+                    double c[kDiisDepth];
+                    adiis_solve(ms.G, m, c);
+                    use = 1;
+                    for (int i = 0; i < m; ++i) ms.coef[i] = c[i];
+                }
+            }
+            ms.flags[0] = use;
+        }
+        __syncthreads();
+        const double* Fuse = F;
+        if (ms.flags[0]) {
+            // T1 = sum_i c_i F_i (unpacked); refuse a non-finite combination
+            double bad = 0.0;
+            for (int64_t idx = t; idx < (int64_t)nn; idx += kWg) {
+                int i = (int)(idx / nao), j = (int)(idx - (int64_t)i * nao);
+                int64_t pk = (i <= j) ? tri_pos(i, j, nao) : tri_pos(j, i, nao);
+                double acc = 0.0;
+                for (int q = 0; q < m; ++q) acc += ms.coef[q] * dv.f[(int64_t)ms.ring[q] * dv.len_s + pk];
+                T1[idx] = acc;
+                if (!isfinite(acc)) bad = 1.0;
+            }
+            bad = wg_max(bad, s);
+            if (bad == 0.0) Fuse = T1;
+        }
+        // eigen-step; the warm start needs T0/T1 as scratch, so move F_use out of T1 first
+        if (Fuse == T1) {
+            for (size_t idx = t; idx < nn; idx += kWg) F[idx] = T1[idx];  // F is rebuilt next pass
+            Fuse = F;
+        }
+        __syncthreads();
+        int sweeps = mol_eigen<double>(nao, Fuse, Ct, T0, T1, lds, g.lds_n, have_c, eps, s);
+        have_c = true;
+        if (sweeps < 0) {
+            status = kLinearAlgebra;
+            break;
+        }
+        // new density into T0, rms change, then P <- T0
+        if (low) mol_density<float>(nao, nocc, 2.0, Ct, T0);
+        else mol_density<double>(nao, nocc, 2.0, Ct, T0);
+        __syncthreads();
+        double dd = 0.0;
+        for (size_t idx = t; idx < nn; idx += kWg) {
+            double x = T0[idx] - P[idx];
+            dd += x * x;
+            P[idx] = T0[idx];
+        }
+        double dp = sqrt(wg_sum(dd, s) / (double)max((size_t)1, nn));
+        double de = fabs(e_elec - e_old);
+        e_old = e_elec;
+        residual = err;
+        if (iter > 0 && de < md.e_tol && (dp < md.p_tol || err < 1.0e-7) && !low) {
+            converged = 1;
+            break;
+        }
+        __syncthreads();
+    }
+    __syncthreads();
+    // final energy from the final density (scf.rs:698-701)
+    mol_fock(md, el, ms, wpool, H, P, P, 0.5, F);
+    __syncthreads();
+    double pe = 0.0;
+    for (size_t idx = t; idx < nn; idx += kWg) pe += P[idx] * (H[idx] + F[idx]);
+    double electronic = 0.5 * wg_sum(pe, s);
+    // Mulliken charges and dipole (scf.rs:400-427)
+    double dip[3] = {0.0, 0.0, 0.0};
+    double eiso = 0.0, eheat = 0.0;
+    const double* xyz = g.xyz + md.atom_off * 3;
+    for (int a = t; a < nat; a += kWg) {
+        const DevElem& e = el[ms.z[a]];
+        int o = ms.off[a];
+        double pop = 0.0;
+        for (int k = 0; k < e.norb; ++k) pop += P[(size_t)(o + k) * nao + o + k];
+        double q = e.core - pop;
+        g.charges[md.atom_off + a] = q;
+        for (int k = 0; k < 3; ++k) dip[k] += xyz[3 * a + k] * q;
+        if (e.norb == 4)
+            for (int k = 0; k < 3; ++k) dip[k] += -2.0 * e.dd * P[(size_t)o * nao + o + 1 + k];
+        eiso += e.e_isol;
+        eheat += e.eheat_ev;
+    }
+    for (int k = 0; k < 3; ++k) dip[k] = wg_sum(dip[k], s);
+    eiso = wg_sum(eiso, s);
+    eheat = wg_sum(eheat, s);
+    if (t == 0) {
+        res.status = (status == kOk && !converged) ? kScfNotConverged : status;
+        res.iterations = iterations;
+        res.converged = converged;
+        res.nao = nao;
+        res.electronic_ev = electronic;
+        res.core_ev = ecore;
+        res.total_ev = electronic + ecore;
+        res.heat_of_formation_kcal = (electronic + ecore - eiso + eheat) * kEvToKcal;
+        res.homo_ev = nocc >= 1 ? eps[nocc - 1] : NAN;
+        res.lumo_ev = nocc < nao ? eps[nocc] : NAN;
+        res.homo_beta_ev = NAN;
+        res.lumo_beta_ev = NAN;
+        for (int k = 0; k < 3; ++k) res.dipole[k] = dip[k] * kAuDipoleToDebye;
+        res.residual = residual;
+        res.fp32_iterations = fp32_iters;
+    }
+}
+
+template <int PREC>
+__global__ __launch_bounds__(kWg) void k_scf_batch(BatchArgs g) {
+    extern __shared__ double lds[];
+    __shared__ MolShared ms;
+    __shared__ WgScratch s;
+    __shared__ int next;
+    for (;;) {
+        if (threadIdx.x == 0) next = atomicAdd(g.queue, 1);
+        __syncthreads();
+        const int slot = next;
+        __syncthreads();
+        if (slot >= g.n_mols) break;
+        const int mi = g.order ? g.order[slot] : slot;
+        const MolDesc md = g.mols[mi];
+        MolResult res;
+        double* M = g.mats + md.mat_off;
+        double ecore = mol_setup(md, g, ms, s, M + kH * (size_t)md.nao * md.nao, g.wpool + md.w_off);
+        __syncthreads();
+        scf_rhf<PREC>(md, g, ms, s, lds, ecore, res);
+        if (threadIdx.x == 0) g.results[mi] = res;
+        __syncthreads();
+    }
+}
+
+template __global__ void k_scf_batch<0>(BatchArgs);
+template __global__ void k_scf_batch<1>(BatchArgs);
+
+}  // namespace am1g
+
+namespace am1g {
+hipError_t launch_scf_batch(const BatchArgs& a, int precision, int grid, size_t lds_bytes, hipStream_t stream) {
+    static bool attr_done = false;
+    if (!attr_done) {
+        // allow the dynamic LDS region to take what the static part leaves of the 160 KiB
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_scf_batch<0>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, 140 * 1024);
+        if (e != hipSuccess) return e;
+        e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_scf_batch<1>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, 140 * 1024);
+        if (e != hipSuccess) return e;
+        attr_done = true;
+    }
+    if (precision == 1)
+        hipLaunchKernelGGL(k_scf_batch<1>, dim3(grid), dim3(kWg), lds_bytes, stream, a);
+    else
+        hipLaunchKernelGGL(k_scf_batch<0>, dim3(grid), dim3(kWg), lds_bytes, stream, a);
+    return hipGetLastError();
+}
+}  // namespace am1g

@@ -1,0 +1,118 @@
+/*
+ * am1g — MI355X-native AM1/RM1 NDDO SCF engine, C ABI.
+ *
+ * This header is the drop-in boundary for am1-rs 0.2.3's molecular SCF entry points. All
+ * buffers are caller-owned; coordinates are Bohr, energies eV, heats of formation kcal/mol,
+ * dipoles Debye. Matrices are dense row-major nao x nao; MO coefficients hold one orbital
+ * per column, orbitals in ascending energy (the layout of am1-rs Am1Result).
+ *
+ * Reference interfaces replaced:
+ *   am1g_scf        <- pub fn run_am1(&Molecule, &Am1Parameters, &Am1Options)
+ *                      -> Result<Am1Result>                    (src/scf.rs:294-298)
+ *                      and Am1Calculator::calculate            (src/scf.rs:257-259)
+ *   am1g_scf_batch  <- many run_am1 calls (no single reference counterpart: the reference runs
+ *                      them one at a time, e.g. python.rs:147-199 per molecule)
+ *   am1g_request    <- Molecule (src/system.rs:19-39) + Am1Options (src/scf.rs:53-197)
+ *   am1g_result     <- Am1Result (src/scf.rs:200-240)
+ *   status codes    <- Am1Error (src/error.rs:213-293)
+ */
+#ifndef AM1G_H
+#define AM1G_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes; same order as am1-rs's Am1Error variants used on this path. */
+enum {
+    AM1G_OK = 0,
+    AM1G_INVALID_INPUT = 1,        /* Am1Error::InvalidInput */
+    AM1G_ELEMENT_NOT_PARAMETERIZED = 2, /* Am1Error::ElementNotParameterized */
+    AM1G_LINEAR_ALGEBRA = 3,       /* Am1Error::LinearAlgebra */
+    AM1G_SCF_NOT_CONVERGED = 4,    /* Am1Error::ScfNotConverged; result is filled, residual set */
+    AM1G_SCF_DIVERGED = 5,         /* Am1Error::ScfDiverged; iterations set */
+    AM1G_DEVICE_ERROR = 6          /* HIP runtime failure (no reference counterpart) */
+};
+
+enum { AM1G_METHOD_AM1 = 0, AM1G_METHOD_RM1 = 1 };                       /* NddoMethod */
+enum { AM1G_REF_AUTO = 0, AM1G_REF_RHF = 1, AM1G_REF_UHF = 2 };           /* ScfReference */
+enum { AM1G_ACCEL_NONE = 0, AM1G_ACCEL_CDIIS = 1, AM1G_ACCEL_ADIIS_CDIIS = 2 }; /* ScfAccelerator */
+enum { AM1G_PRECISION_FP64 = 0, AM1G_PRECISION_MIXED = 1 };
+
+typedef struct am1g_ctx am1g_ctx;
+
+/* One SCF request: Molecule + the Am1Options fields the molecular SCF reads. */
+typedef struct {
+    int n_atoms;
+    const uint8_t* z;        /* [n_atoms] atomic numbers */
+    const double* xyz_bohr;  /* [n_atoms][3] */
+    double charge;
+    int multiplicity;        /* >= 1 */
+    int method;              /* AM1G_METHOD_* */
+    int reference;           /* AM1G_REF_* */
+    int accelerator;         /* AM1G_ACCEL_* */
+    int use_diis;            /* 0 forces AM1G_ACCEL_NONE (Am1Options::use_diis) */
+    int max_scf;             /* default 200 */
+    double e_tol;            /* default 1e-8 eV */
+    double p_tol;            /* default 1e-7 */
+    double adiis_switch;     /* default 0.1 */
+    int precision;           /* AM1G_PRECISION_*; FP32 GEMM phase until the DIIS error < 1e-4 */
+} am1g_request;
+
+/* Result of one SCF. Pointer members are caller-provided outputs; NULL skips that output. */
+typedef struct {
+    int status;
+    int nao, n_occ, n_occ_beta;
+    int iterations, converged, unrestricted;
+    int has_homo, has_lumo, has_homo_beta, has_lumo_beta;
+    double electronic_ev, core_ev, total_ev, heat_of_formation_kcal;
+    double homo_ev, lumo_ev, homo_beta_ev, lumo_beta_ev;
+    double dipole_debye[3];
+    double dipole_magnitude;
+    double residual;          /* ||[F,P]|| of the last iteration */
+    int fp32_iterations;      /* iterations run with FP32 GEMMs (mixed precision) */
+    double* charges;          /* [n_atoms] Mulliken charges */
+    double* mo_energies;      /* [nao] (alpha for UHF) */
+    double* mo_energies_beta; /* [nao] UHF only */
+    double* density;          /* [nao*nao] total density */
+    double* mo_coeff;         /* [nao*nao] orbitals as columns (alpha for UHF) */
+} am1g_result;
+
+/* Fill a request with am1-rs's Am1Options defaults (scf.rs:174-197). */
+void am1g_request_defaults(am1g_request* rq);
+
+/* Create a context on HIP device `device`: uploads the AM1 and RM1 parameter tables and owns
+ * the device pools and stream. No reference counterpart (the reference caches parameters in a
+ * process-wide OnceLock, params.rs:100-115). */
+int am1g_create(int device, am1g_ctx** out);
+void am1g_destroy(am1g_ctx* ctx);
+
+/* Plain path: one molecule, one launch sequence per SCF phase. Replaces run_am1
+ * (src/scf.rs:294-298). */
+int am1g_scf(am1g_ctx* ctx, const am1g_request* rq, am1g_result* res);
+
+/* Batched path: n independent molecules of any size in one persistent launch. Each result is
+ * the same as the molecule's am1g_scf result within the stated tolerance. Returns AM1G_OK when
+ * the launch ran; per-molecule outcomes are in results[i].status. */
+int am1g_scf_batch(am1g_ctx* ctx, int n, const am1g_request* rqs, am1g_result* results);
+
+/* Text of the last error on this context (the Display text of the matching Am1Error). */
+const char* am1g_last_error(const am1g_ctx* ctx);
+
+/* Derived element parameters as the library uses them (for parity checks):
+ * out[23] = n, norb, uss, upp, zs, zp, bs, bp, gss, gsp, gpp, gp2, hsp, alpha, core,
+ *           eheat_ev, e_isol, dd, qq, rho0, rho1, rho2, n_gaussians.
+ * Mirrors Am1Element (src/params.rs:24-55). */
+int am1g_element_info(int z, int method, double* out);
+
+/* Kernel time of the last launch on the context, in milliseconds (HIP events on the
+ * context's stream). */
+double am1g_last_kernel_ms(const am1g_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AM1G_H */
