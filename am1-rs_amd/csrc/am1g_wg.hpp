@@ -42,8 +42,8 @@ __device__ __forceinline__ double wg_max(double v, WgScratch& s) {
     return tot;
 }
 
-// Several independent sums at once (up to 32 values per thread), reduced through `buf`
-// (nv * kWg doubles of scratch). Result in out[0..nv).
+// Several independent sums at once, reduced through `buf` (4 * nv doubles of shared scratch).
+// Every thread receives all nv totals in out[0..nv).
 __device__ void wg_sum_many(const double* v, int nv, double* buf, double* out) {
     const int t = threadIdx.x;
     for (int i = 0; i < nv; ++i) {
@@ -52,7 +52,7 @@ __device__ void wg_sum_many(const double* v, int nv, double* buf, double* out) {
         if ((t & 63) == 0) buf[i * 4 + (t >> 6)] = x;
     }
     __syncthreads();
-    if (t < nv) out[t] = (buf[t * 4 + 0] + buf[t * 4 + 1]) + (buf[t * 4 + 2] + buf[t * 4 + 3]);
+    for (int i = 0; i < nv; ++i) out[i] = (buf[i * 4 + 0] + buf[i * 4 + 1]) + (buf[i * 4 + 2] + buf[i * 4 + 3]);
     __syncthreads();
 }
 

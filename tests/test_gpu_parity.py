@@ -1,0 +1,130 @@
+"""GPU parity: the HIP engine (through the C ABI) against the FP64 CPU oracle on the same inputs.
+
+Tolerances are the engine's contract: total energy and HOMO/LUMO within 1e-6 eV, Mulliken
+charges within 1e-6 e. Iteration counts are reported and must agree within one (reordered
+reductions can move a discontinuous DIIS decision by one step).
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests import molecules as M
+
+pytestmark = pytest.mark.gpu
+
+E_TOL = 1e-6   # eV
+Q_TOL = 1e-6   # e
+ORB_TOL = 1e-6  # eV
+
+
+def _compare(r, o, iter_slack=1):
+    assert r["status"] == o["status"] == 0, (r.get("message"), o.get("message"))
+    assert abs(r["total_ev"] - o["total_ev"]) <= E_TOL
+    assert abs(r["heat_of_formation_kcal"] - o["heat_of_formation_kcal"]) <= E_TOL * 23.061
+    assert np.abs(r["charges"] - o["charges"]).max() <= Q_TOL
+    for k in ("homo_ev", "lumo_ev", "homo_beta_ev", "lumo_beta_ev"):
+        if o[k] is None:
+            assert r[k] is None
+        else:
+            assert abs(r[k] - o[k]) <= ORB_TOL, k
+    assert abs(r["iterations"] - o["iterations"]) <= iter_slack
+    assert np.allclose(r["dipole_debye"], o["dipole_debye"], atol=1e-5)
+
+
+CASES = dict((k, v[0]) for k, v in M.RECORDED_HOF.items() if k != "CH3Cl")
+CASES.update(water=M.WATER, formaldehyde=M.FORMALDEHYDE, methane=M.METHANE, co2=M.co2(1.16))
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_small_molecules_plain(engine, name):
+    z, x = M.parse(CASES[name])
+    _compare(engine.scf(z, x), O.scf(z, x))
+
+
+def test_chlorine_third_row(engine):
+    z, x = M.parse(M.RECORDED_HOF["CH3Cl"][0])
+    r = engine.scf(z, x)
+    _compare(r, O.scf(z, x))
+    assert abs(r["heat_of_formation_kcal"] - M.RECORDED_HOF["CH3Cl"][1]) < 1e-4
+
+
+@pytest.mark.parametrize("acc", ["none", "cdiis", "adiis_cdiis"])
+def test_accelerators(engine, acc):
+    z, x = M.parse(M.FORMALDEHYDE)
+    r = engine.scf(z, x, accelerator=acc)
+    o = O.scf(z, x, accelerator=acc)
+    _compare(r, o)
+    if acc == "adiis_cdiis":
+        assert r["iterations"] == 11  # scf.rs:1039-1040
+
+
+@pytest.mark.parametrize("method,want", [("am1", -77.16706), ("rm1", -81.09054)])
+def test_mopac_co2(engine, method, want):
+    z, x = M.parse(M.co2(1.16))
+    r = engine.scf(z, x, method=method)
+    _compare(r, O.scf(z, x, method=method))
+    assert abs(r["heat_of_formation_kcal"] - want) < 0.06
+
+
+def test_density_and_orbitals(engine):
+    z, x = M.parse(M.FORMALDEHYDE)
+    r = engine.scf(z, x)
+    o = O.scf(z, x)
+    assert np.abs(r["density"] - o["density"]).max() < 1e-6
+    assert np.abs(r["mo_energies"] - o["mo_energies"]).max() < 1e-6
+    C = r["mo_coeff"]
+    assert np.allclose(C.T @ C, np.eye(C.shape[0]), atol=1e-10)
+    P = r["density"]
+    assert np.allclose(P @ P, 2 * P, atol=1e-8)
+
+
+def test_error_paths(engine):
+    z, x = M.parse(M.WATER)
+    assert engine.scf(z, x, max_scf=3)["status"] == 4
+    assert engine.scf(z, x, multiplicity=2, reference="rhf")["status"] == 1
+    assert engine.scf(np.array([2, 1, 1]), x)["status"] == 2
+    bad = x.copy()
+    bad[0, 0] = np.nan
+    assert engine.scf(z, bad)["status"] == 1
+
+
+def test_water_cluster_32(engine):
+    from am1g import geometry as G
+
+    mol = G.water_cluster_config(0)
+    r = engine.scf(mol.z, mol.xyz_bohr, accelerator="cdiis")
+    o = O.scf(mol.z, mol.xyz_bohr, accelerator="cdiis")
+    _compare(r, o)
+
+
+def test_batch_matches_oracle_and_is_batch_invariant(engine):
+    from am1g import geometry as G
+
+    mols = G.chain_batch(96, seed=11)
+    ms = [(m.z, m.xyz_bohr) for m in mols]
+    rb = engine.scf_batch(ms)
+    ob = O.scf_many(ms, threads=8)
+    for a, b in zip(rb, ob):
+        _compare(a, b)
+    # the same molecule alone gives the same numbers as inside the batch
+    for i in (0, 37, 95):
+        alone = engine.scf_batch([ms[i]])[0]
+        assert alone["total_ev"] == rb[i]["total_ev"]
+        assert np.array_equal(alone["charges"], rb[i]["charges"])
+        assert alone["iterations"] == rb[i]["iterations"]
+
+
+def test_full_size_batch_properties(engine):
+    """4096 chains in one launch: every SCF converges, every neutral molecule's charges sum to
+    zero, and a sample of 48 agrees with the oracle."""
+    from am1g import geometry as G
+
+    mols = G.chain_batch(4096, seed=1)
+    ms = [(m.z, m.xyz_bohr) for m in mols]
+    rb = engine.scf_batch(ms)
+    assert all(r["status"] == 0 for r in rb)
+    assert max(abs(r["charges"].sum()) for r in rb) < 1e-8
+    idx = np.random.default_rng(0).choice(len(ms), 48, replace=False)
+    ob = O.scf_many([ms[i] for i in idx], threads=8)
+    for i, o in zip(idx, ob):
+        _compare(rb[i], o)
