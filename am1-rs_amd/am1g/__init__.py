@@ -78,6 +78,7 @@ class Result(ctypes.Structure):
         ("dipole_magnitude", ctypes.c_double),
         ("residual", ctypes.c_double),
         ("fp32_iterations", ctypes.c_int),
+        ("latency_ms", ctypes.c_double),
         ("charges", ctypes.POINTER(ctypes.c_double)),
         ("mo_energies", ctypes.POINTER(ctypes.c_double)),
         ("mo_energies_beta", ctypes.POINTER(ctypes.c_double)),
@@ -229,6 +230,7 @@ class Engine:
             "dipole_magnitude": rs.dipole_magnitude,
             "residual": rs.residual,
             "fp32_iterations": rs.fp32_iterations,
+            "latency_ms": rs.latency_ms,
             "charges": arrays["charges"].copy(),
         }
         if full and rs.status in (0, 4):

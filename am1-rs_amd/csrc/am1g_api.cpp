@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -397,12 +398,15 @@ int am1g_scf_batch(am1g_ctx* ctx, int n, const am1g_request* rqs, am1g_result* r
     float ms = 0.f;
     HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     ctx->last_ms = ms;
+    uint64_t t0 = UINT64_MAX;
+    for (int j = 0; j < m; ++j) t0 = std::min(t0, res[j].t_start);
     std::vector<double> mat;
     for (int j = 0; j < m; ++j) {
         const int i = ok_idx[j];
         const MolDesc& md = descs[j];
         am1g_result& r = results[i];
         fill_result(res[j], md, r);
+        r.latency_ms = (double)(res[j].t_done - t0) / 1.0e5;  // 100 MHz device clock
         if (r.charges) std::memcpy(r.charges, q.data() + md.atom_off, sizeof(double) * md.nat);
         if (r.mo_energies) std::memcpy(r.mo_energies, mo.data() + md.ao_off, sizeof(double) * md.nao);
         const size_t nn = (size_t)md.nao * md.nao;
@@ -423,7 +427,9 @@ int am1g_scf_batch(am1g_ctx* ctx, int n, const am1g_request* rqs, am1g_result* r
 
 int am1g_scf(am1g_ctx* ctx, const am1g_request* rq, am1g_result* res) {
     if (!ctx || !rq || !res) return set_error(ctx, AM1G_INVALID_INPUT, "null argument");
+    auto w0 = std::chrono::steady_clock::now();
     int st = am1g_scf_batch(ctx, 1, rq, res);
+    res->latency_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
     if (st != AM1G_OK) return st;
     return res->status;
 }

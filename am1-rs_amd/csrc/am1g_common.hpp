@@ -64,7 +64,7 @@ struct MolResult {
     double dipole[3];
     double residual;
     double fp32_iterations;  // iterations spent with fp32 GEMMs (mixed precision)
-    double pad;
+    uint64_t t_start, t_done;  // device real-time clock (100 MHz) when the molecule was taken / finished
 };
 
 // Dense matrices kept per molecule (each nao*nao doubles), in this order from mat_off.

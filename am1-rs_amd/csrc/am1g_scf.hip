@@ -681,11 +681,15 @@ __global__ __launch_bounds__(kWg) void k_scf_batch(BatchArgs g) {
         const int mi = g.order ? g.order[slot] : slot;
         const MolDesc md = g.mols[mi];
         MolResult res;
+        res.t_start = __builtin_amdgcn_s_memrealtime();
         double* M = g.mats + md.mat_off;
         double ecore = mol_setup(md, g, ms, s, M + kH * (size_t)md.nao * md.nao, g.wpool + md.w_off);
         __syncthreads();
         scf_rhf<PREC>(md, g, ms, s, lds, ecore, res);
-        if (threadIdx.x == 0) g.results[mi] = res;
+        if (threadIdx.x == 0) {
+            res.t_done = __builtin_amdgcn_s_memrealtime();
+            g.results[mi] = res;
+        }
         __syncthreads();
     }
 }

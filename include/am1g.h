@@ -73,6 +73,8 @@ typedef struct {
     double dipole_magnitude;
     double residual;          /* ||[F,P]|| of the last iteration */
     int fp32_iterations;      /* iterations run with FP32 GEMMs (mixed precision) */
+    double latency_ms;        /* batched: device time from the launch's first molecule start to this
+                                 molecule's completion; plain: wall time of the call */
     double* charges;          /* [n_atoms] Mulliken charges */
     double* mo_energies;      /* [nao] (alpha for UHF) */
     double* mo_energies_beta; /* [nao] UHF only */
