@@ -30,6 +30,7 @@ ACCELERATORS = {"none": 0, "cdiis": 1, "adiis_cdiis": 2, "adiis": 2}
 PRECISIONS = {"fp64": 0, "mixed": 1}
 STATUS_NAMES = {0: "Ok", 1: "InvalidInput", 2: "ElementNotParameterized", 3: "LinearAlgebra",
                 4: "ScfNotConverged", 5: "ScfDiverged", 6: "DeviceError"}
+PHASES = ["setup", "fock", "commutator+diis", "eigen", "density", "final"]
 SYMBOLS = {"H": 1, "Be": 4, "B": 5, "C": 6, "N": 7, "O": 8, "F": 9, "Al": 13, "Si": 14, "P": 15, "S": 16,
            "Cl": 17, "Zn": 30, "Ge": 32, "As": 33, "Se": 34, "Br": 35, "Sb": 51, "Te": 52, "I": 53, "Hg": 80}
 
@@ -79,6 +80,8 @@ class Result(ctypes.Structure):
         ("residual", ctypes.c_double),
         ("fp32_iterations", ctypes.c_int),
         ("latency_ms", ctypes.c_double),
+        ("phase_ms", ctypes.c_double * 6),
+        ("eigen_sweeps", ctypes.c_int),
         ("charges", ctypes.POINTER(ctypes.c_double)),
         ("mo_energies", ctypes.POINTER(ctypes.c_double)),
         ("mo_energies_beta", ctypes.POINTER(ctypes.c_double)),
@@ -231,6 +234,8 @@ class Engine:
             "residual": rs.residual,
             "fp32_iterations": rs.fp32_iterations,
             "latency_ms": rs.latency_ms,
+            "phase_ms": dict(zip(PHASES, list(rs.phase_ms))),
+            "eigen_sweeps": rs.eigen_sweeps,
             "charges": arrays["charges"].copy(),
         }
         if full and rs.status in (0, 4):

@@ -53,6 +53,7 @@ struct am1g_ctx {
     std::string err;
     DevElem host_tab[kMethods * kMaxZ];
     DevBuf elems, mols, z, xyz, mats, wpool, hist, mo_e, mo_eb, charges, results, queue, order;
+    DevBuf alist, aoff, acls, pair_off, ecore_pair;
 };
 
 namespace {
@@ -317,8 +318,9 @@ int am1g_scf_batch(am1g_ctx* ctx, int n, const am1g_request* rqs, am1g_result* r
     if (m == 0) return AM1G_OK;
     // pool layout
     std::vector<MolDesc> descs(m);
-    std::vector<int> zs;
+    std::vector<int> zs, alist, aoff, acls;
     std::vector<double> xyz;
+    std::vector<int64_t> pair_off(1, 0);
     int64_t atom_off = 0, ao_off = 0, mat_off = 0, w_off = 0, hist_off = 0;
     int max_nao = 0;
     for (int j = 0; j < m; ++j) {
@@ -329,9 +331,29 @@ int am1g_scf_batch(am1g_ctx* ctx, int n, const am1g_request* rqs, am1g_result* r
         md.mat_off = mat_off;
         md.w_off = w_off;
         md.hist_off = hist_off;
-        for (int a = 0; a < rq.n_atoms; ++a) {
-            zs.push_back(rq.z[a]);
-            for (int k = 0; k < 3; ++k) xyz.push_back(rq.xyz_bohr[3 * a + k]);
+        {
+            // heavy atoms first, then hydrogens, each in input order; AO offsets s,px,py,pz per
+            // heavy atom and s per hydrogen in input order (basis.rs:37-60)
+            const DevElem* tab = ctx->host_tab + md.method * kMaxZ;
+            int off = 0, nh = 0, nH = 0;
+            std::vector<int> heavy, hyd;
+            for (int a = 0; a < rq.n_atoms; ++a) {
+                int zz = rq.z[a];
+                zs.push_back(zz);
+                for (int k = 0; k < 3; ++k) xyz.push_back(rq.xyz_bohr[3 * a + k]);
+                aoff.push_back(off);
+                off += tab[zz].norb;
+                if (zz != 1) {
+                    acls.push_back(nh++);
+                    heavy.push_back(a);
+                } else {
+                    acls.push_back(nH++);
+                    hyd.push_back(a);
+                }
+            }
+            alist.insert(alist.end(), heavy.begin(), heavy.end());
+            alist.insert(alist.end(), hyd.begin(), hyd.end());
+            pair_off.push_back(pair_off.back() + (int64_t)rq.n_atoms * (rq.n_atoms - 1) / 2);
         }
         atom_off += rq.n_atoms;
         ao_off += md.nao;
@@ -358,17 +380,27 @@ int am1g_scf_batch(am1g_ctx* ctx, int n, const am1g_request* rqs, am1g_result* r
     HIP_TRY(ctx, ctx->results.reserve(sizeof(MolResult) * m));
     HIP_TRY(ctx, ctx->queue.reserve(sizeof(int) * 4));
     HIP_TRY(ctx, ctx->order.reserve(sizeof(int) * m));
+    HIP_TRY(ctx, ctx->alist.reserve(sizeof(int) * alist.size()));
+    HIP_TRY(ctx, ctx->aoff.reserve(sizeof(int) * aoff.size()));
+    HIP_TRY(ctx, ctx->acls.reserve(sizeof(int) * acls.size()));
+    HIP_TRY(ctx, ctx->pair_off.reserve(sizeof(int64_t) * pair_off.size()));
+    HIP_TRY(ctx, ctx->ecore_pair.reserve(sizeof(double) * std::max<int64_t>(pair_off.back(), 1)));
     hipStream_t st = ctx->stream;
     HIP_TRY(ctx, hipMemcpyAsync(ctx->mols.p, descs.data(), sizeof(MolDesc) * m, hipMemcpyHostToDevice, st));
     HIP_TRY(ctx, hipMemcpyAsync(ctx->z.p, zs.data(), sizeof(int) * zs.size(), hipMemcpyHostToDevice, st));
     HIP_TRY(ctx, hipMemcpyAsync(ctx->xyz.p, xyz.data(), sizeof(double) * xyz.size(), hipMemcpyHostToDevice, st));
     HIP_TRY(ctx, hipMemcpyAsync(ctx->order.p, order.data(), sizeof(int) * m, hipMemcpyHostToDevice, st));
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->alist.p, alist.data(), sizeof(int) * alist.size(), hipMemcpyHostToDevice, st));
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->aoff.p, aoff.data(), sizeof(int) * aoff.size(), hipMemcpyHostToDevice, st));
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->acls.p, acls.data(), sizeof(int) * acls.size(), hipMemcpyHostToDevice, st));
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->pair_off.p, pair_off.data(), sizeof(int64_t) * pair_off.size(),
+                                hipMemcpyHostToDevice, st));
     HIP_TRY(ctx, hipMemsetAsync(ctx->queue.p, 0, sizeof(int) * 4, st));
 
     BatchArgs a;
     a.mols = ctx->mols.as<MolDesc>();
     a.n_mols = m;
-    a.lds_n = std::min(max_nao, 128);
+    a.lds_n = 128;
     a.z = ctx->z.as<int>();
     a.xyz = ctx->xyz.as<double>();
     a.elems = ctx->elems.as<DevElem>();
@@ -381,11 +413,18 @@ int am1g_scf_batch(am1g_ctx* ctx, int n, const am1g_request* rqs, am1g_result* r
     a.results = ctx->results.as<MolResult>();
     a.queue = ctx->queue.as<int>();
     a.order = ctx->order.as<int>();
-    size_t lds = std::max<size_t>((size_t)a.lds_n * a.lds_n * sizeof(double), 1024 * sizeof(double));
+    a.alist = ctx->alist.as<int>();
+    a.aoff = ctx->aoff.as<int>();
+    a.acls = ctx->acls.as<int>();
+    a.pair_off = ctx->pair_off.as<int64_t>();
+    a.n_pairs = pair_off.back();
+    a.ecore_pair = ctx->ecore_pair.as<double>();
+    size_t lds = (size_t)kLdsBytes;
     int precision = 0;
     for (int j = 0; j < m; ++j) precision |= descs[j].precision;
-    int grid = std::min(m, ctx->cus * 2);
+    int grid = std::min(m, ctx->cus);
     HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
+    HIP_TRY(ctx, launch_pairs(a, st));
     HIP_TRY(ctx, launch_scf_batch(a, precision, grid, lds, st));
     HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
 
@@ -407,6 +446,8 @@ int am1g_scf_batch(am1g_ctx* ctx, int n, const am1g_request* rqs, am1g_result* r
         am1g_result& r = results[i];
         fill_result(res[j], md, r);
         r.latency_ms = (double)(res[j].t_done - t0) / 1.0e5;  // 100 MHz device clock
+        for (int k = 0; k < kPhases; ++k) r.phase_ms[k] = (double)res[j].phase[k] / 1.0e5;
+        r.eigen_sweeps = res[j].eigen_sweeps;
         if (r.charges) std::memcpy(r.charges, q.data() + md.atom_off, sizeof(double) * md.nat);
         if (r.mo_energies) std::memcpy(r.mo_energies, mo.data() + md.ao_off, sizeof(double) * md.nao);
         const size_t nn = (size_t)md.nao * md.nao;

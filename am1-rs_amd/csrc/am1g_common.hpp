@@ -65,7 +65,14 @@ struct MolResult {
     double residual;
     double fp32_iterations;  // iterations spent with fp32 GEMMs (mixed precision)
     uint64_t t_start, t_done;  // device real-time clock (100 MHz) when the molecule was taken / finished
+    uint64_t phase[6];         // device clock ticks per SCF phase (see kPhaseNames)
+    int eigen_sweeps;          // Jacobi sweeps summed over the SCF
+    int pad1;
 };
+
+// Phase names of MolResult::phase, in the spirit of am1-rs's AM1_TIMING timers
+// (scf.rs:315, 592-660: basis+core, scf:fock, scf:commutator + scf:accel, scf:eigen, scf:density).
+constexpr int kPhases = 6;
 
 // Dense matrices kept per molecule (each nao*nao doubles), in this order from mat_off.
 enum MatSlot : int {

@@ -72,103 +72,109 @@ __device__ __forceinline__ BlockRef block_of(const MolShared& ms, const double* 
 }
 
 // ---------------------------------------------------------------------------------------------
-// Setup: pair integrals, resonance blocks, core-core energy, H_core diagonal blocks.
-__device__ double mol_setup(const MolDesc& md, const BatchArgs& g, MolShared& ms, WgScratch& s, double* H,
-                            double* wpool) {
+// k_pairs: one thread per atom pair of the whole batch. Writes the pair's rotated two-electron
+// block into the integral pool, its resonance block 1/2 (beta_mu + beta_nu) S_mu_nu into both
+// triangles of H_core (each off-diagonal atom block belongs to exactly one pair), and its
+// core-core repulsion (hamiltonian.rs:281-344, repulsion.rs:144-167).
+__global__ __launch_bounds__(256) void k_pairs(BatchArgs g) {
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < g.n_pairs;
+         p += (int64_t)gridDim.x * blockDim.x) {
+        // molecule of this pair: last m with pair_off[m] <= p
+        int lo = 0, hi = g.n_mols - 1;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (g.pair_off[mid] <= p) lo = mid;
+            else hi = mid - 1;
+        }
+        const MolDesc md = g.mols[lo];
+        const int64_t k_all = p - g.pair_off[lo];
+        const DevElem* el = g.elems + md.method * kMaxZ;
+        const double* xyz = g.xyz + md.atom_off * 3;
+        const int* z = g.z + md.atom_off;
+        const int* alist = g.alist + md.atom_off;
+        const int* aoff = g.aoff + md.atom_off;
+        const int nh = md.n_heavy, nH = md.n_h, nao = md.nao;
+        const int64_t n_hh = (int64_t)nh * (nh - 1) / 2, n_hH = (int64_t)nh * nH;
+        double* H = g.mats + md.mat_off + kH * (size_t)nao * nao;
+        double* wpool = g.wpool + md.w_off;
+        int a, b;
+        if (k_all < n_hh) {
+            int i, j;
+            tri_decode(k_all, nh, i, j);
+            a = alist[i];
+            b = alist[j];
+        } else if (k_all < n_hh + n_hH) {
+            int64_t k = k_all - n_hh;
+            a = alist[(int)(k / nH)];
+            b = alist[nh + (int)(k % nH)];
+        } else {
+            int i, j;
+            tri_decode(k_all - n_hh - n_hH, nH, i, j);
+            a = alist[nh + i];
+            b = alist[nh + j];
+        }
+        const DevElem& ea = el[z[a]];
+        const DevElem& eb = el[z[b]];
+        double d[3] = {xyz[3 * b] - xyz[3 * a], xyz[3 * b + 1] - xyz[3 * a + 1], xyz[3 * b + 2] - xyz[3 * a + 2]};
+        double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        const int oa = aoff[a], ob = aoff[b];
+        double S[16];
+        overlap_block(ea, eb, d, S);
+        if (k_all < n_hh) {
+            heavy_heavy_block(ea, eb, d, wpool + k_all * 100);
+#pragma unroll
+            for (int i2 = 0; i2 < 4; ++i2)
+#pragma unroll
+                for (int j2 = 0; j2 < 4; ++j2) {
+                    double v = 0.5 * ((i2 == 0 ? ea.bs : ea.bp) + (j2 == 0 ? eb.bs : eb.bp)) * S[i2 * 4 + j2];
+                    H[(size_t)(oa + i2) * nao + ob + j2] = v;
+                    H[(size_t)(ob + j2) * nao + oa + i2] = v;
+                }
+        } else if (k_all < n_hh + n_hH) {
+            double w[10];
+            heavy_h_block(ea, eb, d, w);
+            double* dst = wpool + n_hh * 100 + (k_all - n_hh) * 10;
+#pragma unroll
+            for (int q = 0; q < 10; ++q) dst[q] = w[q];
+#pragma unroll
+            for (int i2 = 0; i2 < 4; ++i2) {
+                double v = 0.5 * ((i2 == 0 ? ea.bs : ea.bp) + eb.bs) * S[i2 * 4];
+                H[(size_t)(oa + i2) * nao + ob] = v;
+                H[(size_t)ob * nao + oa + i2] = v;
+            }
+        } else {
+            wpool[n_hh * 100 + n_hH * 10 + (k_all - n_hh - n_hH)] = hh_integral(ea, eb, r);
+            double v = 0.5 * (ea.bs + eb.bs) * S[0];
+            H[(size_t)oa * nao + ob] = v;
+            H[(size_t)ob * nao + oa] = v;
+        }
+        // the core-core term is symmetric in the pair; evaluate it in index order (i < j)
+        const int lo_a = min(a, b), hi_a = max(a, b);
+        g.ecore_pair[p] = core_core_pair(el[z[lo_a]], el[z[hi_a]], z[lo_a], z[hi_a], r);
+    }
+}
+
+// Per-molecule setup inside the SCF kernel: atom tables into LDS, H_core diagonal blocks
+// U + sum over partners of (-Z_y)(mu nu|s_y s_y) (hamiltonian.rs:229-332), core-core sum.
+__device__ double mol_setup(const MolDesc& md, int mi, const BatchArgs& g, MolShared& ms, WgScratch& s, double* H,
+                            const double* wpool) {
     const int t = threadIdx.x;
     const int nat = md.nat, nao = md.nao;
     const DevElem* el = g.elems + md.method * kMaxZ;
-    const double* xyz = g.xyz + md.atom_off * 3;
+    for (int a = t; a < nat; a += kWg) {
+        ms.z[a] = g.z[md.atom_off + a];
+        ms.off[a] = g.aoff[md.atom_off + a];
+        ms.cls[a] = g.acls[md.atom_off + a];
+        int x = g.alist[md.atom_off + a];
+        if (a < md.n_heavy) ms.heavy[a] = x;
+        else ms.hyd[a - md.n_heavy] = x;
+    }
     if (t == 0) {
-        int off = 0, nh = 0, nH = 0;
-        for (int a = 0; a < nat; ++a) {
-            int zz = g.z[md.atom_off + a];
-            ms.z[a] = zz;
-            ms.off[a] = off;
-            off += el[zz].norb;
-            if (zz != 1) {
-                ms.cls[a] = nh;
-                ms.heavy[nh++] = a;
-            } else {
-                ms.cls[a] = nH;
-                ms.hyd[nH++] = a;
-            }
-        }
-        ms.nh = nh;
-        ms.nH = nH;
+        ms.nh = md.n_heavy;
+        ms.nH = md.n_h;
     }
     __syncthreads();
     const int nh = ms.nh, nH = ms.nH;
-    const int64_t n_hh = (int64_t)nh * (nh - 1) / 2, n_hH = (int64_t)nh * nH, n_HH = (int64_t)nH * (nH - 1) / 2;
-    double ecore = 0.0;
-    // heavy-heavy pairs
-    for (int64_t k = t; k < n_hh; k += kWg) {
-        int i, j;
-        tri_decode(k, nh, i, j);
-        int a = ms.heavy[i], b = ms.heavy[j];
-        const DevElem& ea = el[ms.z[a]];
-        const DevElem& eb = el[ms.z[b]];
-        double d[3] = {xyz[3 * b] - xyz[3 * a], xyz[3 * b + 1] - xyz[3 * a + 1], xyz[3 * b + 2] - xyz[3 * a + 2]};
-        heavy_heavy_block(ea, eb, d, wpool + k * 100);
-        double S[16];
-        overlap_block(ea, eb, d, S);
-        int oa = ms.off[a], ob = ms.off[b];
-#pragma unroll
-        for (int i2 = 0; i2 < 4; ++i2)
-#pragma unroll
-            for (int j2 = 0; j2 < 4; ++j2) {
-                double bi = i2 == 0 ? ea.bs : ea.bp, bj = j2 == 0 ? eb.bs : eb.bp;
-                double v = 0.5 * (bi + bj) * S[i2 * 4 + j2];
-                H[(size_t)(oa + i2) * nao + ob + j2] = v;
-                H[(size_t)(ob + j2) * nao + oa + i2] = v;
-            }
-        ecore += core_core_pair(ea, eb, ms.z[a], ms.z[b], sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]));
-    }
-    // heavy-H pairs (a heavy, b hydrogen; displacement R_b - R_a)
-    for (int64_t k = t; k < n_hH; k += kWg) {
-        int i = (int)(k / nH), m = (int)(k - (int64_t)i * nH);
-        int a = ms.heavy[i], b = ms.hyd[m];
-        const DevElem& ea = el[ms.z[a]];
-        const DevElem& eb = el[1];
-        double d[3] = {xyz[3 * b] - xyz[3 * a], xyz[3 * b + 1] - xyz[3 * a + 1], xyz[3 * b + 2] - xyz[3 * a + 2]};
-        double w[10];
-        heavy_h_block(ea, eb, d, w);
-        double* dst = wpool + n_hh * 100 + k * 10;
-#pragma unroll
-        for (int q = 0; q < 10; ++q) dst[q] = w[q];
-        double S[16];
-        overlap_block(ea, eb, d, S);
-        int oa = ms.off[a], ob = ms.off[b];
-#pragma unroll
-        for (int i2 = 0; i2 < 4; ++i2) {
-            double v = 0.5 * ((i2 == 0 ? ea.bs : ea.bp) + eb.bs) * S[i2 * 4];
-            H[(size_t)(oa + i2) * nao + ob] = v;
-            H[(size_t)ob * nao + oa + i2] = v;
-        }
-        // core-core energy is symmetric in the pair; the reference evaluates it in i<j order
-        int lo = min(a, b), hi = max(a, b);
-        ecore += core_core_pair(el[ms.z[lo]], el[ms.z[hi]], ms.z[lo], ms.z[hi],
-                                sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]));
-    }
-    // H-H pairs
-    for (int64_t k = t; k < n_HH; k += kWg) {
-        int i, j;
-        tri_decode(k, nH, i, j);
-        int a = ms.hyd[i], b = ms.hyd[j];
-        const DevElem& e1 = el[1];
-        double d[3] = {xyz[3 * b] - xyz[3 * a], xyz[3 * b + 1] - xyz[3 * a + 1], xyz[3 * b + 2] - xyz[3 * a + 2]};
-        double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-        wpool[n_hh * 100 + n_hH * 10 + k] = hh_integral(e1, e1, r);
-        double S[16];
-        overlap_block(e1, e1, d, S);
-        double v = 0.5 * (e1.bs + e1.bs) * S[0];
-        int oa = ms.off[a], ob = ms.off[b];
-        H[(size_t)oa * nao + ob] = v;
-        H[(size_t)ob * nao + oa] = v;
-        ecore += core_core_pair(e1, e1, 1, 1, r);
-    }
-    __syncthreads();
-    // diagonal blocks: U + sum over partners of (-Z_y) (mu nu | s_y s_y)
     const int n_items = nh * 10 + nH;
     for (int it = t; it < n_items; it += kWg) {
         int x, k;
@@ -193,6 +199,9 @@ __device__ double mol_setup(const MolDesc& md, const BatchArgs& g, MolShared& ms
         H[(size_t)(o + mu) * nao + o + nu] = u + acc;
         H[(size_t)(o + nu) * nao + o + mu] = u + acc;
     }
+    double ecore = 0.0;
+    const int64_t p0 = g.pair_off[mi], p1 = g.pair_off[mi + 1];
+    for (int64_t p = p0 + t; p < p1; p += kWg) ecore += g.ecore_pair[p];
     ecore = wg_sum(ecore, s);
     return ecore;
 }
@@ -200,7 +209,7 @@ __device__ double mol_setup(const MolDesc& md, const BatchArgs& g, MolShared& ms
 // F = H + one-centre(P, P_spin) + J(P) - K(scale * P_spin) for one spin channel
 // (fock.rs:249-511). Diagonal atom blocks: one thread per (atom, packed orbital pair), summing
 // over partner atoms in index order. Off-diagonal blocks: one thread per pair.
-__device__ void mol_fock(const MolDesc& md, const DevElem* el, const MolShared& ms, const double* wpool,
+__device__ __noinline__ void mol_fock(const MolDesc& md, const DevElem* el, const MolShared& ms, const double* wpool,
                          const double* H, const double* Pt, const double* Ps, double scale, double* F) {
     const int t = threadIdx.x;
     const int nat = md.nat, nao = md.nao;
@@ -254,22 +263,40 @@ __device__ void mol_fock(const MolDesc& md, const DevElem* el, const MolShared& 
         int a = ms.heavy[i], b = ms.heavy[j];
         int oa = ms.off[a], ob = ms.off[b];
         const double* w = wpool + k * 100;
-        double pab[16];
+        double pab[16], acc[16];
 #pragma unroll
         for (int u = 0; u < 4; ++u)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) pab[u * 4 + v] = scale * Ps[(size_t)(oa + u) * nao + ob + v];
+            for (int v = 0; v < 4; ++v) {
+                pab[u * 4 + v] = scale * Ps[(size_t)(oa + u) * nao + ob + v];
+                acc[u * 4 + v] = 0.0;
+            }
+        // K[mu][la] = sum_{nu,si} Ps[nu][si] (mu nu | la si): walk the 10 packed bra rows once,
+        // each serving (mu, nu) and, off the diagonal, (nu, mu)
+        for (int kb = 0; kb < 10; ++kb) {
+            const int mu = kPackRow[kb], nu = kPackCol[kb];
+            double row[10];
+#pragma unroll
+            for (int q = 0; q < 10; ++q) row[q] = w[kb * 10 + q];
+#pragma unroll
+            for (int la = 0; la < 4; ++la) {
+                double x = 0.0, y = 0.0;
+#pragma unroll
+                for (int si = 0; si < 4; ++si) {
+                    const double wv = row[pack4(la, si)];
+                    x += pab[nu * 4 + si] * wv;
+                    y += pab[mu * 4 + si] * wv;
+                }
+                acc[mu * 4 + la] += x;
+                if (mu != nu) acc[nu * 4 + la] += y;
+            }
+        }
 #pragma unroll
         for (int mu = 0; mu < 4; ++mu)
 #pragma unroll
             for (int la = 0; la < 4; ++la) {
-                double acc = 0.0;
-#pragma unroll
-                for (int nu = 0; nu < 4; ++nu)
-#pragma unroll
-                    for (int si = 0; si < 4; ++si) acc += pab[nu * 4 + si] * w[pack4(mu, nu) * 10 + pack4(la, si)];
-                F[(size_t)(oa + mu) * nao + ob + la] = H[(size_t)(oa + mu) * nao + ob + la] - acc;
-                F[(size_t)(ob + la) * nao + oa + mu] = H[(size_t)(ob + la) * nao + oa + mu] - acc;
+                F[(size_t)(oa + mu) * nao + ob + la] = H[(size_t)(oa + mu) * nao + ob + la] - acc[mu * 4 + la];
+                F[(size_t)(ob + la) * nao + oa + mu] = H[(size_t)(ob + la) * nao + oa + mu] - acc[mu * 4 + la];
             }
     }
     for (int64_t k = t; k < n_hH; k += kWg) {
@@ -300,68 +327,101 @@ __device__ void mol_fock(const MolDesc& md, const DevElem* el, const MolShared& 
     }
 }
 
+// LDS of one workgroup: 150 KiB of dynamic space (the static part holds the molecule tables).
+// The eigensolver keeps A (packed lower triangle) there and, while it fits, the eigenvector
+// matrix too; the GEMMs use the whole region as panel scratch when the eigensolver is idle.
+constexpr int kLdsDoubles = 19200;
+constexpr int kALds = 128;  // largest order diagonalized with A in LDS
+
 // P = w * C_occ C_occ^T from transposed eigenvectors Ct (row k = MO k): P = w Ct_occ^T Ct_occ.
 template <typename T>
-__device__ void mol_density(int nao, int nocc, double w, const double* Ct, double* P) {
+__device__ void mol_density(int nao, int nocc, double w, const double* Ct, double* P, double* lds) {
     if (nocc == 0) {
         for (int idx = threadIdx.x; idx < nao * nao; idx += kWg) P[idx] = 0.0;
         return;
     }
-    wg_gemm<T, true, false>(nao, nao, nocc, w, Ct, nao, Ct, nao, P, nao);
-}
-
-// Sort eigenvalues ascending (stable, by rank) and permute the rows of Ct into Cs.
-__device__ void mol_sort_eigen(int n, const double* A, int lda, const double* Ct, double* Cs, double* eps) {
-    for (int i = threadIdx.x; i < n; i += kWg) {
-        double ei = A[(size_t)i * lda + i];
-        int rank = 0;
-        for (int j = 0; j < n; ++j) {
-            double ej = A[(size_t)j * lda + j];
-            rank += (ej < ei) || (ej == ei && j < i);
-        }
-        eps[rank] = ei;
-        for (int c = 0; c < n; ++c) Cs[(size_t)rank * n + c] = Ct[(size_t)i * n + c];
-    }
+    wg_gemm_any<T, true, false>(nao, nao, nocc, w, Ct, nao, Ct, nao, P, nao, lds, kLdsDoubles);
 }
 
 // One spin channel's eigen-step: A = Ct F Ct^T (warm start in the previous orbitals, or F itself
-// on the first pass), Jacobi, sort. Ct is updated in place; eps receives sorted eigenvalues.
+// on the first pass), Jacobi, sort ascending. Ct is replaced by the sorted eigenvectors
+// (transposed) and eps by the sorted eigenvalues.
 template <typename T>
 __device__ int mol_eigen(int nao, const double* Fuse, double* Ct, double* T0, double* T1, double* lds, int lds_n,
-                         bool warm, double* eps, WgScratch& s) {
+                         bool warm, double* eps, WgScratch& s, double tol_abs) {
     const int t = threadIdx.x;
-    double* A;
-    int lda = nao;
-    A = (nao <= lds_n) ? lds : T1;
+    const int n = nao;
+    const size_t nn = (size_t)n * n;
+    const size_t tri = (size_t)n * (n + 1) / 2;
+    const bool a_lds = n <= kALds && n <= lds_n;
+    const bool v_lds = a_lds && tri + nn <= (size_t)kLdsDoubles;
+    double* V = v_lds ? lds + tri : Ct;
+    const double* src = Fuse;
     if (warm) {
-        // T0 = Ct F  (row k: sum_nu Ct[k][nu] F[nu][mu]); A = T0 Ct^T
-        wg_gemm<T, false, false>(nao, nao, nao, 1.0, Ct, nao, Fuse, nao, T0, nao);
+        // T0 = Ct F ; T1 = T0 Ct^T = the Fock matrix in the previous orbital basis
+        wg_gemm_any<T, false, false>(n, n, n, 1.0, Ct, n, Fuse, n, T0, n, lds, kLdsDoubles);
         __syncthreads();
-        wg_gemm<T, false, true>(nao, nao, nao, 1.0, T0, nao, Ct, nao, A, lda);
+        wg_gemm_any<T, false, true>(n, n, n, 1.0, T0, n, Ct, n, T1, n, lds, kLdsDoubles);
         __syncthreads();
-        // symmetrize the rotated matrix (the products are symmetric up to rounding)
-        for (int idx = t; idx < nao * nao; idx += kWg) {
-            int i = idx / nao, j = idx - i * nao;
-            if (i < j) {
-                double v = 0.5 * (A[(size_t)i * lda + j] + A[(size_t)j * lda + i]);
-                A[(size_t)i * lda + j] = v;
-                A[(size_t)j * lda + i] = v;
-            }
+        src = T1;
+    }
+    int sweeps;
+    if (a_lds) {
+        // packed lower triangle, symmetrized; eigenvectors start from the previous ones
+        for (size_t k = t; k < tri; k += kWg) {
+            int i = (int)((sqrt(8.0 * (double)k + 1.0) - 1.0) * 0.5);
+            while ((size_t)i * (i + 1) / 2 > k) --i;
+            while ((size_t)(i + 1) * (i + 2) / 2 <= k) ++i;
+            int j = (int)(k - (size_t)i * (i + 1) / 2);
+            lds[k] = (i == j) ? src[(size_t)i * n + i] : 0.5 * (src[(size_t)i * n + j] + src[(size_t)j * n + i]);
         }
+        for (size_t idx = t; idx < nn; idx += kWg) {
+            int i = (int)(idx / n), j = (int)(idx - (size_t)i * n);
+            if (!warm) V[idx] = (i == j) ? 1.0 : 0.0;
+            else if (v_lds) V[idx] = Ct[idx];
+        }
+        __syncthreads();
+        sweeps = v_lds ? wg_jacobi_lds<true>(n, (int)tri, nullptr, s, 60, tol_abs)
+                       : wg_jacobi_lds<false>(n, 0, Ct, s, 60, tol_abs);
     } else {
-        for (int idx = t; idx < nao * nao; idx += kWg) {
-            int i = idx / nao, j = idx - i * nao;
-            A[(size_t)i * lda + j] = Fuse[idx];
-            Ct[idx] = (i == j) ? 1.0 : 0.0;
-        }
+        double* A = T1;
+        if (!warm)
+            for (size_t idx = t; idx < nn; idx += kWg) {
+                int i = (int)(idx / n), j = (int)(idx - (size_t)i * n);
+                A[idx] = Fuse[idx];
+                Ct[idx] = (i == j) ? 1.0 : 0.0;
+            }
+        else
+            for (size_t idx = t; idx < nn; idx += kWg) {
+                int i = (int)(idx / n), j = (int)(idx - (size_t)i * n);
+                if (i < j) {
+                    double v = 0.5 * (A[idx] + A[(size_t)j * n + i]);
+                    A[idx] = v;
+                    A[(size_t)j * n + i] = v;
+                }
+            }
+        __syncthreads();
+        sweeps = wg_jacobi_big(n, A, n, Ct, n, s, 60, tol_abs);
     }
     __syncthreads();
-    int sweeps = wg_jacobi(nao, A, lda, Ct, nao, s, 60);
+    // sort by rank (stable on ties) and write the permuted eigenvectors out
+    double* dst = v_lds ? Ct : T0;
+    for (int i = t / 16; i < n; i += kWg / 16) {
+        const double ei = a_lds ? lds[(size_t)i * (i + 1) / 2 + i] : T1[(size_t)i * n + i];
+        int rank = 0;
+        for (int j = t % 16; j < n; j += 16) {
+            const double ej = a_lds ? lds[(size_t)j * (j + 1) / 2 + j] : T1[(size_t)j * n + j];
+            rank += (ej < ei) || (ej == ei && j < i);
+        }
+        for (int off = 8; off > 0; off >>= 1) rank += __shfl_xor(rank, off, 16);
+        if (t % 16 == 0) eps[rank] = ei;
+        for (int c = t % 16; c < n; c += 16) dst[(size_t)rank * n + c] = V[(size_t)i * n + c];
+    }
     __syncthreads();
-    mol_sort_eigen(nao, A, lda, Ct, T0, eps);
-    __syncthreads();
-    for (int idx = t; idx < nao * nao; idx += kWg) Ct[idx] = T0[idx];
-    __syncthreads();
+    if (!v_lds) {
+        for (size_t idx = t; idx < nn; idx += kWg) Ct[idx] = T0[idx];
+        __syncthreads();
+    }
     return sweeps;
 }
 
@@ -390,10 +450,25 @@ __device__ __forceinline__ DiisView diis_view(double* hist, int nao, int channel
     return v;
 }
 
+// Per-phase device clock, kept by thread 0 (phases: 0 setup, 1 Fock + energy, 2 commutator +
+// DIIS, 3 eigensolve, 4 density + convergence, 5 final energy and properties).
+struct PhaseClock {
+    uint64_t last;
+    uint64_t acc[kPhases];
+};
+
+__device__ __forceinline__ void phase_mark(PhaseClock& pc, int k) {
+    if (threadIdx.x == 0) {
+        uint64_t now = __builtin_amdgcn_s_memrealtime();
+        pc.acc[k] += now - pc.last;
+        pc.last = now;
+    }
+}
+
 // ---------------------------------------------------------------------------------------------
 template <int PREC>
 __device__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolShared& ms, WgScratch& s, double* lds,
-                        double ecore, MolResult& res) {
+                        double ecore, MolResult& res, PhaseClock& pc) {
     const int t = threadIdx.x;
     const int nao = md.nao, nat = md.nat, nocc = md.n_alpha;
     const DevElem* el = g.elems + md.method * kMaxZ;
@@ -431,7 +506,7 @@ __device__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolShared& ms, Wg
     double e_old = 0.0, residual = INFINITY;
     int iterations = 0, converged = 0, status = kOk;
     bool have_c = false;
-    int fp32_iters = 0;
+    int fp32_iters = 0, total_sweeps = 0;
     for (int iter = 0; iter < md.max_scf; ++iter) {
         iterations = iter + 1;
         mol_fock(md, el, ms, wpool, H, P, P, 0.5, F);
@@ -440,13 +515,14 @@ __device__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolShared& ms, Wg
         double pe = 0.0;
         for (size_t idx = t; idx < nn; idx += kWg) pe += P[idx] * (H[idx] + F[idx]);
         double e_elec = 0.5 * wg_sum(pe, s);
+        phase_mark(pc, 1);
         // error = FP - (FP)^T; F and P are symmetric so FP = F^T P
         bool low = (PREC == 1) && (residual > 1.0e-4);
         if (low) {
-            wg_gemm<float, true, false>(nao, nao, nao, 1.0, F, nao, P, nao, T0, nao);
+            wg_gemm_any<float, true, false>(nao, nao, nao, 1.0, F, nao, P, nao, T0, nao, lds, kLdsDoubles);
             ++fp32_iters;
         } else {
-            wg_gemm<double, true, false>(nao, nao, nao, 1.0, F, nao, P, nao, T0, nao);
+            wg_gemm_any<double, true, false>(nao, nao, nao, 1.0, F, nao, P, nao, T0, nao, lds, kLdsDoubles);
         }
         __syncthreads();
         // push history: logical slot = count (or overwrite oldest)
@@ -495,7 +571,9 @@ __device__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolShared& ms, Wg
             const double* en_p = dv.e + (int64_t)ms.ring[newest] * dv.len_a;
             for (int64_t idx = t; idx < dv.len_a; idx += kWg) {
                 double x = en_p[idx];
-                for (int i = 0; i < m; ++i) part[i] += x * dv.e[(int64_t)ms.ring[i] * dv.len_a + idx];
+#pragma unroll
+                for (int i = 0; i < kDiisDepth; ++i)
+                    if (i < m) part[i] += x * dv.e[(int64_t)ms.ring[i] * dv.len_a + idx];
             }
             if (need_d) {
                 const double* dn = dv.d + (int64_t)ms.ring[newest] * dv.len_s;
@@ -513,7 +591,9 @@ __device__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolShared& ms, Wg
                     }
                     double wgt = (tri_pos(i, i, nao) == idx) ? 1.0 : 2.0;
                     double dnv = dn[idx] * wgt, fnv = fn[idx] * wgt;
-                    for (int j = 0; j < m; ++j) {
+#pragma unroll
+                    for (int j = 0; j < kDiisDepth; ++j) {
+                        if (j >= m) continue;
                         part[kDiisDepth + j] += dnv * dv.f[(int64_t)ms.ring[j] * dv.len_s + idx];
                         part[2 * kDiisDepth + j] += fnv * dv.d[(int64_t)ms.ring[j] * dv.len_s + idx];
                     }
@@ -594,15 +674,18 @@ __device__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolShared& ms, Wg
             Fuse = F;
         }
         __syncthreads();
-        int sweeps = mol_eigen<double>(nao, Fuse, Ct, T0, T1, lds, g.lds_n, have_c, eps, s);
+        phase_mark(pc, 2);
+        int sweeps = mol_eigen<double>(nao, Fuse, Ct, T0, T1, lds, g.lds_n, have_c, eps, s, 1.0e-7 * err);
+        phase_mark(pc, 3);
+        total_sweeps += sweeps;
         have_c = true;
         if (sweeps < 0) {
             status = kLinearAlgebra;
             break;
         }
         // new density into T0, rms change, then P <- T0
-        if (low) mol_density<float>(nao, nocc, 2.0, Ct, T0);
-        else mol_density<double>(nao, nocc, 2.0, Ct, T0);
+        if (low) mol_density<float>(nao, nocc, 2.0, Ct, T0, lds);
+        else mol_density<double>(nao, nocc, 2.0, Ct, T0, lds);
         __syncthreads();
         double dd = 0.0;
         for (size_t idx = t; idx < nn; idx += kWg) {
@@ -611,6 +694,7 @@ __device__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolShared& ms, Wg
             P[idx] = T0[idx];
         }
         double dp = sqrt(wg_sum(dd, s) / (double)max((size_t)1, nn));
+        phase_mark(pc, 4);
         double de = fabs(e_elec - e_old);
         e_old = e_elec;
         residual = err;
@@ -663,12 +747,14 @@ __device__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolShared& ms, Wg
         for (int k = 0; k < 3; ++k) res.dipole[k] = dip[k] * kAuDipoleToDebye;
         res.residual = residual;
         res.fp32_iterations = fp32_iters;
+        res.eigen_sweeps = total_sweeps;
     }
+    phase_mark(pc, 5);
 }
 
 template <int PREC>
 __global__ __launch_bounds__(kWg) void k_scf_batch(BatchArgs g) {
-    extern __shared__ double lds[];
+    double* lds = lds_dyn;
     __shared__ MolShared ms;
     __shared__ WgScratch s;
     __shared__ int next;
@@ -681,12 +767,17 @@ __global__ __launch_bounds__(kWg) void k_scf_batch(BatchArgs g) {
         const int mi = g.order ? g.order[slot] : slot;
         const MolDesc md = g.mols[mi];
         MolResult res;
+        PhaseClock pc;
         res.t_start = __builtin_amdgcn_s_memrealtime();
+        pc.last = res.t_start;
+        for (int k = 0; k < kPhases; ++k) pc.acc[k] = 0;
         double* M = g.mats + md.mat_off;
-        double ecore = mol_setup(md, g, ms, s, M + kH * (size_t)md.nao * md.nao, g.wpool + md.w_off);
+        double ecore = mol_setup(md, mi, g, ms, s, M + kH * (size_t)md.nao * md.nao, g.wpool + md.w_off);
         __syncthreads();
-        scf_rhf<PREC>(md, g, ms, s, lds, ecore, res);
+        phase_mark(pc, 0);
+        scf_rhf<PREC>(md, g, ms, s, lds, ecore, res, pc);
         if (threadIdx.x == 0) {
+            for (int k = 0; k < kPhases; ++k) res.phase[k] = pc.acc[k];
             res.t_done = __builtin_amdgcn_s_memrealtime();
             g.results[mi] = res;
         }
@@ -700,15 +791,23 @@ template __global__ void k_scf_batch<1>(BatchArgs);
 }  // namespace am1g
 
 namespace am1g {
+hipError_t launch_pairs(const BatchArgs& a, hipStream_t stream) {
+    if (a.n_pairs == 0) return hipSuccess;
+    int64_t blocks = (a.n_pairs + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_pairs, dim3((unsigned)blocks), dim3(256), 0, stream, a);
+    return hipGetLastError();
+}
+
 hipError_t launch_scf_batch(const BatchArgs& a, int precision, int grid, size_t lds_bytes, hipStream_t stream) {
     static bool attr_done = false;
     if (!attr_done) {
         // allow the dynamic LDS region to take what the static part leaves of the 160 KiB
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_scf_batch<0>),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, 140 * 1024);
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes);
         if (e != hipSuccess) return e;
         e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_scf_batch<1>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, 140 * 1024);
+                                hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes);
         if (e != hipSuccess) return e;
         attr_done = true;
     }

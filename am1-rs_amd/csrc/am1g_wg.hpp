@@ -9,10 +9,11 @@
 
 namespace am1g {
 
-constexpr int kWg = 256;  // threads per workgroup for the per-molecule SCF
+constexpr int kWg = 512;  // threads per workgroup for the per-molecule SCF (8 wavefronts)
+constexpr int kWaves = kWg / 64;
 
 struct WgScratch {
-    double red[kWg];      // reduction scratch
+    double red[kWaves * 4];  // reduction scratch
     double bcast[16];     // broadcast slots
     double jc[128], js[128];  // Jacobi rotations of one round
     int jp[128], jq[128];
@@ -26,7 +27,9 @@ __device__ __forceinline__ double wg_sum(double v, WgScratch& s) {
     __syncthreads();
     if ((t & 63) == 0) s.red[t >> 6] = v;
     __syncthreads();
-    double tot = (s.red[0] + s.red[1]) + (s.red[2] + s.red[3]);
+    double tot = 0.0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) tot += s.red[w];
     __syncthreads();
     return tot;
 }
@@ -37,22 +40,29 @@ __device__ __forceinline__ double wg_max(double v, WgScratch& s) {
     __syncthreads();
     if ((t & 63) == 0) s.red[t >> 6] = v;
     __syncthreads();
-    double tot = fmax(fmax(s.red[0], s.red[1]), fmax(s.red[2], s.red[3]));
+    double tot = s.red[0];
+#pragma unroll
+    for (int w = 1; w < kWaves; ++w) tot = fmax(tot, s.red[w]);
     __syncthreads();
     return tot;
 }
 
-// Several independent sums at once, reduced through `buf` (4 * nv doubles of shared scratch).
-// Every thread receives all nv totals in out[0..nv).
+// Several independent sums at once, reduced through `buf` (kWaves * nv doubles of shared
+// scratch). Every thread receives all nv totals in out[0..nv).
 __device__ void wg_sum_many(const double* v, int nv, double* buf, double* out) {
     const int t = threadIdx.x;
     for (int i = 0; i < nv; ++i) {
         double x = v[i];
         for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
-        if ((t & 63) == 0) buf[i * 4 + (t >> 6)] = x;
+        if ((t & 63) == 0) buf[i * kWaves + (t >> 6)] = x;
     }
     __syncthreads();
-    for (int i = 0; i < nv; ++i) out[i] = (buf[i * 4 + 0] + buf[i * 4 + 1]) + (buf[i * 4 + 2] + buf[i * 4 + 3]);
+    for (int i = 0; i < nv; ++i) {
+        double tot = 0.0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) tot += buf[i * kWaves + w];
+        out[i] = tot;
+    }
     __syncthreads();
 }
 
@@ -60,7 +70,7 @@ __device__ void wg_sum_many(const double* v, int nv, double* buf, double* out) {
 // opB(k,j) = TB ? B[j*ldb+k] : B[k*ldb+j]. Each thread owns 4x4 output tiles; operands are
 // read through the cache hierarchy (they are the molecule's own, L2-resident matrices).
 template <typename T, bool TA, bool TB>
-__device__ void wg_gemm(int M, int N, int K, double alpha, const double* __restrict__ A, int lda,
+__device__ __noinline__ void wg_gemm(int M, int N, int K, double alpha, const double* __restrict__ A, int lda,
                         const double* __restrict__ B, int ldb, double* __restrict__ C, int ldc) {
     const int tm = (M + 3) >> 2, tn = (N + 3) >> 2;
     for (int tile = threadIdx.x; tile < tm * tn; tile += kWg) {
@@ -95,6 +105,100 @@ __device__ void wg_gemm(int M, int N, int K, double alpha, const double* __restr
     }
 }
 
+// C = alpha * opA (M x K) * opB (K x N) with k-panels staged through LDS scratch `sm`
+// (`sm_doubles` available). Tiles of 4x4 outputs, at most two per thread, so M and N are at
+// most 128; callers use wg_gemm beyond that. Panels are loaded with the contiguous direction of
+// each operand across consecutive threads, so global reads are coalesced.
+template <typename T, bool TA, bool TB>
+__device__ __noinline__ void wg_gemm_staged(int M, int N, int K, double alpha, const double* __restrict__ A, int lda,
+                               const double* __restrict__ B, int ldb, double* __restrict__ C, int ldc, double* sm,
+                               int sm_doubles) {
+    const int t = threadIdx.x;
+    const int tm = (M + 3) >> 2, tn = (N + 3) >> 2, ntile = tm * tn;
+    const int Mp = tm * 4, Np = tn * 4;
+    int kp = sm_doubles / (Mp + Np);
+    kp = kp > 32 ? 32 : kp;
+    double* As = sm;
+    double* Bs = sm + kp * Mp;
+    T acc[2][4][4];
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[q][a][b] = (T)0;
+    for (int k0 = 0; k0 < K; k0 += kp) {
+        const int kc = min(kp, K - k0);
+        for (int idx = t; idx < kc * Mp; idx += kWg) {
+            int kk, i;
+            if (TA) {
+                kk = idx / Mp;
+                i = idx - kk * Mp;
+            } else {
+                i = idx / kc;
+                kk = idx - i * kc;
+            }
+            int ic = min(i, M - 1);
+            As[kk * Mp + i] = TA ? A[(size_t)(k0 + kk) * lda + ic] : A[(size_t)ic * lda + k0 + kk];
+        }
+        for (int idx = t; idx < kc * Np; idx += kWg) {
+            int kk, j;
+            if (TB) {
+                j = idx / kc;
+                kk = idx - j * kc;
+            } else {
+                kk = idx / Np;
+                j = idx - kk * Np;
+            }
+            int jc = min(j, N - 1);
+            Bs[kk * Np + j] = TB ? B[(size_t)jc * ldb + k0 + kk] : B[(size_t)(k0 + kk) * ldb + jc];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int tile = t + q * kWg;
+            if (tile < ntile) {
+                const int i0 = (tile / tn) << 2, j0 = (tile % tn) << 2;
+                for (int kk = 0; kk < kc; ++kk) {
+                    T av[4], bv[4];
+#pragma unroll
+                    for (int a = 0; a < 4; ++a) av[a] = (T)As[kk * Mp + i0 + a];
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) bv[b] = (T)Bs[kk * Np + j0 + b];
+#pragma unroll
+                    for (int a = 0; a < 4; ++a)
+#pragma unroll
+                        for (int b = 0; b < 4; ++b) acc[q][a][b] = fma(av[a], bv[b], acc[q][a][b]);
+                }
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int tile = t + q * kWg;
+        if (tile < ntile) {
+            const int i0 = (tile / tn) << 2, j0 = (tile % tn) << 2;
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b)
+                    if (i0 + a < M && j0 + b < N) C[(size_t)(i0 + a) * ldc + j0 + b] = alpha * (double)acc[q][a][b];
+        }
+    }
+}
+
+// Dispatch: LDS-staged when the output fits two tiles per thread, register/global otherwise.
+template <typename T, bool TA, bool TB>
+__device__ __forceinline__ void wg_gemm_any(int M, int N, int K, double alpha, const double* A, int lda,
+                                            const double* B, int ldb, double* C, int ldc, double* sm,
+                                            int sm_doubles) {
+    if (M <= 128 && N <= 128 && ((M + 3) / 4) * ((N + 3) / 4) <= 2 * kWg)
+        wg_gemm_staged<T, TA, TB>(M, N, K, alpha, A, lda, B, ldb, C, ldc, sm, sm_doubles);
+    else
+        wg_gemm<T, TA, TB>(M, N, K, alpha, A, lda, B, ldb, C, ldc);
+}
+
 // Round-robin partner schedule for m (even) players: round r, slot k -> (p, q), p < q.
 __device__ __forceinline__ void rr_pair(int m, int r, int k, int& p, int& q) {
     const int L = m - 1;
@@ -110,31 +214,195 @@ __device__ __forceinline__ void rr_pair(int m, int r, int k, int& p, int& q) {
     q = max(a, b);
 }
 
-// Parallel cyclic Jacobi on the symmetric n x n matrix A (row stride lda; LDS or global),
-// rotating the rows of Vt (Vt = V^T, row k = eigenvector k) along with it. On return the
-// eigenvalues are on A's diagonal (unsorted). Each round applies m/2 disjoint rotations: one
-// thread per 2x2 block of the (row pair, column pair) grid updates it in place, which is
-// race-free because a block's new values depend only on the block itself.
-// Returns the number of sweeps, or -1 when it did not converge in max_sweeps.
-__device__ int wg_jacobi(int n, double* A, int lda, double* Vt, int ldv, WgScratch& s, int max_sweeps) {
+// Parallel cyclic Jacobi on a symmetric n x n matrix, rotating the rows of Vt (Vt = V^T, row k =
+// eigenvector k) along with it. A is either packed (PACKED: lower triangle, a(i, j) at
+// i(i+1)/2 + j for i >= j, any address space) or full row-major with stride lda. On return the
+// eigenvalues are on A's diagonal (unsorted).
+//
+// Each round applies the m/2 disjoint rotations of a round-robin schedule (m = n rounded up to
+// even). The matrix is cut into 2x2 blocks (row pair a, column pair b); a block's new values
+// depend only on its old values and the two rotations, so each block is updated in place by
+// one thread without races, and with packed storage the four entries of an off-diagonal block
+// are four distinct stored numbers (nothing to mirror). A thread's blocks and eigenvector
+// elements are the same slots every round, so they are decoded once per solve.
+//
+// Converged when the largest off-diagonal element is below tau = max(1e-13 x (rms of the
+// diagonal), tol_abs); the same tau decides which rotations are applied, so a converged sweep is
+// exactly one in which nothing would rotate (a tighter target sits at the rotations' own
+// rounding floor). The SCF passes tol_abs = 1e-7 x its current commutator error: an intermediate
+// Fock matrix needs its eigenvectors only to well inside the error the next iteration removes.
+// Returns the number of sweeps, or -1 when max_sweeps did not suffice.
+constexpr int kBlkPer = 5;   // blocks per thread: ceil(64*65/2 / kWg) for n <= 128
+constexpr int kVecPer = 16;  // eigenvector element pairs per thread: 64*128 / kWg
+
+template <bool PACKED>
+__device__ __forceinline__ size_t jidx(int i, int j, int lda) {
+    if (PACKED) return i >= j ? (size_t)i * (i + 1) / 2 + j : (size_t)j * (j + 1) / 2 + i;
+    return (size_t)i * lda + j;
+}
+
+// LDS-resident Jacobi: A packed at lds_dyn[0, n(n+1)/2); eigenvectors (rows of Vt) at
+// lds_dyn[v_off, v_off + n*n) when V_LDS, else in global memory at Vg. Addresses are LDS
+// offsets, so every access is a ds_read/ds_write.
+extern __shared__ double lds_dyn[];
+
+template <bool V_LDS>
+__device__ __noinline__ int wg_jacobi_lds(int n, int v_off, double* __restrict__ Vg, WgScratch& s, int max_sweeps,
+                                          double tol_abs) {
+    const int t = threadIdx.x;
+    const int m = (n + 1) & ~1;
+    const int h = m >> 1;
+    const int nblk = h * (h + 1) / 2;
+    const int nvec = h * n;
+    double* A = lds_dyn;
+    int bslot[kBlkPer], vslot[kVecPer];
+#pragma unroll
+    for (int u = 0; u < kBlkPer; ++u) {
+        int blk = t + u * kWg;
+        bslot[u] = -1;
+        if (blk < nblk) {
+            int a = (int)((sqrt(8.0 * blk + 1.0) - 1.0) * 0.5);
+            while (a * (a + 1) / 2 > blk) --a;
+            while ((a + 1) * (a + 2) / 2 <= blk) ++a;
+            bslot[u] = (a << 8) | (blk - a * (a + 1) / 2);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < kVecPer; ++u) {
+        int idx = t + u * kWg;
+        vslot[u] = idx < nvec ? ((idx / n) << 16) | (idx % n) : -1;
+    }
+    for (int sweep = 0; sweep <= max_sweeps; ++sweep) {
+        double offmax = 0.0, dg = 0.0;
+        for (int i = t / 32; i < n; i += kWg / 32) {
+            const int rb = i * (i + 1) / 2;
+            for (int j = t % 32; j <= i; j += 32) {
+                double v = A[rb + j];
+                if (i == j) dg += v * v;
+                else offmax = fmax(offmax, fabs(v));
+            }
+        }
+        offmax = wg_max(offmax, s);
+        dg = wg_sum(dg, s);
+        const double tau = fmax(1.0e-13 * sqrt(dg / (double)max(n, 1)), tol_abs);
+        if (offmax <= tau) return sweep;
+        if (sweep == max_sweeps) break;
+        for (int r = 0; r < m - 1; ++r) {
+            if (t < h) {
+                int p, q;
+                rr_pair(m, r, t, p, q);
+                double c = 1.0, sn = 0.0;
+                if (q < n) {
+                    double apq = A[q * (q + 1) / 2 + p];
+                    if (fabs(apq) > tau) {
+                        double app = A[p * (p + 1) / 2 + p], aqq = A[q * (q + 1) / 2 + q];
+                        double theta = (aqq - app) / (2.0 * apq);
+                        double tt = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                        c = 1.0 / sqrt(tt * tt + 1.0);
+                        sn = tt * c;
+                    }
+                }
+                s.jc[t] = c;
+                s.js[t] = sn;
+                s.jp[t] = p;
+                s.jq[t] = q < n ? q : -1;  // -1: the padding slot of an odd-order matrix
+            }
+            __syncthreads();
+#pragma unroll
+            for (int u = 0; u < kBlkPer; ++u) {
+                if (bslot[u] < 0) continue;
+                const int a = bslot[u] >> 8, b = bslot[u] & 255;
+                const double ca = s.jc[a], sa = s.js[a], cb = s.jc[b], sb = s.js[b];
+                // a slot with s == 0 still owns rows the other side's rotation moves
+                if (sa == 0.0 && (a == b || sb == 0.0)) continue;
+                const int pa = s.jp[a], qa = s.jq[a], pb = s.jp[b], qb = s.jq[b];
+                if (a == b) {
+                    const int i00 = pa * (pa + 1) / 2 + pa, i11 = qa * (qa + 1) / 2 + qa, i01 = qa * (qa + 1) / 2 + pa;
+                    const double x00 = A[i00], x01 = A[i01], x11 = A[i11];
+                    A[i00] = ca * (ca * x00 - sa * x01) - sa * (ca * x01 - sa * x11);
+                    A[i11] = sa * (sa * x00 + ca * x01) + ca * (sa * x01 + ca * x11);
+                    A[i01] = ca * (sa * x00 + ca * x01) - sa * (sa * x01 + ca * x11);
+                } else {
+                    // rows {pa, qa} x columns {pb, qb}; a missing (padding) index reads as zero
+                    const int i00 = (int)jidx<true>(pa, pb, 0);
+                    const int i01 = qb >= 0 ? (int)jidx<true>(pa, qb, 0) : -1;
+                    const int i10 = qa >= 0 ? (int)jidx<true>(qa, pb, 0) : -1;
+                    const int i11 = (qa >= 0 && qb >= 0) ? (int)jidx<true>(qa, qb, 0) : -1;
+                    const double x00 = A[i00], x01 = i01 >= 0 ? A[i01] : 0.0, x10 = i10 >= 0 ? A[i10] : 0.0,
+                                 x11 = i11 >= 0 ? A[i11] : 0.0;
+                    // rows: J_a^T = [[ca, -sa], [sa, ca]]; columns: J_b = [[cb, sb], [-sb, cb]]
+                    const double r00 = ca * x00 - sa * x10, r01 = ca * x01 - sa * x11;
+                    const double r10 = sa * x00 + ca * x10, r11 = sa * x01 + ca * x11;
+                    A[i00] = r00 * cb - r01 * sb;
+                    if (i01 >= 0) A[i01] = r00 * sb + r01 * cb;
+                    if (i10 >= 0) A[i10] = r10 * cb - r11 * sb;
+                    if (i11 >= 0) A[i11] = r10 * sb + r11 * cb;
+                }
+            }
+            // Vt rows p, q: (v_p, v_q) <- (c v_p - s v_q, s v_p + c v_q), four slots in flight
+#pragma unroll
+            for (int c0 = 0; c0 < kVecPer; c0 += 4) {
+                double vp[4], vq[4];
+                int ip[4], iq[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int sl = vslot[c0 + u];
+                    ip[u] = -1;
+                    if (sl < 0) continue;
+                    const int a = sl >> 16, i = sl & 0xffff;
+                    if (s.js[a] == 0.0) continue;
+                    ip[u] = s.jp[a] * n + i;
+                    iq[u] = s.jq[a] * n + i;
+                    if (V_LDS) {
+                        vp[u] = lds_dyn[v_off + ip[u]];
+                        vq[u] = lds_dyn[v_off + iq[u]];
+                    } else {
+                        vp[u] = Vg[ip[u]];
+                        vq[u] = Vg[iq[u]];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (ip[u] < 0) continue;
+                    const int a = vslot[c0 + u] >> 16;
+                    const double ca = s.jc[a], sa = s.js[a];
+                    const double np_ = ca * vp[u] - sa * vq[u], nq_ = sa * vp[u] + ca * vq[u];
+                    if (V_LDS) {
+                        lds_dyn[v_off + ip[u]] = np_;
+                        lds_dyn[v_off + iq[u]] = nq_;
+                    } else {
+                        Vg[ip[u]] = np_;
+                        Vg[iq[u]] = nq_;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    return -1;
+}
+
+// Jacobi for orders above 128 (global memory, full storage): same algorithm, slots decoded on
+// the fly because a thread owns more of them than the register budget of the fast path holds.
+__device__ __noinline__ int wg_jacobi_big(int n, double* A, int lda, double* Vt, int ldv, WgScratch& s, int max_sweeps,
+                                          double tol_abs) {
     const int t = threadIdx.x;
     const int m = (n + 1) & ~1;
     const int h = m >> 1;
     const int nblk = h * (h + 1) / 2;
     for (int sweep = 0; sweep <= max_sweeps; ++sweep) {
-        // convergence: off-diagonal Frobenius norm against the diagonal's
-        double off = 0.0, dg = 0.0;
-        for (int idx = t; idx < n * n; idx += kWg) {
-            int i = idx / n, j = idx - i * n;
-            double v = A[(size_t)i * lda + j];
-            if (i == j) dg += v * v;
-            else off += v * v;
-        }
-        off = wg_sum(off, s);
+        double offmax = 0.0, dg = 0.0;
+        for (int i = t / 32; i < n; i += kWg / 32)
+            for (int j = t % 32; j <= i; j += 32) {
+                double v = A[(size_t)i * lda + j];
+                if (i == j) dg += v * v;
+                else offmax = fmax(offmax, fabs(v));
+            }
+        offmax = wg_max(offmax, s);
         dg = wg_sum(dg, s);
-        if (off <= 1.0e-28 * dg || off == 0.0) return sweep;
+        const double tau = fmax(1.0e-13 * sqrt(dg / (double)max(n, 1)), tol_abs);
+        if (offmax <= tau) return sweep;
         if (sweep == max_sweeps) break;
-        const double skip = 1.0e-15 * sqrt(dg / (double)max(n, 1));
         for (int r = 0; r < m - 1; ++r) {
             if (t < h) {
                 int p, q;
@@ -142,7 +410,7 @@ __device__ int wg_jacobi(int n, double* A, int lda, double* Vt, int ldv, WgScrat
                 double c = 1.0, sn = 0.0;
                 if (q < n) {
                     double apq = A[(size_t)p * lda + q];
-                    if (fabs(apq) > skip) {
+                    if (fabs(apq) > tau) {
                         double app = A[(size_t)p * lda + p], aqq = A[(size_t)q * lda + q];
                         double theta = (aqq - app) / (2.0 * apq);
                         double tt = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
@@ -153,23 +421,20 @@ __device__ int wg_jacobi(int n, double* A, int lda, double* Vt, int ldv, WgScrat
                 s.jc[t] = c;
                 s.js[t] = sn;
                 s.jp[t] = p;
-                s.jq[t] = q;
+                s.jq[t] = q < n ? q : -1;
             }
             __syncthreads();
-            // A <- J^T A J over (a >= b) blocks, mirrored
             for (int blk = t; blk < nblk; blk += kWg) {
                 int a = (int)((sqrt(8.0 * blk + 1.0) - 1.0) * 0.5);
                 while (a * (a + 1) / 2 > blk) --a;
                 while ((a + 1) * (a + 2) / 2 <= blk) ++a;
-                int b = blk - a * (a + 1) / 2;
-                int pa = s.jp[a], qa = s.jq[a], pb = s.jp[b], qb = s.jq[b];
-                double ca = s.jc[a], sa = s.js[a], cb = s.jc[b], sb = s.js[b];
-                bool va = qa < n, vb = qb < n;  // padding player never touches memory
+                const int b = blk - a * (a + 1) / 2;
+                if (!((s.js[a] != 0.0) || (a != b && s.js[b] != 0.0))) continue;
+                const int pa = s.jp[a], qa = s.jq[a], pb = s.jp[b], qb = s.jq[b];
+                const double ca = s.jc[a], sa = s.js[a], cb = s.jc[b], sb = s.js[b];
                 if (a == b) {
-                    if (!va || sa == 0.0) continue;
                     double x00 = A[(size_t)pa * lda + pa], x01 = A[(size_t)pa * lda + qa];
                     double x11 = A[(size_t)qa * lda + qa];
-                    // J^T X J for the symmetric 2x2 block
                     double y00 = ca * (ca * x00 - sa * x01) - sa * (ca * x01 - sa * x11);
                     double y11 = sa * (sa * x00 + ca * x01) + ca * (sa * x01 + ca * x11);
                     double y01 = ca * (sa * x00 + ca * x01) - sa * (sa * x01 + ca * x11);
@@ -178,44 +443,37 @@ __device__ int wg_jacobi(int n, double* A, int lda, double* Vt, int ldv, WgScrat
                     A[(size_t)pa * lda + qa] = y01;
                     A[(size_t)qa * lda + pa] = y01;
                 } else {
-                    // rows {pa, qa} (or the single row pa when a is the padding slot) x cols {pb, qb}
-                    if ((!va || sa == 0.0) && (!vb || sb == 0.0)) continue;
                     double x00 = A[(size_t)pa * lda + pb];
-                    double x01 = vb ? A[(size_t)pa * lda + qb] : 0.0;
-                    double x10 = va ? A[(size_t)qa * lda + pb] : 0.0;
-                    double x11 = (va && vb) ? A[(size_t)qa * lda + qb] : 0.0;
-                    if (!va) { ca = 1.0; sa = 0.0; }
-                    if (!vb) { cb = 1.0; sb = 0.0; }
-                    // rows: J_a^T = [[ca, -sa], [sa, ca]]
+                    double x01 = qb >= 0 ? A[(size_t)pa * lda + qb] : 0.0;
+                    double x10 = qa >= 0 ? A[(size_t)qa * lda + pb] : 0.0;
+                    double x11 = (qa >= 0 && qb >= 0) ? A[(size_t)qa * lda + qb] : 0.0;
                     double r00 = ca * x00 - sa * x10, r01 = ca * x01 - sa * x11;
                     double r10 = sa * x00 + ca * x10, r11 = sa * x01 + ca * x11;
-                    // cols: J_b = [[cb, sb], [-sb, cb]]
                     double y00 = r00 * cb - r01 * sb, y01 = r00 * sb + r01 * cb;
                     double y10 = r10 * cb - r11 * sb, y11 = r10 * sb + r11 * cb;
                     A[(size_t)pa * lda + pb] = y00;
                     A[(size_t)pb * lda + pa] = y00;
-                    if (vb) {
+                    if (qb >= 0) {
                         A[(size_t)pa * lda + qb] = y01;
                         A[(size_t)qb * lda + pa] = y01;
                     }
-                    if (va) {
+                    if (qa >= 0) {
                         A[(size_t)qa * lda + pb] = y10;
                         A[(size_t)pb * lda + qa] = y10;
                     }
-                    if (va && vb) {
+                    if (qa >= 0 && qb >= 0) {
                         A[(size_t)qa * lda + qb] = y11;
                         A[(size_t)qb * lda + qa] = y11;
                     }
                 }
             }
-            // Vt rows p, q: (v_p, v_q) <- (c v_p - s v_q, s v_p + c v_q)
             for (int idx = t; idx < h * n; idx += kWg) {
-                int a = idx / n, i = idx - a * n;
-                double sa = s.js[a];
-                if (s.jq[a] >= n || sa == 0.0) continue;
-                double ca = s.jc[a];
-                int p = s.jp[a], q = s.jq[a];
-                double vp = Vt[(size_t)p * ldv + i], vq = Vt[(size_t)q * ldv + i];
+                const int a = idx / n, i = idx - a * n;
+                const double sa = s.js[a];
+                if (sa == 0.0) continue;
+                const double ca = s.jc[a];
+                const int p = s.jp[a], q = s.jq[a];
+                const double vp = Vt[(size_t)p * ldv + i], vq = Vt[(size_t)q * ldv + i];
                 Vt[(size_t)p * ldv + i] = ca * vp - sa * vq;
                 Vt[(size_t)q * ldv + i] = sa * vp + ca * vq;
             }

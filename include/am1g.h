@@ -75,6 +75,10 @@ typedef struct {
     int fp32_iterations;      /* iterations run with FP32 GEMMs (mixed precision) */
     double latency_ms;        /* batched: device time from the launch's first molecule start to this
                                  molecule's completion; plain: wall time of the call */
+    double phase_ms[6];       /* device time per phase: setup (integrals, H_core), Fock + energy,
+                                 commutator + DIIS, eigensolve, density + convergence, final
+                                 energy and properties (am1-rs AM1_TIMING phases, timing.rs:34-85) */
+    int eigen_sweeps;         /* Jacobi sweeps summed over all iterations (diagnostic) */
     double* charges;          /* [n_atoms] Mulliken charges */
     double* mo_energies;      /* [nao] (alpha for UHF) */
     double* mo_energies_beta; /* [nao] UHF only */
