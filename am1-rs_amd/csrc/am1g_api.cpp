@@ -180,6 +180,7 @@ int prepare(am1g_ctx* ctx, const am1g_request& rq, Prepared& out, std::string& m
     md.method = rq.method;
     md.uhf = uhf ? 1 : 0;
     md.accel = rq.use_diis ? rq.accelerator : 0;
+    md.use_diis = rq.use_diis ? 1 : 0;
     md.max_scf = rq.max_scf;
     md.precision = rq.precision;
     md.n_heavy = nh;
@@ -307,11 +308,6 @@ int am1g_scf_batch(am1g_ctx* ctx, int n, const am1g_request* rqs, am1g_result* r
             set_error(ctx, st, "%s", msg.c_str());
             continue;
         }
-        if (prep[i].uhf) {
-            r.status = AM1G_INVALID_INPUT;
-            set_error(ctx, AM1G_INVALID_INPUT, "unrestricted SCF is not available on this path yet");
-            continue;
-        }
         ok_idx.push_back(i);
     }
     const int m = (int)ok_idx.size();
@@ -429,10 +425,11 @@ int am1g_scf_batch(am1g_ctx* ctx, int n, const am1g_request* rqs, am1g_result* r
     HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
 
     std::vector<MolResult> res(m);
-    std::vector<double> mo(ao_off), q(atom_off);
+    std::vector<double> mo(ao_off), mob(ao_off), q(atom_off);
     HIP_TRY(ctx, hipMemcpyAsync(res.data(), ctx->results.p, sizeof(MolResult) * m, hipMemcpyDeviceToHost, st));
     HIP_TRY(ctx, hipMemcpyAsync(mo.data(), ctx->mo_e.p, sizeof(double) * ao_off, hipMemcpyDeviceToHost, st));
     HIP_TRY(ctx, hipMemcpyAsync(q.data(), ctx->charges.p, sizeof(double) * atom_off, hipMemcpyDeviceToHost, st));
+    HIP_TRY(ctx, hipMemcpyAsync(mob.data(), ctx->mo_eb.p, sizeof(double) * ao_off, hipMemcpyDeviceToHost, st));
     HIP_TRY(ctx, hipStreamSynchronize(st));
     float ms = 0.f;
     HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
@@ -450,9 +447,12 @@ int am1g_scf_batch(am1g_ctx* ctx, int n, const am1g_request* rqs, am1g_result* r
         r.eigen_sweeps = res[j].eigen_sweeps;
         if (r.charges) std::memcpy(r.charges, q.data() + md.atom_off, sizeof(double) * md.nat);
         if (r.mo_energies) std::memcpy(r.mo_energies, mo.data() + md.ao_off, sizeof(double) * md.nao);
+        if (r.mo_energies_beta && md.uhf)
+            std::memcpy(r.mo_energies_beta, mob.data() + md.ao_off, sizeof(double) * md.nao);
         const size_t nn = (size_t)md.nao * md.nao;
         if (r.density) {
-            HIP_TRY(ctx, hipMemcpy(r.density, ctx->mats.as<double>() + md.mat_off + kP * nn, nn * sizeof(double),
+            const int slot = md.uhf ? kPt : kP;
+            HIP_TRY(ctx, hipMemcpy(r.density, ctx->mats.as<double>() + md.mat_off + slot * nn, nn * sizeof(double),
                                    hipMemcpyDeviceToHost));
         }
         if (r.mo_coeff) {

@@ -47,7 +47,7 @@ struct MolDesc {
     int nat, nao, n_alpha, n_beta;
     int method, uhf, accel, max_scf;
     int precision;   // 0 fp64, 1 mixed
-    int n_heavy, n_h, pad0;
+    int n_heavy, n_h, use_diis;
     double e_tol, p_tol, adiis_switch;
     int64_t atom_off;   // into z[] / xyz[] (xyz: 3 per atom)
     int64_t ao_off;     // into per-AO arrays (mo energies)
@@ -85,8 +85,9 @@ enum MatSlot : int {
     kFb,      // UHF: beta Fock
     kPb,      // UHF: beta density
     kCb,      // UHF: beta eigenvectors (transposed)
+    kPt,      // UHF: total density
     kMatSlotsRhf = kT1 + 1,
-    kMatSlotsUhf = kCb + 1,
+    kMatSlotsUhf = kPt + 1,
 };
 
 AM1G_HD inline int64_t w_pool_size(int n_heavy, int n_h) {

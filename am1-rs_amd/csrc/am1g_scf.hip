@@ -467,7 +467,7 @@ __device__ __forceinline__ void phase_mark(PhaseClock& pc, int k) {
 
 // ---------------------------------------------------------------------------------------------
 template <int PREC>
-__device__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolShared& ms, WgScratch& s, double* lds,
+__device__ __noinline__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolShared& ms, WgScratch& s, double* lds,
                         double ecore, MolResult& res, PhaseClock& pc) {
     const int t = threadIdx.x;
     const int nao = md.nao, nat = md.nat, nocc = md.n_alpha;
@@ -752,6 +752,251 @@ __device__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolShared& ms, Wg
     phase_mark(pc, 5);
 }
 
+
+// Unrestricted SCF (am1-rs uhf_loop, scf.rs:719-858): F_sigma = H + J(Pa + Pb) - K(P_sigma),
+// E = 1/2 (Pt.H + Pa.Fa + Pb.Fb); CDIIS on the stacked error [Ea ; Eb] with one coefficient set
+// for both spins (no A-DIIS); the divergence check comes after the extrapolation; rms changes of
+// the two spin densities are summed.
+template <int PREC>
+__device__ __noinline__ void scf_uhf(const MolDesc& md, const BatchArgs& g, MolShared& ms, WgScratch& s,
+                                     double* lds, double ecore, MolResult& res, PhaseClock& pc) {
+    const int t = threadIdx.x;
+    const int nao = md.nao, nat = md.nat, na = md.n_alpha, nb = md.n_beta;
+    const DevElem* el = g.elems + md.method * kMaxZ;
+    double* M = g.mats + md.mat_off;
+    const size_t nn = (size_t)nao * nao;
+    double* H = M + kH * nn;
+    double* T0 = M + kT0 * nn;
+    double* T1 = M + kT1 * nn;
+    double* Pt = M + kPt * nn;
+    double* Fs[2] = {M + kF * nn, M + kFb * nn};
+    double* Ps[2] = {M + kP * nn, M + kPb * nn};
+    double* Cs[2] = {M + kC * nn, M + kCb * nn};
+    double* eps[2] = {g.mo_e + md.ao_off, g.mo_e_beta + md.ao_off};
+    const int nocc[2] = {na, nb};
+    double* wpool = g.wpool + md.w_off;
+    DiisView dv[2] = {diis_view(g.hist + md.hist_off, nao, 0), diis_view(g.hist + md.hist_off, nao, 1)};
+    const bool diis = md.use_diis != 0;  // UHF runs CDIIS whenever DIIS is on (scf.rs:777-797)
+
+    // SAD guess split by the alpha / beta electron fractions (scf.rs:730-740)
+    const double ntot = (double)max(na + nb, 1);
+    const double frac[2] = {na / ntot, nb / ntot};
+    for (size_t idx = t; idx < nn; idx += kWg) Ps[0][idx] = Ps[1][idx] = 0.0;
+    __syncthreads();
+    for (int a = t; a < nat; a += kWg) {
+        const DevElem& e = el[ms.z[a]];
+        double nsv = fmin(e.core, 2.0), npv = fmax(e.core - nsv, 0.0);
+        int o = ms.off[a];
+        for (int c = 0; c < 2; ++c) {
+            Ps[c][(size_t)o * nao + o] = nsv * frac[c];
+            if (e.norb == 4)
+                for (int k = 1; k < 4; ++k) Ps[c][(size_t)(o + k) * nao + o + k] = npv / 3.0 * frac[c];
+        }
+    }
+    if (t == 0) {
+        ms.hcount = 0;
+        ms.head = 0;
+    }
+    __syncthreads();
+    double e_old = 0.0, residual = INFINITY, err = INFINITY;
+    int iterations = 0, converged = 0, status = kOk, fp32_iters = 0, total_sweeps = 0;
+    bool have_c = false;
+    for (int iter = 0; iter < md.max_scf; ++iter) {
+        iterations = iter + 1;
+        for (size_t idx = t; idx < nn; idx += kWg) Pt[idx] = Ps[0][idx] + Ps[1][idx];
+        __syncthreads();
+        mol_fock(md, el, ms, wpool, H, Pt, Ps[0], 1.0, Fs[0]);
+        mol_fock(md, el, ms, wpool, H, Pt, Ps[1], 1.0, Fs[1]);
+        __syncthreads();
+        double pe = 0.0;
+        for (size_t idx = t; idx < nn; idx += kWg)
+            pe += Pt[idx] * H[idx] + Ps[0][idx] * Fs[0][idx] + Ps[1][idx] * Fs[1][idx];
+        const double e_elec = 0.5 * wg_sum(pe, s);
+        phase_mark(pc, 1);
+        const bool low = (PREC == 1) && (residual > 1.0e-4);
+        if (low) ++fp32_iters;
+        int slot = 0;
+        if (diis) slot = ms.hcount < kDiisDepth ? (ms.head + ms.hcount) % kDiisDepth : ms.head;
+        double en = 0.0;
+        for (int c = 0; c < 2; ++c) {
+            if (low)
+                wg_gemm_any<float, true, false>(nao, nao, nao, 1.0, Fs[c], nao, Ps[c], nao, T0, nao, lds, kLdsDoubles);
+            else
+                wg_gemm_any<double, true, false>(nao, nao, nao, 1.0, Fs[c], nao, Ps[c], nao, T0, nao, lds, kLdsDoubles);
+            __syncthreads();
+            double* hf = dv[c].f + (int64_t)slot * dv[c].len_s;
+            double* he = dv[c].e + (int64_t)slot * dv[c].len_a;
+            for (int64_t idx = t; idx < (int64_t)nn; idx += kWg) {
+                int i = (int)(idx / nao), j = (int)(idx - (int64_t)i * nao);
+                if (j <= i) continue;
+                double e = T0[idx] - T0[(size_t)j * nao + i];
+                en += e * e;
+                if (diis) he[(int64_t)i * nao - (int64_t)i * (i + 1) / 2 + (j - i - 1)] = e;
+            }
+            if (diis)
+                for (int64_t idx = t; idx < (int64_t)nn; idx += kWg) {
+                    int i = (int)(idx / nao), j = (int)(idx - (int64_t)i * nao);
+                    if (j >= i) hf[tri_pos(i, j, nao)] = Fs[c][idx];
+                }
+            __syncthreads();
+        }
+        err = sqrt(2.0 * wg_sum(en, s));
+        const double* Fuse[2] = {Fs[0], Fs[1]};
+        if (diis) {
+            if (t == 0) {
+                if (ms.hcount < kDiisDepth) {
+                    ms.hcount++;
+                } else {
+                    ms.head = (ms.head + 1) % kDiisDepth;
+                    for (int i = 0; i < kDiisDepth - 1; ++i)
+                        for (int j = 0; j < kDiisDepth - 1; ++j) ms.B[i * kDiisDepth + j] = ms.B[(i + 1) * kDiisDepth + j + 1];
+                }
+                for (int i = 0; i < ms.hcount; ++i) ms.ring[i] = (ms.head + i) % kDiisDepth;
+            }
+            __syncthreads();
+            const int m = ms.hcount, newest = m - 1;
+            double part[kDiisDepth];
+#pragma unroll
+            for (int q = 0; q < kDiisDepth; ++q) part[q] = 0.0;
+            for (int c = 0; c < 2; ++c) {
+                const double* en_p = dv[c].e + (int64_t)ms.ring[newest] * dv[c].len_a;
+                for (int64_t idx = t; idx < dv[c].len_a; idx += kWg) {
+                    double x = en_p[idx];
+#pragma unroll
+                    for (int i = 0; i < kDiisDepth; ++i)
+                        if (i < m) part[i] += x * dv[c].e[(int64_t)ms.ring[i] * dv[c].len_a + idx];
+                }
+            }
+            double tot[kDiisDepth];
+            wg_sum_many(part, kDiisDepth, lds, tot);
+            if (t == 0) {
+                for (int i = 0; i < m; ++i) {
+                    ms.B[newest * kDiisDepth + i] = 2.0 * tot[i];
+                    ms.B[i * kDiisDepth + newest] = 2.0 * tot[i];
+                }
+                int use = 0;
+                if (m >= 2) {
+                    double Bm[(kDiisDepth + 1) * (kDiisDepth + 1)], rhs[kDiisDepth + 1], x[kDiisDepth + 1];
+                    const int dim = m + 1;
+                    for (int i = 0; i < m; ++i) {
+                        for (int j = 0; j < m; ++j) Bm[i * dim + j] = ms.B[i * kDiisDepth + j];
+                        Bm[i * dim + m] = -1.0;
+                        Bm[m * dim + i] = -1.0;
+                        rhs[i] = 0.0;
+                    }
+                    Bm[m * dim + m] = 0.0;
+                    rhs[m] = -1.0;
+                    if (solve_bordered(Bm, rhs, dim, x)) {
+                        use = 1;
+                        for (int i = 0; i < m; ++i) ms.coef[i] = x[i];
+                    }
+                }
+                ms.flags[0] = use;
+            }
+            __syncthreads();
+            if (ms.flags[0]) {
+                // combined Focks into T0 (alpha) and T1 (beta)
+                double* outs[2] = {T0, T1};
+                for (int c = 0; c < 2; ++c)
+                    for (int64_t idx = t; idx < (int64_t)nn; idx += kWg) {
+                        int i = (int)(idx / nao), j = (int)(idx - (int64_t)i * nao);
+                        int64_t pk = (i <= j) ? tri_pos(i, j, nao) : tri_pos(j, i, nao);
+                        double acc = 0.0;
+                        for (int q = 0; q < m; ++q) acc += ms.coef[q] * dv[c].f[(int64_t)ms.ring[q] * dv[c].len_s + pk];
+                        outs[c][idx] = acc;
+                    }
+                __syncthreads();
+                // the eigen-step needs T0/T1 as scratch: the extrapolated Focks replace F
+                for (int c = 0; c < 2; ++c)
+                    for (size_t idx = t; idx < nn; idx += kWg) Fs[c][idx] = (c == 0 ? T0 : T1)[idx];
+                __syncthreads();
+            }
+        }
+        if (!isfinite(e_elec) || !isfinite(err)) {
+            status = kScfDiverged;
+            break;
+        }
+        phase_mark(pc, 2);
+        for (int c = 0; c < 2; ++c) {
+            int sw = mol_eigen<double>(nao, Fuse[c], Cs[c], T0, T1, lds, g.lds_n, have_c, eps[c], s, 1.0e-7 * err);
+            if (sw < 0) status = kLinearAlgebra;
+            else total_sweeps += sw;
+        }
+        have_c = true;
+        phase_mark(pc, 3);
+        if (status != kOk) break;
+        double dsum[2];
+        for (int c = 0; c < 2; ++c) {
+            if (low) mol_density<float>(nao, nocc[c], 1.0, Cs[c], T0, lds);
+            else mol_density<double>(nao, nocc[c], 1.0, Cs[c], T0, lds);
+            __syncthreads();
+            double dd = 0.0;
+            for (size_t idx = t; idx < nn; idx += kWg) {
+                double x = T0[idx] - Ps[c][idx];
+                dd += x * x;
+                Ps[c][idx] = T0[idx];
+            }
+            dsum[c] = sqrt(wg_sum(dd, s) / (double)max((size_t)1, nn));
+        }
+        const double dp = dsum[0] + dsum[1];
+        phase_mark(pc, 4);
+        const double de = fabs(e_elec - e_old);
+        e_old = e_elec;
+        residual = err;
+        if (iter > 0 && de < md.e_tol && (dp < md.p_tol || err < 1.0e-7) && !low) {
+            converged = 1;
+            break;
+        }
+        __syncthreads();
+    }
+    __syncthreads();
+    for (size_t idx = t; idx < nn; idx += kWg) Pt[idx] = Ps[0][idx] + Ps[1][idx];
+    __syncthreads();
+    mol_fock(md, el, ms, wpool, H, Pt, Ps[0], 1.0, Fs[0]);
+    mol_fock(md, el, ms, wpool, H, Pt, Ps[1], 1.0, Fs[1]);
+    __syncthreads();
+    double pe = 0.0;
+    for (size_t idx = t; idx < nn; idx += kWg) pe += Pt[idx] * H[idx] + Ps[0][idx] * Fs[0][idx] + Ps[1][idx] * Fs[1][idx];
+    const double electronic = 0.5 * wg_sum(pe, s);
+    double dip[3] = {0.0, 0.0, 0.0}, eiso = 0.0, eheat = 0.0;
+    const double* xyz = g.xyz + md.atom_off * 3;
+    for (int a = t; a < nat; a += kWg) {
+        const DevElem& e = el[ms.z[a]];
+        int o = ms.off[a];
+        double pop = 0.0;
+        for (int k = 0; k < e.norb; ++k) pop += Pt[(size_t)(o + k) * nao + o + k];
+        double q = e.core - pop;
+        g.charges[md.atom_off + a] = q;
+        for (int k = 0; k < 3; ++k) dip[k] += xyz[3 * a + k] * q;
+        if (e.norb == 4)
+            for (int k = 0; k < 3; ++k) dip[k] += -2.0 * e.dd * Pt[(size_t)o * nao + o + 1 + k];
+        eiso += e.e_isol;
+        eheat += e.eheat_ev;
+    }
+    for (int k = 0; k < 3; ++k) dip[k] = wg_sum(dip[k], s);
+    eiso = wg_sum(eiso, s);
+    eheat = wg_sum(eheat, s);
+    if (t == 0) {
+        res.status = (status == kOk && !converged) ? kScfNotConverged : status;
+        res.iterations = iterations;
+        res.converged = converged;
+        res.nao = nao;
+        res.electronic_ev = electronic;
+        res.core_ev = ecore;
+        res.total_ev = electronic + ecore;
+        res.heat_of_formation_kcal = (electronic + ecore - eiso + eheat) * kEvToKcal;
+        res.homo_ev = na >= 1 ? eps[0][na - 1] : NAN;
+        res.lumo_ev = na < nao ? eps[0][na] : NAN;
+        res.homo_beta_ev = nb >= 1 ? eps[1][nb - 1] : NAN;
+        res.lumo_beta_ev = nb < nao ? eps[1][nb] : NAN;
+        for (int k = 0; k < 3; ++k) res.dipole[k] = dip[k] * kAuDipoleToDebye;
+        res.residual = residual;
+        res.fp32_iterations = fp32_iters;
+        res.eigen_sweeps = total_sweeps;
+    }
+    phase_mark(pc, 5);
+}
+
 template <int PREC>
 __global__ __launch_bounds__(kWg) void k_scf_batch(BatchArgs g) {
     double* lds = lds_dyn;
@@ -775,7 +1020,8 @@ __global__ __launch_bounds__(kWg) void k_scf_batch(BatchArgs g) {
         double ecore = mol_setup(md, mi, g, ms, s, M + kH * (size_t)md.nao * md.nao, g.wpool + md.w_off);
         __syncthreads();
         phase_mark(pc, 0);
-        scf_rhf<PREC>(md, g, ms, s, lds, ecore, res, pc);
+        if (md.uhf) scf_uhf<PREC>(md, g, ms, s, lds, ecore, res, pc);
+        else scf_rhf<PREC>(md, g, ms, s, lds, ecore, res, pc);
         if (threadIdx.x == 0) {
             for (int k = 0; k < kPhases; ++k) res.phase[k] = pc.acc[k];
             res.t_done = __builtin_amdgcn_s_memrealtime();

@@ -128,3 +128,41 @@ def test_full_size_batch_properties(engine):
     ob = O.scf_many([ms[i] for i in idx], threads=8)
     for i, o in zip(idx, ob):
         _compare(rb[i], o)
+
+
+# ------------------------------------------------------------------------------ UHF
+def test_methyl_radical_uhf(engine):
+    # scf.rs:1372-1388
+    z, x = M.parse(M.METHYL)
+    r = engine.scf(z, x, multiplicity=2)
+    o = O.scf(z, x, multiplicity=2)
+    assert r["unrestricted"]
+    _compare(r, o)
+    assert np.abs(r["mo_energies_beta"] - o["mo_energies_beta"]).max() < 1e-6
+
+
+def test_forced_uhf_singlet_equals_rhf(engine):
+    # scf.rs:1391-1418
+    z, x = M.parse(M.WATER)
+    u = engine.scf(z, x, reference="uhf")
+    r = engine.scf(z, x)
+    assert u["unrestricted"] and not r["unrestricted"]
+    assert abs(u["total_ev"] - r["total_ev"]) < 1e-6
+    _compare(u, O.scf(z, x, reference="uhf"))
+
+
+def test_radical_chains_batch(engine):
+    from am1g import geometry as G
+
+    mols = G.chain_batch(24, seed=21)
+    ms = []
+    for m in mols:
+        hs = np.nonzero(m.z == 1)[0]
+        keep = np.ones(len(m.z), dtype=bool)
+        keep[hs[len(hs) // 2]] = False
+        ms.append((m.z[keep], m.xyz_bohr[keep], {"multiplicity": 2}))
+    rb = engine.scf_batch(ms)
+    ob = [O.scf(z, x, multiplicity=2, full=False) for z, x, _ in ms]
+    for a, b in zip(rb, ob):
+        assert a["unrestricted"]
+        _compare(a, b)
