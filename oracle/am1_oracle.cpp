@@ -1149,6 +1149,410 @@ bool adiis_coeffs(const std::vector<std::vector<double>>& ds, const std::vector<
     return true;
 }
 
+
+// ==================================================================== divide and conquer
+// divide_conquer.rs:249-379 (partition, subsystems), :588-744 (solve, fermi, assemble),
+// :800-1019 (density DIIS), :1033-1254 (loops); fermi.rs:110-283 (filling).
+
+// divide_conquer.rs:249-297: recursive proportional bisection along the widest axis.
+void dc_bisect(const System& s, std::vector<int> idx, int core_size, std::vector<std::vector<int>>& out) {
+    if ((int)idx.size() <= core_size) {
+        out.push_back(idx);
+        return;
+    }
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int i : idx)
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = std::min(lo[k], s.xyz[3 * i + k]);
+            hi[k] = std::max(hi[k], s.xyz[3 * i + k]);
+        }
+    int axis = 0;
+    for (int a = 1; a < 3; ++a)
+        if (hi[a] - lo[a] > hi[axis] - lo[axis]) axis = a;
+    std::sort(idx.begin(), idx.end(), [&](int a, int b) {
+        double ka = s.xyz[3 * a + axis], kb = s.xyz[3 * b + axis];
+        if (ka < kb) return true;
+        if (ka > kb) return false;
+        return a < b;
+    });
+    int n = (int)idx.size();
+    int parts = (n + core_size - 1) / core_size;
+    int left_parts = parts / 2;
+    int split_at = std::min(std::max(n * left_parts / parts, 1), n - 1);
+    std::vector<int> right(idx.begin() + split_at, idx.end());
+    idx.resize(split_at);
+    dc_bisect(s, idx, core_size, out);
+    dc_bisect(s, right, core_size, out);
+}
+
+struct DcSub {
+    std::vector<int> core, atoms, aos;
+    std::vector<char> ao_core;
+    std::vector<double> weight;  // atoms.size()^2
+};
+
+// divide_conquer.rs:300-379
+std::vector<DcSub> dc_build_subsystems(const System& s, const std::vector<std::vector<int>>& cores, double rbuf) {
+    std::vector<DcSub> subs(cores.size());
+    const double r2 = rbuf * rbuf;
+    auto d2 = [&](int a, int b) {
+        double acc = 0.0;
+        for (int k = 0; k < 3; ++k) {
+            double d = s.xyz[3 * b + k] - s.xyz[3 * a + k];
+            acc += d * d;
+        }
+        return acc;
+    };
+#pragma omp parallel for schedule(dynamic, 4)
+    for (long c = 0; c < (long)cores.size(); ++c) {
+        DcSub& sub = subs[c];
+        sub.core = cores[c];
+        std::vector<char> in_core(s.nat, 0);
+        for (int a : sub.core) in_core[a] = 1;
+        for (int j = 0; j < s.nat; ++j) {
+            bool take = in_core[j];
+            if (!take)
+                for (int a : sub.core)
+                    if (d2(a, j) <= r2) {
+                        take = true;
+                        break;
+                    }
+            if (take) sub.atoms.push_back(j);
+        }
+        int n = (int)sub.atoms.size();
+        for (int a : sub.atoms)
+            for (int k = 0; k < s.norb[a]; ++k) {
+                sub.aos.push_back(s.off[a] + k);
+                sub.ao_core.push_back(in_core[a]);
+            }
+        sub.weight.assign((size_t)n * n, 0.0);
+        for (int p = 0; p < n; ++p)
+            for (int q = 0; q < n; ++q) {
+                double w = 0.5 * ((double)in_core[sub.atoms[p]] + (double)in_core[sub.atoms[q]]);
+                if (w > 0.0 && d2(sub.atoms[p], sub.atoms[q]) <= r2) sub.weight[(size_t)p * n + q] = w;
+            }
+    }
+    return subs;
+}
+
+struct DcSol {
+    std::vector<double> eps, C, proj;  // C: n x n row-major, eigenvectors as columns
+};
+
+// divide_conquer.rs:588-618
+bool dc_solve(const Mat& F, const std::vector<DcSub>& subs, std::vector<DcSol>& sols) {
+    sols.resize(subs.size());
+    bool ok = true;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (long c = 0; c < (long)subs.size(); ++c) {
+        const DcSub& sub = subs[c];
+        int n = (int)sub.aos.size();
+        std::vector<double> blk((size_t)n * n);
+        for (int p = 0; p < n; ++p)
+            for (int q = 0; q < n; ++q) blk[(size_t)p * n + q] = F(sub.aos[p], sub.aos[q]);
+        DcSol& so = sols[c];
+        so.eps.assign(n, 0.0);
+        so.C.assign((size_t)n * n, 0.0);
+        if (!sym_eigen(n, blk.data(), so.eps.data(), so.C.data())) {
+#pragma omp atomic write
+            ok = false;
+            continue;
+        }
+        so.proj.assign(n, 0.0);
+        for (int i = 0; i < n; ++i) {
+            double acc = 0.0;
+            for (int p = 0; p < n; ++p)
+                if (sub.ao_core[p]) acc += so.C[(size_t)p * n + i] * so.C[(size_t)p * n + i];
+            so.proj[i] = acc;
+        }
+    }
+    return ok;
+}
+
+double fermi_dirac(double x) {  // fermi.rs:110-119
+    if (x > 40.0) return 0.0;
+    if (x < -40.0) return 1.0;
+    return 1.0 / (1.0 + std::exp(x));
+}
+
+// fermi.rs:124-269 restricted to what the D&C filling uses (Fermi-Dirac bisection, or aufbau
+// when kt <= 0). Returns false when the electrons do not fit.
+bool fill_levels(const std::vector<double>& e, const std::vector<double>& w, double nel, double kt,
+                 std::vector<double>& frac, double& mu, double& ts) {
+    double cap = 0.0;
+    for (double x : w) cap += x;
+    if (nel < -1e-9 || nel > cap + 1e-9) return false;
+    size_t n = e.size();
+    frac.assign(n, 0.0);
+    ts = 0.0;
+    if (n == 0) {
+        mu = 0.0;
+        return true;
+    }
+    if (kt <= 0.0) {  // fill_aufbau, fermi.rs:153-206
+        std::vector<size_t> order(n);
+        for (size_t i = 0; i < n; ++i) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return e[a] < e[b]; });
+        double left = nel, fermi = e[order.back()];
+        size_t i = 0;
+        while (i < n) {
+            if (left <= 1e-14) {
+                fermi = 0.5 * (fermi + e[order[i]]);
+                break;
+            }
+            double ei = e[order[i]];
+            size_t j = i;
+            double gc = 0.0;
+            while (j < n && std::fabs(e[order[j]] - ei) < 1e-10) gc += w[order[j++]];
+            double take = std::min(left, gc);
+            double f = gc > 0.0 ? take / gc : 0.0;
+            for (size_t k = i; k < j; ++k) frac[order[k]] = f;
+            left -= take;
+            fermi = ei;
+            i = j;
+        }
+        mu = fermi;
+        return true;
+    }
+    double lo = *std::min_element(e.begin(), e.end()) - 50.0 * kt - 1.0;
+    double hi = *std::max_element(e.begin(), e.end()) + 50.0 * kt + 1.0;
+    auto count = [&](double m) {
+        double acc = 0.0;
+        for (size_t i = 0; i < n; ++i) acc += w[i] * fermi_dirac((e[i] - m) / kt);
+        return acc;
+    };
+    mu = 0.5 * (lo + hi);
+    for (int it = 0; it < 200; ++it) {
+        mu = 0.5 * (lo + hi);
+        double c = count(mu);
+        if (std::fabs(c - nel) < 1e-13) break;
+        if (c < nel) lo = mu;
+        else hi = mu;
+    }
+    double sk = 0.0;
+    for (size_t i = 0; i < n; ++i) {
+        double f = fermi_dirac((e[i] - mu) / kt);
+        frac[i] = f;
+        if (f > 1e-14 && f < 1.0 - 1e-14) sk -= w[i] * (f * std::log(f) + (1.0 - f) * std::log(1.0 - f));
+    }
+    ts = kt * sk;
+    return true;
+}
+
+// divide_conquer.rs:630-656; fractions per subsystem
+bool dc_common_fermi(const std::vector<DcSol>& sols, double cap, double nel, double kt,
+                     std::vector<std::vector<double>>& fr, double& mu, double& ts) {
+    std::vector<double> e, w;
+    for (const DcSol& so : sols)
+        for (size_t i = 0; i < so.eps.size(); ++i) {
+            e.push_back(so.eps[i]);
+            w.push_back(cap * so.proj[i]);
+        }
+    std::vector<double> f;
+    if (!fill_levels(e, w, nel, kt, f, mu, ts)) return false;
+    fr.resize(sols.size());
+    size_t cur = 0;
+    for (size_t c = 0; c < sols.size(); ++c) {
+        fr[c].assign(f.begin() + cur, f.begin() + cur + sols[c].eps.size());
+        cur += sols[c].eps.size();
+    }
+    return true;
+}
+
+double dc_smallest_gap(const std::vector<DcSol>& sols, double mu) {  // :659-683
+    double gap = INFINITY;
+    for (const DcSol& so : sols) {
+        double below = -INFINITY, above = INFINITY;
+        for (double e : so.eps) {
+            if (e <= mu) below = std::max(below, e);
+            else above = std::min(above, e);
+        }
+        if (std::isfinite(below) && std::isfinite(above)) gap = std::min(gap, above - below);
+    }
+    return std::isfinite(gap) ? gap : 0.0;
+}
+
+// divide_conquer.rs:686-744
+Mat dc_assemble(const System& s, const std::vector<DcSub>& subs, const std::vector<DcSol>& sols,
+                const std::vector<std::vector<double>>& fr, double cap) {
+    std::vector<std::vector<double>> blocks(subs.size());
+#pragma omp parallel for schedule(dynamic, 1)
+    for (long c = 0; c < (long)subs.size(); ++c) {
+        int n = (int)subs[c].aos.size();
+        std::vector<int> kept;
+        for (int i = 0; i < n; ++i)
+            if (fr[c][i] > 1e-14) kept.push_back(i);
+        int k = (int)kept.size();
+        std::vector<double> sc((size_t)n * std::max(k, 1), 0.0);
+        for (int col = 0; col < k; ++col) {
+            double sq = std::sqrt(cap * fr[c][kept[col]]);
+            for (int p = 0; p < n; ++p) sc[(size_t)p * k + col] = sols[c].C[(size_t)p * n + kept[col]] * sq;
+        }
+        blocks[c].assign((size_t)n * n, 0.0);
+        for (int p = 0; p < n; ++p)
+            for (int q = 0; q < n; ++q) {
+                double acc = 0.0;
+                for (int col = 0; col < k; ++col) acc += sc[(size_t)p * k + col] * sc[(size_t)q * k + col];
+                blocks[c][(size_t)p * n + q] = acc;
+            }
+    }
+    Mat P(s.nao);
+    for (size_t c = 0; c < subs.size(); ++c) {
+        const DcSub& sub = subs[c];
+        int na = (int)sub.atoms.size(), n = (int)sub.aos.size();
+        std::vector<int> loff(na);
+        int acc = 0;
+        for (int pa = 0; pa < na; ++pa) {
+            loff[pa] = acc;
+            acc += s.norb[sub.atoms[pa]];
+        }
+        for (int pa = 0; pa < na; ++pa)
+            for (int qb = 0; qb < na; ++qb) {
+                double w = sub.weight[(size_t)pa * na + qb];
+                if (w == 0.0) continue;
+                int a = sub.atoms[pa], b = sub.atoms[qb];
+                for (int i = 0; i < s.norb[a]; ++i)
+                    for (int j = 0; j < s.norb[b]; ++j)
+                        P(s.off[a] + i, s.off[b] + j) += w * blocks[c][(size_t)(loff[pa] + i) * n + loff[qb] + j];
+            }
+    }
+    return P;
+}
+
+// divide_conquer.rs:835-857
+std::vector<size_t> dc_pattern(const System& s, const std::vector<DcSub>& subs) {
+    std::vector<size_t> out;
+    size_t nao = s.nao;
+    for (const DcSub& sub : subs) {
+        int na = (int)sub.atoms.size();
+        for (int pa = 0; pa < na; ++pa)
+            for (int qb = 0; qb < na; ++qb) {
+                if (sub.weight[(size_t)pa * na + qb] == 0.0) continue;
+                int a = sub.atoms[pa], b = sub.atoms[qb];
+                for (int i = 0; i < s.norb[a]; ++i)
+                    for (int j = 0; j < s.norb[b]; ++j) {
+                        size_t r = s.off[a] + i, c = s.off[b] + j;
+                        if (r > c) std::swap(r, c);
+                        out.push_back(r * nao + c);
+                    }
+            }
+    }
+    std::sort(out.begin(), out.end());
+    out.erase(std::unique(out.begin(), out.end()), out.end());
+    return out;
+}
+
+// linalg.rs:334-361 (partial-pivot LU; reject non-finite or |x| > 1e8 max(|b|, 1))
+bool solve_linear(std::vector<double> a, std::vector<double> b, int n, std::vector<double>& x) {
+    std::vector<int> perm(n);
+    for (int i = 0; i < n; ++i) perm[i] = i;
+    for (int col = 0; col < n; ++col) {
+        int piv = col;
+        double best = std::fabs(a[col * n + col]);
+        for (int r = col + 1; r < n; ++r)
+            if (std::fabs(a[r * n + col]) > best) {
+                best = std::fabs(a[r * n + col]);
+                piv = r;
+            }
+        if (piv != col) {
+            for (int j = 0; j < n; ++j) std::swap(a[col * n + j], a[piv * n + j]);
+            std::swap(b[col], b[piv]);
+        }
+        for (int r = col + 1; r < n; ++r) {
+            double f = a[r * n + col] / a[col * n + col];
+            for (int j = col; j < n; ++j) a[r * n + j] -= f * a[col * n + j];
+            b[r] -= f * b[col];
+        }
+    }
+    x.assign(n, 0.0);
+    for (int col = n - 1; col >= 0; --col) {
+        double sum = b[col];
+        for (int j = col + 1; j < n; ++j) sum -= a[col * n + j] * x[j];
+        x[col] = sum / a[col * n + col];
+    }
+    double bmax = 1.0;
+    for (double v : b) bmax = std::max(bmax, std::fabs(v));
+    for (double v : x)
+        if (!std::isfinite(v) || std::fabs(v) > 1e8 * bmax) return false;
+    return true;
+}
+
+// divide_conquer.rs:800-1019
+struct DensityDiis {
+    std::vector<std::vector<double>> inputs, residuals, dots;
+    std::vector<size_t> pattern;
+    std::vector<double> mult;
+    size_t nao;
+    size_t max;
+    DensityDiis(size_t maxd, size_t nao_, std::vector<size_t> pat) : pattern(std::move(pat)), nao(nao_) {
+        size_t per = 2 * std::max<size_t>(pattern.size(), 1) * sizeof(double);
+        size_t affordable = std::max<size_t>((size_t(512) << 20) / std::max<size_t>(per, 1), 2);
+        max = std::min(maxd, affordable);
+        mult.resize(pattern.size());
+        for (size_t k = 0; k < pattern.size(); ++k) mult[k] = (pattern[k] / nao == pattern[k] % nao) ? 1.0 : 2.0;
+    }
+    std::vector<double> gather(const Mat& m) const {
+        std::vector<double> out(pattern.size());
+        for (size_t k = 0; k < pattern.size(); ++k) out[k] = m.d[pattern[k]];
+        return out;
+    }
+    double rdot(size_t i, size_t j) const {
+        double acc = 0.0;
+        for (size_t k = 0; k < pattern.size(); ++k) acc += mult[k] * residuals[i][k] * residuals[j][k];
+        return acc;
+    }
+    void push(const Mat& in, const Mat& res) {
+        inputs.push_back(gather(in));
+        residuals.push_back(gather(res));
+        size_t i = residuals.size() - 1;
+        std::vector<double> row(i + 1);
+        for (size_t j = 0; j <= i; ++j) row[j] = rdot(i, j);
+        dots.push_back(row);
+        if (inputs.size() > max) {
+            inputs.erase(inputs.begin());
+            residuals.erase(residuals.begin());
+            dots.erase(dots.begin());
+            for (auto& r : dots) r.erase(r.begin());
+        }
+    }
+    bool extrapolate(Mat& out) const {
+        size_t n = inputs.size();
+        if (n < 2) return false;
+        int dim = (int)n + 1;
+        std::vector<double> B((size_t)dim * dim, 0.0);
+        double scale = 0.0;
+        for (size_t i = 0; i < n; ++i) {
+            for (size_t j = 0; j <= i; ++j) {
+                double v = dots[i][j];
+                B[i * dim + j] = v;
+                B[j * dim + i] = v;
+                scale = std::max(scale, std::fabs(v));
+            }
+            B[i * dim + n] = -1.0;
+            B[n * dim + i] = -1.0;
+        }
+        if (scale <= 0.0) return false;
+        for (size_t i = 0; i < n; ++i) {
+            for (size_t j = 0; j < n; ++j) B[i * dim + j] /= scale;
+            B[i * dim + i] += 1e-10;
+        }
+        std::vector<double> rhs(dim, 0.0), c;
+        rhs[n] = -1.0;
+        if (!solve_linear(B, rhs, dim, c)) return false;
+        for (size_t i = 0; i < n; ++i)
+            if (!std::isfinite(c[i])) return false;
+        std::vector<double> packed(pattern.size(), 0.0);
+        for (size_t i = 0; i < n; ++i)
+            for (size_t k = 0; k < pattern.size(); ++k) packed[k] += c[i] * (inputs[i][k] + residuals[i][k]);
+        out = Mat((int)nao);
+        for (size_t k = 0; k < pattern.size(); ++k) {
+            size_t r = pattern[k] / nao, cc = pattern[k] % nao;
+            out.d[pattern[k]] = packed[k];
+            out.d[cc * nao + r] = packed[k];
+        }
+        return true;
+    }
+};
 }  // namespace
 
 // ===================================================================================== C API
@@ -1545,6 +1949,215 @@ int oracle_scf_many(int n, const OracleRequest* rqs, OracleResult* res, int thre
 #ifdef _OPENMP
     omp_set_max_active_levels(prev_nested);
 #endif
+    return 0;
+}
+
+
+struct OracleDcOptions {
+    int core_size;        // used when cores is NULL
+    double buffer_radius; // Bohr
+    double kt;            // eV; <= 0 selects aufbau filling
+    int max_scf;
+    double e_tol, p_tol, mixing;
+    const int* core_of_atom;  // optional [n_atoms]: core index of each atom (caller-given cores)
+};
+
+struct OracleDcResult {
+    int status, iterations, converged, unrestricted, n_subsystems, largest_subsystem_aos;
+    double electronic_ev, core_ev, total_ev, heat_of_formation_kcal;
+    double fermi_ev, fermi_beta_ev, entropy_ev, gap_ev;
+    double* charges;  // [n_atoms]
+    double* density;  // [nao*nao] optional
+    char message[256];
+};
+
+// run_divide_conquer (divide_conquer.rs:406-572) with dc_loop_restricted / _unrestricted.
+int oracle_dc(const OracleRequest* rq, const OracleDcOptions* op, OracleDcResult* res) {
+    res->message[0] = 0;
+    res->iterations = 0;
+    res->converged = 0;
+    auto fail = [&](int st, const std::string& m) {
+        res->status = st;
+        std::snprintf(res->message, sizeof(res->message), "%s", m.c_str());
+        return st;
+    };
+    if (op->buffer_radius <= 0.0) return fail(ST_INVALID, "divide-and-conquer buffer_radius must be positive");
+    System s;
+    if (!build_system(s, rq->n_atoms, rq->z, rq->xyz_bohr, rq->method)) return fail(s.status, s.err);
+    const ParamSet& ps = *s.ps;
+    Mat H;
+    std::vector<Pair> pairs;
+    build_core(s, H, pairs);
+    std::vector<std::vector<int>> cores;
+    if (op->core_of_atom) {
+        int nc = 0;
+        for (int a = 0; a < s.nat; ++a) nc = std::max(nc, op->core_of_atom[a] + 1);
+        cores.assign(nc, {});
+        for (int a = 0; a < s.nat; ++a) cores[op->core_of_atom[a]].push_back(a);
+    } else {
+        std::vector<int> all(s.nat);
+        for (int i = 0; i < s.nat; ++i) all[i] = i;
+        dc_bisect(s, all, std::max(op->core_size, 1), cores);
+    }
+    std::vector<DcSub> subs = dc_build_subsystems(s, cores, op->buffer_radius);
+    res->n_subsystems = (int)subs.size();
+    res->largest_subsystem_aos = 0;
+    for (const DcSub& sub : subs) res->largest_subsystem_aos = std::max(res->largest_subsystem_aos, (int)sub.aos.size());
+    double n_elec = 0.0;
+    for (int i = 0; i < s.nat; ++i) n_elec += ps.el[s.z[i]].core;
+    n_elec -= rq->charge;
+    long long ne = std::llround(n_elec);
+    if (std::fabs(n_elec - (double)ne) > 1e-6 || ne < 0) return fail(ST_INVALID, "invalid electron count");
+    long long nun = std::max(rq->multiplicity, 1) - 1;
+    if (ne - nun < 0 || (ne - nun) % 2 != 0) return fail(ST_INVALID, "electron count is incompatible with multiplicity");
+    double na = (double)((ne + nun) / 2), nb = (double)((ne - nun) / 2);
+    bool closed = std::fabs(na - nb) < 1e-12;
+    bool uhf = rq->reference == 0 ? !closed : rq->reference == 2;
+    if (rq->reference == 1 && !closed) return fail(ST_INVALID, "restricted (RHF) reference requires a closed shell");
+    res->unrestricted = uhf;
+    std::vector<size_t> pat = dc_pattern(s, subs);
+    const int nao = s.nao;
+    Mat P0 = sad_density(s);
+    double e_old = 0.0, fermi = 0.0, fermi_b = 0.0, entropy = 0.0, gap = 0.0;
+    bool converged = false;
+    int iterations = 0;
+    Mat Ptot;
+    if (!uhf) {
+        Mat P = P0;
+        DensityDiis diis(8, nao, pat);
+        for (int iter = 0; iter < op->max_scf; ++iter) {
+            iterations = iter + 1;
+            Mat F;
+            build_fock(s, H, pairs, P, P, 0.5, F);
+            double e = 0.5 * (frob(P, H) + frob(P, F));
+            std::vector<DcSol> sols;
+            if (!dc_solve(F, subs, sols)) return fail(ST_LINALG, "eigensolver failed");
+            std::vector<std::vector<double>> fr;
+            double mu, ts;
+            if (!dc_common_fermi(sols, 2.0, n_elec, op->kt, fr, mu, ts)) return fail(ST_INVALID, "filling failed");
+            Mat Pout = dc_assemble(s, subs, sols, fr, 2.0);
+            fermi = mu;
+            entropy = ts;
+            gap = dc_smallest_gap(sols, mu);
+            double dp = rms_diff(Pout, P), de = std::fabs(e - e_old);
+            e_old = e;
+            if (iter > 0 && de < op->e_tol && dp < op->p_tol) {
+                P = Pout;
+                converged = true;
+                break;
+            }
+            Mat r(nao);
+            for (size_t k = 0; k < r.d.size(); ++k) r.d[k] = Pout.d[k] - P.d[k];
+            diis.push(P, r);
+            Mat next;
+            if (diis.extrapolate(next)) P = next;
+            else
+                for (size_t k = 0; k < P.d.size(); ++k) P.d[k] += op->mixing * (Pout.d[k] - P.d[k]);
+        }
+        Mat F;
+        build_fock(s, H, pairs, P, P, 0.5, F);
+        res->electronic_ev = 0.5 * (frob(P, H) + frob(P, F));
+        Ptot = P;
+    } else {
+        double nt = std::max(na + nb, 1.0);
+        Mat Pa = P0, Pb = P0;
+        for (auto& v : Pa.d) v *= na / nt;
+        for (auto& v : Pb.d) v *= nb / nt;
+        DensityDiis da(8, nao, pat), db(8, nao, pat);
+        Mat Pt(nao);
+        for (int iter = 0; iter < op->max_scf; ++iter) {
+            iterations = iter + 1;
+            for (size_t k = 0; k < Pt.d.size(); ++k) Pt.d[k] = Pa.d[k] + Pb.d[k];
+            Mat Fa, Fb;
+            build_fock(s, H, pairs, Pt, Pa, 1.0, Fa);
+            build_fock(s, H, pairs, Pt, Pb, 1.0, Fb);
+            double e = 0.5 * (frob(Pt, H) + frob(Pa, Fa) + frob(Pb, Fb));
+            std::vector<DcSol> sa, sb;
+            if (!dc_solve(Fa, subs, sa) || !dc_solve(Fb, subs, sb)) return fail(ST_LINALG, "eigensolver failed");
+            std::vector<std::vector<double>> fa, fb;
+            double mua, mub, tsa, tsb;
+            if (!dc_common_fermi(sa, 1.0, na, op->kt, fa, mua, tsa) || !dc_common_fermi(sb, 1.0, nb, op->kt, fb, mub, tsb))
+                return fail(ST_INVALID, "filling failed");
+            Mat Pao = dc_assemble(s, subs, sa, fa, 1.0), Pbo = dc_assemble(s, subs, sb, fb, 1.0);
+            fermi = mua;
+            fermi_b = mub;
+            entropy = tsa + tsb;
+            gap = std::min(dc_smallest_gap(sa, mua), dc_smallest_gap(sb, mub));
+            double dp = std::max(rms_diff(Pao, Pa), rms_diff(Pbo, Pb)), de = std::fabs(e - e_old);
+            e_old = e;
+            if (iter > 0 && de < op->e_tol && dp < op->p_tol) {
+                Pa = Pao;
+                Pb = Pbo;
+                converged = true;
+                break;
+            }
+            Mat* ps_[2] = {&Pa, &Pb};
+            Mat* po_[2] = {&Pao, &Pbo};
+            DensityDiis* dd_[2] = {&da, &db};
+            for (int c = 0; c < 2; ++c) {
+                Mat r(nao);
+                for (size_t k = 0; k < r.d.size(); ++k) r.d[k] = po_[c]->d[k] - ps_[c]->d[k];
+                dd_[c]->push(*ps_[c], r);
+                Mat next;
+                if (dd_[c]->extrapolate(next)) *ps_[c] = next;
+                else
+                    for (size_t k = 0; k < ps_[c]->d.size(); ++k)
+                        ps_[c]->d[k] += op->mixing * (po_[c]->d[k] - ps_[c]->d[k]);
+            }
+        }
+        for (size_t k = 0; k < Pt.d.size(); ++k) Pt.d[k] = Pa.d[k] + Pb.d[k];
+        Mat Fa, Fb;
+        build_fock(s, H, pairs, Pt, Pa, 1.0, Fa);
+        build_fock(s, H, pairs, Pt, Pb, 1.0, Fb);
+        res->electronic_ev = 0.5 * (frob(Pt, H) + frob(Pa, Fa) + frob(Pb, Fb));
+        Ptot = Pt;
+    }
+    res->iterations = iterations;
+    res->converged = converged;
+    res->fermi_ev = fermi;
+    res->fermi_beta_ev = uhf ? fermi_b : fermi;
+    res->entropy_ev = entropy;
+    res->gap_ev = gap;
+    res->core_ev = core_core_energy(s);
+    res->total_ev = res->electronic_ev + res->core_ev;
+    double eiso = 0.0, eheat = 0.0;
+    for (int i = 0; i < s.nat; ++i) {
+        eiso += ps.el[s.z[i]].e_isol;
+        eheat += ps.el[s.z[i]].eheat_ev;
+    }
+    res->heat_of_formation_kcal = (res->total_ev - eiso + eheat) * kEvToKcal;
+    for (int i = 0; i < s.nat; ++i) {
+        double pop = 0.0;
+        for (int k = 0; k < s.norb[i]; ++k) pop += Ptot(s.off[i] + k, s.off[i] + k);
+        if (res->charges) res->charges[i] = ps.el[s.z[i]].core - pop;
+    }
+    if (res->density) std::copy(Ptot.d.begin(), Ptot.d.end(), res->density);
+    res->status = converged ? ST_OK : ST_NOTCONV;
+    return res->status;
+}
+
+// Yang-weight sum of every atom-pair block over the subsystems (divide_conquer.rs:381-396).
+int oracle_dc_weight_sum(const OracleRequest* rq, const OracleDcOptions* op, double* out) {
+    System s;
+    if (!build_system(s, rq->n_atoms, rq->z, rq->xyz_bohr, rq->method)) return s.status;
+    std::vector<std::vector<int>> cores;
+    if (op->core_of_atom) {
+        int nc = 0;
+        for (int a = 0; a < s.nat; ++a) nc = std::max(nc, op->core_of_atom[a] + 1);
+        cores.assign(nc, {});
+        for (int a = 0; a < s.nat; ++a) cores[op->core_of_atom[a]].push_back(a);
+    } else {
+        std::vector<int> all(s.nat);
+        for (int i = 0; i < s.nat; ++i) all[i] = i;
+        dc_bisect(s, all, std::max(op->core_size, 1), cores);
+    }
+    std::vector<DcSub> subs = dc_build_subsystems(s, cores, op->buffer_radius);
+    std::fill(out, out + (size_t)s.nat * s.nat, 0.0);
+    for (const DcSub& sub : subs) {
+        int n = (int)sub.atoms.size();
+        for (int p = 0; p < n; ++p)
+            for (int q = 0; q < n; ++q) out[(size_t)sub.atoms[p] * s.nat + sub.atoms[q]] += sub.weight[(size_t)p * n + q];
+    }
     return 0;
 }
 

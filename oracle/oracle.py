@@ -334,3 +334,96 @@ def adiis_simplex(d, s):
     c = np.zeros(len(d))
     lib().oracle_adiis_simplex(len(d), _dp(d), _dp(s), _dp(c))
     return c
+
+
+# ---------------------------------------------------------------------------- divide and conquer
+class _DcOptions(ctypes.Structure):
+    _fields_ = [
+        ("core_size", ctypes.c_int),
+        ("buffer_radius", ctypes.c_double),
+        ("kt", ctypes.c_double),
+        ("max_scf", ctypes.c_int),
+        ("e_tol", ctypes.c_double),
+        ("p_tol", ctypes.c_double),
+        ("mixing", ctypes.c_double),
+        ("core_of_atom", ctypes.POINTER(ctypes.c_int)),
+    ]
+
+
+class _DcResult(ctypes.Structure):
+    _fields_ = [
+        ("status", ctypes.c_int),
+        ("iterations", ctypes.c_int),
+        ("converged", ctypes.c_int),
+        ("unrestricted", ctypes.c_int),
+        ("n_subsystems", ctypes.c_int),
+        ("largest_subsystem_aos", ctypes.c_int),
+        ("electronic_ev", ctypes.c_double),
+        ("core_ev", ctypes.c_double),
+        ("total_ev", ctypes.c_double),
+        ("heat_of_formation_kcal", ctypes.c_double),
+        ("fermi_ev", ctypes.c_double),
+        ("fermi_beta_ev", ctypes.c_double),
+        ("entropy_ev", ctypes.c_double),
+        ("gap_ev", ctypes.c_double),
+        ("charges", ctypes.POINTER(ctypes.c_double)),
+        ("density", ctypes.POINTER(ctypes.c_double)),
+        ("message", ctypes.c_char * 256),
+    ]
+
+
+def _dc_options(core_size, buffer_radius, kt, max_scf, e_tol, p_tol, mixing, core_of_atom, keep):
+    op = _DcOptions()
+    op.core_size = int(core_size)
+    op.buffer_radius = float(buffer_radius)
+    op.kt = float(kt)
+    op.max_scf = int(max_scf)
+    op.e_tol = float(e_tol)
+    op.p_tol = float(p_tol)
+    op.mixing = float(mixing)
+    if core_of_atom is not None:
+        c = np.ascontiguousarray(core_of_atom, dtype=np.int32)
+        keep.append(c)
+        op.core_of_atom = _ip(c)
+    return op
+
+
+def dc(z, xyz_bohr, core_size=12, buffer_radius=11.0, kt=0.05, max_scf=300, e_tol=1e-7, p_tol=1e-6,
+       mixing=0.4, core_of_atom=None, density=False, **kw):
+    """``run_divide_conquer`` (divide_conquer.rs:406-572). ``buffer_radius`` in Bohr, ``kt`` in eV
+    (``kt <= 0`` is aufbau filling). ``core_of_atom`` gives caller-chosen cores."""
+    L = lib()
+    L.oracle_dc.argtypes = [ctypes.POINTER(_Request), ctypes.POINTER(_DcOptions), ctypes.POINTER(_DcResult)]
+    keep = []
+    rq = _make_request(z, xyz_bohr, keep=keep, **kw)
+    op = _dc_options(core_size, buffer_radius, kt, max_scf, e_tol, p_tol, mixing, core_of_atom, keep)
+    rs = _DcResult()
+    q = np.zeros(len(z))
+    rs.charges = _dp(q)
+    P = None
+    if density:
+        nao = sum(1 if int(v) == 1 else 4 for v in z)
+        P = np.zeros(nao * nao)
+        rs.density = _dp(P)
+    L.oracle_dc(ctypes.byref(rq), ctypes.byref(op), ctypes.byref(rs))
+    out = {k: getattr(rs, k) for k, _ in _DcResult._fields_ if k not in ("charges", "density", "message")}
+    out["message"] = rs.message.decode(errors="replace")
+    out["charges"] = q
+    out["converged"] = bool(rs.converged)
+    if density:
+        n = int(round(len(P) ** 0.5))
+        out["density"] = P.reshape(n, n)
+    return out
+
+
+def dc_weight_sum(z, xyz_bohr, core_size=12, buffer_radius=11.0, core_of_atom=None, method="am1"):
+    L = lib()
+    L.oracle_dc_weight_sum.argtypes = [ctypes.POINTER(_Request), ctypes.POINTER(_DcOptions),
+                                       ctypes.POINTER(ctypes.c_double)]
+    keep = []
+    rq = _make_request(z, xyz_bohr, keep=keep, method=method)
+    op = _dc_options(core_size, buffer_radius, 0.0, 1, 0.0, 0.0, 0.0, core_of_atom, keep)
+    n = len(z)
+    out = np.zeros(n * n)
+    L.oracle_dc_weight_sum(ctypes.byref(rq), ctypes.byref(op), _dp(out))
+    return out.reshape(n, n)

@@ -37,3 +37,14 @@ def parse(text):
         z.append(_Z[p[0]])
         x.append([float(v) * ANGSTROM_TO_BOHR for v in p[1:4]])
     return np.array(z, dtype=np.int32), np.array(x)
+
+
+def water_chain(n):
+    """tests/divide_conquer.rs:22-43: waters along x at 2.9 A with alternating tilt (Bohr out)."""
+    z, x = [], []
+    for i in range(n):
+        base = np.array([i * 2.9, 0.0 if i % 2 == 0 else 0.35, 0.0])
+        for zz, d in ((8, (0.0, 0.0, 0.0)), (1, (0.9584, 0.0, 0.0)), (1, (-0.2400, 0.9279, 0.0))):
+            z.append(zz)
+            x.append((base + np.array(d)) * ANGSTROM_TO_BOHR)
+    return np.array(z, dtype=np.int32), np.array(x)
