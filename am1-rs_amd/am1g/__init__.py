@@ -90,9 +90,47 @@ class Result(ctypes.Structure):
     ]
 
 
+class DcOptions(ctypes.Structure):
+    _fields_ = [
+        ("core_size", ctypes.c_int),
+        ("buffer_radius", ctypes.c_double),
+        ("kt", ctypes.c_double),
+        ("max_scf", ctypes.c_int),
+        ("e_tol", ctypes.c_double),
+        ("p_tol", ctypes.c_double),
+        ("mixing", ctypes.c_double),
+        ("core_of_atom", ctypes.POINTER(ctypes.c_int)),
+    ]
+
+
+class DcResult(ctypes.Structure):
+    _fields_ = [
+        ("status", ctypes.c_int),
+        ("iterations", ctypes.c_int),
+        ("converged", ctypes.c_int),
+        ("unrestricted", ctypes.c_int),
+        ("n_subsystems", ctypes.c_int),
+        ("largest_subsystem_aos", ctypes.c_int),
+        ("electronic_ev", ctypes.c_double),
+        ("core_ev", ctypes.c_double),
+        ("total_ev", ctypes.c_double),
+        ("heat_of_formation_kcal", ctypes.c_double),
+        ("fermi_ev", ctypes.c_double),
+        ("entropy_ev", ctypes.c_double),
+        ("homo_lumo_gap_ev", ctypes.c_double),
+        ("diagonalization_work", ctypes.c_double),
+        ("retained_pattern", ctypes.c_longlong),
+        ("phase_ms", ctypes.c_double * 5),
+        ("charges", ctypes.POINTER(ctypes.c_double)),
+        ("density", ctypes.POINTER(ctypes.c_double)),
+    ]
+
+
+DC_PHASES = ["fock", "eigen", "fermi", "assembly", "diis"]
+
 # Every symbol include/am1g.h declares (checked by the CPU test-suite against the built library).
 EXPORTS = ["am1g_request_defaults", "am1g_create", "am1g_destroy", "am1g_scf", "am1g_scf_batch",
-           "am1g_last_error", "am1g_element_info", "am1g_last_kernel_ms"]
+           "am1g_dc_defaults", "am1g_dc", "am1g_last_error", "am1g_element_info", "am1g_last_kernel_ms"]
 
 _lib = None
 
@@ -116,6 +154,9 @@ def load():
     L.am1g_destroy.argtypes = [ctypes.c_void_p]
     L.am1g_scf.argtypes = [ctypes.c_void_p, ctypes.POINTER(Request), ctypes.POINTER(Result)]
     L.am1g_scf_batch.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(Request), ctypes.POINTER(Result)]
+    L.am1g_dc_defaults.argtypes = [ctypes.POINTER(DcOptions)]
+    L.am1g_dc.argtypes = [ctypes.c_void_p, ctypes.POINTER(Request), ctypes.POINTER(DcOptions),
+                          ctypes.POINTER(DcResult)]
     L.am1g_last_error.argtypes = [ctypes.c_void_p]
     L.am1g_last_error.restype = ctypes.c_char_p
     L.am1g_request_defaults.argtypes = [ctypes.POINTER(Request)]
@@ -277,6 +318,45 @@ class Engine:
         if st != 0:
             raise RuntimeError(f"am1g_scf_batch failed ({STATUS_NAMES.get(st, st)}): {self.last_error()}")
         return [self._to_dict(rss[i], arrays[i], full) for i in range(n)]
+
+    def dc(self, z, xyz_bohr, core_size=None, buffer_radius=None, kt=None, max_scf=None, e_tol=None,
+           p_tol=None, mixing=None, core_of_atom=None, density=False, **kw) -> dict:
+        """Divide-and-conquer SCF (``am1g_dc``; run_divide_conquer, divide_conquer.rs:406-572).
+        ``buffer_radius`` in Bohr, ``kt`` in eV (``kt <= 0`` is aufbau filling); options left
+        as None take the library defaults. ``core_of_atom`` gives caller-chosen cores."""
+        keep = []
+        rq = self._request(z, xyz_bohr, keep, **kw)
+        op = DcOptions()
+        self._L.am1g_dc_defaults(ctypes.byref(op))
+        for k, v in dict(core_size=core_size, buffer_radius=buffer_radius, kt=kt, max_scf=max_scf, e_tol=e_tol,
+                         p_tol=p_tol, mixing=mixing).items():
+            if v is not None:
+                setattr(op, k, v)
+        if core_of_atom is not None:
+            ca = np.ascontiguousarray(core_of_atom, dtype=np.int32)
+            keep.append(ca)
+            op.core_of_atom = ca.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        rs = DcResult()
+        q = np.zeros(rq.n_atoms)
+        rs.charges = q.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        P = None
+        if density:
+            nao = sum(1 if int(v) == 1 else 4 for v in np.asarray(z).ravel())
+            P = np.zeros(nao * nao)
+            rs.density = P.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        self._L.am1g_dc(self._ctx, ctypes.byref(rq), ctypes.byref(op), ctypes.byref(rs))
+        out = {k: getattr(rs, k) for k, _ in DcResult._fields_ if k not in ("charges", "density", "phase_ms")}
+        out["converged"] = bool(rs.converged)
+        out["status_name"] = STATUS_NAMES.get(rs.status, str(rs.status))
+        out["phase_ms"] = dict(zip(DC_PHASES, list(rs.phase_ms)))
+        out["wall_ms"] = self.last_kernel_ms()
+        out["charges"] = q
+        if density:
+            n = int(round(len(P) ** 0.5))
+            out["density"] = P.reshape(n, n)
+        if rs.status not in (0, 4):
+            out["message"] = self.last_error()
+        return out
 
 
 _default_engine = None

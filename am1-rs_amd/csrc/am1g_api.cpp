@@ -18,49 +18,13 @@
 #include "../../include/am1g.h"
 #include "am1g_common.hpp"
 #include "am1g_launch.hpp"
+#include "am1g_ctx.hpp"
 
 using namespace am1g;
 
-struct DevBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    hipError_t reserve(size_t need) {
-        if (need <= bytes) return hipSuccess;
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-        size_t want = std::max(need, (size_t)4096);
-        hipError_t e = hipMalloc(&p, want);
-        if (e == hipSuccess) bytes = want;
-        return e;
-    }
-    template <typename T>
-    T* as() const {
-        return static_cast<T*>(p);
-    }
-    ~DevBuf() {
-        if (p) (void)hipFree(p);
-    }
-};
-
-struct am1g_ctx {
-    int device = 0;
-    int cus = 256;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    double last_ms = 0.0;
-    std::mutex mu;
-    std::string err;
-    DevElem host_tab[kMethods * kMaxZ];
-    DevBuf elems, mols, z, xyz, mats, wpool, hist, mo_e, mo_eb, charges, results, queue, order;
-    DevBuf alist, aoff, acls, pair_off, ecore_pair;
-};
-
-namespace {
-
 thread_local std::string g_tls_error;
 
-int set_error(am1g_ctx* ctx, int code, const char* fmt, ...) {
+int am1g_set_error(am1g_ctx* ctx, int code, const char* fmt, ...) {
     char buf[512];
     va_list ap;
     va_start(ap, fmt);
@@ -71,27 +35,14 @@ int set_error(am1g_ctx* ctx, int code, const char* fmt, ...) {
     return code;
 }
 
-#define HIP_TRY(ctx, expr)                                                                              \
-    do {                                                                                                \
-        hipError_t e_ = (expr);                                                                         \
-        if (e_ != hipSuccess)                                                                           \
-            return set_error(ctx, AM1G_DEVICE_ERROR, "HIP error %s at %s", hipGetErrorString(e_), #expr); \
-    } while (0)
-
-struct Prepared {
-    MolDesc md;
-    int nat;
-    bool uhf;
-};
-
 // run_am1's input checks (scf.rs:300-366) plus the element checks of Basis::build.
-int prepare(am1g_ctx* ctx, const am1g_request& rq, Prepared& out, std::string& msg) {
+int am1g_prepare(am1g_ctx* ctx, const am1g_request& rq, Prepared& out, std::string& msg, int max_ao) {
     char buf[256];
     if (rq.n_atoms < 1 || !rq.z || !rq.xyz_bohr) {
         msg = "molecule has no atoms";
         return AM1G_INVALID_INPUT;
     }
-    if (rq.n_atoms > kMaxAtoms) {
+    if (max_ao <= kMaxBatchAo && rq.n_atoms > kMaxAtoms) {
         std::snprintf(buf, sizeof(buf), "molecule has %d atoms; this entry point takes at most %d", rq.n_atoms,
                       kMaxAtoms);
         msg = buf;
@@ -161,9 +112,8 @@ int prepare(am1g_ctx* ctx, const am1g_request& rq, Prepared& out, std::string& m
         }
         uhf = false;
     }
-    if (nao > kMaxBatchAo) {
-        std::snprintf(buf, sizeof(buf), "%d basis functions; the per-molecule SCF kernel takes at most %d", nao,
-                      kMaxBatchAo);
+    if (nao > max_ao) {
+        std::snprintf(buf, sizeof(buf), "%d basis functions; this entry point takes at most %d", nao, max_ao);
         msg = buf;
         return AM1G_INVALID_INPUT;
     }
@@ -192,6 +142,8 @@ int prepare(am1g_ctx* ctx, const am1g_request& rq, Prepared& out, std::string& m
     out.uhf = uhf;
     return AM1G_OK;
 }
+
+namespace {
 
 void fill_result(const MolResult& r, const MolDesc& md, am1g_result& out) {
     out.status = r.status;
@@ -238,14 +190,14 @@ void am1g_request_defaults(am1g_request* rq) {
 }
 
 int am1g_create(int device, am1g_ctx** out) {
-    if (!out) return set_error(nullptr, AM1G_INVALID_INPUT, "null output pointer");
+    if (!out) return am1g_set_error(nullptr, AM1G_INVALID_INPUT, "null output pointer");
     *out = nullptr;
     int count = 0;
     hipError_t e = hipGetDeviceCount(&count);
     if (e != hipSuccess || count == 0)
-        return set_error(nullptr, AM1G_DEVICE_ERROR, "no HIP device available (%s)", hipGetErrorString(e));
+        return am1g_set_error(nullptr, AM1G_DEVICE_ERROR, "no HIP device available (%s)", hipGetErrorString(e));
     if (device < 0 || device >= count)
-        return set_error(nullptr, AM1G_INVALID_INPUT, "device %d out of range (%d devices)", device, count);
+        return am1g_set_error(nullptr, AM1G_INVALID_INPUT, "device %d out of range (%d devices)", device, count);
     am1g_ctx* ctx = new am1g_ctx();
     ctx->device = device;
     HIP_TRY(ctx, hipSetDevice(device));
@@ -264,6 +216,7 @@ int am1g_create(int device, am1g_ctx** out) {
 
 void am1g_destroy(am1g_ctx* ctx) {
     if (!ctx) return;
+    am1g_dc_release(ctx);
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
@@ -290,8 +243,8 @@ int am1g_element_info(int z, int method, double* out) {
 }
 
 int am1g_scf_batch(am1g_ctx* ctx, int n, const am1g_request* rqs, am1g_result* results) {
-    if (!ctx) return set_error(nullptr, AM1G_INVALID_INPUT, "null context");
-    if (n < 0 || (n > 0 && (!rqs || !results))) return set_error(ctx, AM1G_INVALID_INPUT, "bad batch arguments");
+    if (!ctx) return am1g_set_error(nullptr, AM1G_INVALID_INPUT, "null context");
+    if (n < 0 || (n > 0 && (!rqs || !results))) return am1g_set_error(ctx, AM1G_INVALID_INPUT, "bad batch arguments");
     std::lock_guard<std::mutex> lock(ctx->mu);
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     std::vector<Prepared> prep(n);
@@ -299,13 +252,13 @@ int am1g_scf_batch(am1g_ctx* ctx, int n, const am1g_request* rqs, am1g_result* r
     ok_idx.reserve(n);
     for (int i = 0; i < n; ++i) {
         std::string msg;
-        int st = prepare(ctx, rqs[i], prep[i], msg);
+        int st = am1g_prepare(ctx, rqs[i], prep[i], msg, kMaxBatchAo);
         am1g_result& r = results[i];
         if (st != AM1G_OK) {
             r.status = st;
             r.iterations = 0;
             r.converged = 0;
-            set_error(ctx, st, "%s", msg.c_str());
+            am1g_set_error(ctx, st, "%s", msg.c_str());
             continue;
         }
         ok_idx.push_back(i);
@@ -467,7 +420,7 @@ int am1g_scf_batch(am1g_ctx* ctx, int n, const am1g_request* rqs, am1g_result* r
 }
 
 int am1g_scf(am1g_ctx* ctx, const am1g_request* rq, am1g_result* res) {
-    if (!ctx || !rq || !res) return set_error(ctx, AM1G_INVALID_INPUT, "null argument");
+    if (!ctx || !rq || !res) return am1g_set_error(ctx, AM1G_INVALID_INPUT, "null argument");
     auto w0 = std::chrono::steady_clock::now();
     int st = am1g_scf_batch(ctx, 1, rq, res);
     res->latency_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();

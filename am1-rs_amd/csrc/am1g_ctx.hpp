@@ -1,0 +1,76 @@
+// Internal host-side context of the am1g library (not part of the C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <mutex>
+#include <string>
+
+#include "../../include/am1g.h"
+#include "am1g_common.hpp"
+#include "am1g_launch.hpp"
+
+using namespace am1g;  // internal header of the library
+
+// Grow-only device buffer.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    hipError_t reserve(size_t need) {
+        if (need <= bytes) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        size_t want = std::max(need, (size_t)4096);
+        hipError_t e = hipMalloc(&p, want);
+        if (e == hipSuccess) bytes = want;
+        return e;
+    }
+    template <typename T>
+    T* as() const {
+        return static_cast<T*>(p);
+    }
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+};
+
+struct am1g_ctx {
+    int device = 0;
+    int cus = 256;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0.0;
+    std::mutex mu;
+    std::string err;
+    DevElem host_tab[kMethods * kMaxZ];
+    DevBuf elems, mols, z, xyz, mats, wpool, hist, mo_e, mo_eb, charges, results, queue, order;
+    DevBuf alist, aoff, acls, pair_off, ecore_pair;
+    struct DcBuffers* dc = nullptr;
+};
+
+// Buffers of the divide-and-conquer driver, kept on the context so repeated calls reuse them.
+struct DcBuffers {
+    DevBuf F, P, pdiag, fbuf, cbuf, t0, t1, eps, proj, frac, status, sub_n, sub_ao, sub_mat, ao_list, ao_core, pat,
+        mult, pout, hin, hres, ring, coef, ao_atom, core_sub, partial, scal;
+};
+
+struct Prepared {
+    MolDesc md;
+    int nat;
+    bool uhf;
+};
+
+
+int am1g_set_error(am1g_ctx* ctx, int code, const char* fmt, ...);
+int am1g_prepare(am1g_ctx* ctx, const am1g_request& rq, Prepared& out, std::string& msg, int max_ao);
+void am1g_dc_release(am1g_ctx* ctx);
+
+#define HIP_TRY(ctx, expr)                                                                                    \
+    do {                                                                                                      \
+        hipError_t e_ = (expr);                                                                               \
+        if (e_ != hipSuccess)                                                                                 \
+            return am1g_set_error(ctx, AM1G_DEVICE_ERROR, "HIP error %s at %s", hipGetErrorString(e_), #expr); \
+    } while (0)

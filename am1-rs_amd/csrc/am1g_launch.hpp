@@ -42,3 +42,61 @@ hipError_t launch_pairs(const BatchArgs& a, hipStream_t stream);
 void build_element_tables(DevElem* tab);
 
 }  // namespace am1g
+
+namespace am1g {
+
+// One large system handled by grid-level kernels (divide-and-conquer and the plain path).
+struct SysArgs {
+    MolDesc md;
+    const int* z;        // [nat]
+    const double* xyz;   // [nat][3]
+    const int* alist;    // heavy atoms then hydrogens
+    const int* aoff;     // AO offset per atom
+    const int* acls;     // index inside the heavy / hydrogen list
+    const DevElem* elems;
+    const double* wpool;
+    double* H;           // [nao][nao]
+    double* pdiag;       // [nat][10] packed diagonal density blocks (off-diagonal entries doubled)
+};
+
+// Divide-and-conquer subsystem tables (all device pointers).
+struct DcArgs {
+    int n_sub;
+    int lds_n;
+    const int* sub_n;        // [n_sub] AO count
+    const int64_t* sub_ao;   // [n_sub + 1] offset into ao_list / level arrays
+    const int64_t* sub_mat;  // [n_sub] offset of the subsystem's n*n matrices
+    const int* ao_list;      // global AO index of every subsystem AO (ascending within a subsystem)
+    const unsigned char* ao_core;  // 1 when the AO sits on a core atom
+    double* fbuf;   // per subsystem: gathered Fock block / density block
+    double* cbuf;   // per subsystem: eigenvectors (rows)
+    double* t0;     // per subsystem scratch
+    double* t1;     // per subsystem scratch
+    double* eps;    // [levels]
+    double* proj;   // [levels]
+    double* frac;   // [levels]
+    int warm;
+    double cap;     // 2 restricted, 1 per spin channel
+    double tol_abs; // Jacobi tolerance floor
+    int* status;    // [n_sub] eigensolver status (sweeps, or -1)
+};
+
+hipError_t launch_hcore_diag(const SysArgs& a, hipStream_t st);
+hipError_t launch_fock(const SysArgs& a, const double* Pt, const double* Ps, double scale, double* F, hipStream_t st);
+hipError_t launch_dot3(int64_t n, const double* a, const double* b, const double* c, double* partial, double* out,
+                       hipStream_t st);
+hipError_t launch_dc_eigen(const DcArgs& d, const double* F, int nao, hipStream_t st);
+hipError_t launch_fermi(const DcArgs& d, int64_t levels, double n_elec, double kt, double* out, hipStream_t st);
+hipError_t launch_dc_density(const DcArgs& d, hipStream_t st);
+hipError_t launch_dc_gather(const DcArgs& d, int64_t npat, const int64_t* pat, const int* ao_atom,
+                            const int* core_sub, int nao, double* pout, hipStream_t st);
+hipError_t launch_dc_hist(int64_t npat, const int64_t* pat, const double* mult, const double* P, const double* pout,
+                          double* hin, double* hres, const double* hres_all, const int* ring, int m, int64_t stride,
+                          double* partial, double* out, hipStream_t st);
+hipError_t launch_dc_update(int64_t npat, const int64_t* pat, int nao, const double* coef, const double* hin_all,
+                            const double* hres_all, const int* ring, int m, int64_t stride, double mix,
+                            const double* P_in, const double* pout, double* P, hipStream_t st);
+hipError_t launch_sad(const SysArgs& a, double scale, double* P, hipStream_t st);
+hipError_t launch_sum(int64_t n, const double* x, double* partial, double* out, hipStream_t st);
+
+}  // namespace am1g

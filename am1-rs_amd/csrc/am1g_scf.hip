@@ -1064,3 +1064,506 @@ hipError_t launch_scf_batch(const BatchArgs& a, int precision, int grid, size_t 
     return hipGetLastError();
 }
 }  // namespace am1g
+
+// =============================================================================================
+// Grid-level kernels for one large system: core Hamiltonian diagonal blocks, Fock build and
+// reductions (the plain path and divide-and-conquer), plus the divide-and-conquer stages.
+namespace am1g {
+
+__device__ __forceinline__ BlockRef block_of_g(const SysArgs& a, int x, int y) {
+    const int nh = a.md.n_heavy, nH = a.md.n_h;
+    const int64_t hh = (int64_t)nh * (nh - 1) / 2, hH = (int64_t)nh * nH;
+    const bool hx = a.z[x] != 1, hy = a.z[y] != 1;
+    BlockRef r;
+    const int cx = a.acls[x], cy = a.acls[y];
+    if (hx && hy) {
+        r.w = a.wpool + tri_index(min(cx, cy), max(cx, cy), nh) * 100;
+        r.wcols = 10;
+        r.x_is_bra = cx < cy;
+    } else if (hx != hy) {
+        const int hi = hx ? cx : cy, mi = hx ? cy : cx;
+        r.w = a.wpool + hh * 100 + ((int64_t)hi * nH + mi) * 10;
+        r.wcols = 1;
+        r.x_is_bra = hx;
+    } else {
+        r.w = a.wpool + hh * 100 + hH * 10 + tri_index(min(cx, cy), max(cx, cy), nH);
+        r.wcols = 1;
+        r.x_is_bra = cx < cy;
+    }
+    return r;
+}
+
+__device__ __forceinline__ bool diag_item(const SysArgs& a, int it, int& x, int& k) {
+    const int nh = a.md.n_heavy, nH = a.md.n_h;
+    if (it < nh * 10) {
+        x = a.alist[it / 10];
+        k = it % 10;
+        return true;
+    }
+    if (it < nh * 10 + nH) {
+        x = a.alist[nh + it - nh * 10];
+        k = 0;
+        return true;
+    }
+    return false;
+}
+
+// H_core diagonal blocks: U + sum over partners y (index order) of (-Z_y)(mu nu | s_y s_y).
+__global__ __launch_bounds__(256) void k_hcore_diag(SysArgs a) {
+    int x, k;
+    if (!diag_item(a, blockIdx.x * blockDim.x + threadIdx.x, x, k)) return;
+    const DevElem* el = a.elems + a.md.method * kMaxZ;
+    const int nao = a.md.nao, nat = a.md.nat;
+    const int mu = kPackRow[k], nu = kPackCol[k];
+    double acc = 0.0;
+    for (int y = 0; y < nat; ++y) {
+        if (y == x) continue;
+        BlockRef br = block_of_g(a, x, y);
+        acc += -el[a.z[y]].core * (br.x_is_bra ? br.w[k * br.wcols] : br.w[k]);
+    }
+    const DevElem& ex = el[a.z[x]];
+    const double u = (mu == nu) ? (mu == 0 ? ex.uss : ex.upp) : 0.0;
+    const int o = a.aoff[x];
+    a.H[(size_t)(o + mu) * nao + o + nu] = u + acc;
+    a.H[(size_t)(o + nu) * nao + o + mu] = u + acc;
+}
+
+// Packed diagonal density blocks, off-diagonal entries doubled (the 2 of the symmetric sum).
+__global__ __launch_bounds__(256) void k_pack_diag(SysArgs a, const double* Pt) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= a.md.nat) return;
+    const int nao = a.md.nao, o = a.aoff[x];
+    if (a.z[x] == 1) {
+        a.pdiag[x * 10] = Pt[(size_t)o * nao + o];
+        return;
+    }
+#pragma unroll
+    for (int q = 0; q < 10; ++q) {
+        const int l = kPackRow[q], c = kPackCol[q];
+        a.pdiag[x * 10 + q] = Pt[(size_t)(o + l) * nao + o + c] * (l == c ? 1.0 : 2.0);
+    }
+}
+
+// Fock diagonal blocks: H + one-centre terms + Coulomb from every partner's diagonal block.
+__global__ __launch_bounds__(256) void k_fock_diag(SysArgs a, const double* Pt, const double* Ps, double scale,
+                                                   double* F) {
+    int x, k;
+    if (!diag_item(a, blockIdx.x * blockDim.x + threadIdx.x, x, k)) return;
+    const DevElem* el = a.elems + a.md.method * kMaxZ;
+    const int nao = a.md.nao, nat = a.md.nat;
+    const DevElem& ex = el[a.z[x]];
+    const int ox = a.aoff[x], nx = ex.norb;
+    const int mu = kPackRow[k], nu = kPackCol[k];
+    double acc = a.H[(size_t)(ox + mu) * nao + ox + nu];
+    for (int la = 0; la < nx; ++la)
+        for (int si = 0; si < nx; ++si) {
+            acc += Pt[(size_t)(ox + la) * nao + ox + si] * one_centre(mu, nu, la, si, ex);
+            acc -= scale * Ps[(size_t)(ox + la) * nao + ox + si] * one_centre(mu, la, nu, si, ex);
+        }
+    for (int y = 0; y < nat; ++y) {
+        if (y == x) continue;
+        BlockRef br = block_of_g(a, x, y);
+        const double* pd = a.pdiag + y * 10;
+        if (a.z[y] == 1) {
+            acc += pd[0] * (br.x_is_bra ? br.w[k * br.wcols] : br.w[k]);
+        } else if (br.x_is_bra) {
+#pragma unroll
+            for (int q = 0; q < 10; ++q) acc += pd[q] * br.w[k * br.wcols + q];
+        } else {
+#pragma unroll
+            for (int q = 0; q < 10; ++q) acc += pd[q] * br.w[q * br.wcols + k];
+        }
+    }
+    F[(size_t)(ox + mu) * nao + ox + nu] = acc;
+    F[(size_t)(ox + nu) * nao + ox + mu] = acc;
+}
+
+// Fock off-diagonal blocks: H - K, one atom pair per thread.
+__global__ __launch_bounds__(256) void k_fock_off(SysArgs a, const double* Ps, double scale, double* F) {
+    const int nh = a.md.n_heavy, nH = a.md.n_h, nao = a.md.nao;
+    const int64_t n_hh = (int64_t)nh * (nh - 1) / 2, n_hH = (int64_t)nh * nH, n_HH = (int64_t)nH * (nH - 1) / 2;
+    const int64_t total = n_hh + n_hH + n_HH;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < total; p += (int64_t)gridDim.x * blockDim.x) {
+        if (p < n_hh) {
+            int i, j;
+            tri_decode(p, nh, i, j);
+            const int oa = a.aoff[a.alist[i]], ob = a.aoff[a.alist[j]];
+            const double* w = a.wpool + p * 100;
+            double pab[16], acc[16];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    pab[u * 4 + v] = scale * Ps[(size_t)(oa + u) * nao + ob + v];
+                    acc[u * 4 + v] = 0.0;
+                }
+            for (int kb = 0; kb < 10; ++kb) {
+                const int mu = kPackRow[kb], nu = kPackCol[kb];
+                double row[10];
+#pragma unroll
+                for (int q = 0; q < 10; ++q) row[q] = w[kb * 10 + q];
+#pragma unroll
+                for (int la = 0; la < 4; ++la) {
+                    double xx = 0.0, yy = 0.0;
+#pragma unroll
+                    for (int si = 0; si < 4; ++si) {
+                        const double wv = row[pack4(la, si)];
+                        xx += pab[nu * 4 + si] * wv;
+                        yy += pab[mu * 4 + si] * wv;
+                    }
+                    acc[mu * 4 + la] += xx;
+                    if (mu != nu) acc[nu * 4 + la] += yy;
+                }
+            }
+#pragma unroll
+            for (int mu = 0; mu < 4; ++mu)
+#pragma unroll
+                for (int la = 0; la < 4; ++la) {
+                    F[(size_t)(oa + mu) * nao + ob + la] = a.H[(size_t)(oa + mu) * nao + ob + la] - acc[mu * 4 + la];
+                    F[(size_t)(ob + la) * nao + oa + mu] = a.H[(size_t)(ob + la) * nao + oa + mu] - acc[mu * 4 + la];
+                }
+        } else if (p < n_hh + n_hH) {
+            const int64_t k = p - n_hh;
+            const int oa = a.aoff[a.alist[(int)(k / nH)]], ob = a.aoff[a.alist[nh + (int)(k % nH)]];
+            const double* w = a.wpool + n_hh * 100 + k * 10;
+            double pab[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) pab[u] = scale * Ps[(size_t)(oa + u) * nao + ob];
+#pragma unroll
+            for (int mu = 0; mu < 4; ++mu) {
+                double acc = 0.0;
+#pragma unroll
+                for (int nu = 0; nu < 4; ++nu) acc += pab[nu] * w[pack4(mu, nu)];
+                F[(size_t)(oa + mu) * nao + ob] = a.H[(size_t)(oa + mu) * nao + ob] - acc;
+                F[(size_t)ob * nao + oa + mu] = a.H[(size_t)ob * nao + oa + mu] - acc;
+            }
+        } else {
+            int i, j;
+            tri_decode(p - n_hh - n_hH, nH, i, j);
+            const int oa = a.aoff[a.alist[nh + i]], ob = a.aoff[a.alist[nh + j]];
+            const double acc = scale * Ps[(size_t)oa * nao + ob] * a.wpool[p - n_hh - n_hH + n_hh * 100 + n_hH * 10];
+            F[(size_t)oa * nao + ob] = a.H[(size_t)oa * nao + ob] - acc;
+            F[(size_t)ob * nao + oa] = a.H[(size_t)ob * nao + oa] - acc;
+        }
+    }
+}
+
+// Deterministic two-stage sums: fixed-size grid of block partials, then one block in order.
+constexpr int kRedBlocks = 1024;
+
+__global__ __launch_bounds__(256) void k_dot3_partial(int64_t n, const double* a, const double* b, const double* c,
+                                                      double* partial) {
+    __shared__ double sh[4];
+    double v = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)kRedBlocks * 256)
+        v += a[i] * (b ? (c ? b[i] + c[i] : b[i]) : 1.0);
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+
+__global__ __launch_bounds__(256) void k_final_sum(int nparts, const double* partial, double* out) {
+    __shared__ double sh[4];
+    double v = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += 256) v += partial[i];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) out[0] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+
+// SAD guess (scf.rs:475-494) scaled by `scale` into a zeroed density matrix.
+__global__ __launch_bounds__(256) void k_sad(SysArgs a, double scale, double* P) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= a.md.nat) return;
+    const DevElem& e = a.elems[a.md.method * kMaxZ + a.z[x]];
+    const int nao = a.md.nao, o = a.aoff[x];
+    const double nsv = fmin(e.core, 2.0), npv = fmax(e.core - nsv, 0.0);
+    P[(size_t)o * nao + o] = nsv * scale;
+    if (e.norb == 4)
+        for (int k = 1; k < 4; ++k) P[(size_t)(o + k) * nao + o + k] = npv / 3.0 * scale;
+}
+
+// ---------------------------------------------------------------- divide and conquer stages
+// One workgroup per subsystem: gather its Fock block, diagonalize it (warm-started from the
+// previous iteration's subsystem orbitals), sort, and project every level on the core AOs
+// (divide_conquer.rs:588-618).
+__global__ __launch_bounds__(kWg) void k_dc_eigen(DcArgs d, const double* F, int nao) {
+    __shared__ WgScratch s;
+    const int sub = blockIdx.x;
+    const int n = d.sub_n[sub];
+    const int t = threadIdx.x;
+    const int* ao = d.ao_list + d.sub_ao[sub];
+    double* Fs = d.fbuf + d.sub_mat[sub];
+    double* Ct = d.cbuf + d.sub_mat[sub];
+    for (int idx = t; idx < n * n; idx += kWg) {
+        const int i = idx / n, j = idx - i * n;
+        Fs[idx] = F[(size_t)ao[i] * nao + ao[j]];
+    }
+    __syncthreads();
+    const int sw = mol_eigen<double>(n, Fs, Ct, d.t0 + d.sub_mat[sub], d.t1 + d.sub_mat[sub], lds_dyn, d.lds_n,
+                                     d.warm != 0, d.eps + d.sub_ao[sub], s, d.tol_abs);
+    const unsigned char* core = d.ao_core + d.sub_ao[sub];
+    for (int i = t / 16; i < n; i += kWg / 16) {
+        double acc = 0.0;
+        for (int p = t % 16; p < n; p += 16) {
+            const double c = Ct[(size_t)i * n + p];
+            if (core[p]) acc += c * c;
+        }
+        for (int off = 8; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 16);
+        if (t % 16 == 0) d.proj[d.sub_ao[sub] + i] = acc;
+    }
+    if (t == 0) d.status[sub] = sw;
+}
+
+__device__ __forceinline__ double fermi_dirac(double x) {  // fermi.rs:110-119
+    if (x > 40.0) return 0.0;
+    if (x < -40.0) return 1.0;
+    return 1.0 / (1.0 + exp(x));
+}
+
+// Common chemical potential by bisection over every subsystem level (fermi.rs:208-269):
+// bracket [min - 50 kT - 1, max + 50 kT + 1], at most 200 halvings, stop at |N(mu) - N| < 1e-13.
+// out: [0] mu, [1] T*S, [2] electrons at mu.
+__global__ __launch_bounds__(kWg) void k_fermi(DcArgs d, int64_t L, double n_elec, double kt, double* out) {
+    __shared__ WgScratch s;
+    const int t = threadIdx.x;
+    double lo = INFINITY, hi = -INFINITY;
+    for (int64_t i = t; i < L; i += kWg) {
+        lo = fmin(lo, d.eps[i]);
+        hi = fmax(hi, d.eps[i]);
+    }
+    lo = -wg_max(-lo, s) - 50.0 * kt - 1.0;
+    hi = wg_max(hi, s) + 50.0 * kt + 1.0;
+    double mu = 0.5 * (lo + hi), cnt = 0.0;
+    for (int it = 0; it < 200; ++it) {
+        mu = 0.5 * (lo + hi);
+        double c = 0.0;
+        for (int64_t i = t; i < L; i += kWg) c += d.cap * d.proj[i] * fermi_dirac((d.eps[i] - mu) / kt);
+        c = wg_sum(c, s);
+        cnt = c;
+        if (fabs(c - n_elec) < 1.0e-13) break;
+        if (c < n_elec) lo = mu;
+        else hi = mu;
+    }
+    double sk = 0.0;
+    for (int64_t i = t; i < L; i += kWg) {
+        const double f = fermi_dirac((d.eps[i] - mu) / kt);
+        d.frac[i] = f;
+        if (f > 1.0e-14 && f < 1.0 - 1.0e-14) sk -= d.cap * d.proj[i] * (f * log(f) + (1.0 - f) * log(1.0 - f));
+    }
+    sk = wg_sum(sk, s);
+    if (t == 0) {
+        out[0] = mu;
+        out[1] = kt * sk;
+        out[2] = cnt;
+    }
+}
+
+// Subsystem density blocks D_s = sum_i cap f_i c_i c_i^T over levels with f_i > 1e-14
+// (divide_conquer.rs:686-713), written over the gathered Fock block.
+__global__ __launch_bounds__(kWg) void k_dc_density(DcArgs d) {
+    const int sub = blockIdx.x;
+    const int n = d.sub_n[sub];
+    const int t = threadIdx.x;
+    const double* Ct = d.cbuf + d.sub_mat[sub];
+    const double* f = d.frac + d.sub_ao[sub];
+    double* T0 = d.t0 + d.sub_mat[sub];
+    for (int idx = t; idx < n * n; idx += kWg) {
+        const int i = idx / n;
+        T0[idx] = f[i] > 1.0e-14 ? Ct[idx] * sqrt(d.cap * f[i]) : 0.0;
+    }
+    __syncthreads();
+    wg_gemm_any<double, true, false>(n, n, n, 1.0, T0, n, T0, n, d.fbuf + d.sub_mat[sub], n, lds_dyn, kLdsDoubles);
+}
+
+__device__ __forceinline__ int local_ao(const DcArgs& d, int sub, int g) {
+    const int* ao = d.ao_list + d.sub_ao[sub];
+    int lo = 0, hi = d.sub_n[sub] - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (ao[mid] < g) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Yang assembly as a gather over the retained pattern: an AO pair (r, c) on atoms (a, b) within
+// the buffer radius gets weight 1/2 from the subsystem whose core holds a and 1/2 from the one
+// whose core holds b (1 when that is one subsystem); every other subsystem weighs it 0.
+__global__ __launch_bounds__(256) void k_dc_gather(DcArgs d, int64_t npat, const int64_t* pat, const int* ao_atom,
+                                                   const int* core_sub, int nao, double* pout) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= npat) return;
+    const int r = (int)(pat[k] / nao), c = (int)(pat[k] % nao);
+    const int s1 = core_sub[ao_atom[r]], s2 = core_sub[ao_atom[c]];
+    const int n1 = d.sub_n[s1];
+    const double v1 = d.fbuf[d.sub_mat[s1] + (int64_t)local_ao(d, s1, r) * n1 + local_ao(d, s1, c)];
+    if (s1 == s2) {
+        pout[k] = v1;
+        return;
+    }
+    const int n2 = d.sub_n[s2];
+    const double v2 = d.fbuf[d.sub_mat[s2] + (int64_t)local_ao(d, s2, r) * n2 + local_ao(d, s2, c)];
+    pout[k] = 0.5 * v1 + 0.5 * v2;
+}
+
+// Push one density-DIIS entry (input and residual on the pattern) and form, in the same pass,
+// the multiplicity-weighted sum of squared residuals and the new residual's dots with every
+// history entry (divide_conquer.rs:800-1019). Block partials: [grid][1 + m].
+__global__ __launch_bounds__(256) void k_dc_hist(int64_t npat, const int64_t* pat, const double* mult, const double* P,
+                                                 const double* pout, double* hin, double* hres, const double* hres_all,
+                                                 const int* ring, int m, int64_t stride, double* partial) {
+    __shared__ double sh[4][kDiisDepth + 1];
+    double v[kDiisDepth + 1];
+#pragma unroll
+    for (int q = 0; q <= kDiisDepth; ++q) v[q] = 0.0;
+    for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < npat; k += (int64_t)kRedBlocks * 256) {
+        const double pin = P[pat[k]];
+        const double r = pout[k] - pin;
+        if (hin) {
+            hin[k] = pin;
+            hres[k] = r;
+        }
+        const double w = mult[k];
+        v[0] += w * r * r;
+#pragma unroll
+        for (int j = 0; j < kDiisDepth; ++j)
+            if (j < m) v[1 + j] += w * r * (ring[j] < 0 ? r : hres_all[(int64_t)ring[j] * stride + k]);
+    }
+#pragma unroll
+    for (int q = 0; q <= kDiisDepth; ++q) {
+        double x = v[q];
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+        if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6][q] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x <= kDiisDepth)
+        partial[(int64_t)blockIdx.x * (kDiisDepth + 1) + threadIdx.x] =
+            (sh[0][threadIdx.x] + sh[1][threadIdx.x]) + (sh[2][threadIdx.x] + sh[3][threadIdx.x]);
+}
+
+__global__ __launch_bounds__(256) void k_final_sum_many(int nparts, int width, const double* partial, double* out) {
+    __shared__ double sh[4];
+    for (int q = 0; q < width; ++q) {
+        double v = 0.0;
+        for (int i = threadIdx.x; i < nparts; i += 256) v += partial[(int64_t)i * width + q];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        __syncthreads();
+        if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+        __syncthreads();
+        if (threadIdx.x == 0) out[q] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+    }
+}
+
+// Next input density on the pattern (both triangles): sum_i c_i (in_i + res_i) when coef is given,
+// linear mixing in + mix (out - in) when mix >= 0, or the output density itself when mix < 0.
+__global__ __launch_bounds__(256) void k_dc_update(int64_t npat, const int64_t* pat, int nao, const double* coef,
+                                                   const double* hin_all, const double* hres_all, const int* ring,
+                                                   int m, int64_t stride, double mix, const double* pout, double* P) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= npat) return;
+    double v;
+    if (coef) {
+        v = 0.0;
+        for (int i = 0; i < m; ++i) {
+            const int64_t o = (int64_t)ring[i] * stride + k;
+            v += coef[i] * (hin_all[o] + hres_all[o]);
+        }
+    } else if (mix >= 0.0) {
+        const double pin = P[pat[k]];
+        v = pin + mix * (pout[k] - pin);
+    } else {
+        v = pout[k];
+    }
+    const int r = (int)(pat[k] / nao), c = (int)(pat[k] % nao);
+    P[(size_t)r * nao + c] = v;
+    P[(size_t)c * nao + r] = v;
+}
+
+// ---------------------------------------------------------------- launchers
+static inline unsigned blocks_for(int64_t n, int per = 256) { return (unsigned)((n + per - 1) / per); }
+
+hipError_t launch_hcore_diag(const SysArgs& a, hipStream_t st) {
+    const int items = a.md.n_heavy * 10 + a.md.n_h;
+    if (items > 0) hipLaunchKernelGGL(k_hcore_diag, dim3(blocks_for(items)), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_fock(const SysArgs& a, const double* Pt, const double* Ps, double scale, double* F, hipStream_t st) {
+    hipLaunchKernelGGL(k_pack_diag, dim3(blocks_for(a.md.nat)), dim3(256), 0, st, a, Pt);
+    const int items = a.md.n_heavy * 10 + a.md.n_h;
+    hipLaunchKernelGGL(k_fock_diag, dim3(blocks_for(items)), dim3(256), 0, st, a, Pt, Ps, scale, F);
+    const int64_t np = (int64_t)a.md.nat * (a.md.nat - 1) / 2;
+    if (np > 0) {
+        int64_t b = blocks_for(np);
+        if (b > 65536) b = 65536;
+        hipLaunchKernelGGL(k_fock_off, dim3((unsigned)b), dim3(256), 0, st, a, Ps, scale, F);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_dot3(int64_t n, const double* a, const double* b, const double* c, double* partial, double* out,
+                       hipStream_t st) {
+    hipLaunchKernelGGL(k_dot3_partial, dim3(kRedBlocks), dim3(256), 0, st, n, a, b, c, partial);
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(256), 0, st, kRedBlocks, partial, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_sum(int64_t n, const double* x, double* partial, double* out, hipStream_t st) {
+    return launch_dot3(n, x, nullptr, nullptr, partial, out, st);
+}
+
+hipError_t launch_sad(const SysArgs& a, double scale, double* P, hipStream_t st) {
+    hipLaunchKernelGGL(k_sad, dim3(blocks_for(a.md.nat)), dim3(256), 0, st, a, scale, P);
+    return hipGetLastError();
+}
+
+hipError_t launch_dc_eigen(const DcArgs& d, const double* F, int nao, hipStream_t st) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_dc_eigen),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_dc_eigen, dim3(d.n_sub), dim3(kWg), kLdsBytes, st, d, F, nao);
+    return hipGetLastError();
+}
+
+hipError_t launch_fermi(const DcArgs& d, int64_t levels, double n_elec, double kt, double* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_fermi, dim3(1), dim3(kWg), 0, st, d, levels, n_elec, kt, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_dc_density(const DcArgs& d, hipStream_t st) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_dc_density),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_dc_density, dim3(d.n_sub), dim3(kWg), kLdsBytes, st, d);
+    return hipGetLastError();
+}
+
+hipError_t launch_dc_gather(const DcArgs& d, int64_t npat, const int64_t* pat, const int* ao_atom, const int* core_sub,
+                            int nao, double* pout, hipStream_t st) {
+    if (npat > 0) hipLaunchKernelGGL(k_dc_gather, dim3(blocks_for(npat)), dim3(256), 0, st, d, npat, pat, ao_atom, core_sub, nao, pout);
+    return hipGetLastError();
+}
+
+hipError_t launch_dc_hist(int64_t npat, const int64_t* pat, const double* mult, const double* P, const double* pout,
+                          double* hin, double* hres, const double* hres_all, const int* ring, int m, int64_t stride,
+                          double* partial, double* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_dc_hist, dim3(kRedBlocks), dim3(256), 0, st, npat, pat, mult, P, pout, hin, hres, hres_all,
+                       ring, m, stride, partial);
+    hipLaunchKernelGGL(k_final_sum_many, dim3(1), dim3(256), 0, st, kRedBlocks, kDiisDepth + 1, partial, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_dc_update(int64_t npat, const int64_t* pat, int nao, const double* coef, const double* hin_all,
+                            const double* hres_all, const int* ring, int m, int64_t stride, double mix,
+                            const double* P_in, const double* pout, double* P, hipStream_t st) {
+    (void)P_in;
+    if (npat > 0)
+        hipLaunchKernelGGL(k_dc_update, dim3(blocks_for(npat)), dim3(256), 0, st, npat, pat, nao, coef, hin_all,
+                           hres_all, ring, m, stride, mix, pout, P);
+    return hipGetLastError();
+}
+
+}  // namespace am1g

@@ -104,6 +104,37 @@ int am1g_scf(am1g_ctx* ctx, const am1g_request* rq, am1g_result* res);
  * the launch ran; per-molecule outcomes are in results[i].status. */
 int am1g_scf_batch(am1g_ctx* ctx, int n, const am1g_request* rqs, am1g_result* results);
 
+/* Divide-and-conquer controls: DcOptions (src/divide_conquer.rs:66-119). */
+typedef struct {
+    int core_size;            /* target atoms per core when core_of_atom is NULL (default 12) */
+    double buffer_radius;     /* Bohr (default 11.0) */
+    double kt;                /* Fermi-Dirac electronic temperature in eV (default 0.05); <= 0 is aufbau */
+    int max_scf;              /* default 300 */
+    double e_tol;             /* default 1e-7 eV */
+    double p_tol;             /* default 1e-6 */
+    double mixing;            /* linear mixing fraction when DIIS declines (default 0.4) */
+    const int* core_of_atom;  /* optional [n_atoms]: caller-chosen core index of every atom */
+} am1g_dc_options;
+
+/* Divide-and-conquer result: DcResult (src/divide_conquer.rs:167-220). */
+typedef struct {
+    int status, iterations, converged, unrestricted;
+    int n_subsystems, largest_subsystem_aos;
+    double electronic_ev, core_ev, total_ev, heat_of_formation_kcal;
+    double fermi_ev, entropy_ev, homo_lumo_gap_ev;
+    double diagonalization_work;     /* sum over subsystems of n^3 */
+    long long retained_pattern;      /* AO pairs of the retained density pattern (upper triangle) */
+    double phase_ms[5];              /* Fock, subsystem eigensolves, Fermi level, assembly, DIIS */
+    double* charges;                 /* [n_atoms] */
+    double* density;                 /* [nao*nao] optional */
+} am1g_dc_result;
+
+void am1g_dc_defaults(am1g_dc_options* op);
+
+/* Divide-and-conquer SCF on the GPU: run_divide_conquer (src/divide_conquer.rs:406-572), closed
+ * shell. All subsystem eigensolves of an iteration run as one batched launch. */
+int am1g_dc(am1g_ctx* ctx, const am1g_request* rq, const am1g_dc_options* op, am1g_dc_result* res);
+
 /* Text of the last error on this context (the Display text of the matching Am1Error). */
 const char* am1g_last_error(const am1g_ctx* ctx);
 
