@@ -2137,6 +2137,16 @@ int oracle_dc(const OracleRequest* rq, const OracleDcOptions* op, OracleDcResult
 }
 
 // Yang-weight sum of every atom-pair block over the subsystems (divide_conquer.rs:381-396).
+// fermi.rs:124-269 on a caller's level list: levels e[n] with weights w[n] (capacity times
+// projection), nel electrons, kt in eV (<= 0: aufbau). Returns 0, or 1 when the electrons do not fit.
+int oracle_fill(int n, const double* e, const double* w, double nel, double kt, double* frac, double* mu,
+                double* ts) {
+    std::vector<double> ev(e, e + n), wv(w, w + n), f;
+    if (!fill_levels(ev, wv, nel, kt, f, *mu, *ts)) return 1;
+    std::copy(f.begin(), f.end(), frac);
+    return 0;
+}
+
 int oracle_dc_weight_sum(const OracleRequest* rq, const OracleDcOptions* op, double* out) {
     System s;
     if (!build_system(s, rq->n_atoms, rq->z, rq->xyz_bohr, rq->method)) return s.status;

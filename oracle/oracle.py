@@ -320,6 +320,23 @@ def eigen(a):
     return w, v
 
 
+def fill(energies, weights, n_elec, kt):
+    """``fermi::fill`` (fermi.rs:124-269): fractions, chemical potential and T*S for levels with
+    the given weights; ``kt <= 0`` is aufbau. Raises ``ValueError`` when the electrons do not fit."""
+    e = np.ascontiguousarray(energies, dtype=np.float64)
+    w = np.ascontiguousarray(weights, dtype=np.float64)
+    f = np.zeros(len(e))
+    mu = ctypes.c_double()
+    ts = ctypes.c_double()
+    L = lib()
+    L.oracle_fill.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                              ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_double),
+                              ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+    if L.oracle_fill(len(e), _dp(e), _dp(w), float(n_elec), float(kt), _dp(f), ctypes.byref(mu), ctypes.byref(ts)):
+        raise ValueError("cannot fill: the electron count does not fit the levels")
+    return f, mu.value, ts.value
+
+
 def diis_solve(B):
     B = np.ascontiguousarray(B, dtype=np.float64)
     m = B.shape[0]

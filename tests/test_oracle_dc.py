@@ -61,3 +61,77 @@ def test_charge_is_conserved_and_buffer_converges_monotonically():
         assert abs(r["charges"].sum()) < 1e-8
         errs.append(abs(r["total_ev"] - full["total_ev"]))
     assert errs[0] >= errs[1] >= errs[2]
+
+
+# ------------------------------------------------------------------------------ filling (fermi.rs tests)
+def _restricted(levels):
+    return np.array(levels, dtype=float), np.full(len(levels), 2.0)
+
+
+def test_aufbau_on_a_gapped_spectrum_is_integral():
+    # fermi.rs:295-305
+    e, w = _restricted([-10.0, -8.0, -6.0, 2.0, 4.0])
+    f, mu, ts = O.fill(e, w, 6.0, 0.0)
+    assert list(f) == [1.0, 1.0, 1.0, 0.0, 0.0]
+    assert abs((w * f).sum() - 6.0) < 1e-14
+    assert abs((w * f * e).sum() - 2.0 * (-10.0 - 8.0 - 6.0)) < 1e-12
+    assert -6.0 < mu < 2.0
+    assert ts == 0.0
+
+
+def test_aufbau_shares_a_partial_shell_over_degenerate_levels():
+    # fermi.rs:307-317
+    e, w = _restricted([-5.0, -5.0, -5.0])
+    f, _, _ = O.fill(e, w, 2.0, 0.0)
+    assert np.abs(f - 1.0 / 3.0).max() < 1e-12
+
+
+@pytest.mark.parametrize("kt", [0.001, 0.01, 0.1, 0.5])
+def test_fermi_conserves_the_electron_count(kt):
+    # fermi.rs:319-330
+    e, w = _restricted([-10.0, -8.0, -7.9, -0.1, 0.0, 3.0])
+    f, _, _ = O.fill(e, w, 7.0, kt)
+    assert abs((w * f).sum() - 7.0) < 1e-10
+
+
+def test_fermi_approaches_aufbau_as_the_temperature_falls():
+    # fermi.rs:332-371
+    e, w = _restricted([-10.0, -8.0, -6.0, 2.0, 4.0])
+    sharp, _, _ = O.fill(e, w, 6.0, 0.0)
+    prev_diff, prev_ts = np.inf, np.inf
+    for kt in (0.5, 0.1, 0.01, 0.001):
+        f, _, ts = O.fill(e, w, 6.0, kt)
+        diff = np.abs(f - sharp).sum()
+        assert diff <= prev_diff and 0.0 <= ts <= prev_ts
+        prev_diff, prev_ts = diff, ts
+
+
+def test_symmetric_two_level_system_has_the_closed_form_entropy():
+    # two levels at -d and +d holding half their capacity: mu = 0 by symmetry and, with
+    # f = 1/(1+e^{-d/kT}) below and 1-f above, T*S = -2 kT sum_w [f ln f + (1-f) ln(1-f)]
+    d, kt = 0.3, 0.2
+    f, mu, ts = O.fill([-d, d], [2.0, 2.0], 2.0, kt)
+    lo = 1.0 / (1.0 + np.exp(-d / kt))
+    assert abs(mu) < 1e-12
+    assert abs(f[0] - lo) < 1e-12 and abs(f[1] - (1.0 - lo)) < 1e-12
+    want = -kt * 2.0 * 2.0 * (lo * np.log(lo) + (1.0 - lo) * np.log(1.0 - lo))
+    assert abs(ts - want) < 1e-12
+
+
+def test_weights_carry_subsystem_factors():
+    # fermi.rs:391-419: one chemical potential across differently weighted levels
+    f, _, _ = O.fill([-5.0, 1.0, -4.0, -0.5], [1.0, 1.0, 1.0, 1.0], 3.0, 0.01)
+    assert f[0] > 0.99 and f[2] > 0.99 and f[3] > 0.99 and f[1] < 0.01
+
+
+def test_overfilling_is_an_error_and_empty_or_full_fills():
+    # fermi.rs:421-438
+    e, w = _restricted([-1.0, 0.0])
+    with pytest.raises(ValueError):
+        O.fill(e, w, 5.0, 0.0)
+    with pytest.raises(ValueError):
+        O.fill(e, w, -1.0, 0.0)
+    f, _, _ = O.fill(e, w, 0.0, 0.1)
+    assert abs((w * f).sum()) < 1e-10
+    f, _, ts = O.fill(e, w, 4.0, 0.1)
+    assert abs((w * f).sum() - 4.0) < 1e-10 and abs(ts) < 1e-9
