@@ -29,7 +29,9 @@ REFERENCES = {"auto": 0, "rhf": 1, "uhf": 2, "restricted": 1, "unrestricted": 2}
 ACCELERATORS = {"none": 0, "cdiis": 1, "adiis_cdiis": 2, "adiis": 2}
 PRECISIONS = {"fp64": 0, "mixed": 1}
 STATUS_NAMES = {0: "Ok", 1: "InvalidInput", 2: "ElementNotParameterized", 3: "LinearAlgebra",
-                4: "ScfNotConverged", 5: "ScfDiverged", 6: "DeviceError"}
+                4: "ScfNotConverged", 5: "ScfDiverged", 6: "DeviceError", 7: "Pending", 8: "Cancelled"}
+PENDING = 7
+CANCELLED = 8
 PHASES = ["setup", "fock", "commutator+diis", "eigen", "density", "final"]
 SYMBOLS = {"H": 1, "Be": 4, "B": 5, "C": 6, "N": 7, "O": 8, "F": 9, "Al": 13, "Si": 14, "P": 15, "S": 16,
            "Cl": 17, "Zn": 30, "Ge": 32, "As": 33, "Se": 34, "Br": 35, "Sb": 51, "Te": 52, "I": 53, "Hg": 80}
@@ -87,6 +89,29 @@ class Result(ctypes.Structure):
         ("mo_energies_beta", ctypes.POINTER(ctypes.c_double)),
         ("density", ctypes.POINTER(ctypes.c_double)),
         ("mo_coeff", ctypes.POINTER(ctypes.c_double)),
+        ("mo_coeff_beta", ctypes.POINTER(ctypes.c_double)),
+        ("spin_density", ctypes.POINTER(ctypes.c_double)),
+    ]
+
+
+class ServeOptions(ctypes.Structure):
+    _fields_ = [
+        ("lanes", ctypes.c_int),
+        ("max_batch", ctypes.c_int),
+        ("memory_budget_mb", ctypes.c_double),
+    ]
+
+
+class ServeStats(ctypes.Structure):
+    _fields_ = [
+        ("submitted", ctypes.c_longlong),
+        ("completed", ctypes.c_longlong),
+        ("cancelled", ctypes.c_longlong),
+        ("queued", ctypes.c_longlong),
+        ("running", ctypes.c_longlong),
+        ("launches", ctypes.c_longlong),
+        ("admitted_mb", ctypes.c_double),
+        ("peak_admitted_mb", ctypes.c_double),
     ]
 
 
@@ -130,7 +155,9 @@ DC_PHASES = ["fock", "eigen", "fermi", "assembly", "diis"]
 
 # Every symbol include/am1g.h declares (checked by the CPU test-suite against the built library).
 EXPORTS = ["am1g_request_defaults", "am1g_create", "am1g_destroy", "am1g_scf", "am1g_scf_batch",
-           "am1g_dc_defaults", "am1g_dc", "am1g_last_error", "am1g_element_info", "am1g_last_kernel_ms"]
+           "am1g_dc_defaults", "am1g_dc", "am1g_serve_defaults", "am1g_serve_create", "am1g_serve_destroy",
+           "am1g_serve_submit", "am1g_serve_poll", "am1g_serve_cancel", "am1g_serve_get_stats",
+           "am1g_serve_last_error", "am1g_last_error", "am1g_element_info", "am1g_last_kernel_ms"]
 
 _lib = None
 
@@ -157,6 +184,15 @@ def load():
     L.am1g_dc_defaults.argtypes = [ctypes.POINTER(DcOptions)]
     L.am1g_dc.argtypes = [ctypes.c_void_p, ctypes.POINTER(Request), ctypes.POINTER(DcOptions),
                           ctypes.POINTER(DcResult)]
+    L.am1g_serve_defaults.argtypes = [ctypes.POINTER(ServeOptions)]
+    L.am1g_serve_create.argtypes = [ctypes.c_int, ctypes.POINTER(ServeOptions), ctypes.POINTER(ctypes.c_void_p)]
+    L.am1g_serve_destroy.argtypes = [ctypes.c_void_p]
+    L.am1g_serve_submit.argtypes = [ctypes.c_void_p, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_longlong)]
+    L.am1g_serve_poll.argtypes = [ctypes.c_void_p, ctypes.c_longlong, ctypes.c_double, ctypes.POINTER(Result)]
+    L.am1g_serve_cancel.argtypes = [ctypes.c_void_p, ctypes.c_longlong]
+    L.am1g_serve_get_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(ServeStats)]
+    L.am1g_serve_last_error.argtypes = [ctypes.c_void_p]
+    L.am1g_serve_last_error.restype = ctypes.c_char_p
     L.am1g_last_error.argtypes = [ctypes.c_void_p]
     L.am1g_last_error.restype = ctypes.c_char_p
     L.am1g_request_defaults.argtypes = [ctypes.POINTER(Request)]
@@ -248,7 +284,12 @@ class Engine:
             rs.mo_energies_beta = mob.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
             rs.density = dens.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
             rs.mo_coeff = coeff.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
-            arrays.update(mo=mo, mob=mob, dens=dens, coeff=coeff)
+            coeff_b = np.zeros(nao_max * nao_max)
+            spin = np.zeros(nao_max * nao_max)
+            keep.extend([coeff_b, spin])
+            rs.mo_coeff_beta = coeff_b.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+            rs.spin_density = spin.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+            arrays.update(mo=mo, mob=mob, dens=dens, coeff=coeff, coeff_b=coeff_b, spin=spin)
         return arrays
 
     @staticmethod
@@ -285,6 +326,8 @@ class Engine:
             d["mo_energies_beta"] = arrays["mob"][:n].copy() if rs.unrestricted else None
             d["density"] = arrays["dens"][: n * n].reshape(n, n).copy()
             d["mo_coeff"] = arrays["coeff"][: n * n].reshape(n, n).copy()
+            d["mo_coeff_beta"] = arrays["coeff_b"][: n * n].reshape(n, n).copy() if rs.unrestricted else None
+            d["spin_density"] = arrays["spin"][: n * n].reshape(n, n).copy() if rs.unrestricted else None
         return d
 
     def scf(self, z, xyz_bohr, full: bool = True, **kw) -> dict:
@@ -357,6 +400,85 @@ class Engine:
         if rs.status not in (0, 4):
             out["message"] = self.last_error()
         return out
+
+
+class Server:
+    """Continuous-batching server (``am1g_serve_*``): ``submit`` queues a request and returns a
+    ticket; ``poll`` returns None while it runs, else the result dict (``latency_ms`` = submit to
+    completion)."""
+
+    def __init__(self, device: int = 0, lanes: int | None = None, max_batch: int | None = None,
+                 memory_budget_mb: float | None = None):
+        self._L = load()
+        op = ServeOptions()
+        self._L.am1g_serve_defaults(ctypes.byref(op))
+        if lanes is not None:
+            op.lanes = lanes
+        if max_batch is not None:
+            op.max_batch = max_batch
+        if memory_budget_mb is not None:
+            op.memory_budget_mb = memory_budget_mb
+        self._sv = ctypes.c_void_p()
+        st = self._L.am1g_serve_create(int(device), ctypes.byref(op), ctypes.byref(self._sv))
+        if st != 0:
+            msg = self._L.am1g_last_error(None)
+            raise RuntimeError(f"am1g_serve_create failed ({STATUS_NAMES.get(st, st)}): "
+                               f"{msg.decode() if msg else ''}")
+        self._req = Engine.__new__(Engine)
+        self._req._L = self._L
+        self._n_atoms = {}
+
+    def close(self):
+        if self._sv:
+            self._L.am1g_serve_destroy(self._sv)
+            self._sv = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, z, xyz_bohr, **kw) -> int:
+        keep = []
+        rq = self._req._request(z, xyz_bohr, keep, **kw)
+        t = ctypes.c_longlong()
+        st = self._L.am1g_serve_submit(self._sv, ctypes.byref(rq), ctypes.byref(t))
+        if st != 0:
+            raise ValueError(self._L.am1g_serve_last_error(self._sv).decode())
+        self._n_atoms[t.value] = rq.n_atoms
+        return t.value
+
+    def poll(self, ticket: int, wait_ms: float = 0.0):
+        keep = []
+        rs = Result()
+        n = self._n_atoms[ticket]
+        q = np.zeros(n)
+        mo = np.zeros(4 * n)
+        mob = np.zeros(4 * n)
+        keep.extend([q, mo, mob])
+        rs.charges = q.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        rs.mo_energies = mo.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        rs.mo_energies_beta = mob.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        st = self._L.am1g_serve_poll(self._sv, int(ticket), float(wait_ms), ctypes.byref(rs))
+        if st == PENDING:
+            return None
+        del self._n_atoms[ticket]
+        d = Engine._to_dict(rs, {"charges": q}, False)
+        d["status"] = st
+        d["status_name"] = STATUS_NAMES.get(st, str(st))
+        if st in (0, 4):
+            d["mo_energies"] = mo[: rs.nao].copy()
+            d["mo_energies_beta"] = mob[: rs.nao].copy() if rs.unrestricted else None
+        return d
+
+    def cancel(self, ticket: int) -> None:
+        self._L.am1g_serve_cancel(self._sv, int(ticket))
+
+    def stats(self) -> dict:
+        s = ServeStats()
+        self._L.am1g_serve_get_stats(self._sv, ctypes.byref(s))
+        return {k: getattr(s, k) for k, _ in ServeStats._fields_}
 
 
 _default_engine = None

@@ -408,12 +408,23 @@ int am1g_scf_batch(am1g_ctx* ctx, int n, const am1g_request* rqs, am1g_result* r
             HIP_TRY(ctx, hipMemcpy(r.density, ctx->mats.as<double>() + md.mat_off + slot * nn, nn * sizeof(double),
                                    hipMemcpyDeviceToHost));
         }
-        if (r.mo_coeff) {
+        const double* M = ctx->mats.as<double>() + md.mat_off;
+        // eigenvectors are stored transposed (row k = MO k); the result holds MOs as columns
+        auto columns = [&](int slot, double* out) -> hipError_t {
             mat.resize(nn);
-            HIP_TRY(ctx, hipMemcpy(mat.data(), ctx->mats.as<double>() + md.mat_off + kC * nn, nn * sizeof(double),
-                                   hipMemcpyDeviceToHost));
+            hipError_t e = hipMemcpy(mat.data(), M + slot * nn, nn * sizeof(double), hipMemcpyDeviceToHost);
+            if (e != hipSuccess) return e;
             for (int mu = 0; mu < md.nao; ++mu)
-                for (int k = 0; k < md.nao; ++k) r.mo_coeff[(size_t)mu * md.nao + k] = mat[(size_t)k * md.nao + mu];
+                for (int k = 0; k < md.nao; ++k) out[(size_t)mu * md.nao + k] = mat[(size_t)k * md.nao + mu];
+            return hipSuccess;
+        };
+        if (r.mo_coeff) HIP_TRY(ctx, columns(kC, r.mo_coeff));
+        if (md.uhf && r.mo_coeff_beta) HIP_TRY(ctx, columns(kCb, r.mo_coeff_beta));
+        if (md.uhf && r.spin_density) {
+            std::vector<double> pb(nn);
+            HIP_TRY(ctx, hipMemcpy(r.spin_density, M + kP * nn, nn * sizeof(double), hipMemcpyDeviceToHost));
+            HIP_TRY(ctx, hipMemcpy(pb.data(), M + kPb * nn, nn * sizeof(double), hipMemcpyDeviceToHost));
+            for (size_t k = 0; k < nn; ++k) r.spin_density[k] -= pb[k];
         }
     }
     return AM1G_OK;

@@ -517,7 +517,7 @@ __device__ __noinline__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolS
         double e_elec = 0.5 * wg_sum(pe, s);
         phase_mark(pc, 1);
         // error = FP - (FP)^T; F and P are symmetric so FP = F^T P
-        bool low = (PREC == 1) && (residual > 1.0e-4);
+        bool low = (PREC == 1) && md.precision == 1 && (residual > 1.0e-4);
         if (low) {
             wg_gemm_any<float, true, false>(nao, nao, nao, 1.0, F, nao, P, nao, T0, nao, lds, kLdsDoubles);
             ++fp32_iters;
@@ -813,7 +813,7 @@ __device__ __noinline__ void scf_uhf(const MolDesc& md, const BatchArgs& g, MolS
             pe += Pt[idx] * H[idx] + Ps[0][idx] * Fs[0][idx] + Ps[1][idx] * Fs[1][idx];
         const double e_elec = 0.5 * wg_sum(pe, s);
         phase_mark(pc, 1);
-        const bool low = (PREC == 1) && (residual > 1.0e-4);
+        const bool low = (PREC == 1) && md.precision == 1 && (residual > 1.0e-4);
         if (low) ++fp32_iters;
         int slot = 0;
         if (diis) slot = ms.hcount < kDiisDepth ? (ms.head + ms.hcount) % kDiisDepth : ms.head;

@@ -1,0 +1,307 @@
+// Continuous-batching SCF server (am1g_serve_*).
+//
+// Each lane is one host thread that owns one am1g context (its own HIP stream and device
+// pools). A free lane takes every queued request of the head request's precision that fits the
+// device-memory budget, up to max_batch, and runs them as one persistent batched launch; other
+// lanes keep admitting newly arrived requests meanwhile, so launches overlap on the device.
+// Requests are copied at submission, so callers may free their buffers at once.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "am1g_ctx.hpp"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+enum class JobState { kQueued, kRunning, kDone };
+
+struct Job {
+    am1g_request rq;
+    std::vector<uint8_t> z;
+    std::vector<double> xyz;
+    size_t bytes = 0;
+    JobState state = JobState::kQueued;
+    bool cancelled = false;
+    Clock::time_point submitted, done;
+    am1g_result res;
+    std::vector<double> charges, mo, mob;
+};
+
+// Device bytes one request holds inside a batched launch: dense matrices, pair pool, DIIS
+// history and per-AO / per-atom arrays (the pool layout of am1g_scf_batch).
+size_t job_bytes(const am1g_request& rq) {
+    int64_t nao = 0, nh = 0, nH = 0;
+    for (int a = 0; a < rq.n_atoms; ++a) {
+        if (rq.z[a] == 1) {
+            ++nao;
+            ++nH;
+        } else {
+            nao += 4;
+            ++nh;
+        }
+    }
+    const bool uhf = rq.reference == AM1G_REF_UHF || rq.multiplicity > 1;
+    const int64_t doubles = (int64_t)(uhf ? kMatSlotsUhf : kMatSlotsRhf) * nao * nao + w_pool_size((int)nh, (int)nH) +
+                            hist_size((int)nao, uhf) + 2 * nao + (int64_t)rq.n_atoms * (rq.n_atoms + 8);
+    return (size_t)doubles * sizeof(double) + sizeof(MolDesc) + sizeof(MolResult);
+}
+
+}  // namespace
+
+struct am1g_server {
+    am1g_serve_options op;
+    std::vector<am1g_ctx*> ctxs;
+    std::vector<std::thread> lanes;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<long long> queue;
+    std::unordered_map<long long, Job> jobs;
+    long long next_ticket = 1;
+    bool stop = false;
+    size_t admitted = 0, peak = 0;
+    am1g_serve_stats st{};
+    std::string err;
+
+    void lane_loop(int lane) {
+        am1g_ctx* ctx = ctxs[lane];
+        const size_t budget = (size_t)(op.memory_budget_mb * 1048576.0);
+        std::unique_lock<std::mutex> lk(mu);
+        for (;;) {
+            cv_work.wait(lk, [&] { return stop || !queue.empty(); });
+            if (stop) return;
+            // admit: head request's precision, in arrival order, within the memory budget
+            std::vector<long long> take;
+            size_t bytes = 0;
+            const int prec = jobs.at(queue.front()).rq.precision;
+            for (auto it = queue.begin(); it != queue.end() && (int)take.size() < op.max_batch;) {
+                Job& j = jobs.at(*it);
+                if (j.rq.precision != prec) {
+                    ++it;
+                    continue;
+                }
+                const bool alone = take.empty() && admitted == 0;  // an oversized request runs alone
+                if (!alone && admitted + bytes + j.bytes > budget) break;
+                take.push_back(*it);
+                bytes += j.bytes;
+                j.state = JobState::kRunning;
+                it = queue.erase(it);
+            }
+            if (take.empty()) {  // memory is held by running launches: wait for one to finish
+                cv_work.wait_for(lk, std::chrono::milliseconds(2));
+                continue;
+            }
+            admitted += bytes;
+            peak = std::max(peak, admitted);
+            st.launches++;
+            st.queued = (long long)queue.size();
+            st.running += (long long)take.size();
+            std::vector<am1g_request> rqs(take.size());
+            std::vector<am1g_result> res(take.size());
+            for (size_t k = 0; k < take.size(); ++k) {
+                Job& j = jobs.at(take[k]);  // element references stay valid across rehashing
+                rqs[k] = j.rq;
+                std::memset(&res[k], 0, sizeof(am1g_result));
+                res[k].charges = j.charges.data();
+                res[k].mo_energies = j.mo.data();
+                res[k].mo_energies_beta = j.mob.data();
+            }
+            lk.unlock();
+            const int rc = am1g_scf_batch(ctx, (int)take.size(), rqs.data(), res.data());
+            const std::string msg = rc == AM1G_OK ? std::string() : std::string(am1g_last_error(ctx));
+            const auto now = Clock::now();
+            lk.lock();
+            admitted -= bytes;
+            st.running -= (long long)take.size();
+            for (size_t k = 0; k < take.size(); ++k) {
+                Job& j = jobs.at(take[k]);
+                j.res = res[k];
+                if (rc != AM1G_OK) j.res.status = rc;
+                j.state = JobState::kDone;
+                j.done = now;
+                st.completed++;
+            }
+            if (rc != AM1G_OK) err = msg;
+            cv_done.notify_all();
+            cv_work.notify_all();
+        }
+    }
+};
+
+extern "C" {
+
+void am1g_serve_defaults(am1g_serve_options* op) {
+    op->lanes = 4;
+    op->max_batch = 512;
+    op->memory_budget_mb = 32768.0;
+}
+
+int am1g_serve_create(int device, const am1g_serve_options* op, am1g_server** out) {
+    if (!out) return AM1G_INVALID_INPUT;
+    *out = nullptr;
+    am1g_serve_options o;
+    am1g_serve_defaults(&o);
+    if (op) o = *op;
+    if (o.lanes < 1 || o.lanes > 16 || o.max_batch < 1 || !(o.memory_budget_mb > 0.0))
+        return am1g_set_error(nullptr, AM1G_INVALID_INPUT,
+                              "serving needs 1..16 lanes, max_batch >= 1 and a positive memory budget");
+    am1g_server* sv = new am1g_server();
+    sv->op = o;
+    for (int l = 0; l < o.lanes; ++l) {
+        am1g_ctx* c = nullptr;
+        const int rc = am1g_create(device, &c);
+        if (rc != AM1G_OK) {
+            for (am1g_ctx* x : sv->ctxs) am1g_destroy(x);
+            delete sv;
+            return rc;
+        }
+        sv->ctxs.push_back(c);
+    }
+    for (int l = 0; l < o.lanes; ++l) sv->lanes.emplace_back([sv, l] { sv->lane_loop(l); });
+    *out = sv;
+    return AM1G_OK;
+}
+
+void am1g_serve_destroy(am1g_server* sv) {
+    if (!sv) return;
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        sv->stop = true;
+    }
+    sv->cv_work.notify_all();
+    for (auto& t : sv->lanes) t.join();
+    for (am1g_ctx* c : sv->ctxs) am1g_destroy(c);
+    delete sv;
+}
+
+int am1g_serve_submit(am1g_server* sv, const am1g_request* rq, long long* ticket) {
+    if (!sv || !rq || !ticket) return AM1G_INVALID_INPUT;
+    if (rq->n_atoms < 1 || !rq->z || !rq->xyz_bohr) {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        sv->err = "molecule has no atoms";
+        return AM1G_INVALID_INPUT;
+    }
+    Job j;
+    j.z.assign(rq->z, rq->z + rq->n_atoms);
+    j.xyz.assign(rq->xyz_bohr, rq->xyz_bohr + 3 * (size_t)rq->n_atoms);
+    j.rq = *rq;
+    j.rq.z = j.z.data();
+    j.rq.xyz_bohr = j.xyz.data();
+    j.bytes = job_bytes(j.rq);
+    int nao = 0;
+    for (uint8_t zz : j.z) nao += zz == 1 ? 1 : 4;
+    j.charges.assign(rq->n_atoms, 0.0);
+    j.mo.assign(nao, 0.0);
+    j.mob.assign(nao, 0.0);
+    j.submitted = Clock::now();
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        if (sv->stop) return AM1G_INVALID_INPUT;
+        const long long t = sv->next_ticket++;
+        Job& slot = sv->jobs[t];
+        slot = std::move(j);
+        slot.rq.z = slot.z.data();  // the vectors moved with their heap buffers; re-point anyway
+        slot.rq.xyz_bohr = slot.xyz.data();
+        sv->queue.push_back(t);
+        sv->st.submitted++;
+        sv->st.queued = (long long)sv->queue.size();
+        *ticket = t;
+    }
+    sv->cv_work.notify_one();
+    return AM1G_OK;
+}
+
+int am1g_serve_poll(am1g_server* sv, long long ticket, double wait_ms, am1g_result* out) {
+    if (!sv) return AM1G_INVALID_INPUT;
+    std::unique_lock<std::mutex> lk(sv->mu);
+    auto it = sv->jobs.find(ticket);
+    if (it == sv->jobs.end()) {
+        sv->err = "unknown ticket";
+        return AM1G_INVALID_INPUT;
+    }
+    auto ready = [&] { return sv->jobs.at(ticket).state == JobState::kDone || sv->jobs.at(ticket).cancelled; };
+    if (!ready()) {
+        if (wait_ms < 0) sv->cv_done.wait(lk, ready);
+        else if (wait_ms > 0)
+            sv->cv_done.wait_for(lk, std::chrono::duration<double, std::milli>(wait_ms), ready);
+    }
+    Job& j = sv->jobs.at(ticket);
+    if (j.cancelled && j.state == JobState::kQueued) {
+        sv->jobs.erase(ticket);
+        return AM1G_CANCELLED;
+    }
+    if (j.state != JobState::kDone) return AM1G_PENDING;
+    int status = j.cancelled ? AM1G_CANCELLED : j.res.status;
+    if (out) {
+        double* q = out->charges;
+        double* mo = out->mo_energies;
+        double* mob = out->mo_energies_beta;
+        double* dens = out->density;
+        double* coeff = out->mo_coeff;
+        double* coeff_b = out->mo_coeff_beta;
+        double* spin = out->spin_density;
+        *out = j.res;
+        out->charges = q;
+        out->mo_energies = mo;
+        out->mo_energies_beta = mob;
+        out->density = dens;  // not kept by the server
+        out->mo_coeff = coeff;
+        out->mo_coeff_beta = coeff_b;
+        out->spin_density = spin;
+        out->latency_ms = std::chrono::duration<double, std::milli>(j.done - j.submitted).count();
+        if (j.cancelled) out->status = AM1G_CANCELLED;
+        if (q) std::memcpy(q, j.charges.data(), sizeof(double) * j.charges.size());
+        if (mo && j.res.nao > 0) std::memcpy(mo, j.mo.data(), sizeof(double) * j.res.nao);
+        if (mob && j.res.nao > 0 && j.res.unrestricted) std::memcpy(mob, j.mob.data(), sizeof(double) * j.res.nao);
+    }
+    sv->jobs.erase(ticket);
+    return status;
+}
+
+int am1g_serve_cancel(am1g_server* sv, long long ticket) {
+    if (!sv) return AM1G_INVALID_INPUT;
+    std::lock_guard<std::mutex> lk(sv->mu);
+    auto it = sv->jobs.find(ticket);
+    if (it == sv->jobs.end()) {
+        sv->err = "unknown ticket";
+        return AM1G_INVALID_INPUT;
+    }
+    Job& j = it->second;
+    if (j.cancelled) return AM1G_OK;
+    j.cancelled = true;
+    sv->st.cancelled++;
+    if (j.state == JobState::kQueued) {
+        sv->queue.erase(std::remove(sv->queue.begin(), sv->queue.end(), ticket), sv->queue.end());
+        sv->st.queued = (long long)sv->queue.size();
+    }
+    sv->cv_done.notify_all();
+    return AM1G_OK;
+}
+
+int am1g_serve_get_stats(am1g_server* sv, am1g_serve_stats* out) {
+    if (!sv || !out) return AM1G_INVALID_INPUT;
+    std::lock_guard<std::mutex> lk(sv->mu);
+    *out = sv->st;
+    out->queued = (long long)sv->queue.size();
+    out->admitted_mb = (double)sv->admitted / 1048576.0;
+    out->peak_admitted_mb = (double)sv->peak / 1048576.0;
+    return AM1G_OK;
+}
+
+const char* am1g_serve_last_error(const am1g_server* sv) {
+    static thread_local std::string copy;
+    if (!sv) return "";
+    am1g_server* m = const_cast<am1g_server*>(sv);
+    std::lock_guard<std::mutex> lk(m->mu);
+    copy = m->err;
+    return copy.c_str();
+}
+
+}  // extern "C"

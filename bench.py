@@ -48,6 +48,32 @@ def percentile(x, q):
     return float(np.percentile(np.asarray(x, dtype=float), q)) if len(x) else None
 
 
+def serve_stream(am1g, stream, rate, lanes, max_batch, precision):
+    """Submit the requests at their Poisson arrival times to a continuous-batching server and
+    wait for all of them: throughput over the whole run and per-request latency percentiles."""
+    sv = am1g.Server(0, lanes=lanes, max_batch=max_batch)
+    tickets = []
+    t0 = time.perf_counter()
+    for req in stream:
+        wait = req.arrival_s - (time.perf_counter() - t0)
+        if wait > 0:
+            time.sleep(wait)
+        tickets.append(sv.submit(req.molecule.z, req.molecule.xyz_bohr, method=req.method,
+                                 multiplicity=req.molecule.multiplicity, precision=precision))
+    res = [sv.poll(t, wait_ms=-1) for t in tickets]
+    elapsed = time.perf_counter() - t0
+    stats = sv.stats()
+    sv.close()
+    lat = [r["latency_ms"] for r in res]
+    ok = sum(1 for r in res if r["status"] == 0)
+    return {"offered_per_s": rate, "requests": len(stream), "ok": ok, "scfs_per_s": round(ok / elapsed, 2),
+            "p50_ms": percentile(lat, 50), "p95_ms": percentile(lat, 95), "p99_ms": percentile(lat, 99),
+            "lanes": lanes, "max_batch": max_batch, "precision": precision, "launches": stats["launches"],
+            "uhf": sum(1 for r in res if r["unrestricted"]),
+            "rm1": sum(1 for q in stream if q.method == "rm1"),
+            "bcc_requested": sum(1 for q in stream if q.bcc)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -58,6 +84,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=1000)
+    ap.add_argument("--serve-rate", type=float, default=2000.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (every box in this setup has one); --gpus {args.gpus} "
@@ -171,6 +200,24 @@ def main():
                  "mean_fp32_iterations": float(np.mean([r["fp32_iterations"] for r in rm]))}
         log(f"mixed: {mixed['value']:.0f} SCFs/s")
 
+    # ---------------------------------------------------------------- serving stream
+    serving = None
+    if not args.no_serve:
+        slow_rate, slow_n = 20.0, 60
+        fast = G.serving_stream(args.serve_requests, args.serve_rate)
+        slow = G.serving_stream(slow_n, slow_rate)
+        serving = {
+            "plain": serve_stream(am1g, slow, slow_rate, lanes=1, max_batch=1, precision="fp64"),
+            "batched_mixed_same_load": serve_stream(am1g, slow, slow_rate, lanes=4, max_batch=512,
+                                                    precision="mixed"),
+            "batched_mixed": serve_stream(am1g, fast, args.serve_rate, lanes=4, max_batch=512, precision="mixed"),
+            "note": "AM1-BCC charges are not computed (the SCF each such request needs is); "
+                    "latency = submit to completion on the host clock",
+        }
+        log(f"serving: plain {serving['plain']['scfs_per_s']} SCFs/s p99 {serving['plain']['p99_ms']:.1f} ms; "
+            f"batched mixed at {args.serve_rate:g}/s: {serving['batched_mixed']['scfs_per_s']} SCFs/s "
+            f"p99 {serving['batched_mixed']['p99_ms']:.1f} ms")
+
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -196,6 +243,7 @@ def main():
                   "p99_ms": percentile(plain_ms, 99)},
         "batched_over_plain": round(value / plain_rate, 2) if plain_rate > 0 else None,
         "mixed": mixed,
+        "serving": serving,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -33,7 +33,9 @@ enum {
     AM1G_LINEAR_ALGEBRA = 3,       /* Am1Error::LinearAlgebra */
     AM1G_SCF_NOT_CONVERGED = 4,    /* Am1Error::ScfNotConverged; result is filled, residual set */
     AM1G_SCF_DIVERGED = 5,         /* Am1Error::ScfDiverged; iterations set */
-    AM1G_DEVICE_ERROR = 6          /* HIP runtime failure (no reference counterpart) */
+    AM1G_DEVICE_ERROR = 6,         /* HIP runtime failure (no reference counterpart) */
+    AM1G_PENDING = 7,              /* serving: the request has not finished yet */
+    AM1G_CANCELLED = 8             /* serving: the request was cancelled */
 };
 
 enum { AM1G_METHOD_AM1 = 0, AM1G_METHOD_RM1 = 1 };                       /* NddoMethod */
@@ -84,6 +86,8 @@ typedef struct {
     double* mo_energies_beta; /* [nao] UHF only */
     double* density;          /* [nao*nao] total density */
     double* mo_coeff;         /* [nao*nao] orbitals as columns (alpha for UHF) */
+    double* mo_coeff_beta;    /* [nao*nao] UHF only: beta orbitals as columns (BetaOrbitals::coeff) */
+    double* spin_density;     /* [nao*nao] UHF only: P_alpha - P_beta (Am1Result::spin_density) */
 } am1g_result;
 
 /* Fill a request with am1-rs's Am1Options defaults (scf.rs:174-197). */
@@ -134,6 +138,45 @@ void am1g_dc_defaults(am1g_dc_options* op);
 /* Divide-and-conquer SCF on the GPU: run_divide_conquer (src/divide_conquer.rs:406-572), closed
  * shell. All subsystem eigensolves of an iteration run as one batched launch. */
 int am1g_dc(am1g_ctx* ctx, const am1g_request* rq, const am1g_dc_options* op, am1g_dc_result* res);
+
+/* ---------------------------------------------------------------- serving
+ * Continuous batching. No reference counterpart: am1-rs's callers run run_am1 concurrently,
+ * one molecule per call, with the GIL released (python.rs:163, 551). A server admits requests
+ * as they arrive into concurrent batched launches ("lanes", one HIP stream and one set of
+ * device pools each): whenever a lane is free it takes every queued request of one precision
+ * that fits the device-memory budget (up to max_batch), so requests that arrive while others
+ * are on the GPU start as soon as a lane and memory free up, and launches of different lanes
+ * overlap on the device. Each result equals the molecule's am1g_scf_batch result alone. */
+typedef struct am1g_server am1g_server;
+
+typedef struct {
+    int lanes;                /* concurrent launches (default 4, at most 16) */
+    int max_batch;            /* molecules per launch (default 512) */
+    double memory_budget_mb;  /* device memory the running launches may hold (default 32768) */
+} am1g_serve_options;
+
+typedef struct {
+    long long submitted, completed, cancelled, queued, running, launches;
+    double admitted_mb;       /* device memory held by running launches now */
+    double peak_admitted_mb;
+} am1g_serve_stats;
+
+void am1g_serve_defaults(am1g_serve_options* op);
+int am1g_serve_create(int device, const am1g_serve_options* op, am1g_server** out);
+/* Stops the lanes after their running launches; queued requests are dropped. */
+void am1g_serve_destroy(am1g_server* sv);
+/* Copies the request (atoms included) and queues it; *ticket identifies it. */
+int am1g_serve_submit(am1g_server* sv, const am1g_request* rq, long long* ticket);
+/* Waits up to wait_ms (0: do not wait, < 0: until done). Returns AM1G_PENDING while the request
+ * runs; otherwise fills *out (scalars, and charges / mo_energies / mo_energies_beta when those
+ * pointers are set; latency_ms = submit to completion), returns the request's status and forgets
+ * the ticket. */
+int am1g_serve_poll(am1g_server* sv, long long ticket, double wait_ms, am1g_result* out);
+/* A queued request is dropped at once; a running one finishes on the device and is reported as
+ * AM1G_CANCELLED. */
+int am1g_serve_cancel(am1g_server* sv, long long ticket);
+int am1g_serve_get_stats(am1g_server* sv, am1g_serve_stats* out);
+const char* am1g_serve_last_error(const am1g_server* sv);
 
 /* Text of the last error on this context (the Display text of the matching Am1Error). */
 const char* am1g_last_error(const am1g_ctx* ctx);

@@ -79,3 +79,25 @@ def test_error_paths(engine):
     keep[4] = False
     r = engine.dc(z[keep], x[keep], multiplicity=2)
     assert r["status"] == 1 and "unrestricted" in r["message"]
+
+
+def test_water2000_full_size_against_oracle_fixture(engine):
+    """The 2000-water workload at full size (6000 atoms, 12,000 AOs, 2000 one-molecule cores,
+    4.0 A buffer, 1000 K) against the oracle's result committed in tests/golden (made by
+    tests/golden/make_dc_water2000.py with the same tight tolerances)."""
+    import os
+
+    from am1g import geometry as G
+
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "dc_water2000.npz"))
+    mol = G.water_cluster_config(3)
+    r = engine.dc(mol.z, mol.xyz_bohr, core_of_atom=np.arange(len(mol.z)) // 3, buffer_radius=4.0 / 0.529167,
+                  kt=KT_1000K, e_tol=1e-9, p_tol=1e-10, max_scf=300)
+    assert r["status"] == 0, r.get("message")
+    assert r["n_subsystems"] == int(g["n_subsystems"]) == 2000
+    assert r["largest_subsystem_aos"] == int(g["largest_subsystem_aos"])
+    assert abs(r["total_ev"] - float(g["total_ev"])) <= E_TOL
+    assert abs(r["heat_of_formation_kcal"] - float(g["heat_of_formation_kcal"])) <= E_TOL * 23.061
+    assert np.abs(r["charges"] - g["charges"]).max() <= Q_TOL
+    assert abs(r["charges"].sum()) < 1e-8
+    assert abs(r["fermi_ev"] - float(g["fermi_ev"])) <= 1e-5

@@ -139,6 +139,15 @@ def test_methyl_radical_uhf(engine):
     assert r["unrestricted"]
     _compare(r, o)
     assert np.abs(r["mo_energies_beta"] - o["mo_energies_beta"]).max() < 1e-6
+    # Am1Result::spin_density and BetaOrbitals::coeff: Tr(P_a - P_b) = N_a - N_b = 1, and the beta
+    # orbitals are orthonormal and reproduce the beta density from the n_occ_beta lowest
+    n = r["nao"]
+    assert abs(np.trace(r["spin_density"]) - 1.0) < 1e-9
+    Cb = r["mo_coeff_beta"]
+    assert np.allclose(Cb.T @ Cb, np.eye(n), atol=1e-10)
+    Pb = 0.5 * (r["density"] - r["spin_density"])
+    occ = Cb[:, : r["n_occ_beta"]]
+    assert np.abs(occ @ occ.T - Pb).max() < 1e-9
 
 
 def test_forced_uhf_singlet_equals_rhf(engine):

@@ -1,0 +1,107 @@
+"""Continuous-batching server (``am1g_serve_*``): every result is the same alone and in the mix,
+requests that arrive while others run are admitted, cancellation and the memory budget work."""
+import time
+
+import numpy as np
+import pytest
+
+import am1g
+from am1g import geometry as G
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _mix(seed=7, n_chain=24, n_water=8):
+    """Chains, water clusters, doublet radicals (UHF) and RM1 requests."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i, m in enumerate(G.chain_batch(n_chain, seed=seed)):
+        z, x = m.z, m.xyz_bohr
+        kw = {"method": "rm1" if i % 5 == 0 else "am1"}
+        if i % 6 == 3:  # drop one hydrogen: neutral doublet, UHF
+            keep = np.ones(len(z), dtype=bool)
+            keep[np.nonzero(z == 1)[0][0]] = False
+            z, x = z[keep], x[keep]
+            kw["multiplicity"] = 2
+        out.append((z, x, kw))
+    for _ in range(n_water):
+        w = G.small_water_cluster(int(rng.integers(2, 9)), rng)
+        out.append((w.z, w.xyz_bohr, {"method": "am1"}))
+    return out
+
+
+def test_results_in_the_mix_equal_results_alone(engine):
+    reqs = _mix()
+    sv = am1g.Server(0, lanes=3, max_batch=8)
+    try:
+        tickets = [sv.submit(z, x, **kw) for z, x, kw in reqs]
+        got = [sv.poll(t, wait_ms=-1) for t in tickets]
+        stats = sv.stats()
+    finally:
+        sv.close()
+    assert stats["completed"] == len(reqs) and stats["launches"] >= len(reqs) // 8
+    for (z, x, kw), r in zip(reqs, got):
+        alone = engine.scf_batch([(z, x, kw)])[0]
+        assert r["status"] == alone["status"] == 0
+        assert r["total_ev"] == alone["total_ev"]
+        assert np.array_equal(r["charges"], alone["charges"])
+        assert r["iterations"] == alone["iterations"]
+        assert r["unrestricted"] == (kw.get("multiplicity", 1) == 2)
+    for i in (0, 3, len(reqs) - 1):  # and against the oracle
+        z, x, kw = reqs[i]
+        o = O.scf(z, x, full=False, **kw)
+        assert abs(got[i]["total_ev"] - o["total_ev"]) <= 1e-6
+        assert np.abs(got[i]["charges"] - o["charges"]).max() <= 1e-6
+
+
+def test_requests_arriving_during_a_launch_are_admitted(engine):
+    mols = G.chain_batch(400, seed=3)
+    sv = am1g.Server(0, lanes=2, max_batch=256)
+    try:
+        first = [sv.submit(m.z, m.xyz_bohr) for m in mols[:300]]
+        time.sleep(0.005)
+        late = [sv.submit(m.z, m.xyz_bohr, precision="mixed") for m in mols[300:]]
+        res = [sv.poll(t, wait_ms=-1) for t in first + late]
+        stats = sv.stats()
+    finally:
+        sv.close()
+    assert all(r["status"] == 0 for r in res)
+    assert stats["launches"] >= 2 and stats["completed"] == 400
+    # mixed-precision requests stay within the FP64 contract
+    ref = engine.scf_batch([(m.z, m.xyz_bohr) for m in mols[300:310]])
+    for a, b in zip(res[300:310], ref):
+        assert abs(a["total_ev"] - b["total_ev"]) <= 1e-6
+
+
+def test_cancel_and_unknown_ticket():
+    mols = G.chain_batch(64, seed=5)
+    sv = am1g.Server(0, lanes=1, max_batch=4)
+    try:
+        ts = [sv.submit(m.z, m.xyz_bohr) for m in mols]
+        sv.cancel(ts[-1])
+        last = sv.poll(ts[-1], wait_ms=-1)
+        assert last["status"] == am1g.CANCELLED
+        rest = [sv.poll(t, wait_ms=-1) for t in ts[:-1]]
+        assert all(r["status"] == 0 for r in rest)
+        with pytest.raises(KeyError):
+            sv.poll(ts[0])
+    finally:
+        sv.close()
+
+
+def test_memory_budget_bounds_admission():
+    mols = G.chain_batch(48, seed=9)
+    sv = am1g.Server(0, lanes=4, max_batch=512, memory_budget_mb=2.0)
+    try:
+        ts = [sv.submit(m.z, m.xyz_bohr) for m in mols]
+        res = [sv.poll(t, wait_ms=-1) for t in ts]
+        stats = sv.stats()
+    finally:
+        sv.close()
+    assert all(r["status"] == 0 for r in res)
+    # a closed-shell molecule holds at least its six dense nao x nao FP64 matrices on the device,
+    # so a 2 MB budget needs at least this many launches however many lanes are free
+    dense_mb = sum(6 * 8 * (4 * int((m.z != 1).sum()) + int((m.z == 1).sum())) ** 2 for m in mols) / 2**20
+    assert stats["launches"] >= int(dense_mb / 2.0)
+    assert stats["launches"] > 1
