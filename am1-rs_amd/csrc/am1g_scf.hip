@@ -354,8 +354,9 @@ __device__ int mol_eigen(int nao, const double* Fuse, double* Ct, double* T0, do
     const size_t nn = (size_t)n * n;
     const size_t tri = (size_t)n * (n + 1) / 2;
     const bool a_lds = n <= kALds && n <= lds_n;
-    const bool v_lds = a_lds && tri + nn <= (size_t)kLdsDoubles;
-    double* V = v_lds ? lds + tri : Ct;
+    // eigenvector rows [0, v_rows) beside A in LDS, the rest in place in Ct
+    const int v_rows = a_lds ? (int)min((size_t)n, ((size_t)kLdsDoubles - tri) / (size_t)n) : 0;
+    const bool v_lds = a_lds && v_rows == n;
     const double* src = Fuse;
     if (warm) {
         // T0 = Ct F ; T1 = T0 Ct^T = the Fock matrix in the previous orbital basis
@@ -377,12 +378,12 @@ __device__ int mol_eigen(int nao, const double* Fuse, double* Ct, double* T0, do
         }
         for (size_t idx = t; idx < nn; idx += kWg) {
             int i = (int)(idx / n), j = (int)(idx - (size_t)i * n);
-            if (!warm) V[idx] = (i == j) ? 1.0 : 0.0;
-            else if (v_lds) V[idx] = Ct[idx];
+            const double v = warm ? Ct[idx] : (i == j ? 1.0 : 0.0);
+            if (i < v_rows) lds[tri + idx] = v;
+            else if (!warm) Ct[idx] = v;
         }
         __syncthreads();
-        sweeps = v_lds ? wg_jacobi_lds<true>(n, (int)tri, nullptr, s, 60, tol_abs)
-                       : wg_jacobi_lds<false>(n, 0, Ct, s, 60, tol_abs);
+        sweeps = wg_jacobi_lds(n, (int)tri, v_rows, Ct, s, 60, tol_abs);
     } else {
         double* A = T1;
         if (!warm)
@@ -415,7 +416,8 @@ __device__ int mol_eigen(int nao, const double* Fuse, double* Ct, double* T0, do
         }
         for (int off = 8; off > 0; off >>= 1) rank += __shfl_xor(rank, off, 16);
         if (t % 16 == 0) eps[rank] = ei;
-        for (int c = t % 16; c < n; c += 16) dst[(size_t)rank * n + c] = V[(size_t)i * n + c];
+        const double* vi = i < v_rows ? lds + tri + (size_t)i * n : Ct + (size_t)i * n;
+        for (int c = t % 16; c < n; c += 16) dst[(size_t)rank * n + c] = vi[c];
     }
     __syncthreads();
     if (!v_lds) {
@@ -1251,29 +1253,80 @@ __global__ __launch_bounds__(256) void k_fock_off(SysArgs a, const double* Ps, d
 // Deterministic two-stage sums: fixed-size grid of block partials, then one block in order.
 constexpr int kRedBlocks = 1024;
 
+// Compensated (sum, error) pairs. The large reductions of one big system (E_core over ~1e7
+// pairs, Tr P(H+F) over ~1e8 entries) are +-1e8 eV terms that cancel to ~1e6 eV; plain FP64
+// summation in any order moves the result by ~1e-5 eV at 6000 atoms, so these sums carry their
+// rounding error (Neumaier within a thread, error-free TwoSum when combining) and are accurate to
+// a few ulps of the result whatever the grid shape.
+struct KSum {
+    double s = 0.0, c = 0.0;
+};
+
+__device__ __forceinline__ void ksum_add(KSum& k, double x) {
+    const double t = k.s + x;
+    k.c += fabs(k.s) >= fabs(x) ? (k.s - t) + x : (x - t) + k.s;
+    k.s = t;
+}
+
+__device__ __forceinline__ KSum ksum_join(KSum a, KSum b) {
+    KSum r;
+    r.s = a.s + b.s;
+    const double bp = r.s - a.s;
+    r.c = a.c + b.c + ((a.s - (r.s - bp)) + (b.s - bp));
+    return r;
+}
+
+__device__ __forceinline__ KSum ksum_block(KSum v) {  // 256 threads, fixed order; result in thread 0
+    __shared__ double sh[2][4];
+    for (int off = 32; off > 0; off >>= 1) {
+        KSum o;
+        o.s = __shfl_xor(v.s, off, 64);
+        o.c = __shfl_xor(v.c, off, 64);
+        v = ksum_join(v, o);
+    }
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) {
+        sh[0][threadIdx.x >> 6] = v.s;
+        sh[1][threadIdx.x >> 6] = v.c;
+    }
+    __syncthreads();
+    KSum r;
+    if (threadIdx.x == 0) {
+        KSum w[4];
+        for (int q = 0; q < 4; ++q) {
+            w[q].s = sh[0][q];
+            w[q].c = sh[1][q];
+        }
+        r = ksum_join(ksum_join(w[0], w[1]), ksum_join(w[2], w[3]));
+    }
+    return r;
+}
+
+// Block partials of sum a * (b + c) (b, c optional), as (sum, error) pairs.
 __global__ __launch_bounds__(256) void k_dot3_partial(int64_t n, const double* a, const double* b, const double* c,
                                                       double* partial) {
-    __shared__ double sh[4];
-    double v = 0.0;
+    KSum v;
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)kRedBlocks * 256)
-        v += a[i] * (b ? (c ? b[i] + c[i] : b[i]) : 1.0);
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) partial[blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+        ksum_add(v, a[i] * (b ? (c ? b[i] + c[i] : b[i]) : 1.0));
+    const KSum r = ksum_block(v);
+    if (threadIdx.x == 0) {
+        partial[2 * blockIdx.x] = r.s;
+        partial[2 * blockIdx.x + 1] = r.c;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_final_sum(int nparts, const double* partial, double* out) {
-    __shared__ double sh[4];
-    double v = 0.0;
-    for (int i = threadIdx.x; i < nparts; i += 256) v += partial[i];
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) out[0] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+    KSum v;
+    for (int i = threadIdx.x; i < nparts; i += 256) {
+        KSum p;
+        p.s = partial[2 * i];
+        p.c = partial[2 * i + 1];
+        v = ksum_join(v, p);
+    }
+    const KSum r = ksum_block(v);
+    if (threadIdx.x == 0) out[0] = r.s + r.c;
 }
 
-// SAD guess (scf.rs:475-494) scaled by `scale` into a zeroed density matrix.
 __global__ __launch_bounds__(256) void k_sad(SysArgs a, double scale, double* P) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= a.md.nat) return;

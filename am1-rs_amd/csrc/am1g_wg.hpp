@@ -15,8 +15,8 @@ constexpr int kWaves = kWg / 64;
 struct WgScratch {
     double red[kWaves * 4];  // reduction scratch
     double bcast[16];     // broadcast slots
-    double jc[128], js[128];  // Jacobi rotations of one round
-    int jp[128], jq[128];
+    double2 rot[128];     // Jacobi rotations of one round: (cos, sin)
+    int2 pq[128];         // and their index pairs (q = -1: padding slot of an odd order)
 };
 
 // Deterministic sum over the workgroup: fixed-shape tree over thread slots.
@@ -233,7 +233,6 @@ __device__ __forceinline__ void rr_pair(int m, int r, int k, int& p, int& q) {
 // Fock matrix needs its eigenvectors only to well inside the error the next iteration removes.
 // Returns the number of sweeps, or -1 when max_sweeps did not suffice.
 constexpr int kBlkPer = 5;   // blocks per thread: ceil(64*65/2 / kWg) for n <= 128
-constexpr int kVecPer = 16;  // eigenvector element pairs per thread: 64*128 / kWg
 
 template <bool PACKED>
 __device__ __forceinline__ size_t jidx(int i, int j, int lda) {
@@ -241,21 +240,45 @@ __device__ __forceinline__ size_t jidx(int i, int j, int lda) {
     return (size_t)i * lda + j;
 }
 
-// LDS-resident Jacobi: A packed at lds_dyn[0, n(n+1)/2); eigenvectors (rows of Vt) at
-// lds_dyn[v_off, v_off + n*n) when V_LDS, else in global memory at Vg. Addresses are LDS
-// offsets, so every access is a ds_read/ds_write.
+// One Jacobi rotation angle for the 2x2 block (app, apq; apq, aqq): (cos, sin), or (1, 0)
+// when |apq| <= tau.
+__device__ __forceinline__ double2 jacobi_angle(double app, double apq, double aqq, double tau) {
+    double2 r = make_double2(1.0, 0.0);
+    if (fabs(apq) > tau) {
+        const double theta = (aqq - app) / (2.0 * apq);
+        const double tt = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+        r.x = 1.0 / sqrt(tt * tt + 1.0);
+        r.y = tt * r.x;
+    }
+    return r;
+}
+
+// LDS-resident Jacobi: A packed at lds_dyn[0, n(n+1)/2). Eigenvector rows (rows of Vt) below
+// v_rows live in LDS at lds_dyn[v_off + row * n], the others in global memory at Vg + row * n, so
+// orders whose full eigenvector matrix does not fit beside A still keep most rows on chip.
+// The eigenvector update gives each rotation pair a fixed group of threads that splits the
+// pair's two rows by columns, so a thread reads its rotation once per round and then streams
+// contiguous elements.
 extern __shared__ double lds_dyn[];
 
-template <bool V_LDS>
-__device__ __noinline__ int wg_jacobi_lds(int n, int v_off, double* __restrict__ Vg, WgScratch& s, int max_sweeps,
-                                          double tol_abs) {
+__device__ __forceinline__ void rotate_rows(double* __restrict__ xp, double* __restrict__ xq, int i0, int n, int step,
+                                            double c, double sn) {
+#pragma unroll 4
+    for (int i = i0; i < n; i += step) {
+        const double vp = xp[i], vq = xq[i];
+        xp[i] = c * vp - sn * vq;
+        xq[i] = sn * vp + c * vq;
+    }
+}
+
+__device__ __noinline__ int wg_jacobi_lds(int n, int v_off, int v_rows, double* __restrict__ Vg, WgScratch& s,
+                                          int max_sweeps, double tol_abs) {
     const int t = threadIdx.x;
     const int m = (n + 1) & ~1;
     const int h = m >> 1;
     const int nblk = h * (h + 1) / 2;
-    const int nvec = h * n;
     double* A = lds_dyn;
-    int bslot[kBlkPer], vslot[kVecPer];
+    int bslot[kBlkPer];
 #pragma unroll
     for (int u = 0; u < kBlkPer; ++u) {
         int blk = t + u * kWg;
@@ -267,11 +290,10 @@ __device__ __noinline__ int wg_jacobi_lds(int n, int v_off, double* __restrict__
             bslot[u] = (a << 8) | (blk - a * (a + 1) / 2);
         }
     }
-#pragma unroll
-    for (int u = 0; u < kVecPer; ++u) {
-        int idx = t + u * kWg;
-        vslot[u] = idx < nvec ? ((idx / n) << 16) | (idx % n) : -1;
-    }
+    // eigenvector work: pair va, columns vc0, vc0 + g, ... (g threads per pair)
+    const int g = kWg / max(h, 1);
+    const int va = t / g, vc0 = t - va * g;
+    const bool vown = va < h;
     for (int sweep = 0; sweep <= max_sweeps; ++sweep) {
         double offmax = 0.0, dg = 0.0;
         for (int i = t / 32; i < n; i += kWg / 32) {
@@ -291,31 +313,22 @@ __device__ __noinline__ int wg_jacobi_lds(int n, int v_off, double* __restrict__
             if (t < h) {
                 int p, q;
                 rr_pair(m, r, t, p, q);
-                double c = 1.0, sn = 0.0;
-                if (q < n) {
-                    double apq = A[q * (q + 1) / 2 + p];
-                    if (fabs(apq) > tau) {
-                        double app = A[p * (p + 1) / 2 + p], aqq = A[q * (q + 1) / 2 + q];
-                        double theta = (aqq - app) / (2.0 * apq);
-                        double tt = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-                        c = 1.0 / sqrt(tt * tt + 1.0);
-                        sn = tt * c;
-                    }
-                }
-                s.jc[t] = c;
-                s.js[t] = sn;
-                s.jp[t] = p;
-                s.jq[t] = q < n ? q : -1;  // -1: the padding slot of an odd-order matrix
+                double2 cs = make_double2(1.0, 0.0);
+                if (q < n) cs = jacobi_angle(A[p * (p + 1) / 2 + p], A[q * (q + 1) / 2 + p], A[q * (q + 1) / 2 + q], tau);
+                s.rot[t] = cs;
+                s.pq[t] = make_int2(p, q < n ? q : -1);
             }
             __syncthreads();
 #pragma unroll
             for (int u = 0; u < kBlkPer; ++u) {
                 if (bslot[u] < 0) continue;
                 const int a = bslot[u] >> 8, b = bslot[u] & 255;
-                const double ca = s.jc[a], sa = s.js[a], cb = s.jc[b], sb = s.js[b];
+                const double2 ra = s.rot[a], rb = s.rot[b];
+                const double ca = ra.x, sa = ra.y, cb = rb.x, sb = rb.y;
                 // a slot with s == 0 still owns rows the other side's rotation moves
                 if (sa == 0.0 && (a == b || sb == 0.0)) continue;
-                const int pa = s.jp[a], qa = s.jq[a], pb = s.jp[b], qb = s.jq[b];
+                const int2 ia = s.pq[a], ib = s.pq[b];
+                const int pa = ia.x, qa = ia.y, pb = ib.x, qb = ib.y;
                 if (a == b) {
                     const int i00 = pa * (pa + 1) / 2 + pa, i11 = qa * (qa + 1) / 2 + qa, i01 = qa * (qa + 1) / 2 + pa;
                     const double x00 = A[i00], x01 = A[i01], x11 = A[i11];
@@ -339,40 +352,26 @@ __device__ __noinline__ int wg_jacobi_lds(int n, int v_off, double* __restrict__
                     if (i11 >= 0) A[i11] = r10 * sb + r11 * cb;
                 }
             }
-            // Vt rows p, q: (v_p, v_q) <- (c v_p - s v_q, s v_p + c v_q), four slots in flight
-#pragma unroll
-            for (int c0 = 0; c0 < kVecPer; c0 += 4) {
-                double vp[4], vq[4];
-                int ip[4], iq[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int sl = vslot[c0 + u];
-                    ip[u] = -1;
-                    if (sl < 0) continue;
-                    const int a = sl >> 16, i = sl & 0xffff;
-                    if (s.js[a] == 0.0) continue;
-                    ip[u] = s.jp[a] * n + i;
-                    iq[u] = s.jq[a] * n + i;
-                    if (V_LDS) {
-                        vp[u] = lds_dyn[v_off + ip[u]];
-                        vq[u] = lds_dyn[v_off + iq[u]];
+            // Vt rows p, q: (v_p, v_q) <- (c v_p - s v_q, s v_p + c v_q)
+            if (vown) {
+                const double2 rv = s.rot[va];
+                if (rv.y != 0.0) {
+                    const int2 iv = s.pq[va];
+                    const bool pl = iv.x < v_rows, ql = iv.y < v_rows;
+                    if (pl && ql) {
+                        rotate_rows(lds_dyn + v_off + iv.x * n, lds_dyn + v_off + iv.y * n, vc0, n, g, rv.x, rv.y);
+                    } else if (!pl && !ql) {
+                        rotate_rows(Vg + iv.x * n, Vg + iv.y * n, vc0, n, g, rv.x, rv.y);
                     } else {
-                        vp[u] = Vg[ip[u]];
-                        vq[u] = Vg[iq[u]];
-                    }
-                }
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    if (ip[u] < 0) continue;
-                    const int a = vslot[c0 + u] >> 16;
-                    const double ca = s.jc[a], sa = s.js[a];
-                    const double np_ = ca * vp[u] - sa * vq[u], nq_ = sa * vp[u] + ca * vq[u];
-                    if (V_LDS) {
-                        lds_dyn[v_off + ip[u]] = np_;
-                        lds_dyn[v_off + iq[u]] = nq_;
-                    } else {
-                        Vg[ip[u]] = np_;
-                        Vg[iq[u]] = nq_;
+                        // one row on chip, one in global memory (p < q, so p is the LDS one)
+                        double* xp = lds_dyn + v_off + iv.x * n;
+                        double* xq = Vg + iv.y * n;
+#pragma unroll 4
+                        for (int i = vc0; i < n; i += g) {
+                            const double vp = xp[i], vq = xq[i];
+                            xp[i] = rv.x * vp - rv.y * vq;
+                            xq[i] = rv.y * vp + rv.x * vq;
+                        }
                     }
                 }
             }
@@ -407,21 +406,11 @@ __device__ __noinline__ int wg_jacobi_big(int n, double* A, int lda, double* Vt,
             if (t < h) {
                 int p, q;
                 rr_pair(m, r, t, p, q);
-                double c = 1.0, sn = 0.0;
-                if (q < n) {
-                    double apq = A[(size_t)p * lda + q];
-                    if (fabs(apq) > tau) {
-                        double app = A[(size_t)p * lda + p], aqq = A[(size_t)q * lda + q];
-                        double theta = (aqq - app) / (2.0 * apq);
-                        double tt = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-                        c = 1.0 / sqrt(tt * tt + 1.0);
-                        sn = tt * c;
-                    }
-                }
-                s.jc[t] = c;
-                s.js[t] = sn;
-                s.jp[t] = p;
-                s.jq[t] = q < n ? q : -1;
+                double2 cs = make_double2(1.0, 0.0);
+                if (q < n)
+                    cs = jacobi_angle(A[(size_t)p * lda + p], A[(size_t)p * lda + q], A[(size_t)q * lda + q], tau);
+                s.rot[t] = cs;
+                s.pq[t] = make_int2(p, q < n ? q : -1);
             }
             __syncthreads();
             for (int blk = t; blk < nblk; blk += kWg) {
@@ -429,9 +418,10 @@ __device__ __noinline__ int wg_jacobi_big(int n, double* A, int lda, double* Vt,
                 while (a * (a + 1) / 2 > blk) --a;
                 while ((a + 1) * (a + 2) / 2 <= blk) ++a;
                 const int b = blk - a * (a + 1) / 2;
-                if (!((s.js[a] != 0.0) || (a != b && s.js[b] != 0.0))) continue;
-                const int pa = s.jp[a], qa = s.jq[a], pb = s.jp[b], qb = s.jq[b];
-                const double ca = s.jc[a], sa = s.js[a], cb = s.jc[b], sb = s.js[b];
+                const double2 ra = s.rot[a], rb = s.rot[b];
+                if (!((ra.y != 0.0) || (a != b && rb.y != 0.0))) continue;
+                const int pa = s.pq[a].x, qa = s.pq[a].y, pb = s.pq[b].x, qb = s.pq[b].y;
+                const double ca = ra.x, sa = ra.y, cb = rb.x, sb = rb.y;
                 if (a == b) {
                     double x00 = A[(size_t)pa * lda + pa], x01 = A[(size_t)pa * lda + qa];
                     double x11 = A[(size_t)qa * lda + qa];
@@ -469,10 +459,10 @@ __device__ __noinline__ int wg_jacobi_big(int n, double* A, int lda, double* Vt,
             }
             for (int idx = t; idx < h * n; idx += kWg) {
                 const int a = idx / n, i = idx - a * n;
-                const double sa = s.js[a];
+                const double sa = s.rot[a].y;
                 if (sa == 0.0) continue;
-                const double ca = s.jc[a];
-                const int p = s.jp[a], q = s.jq[a];
+                const double ca = s.rot[a].x;
+                const int p = s.pq[a].x, q = s.pq[a].y;
                 const double vp = Vt[(size_t)p * ldv + i], vq = Vt[(size_t)q * ldv + i];
                 Vt[(size_t)p * ldv + i] = ca * vp - sa * vq;
                 Vt[(size_t)q * ldv + i] = sa * vp + ca * vq;

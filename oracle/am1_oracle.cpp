@@ -608,10 +608,24 @@ struct Mat {
     double operator()(int i, int j) const { return d[(size_t)i * n + j]; }
 };
 
-double frob(const Mat& a, const Mat& b) {
-    double acc = 0.0;
-    for (size_t k = 0; k < a.d.size(); ++k) acc += a.d[k] * b.d[k];
-    return acc;
+// Neumaier-compensated sum. Used for the two large energy reductions (Tr P X, E_core): at
+// thousands of atoms they are +-1e8 eV terms cancelling to ~1e6 eV, where a plain sum in any
+// order carries ~1e-5 eV of rounding; compensated, the result is within a few ulps of the exact
+// sum, which is what both this oracle and the GPU path approximate.
+struct KSum {
+    double s = 0.0, c = 0.0;
+    void add(double x) {
+        const double t = s + x;
+        c += std::fabs(s) >= std::fabs(x) ? (s - t) + x : (x - t) + s;
+        s = t;
+    }
+    double value() const { return s + c; }
+};
+
+double frob(const Mat& a, const Mat& b) {  // linalg.rs:237-255 (compensated here, see KSum)
+    KSum acc;
+    for (size_t k = 0; k < a.d.size(); ++k) acc.add(a.d[k] * b.d[k]);
+    return acc.value();
 }
 
 // -------------------------------------------------------- optional LAPACK/BLAS through dlopen
@@ -827,7 +841,7 @@ void build_core(const System& s, Mat& H, std::vector<Pair>& pairs) {
 // repulsion.rs:38-56, 144-188
 double core_core_energy(const System& s) {
     const ParamSet& ps = *s.ps;
-    double energy = 0.0;
+    KSum energy;  // compensated: see KSum
     for (int i = 0; i < s.nat; ++i)
         for (int j = i + 1; j < s.nat; ++j) {
             const Elem& ei = ps.el[s.z[i]];
@@ -849,9 +863,9 @@ double core_core_energy(const System& s) {
             double g = 0.0;
             for (int k = 0; k < ei.ng; ++k) g += ei.gk[k] * std::exp(-ei.gl[k] * (sA - ei.gm[k]) * (sA - ei.gm[k]));
             for (int k = 0; k < ej.ng; ++k) g += ej.gk[k] * std::exp(-ej.gl[k] * (sA - ej.gm[k]) * (sA - ej.gm[k]));
-            energy += term1 + g * zz / sA;
+            energy.add(term1 + g * zz / sA);
         }
-    return energy;
+    return energy.value();
 }
 
 // fock.rs:27-65
