@@ -157,7 +157,7 @@ DC_PHASES = ["fock", "eigen", "fermi", "assembly", "diis"]
 EXPORTS = ["am1g_request_defaults", "am1g_create", "am1g_destroy", "am1g_scf", "am1g_scf_batch",
            "am1g_dc_defaults", "am1g_dc", "am1g_serve_defaults", "am1g_serve_create", "am1g_serve_destroy",
            "am1g_serve_submit", "am1g_serve_poll", "am1g_serve_cancel", "am1g_serve_get_stats",
-           "am1g_serve_last_error", "am1g_last_error", "am1g_element_info", "am1g_last_kernel_ms"]
+           "am1g_serve_last_error", "am1g_symmetric_eigen", "am1g_last_error", "am1g_element_info", "am1g_last_kernel_ms"]
 
 _lib = None
 
@@ -193,6 +193,8 @@ def load():
     L.am1g_serve_get_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(ServeStats)]
     L.am1g_serve_last_error.argtypes = [ctypes.c_void_p]
     L.am1g_serve_last_error.restype = ctypes.c_char_p
+    L.am1g_symmetric_eigen.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                       ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
     L.am1g_last_error.argtypes = [ctypes.c_void_p]
     L.am1g_last_error.restype = ctypes.c_char_p
     L.am1g_request_defaults.argtypes = [ctypes.POINTER(Request)]
@@ -241,6 +243,19 @@ class Engine:
 
     def last_kernel_ms(self) -> float:
         return float(self._L.am1g_last_kernel_ms(self._ctx))
+
+    def eigh(self, a):
+        """``symmetric_eigen`` on the device: (eigenvalues ascending, eigenvectors as columns)."""
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        n = a.shape[0]
+        w = np.zeros(n)
+        v = np.zeros((n, n))
+        dp = ctypes.POINTER(ctypes.c_double)
+        st = self._L.am1g_symmetric_eigen(self._ctx, n, a.ctypes.data_as(dp), w.ctypes.data_as(dp),
+                                          v.ctypes.data_as(dp))
+        if st != 0:
+            raise ValueError(f"{STATUS_NAMES.get(st, st)}: {self.last_error()}")
+        return w, v
 
     # -- request marshalling -------------------------------------------------------------
     def _request(self, z, xyz_bohr, keep, charge=0.0, multiplicity=1, method="am1", reference="auto",

@@ -216,8 +216,9 @@ int am1g_create(int device, am1g_ctx** out) {
 
 void am1g_destroy(am1g_ctx* ctx) {
     if (!ctx) return;
-    am1g_dc_release(ctx);
     (void)hipSetDevice(ctx->device);
+    am1g_dc_release(ctx);
+    am1g_large_release(ctx);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);

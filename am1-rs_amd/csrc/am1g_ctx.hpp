@@ -10,6 +10,7 @@
 
 #include "../../include/am1g.h"
 #include "am1g_common.hpp"
+#include "am1g_eig.hpp"
 #include "am1g_launch.hpp"
 
 using namespace am1g;  // internal header of the library
@@ -49,6 +50,9 @@ struct am1g_ctx {
     DevBuf elems, mols, z, xyz, mats, wpool, hist, mo_e, mo_eb, charges, results, queue, order;
     DevBuf alist, aoff, acls, pair_off, ecore_pair;
     struct DcBuffers* dc = nullptr;
+    struct LargeBuffers* large = nullptr;  // one large system: dense matrices, DIIS history
+    EigWork* eig = nullptr;                // large-order eigensolver workspace
+    rocblas_handle blas = nullptr;         // created on first use, bound to `stream`
 };
 
 // Buffers of the divide-and-conquer driver, kept on the context so repeated calls reuse them.
@@ -67,6 +71,9 @@ struct Prepared {
 int am1g_set_error(am1g_ctx* ctx, int code, const char* fmt, ...);
 int am1g_prepare(am1g_ctx* ctx, const am1g_request& rq, Prepared& out, std::string& msg, int max_ao);
 void am1g_dc_release(am1g_ctx* ctx);
+void am1g_large_release(am1g_ctx* ctx);
+// Creates the context's rocBLAS handle and eigensolver workspace on first use.
+int am1g_large_setup(am1g_ctx* ctx);
 
 #define HIP_TRY(ctx, expr)                                                                                    \
     do {                                                                                                      \

@@ -1,0 +1,643 @@
+// Dense symmetric eigensolver for one large matrix on the whole device.
+//
+// Replaces faer's self_adjoint_eigen behind am1-rs symmetric_eigen (linalg.rs:276-331) for
+// orders beyond what one workgroup holds on chip: eigenvalues ascending, eigenvectors
+// orthonormal. Three stages:
+//   1. Householder tridiagonalization (lower, unblocked): per column a reflector kernel, a
+//      row-parallel symmetric matrix-vector product, and a rank-2 update of the trailing block.
+//   2. Tridiagonal divide and conquer (Cuppen's splitting with the deflation rules of
+//      Dongarra-Sorensen and Gu-Eisenstat's Loewner correction for orthogonality). Leaves of at
+//      most 64 are diagonalized by the workgroup Jacobi solver; each merge solves its secular
+//      equation one wavefront per root and multiplies the children's eigenvectors by the
+//      merge's eigenvector matrix with a library GEMM.
+//   3. Back-transformation by the Householder reflectors in blocks of 64 (compact WY form,
+//      three GEMMs per block).
+// The deflation scan and the final ordering of each merge run on the host (O(m) each).
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <numeric>
+
+#include "am1g_common.hpp"
+#include "am1g_eig.hpp"
+#include "am1g_wg.hpp"
+
+namespace am1g {
+namespace {
+
+constexpr int kLeaf = 64;  // largest tridiagonal block diagonalized directly
+constexpr int kNb = 64;    // reflectors per back-transformation block
+constexpr int kRedThreads = 1024;
+
+__device__ __forceinline__ double block_sum_1024(double v, double* red) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double tot = 0.0;
+    for (int w = 0; w < kRedThreads / 64; ++w) tot += red[w];
+    __syncthreads();
+    return tot;
+}
+
+// ---------------------------------------------------------------- tridiagonalization
+// Reflector for column k (read as row k of the symmetric matrix): H = I - tau v v^T with
+// v[0] = 1 maps x = A[k, k+1:n] to beta e_1. v[1:] is stored back into A[k, k+2:n].
+__global__ __launch_bounds__(kRedThreads) void k_house(int n, int k, double* A, double* d, double* e, double* tau,
+                                                       double* v) {
+    __shared__ double red[kRedThreads / 64];
+    const int m = n - k - 1;
+    double* x = A + (size_t)k * n + k + 1;
+    double ss = 0.0;
+    for (int i = 1 + threadIdx.x; i < m; i += kRedThreads) ss += x[i] * x[i];
+    ss = block_sum_1024(ss, red);
+    const double alpha = x[0];
+    double beta, t, scale;
+    if (ss == 0.0) {
+        beta = alpha;
+        t = 0.0;
+        scale = 0.0;
+    } else {
+        beta = -copysign(sqrt(alpha * alpha + ss), alpha);
+        t = (beta - alpha) / beta;
+        scale = 1.0 / (alpha - beta);
+    }
+    for (int i = 1 + threadIdx.x; i < m; i += kRedThreads) {
+        const double vi = x[i] * scale;
+        x[i] = vi;
+        v[i] = vi;
+    }
+    if (threadIdx.x == 0) {
+        v[0] = 1.0;
+        d[k] = A[(size_t)k * n + k];
+        e[k] = beta;
+        tau[k] = t;
+        if (k == n - 2) d[n - 1] = A[(size_t)(n - 1) * n + n - 1];
+    }
+}
+
+// p = tau * A22 v, one wavefront per row of the trailing block A22 = A[k+1:, k+1:].
+__global__ __launch_bounds__(256) void k_symv(int n, int k, const double* A, const double* tau, const double* v,
+                                              double* p) {
+    const int m = n - k - 1;
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= m) return;
+    const double* a = A + (size_t)(k + 1 + row) * n + k + 1;
+    double acc = 0.0;
+    for (int j = lane; j < m; j += 64) acc += a[j] * v[j];
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if (lane == 0) p[row] = tau[k] * acc;
+}
+
+// w = p - (tau/2)(p.v) v, in place over p.
+__global__ __launch_bounds__(kRedThreads) void k_wvec(int n, int k, const double* tau, const double* v, double* p) {
+    __shared__ double red[kRedThreads / 64];
+    const int m = n - k - 1;
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < m; i += kRedThreads) acc += p[i] * v[i];
+    acc = block_sum_1024(acc, red);
+    const double alpha = 0.5 * tau[k] * acc;
+    for (int i = threadIdx.x; i < m; i += kRedThreads) p[i] -= alpha * v[i];
+}
+
+// A22 -= v w^T + w v^T (both triangles).
+__global__ __launch_bounds__(256) void k_rank2(int n, int k, double* A, const double* v, const double* w) {
+    const int m = n - k - 1;
+    const int i = blockIdx.y;
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (i >= m || j >= m) return;
+    A[(size_t)(k + 1 + i) * n + k + 1 + j] -= v[i] * w[j] + w[i] * v[j];
+}
+
+// ---------------------------------------------------------------- divide and conquer
+// Leaves: the tridiagonal block [s, s + m) with the split-modified diagonal, diagonalized by the
+// workgroup Jacobi solver; eigenvectors go to the columns of Q (column-major, ld n) in ascending
+// order of their eigenvalues, eigenvalues to lam[s..s+m).
+__global__ __launch_bounds__(kWg) void k_leaf(int n, const int* leaf_s, const int* leaf_m, const double* d,
+                                              const double* e, double* Q, double* lam, int* status) {
+    __shared__ WgScratch sc;
+    const int s = leaf_s[blockIdx.x], m = leaf_m[blockIdx.x];
+    const int t = threadIdx.x;
+    const int tri = m * (m + 1) / 2;
+    for (int k = t; k < tri; k += kWg) {
+        int i = (int)((sqrt(8.0 * (double)k + 1.0) - 1.0) * 0.5);
+        while (i * (i + 1) / 2 > k) --i;
+        while ((i + 1) * (i + 2) / 2 <= k) ++i;
+        const int j = k - i * (i + 1) / 2;
+        lds_dyn[k] = (i == j) ? d[s + i] : (i == j + 1 ? e[s + j] : 0.0);
+    }
+    for (int idx = t; idx < m * m; idx += kWg) lds_dyn[tri + idx] = (idx / m == idx % m) ? 1.0 : 0.0;
+    __syncthreads();
+    const int sweeps = wg_jacobi_lds(m, tri, m, nullptr, sc, 60, 0.0);
+    __syncthreads();
+    for (int i = t / 16; i < m; i += kWg / 16) {
+        const double ei = lds_dyn[i * (i + 1) / 2 + i];
+        int rank = 0;
+        for (int j = t % 16; j < m; j += 16) {
+            const double ej = lds_dyn[j * (j + 1) / 2 + j];
+            rank += (ej < ei) || (ej == ei && j < i);
+        }
+        for (int off = 8; off > 0; off >>= 1) rank += __shfl_xor(rank, off, 16);
+        if (t % 16 == 0) lam[s + rank] = ei;
+        for (int c = t % 16; c < m; c += 16) Q[(size_t)(s + c) + (size_t)(s + rank) * n] = lds_dyn[tri + i * m + c];
+    }
+    if (t == 0 && sweeps < 0) atomicExch(status, 1);
+}
+
+// z = (last row of Q1, sg * first row of Q2) for the merge of [s, s+m1) and [s+m1, s+m).
+__global__ __launch_bounds__(256) void k_gather_z(int n, const double* Q, int s, int m, int m1, double sg, double* z) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= m) return;
+    z[i] = i < m1 ? Q[(size_t)(s + m1 - 1) + (size_t)(s + i) * n] : sg * Q[(size_t)(s + m1) + (size_t)(s + i) * n];
+}
+
+struct Rot {
+    int a, b;       // local columns
+    double c, s;    // col_a <- c col_a + s col_b ; col_b <- c col_b - s col_a
+};
+
+// Deflation rotations, in order, one thread per row of the merge block.
+__global__ __launch_bounds__(256) void k_givens(int n, double* Q, int s, int m, int nrot, const Rot* rot) {
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= m) return;
+    double* row = Q + (size_t)(s + r);
+    for (int q = 0; q < nrot; ++q) {
+        const Rot g = rot[q];
+        const double x = row[(size_t)(s + g.a) * n], y = row[(size_t)(s + g.b) * n];
+        row[(size_t)(s + g.a) * n] = g.c * x + g.s * y;
+        row[(size_t)(s + g.b) * n] = g.c * y - g.s * x;
+    }
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Secular equation 1 + rho sum_i z_i^2 / (d_i - lambda) = 0 for the roots of D + rho z z^T
+// (rho > 0, d strictly increasing): root j lies in (d_j, d_{j+1}), the last in
+// (d_{k-1}, d_{k-1} + rho |z|^2]. Each root is kept as origin + offset (lambda = d_o + tau with
+// o the nearer pole), so differences to the poles stay accurate. Bisection on the offset, one
+// wavefront per root.
+__global__ __launch_bounds__(256) void k_secular(int k, const double* dk, const double* zk, double rho, double znorm2,
+                                                 int* orig, double* toff) {
+    const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (j >= k) return;
+    auto g = [&](int o, double tau) {
+        double acc = 0.0;
+        for (int i = lane; i < k; i += 64) acc += zk[i] * zk[i] / ((dk[i] - dk[o]) - tau);
+        return 1.0 + rho * wave_sum(acc);
+    };
+    int o;
+    double lo, hi;
+    if (j < k - 1) {
+        const double half = 0.5 * (dk[j + 1] - dk[j]);
+        if (g(j, half) >= 0.0) {
+            o = j;
+            lo = 0.0;
+            hi = half;
+        } else {
+            o = j + 1;
+            lo = -half;
+            hi = 0.0;
+        }
+    } else {
+        o = j;
+        lo = 0.0;
+        hi = rho * znorm2;
+    }
+    for (int it = 0; it < 300; ++it) {
+        const double mid = 0.5 * (lo + hi);
+        if (mid <= lo || mid >= hi) break;
+        if (g(o, mid) > 0.0) hi = mid;
+        else lo = mid;
+    }
+    if (lane == 0) {
+        orig[j] = o;
+        toff[j] = 0.5 * (lo + hi);
+    }
+}
+
+// Loewner: zhat_i^2 = (lambda_{k-1} - d_i)/rho * prod_{j<i} (lambda_j - d_i)/(d_j - d_i)
+//                     * prod_{i<=j<k-1} (lambda_j - d_i)/(d_{j+1} - d_i), with the sign of z_i.
+__global__ __launch_bounds__(256) void k_lowner(int k, const double* dk, const double* zk, double rho, const int* orig,
+                                                const double* toff, double* zhat) {
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (i >= k) return;
+    double prod = 1.0;
+    for (int j = lane; j < k - 1; j += 64) {
+        const double lj = (dk[orig[j]] - dk[i]) + toff[j];
+        prod *= j < i ? lj / (dk[j] - dk[i]) : lj / (dk[j + 1] - dk[i]);
+    }
+    for (int off = 32; off > 0; off >>= 1) prod *= __shfl_xor(prod, off, 64);
+    if (lane == 0) {
+        const double last = ((dk[orig[k - 1]] - dk[i]) + toff[k - 1]) / rho;
+        const double z2 = fabs(prod * last);
+        zhat[i] = copysign(sqrt(z2), zk[i]);
+    }
+}
+
+// Eigenvector j of D + rho zhat zhat^T: u_i = zhat_i / (d_i - lambda_j), normalized; U is k x k
+// column-major.
+__global__ __launch_bounds__(256) void k_build_u(int k, const double* dk, const double* zhat, const int* orig,
+                                                 const double* toff, double* U, double* lamk) {
+    const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (j >= k) return;
+    const int o = orig[j];
+    const double tj = toff[j];
+    double ss = 0.0;
+    for (int i = lane; i < k; i += 64) {
+        const double u = zhat[i] / ((dk[i] - dk[o]) - tj);
+        U[(size_t)i + (size_t)j * k] = u;
+        ss += u * u;
+    }
+    ss = wave_sum(ss);
+    const double inv = 1.0 / sqrt(ss);
+    for (int i = lane; i < k; i += 64) U[(size_t)i + (size_t)j * k] *= inv;
+    if (lane == 0) lamk[j] = dk[o] + tj;
+}
+
+// W[:, j] = Q[s:s+m, s + col[j]] (m x k, column-major).
+__global__ __launch_bounds__(256) void k_gather_cols(int n, const double* Q, int s, int m, int k, const int* col,
+                                                     double* W) {
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    const int j = blockIdx.y;
+    if (r >= m || j >= k) return;
+    W[(size_t)r + (size_t)j * m] = Q[(size_t)(s + r) + (size_t)(s + col[j]) * n];
+}
+
+// Qn[:, c] = src[c] >= 0 ? Qt[:, src[c]] : Q[s:s+m, s + (-1 - src[c])]  (m x m, column-major).
+__global__ __launch_bounds__(256) void k_assemble(int n, const double* Q, const double* Qt, int s, int m,
+                                                  const int* src, double* Qn) {
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    const int c = blockIdx.y;
+    if (r >= m) return;
+    const int sc = src[c];
+    Qn[(size_t)r + (size_t)c * m] = sc >= 0 ? Qt[(size_t)r + (size_t)sc * m] : Q[(size_t)(s + r) + (size_t)(s - 1 - sc) * n];
+}
+
+__global__ __launch_bounds__(256) void k_copy_block(int n, double* Q, int s, int m, const double* Qn) {
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    const int c = blockIdx.y;
+    if (r >= m) return;
+    Q[(size_t)(s + r) + (size_t)(s + c) * n] = Qn[(size_t)r + (size_t)c * m];
+}
+
+// ---------------------------------------------------------------- back-transformation
+// Vb (rows r0 = k0+1 .. n-1, nb columns, column-major): reflector k0+j has 1 at row k0+j+1 and
+// the stored A[k0+j, i] below it.
+__global__ __launch_bounds__(256) void k_build_v(int n, int k0, int nb, const double* A, double* Vb) {
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    const int j = blockIdx.y;
+    const int r0 = k0 + 1, mr = n - r0;
+    if (r >= mr || j >= nb) return;
+    const int row = r0 + r, k = k0 + j;
+    double v = 0.0;
+    if (row == k + 1) v = 1.0;
+    else if (row > k + 1) v = A[(size_t)k * n + row];
+    Vb[(size_t)r + (size_t)j * mr] = v;
+}
+
+// Compact WY factor T (nb x nb upper triangular, column-major) from G = Vb^T Vb (forward,
+// columnwise): T[j][j] = tau_j, T[0:j, j] = -tau_j T[0:j, 0:j] G[0:j, j].
+__global__ __launch_bounds__(256) void k_larft(int nb, int k0, const double* tau, const double* G, double* T) {
+    __shared__ double col[kNb];
+    const int t = threadIdx.x;
+    for (int idx = t; idx < nb * nb; idx += 256) T[idx] = 0.0;
+    __syncthreads();
+    for (int j = 0; j < nb; ++j) {
+        const double tj = tau[k0 + j];
+        if (t < j) {
+            double acc = 0.0;
+            for (int l = t; l < j; ++l) acc += T[t + l * nb] * G[l + j * nb];
+            col[t] = -tj * acc;
+        }
+        __syncthreads();
+        if (t < j) T[t + j * nb] = col[t];
+        if (t == 0) T[j + j * nb] = tj;
+        __syncthreads();
+    }
+}
+
+inline unsigned blocks(int64_t n, int per) { return (unsigned)((n + per - 1) / per); }
+
+#define EIG_TRY(x)                                   \
+    do {                                             \
+        hipError_t e_ = (x);                         \
+        if (e_ != hipSuccess) return e_;             \
+    } while (0)
+#define BLAS_TRY(x)                                  \
+    do {                                             \
+        if ((x) != rocblas_status_success) {         \
+            err = "rocBLAS call failed: " #x;        \
+            return hipErrorUnknown;                  \
+        }                                            \
+    } while (0)
+
+struct Node {
+    int s, m, m1;
+};
+
+void split(int s, int m, std::vector<Node>& merges, std::vector<int>& ls, std::vector<int>& lm) {
+    if (m <= kLeaf) {
+        ls.push_back(s);
+        lm.push_back(m);
+        return;
+    }
+    const int m1 = m / 2;
+    split(s, m1, merges, ls, lm);
+    split(s + m1, m - m1, merges, ls, lm);
+    merges.push_back({s, m, m1});
+}
+
+}  // namespace
+
+hipError_t eig_large(EigWork& w, rocblas_handle blas, int n, double* A, double* evals, double* Ct, hipStream_t st,
+                     std::string& err) {
+    err.clear();
+    if (n < 1) return hipSuccess;
+    const size_t nn = (size_t)n * n;
+    EIG_TRY(w.d.reserve(sizeof(double) * n));
+    EIG_TRY(w.e.reserve(sizeof(double) * n));
+    EIG_TRY(w.tau.reserve(sizeof(double) * n));
+    EIG_TRY(w.vbuf.reserve(sizeof(double) * n));
+    EIG_TRY(w.pbuf.reserve(sizeof(double) * n));
+    EIG_TRY(w.scal.reserve(sizeof(int) * 16));
+    EIG_TRY(w.Q.reserve(sizeof(double) * nn));
+    double* d = w.d.as<double>();
+    double* e = w.e.as<double>();
+    double* tau = w.tau.as<double>();
+    double* v = w.vbuf.as<double>();
+    double* p = w.pbuf.as<double>();
+    double* Q = w.Q.as<double>();
+
+    // ---- 1. tridiagonalization
+    if (n == 1) {
+        EIG_TRY(hipMemcpyAsync(d, A, sizeof(double), hipMemcpyDeviceToDevice, st));
+    }
+    for (int k = 0; k + 1 < n; ++k) {
+        const int m = n - k - 1;
+        hipLaunchKernelGGL(k_house, dim3(1), dim3(kRedThreads), 0, st, n, k, A, d, e, tau, v);
+        if (m > 1) {
+            hipLaunchKernelGGL(k_symv, dim3(blocks(m, 4)), dim3(256), 0, st, n, k, A, tau, v, p);
+            hipLaunchKernelGGL(k_wvec, dim3(1), dim3(kRedThreads), 0, st, n, k, tau, v, p);
+            hipLaunchKernelGGL(k_rank2, dim3(blocks(m, 256), m), dim3(256), 0, st, n, k, A, v, p);
+        }
+    }
+    EIG_TRY(hipGetLastError());
+    w.h_d.resize(n);
+    w.h_e.resize(n);
+    EIG_TRY(hipMemcpyAsync(w.h_d.data(), d, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    if (n > 1) EIG_TRY(hipMemcpyAsync(w.h_e.data(), e, sizeof(double) * (n - 1), hipMemcpyDeviceToHost, st));
+    EIG_TRY(hipStreamSynchronize(st));
+    for (int i = 0; i < n; ++i)
+        if (!std::isfinite(w.h_d[i]) || (i + 1 < n && !std::isfinite(w.h_e[i]))) {
+            err = "symmetric eigensolver: non-finite matrix";
+            return hipErrorUnknown;
+        }
+
+    // ---- 2. tridiagonal divide and conquer
+    std::vector<Node> merges;
+    std::vector<int> ls, lm;
+    split(0, n, merges, ls, lm);
+    std::vector<double> hd = w.h_d;
+    for (const Node& nd : merges) {  // T = diag(T1', T2') + |beta| u u^T
+        const double b = std::fabs(w.h_e[nd.s + nd.m1 - 1]);
+        hd[nd.s + nd.m1 - 1] -= b;
+        hd[nd.s + nd.m1] -= b;
+    }
+    EIG_TRY(hipMemcpyAsync(d, hd.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
+    const int nl = (int)ls.size();
+    EIG_TRY(w.idx.reserve(sizeof(int) * std::max<size_t>(2 * nl + 4 * (size_t)n + 8, 64)));
+    EIG_TRY(w.lam.reserve(sizeof(double) * 4 * (size_t)n));
+    int* ileaf = w.idx.as<int>();
+    double* lam = w.lam.as<double>();
+    EIG_TRY(hipMemcpyAsync(ileaf, ls.data(), sizeof(int) * nl, hipMemcpyHostToDevice, st));
+    EIG_TRY(hipMemcpyAsync(ileaf + nl, lm.data(), sizeof(int) * nl, hipMemcpyHostToDevice, st));
+    EIG_TRY(hipMemsetAsync(Q, 0, sizeof(double) * nn, st));
+    int* status = w.scal.as<int>();
+    EIG_TRY(hipMemsetAsync(status, 0, sizeof(int), st));
+    const size_t leaf_lds = sizeof(double) * (kLeaf * (kLeaf + 1) / 2 + kLeaf * kLeaf);
+    EIG_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_leaf), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)leaf_lds));
+    hipLaunchKernelGGL(k_leaf, dim3(nl), dim3(kWg), leaf_lds, st, n, ileaf, ileaf + nl, d, e, Q, lam, status);
+    EIG_TRY(hipGetLastError());
+    w.h_lam.resize(n);
+    int h_status = 0;
+    EIG_TRY(hipMemcpyAsync(w.h_lam.data(), lam, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    EIG_TRY(hipMemcpyAsync(&h_status, status, sizeof(int), hipMemcpyDeviceToHost, st));
+    EIG_TRY(hipStreamSynchronize(st));
+    if (h_status) {
+        err = "symmetric eigensolver: a leaf block did not converge";
+        return hipErrorUnknown;
+    }
+
+    if (!merges.empty()) {
+        EIG_TRY(w.W.reserve(sizeof(double) * nn));
+        EIG_TRY(w.Qt.reserve(sizeof(double) * nn));
+        EIG_TRY(w.U.reserve(sizeof(double) * nn));
+        EIG_TRY(w.Qn.reserve(sizeof(double) * nn));
+        EIG_TRY(w.rot.reserve(sizeof(Rot) * n));
+        EIG_TRY(w.flt.reserve(sizeof(double) * 6 * (size_t)n));
+    }
+    const double eps = DBL_EPSILON;
+    std::vector<double> hz, ds, zs, dk, zk, lamk;
+    std::vector<int> perm, kept, src;
+    std::vector<Rot> rots;
+    std::vector<std::pair<double, int>> fin;
+    for (const Node& nd : merges) {
+        const int s = nd.s, m = nd.m, m1 = nd.m1;
+        const double beta = w.h_e[s + m1 - 1];
+        const double rho = std::fabs(beta);
+        double* zbuf = w.flt.as<double>();
+        hipLaunchKernelGGL(k_gather_z, dim3(blocks(m, 256)), dim3(256), 0, st, n, Q, s, m, m1, beta >= 0.0 ? 1.0 : -1.0,
+                           zbuf);
+        hz.resize(m);
+        EIG_TRY(hipMemcpyAsync(hz.data(), zbuf, sizeof(double) * m, hipMemcpyDeviceToHost, st));
+        EIG_TRY(hipStreamSynchronize(st));
+        double z2 = 0.0;
+        for (double x : hz) z2 += x * x;
+        const double zn = std::sqrt(z2);
+        const double rho_eff = rho * z2;
+        // sort the children's eigenvalues (each child's list is already ascending)
+        perm.resize(m);
+        std::iota(perm.begin(), perm.end(), 0);
+        std::stable_sort(perm.begin(), perm.end(),
+                         [&](int a, int b) { return w.h_lam[s + a] < w.h_lam[s + b]; });
+        ds.resize(m);
+        zs.resize(m);
+        double dmax = 0.0, zmax = 0.0;
+        for (int i = 0; i < m; ++i) {
+            ds[i] = w.h_lam[s + perm[i]];
+            zs[i] = zn > 0.0 ? hz[perm[i]] / zn : 0.0;
+            dmax = std::max(dmax, std::fabs(ds[i]));
+            zmax = std::max(zmax, std::fabs(zs[i]));
+        }
+        const double tol = 8.0 * eps * std::max(dmax, zmax);
+        // deflation scan (small z components; nearly equal poles combined by a rotation)
+        kept.clear();
+        rots.clear();
+        fin.clear();
+        int pj = -1;
+        for (int j = 0; j < m; ++j) {
+            if (rho_eff * std::fabs(zs[j]) <= tol) {
+                fin.push_back({ds[j], -1 - perm[j]});
+                continue;
+            }
+            if (pj < 0) {
+                pj = j;
+                continue;
+            }
+            double sv = zs[pj], cv = zs[j];
+            const double tv = std::hypot(cv, sv);
+            const double gap = ds[j] - ds[pj];
+            cv /= tv;
+            sv = -sv / tv;
+            if (std::fabs(gap * cv * sv) <= tol) {
+                zs[j] = tv;
+                zs[pj] = 0.0;
+                rots.push_back({perm[pj], perm[j], cv, sv});
+                const double t1 = ds[pj] * cv * cv + ds[j] * sv * sv;
+                ds[j] = ds[pj] * sv * sv + ds[j] * cv * cv;
+                ds[pj] = t1;
+                fin.push_back({ds[pj], -1 - perm[pj]});
+                pj = j;
+            } else {
+                kept.push_back(pj);
+                pj = j;
+            }
+        }
+        if (pj >= 0) kept.push_back(pj);
+        if (!rots.empty()) {
+            Rot* drot = w.rot.as<Rot>();
+            EIG_TRY(hipMemcpyAsync(drot, rots.data(), sizeof(Rot) * rots.size(), hipMemcpyHostToDevice, st));
+            hipLaunchKernelGGL(k_givens, dim3(blocks(m, 256)), dim3(256), 0, st, n, Q, s, m, (int)rots.size(), drot);
+        }
+        const int k = (int)kept.size();
+        if (k > 0) {
+            dk.resize(k);
+            zk.resize(k);
+            std::vector<int> colk(k);
+            double zk2 = 0.0;
+            for (int j = 0; j < k; ++j) {
+                dk[j] = ds[kept[j]];
+                zk[j] = zs[kept[j]];
+                colk[j] = perm[kept[j]];
+                zk2 += zk[j] * zk[j];
+            }
+            double* f = w.flt.as<double>();
+            double* ddk = f;
+            double* dzk = f + n;
+            double* dtoff = f + 2 * (size_t)n;
+            double* dzhat = f + 3 * (size_t)n;
+            double* dlamk = f + 4 * (size_t)n;
+            int* iorig = w.idx.as<int>() + 2 * nl;
+            int* icol = iorig + n;
+            int* isrc = icol + n;
+            EIG_TRY(hipMemcpyAsync(ddk, dk.data(), sizeof(double) * k, hipMemcpyHostToDevice, st));
+            EIG_TRY(hipMemcpyAsync(dzk, zk.data(), sizeof(double) * k, hipMemcpyHostToDevice, st));
+            EIG_TRY(hipMemcpyAsync(icol, colk.data(), sizeof(int) * k, hipMemcpyHostToDevice, st));
+            hipLaunchKernelGGL(k_secular, dim3(blocks(k, 4)), dim3(256), 0, st, k, ddk, dzk, rho_eff, zk2, iorig, dtoff);
+            hipLaunchKernelGGL(k_lowner, dim3(blocks(k, 4)), dim3(256), 0, st, k, ddk, dzk, rho_eff, iorig, dtoff,
+                               dzhat);
+            double* U = w.U.as<double>();
+            hipLaunchKernelGGL(k_build_u, dim3(blocks(k, 4)), dim3(256), 0, st, k, ddk, dzhat, iorig, dtoff, U, dlamk);
+            double* W = w.W.as<double>();
+            double* Qt = w.Qt.as<double>();
+            hipLaunchKernelGGL(k_gather_cols, dim3(blocks(m, 256), k), dim3(256), 0, st, n, Q, s, m, k, icol, W);
+            EIG_TRY(hipGetLastError());
+            const double one = 1.0, zero = 0.0;
+            BLAS_TRY(rocblas_dgemm(blas, rocblas_operation_none, rocblas_operation_none, m, k, k, &one, W, m, U, k,
+                                   &zero, Qt, m));
+            lamk.resize(k);
+            EIG_TRY(hipMemcpyAsync(lamk.data(), dlamk, sizeof(double) * k, hipMemcpyDeviceToHost, st));
+            EIG_TRY(hipStreamSynchronize(st));
+            for (int j = 0; j < k; ++j) {
+                if (!std::isfinite(lamk[j])) {
+                    err = "symmetric eigensolver: secular equation failed";
+                    return hipErrorUnknown;
+                }
+                fin.push_back({lamk[j], j});
+            }
+            std::stable_sort(fin.begin(), fin.end(),
+                             [](const std::pair<double, int>& a, const std::pair<double, int>& b) {
+                                 return a.first < b.first;
+                             });
+            src.resize(m);
+            for (int c = 0; c < m; ++c) {
+                src[c] = fin[c].second;
+                w.h_lam[s + c] = fin[c].first;
+            }
+            EIG_TRY(hipMemcpyAsync(isrc, src.data(), sizeof(int) * m, hipMemcpyHostToDevice, st));
+            double* Qn = w.Qn.as<double>();
+            hipLaunchKernelGGL(k_assemble, dim3(blocks(m, 256), m), dim3(256), 0, st, n, Q, Qt, s, m, isrc, Qn);
+            hipLaunchKernelGGL(k_copy_block, dim3(blocks(m, 256), m), dim3(256), 0, st, n, Q, s, m, Qn);
+            EIG_TRY(hipGetLastError());
+        } else {
+            // fully deflated: the block is already diagonal in the rotated columns; order them
+            std::stable_sort(fin.begin(), fin.end(),
+                             [](const std::pair<double, int>& a, const std::pair<double, int>& b) {
+                                 return a.first < b.first;
+                             });
+            src.resize(m);
+            for (int c = 0; c < m; ++c) {
+                src[c] = fin[c].second;
+                w.h_lam[s + c] = fin[c].first;
+            }
+            int* isrc = w.idx.as<int>() + 2 * nl + 2 * n;
+            EIG_TRY(hipMemcpyAsync(isrc, src.data(), sizeof(int) * m, hipMemcpyHostToDevice, st));
+            double* Qn = w.Qn.as<double>();
+            hipLaunchKernelGGL(k_assemble, dim3(blocks(m, 256), m), dim3(256), 0, st, n, Q, Q, s, m, isrc, Qn);
+            hipLaunchKernelGGL(k_copy_block, dim3(blocks(m, 256), m), dim3(256), 0, st, n, Q, s, m, Qn);
+            EIG_TRY(hipGetLastError());
+            EIG_TRY(hipStreamSynchronize(st));
+        }
+    }
+
+    // ---- 3. back-transformation: Z <- H_0 ... H_{n-3} Z, last block first
+    if (n > 2) {
+        EIG_TRY(w.V.reserve(sizeof(double) * (size_t)n * kNb));
+        EIG_TRY(w.Tm.reserve(sizeof(double) * kNb * kNb));
+        EIG_TRY(w.G.reserve(sizeof(double) * kNb * kNb));
+        EIG_TRY(w.X.reserve(sizeof(double) * (size_t)n * kNb));
+        EIG_TRY(w.Y.reserve(sizeof(double) * (size_t)n * kNb));
+        const int nref = n - 2;  // reflectors 0 .. n-3 (the one of column n-2 is the identity)
+        const int nblk = (nref + kNb - 1) / kNb;
+        double* Vb = w.V.as<double>();
+        double* T = w.Tm.as<double>();
+        double* G = w.G.as<double>();
+        double* X = w.X.as<double>();
+        double* Y = w.Y.as<double>();
+        const double one = 1.0, zero = 0.0, mone = -1.0;
+        for (int b = nblk - 1; b >= 0; --b) {
+            const int k0 = b * kNb;
+            const int nb = std::min(kNb, nref - k0);
+            const int r0 = k0 + 1, mr = n - r0;
+            hipLaunchKernelGGL(k_build_v, dim3(blocks(mr, 256), nb), dim3(256), 0, st, n, k0, nb, A, Vb);
+            EIG_TRY(hipGetLastError());
+            BLAS_TRY(rocblas_dgemm(blas, rocblas_operation_transpose, rocblas_operation_none, nb, nb, mr, &one, Vb, mr,
+                                   Vb, mr, &zero, G, nb));
+            hipLaunchKernelGGL(k_larft, dim3(1), dim3(256), 0, st, nb, k0, tau, G, T);
+            EIG_TRY(hipGetLastError());
+            BLAS_TRY(rocblas_dgemm(blas, rocblas_operation_transpose, rocblas_operation_none, nb, n, mr, &one, Vb, mr,
+                                   Q + r0, n, &zero, X, nb));
+            BLAS_TRY(rocblas_dgemm(blas, rocblas_operation_none, rocblas_operation_none, nb, n, nb, &one, T, nb, X, nb,
+                                   &zero, Y, nb));
+            BLAS_TRY(rocblas_dgemm(blas, rocblas_operation_none, rocblas_operation_none, mr, n, nb, &mone, Vb, mr, Y,
+                                   nb, &one, Q + r0, n));
+        }
+    }
+    EIG_TRY(hipMemcpyAsync(evals, w.h_lam.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
+    EIG_TRY(hipMemcpyAsync(Ct, Q, sizeof(double) * nn, hipMemcpyDeviceToDevice, st));
+    EIG_TRY(hipStreamSynchronize(st));
+    return hipSuccess;
+}
+
+}  // namespace am1g

@@ -1,0 +1,51 @@
+// Dense symmetric eigensolver for large orders (one matrix on the whole device): Householder
+// tridiagonalization, tridiagonal divide-and-conquer, blocked back-transformation. Internal
+// header of the library.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+
+#include <string>
+#include <vector>
+
+struct EigBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    hipError_t reserve(size_t need) {
+        if (need <= bytes) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        hipError_t e = hipMalloc(&p, need < 4096 ? 4096 : need);
+        if (e == hipSuccess) bytes = need < 4096 ? 4096 : need;
+        return e;
+    }
+    template <typename T>
+    T* as() const {
+        return static_cast<T*>(p);
+    }
+    ~EigBuf() {
+        if (p) (void)hipFree(p);
+    }
+};
+
+// Device and host workspace, grown on demand and kept between calls.
+struct EigWork {
+    EigBuf d, e, tau, vbuf, pbuf, scal;       // tridiagonal and Householder vectors
+    EigBuf Q;                                 // n x n column-major: eigenvectors of T, then of A
+    EigBuf W, Qt, U, Qn;                      // merge scratch
+    EigBuf V, Tm, X, Y, G;                    // back-transformation blocks
+    EigBuf rot, idx, flt, lam;                // merge tables
+    std::vector<double> h_d, h_e, h_lam;
+};
+
+namespace am1g {
+
+// Eigen-decomposition of the symmetric n x n matrix A (row-major; overwritten). On return
+// evals[0..n) holds the eigenvalues in ascending order and Ct (row-major n x n) holds the
+// eigenvectors as rows (row k = eigenvector of evals[k]). Returns hipSuccess or the first
+// device error; `err` names a numerical failure (then the return is hipErrorUnknown).
+hipError_t eig_large(EigWork& w, rocblas_handle blas, int n, double* A, double* evals, double* Ct, hipStream_t st,
+                     std::string& err);
+
+}  // namespace am1g

@@ -178,6 +178,13 @@ int am1g_serve_cancel(am1g_server* sv, long long ticket);
 int am1g_serve_get_stats(am1g_server* sv, am1g_serve_stats* out);
 const char* am1g_serve_last_error(const am1g_server* sv);
 
+/* Dense symmetric eigensolver on the device: symmetric_eigen (src/linalg.rs:276-331). `a` is
+ * n x n row-major and symmetric; eigenvalues come out ascending, eigenvectors as the columns of
+ * the row-major n x n `eigenvectors`. Non-finite input is AM1G_LINEAR_ALGEBRA, as in the
+ * reference. Householder tridiagonalization, tridiagonal divide and conquer and a blocked
+ * back-transformation; the same solver serves the SCF of systems above 256 basis functions. */
+int am1g_symmetric_eigen(am1g_ctx* ctx, int n, const double* a, double* eigenvalues, double* eigenvectors);
+
 /* Text of the last error on this context (the Display text of the matching Am1Error). */
 const char* am1g_last_error(const am1g_ctx* ctx);
 
