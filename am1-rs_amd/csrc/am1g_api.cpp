@@ -9,6 +9,8 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
+#include <climits>
 #include <cstring>
 #include <mutex>
 #include <numeric>
@@ -145,7 +147,9 @@ int am1g_prepare(am1g_ctx* ctx, const am1g_request& rq, Prepared& out, std::stri
 
 namespace {
 
-void fill_result(const MolResult& r, const MolDesc& md, am1g_result& out) {
+}  // namespace
+
+void am1g_fill_result(const MolResult& r, const MolDesc& md, am1g_result& out) {
     out.status = r.status;
     out.nao = md.nao;
     out.n_occ = md.n_alpha;
@@ -170,8 +174,6 @@ void fill_result(const MolResult& r, const MolDesc& md, am1g_result& out) {
     out.residual = r.residual;
     out.fp32_iterations = (int)r.fp32_iterations;
 }
-
-}  // namespace
 
 extern "C" {
 
@@ -395,7 +397,7 @@ int am1g_scf_batch(am1g_ctx* ctx, int n, const am1g_request* rqs, am1g_result* r
         const int i = ok_idx[j];
         const MolDesc& md = descs[j];
         am1g_result& r = results[i];
-        fill_result(res[j], md, r);
+        am1g_fill_result(res[j], md, r);
         r.latency_ms = (double)(res[j].t_done - t0) / 1.0e5;  // 100 MHz device clock
         for (int k = 0; k < kPhases; ++k) r.phase_ms[k] = (double)res[j].phase[k] / 1.0e5;
         r.eigen_sweeps = res[j].eigen_sweeps;
@@ -434,7 +436,28 @@ int am1g_scf_batch(am1g_ctx* ctx, int n, const am1g_request* rqs, am1g_result* r
 int am1g_scf(am1g_ctx* ctx, const am1g_request* rq, am1g_result* res) {
     if (!ctx || !rq || !res) return am1g_set_error(ctx, AM1G_INVALID_INPUT, "null argument");
     auto w0 = std::chrono::steady_clock::now();
-    int st = am1g_scf_batch(ctx, 1, rq, res);
+    // Molecules beyond one workgroup (more than 256 atoms or basis functions) take the
+    // large-system path; AM1G_FORCE_LARGE_PATH=1 sends every molecule there (for testing it on
+    // sizes the oracle finishes quickly).
+    Prepared prep;
+    std::string msg;
+    int st = am1g_prepare(ctx, *rq, prep, msg, INT_MAX);
+    if (st != AM1G_OK) {
+        res->status = st;
+        res->iterations = 0;
+        res->converged = 0;
+        return am1g_set_error(ctx, st, "%s", msg.c_str());
+    }
+    const char* force = std::getenv("AM1G_FORCE_LARGE_PATH");
+    if (prep.md.nao > kMaxBatchAo || rq->n_atoms > kMaxAtoms || (force && force[0] == '1')) {
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        HIP_TRY(ctx, hipSetDevice(ctx->device));
+        st = am1g_large_scf(ctx, *rq, prep, res);
+        res->latency_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
+        ctx->last_ms = res->latency_ms;
+        return st;
+    }
+    st = am1g_scf_batch(ctx, 1, rq, res);
     res->latency_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
     if (st != AM1G_OK) return st;
     return res->status;

@@ -71,9 +71,13 @@ struct Prepared {
 int am1g_set_error(am1g_ctx* ctx, int code, const char* fmt, ...);
 int am1g_prepare(am1g_ctx* ctx, const am1g_request& rq, Prepared& out, std::string& msg, int max_ao);
 void am1g_dc_release(am1g_ctx* ctx);
+// Scalars of a device-side MolResult into the C-ABI result.
+void am1g_fill_result(const MolResult& r, const MolDesc& md, am1g_result& out);
 void am1g_large_release(am1g_ctx* ctx);
 // Creates the context's rocBLAS handle and eigensolver workspace on first use.
 int am1g_large_setup(am1g_ctx* ctx);
+// Restricted SCF of one molecule on the whole device (no size limit beyond device memory).
+int am1g_large_scf(am1g_ctx* ctx, const am1g_request& rq, const Prepared& prep, am1g_result* res);
 
 #define HIP_TRY(ctx, expr)                                                                                    \
     do {                                                                                                      \

@@ -12,10 +12,13 @@
 #include <vector>
 
 #include "am1g_ctx.hpp"
+#include "am1g_wg.hpp"
 
 // Device buffers of the large-system SCF, kept on the context so repeated calls reuse them.
 struct LargeBuffers {
-    DevBuf A, evals, Ct;
+    DevBuf A, evals, Ct;                       // eigensolver entry point
+    DevBuf H, F, P, T0, T1, Pn, f32a, f32b, f32c;  // dense n x n (FP32 staging for mixed precision)
+    DevBuf hist_f, hist_e, hist_d, ring, coef, partial, scal, pdiag, props;
 };
 
 int am1g_large_setup(am1g_ctx* ctx) {
@@ -39,6 +42,372 @@ void am1g_large_release(am1g_ctx* ctx) {
     ctx->eig = nullptr;
     if (ctx->blas) (void)rocblas_destroy_handle(ctx->blas);
     ctx->blas = nullptr;
+}
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+double ms_since(Clock::time_point& t) {
+    const auto now = Clock::now();
+    const double d = std::chrono::duration<double, std::milli>(now - t).count();
+    t = now;
+    return d;
+}
+
+}  // namespace
+
+// Restricted SCF of one molecule on the whole device (rhf_loop, scf.rs:557-716): grid Fock
+// kernels, library GEMMs for the commutator and the density (FP32 while the commutator norm is
+// above 1e-4 in mixed precision), packed DIIS histories on the device with the small CDIIS and
+// A-DIIS solves on the host, and the large-order eigensolver.
+int am1g_large_scf(am1g_ctx* ctx, const am1g_request& rq, const Prepared& prep, am1g_result* res) {
+    const MolDesc& md0 = prep.md;
+    if (prep.uhf)
+        return res->status = am1g_set_error(ctx, AM1G_INVALID_INPUT,
+                                            "unrestricted SCF above 256 atoms or basis functions is not available "
+                                            "on the GPU path");
+    int st = am1g_large_setup(ctx);
+    if (st != AM1G_OK) return res->status = st;
+    LargeBuffers& L = *ctx->large;
+    hipStream_t s = ctx->stream;
+    MolDesc md = md0;
+    const int nat = md.nat, n = md.nao, nocc = md.n_alpha;
+    const size_t nn = (size_t)n * n;
+    const DevElem* tab = ctx->host_tab + md.method * kMaxZ;
+    auto t_all = Clock::now(), t_ph = t_all;
+    double phase[6] = {0, 0, 0, 0, 0, 0};
+
+    // ---- atom tables, pair integrals, H_core, core-core energy
+    std::vector<int> zs(nat), alist, aoff(nat), acls(nat);
+    {
+        int off = 0, nh = 0, nH = 0;
+        std::vector<int> heavy, hyd;
+        for (int a = 0; a < nat; ++a) {
+            zs[a] = rq.z[a];
+            aoff[a] = off;
+            off += tab[zs[a]].norb;
+            if (zs[a] != 1) {
+                acls[a] = nh++;
+                heavy.push_back(a);
+            } else {
+                acls[a] = nH++;
+                hyd.push_back(a);
+            }
+        }
+        alist = heavy;
+        alist.insert(alist.end(), hyd.begin(), hyd.end());
+    }
+    const int64_t npairs = (int64_t)nat * (nat - 1) / 2;
+    std::vector<int64_t> pair_off = {0, npairs};
+    HIP_TRY(ctx, ctx->mols.reserve(sizeof(MolDesc)));
+    HIP_TRY(ctx, ctx->z.reserve(sizeof(int) * nat));
+    HIP_TRY(ctx, ctx->xyz.reserve(sizeof(double) * 3 * nat));
+    HIP_TRY(ctx, ctx->alist.reserve(sizeof(int) * nat));
+    HIP_TRY(ctx, ctx->aoff.reserve(sizeof(int) * nat));
+    HIP_TRY(ctx, ctx->acls.reserve(sizeof(int) * nat));
+    HIP_TRY(ctx, ctx->pair_off.reserve(sizeof(int64_t) * 2));
+    HIP_TRY(ctx, ctx->ecore_pair.reserve(sizeof(double) * std::max<int64_t>(npairs, 1)));
+    HIP_TRY(ctx, ctx->wpool.reserve(sizeof(double) * std::max<int64_t>(w_pool_size(md.n_heavy, md.n_h), 1)));
+    HIP_TRY(ctx, L.H.reserve(sizeof(double) * nn));
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->mols.p, &md, sizeof(MolDesc), hipMemcpyHostToDevice, s));
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->z.p, zs.data(), sizeof(int) * nat, hipMemcpyHostToDevice, s));
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->xyz.p, rq.xyz_bohr, sizeof(double) * 3 * nat, hipMemcpyHostToDevice, s));
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->alist.p, alist.data(), sizeof(int) * nat, hipMemcpyHostToDevice, s));
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->aoff.p, aoff.data(), sizeof(int) * nat, hipMemcpyHostToDevice, s));
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->acls.p, acls.data(), sizeof(int) * nat, hipMemcpyHostToDevice, s));
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->pair_off.p, pair_off.data(), sizeof(int64_t) * 2, hipMemcpyHostToDevice, s));
+    BatchArgs ba;
+    std::memset(&ba, 0, sizeof(ba));
+    ba.mols = ctx->mols.as<MolDesc>();
+    ba.n_mols = 1;
+    ba.z = ctx->z.as<int>();
+    ba.xyz = ctx->xyz.as<double>();
+    ba.elems = ctx->elems.as<DevElem>();
+    ba.mats = L.H.as<double>();  // kH == 0: H_core at the start
+    ba.wpool = ctx->wpool.as<double>();
+    ba.alist = ctx->alist.as<int>();
+    ba.aoff = ctx->aoff.as<int>();
+    ba.acls = ctx->acls.as<int>();
+    ba.pair_off = ctx->pair_off.as<int64_t>();
+    ba.n_pairs = npairs;
+    ba.ecore_pair = ctx->ecore_pair.as<double>();
+    HIP_TRY(ctx, L.pdiag.reserve(sizeof(double) * 10 * nat));
+    SysArgs sa;
+    sa.md = md;
+    sa.z = ba.z;
+    sa.xyz = ba.xyz;
+    sa.alist = ba.alist;
+    sa.aoff = ba.aoff;
+    sa.acls = ba.acls;
+    sa.elems = ba.elems;
+    sa.wpool = ba.wpool;
+    sa.H = L.H.as<double>();
+    sa.pdiag = L.pdiag.as<double>();
+
+    const bool need_d = md.accel == 2;
+    const int64_t len_s = (int64_t)n * (n + 1) / 2, len_a = (int64_t)n * (n - 1) / 2;
+    HIP_TRY(ctx, L.F.reserve(sizeof(double) * nn));
+    HIP_TRY(ctx, L.P.reserve(sizeof(double) * nn));
+    HIP_TRY(ctx, L.T0.reserve(sizeof(double) * nn));
+    HIP_TRY(ctx, L.T1.reserve(sizeof(double) * nn));
+    HIP_TRY(ctx, L.Pn.reserve(sizeof(double) * nn));
+    HIP_TRY(ctx, L.Ct.reserve(sizeof(double) * nn));
+    HIP_TRY(ctx, L.evals.reserve(sizeof(double) * n));
+    HIP_TRY(ctx, L.hist_f.reserve(sizeof(double) * kDiisDepth * len_s));
+    HIP_TRY(ctx, L.hist_e.reserve(sizeof(double) * kDiisDepth * std::max<int64_t>(len_a, 1)));
+    if (need_d) HIP_TRY(ctx, L.hist_d.reserve(sizeof(double) * kDiisDepth * len_s));
+    HIP_TRY(ctx, L.ring.reserve(sizeof(int) * kDiisDepth));
+    HIP_TRY(ctx, L.coef.reserve(sizeof(double) * kDiisDepth));
+    HIP_TRY(ctx, L.partial.reserve(sizeof(double) * std::max<int64_t>(large_partials(n), 4096)));
+    HIP_TRY(ctx, L.scal.reserve(sizeof(double) * 64));
+    HIP_TRY(ctx, L.props.reserve(sizeof(double) * 4 * nat));
+    if (md.precision == 1) {
+        HIP_TRY(ctx, L.f32a.reserve(sizeof(float) * nn));
+        HIP_TRY(ctx, L.f32b.reserve(sizeof(float) * nn));
+        HIP_TRY(ctx, L.f32c.reserve(sizeof(float) * nn));
+    }
+    double* H = L.H.as<double>();
+    double* F = L.F.as<double>();
+    double* P = L.P.as<double>();
+    double* T0 = L.T0.as<double>();
+    double* T1 = L.T1.as<double>();
+    double* Pn = L.Pn.as<double>();
+    double* Ct = L.Ct.as<double>();
+    double* eps = L.evals.as<double>();
+    double* hf = L.hist_f.as<double>();
+    double* he = L.hist_e.as<double>();
+    double* hd = need_d ? L.hist_d.as<double>() : nullptr;
+    double* partial = L.partial.as<double>();
+    double* scal = L.scal.as<double>();
+    int* dring = L.ring.as<int>();
+    double* dcoef = L.coef.as<double>();
+    // scal layout: 0 sum P(H+F), 1 sum E^2, 2..25 DIIS dots, 26 non-finite count, 27 sum dP^2, 30 E_core
+    HIP_TRY(ctx, launch_pairs(ba, s));
+    HIP_TRY(ctx, launch_hcore_diag(sa, s));
+    HIP_TRY(ctx, launch_sum(npairs, ba.ecore_pair, partial, scal + 30, s));
+    HIP_TRY(ctx, hipMemsetAsync(P, 0, sizeof(double) * nn, s));
+    HIP_TRY(ctx, launch_sad(sa, 1.0, P, s));
+    double h_scal[64];
+    HIP_TRY(ctx, hipMemcpyAsync(h_scal, scal, sizeof(double) * 64, hipMemcpyDeviceToHost, s));
+    HIP_TRY(ctx, hipStreamSynchronize(s));
+    const double ecore = h_scal[30];
+    phase[0] += ms_since(t_ph);
+
+    // ---- SCF loop
+    double B[kDiisDepth * kDiisDepth] = {0}, G[kDiisDepth * kDiisDepth] = {0};
+    int hcount = 0, head = 0, ring[kDiisDepth];
+    double e_old = 0.0, residual = INFINITY;
+    int iterations = 0, converged = 0, status = AM1G_OK, fp32_iters = 0;
+    // Mixed precision: FP32 GEMMs until the commutator norm drops below 1e-4, or until the norm
+    // measured in FP32 stops improving (it has a rounding floor of about eps_32 |F||P|, above
+    // 1e-4 for thousands of basis functions); then FP64 for the rest of the SCF.
+    bool fp32_phase = md.precision == 1;
+    double fp32_best = INFINITY;
+    int fp32_stall = 0;
+    const double one = 1.0, zero = 0.0, two = 2.0;
+    const float onef = 1.0f, zerof = 0.0f, twof = 2.0f;
+    std::string eig_err;
+    for (int iter = 0; iter < md.max_scf; ++iter) {
+        iterations = iter + 1;
+        HIP_TRY(ctx, launch_fock(sa, P, P, 0.5, F, s));
+        HIP_TRY(ctx, launch_dot3((int64_t)nn, P, H, F, partial, scal + 0, s));
+        phase[1] += ms_since(t_ph);
+        const bool low = fp32_phase;
+        // commutator FP - PF: column-major P*F read row-major is F*P
+        const void* Tc = T0;
+        if (low) {
+            float* Ff = L.f32a.as<float>();
+            float* Pf = L.f32b.as<float>();
+            float* Tf = L.f32c.as<float>();
+            HIP_TRY(ctx, launch_to_f32((int64_t)nn, F, Ff, s));
+            HIP_TRY(ctx, launch_to_f32((int64_t)nn, P, Pf, s));
+            if (rocblas_sgemm(ctx->blas, rocblas_operation_none, rocblas_operation_none, n, n, n, &onef, Pf, n, Ff, n,
+                              &zerof, Tf, n) != rocblas_status_success)
+                return res->status = am1g_set_error(ctx, AM1G_DEVICE_ERROR, "rocBLAS sgemm failed");
+            Tc = Tf;
+            ++fp32_iters;
+        } else if (rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, P, n, F, n,
+                                 &zero, T0, n) != rocblas_status_success) {
+            return res->status = am1g_set_error(ctx, AM1G_DEVICE_ERROR, "rocBLAS dgemm failed");
+        }
+        const int slot = hcount < kDiisDepth ? (head + hcount) % kDiisDepth : head;
+        HIP_TRY(ctx, launch_push(n, Tc, low, F, P, hf + slot * len_s, he + slot * len_a, hd ? hd + slot * len_s : nullptr,
+                                 partial, scal + 1, s));
+        if (hcount < kDiisDepth) {
+            hcount++;
+        } else {
+            head = (head + 1) % kDiisDepth;
+            for (int i = 0; i < kDiisDepth - 1; ++i)
+                for (int j = 0; j < kDiisDepth - 1; ++j) {
+                    B[i * kDiisDepth + j] = B[(i + 1) * kDiisDepth + j + 1];
+                    G[i * kDiisDepth + j] = G[(i + 1) * kDiisDepth + j + 1];
+                }
+        }
+        for (int i = 0; i < hcount; ++i) ring[i] = (head + i) % kDiisDepth;
+        const int m = hcount, newest = m - 1;
+        HIP_TRY(ctx, hipMemcpyAsync(dring, ring, sizeof(int) * m, hipMemcpyHostToDevice, s));
+        HIP_TRY(ctx, launch_diis_dots(n, hf, he, hd, dring, m, newest, partial, scal + 2, s));
+        HIP_TRY(ctx, hipMemcpyAsync(h_scal, scal, sizeof(double) * 30, hipMemcpyDeviceToHost, s));
+        HIP_TRY(ctx, hipStreamSynchronize(s));
+        const double e_elec = 0.5 * h_scal[0];
+        const double err = std::sqrt(2.0 * h_scal[1]);
+        for (int i = 0; i < m; ++i) {
+            B[newest * kDiisDepth + i] = 2.0 * h_scal[2 + i];
+            B[i * kDiisDepth + newest] = 2.0 * h_scal[2 + i];
+        }
+        if (need_d)
+            for (int j = 0; j < m; ++j) {
+                G[newest * kDiisDepth + j] = h_scal[2 + kDiisDepth + j];
+                G[j * kDiisDepth + newest] = h_scal[2 + 2 * kDiisDepth + j];
+            }
+        if (!std::isfinite(e_elec) || !std::isfinite(err)) {
+            status = AM1G_SCF_DIVERGED;
+            break;
+        }
+        // extrapolation coefficients (scf.rs:627-653)
+        bool use = false;
+        double c[kDiisDepth + 1];
+        if (md.accel == 1 || (md.accel == 2 && err <= md.adiis_switch)) {
+            if (m >= 2) {
+                double Bm[(kDiisDepth + 1) * (kDiisDepth + 1)], rhs[kDiisDepth + 1];
+                const int dim = m + 1;
+                for (int i = 0; i < m; ++i) {
+                    for (int j = 0; j < m; ++j) Bm[i * dim + j] = B[i * kDiisDepth + j];
+                    Bm[i * dim + m] = -1.0;
+                    Bm[m * dim + i] = -1.0;
+                    rhs[i] = 0.0;
+                }
+                Bm[m * dim + m] = 0.0;
+                rhs[m] = -1.0;
+                if (solve_bordered(Bm, rhs, dim, c)) {
+                    double wsum = 0.0;
+                    for (int i = 0; i < m; ++i) wsum += std::fabs(c[i]);
+                    use = std::isfinite(wsum);
+                }
+            }
+        } else if (md.accel == 2 && m >= 2) {
+            adiis_solve(G, m, c);
+            use = true;
+        }
+        double* Fuse = F;
+        if (use) {
+            HIP_TRY(ctx, hipMemcpyAsync(dcoef, c, sizeof(double) * m, hipMemcpyHostToDevice, s));
+            HIP_TRY(ctx, launch_combine(n, hf, dring, dcoef, m, T1, partial, scal + 26, s));
+            double bad = 0.0;
+            HIP_TRY(ctx, hipMemcpyAsync(&bad, scal + 26, sizeof(double), hipMemcpyDeviceToHost, s));
+            HIP_TRY(ctx, hipStreamSynchronize(s));
+            if (bad == 0.0) Fuse = T1;
+        }
+        phase[2] += ms_since(t_ph);
+        // eigensolve (destroys Fuse; F is rebuilt by the next Fock pass)
+        hipError_t e = eig_large(*ctx->eig, ctx->blas, n, Fuse, eps, Ct, s, eig_err);
+        if (e != hipSuccess) {
+            if (!eig_err.empty()) {
+                status = AM1G_LINEAR_ALGEBRA;
+                break;
+            }
+            return res->status = am1g_set_error(ctx, AM1G_DEVICE_ERROR, "HIP error %s in the eigensolver",
+                                                hipGetErrorString(e));
+        }
+        phase[3] += ms_since(t_ph);
+        // density 2 C_occ C_occ^T: rows 0..nocc-1 of Ct are the occupied orbitals (column-major n x nocc)
+        if (nocc > 0) {
+            if (low) {
+                float* Cf = L.f32a.as<float>();
+                float* Pf = L.f32b.as<float>();
+                HIP_TRY(ctx, launch_to_f32((int64_t)n * nocc, Ct, Cf, s));
+                if (rocblas_sgemm(ctx->blas, rocblas_operation_none, rocblas_operation_transpose, n, n, nocc, &twof, Cf,
+                                  n, Cf, n, &zerof, Pf, n) != rocblas_status_success)
+                    return res->status = am1g_set_error(ctx, AM1G_DEVICE_ERROR, "rocBLAS sgemm failed");
+                HIP_TRY(ctx, launch_density_update(n, Pf, true, P, partial, scal + 27, s));
+            } else {
+                if (rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_transpose, n, n, nocc, &two, Ct, n,
+                                  Ct, n, &zero, Pn, n) != rocblas_status_success)
+                    return res->status = am1g_set_error(ctx, AM1G_DEVICE_ERROR, "rocBLAS dgemm failed");
+                HIP_TRY(ctx, launch_density_update(n, Pn, false, P, partial, scal + 27, s));
+            }
+        } else {
+            HIP_TRY(ctx, hipMemsetAsync(Pn, 0, sizeof(double) * nn, s));
+            HIP_TRY(ctx, launch_density_update(n, Pn, false, P, partial, scal + 27, s));
+        }
+        double dd = 0.0;
+        HIP_TRY(ctx, hipMemcpyAsync(&dd, scal + 27, sizeof(double), hipMemcpyDeviceToHost, s));
+        HIP_TRY(ctx, hipStreamSynchronize(s));
+        const double dp = std::sqrt(dd / (double)std::max<size_t>(1, nn));
+        phase[4] += ms_since(t_ph);
+        const double de = std::fabs(e_elec - e_old);
+        e_old = e_elec;
+        residual = err;
+        if (fp32_phase) {
+            if (err < fp32_best) {
+                fp32_best = err;
+                fp32_stall = 0;
+            } else {
+                ++fp32_stall;
+            }
+            if (err <= 1.0e-4 || fp32_stall >= 2) fp32_phase = false;
+        }
+        if (iter > 0 && de < md.e_tol && (dp < md.p_tol || err < 1.0e-7) && !low) {
+            converged = 1;
+            break;
+        }
+    }
+    // ---- final energy, populations, dipole (scf.rs:327-468)
+    HIP_TRY(ctx, launch_fock(sa, P, P, 0.5, F, s));
+    HIP_TRY(ctx, launch_dot3((int64_t)nn, P, H, F, partial, scal + 0, s));
+    HIP_TRY(ctx, launch_atom_props(sa, P, L.props.as<double>(), s));
+    std::vector<double> props(4 * (size_t)nat), heps(n);
+    HIP_TRY(ctx, hipMemcpyAsync(h_scal, scal, sizeof(double) * 30, hipMemcpyDeviceToHost, s));
+    HIP_TRY(ctx, hipMemcpyAsync(props.data(), L.props.p, sizeof(double) * 4 * nat, hipMemcpyDeviceToHost, s));
+    HIP_TRY(ctx, hipMemcpyAsync(heps.data(), eps, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    HIP_TRY(ctx, hipStreamSynchronize(s));
+    const double electronic = 0.5 * h_scal[0];
+    MolResult r;
+    std::memset(&r, 0, sizeof(r));
+    double dip[3] = {0, 0, 0}, eiso = 0.0, eheat = 0.0;
+    for (int a = 0; a < nat; ++a) {
+        const DevElem& el = tab[zs[a]];
+        const double q = el.core - props[4 * a];
+        if (res->charges) res->charges[a] = q;
+        for (int k = 0; k < 3; ++k) dip[k] += rq.xyz_bohr[3 * a + k] * q;
+        if (el.norb == 4)
+            for (int k = 0; k < 3; ++k) dip[k] += -2.0 * el.dd * props[4 * a + 1 + k];
+        eiso += el.e_isol;
+        eheat += el.eheat_ev;
+    }
+    r.status = (status == AM1G_OK && !converged) ? AM1G_SCF_NOT_CONVERGED : status;
+    r.iterations = iterations;
+    r.converged = converged;
+    r.nao = n;
+    r.electronic_ev = electronic;
+    r.core_ev = ecore;
+    r.total_ev = electronic + ecore;
+    r.heat_of_formation_kcal = (electronic + ecore - eiso + eheat) * kEvToKcal;
+    r.homo_ev = nocc >= 1 ? heps[nocc - 1] : NAN;
+    r.lumo_ev = nocc < n ? heps[nocc] : NAN;
+    r.homo_beta_ev = NAN;
+    r.lumo_beta_ev = NAN;
+    for (int k = 0; k < 3; ++k) r.dipole[k] = dip[k] * kAuDipoleToDebye;
+    r.residual = residual;
+    r.fp32_iterations = fp32_iters;
+    am1g_fill_result(r, md, *res);
+    if (res->mo_energies) std::memcpy(res->mo_energies, heps.data(), sizeof(double) * n);
+    if (res->density) HIP_TRY(ctx, hipMemcpy(res->density, P, sizeof(double) * nn, hipMemcpyDeviceToHost));
+    if (res->mo_coeff) {
+        std::vector<double> ct(nn);
+        HIP_TRY(ctx, hipMemcpy(ct.data(), Ct, sizeof(double) * nn, hipMemcpyDeviceToHost));
+        for (int mu = 0; mu < n; ++mu)
+            for (int k = 0; k < n; ++k) res->mo_coeff[(size_t)mu * n + k] = ct[(size_t)k * n + mu];
+    }
+    phase[5] += ms_since(t_ph);
+    for (int k = 0; k < 6; ++k) res->phase_ms[k] = phase[k];
+    res->eigen_sweeps = 0;
+    if (status == AM1G_LINEAR_ALGEBRA) am1g_set_error(ctx, status, "%s", eig_err.c_str());
+    else if (status == AM1G_SCF_DIVERGED) am1g_set_error(ctx, status, "SCF diverged after %d iterations", iterations);
+    else if (r.status == AM1G_SCF_NOT_CONVERGED)
+        am1g_set_error(ctx, r.status, "SCF did not converge in %d iterations (residual %.3e)", iterations, residual);
+    return r.status;
 }
 
 extern "C" {

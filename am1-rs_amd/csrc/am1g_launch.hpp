@@ -97,6 +97,20 @@ hipError_t launch_dc_update(int64_t npat, const int64_t* pat, int nao, const dou
                             const double* hres_all, const int* ring, int m, int64_t stride, double mix,
                             const double* P_in, const double* pout, double* P, hipStream_t st);
 hipError_t launch_sad(const SysArgs& a, double scale, double* P, hipStream_t st);
+
+// Large-system SCF (one molecule on the whole device). `partial` holds large_partials(n)
+// doubles; `out` receives the reduced scalars.
+int64_t large_partials(int n);
+hipError_t launch_push(int n, const void* T, bool t_f32, const double* F, const double* P, double* hf, double* he,
+                       double* hd, double* partial, double* out, hipStream_t st);
+hipError_t launch_diis_dots(int n, const double* hf, const double* he, const double* hd, const int* ring, int m,
+                            int newest, double* partial, double* out, hipStream_t st);
+hipError_t launch_combine(int n, const double* hf, const int* ring, const double* coef, int m, double* Fout,
+                          double* partial, double* out, hipStream_t st);
+hipError_t launch_density_update(int n, const void* Pn, bool f32, double* P, double* partial, double* out,
+                                 hipStream_t st);
+hipError_t launch_to_f32(int64_t len, const double* x, float* y, hipStream_t st);
+hipError_t launch_atom_props(const SysArgs& a, const double* P, double* out, hipStream_t st);
 hipError_t launch_sum(int64_t n, const double* x, double* partial, double* out, hipStream_t st);
 
 }  // namespace am1g

@@ -475,7 +475,7 @@ __device__ __noinline__ int wg_jacobi_big(int n, double* A, int lda, double* Vt,
 
 // Gaussian elimination with partial pivoting on the bordered DIIS system (am1-rs
 // scf.rs:1095-1141): `m` is dim x dim row-major, modified in place; false on a pivot below 1e-12.
-__device__ bool solve_bordered(double* m, double* rhs, int n, double* x) {
+__host__ __device__ inline bool solve_bordered(double* m, double* rhs, int n, double* x) {
     for (int col = 0; col < n; ++col) {
         int pivot = col;
         double best = fabs(m[col * n + col]);
@@ -513,7 +513,7 @@ __device__ bool solve_bordered(double* m, double* rhs, int n, double* x) {
 }
 
 // Euclidean projection onto the probability simplex (Duchi et al.; am1-rs scf.rs:1181-1204).
-__device__ void simplex_project(const double* v, int n, double* out) {
+__host__ __device__ inline void simplex_project(const double* v, int n, double* out) {
     double u[kDiisDepth];
     for (int i = 0; i < n; ++i) u[i] = v[i];
     for (int i = 1; i < n; ++i) {  // insertion sort, descending
@@ -536,7 +536,7 @@ __device__ void simplex_project(const double* v, int n, double* out) {
 
 // A-DIIS coefficients from the Gram matrix g[i][j] = <D_i, F_j> (logical order, newest last):
 // projected-gradient descent on the simplex (am1-rs scf.rs:1061-1091, 1144-1179).
-__device__ void adiis_solve(const double* g, int m, double* c) {
+__host__ __device__ inline void adiis_solve(const double* g, int m, double* c) {
     const int last = m - 1;
     double d[kDiisDepth], S[kDiisDepth * kDiisDepth];
     for (int i = 0; i < m; ++i) d[i] = g[i * kDiisDepth + last] - g[last * kDiisDepth + last];

@@ -1,0 +1,73 @@
+"""The large-system SCF path (one molecule on the whole device: grid Fock kernels, library GEMMs,
+device DIIS histories, the large-order eigensolver) against the FP64 oracle. Molecules above 256
+atoms or basis functions take it automatically; AM1G_FORCE_LARGE_PATH=1 sends small ones there
+too, so it is checked where the oracle is quick."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests import molecules as M
+from tests.test_gpu_parity import _compare
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture
+def forced(monkeypatch):
+    monkeypatch.setenv("AM1G_FORCE_LARGE_PATH", "1")
+
+
+@pytest.mark.parametrize("acc", ["none", "cdiis", "adiis_cdiis"])
+def test_formaldehyde_accelerators(engine, forced, acc):
+    z, x = M.parse(M.FORMALDEHYDE)
+    r = engine.scf(z, x, accelerator=acc)
+    _compare(r, O.scf(z, x, accelerator=acc))
+    if acc == "adiis_cdiis":
+        assert r["iterations"] == 11  # scf.rs:1039-1040
+
+
+@pytest.mark.parametrize("name", ["H2O", "CH3Cl", "HCN"])
+def test_recorded_molecules(engine, forced, name):
+    z, x = M.parse(M.RECORDED_HOF[name][0])
+    r = engine.scf(z, x)
+    _compare(r, O.scf(z, x))
+    assert abs(r["heat_of_formation_kcal"] - M.RECORDED_HOF[name][1]) < 1e-4
+
+
+def test_density_and_orbitals(engine, forced):
+    z, x = M.parse(M.FORMALDEHYDE)
+    r = engine.scf(z, x)
+    o = O.scf(z, x)
+    assert np.abs(r["density"] - o["density"]).max() < 1e-6
+    assert np.abs(r["mo_energies"] - o["mo_energies"]).max() < 1e-6
+    C = r["mo_coeff"]
+    assert np.allclose(C.T @ C, np.eye(C.shape[0]), atol=1e-10)
+
+
+def test_water_cluster_32(engine, forced):
+    from am1g import geometry as G
+
+    mol = G.water_cluster_config(0)
+    r = engine.scf(mol.z, mol.xyz_bohr, accelerator="cdiis")
+    _compare(r, O.scf(mol.z, mol.xyz_bohr, accelerator="cdiis"))
+
+
+def test_water_cluster_100_takes_the_large_path(engine):
+    """300 atoms, 600 basis functions: beyond the batched kernel, so the large path runs without
+    forcing; FP64 and mixed precision both within the contract of the FP64 oracle."""
+    from am1g import geometry as G
+
+    mol = G.water_cluster((5, 5, 4), seed=4)
+    o = O.scf(mol.z, mol.xyz_bohr, full=False)
+    r = engine.scf(mol.z, mol.xyz_bohr, full=False)
+    _compare(r, o)
+    rm = engine.scf(mol.z, mol.xyz_bohr, full=False, precision="mixed")
+    assert rm["status"] == 0 and rm["fp32_iterations"] > 0
+    assert abs(rm["total_ev"] - o["total_ev"]) <= 1e-6
+    assert np.abs(rm["charges"] - o["charges"]).max() <= 1e-6
+
+
+def test_large_uhf_is_refused(engine, forced):
+    z, x = M.parse(M.METHYL)
+    r = engine.scf(z, x, multiplicity=2)
+    assert r["status"] == 1 and "unrestricted" in r["message"]
