@@ -71,3 +71,29 @@ def test_large_uhf_is_refused(engine, forced):
     z, x = M.parse(M.METHYL)
     r = engine.scf(z, x, multiplicity=2)
     assert r["status"] == 1 and "unrestricted" in r["message"]
+
+
+def test_water512_full_size_against_oracle_fixture(engine):
+    """The 512-water workload at full size (1536 atoms, 3072 basis functions) against the
+    oracle's result committed in tests/golden (tests/golden/make_scf_water512.py): FP64 to the
+    parity contract, mixed precision to 1e-6 eV in energy and 1e-6 e in charges (its orbital
+    energies come from a different last extrapolated Fock matrix and are reported, within 1e-5)."""
+    import os
+
+    from am1g import geometry as G
+
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "scf_water512.npz"))
+    mol = G.water_cluster_config(2)
+    r = engine.scf(mol.z, mol.xyz_bohr, full=False)
+    assert r["status"] == 0, r.get("message")
+    assert abs(r["total_ev"] - float(g["total_ev"])) <= 1e-6
+    assert abs(r["heat_of_formation_kcal"] - float(g["heat_of_formation_kcal"])) <= 1e-6 * 23.061
+    assert np.abs(r["charges"] - g["charges"]).max() <= 1e-6
+    assert abs(r["homo_ev"] - float(g["homo_ev"])) <= 1e-6 and abs(r["lumo_ev"] - float(g["lumo_ev"])) <= 1e-6
+    assert abs(r["iterations"] - int(g["iterations"])) <= 1
+    assert np.allclose(r["dipole_debye"], g["dipole_debye"], atol=1e-5)
+    rm = engine.scf(mol.z, mol.xyz_bohr, full=False, precision="mixed")
+    assert rm["status"] == 0 and rm["fp32_iterations"] > 0
+    assert abs(rm["total_ev"] - float(g["total_ev"])) <= 1e-6
+    assert np.abs(rm["charges"] - g["charges"]).max() <= 1e-6
+    assert abs(rm["homo_ev"] - float(g["homo_ev"])) <= 1e-5
