@@ -3,8 +3,9 @@
 // Replaces faer's self_adjoint_eigen behind am1-rs symmetric_eigen (linalg.rs:276-331) for
 // orders beyond what one workgroup holds on chip: eigenvalues ascending, eigenvectors
 // orthonormal. Three stages:
-//   1. Householder tridiagonalization (lower, unblocked): per column a reflector kernel, a
-//      row-parallel symmetric matrix-vector product, and a rank-2 update of the trailing block.
+//   1. Householder tridiagonalization (lower, unblocked): per column, one pass over the trailing
+//      rows that applies the previous column's rank-2 update and the new column's matrix-vector
+//      product, then a one-workgroup kernel that builds the next reflector.
 //   2. Tridiagonal divide and conquer (Cuppen's splitting with the deflation rules of
 //      Dongarra-Sorensen and Gu-Eisenstat's Loewner correction for orthogonality). Leaves of at
 //      most 64 are diagonalized by the workgroup Jacobi solver; each merge solves its secular
@@ -30,30 +31,30 @@ namespace {
 
 constexpr int kLeaf = 64;  // largest tridiagonal block diagonalized directly
 constexpr int kNb = 64;    // reflectors per back-transformation block
-constexpr int kRedThreads = 1024;
-
-__device__ __forceinline__ double block_sum_1024(double v, double* red) {
+// ---------------------------------------------------------------- tridiagonalization
+// Column c's reflector H_c = I - tau_c v_c v_c^T (v_c[0] = 1, indices relative to row c+1) maps
+// the updated row c, A[c, c+1:n], to beta e_1; v_c[1:] is stored back into A[c, c+2:n].
+// Per column c, one grid kernel does, for every trailing row i > c, the deferred rank-2 update
+// of column c-1 (A -= v w^T + w v^T) followed at once by p_c[i] = tau_c A[i, c+1:] v_c, so the
+// trailing block is read and written once per column; a one-workgroup kernel then forms w_c,
+// applies the rank-2 update of column c to row c+1 and builds reflector c+1 from it.
+__device__ __forceinline__ double block_sum_256(double v, double* red) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     __syncthreads();
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
     __syncthreads();
-    double tot = 0.0;
-    for (int w = 0; w < kRedThreads / 64; ++w) tot += red[w];
+    const double tot = (red[0] + red[1]) + (red[2] + red[3]);
     __syncthreads();
     return tot;
 }
 
-// ---------------------------------------------------------------- tridiagonalization
-// Reflector for column k (read as row k of the symmetric matrix): H = I - tau v v^T with
-// v[0] = 1 maps x = A[k, k+1:n] to beta e_1. v[1:] is stored back into A[k, k+2:n].
-__global__ __launch_bounds__(kRedThreads) void k_house(int n, int k, double* A, double* d, double* e, double* tau,
-                                                       double* v) {
-    __shared__ double red[kRedThreads / 64];
-    const int m = n - k - 1;
-    double* x = A + (size_t)k * n + k + 1;
+// Reflector of row r (entries r..n-1 up to date), by one 256-thread workgroup.
+__device__ void reflect_row(int n, int r, double* A, double* d, double* e, double* tau, double* v, double* red) {
+    const int m = n - r - 1;
+    double* x = A + (size_t)r * n + r + 1;
     double ss = 0.0;
-    for (int i = 1 + threadIdx.x; i < m; i += kRedThreads) ss += x[i] * x[i];
-    ss = block_sum_1024(ss, red);
+    for (int i = 1 + threadIdx.x; i < m; i += 256) ss += x[i] * x[i];
+    ss = block_sum_256(ss, red);
     const double alpha = x[0];
     double beta, t, scale;
     if (ss == 0.0) {
@@ -65,52 +66,88 @@ __global__ __launch_bounds__(kRedThreads) void k_house(int n, int k, double* A, 
         t = (beta - alpha) / beta;
         scale = 1.0 / (alpha - beta);
     }
-    for (int i = 1 + threadIdx.x; i < m; i += kRedThreads) {
+    for (int i = 1 + threadIdx.x; i < m; i += 256) {
         const double vi = x[i] * scale;
         x[i] = vi;
         v[i] = vi;
     }
     if (threadIdx.x == 0) {
         v[0] = 1.0;
-        d[k] = A[(size_t)k * n + k];
-        e[k] = beta;
-        tau[k] = t;
-        if (k == n - 2) d[n - 1] = A[(size_t)(n - 1) * n + n - 1];
+        d[r] = A[(size_t)r * n + r];
+        e[r] = beta;
+        tau[r] = t;
     }
 }
 
-// p = tau * A22 v, one wavefront per row of the trailing block A22 = A[k+1:, k+1:].
-__global__ __launch_bounds__(256) void k_symv(int n, int k, const double* A, const double* tau, const double* v,
-                                              double* p) {
-    const int m = n - k - 1;
-    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+__global__ __launch_bounds__(256) void k_reflect0(int n, double* A, double* d, double* e, double* tau, double* v) {
+    __shared__ double red[4];
+    reflect_row(n, 0, A, d, e, tau, v, red);
+    if (threadIdx.x == 0 && n == 2) d[1] = A[(size_t)n + 1];
+}
+
+// Column c (0 <= c <= n-3): vb/wb hold v and p/w of columns c (slot c&1) and c-1 (other slot).
+// Trailing rows i > c: deferred rank-2 update of column c-1, then p_c[i] = tau_c A[i, c+1:] v_c;
+// one wavefront per row, four elements in flight per lane.
+__global__ __launch_bounds__(256) void k_tridiag_rows(int n, int c, double* A, const double* tau, const double* vb,
+                                                      double* wb) {
+    const int m = n - c - 1;
     const int lane = threadIdx.x & 63;
-    if (row >= m) return;
-    const double* a = A + (size_t)(k + 1 + row) * n + k + 1;
+    const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (t >= m) return;
+    const double* vc = vb + (size_t)(c & 1) * n;
+    double* pc = wb + (size_t)(c & 1) * n;
+    const double* vp = vb + (size_t)((c + 1) & 1) * n;  // column c-1 (indices relative to row c)
+    const double* wp = wb + (size_t)((c + 1) & 1) * n;
+    const int i = c + 1 + t;
+    double* row = A + (size_t)i * n;
     double acc = 0.0;
-    for (int j = lane; j < m; j += 64) acc += a[j] * v[j];
+    if (c > 0) {
+        const double vi = vp[t + 1], wi = wp[t + 1];
+        int j = c + 1 + lane;
+        for (; j + 192 < n; j += 256) {
+            double a[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) a[u] = row[j + 64 * u];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int jj = j + 64 * u;
+                a[u] -= vi * wp[jj - c] + wi * vp[jj - c];
+                row[jj] = a[u];
+                acc += a[u] * vc[jj - c - 1];
+            }
+        }
+        for (; j < n; j += 64) {
+            const double a = row[j] - (vi * wp[j - c] + wi * vp[j - c]);
+            row[j] = a;
+            acc += a * vc[j - c - 1];
+        }
+    } else {
+        for (int j = c + 1 + lane; j < n; j += 64) acc += row[j] * vc[j - c - 1];
+    }
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
-    if (lane == 0) p[row] = tau[k] * acc;
+    if (lane == 0) pc[t] = tau[c] * acc;
 }
 
-// w = p - (tau/2)(p.v) v, in place over p.
-__global__ __launch_bounds__(kRedThreads) void k_wvec(int n, int k, const double* tau, const double* v, double* p) {
-    __shared__ double red[kRedThreads / 64];
-    const int m = n - k - 1;
-    double acc = 0.0;
-    for (int i = threadIdx.x; i < m; i += kRedThreads) acc += p[i] * v[i];
-    acc = block_sum_1024(acc, red);
-    const double alpha = 0.5 * tau[k] * acc;
-    for (int i = threadIdx.x; i < m; i += kRedThreads) p[i] -= alpha * v[i];
-}
-
-// A22 -= v w^T + w v^T (both triangles).
-__global__ __launch_bounds__(256) void k_rank2(int n, int k, double* A, const double* v, const double* w) {
-    const int m = n - k - 1;
-    const int i = blockIdx.y;
-    const int j = blockIdx.x * 256 + threadIdx.x;
-    if (i >= m || j >= m) return;
-    A[(size_t)(k + 1 + i) * n + k + 1 + j] -= v[i] * w[j] + w[i] * v[j];
+// One workgroup: w_c = p_c - (tau_c / 2)(p_c . v_c) v_c, the rank-2 update of column c on row
+// c+1 (and on the last diagonal element when c+1 = n-2), then reflector c+1.
+__global__ __launch_bounds__(256) void k_tridiag_next(int n, int c, double* A, double* d, double* e, double* tau,
+                                                      double* vb, double* wb) {
+    __shared__ double red[4];
+    const int m = n - c - 1;
+    const double* vc = vb + (size_t)(c & 1) * n;
+    double* pc = wb + (size_t)(c & 1) * n;
+    double pv = 0.0;
+    for (int q = threadIdx.x; q < m; q += 256) pv += pc[q] * vc[q];
+    const double alpha = 0.5 * tau[c] * block_sum_256(pv, red);
+    for (int q = threadIdx.x; q < m; q += 256) pc[q] -= alpha * vc[q];  // pc now holds w_c
+    __syncthreads();
+    const int r = c + 1;
+    double* rowr = A + (size_t)r * n;
+    const double v0 = vc[0], w0 = pc[0];
+    for (int jj = r + threadIdx.x; jj < n; jj += 256) rowr[jj] -= v0 * pc[jj - r] + w0 * vc[jj - r];
+    if (threadIdx.x == 0 && r == n - 2) d[n - 1] = A[(size_t)(n - 1) * n + n - 1] - 2.0 * vc[1] * pc[1];
+    __syncthreads();
+    reflect_row(n, r, A, d, e, tau, vb + (size_t)(r & 1) * n, red);
 }
 
 // ---------------------------------------------------------------- divide and conquer
@@ -306,24 +343,29 @@ __global__ __launch_bounds__(256) void k_build_v(int n, int k0, int nb, const do
 }
 
 // Compact WY factor T (nb x nb upper triangular, column-major) from G = Vb^T Vb (forward,
-// columnwise): T[j][j] = tau_j, T[0:j, j] = -tau_j T[0:j, 0:j] G[0:j, j].
+// columnwise): T[j][j] = tau_j, T[0:j, j] = -tau_j T[0:j, 0:j] G[0:j, j]. T and G live in LDS.
 __global__ __launch_bounds__(256) void k_larft(int nb, int k0, const double* tau, const double* G, double* T) {
-    __shared__ double col[kNb];
+    __shared__ double sT[kNb * kNb], sG[kNb * kNb];
     const int t = threadIdx.x;
-    for (int idx = t; idx < nb * nb; idx += 256) T[idx] = 0.0;
+    for (int idx = t; idx < nb * nb; idx += 256) {
+        sT[idx] = 0.0;
+        sG[idx] = G[idx];
+    }
     __syncthreads();
     for (int j = 0; j < nb; ++j) {
         const double tj = tau[k0 + j];
+        double val = 0.0;
         if (t < j) {
             double acc = 0.0;
-            for (int l = t; l < j; ++l) acc += T[t + l * nb] * G[l + j * nb];
-            col[t] = -tj * acc;
+            for (int l = t; l < j; ++l) acc += sT[t + l * nb] * sG[l + j * nb];
+            val = -tj * acc;
         }
         __syncthreads();
-        if (t < j) T[t + j * nb] = col[t];
-        if (t == 0) T[j + j * nb] = tj;
+        if (t < j) sT[t + j * nb] = val;
+        if (t == 0) sT[j + j * nb] = tj;
         __syncthreads();
     }
+    for (int idx = t; idx < nb * nb; idx += 256) T[idx] = sT[idx];
 }
 
 inline unsigned blocks(int64_t n, int per) { return (unsigned)((n + per - 1) / per); }
@@ -367,8 +409,8 @@ hipError_t eig_large(EigWork& w, rocblas_handle blas, int n, double* A, double* 
     EIG_TRY(w.d.reserve(sizeof(double) * n));
     EIG_TRY(w.e.reserve(sizeof(double) * n));
     EIG_TRY(w.tau.reserve(sizeof(double) * n));
-    EIG_TRY(w.vbuf.reserve(sizeof(double) * n));
-    EIG_TRY(w.pbuf.reserve(sizeof(double) * n));
+    EIG_TRY(w.vbuf.reserve(sizeof(double) * 2 * (size_t)n));
+    EIG_TRY(w.pbuf.reserve(sizeof(double) * 2 * (size_t)n));
     EIG_TRY(w.scal.reserve(sizeof(int) * 16));
     EIG_TRY(w.Q.reserve(sizeof(double) * nn));
     double* d = w.d.as<double>();
@@ -381,14 +423,11 @@ hipError_t eig_large(EigWork& w, rocblas_handle blas, int n, double* A, double* 
     // ---- 1. tridiagonalization
     if (n == 1) {
         EIG_TRY(hipMemcpyAsync(d, A, sizeof(double), hipMemcpyDeviceToDevice, st));
-    }
-    for (int k = 0; k + 1 < n; ++k) {
-        const int m = n - k - 1;
-        hipLaunchKernelGGL(k_house, dim3(1), dim3(kRedThreads), 0, st, n, k, A, d, e, tau, v);
-        if (m > 1) {
-            hipLaunchKernelGGL(k_symv, dim3(blocks(m, 4)), dim3(256), 0, st, n, k, A, tau, v, p);
-            hipLaunchKernelGGL(k_wvec, dim3(1), dim3(kRedThreads), 0, st, n, k, tau, v, p);
-            hipLaunchKernelGGL(k_rank2, dim3(blocks(m, 256), m), dim3(256), 0, st, n, k, A, v, p);
+    } else {
+        hipLaunchKernelGGL(k_reflect0, dim3(1), dim3(256), 0, st, n, A, d, e, tau, v);
+        for (int c = 0; c + 2 < n; ++c) {
+            hipLaunchKernelGGL(k_tridiag_rows, dim3(blocks(n - c - 1, 4)), dim3(256), 0, st, n, c, A, tau, v, p);
+            hipLaunchKernelGGL(k_tridiag_next, dim3(1), dim3(256), 0, st, n, c, A, d, e, tau, v, p);
         }
     }
     EIG_TRY(hipGetLastError());
