@@ -344,6 +344,7 @@ int am1g_dc(am1g_ctx* ctx, const am1g_request* rq, const am1g_dc_options* op, am
     d.cap = 2.0;
     d.tol_abs = 0.0;
     d.status = B.status.as<int>();
+    d.red = B.partial.as<double>();  // 1024 * 9 doubles >= 8 + 512 * 16
 
     double n_elec = 0.0;
     for (int a = 0; a < nat; ++a) n_elec += tab[zs[a]].core;

@@ -165,7 +165,6 @@ int am1g_large_scf(am1g_ctx* ctx, const am1g_request& rq, const Prepared& prep, 
     if (md.precision == 1) {
         HIP_TRY(ctx, L.f32a.reserve(sizeof(float) * nn));
         HIP_TRY(ctx, L.f32b.reserve(sizeof(float) * nn));
-        HIP_TRY(ctx, L.f32c.reserve(sizeof(float) * nn));
     }
     double* H = L.H.as<double>();
     double* F = L.F.as<double>();
@@ -199,14 +198,16 @@ int am1g_large_scf(am1g_ctx* ctx, const am1g_request& rq, const Prepared& prep, 
     int hcount = 0, head = 0, ring[kDiisDepth];
     double e_old = 0.0, residual = INFINITY;
     int iterations = 0, converged = 0, status = AM1G_OK, fp32_iters = 0;
-    // Mixed precision: FP32 GEMMs until the commutator norm drops below 1e-4, or until the norm
-    // measured in FP32 stops improving (it has a rounding floor of about eps_32 |F||P|, above
-    // 1e-4 for thousands of basis functions); then FP64 for the rest of the SCF.
+    // Mixed precision: the density GEMM runs in FP32 (MFMA, FP32 accumulation) until the DIIS
+    // error ||[F,P]|| drops below 1e-4, then FP64 for the rest of the SCF. The commutator itself
+    // stays FP64 (noisy error vectors slow the DIIS down). An FP32 density carries a rounding
+    // floor of about eps_32 |F| sqrt(n^2) into the error, above 1e-4 for hundreds of basis
+    // functions, so the FP32 phase also ends once the error has not improved for two iterations.
     bool fp32_phase = md.precision == 1;
     double fp32_best = INFINITY;
     int fp32_stall = 0;
     const double one = 1.0, zero = 0.0, two = 2.0;
-    const float onef = 1.0f, zerof = 0.0f, twof = 2.0f;
+    const float zerof = 0.0f, twof = 2.0f;
     std::string eig_err;
     for (int iter = 0; iter < md.max_scf; ++iter) {
         iterations = iter + 1;
@@ -215,25 +216,12 @@ int am1g_large_scf(am1g_ctx* ctx, const am1g_request& rq, const Prepared& prep, 
         phase[1] += ms_since(t_ph);
         const bool low = fp32_phase;
         // commutator FP - PF: column-major P*F read row-major is F*P
-        const void* Tc = T0;
-        if (low) {
-            float* Ff = L.f32a.as<float>();
-            float* Pf = L.f32b.as<float>();
-            float* Tf = L.f32c.as<float>();
-            HIP_TRY(ctx, launch_to_f32((int64_t)nn, F, Ff, s));
-            HIP_TRY(ctx, launch_to_f32((int64_t)nn, P, Pf, s));
-            if (rocblas_sgemm(ctx->blas, rocblas_operation_none, rocblas_operation_none, n, n, n, &onef, Pf, n, Ff, n,
-                              &zerof, Tf, n) != rocblas_status_success)
-                return res->status = am1g_set_error(ctx, AM1G_DEVICE_ERROR, "rocBLAS sgemm failed");
-            Tc = Tf;
-            ++fp32_iters;
-        } else if (rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, P, n, F, n,
-                                 &zero, T0, n) != rocblas_status_success) {
+        if (rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, P, n, F, n, &zero, T0,
+                          n) != rocblas_status_success)
             return res->status = am1g_set_error(ctx, AM1G_DEVICE_ERROR, "rocBLAS dgemm failed");
-        }
         const int slot = hcount < kDiisDepth ? (head + hcount) % kDiisDepth : head;
-        HIP_TRY(ctx, launch_push(n, Tc, low, F, P, hf + slot * len_s, he + slot * len_a, hd ? hd + slot * len_s : nullptr,
-                                 partial, scal + 1, s));
+        HIP_TRY(ctx, launch_push(n, T0, false, F, P, hf + slot * len_s, he + slot * len_a,
+                                 hd ? hd + slot * len_s : nullptr, partial, scal + 1, s));
         if (hcount < kDiisDepth) {
             hcount++;
         } else {
@@ -314,6 +302,7 @@ int am1g_large_scf(am1g_ctx* ctx, const am1g_request& rq, const Prepared& prep, 
         // density 2 C_occ C_occ^T: rows 0..nocc-1 of Ct are the occupied orbitals (column-major n x nocc)
         if (nocc > 0) {
             if (low) {
+                ++fp32_iters;
                 float* Cf = L.f32a.as<float>();
                 float* Pf = L.f32b.as<float>();
                 HIP_TRY(ctx, launch_to_f32((int64_t)n * nocc, Ct, Cf, s));

@@ -79,6 +79,7 @@ struct DcArgs {
     double cap;     // 2 restricted, 1 per spin channel
     double tol_abs; // Jacobi tolerance floor
     int* status;    // [n_sub] eigensolver status (sweeps, or -1)
+    double* red;    // reduction scratch for the Fermi search: 8 + 512 * 16 doubles
 };
 
 hipError_t launch_hcore_diag(const SysArgs& a, hipStream_t st);

@@ -11,6 +11,8 @@
 // (src/fock.rs:249-511), packed CDIIS / A-DIIS (src/scf.rs:876-1204).
 #include <hip/hip_runtime.h>
 
+#include <cfloat>
+
 #include "am1g_common.hpp"
 #include "am1g_pair.hpp"
 #include "am1g_launch.hpp"
@@ -1095,39 +1097,63 @@ __device__ __forceinline__ BlockRef block_of_g(const SysArgs& a, int x, int y) {
     return r;
 }
 
-__device__ __forceinline__ bool diag_item(const SysArgs& a, int it, int& x, int& k) {
-    const int nh = a.md.n_heavy, nH = a.md.n_h;
-    if (it < nh * 10) {
-        x = a.alist[it / 10];
-        k = it % 10;
-        return true;
+
+// Partner sums of the diagonal blocks, one wavefront per atom x: lanes stride over the partners
+// y, each lane accumulates its partners' contributions to the 10 (heavy) or 1 (hydrogen) packed
+// entries, then a fixed butterfly sums the lanes. With `pd` null the partner term is the core
+// attraction -Z_y (mu nu | s_y s_y); otherwise the Coulomb term sum_q P_y[q] (mu nu | q_y).
+__device__ __forceinline__ void diag_partner_sum(const SysArgs& a, int x, const double* pd_all, double (&acc)[10]) {
+    const DevElem* el = a.elems + a.md.method * kMaxZ;
+    const int lane = threadIdx.x & 63;
+    const int nk = a.z[x] == 1 ? 1 : 10;
+#pragma unroll
+    for (int k = 0; k < 10; ++k) acc[k] = 0.0;
+    for (int y = lane; y < a.md.nat; y += 64) {
+        if (y == x) continue;
+        const BlockRef br = block_of_g(a, x, y);
+        if (!pd_all || a.z[y] == 1) {
+            const double c = pd_all ? pd_all[y * 10] : -el[a.z[y]].core;
+#pragma unroll
+            for (int k = 0; k < 10; ++k)
+                if (k < nk) acc[k] += c * (br.x_is_bra ? br.w[k * br.wcols] : br.w[k]);
+        } else {
+            double pd[10];
+#pragma unroll
+            for (int q = 0; q < 10; ++q) pd[q] = pd_all[y * 10 + q];
+#pragma unroll
+            for (int k = 0; k < 10; ++k) {
+                if (k >= nk) continue;
+                double t = 0.0;
+#pragma unroll
+                for (int q = 0; q < 10; ++q) t += pd[q] * (br.x_is_bra ? br.w[k * br.wcols + q] : br.w[q * br.wcols + k]);
+                acc[k] += t;
+            }
+        }
     }
-    if (it < nh * 10 + nH) {
-        x = a.alist[nh + it - nh * 10];
-        k = 0;
-        return true;
-    }
-    return false;
+#pragma unroll
+    for (int k = 0; k < 10; ++k)
+        for (int off = 32; off > 0; off >>= 1) acc[k] += __shfl_xor(acc[k], off, 64);
 }
 
-// H_core diagonal blocks: U + sum over partners y (index order) of (-Z_y)(mu nu | s_y s_y).
+// H_core diagonal blocks: U + sum over partners y of (-Z_y)(mu nu | s_y s_y).
 __global__ __launch_bounds__(256) void k_hcore_diag(SysArgs a) {
-    int x, k;
-    if (!diag_item(a, blockIdx.x * blockDim.x + threadIdx.x, x, k)) return;
-    const DevElem* el = a.elems + a.md.method * kMaxZ;
-    const int nao = a.md.nao, nat = a.md.nat;
+    const int x = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (x >= a.md.nat) return;
+    double acc[10];
+    diag_partner_sum(a, x, nullptr, acc);
+    const int k = threadIdx.x & 63;
+    const DevElem& ex = a.elems[a.md.method * kMaxZ + a.z[x]];
+    if (k >= (ex.norb == 4 ? 10 : 1)) return;
+    double v = acc[0];
+#pragma unroll
+    for (int q = 1; q < 10; ++q)
+        if (q == k) v = acc[q];
+    const int nao = a.md.nao;
     const int mu = kPackRow[k], nu = kPackCol[k];
-    double acc = 0.0;
-    for (int y = 0; y < nat; ++y) {
-        if (y == x) continue;
-        BlockRef br = block_of_g(a, x, y);
-        acc += -el[a.z[y]].core * (br.x_is_bra ? br.w[k * br.wcols] : br.w[k]);
-    }
-    const DevElem& ex = el[a.z[x]];
     const double u = (mu == nu) ? (mu == 0 ? ex.uss : ex.upp) : 0.0;
     const int o = a.aoff[x];
-    a.H[(size_t)(o + mu) * nao + o + nu] = u + acc;
-    a.H[(size_t)(o + nu) * nao + o + mu] = u + acc;
+    a.H[(size_t)(o + mu) * nao + o + nu] = u + v;
+    a.H[(size_t)(o + nu) * nao + o + mu] = u + v;
 }
 
 // Packed diagonal density blocks, off-diagonal entries doubled (the 2 of the symmetric sum).
@@ -1146,38 +1172,34 @@ __global__ __launch_bounds__(256) void k_pack_diag(SysArgs a, const double* Pt) 
     }
 }
 
-// Fock diagonal blocks: H + one-centre terms + Coulomb from every partner's diagonal block.
+// Fock diagonal blocks: H + one-centre terms + Coulomb from every partner's diagonal block,
+// one wavefront per atom.
 __global__ __launch_bounds__(256) void k_fock_diag(SysArgs a, const double* Pt, const double* Ps, double scale,
                                                    double* F) {
-    int x, k;
-    if (!diag_item(a, blockIdx.x * blockDim.x + threadIdx.x, x, k)) return;
-    const DevElem* el = a.elems + a.md.method * kMaxZ;
-    const int nao = a.md.nao, nat = a.md.nat;
-    const DevElem& ex = el[a.z[x]];
-    const int ox = a.aoff[x], nx = ex.norb;
+    const int x = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (x >= a.md.nat) return;
+    double acc[10];
+    diag_partner_sum(a, x, a.pdiag, acc);
+    const int k = threadIdx.x & 63;
+    const DevElem& ex = a.elems[a.md.method * kMaxZ + a.z[x]];
+    const int nx = ex.norb;
+    if (k >= (nx == 4 ? 10 : 1)) return;
+    double v = acc[0];
+#pragma unroll
+    for (int q = 1; q < 10; ++q)
+        if (q == k) v = acc[q];
+    const int nao = a.md.nao;
+    const int ox = a.aoff[x];
     const int mu = kPackRow[k], nu = kPackCol[k];
-    double acc = a.H[(size_t)(ox + mu) * nao + ox + nu];
+    double oc = 0.0;
     for (int la = 0; la < nx; ++la)
         for (int si = 0; si < nx; ++si) {
-            acc += Pt[(size_t)(ox + la) * nao + ox + si] * one_centre(mu, nu, la, si, ex);
-            acc -= scale * Ps[(size_t)(ox + la) * nao + ox + si] * one_centre(mu, la, nu, si, ex);
+            oc += Pt[(size_t)(ox + la) * nao + ox + si] * one_centre(mu, nu, la, si, ex);
+            oc -= scale * Ps[(size_t)(ox + la) * nao + ox + si] * one_centre(mu, la, nu, si, ex);
         }
-    for (int y = 0; y < nat; ++y) {
-        if (y == x) continue;
-        BlockRef br = block_of_g(a, x, y);
-        const double* pd = a.pdiag + y * 10;
-        if (a.z[y] == 1) {
-            acc += pd[0] * (br.x_is_bra ? br.w[k * br.wcols] : br.w[k]);
-        } else if (br.x_is_bra) {
-#pragma unroll
-            for (int q = 0; q < 10; ++q) acc += pd[q] * br.w[k * br.wcols + q];
-        } else {
-#pragma unroll
-            for (int q = 0; q < 10; ++q) acc += pd[q] * br.w[q * br.wcols + k];
-        }
-    }
-    F[(size_t)(ox + mu) * nao + ox + nu] = acc;
-    F[(size_t)(ox + nu) * nao + ox + mu] = acc;
+    const double f = a.H[(size_t)(ox + mu) * nao + ox + nu] + oc + v;
+    F[(size_t)(ox + mu) * nao + ox + nu] = f;
+    F[(size_t)(ox + nu) * nao + ox + mu] = f;
 }
 
 // Fock off-diagonal blocks: H - K, one atom pair per thread.
@@ -1376,41 +1398,167 @@ __device__ __forceinline__ double fermi_dirac(double x) {  // fermi.rs:110-119
     return 1.0 / (1.0 + exp(x));
 }
 
-// Common chemical potential by bisection over every subsystem level (fermi.rs:208-269):
-// bracket [min - 50 kT - 1, max + 50 kT + 1], at most 200 halvings, stop at |N(mu) - N| < 1e-13.
-// out: [0] mu, [1] T*S, [2] electrons at mu.
-__global__ __launch_bounds__(kWg) void k_fermi(DcArgs d, int64_t L, double n_elec, double kt, double* out) {
+// Common chemical potential (fermi.rs:208-269): the reference bisects the electron count
+// N(mu) = sum_i cap proj_i f((e_i - mu) / kT) on [min - 50 kT - 1, max + 50 kT + 1], at most 200
+// steps, stopping at |N - N_e| < 1e-13. Here each pass evaluates the 31 midpoints of the next
+// five bisection steps (a complete binary tree of intervals) over all levels at once, and one
+// thread then replays those five steps exactly as the reference takes them; the replay also
+// stops when the interval no longer shrinks, which is where 200 reference steps end up.
+// State in red[0..4): lo, hi, done, mu, steps; block partials from red[8].
+constexpr int kFermiDepth = 5;
+constexpr int kFermiK = (1 << kFermiDepth) - 1;
+constexpr int kFermiBlocks = 256;
+constexpr int kFermiPasses = 40;
+
+__device__ __forceinline__ void fermi_tree(double lo, double hi, double (&mid)[kFermiK]) {
+    double l[kFermiK], h[kFermiK];
+    l[0] = lo;
+    h[0] = hi;
+#pragma unroll
+    for (int q = 0; q < kFermiK; ++q) {
+        mid[q] = 0.5 * (l[q] + h[q]);
+        if (2 * q + 2 < kFermiK) {
+            l[2 * q + 1] = l[q];
+            h[2 * q + 1] = mid[q];
+            l[2 * q + 2] = mid[q];
+            h[2 * q + 2] = h[q];
+        }
+    }
+}
+
+__global__ __launch_bounds__(kWg) void k_fermi_bounds(DcArgs d, int64_t L, double kt) {
     __shared__ WgScratch s;
-    const int t = threadIdx.x;
     double lo = INFINITY, hi = -INFINITY;
-    for (int64_t i = t; i < L; i += kWg) {
+    for (int64_t i = threadIdx.x; i < L; i += kWg) {
         lo = fmin(lo, d.eps[i]);
         hi = fmax(hi, d.eps[i]);
     }
     lo = -wg_max(-lo, s) - 50.0 * kt - 1.0;
     hi = wg_max(hi, s) + 50.0 * kt + 1.0;
-    double mu = 0.5 * (lo + hi), cnt = 0.0;
-    for (int it = 0; it < 200; ++it) {
-        mu = 0.5 * (lo + hi);
-        double c = 0.0;
-        for (int64_t i = t; i < L; i += kWg) c += d.cap * d.proj[i] * fermi_dirac((d.eps[i] - mu) / kt);
-        c = wg_sum(c, s);
-        cnt = c;
-        if (fabs(c - n_elec) < 1.0e-13) break;
-        if (c < n_elec) lo = mu;
-        else hi = mu;
+    if (threadIdx.x == 0) {
+        d.red[0] = lo;
+        d.red[1] = hi;
+        d.red[2] = 0.0;
+        d.red[3] = 0.5 * (lo + hi);
+        d.red[4] = 0.0;
     }
-    double sk = 0.0;
-    for (int64_t i = t; i < L; i += kWg) {
+}
+
+__global__ __launch_bounds__(256) void k_fermi_count(DcArgs d, int64_t L, double kt) {
+    if (d.red[2] != 0.0) return;
+    double mu[kFermiK], c[kFermiK];
+    fermi_tree(d.red[0], d.red[1], mu);
+#pragma unroll
+    for (int q = 0; q < kFermiK; ++q) c[q] = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < L; i += (int64_t)gridDim.x * 256) {
+        const double w = d.cap * d.proj[i], e = d.eps[i];
+#pragma unroll
+        for (int q = 0; q < kFermiK; ++q) c[q] += w * fermi_dirac((e - mu[q]) / kt);
+    }
+    __shared__ double sh[4][kFermiK];
+#pragma unroll
+    for (int q = 0; q < kFermiK; ++q) {
+        double x = c[q];
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+        if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6][q] = x;
+    }
+    __syncthreads();
+    double* part = d.red + 8;
+    if (threadIdx.x < kFermiK)
+        part[blockIdx.x * kFermiK + threadIdx.x] =
+            (sh[0][threadIdx.x] + sh[1][threadIdx.x]) + (sh[2][threadIdx.x] + sh[3][threadIdx.x]);
+}
+
+__global__ __launch_bounds__(256) void k_fermi_pick(DcArgs d, double n_elec) {
+    if (d.red[2] != 0.0) return;
+    __shared__ double tot[kFermiK];
+    __shared__ double sh[4];
+    const double* part = d.red + 8;
+    for (int q = 0; q < kFermiK; ++q) {
+        double v = 0.0;
+        for (int b = threadIdx.x; b < kFermiBlocks; b += 256) v += part[b * kFermiK + q];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        __syncthreads();
+        if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+        __syncthreads();
+        if (threadIdx.x == 0) tot[q] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double mid[kFermiK];
+    double lo = d.red[0], hi = d.red[1];
+    fermi_tree(lo, hi, mid);
+    int steps = (int)d.red[4];
+    int node = 0;
+    for (int depth = 0; depth < kFermiDepth; ++depth) {
+        const double m = mid[node];
+        ++steps;
+        d.red[3] = m;
+        if (fabs(tot[node] - n_elec) < 1.0e-13 || steps >= 200 || m <= lo || m >= hi) {
+            d.red[2] = 1.0;
+            break;
+        }
+        if (tot[node] < n_elec) {
+            lo = m;
+            node = 2 * node + 2;
+        } else {
+            hi = m;
+            node = 2 * node + 1;
+        }
+    }
+    d.red[0] = lo;
+    d.red[1] = hi;
+    d.red[4] = (double)steps;
+}
+
+// Fractions at the chemical potential, block partials of the entropy and of the count.
+__global__ __launch_bounds__(256) void k_fermi_final(DcArgs d, int64_t L, double kt) {
+    const double mu = d.red[3];
+    double sk = 0.0, cnt = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < L; i += (int64_t)gridDim.x * 256) {
         const double f = fermi_dirac((d.eps[i] - mu) / kt);
+        const double w = d.cap * d.proj[i];
         d.frac[i] = f;
-        if (f > 1.0e-14 && f < 1.0 - 1.0e-14) sk -= d.cap * d.proj[i] * (f * log(f) + (1.0 - f) * log(1.0 - f));
+        cnt += w * f;
+        if (f > 1.0e-14 && f < 1.0 - 1.0e-14) sk -= w * (f * log(f) + (1.0 - f) * log(1.0 - f));
     }
-    sk = wg_sum(sk, s);
-    if (t == 0) {
-        out[0] = mu;
-        out[1] = kt * sk;
-        out[2] = cnt;
+    __shared__ double sh[2][4];
+    for (int off = 32; off > 0; off >>= 1) {
+        sk += __shfl_xor(sk, off, 64);
+        cnt += __shfl_xor(cnt, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        sh[0][threadIdx.x >> 6] = sk;
+        sh[1][threadIdx.x >> 6] = cnt;
+    }
+    __syncthreads();
+    double* part = d.red + 8;
+    if (threadIdx.x < 2)
+        part[blockIdx.x * 2 + threadIdx.x] =
+            (sh[threadIdx.x][0] + sh[threadIdx.x][1]) + (sh[threadIdx.x][2] + sh[threadIdx.x][3]);
+}
+
+__global__ __launch_bounds__(256) void k_fermi_out(DcArgs d, double kt, double* out) {
+    __shared__ double sh[2][4];
+    const double* part = d.red + 8;
+    double sk = 0.0, cnt = 0.0;
+    for (int b = threadIdx.x; b < kFermiBlocks; b += 256) {
+        sk += part[2 * b];
+        cnt += part[2 * b + 1];
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        sk += __shfl_xor(sk, off, 64);
+        cnt += __shfl_xor(cnt, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        sh[0][threadIdx.x >> 6] = sk;
+        sh[1][threadIdx.x >> 6] = cnt;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        out[0] = d.red[3];
+        out[1] = kt * ((sh[0][0] + sh[0][1]) + (sh[0][2] + sh[0][3]));
+        out[2] = (sh[1][0] + sh[1][1]) + (sh[1][2] + sh[1][3]);
     }
 }
 
@@ -1539,15 +1687,13 @@ __global__ __launch_bounds__(256) void k_dc_update(int64_t npat, const int64_t* 
 static inline unsigned blocks_for(int64_t n, int per = 256) { return (unsigned)((n + per - 1) / per); }
 
 hipError_t launch_hcore_diag(const SysArgs& a, hipStream_t st) {
-    const int items = a.md.n_heavy * 10 + a.md.n_h;
-    if (items > 0) hipLaunchKernelGGL(k_hcore_diag, dim3(blocks_for(items)), dim3(256), 0, st, a);
+    if (a.md.nat > 0) hipLaunchKernelGGL(k_hcore_diag, dim3(blocks_for(a.md.nat, 4)), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 
 hipError_t launch_fock(const SysArgs& a, const double* Pt, const double* Ps, double scale, double* F, hipStream_t st) {
     hipLaunchKernelGGL(k_pack_diag, dim3(blocks_for(a.md.nat)), dim3(256), 0, st, a, Pt);
-    const int items = a.md.n_heavy * 10 + a.md.n_h;
-    hipLaunchKernelGGL(k_fock_diag, dim3(blocks_for(items)), dim3(256), 0, st, a, Pt, Ps, scale, F);
+    hipLaunchKernelGGL(k_fock_diag, dim3(blocks_for(a.md.nat, 4)), dim3(256), 0, st, a, Pt, Ps, scale, F);
     const int64_t np = (int64_t)a.md.nat * (a.md.nat - 1) / 2;
     if (np > 0) {
         int64_t b = blocks_for(np);
@@ -1582,7 +1728,13 @@ hipError_t launch_dc_eigen(const DcArgs& d, const double* F, int nao, hipStream_
 }
 
 hipError_t launch_fermi(const DcArgs& d, int64_t levels, double n_elec, double kt, double* out, hipStream_t st) {
-    hipLaunchKernelGGL(k_fermi, dim3(1), dim3(kWg), 0, st, d, levels, n_elec, kt, out);
+    hipLaunchKernelGGL(k_fermi_bounds, dim3(1), dim3(kWg), 0, st, d, levels, kt);
+    for (int pass = 0; pass < kFermiPasses; ++pass) {
+        hipLaunchKernelGGL(k_fermi_count, dim3(kFermiBlocks), dim3(256), 0, st, d, levels, kt);
+        hipLaunchKernelGGL(k_fermi_pick, dim3(1), dim3(256), 0, st, d, n_elec);
+    }
+    hipLaunchKernelGGL(k_fermi_final, dim3(kFermiBlocks), dim3(256), 0, st, d, levels, kt);
+    hipLaunchKernelGGL(k_fermi_out, dim3(1), dim3(256), 0, st, d, kt, out);
     return hipGetLastError();
 }
 
