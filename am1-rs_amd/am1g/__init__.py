@@ -141,6 +141,7 @@ class DcResult(ctypes.Structure):
         ("total_ev", ctypes.c_double),
         ("heat_of_formation_kcal", ctypes.c_double),
         ("fermi_ev", ctypes.c_double),
+        ("fermi_beta_ev", ctypes.c_double),
         ("entropy_ev", ctypes.c_double),
         ("homo_lumo_gap_ev", ctypes.c_double),
         ("diagonalization_work", ctypes.c_double),

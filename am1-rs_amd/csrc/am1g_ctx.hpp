@@ -56,9 +56,13 @@ struct am1g_ctx {
 };
 
 // Buffers of the divide-and-conquer driver, kept on the context so repeated calls reuse them.
+struct DcChannel {  // one spin channel (the only one for a closed shell)
+    DevBuf F, P, cbuf, eps, proj, frac, pout, hin, hres, ring;
+};
 struct DcBuffers {
-    DevBuf F, P, pdiag, fbuf, cbuf, t0, t1, eps, proj, frac, status, sub_n, sub_ao, sub_mat, ao_list, ao_core, pat,
-        mult, pout, hin, hres, ring, coef, ao_atom, core_sub, partial, scal;
+    DcChannel ch[2];
+    DevBuf Pt, pdiag, fbuf, t0, t1, status, sub_n, sub_ao, sub_mat, ao_list, ao_core, pat, mult, coef, ao_atom,
+        core_sub, partial, scal;
 };
 
 struct Prepared {

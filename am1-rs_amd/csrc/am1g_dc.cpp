@@ -1,10 +1,11 @@
 // Divide-and-conquer SCF driver (host side of am1g_dc).
 //
-// Follows am1-rs run_divide_conquer / dc_loop_restricted (src/divide_conquer.rs:406-572,
-// 1033-1130): the full Fock build over every atom pair, one batched eigensolve of all subsystem
-// blocks, a common Fermi level by bisection, Yang-weight assembly, and Pulay DIIS on the density
-// residual over the retained pattern (linear mixing when DIIS declines). The heavy steps are
-// device kernels; the host keeps the partition, the pattern and the tiny DIIS solve.
+// Follows am1-rs run_divide_conquer with dc_loop_restricted / dc_loop_unrestricted
+// (src/divide_conquer.rs:406-572, 1033-1254): the full Fock build over every atom pair, one
+// batched eigensolve of all subsystem blocks per spin channel, a common Fermi level per channel
+// by bisection, Yang-weight assembly, and Pulay DIIS on each channel's density residual over the
+// retained pattern (linear mixing when DIIS declines). The heavy steps are device kernels; the
+// host keeps the partition, the pattern and the tiny DIIS solves.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -108,9 +109,6 @@ int am1g_dc(am1g_ctx* ctx, const am1g_request* rq, const am1g_dc_options* op, am
     std::string msg;
     int st = am1g_prepare(ctx, *rq, prep, msg, INT_MAX);
     if (st != AM1G_OK) return res->status = am1g_set_error(ctx, st, "%s", msg.c_str());
-    if (prep.uhf)
-        return res->status = am1g_set_error(ctx, AM1G_INVALID_INPUT,
-                                            "unrestricted divide-and-conquer is not available on the GPU path");
     if (!ctx->dc) ctx->dc = new DcBuffers();
     DcBuffers& B = *ctx->dc;
     hipStream_t s = ctx->stream;
@@ -287,16 +285,12 @@ int am1g_dc(am1g_ctx* ctx, const am1g_request* rq, const am1g_dc_options* op, am
     const size_t per_entry = 2 * std::max<size_t>((size_t)npat, 1) * sizeof(double);
     const int max_depth = (int)std::min<size_t>(kDiisDepth, std::max<size_t>((size_t(512) << 20) / per_entry, 2));
 
-    HIP_TRY(ctx, B.F.reserve(sizeof(double) * nn));
-    HIP_TRY(ctx, B.P.reserve(sizeof(double) * nn));
+    const bool uhf = prep.uhf;
+    const int nch = uhf ? 2 : 1;
     HIP_TRY(ctx, B.fbuf.reserve(sizeof(double) * mat_total));
-    HIP_TRY(ctx, B.cbuf.reserve(sizeof(double) * mat_total));
     HIP_TRY(ctx, B.t0.reserve(sizeof(double) * mat_total));
     HIP_TRY(ctx, B.t1.reserve(sizeof(double) * mat_total));
-    HIP_TRY(ctx, B.eps.reserve(sizeof(double) * L));
-    HIP_TRY(ctx, B.proj.reserve(sizeof(double) * L));
-    HIP_TRY(ctx, B.frac.reserve(sizeof(double) * L));
-    HIP_TRY(ctx, B.status.reserve(sizeof(int) * n_sub));
+    HIP_TRY(ctx, B.status.reserve(sizeof(int) * 2 * n_sub));
     HIP_TRY(ctx, B.sub_n.reserve(sizeof(int) * n_sub));
     HIP_TRY(ctx, B.sub_ao.reserve(sizeof(int64_t) * (n_sub + 1)));
     HIP_TRY(ctx, B.sub_mat.reserve(sizeof(int64_t) * n_sub));
@@ -304,13 +298,23 @@ int am1g_dc(am1g_ctx* ctx, const am1g_request* rq, const am1g_dc_options* op, am
     HIP_TRY(ctx, B.ao_core.reserve(L));
     HIP_TRY(ctx, B.pat.reserve(sizeof(int64_t) * std::max<int64_t>(npat, 1)));
     HIP_TRY(ctx, B.mult.reserve(sizeof(double) * std::max<int64_t>(npat, 1)));
-    HIP_TRY(ctx, B.pout.reserve(sizeof(double) * std::max<int64_t>(npat, 1)));
-    HIP_TRY(ctx, B.hin.reserve(sizeof(double) * kDiisDepth * std::max<int64_t>(npat, 1)));
-    HIP_TRY(ctx, B.hres.reserve(sizeof(double) * kDiisDepth * std::max<int64_t>(npat, 1)));
-    HIP_TRY(ctx, B.ring.reserve(sizeof(int) * (kDiisDepth + 1)));
     HIP_TRY(ctx, B.coef.reserve(sizeof(double) * (kDiisDepth + 1)));
     HIP_TRY(ctx, B.ao_atom.reserve(sizeof(int) * nao));
     HIP_TRY(ctx, B.core_sub.reserve(sizeof(int) * nat));
+    if (uhf) HIP_TRY(ctx, B.Pt.reserve(sizeof(double) * nn));
+    for (int c = 0; c < nch; ++c) {
+        DcChannel& ch = B.ch[c];
+        HIP_TRY(ctx, ch.F.reserve(sizeof(double) * nn));
+        HIP_TRY(ctx, ch.P.reserve(sizeof(double) * nn));
+        HIP_TRY(ctx, ch.cbuf.reserve(sizeof(double) * mat_total));
+        HIP_TRY(ctx, ch.eps.reserve(sizeof(double) * L));
+        HIP_TRY(ctx, ch.proj.reserve(sizeof(double) * L));
+        HIP_TRY(ctx, ch.frac.reserve(sizeof(double) * L));
+        HIP_TRY(ctx, ch.pout.reserve(sizeof(double) * std::max<int64_t>(npat, 1)));
+        HIP_TRY(ctx, ch.hin.reserve(sizeof(double) * kDiisDepth * std::max<int64_t>(npat, 1)));
+        HIP_TRY(ctx, ch.hres.reserve(sizeof(double) * kDiisDepth * std::max<int64_t>(npat, 1)));
+        HIP_TRY(ctx, ch.ring.reserve(sizeof(int) * (kDiisDepth + 1)));
+    }
     HIP_TRY(ctx, hipMemcpyAsync(B.sub_n.p, sub_n.data(), sizeof(int) * n_sub, hipMemcpyHostToDevice, s));
     HIP_TRY(ctx, hipMemcpyAsync(B.sub_ao.p, sub_ao.data(), sizeof(int64_t) * (n_sub + 1), hipMemcpyHostToDevice, s));
     HIP_TRY(ctx, hipMemcpyAsync(B.sub_mat.p, sub_mat.data(), sizeof(int64_t) * n_sub, hipMemcpyHostToDevice, s));
@@ -321,61 +325,98 @@ int am1g_dc(am1g_ctx* ctx, const am1g_request* rq, const am1g_dc_options* op, am
     HIP_TRY(ctx, hipMemcpyAsync(B.ao_atom.p, ao_atom.data(), sizeof(int) * nao, hipMemcpyHostToDevice, s));
     HIP_TRY(ctx, hipMemcpyAsync(B.core_sub.p, core_sub.data(), sizeof(int) * nat, hipMemcpyHostToDevice, s));
 
-    double* F = B.F.as<double>();
-    double* P = B.P.as<double>();
-    HIP_TRY(ctx, hipMemsetAsync(P, 0, sizeof(double) * nn, s));
-    HIP_TRY(ctx, launch_sad(sa, 1.0, P, s));
+    // initial density: SAD, split between the spins in proportion to their electron counts
+    // (divide_conquer.rs:1143-1159)
+    const double n_alpha = md.n_alpha, n_beta = md.n_beta;
+    const double n_tot = std::max(n_alpha + n_beta, 1.0);
+    double* P[2] = {B.ch[0].P.as<double>(), uhf ? B.ch[1].P.as<double>() : nullptr};
+    double* F[2] = {B.ch[0].F.as<double>(), uhf ? B.ch[1].F.as<double>() : nullptr};
+    for (int c = 0; c < nch; ++c) {
+        HIP_TRY(ctx, hipMemsetAsync(P[c], 0, sizeof(double) * nn, s));
+        HIP_TRY(ctx, launch_sad(sa, uhf ? (c == 0 ? n_alpha : n_beta) / n_tot : 1.0, P[c], s));
+    }
+    double* Pt = uhf ? B.Pt.as<double>() : P[0];
 
-    DcArgs d;
-    d.n_sub = n_sub;
-    d.lds_n = 128;
-    d.sub_n = B.sub_n.as<int>();
-    d.sub_ao = B.sub_ao.as<int64_t>();
-    d.sub_mat = B.sub_mat.as<int64_t>();
-    d.ao_list = B.ao_list.as<int>();
-    d.ao_core = B.ao_core.as<unsigned char>();
-    d.fbuf = B.fbuf.as<double>();
-    d.cbuf = B.cbuf.as<double>();
-    d.t0 = B.t0.as<double>();
-    d.t1 = B.t1.as<double>();
-    d.eps = B.eps.as<double>();
-    d.proj = B.proj.as<double>();
-    d.frac = B.frac.as<double>();
-    d.cap = 2.0;
-    d.tol_abs = 0.0;
-    d.status = B.status.as<int>();
-    d.red = B.partial.as<double>();  // 1024 * 9 doubles >= 8 + 512 * 16
+    DcArgs d0;
+    d0.n_sub = n_sub;
+    d0.lds_n = 128;
+    d0.sub_n = B.sub_n.as<int>();
+    d0.sub_ao = B.sub_ao.as<int64_t>();
+    d0.sub_mat = B.sub_mat.as<int64_t>();
+    d0.ao_list = B.ao_list.as<int>();
+    d0.ao_core = B.ao_core.as<unsigned char>();
+    d0.fbuf = B.fbuf.as<double>();
+    d0.t0 = B.t0.as<double>();
+    d0.t1 = B.t1.as<double>();
+    d0.cap = uhf ? 1.0 : 2.0;
+    d0.tol_abs = 0.0;
+    d0.red = B.partial.as<double>();  // 1024 * 9 doubles >= 8 + 256 * 31
+    DcArgs dch[2];
+    for (int c = 0; c < 2; ++c) {
+        dch[c] = d0;
+        DcChannel& ch = B.ch[c < nch ? c : 0];
+        dch[c].cbuf = ch.cbuf.as<double>();
+        dch[c].eps = ch.eps.as<double>();
+        dch[c].proj = ch.proj.as<double>();
+        dch[c].frac = ch.frac.as<double>();
+        dch[c].status = B.status.as<int>() + c * n_sub;
+    }
+    const double n_elec_ch[2] = {uhf ? n_alpha : n_alpha + n_beta, n_beta};
 
-    double n_elec = 0.0;
-    for (int a = 0; a < nat; ++a) n_elec += tab[zs[a]].core;
-    n_elec -= rq->charge;
-
-    // ------------------------------------------------------------------ SCF loop (divide_conquer.rs:1052-1101)
+    // scal layout: 0 sum Pt.H (+ P.F for a closed shell), 3/4 sum P_s.F_s, 20 E_core,
+    // per channel c at 32 + 16 c: mu, T*S, count, -, and from +4 the DIIS sums (sum r^2, dots)
+    auto cbase = [](int c) { return 32 + 16 * c; };
     hipEvent_t ev[6];
     for (auto& e : ev) HIP_TRY(ctx, hipEventCreate(&e));
     double phase[5] = {0, 0, 0, 0, 0};
-    std::vector<int> ring;             // physical slots in logical order
-    std::vector<std::vector<double>> dots;  // lower triangle of <r_i, r_j>
-    std::vector<int> sub_status(n_sub);
+    struct HostDiis {
+        std::vector<int> ring;                  // physical slots in logical order
+        std::vector<std::vector<double>> dots;  // lower triangle of <r_i, r_j>
+    } hd[2];
+    std::vector<int> sub_status(2 * (size_t)n_sub);
     std::vector<double> host_scal(64);
     // host staging for per-iteration uploads; every iteration synchronizes before reusing them
     std::vector<double> coef_host(kDiisDepth + 1);
-    std::vector<int> ring_host(kDiisDepth + 1);
-    double e_old = 0.0, mu = 0.0, ts = 0.0;
+    std::vector<int> ring_host(2 * (kDiisDepth + 1));
+    double e_old = 0.0, mu[2] = {0.0, 0.0}, ts[2] = {0.0, 0.0};
     bool converged = false;
     int iterations = 0, status = AM1G_OK;
+    double* partial = B.partial.as<double>();
+
+    // Fock matrices of the current densities and the electronic energy sums
+    auto fock_energy = [&]() -> int {
+        if (uhf) {
+            HIP_TRY(ctx, launch_add((int64_t)nn, P[0], P[1], Pt, s));
+            HIP_TRY(ctx, launch_fock(sa, Pt, P[0], 1.0, F[0], s));
+            HIP_TRY(ctx, launch_fock(sa, Pt, P[1], 1.0, F[1], s));
+            HIP_TRY(ctx, launch_dot3((int64_t)nn, Pt, sa.H, nullptr, partial, scal + 0, s));
+            HIP_TRY(ctx, launch_dot3((int64_t)nn, P[0], F[0], nullptr, partial, scal + 3, s));
+            HIP_TRY(ctx, launch_dot3((int64_t)nn, P[1], F[1], nullptr, partial, scal + 4, s));
+        } else {
+            HIP_TRY(ctx, launch_fock(sa, P[0], P[0], 0.5, F[0], s));
+            HIP_TRY(ctx, launch_dot3((int64_t)nn, P[0], sa.H, F[0], partial, scal + 0, s));
+        }
+        return AM1G_OK;
+    };
+    auto e_of = [&](const std::vector<double>& h) { return uhf ? 0.5 * (h[0] + h[3] + h[4]) : 0.5 * h[0]; };
+
+    // ------------------------------------------------------------------ SCF loop (divide_conquer.rs:1033-1254)
     for (int iter = 0; iter < op->max_scf; ++iter) {
         iterations = iter + 1;
         HIP_TRY(ctx, hipEventRecord(ev[0], s));
-        HIP_TRY(ctx, launch_fock(sa, P, P, 0.5, F, s));
-        HIP_TRY(ctx, launch_dot3((int64_t)nn, P, sa.H, F, B.partial.as<double>(), scal + 0, s));
+        if ((st = fock_energy()) != AM1G_OK) return res->status = st;
         HIP_TRY(ctx, hipEventRecord(ev[1], s));
-        d.warm = iter > 0 ? 1 : 0;
-        HIP_TRY(ctx, launch_dc_eigen(d, F, nao, s));
+        for (int c = 0; c < nch; ++c) {
+            dch[c].warm = iter > 0 ? 1 : 0;
+            HIP_TRY(ctx, launch_dc_eigen(dch[c], F[c], nao, s));
+        }
         HIP_TRY(ctx, hipEventRecord(ev[2], s));
-        if (op->kt > 0.0) {
-            HIP_TRY(ctx, launch_fermi(d, L, n_elec, op->kt, scal + 1, s));
-        } else {
+        for (int c = 0; c < nch; ++c) {
+            const DcArgs& d = dch[c];
+            if (op->kt > 0.0) {
+                HIP_TRY(ctx, launch_fermi(d, L, n_elec_ch[c], op->kt, scal + cbase(c), s));
+                continue;
+            }
             // aufbau filling (fermi.rs:153-206) on the host: sort the levels, fill degenerate
             // groups evenly
             std::vector<double> e(L), w(L), f(L, 0.0);
@@ -386,7 +427,7 @@ int am1g_dc(am1g_ctx* ctx, const am1g_request* rq, const am1g_dc_options* op, am
             std::vector<int64_t> order(L);
             std::iota(order.begin(), order.end(), 0);
             std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return e[a] < e[b]; });
-            double left = n_elec, fermi = L ? e[order.back()] : 0.0;
+            double left = n_elec_ch[c], fermi = L ? e[order.back()] : 0.0;
             int64_t i = 0;
             while (i < L) {
                 if (left <= 1e-14) {
@@ -405,37 +446,51 @@ int am1g_dc(am1g_ctx* ctx, const am1g_request* rq, const am1g_dc_options* op, am
                 i = j;
             }
             HIP_TRY(ctx, hipMemcpyAsync(d.frac, f.data(), sizeof(double) * L, hipMemcpyHostToDevice, s));
-            host_scal[1] = fermi;
-            host_scal[2] = 0.0;
-            HIP_TRY(ctx, hipMemcpyAsync(scal + 1, &host_scal[1], sizeof(double) * 2, hipMemcpyHostToDevice, s));
+            host_scal[cbase(c)] = fermi;
+            host_scal[cbase(c) + 1] = 0.0;
+            HIP_TRY(ctx, hipMemcpyAsync(scal + cbase(c), &host_scal[cbase(c)], sizeof(double) * 2,
+                                        hipMemcpyHostToDevice, s));
             HIP_TRY(ctx, hipStreamSynchronize(s));
         }
         HIP_TRY(ctx, hipEventRecord(ev[3], s));
-        HIP_TRY(ctx, launch_dc_density(d, s));
-        HIP_TRY(ctx, launch_dc_gather(d, npat, B.pat.as<int64_t>(), B.ao_atom.as<int>(), B.core_sub.as<int>(), nao,
-                                      B.pout.as<double>(), s));
+        // subsystem densities, assembly and the DIIS sums per channel
+        int slot[2] = {0, 0}, mdots[2] = {0, 0};
+        std::vector<int> ring_next[2];
+        for (int c = 0; c < nch; ++c) {
+            DcChannel& ch = B.ch[c];
+            HIP_TRY(ctx, launch_dc_density(dch[c], s));
+            HIP_TRY(ctx, launch_dc_gather(dch[c], npat, B.pat.as<int64_t>(), B.ao_atom.as<int>(),
+                                          B.core_sub.as<int>(), nao, ch.pout.as<double>(), s));
+            // next history slot: evict the oldest when full
+            const std::vector<int>& ring = hd[c].ring;
+            slot[c] = (int)ring.size() < max_depth ? (int)ring.size() : ring.front();
+            ring_next[c] = ring;
+            if ((int)ring_next[c].size() >= max_depth) ring_next[c].erase(ring_next[c].begin());
+            int* rh = ring_host.data() + c * (kDiisDepth + 1);
+            std::copy(ring_next[c].begin(), ring_next[c].end(), rh);
+            rh[ring_next[c].size()] = -1;  // the new entry, dotted with itself
+            mdots[c] = (int)ring_next[c].size() + 1;
+            HIP_TRY(ctx, hipMemcpyAsync(ch.ring.p, rh, sizeof(int) * mdots[c], hipMemcpyHostToDevice, s));
+            HIP_TRY(ctx, launch_dc_hist(npat, B.pat.as<int64_t>(), B.mult.as<double>(), P[c], ch.pout.as<double>(),
+                                        ch.hin.as<double>() + (int64_t)slot[c] * npat,
+                                        ch.hres.as<double>() + (int64_t)slot[c] * npat, ch.hres.as<double>(),
+                                        ch.ring.as<int>(), mdots[c], npat, partial, scal + cbase(c) + 4, s));
+        }
         HIP_TRY(ctx, hipEventRecord(ev[4], s));
-        // next history slot: evict the oldest when full
-        const int slot = (int)ring.size() < max_depth ? (int)ring.size() : ring.front();
-        std::vector<int> ring_next = ring;
-        if ((int)ring_next.size() >= max_depth) ring_next.erase(ring_next.begin());
-        std::copy(ring_next.begin(), ring_next.end(), ring_host.begin());
-        ring_host[ring_next.size()] = -1;  // the new entry, dotted with itself
-        const int m = (int)ring_next.size() + 1;
-        HIP_TRY(ctx, hipMemcpyAsync(B.ring.p, ring_host.data(), sizeof(int) * m, hipMemcpyHostToDevice, s));
-        HIP_TRY(ctx, launch_dc_hist(npat, B.pat.as<int64_t>(), B.mult.as<double>(), P, B.pout.as<double>(),
-                                    B.hin.as<double>() + (int64_t)slot * npat, B.hres.as<double>() + (int64_t)slot * npat,
-                                    B.hres.as<double>(), B.ring.as<int>(), m, npat, B.partial.as<double>(), scal + 8, s));
-        HIP_TRY(ctx, hipMemcpyAsync(host_scal.data(), scal, sizeof(double) * 24, hipMemcpyDeviceToHost, s));
-        HIP_TRY(ctx, hipMemcpyAsync(sub_status.data(), d.status, sizeof(int) * n_sub, hipMemcpyDeviceToHost, s));
+        HIP_TRY(ctx, hipMemcpyAsync(host_scal.data(), scal, sizeof(double) * 64, hipMemcpyDeviceToHost, s));
+        HIP_TRY(ctx, hipMemcpyAsync(sub_status.data(), B.status.p, sizeof(int) * nch * n_sub, hipMemcpyDeviceToHost,
+                                    s));
         HIP_TRY(ctx, hipStreamSynchronize(s));
-        for (int q = 0; q < n_sub; ++q)
+        for (int q = 0; q < nch * n_sub; ++q)
             if (sub_status[q] < 0) status = AM1G_LINEAR_ALGEBRA;
         if (status != AM1G_OK) break;
-        const double e_elec = 0.5 * host_scal[0];
-        mu = host_scal[1];
-        ts = host_scal[2];
-        const double dp = std::sqrt(host_scal[8] / (double)std::max<size_t>(nn, 1));
+        const double e_elec = e_of(host_scal);
+        double dp = 0.0;
+        for (int c = 0; c < nch; ++c) {
+            mu[c] = host_scal[cbase(c)];
+            ts[c] = host_scal[cbase(c) + 1];
+            dp = std::max(dp, std::sqrt(host_scal[cbase(c) + 4] / (double)std::max<size_t>(nn, 1)));
+        }
         const double de = std::fabs(e_elec - e_old);
         e_old = e_elec;
         if (!std::isfinite(e_elec) || !std::isfinite(dp)) {
@@ -443,60 +498,68 @@ int am1g_dc(am1g_ctx* ctx, const am1g_request* rq, const am1g_dc_options* op, am
             break;
         }
         if (iter > 0 && de < op->e_tol && dp < op->p_tol) {
-            HIP_TRY(ctx, launch_dc_update(npat, B.pat.as<int64_t>(), nao, nullptr, nullptr, nullptr, nullptr, 0, npat,
-                                          -1.0, P, B.pout.as<double>(), P, s));
+            for (int c = 0; c < nch; ++c)
+                HIP_TRY(ctx, launch_dc_update(npat, B.pat.as<int64_t>(), nao, nullptr, nullptr, nullptr, nullptr, 0,
+                                              npat, -1.0, P[c], B.ch[c].pout.as<double>(), P[c], s));
             converged = true;
         } else {
-            // DIIS bookkeeping: the new entry takes `slot`; its dots extend the cached Gram matrix
-            ring = ring_next;
-            if ((int)dots.size() >= max_depth) {
-                dots.erase(dots.begin());
-                for (auto& row : dots) row.erase(row.begin());
-            }
-            std::vector<double> row(m);
-            for (int j = 0; j < m; ++j) row[j] = host_scal[9 + j];
-            dots.push_back(row);
-            ring.push_back(slot);
-            const int n = (int)ring.size();
-            bool used = false;
-            if (n >= 2) {
-                const int dim = n + 1;
-                std::vector<double> Bm((size_t)dim * dim, 0.0), rhs(dim, 0.0), c;
-                double scale = 0.0;
-                for (int i = 0; i < n; ++i) {
-                    for (int j = 0; j <= i; ++j) {
-                        Bm[i * dim + j] = Bm[j * dim + i] = dots[i][j];
-                        scale = std::max(scale, std::fabs(dots[i][j]));
-                    }
-                    Bm[i * dim + n] = Bm[n * dim + i] = -1.0;
+            for (int c = 0; c < nch; ++c) {
+                DcChannel& ch = B.ch[c];
+                HostDiis& h = hd[c];
+                // DIIS bookkeeping: the new entry takes `slot`; its dots extend the cached Gram matrix
+                h.ring = ring_next[c];
+                if ((int)h.dots.size() >= max_depth) {
+                    h.dots.erase(h.dots.begin());
+                    for (auto& row : h.dots) row.erase(row.begin());
                 }
-                if (scale > 0.0) {
+                std::vector<double> row(mdots[c]);
+                for (int j = 0; j < mdots[c]; ++j) row[j] = host_scal[cbase(c) + 5 + j];
+                h.dots.push_back(row);
+                h.ring.push_back(slot[c]);
+                const int n = (int)h.ring.size();
+                bool used = false;
+                if (n >= 2) {
+                    const int dim = n + 1;
+                    std::vector<double> Bm((size_t)dim * dim, 0.0), rhs(dim, 0.0), cf;
+                    double scale = 0.0;
                     for (int i = 0; i < n; ++i) {
-                        for (int j = 0; j < n; ++j) Bm[i * dim + j] /= scale;
-                        Bm[i * dim + i] += 1e-10;
+                        for (int j = 0; j <= i; ++j) {
+                            Bm[i * dim + j] = Bm[j * dim + i] = h.dots[i][j];
+                            scale = std::max(scale, std::fabs(h.dots[i][j]));
+                        }
+                        Bm[i * dim + n] = Bm[n * dim + i] = -1.0;
                     }
-                    rhs[n] = -1.0;
-                    if (lu_solve(Bm, rhs, dim, c)) {
-                        bool finite = true;
-                        for (int i = 0; i < n; ++i) finite = finite && std::isfinite(c[i]);
-                        if (finite) {
-                            std::copy(c.begin(), c.begin() + n, coef_host.begin());
-                            std::copy(ring.begin(), ring.end(), ring_host.begin());
-                            HIP_TRY(ctx, hipMemcpyAsync(B.coef.p, coef_host.data(), sizeof(double) * n,
-                                                        hipMemcpyHostToDevice, s));
-                            HIP_TRY(ctx, hipMemcpyAsync(B.ring.p, ring_host.data(), sizeof(int) * n,
-                                                        hipMemcpyHostToDevice, s));
-                            HIP_TRY(ctx, launch_dc_update(npat, B.pat.as<int64_t>(), nao, B.coef.as<double>(),
-                                                          B.hin.as<double>(), B.hres.as<double>(), B.ring.as<int>(), n,
-                                                          npat, 0.0, P, B.pout.as<double>(), P, s));
-                            used = true;
+                    if (scale > 0.0) {
+                        for (int i = 0; i < n; ++i) {
+                            for (int j = 0; j < n; ++j) Bm[i * dim + j] /= scale;
+                            Bm[i * dim + i] += 1e-10;
+                        }
+                        rhs[n] = -1.0;
+                        if (lu_solve(Bm, rhs, dim, cf)) {
+                            bool finite = true;
+                            for (int i = 0; i < n; ++i) finite = finite && std::isfinite(cf[i]);
+                            if (finite) {
+                                std::copy(cf.begin(), cf.begin() + n, coef_host.begin());
+                                int* rh = ring_host.data() + c * (kDiisDepth + 1);
+                                std::copy(h.ring.begin(), h.ring.end(), rh);
+                                HIP_TRY(ctx, hipMemcpyAsync(B.coef.p, coef_host.data(), sizeof(double) * n,
+                                                            hipMemcpyHostToDevice, s));
+                                HIP_TRY(ctx, hipMemcpyAsync(ch.ring.p, rh, sizeof(int) * n, hipMemcpyHostToDevice, s));
+                                HIP_TRY(ctx, launch_dc_update(npat, B.pat.as<int64_t>(), nao, B.coef.as<double>(),
+                                                              ch.hin.as<double>(), ch.hres.as<double>(),
+                                                              ch.ring.as<int>(), n, npat, 0.0, P[c],
+                                                              ch.pout.as<double>(), P[c], s));
+                                // coef_host is reused by the other channel: finish this upload first
+                                HIP_TRY(ctx, hipStreamSynchronize(s));
+                                used = true;
+                            }
                         }
                     }
                 }
+                if (!used)
+                    HIP_TRY(ctx, launch_dc_update(npat, B.pat.as<int64_t>(), nao, nullptr, nullptr, nullptr, nullptr,
+                                                  0, npat, op->mixing, P[c], ch.pout.as<double>(), P[c], s));
             }
-            if (!used)
-                HIP_TRY(ctx, launch_dc_update(npat, B.pat.as<int64_t>(), nao, nullptr, nullptr, nullptr, nullptr, 0,
-                                              npat, op->mixing, P, B.pout.as<double>(), P, s));
         }
         HIP_TRY(ctx, hipEventRecord(ev[5], s));
         HIP_TRY(ctx, hipEventSynchronize(ev[5]));
@@ -510,47 +573,52 @@ int am1g_dc(am1g_ctx* ctx, const am1g_request* rq, const am1g_dc_options* op, am
     for (auto& e : ev) (void)hipEventDestroy(e);
 
     // ------------------------------------------------------------------ final energy and properties
-    HIP_TRY(ctx, launch_fock(sa, P, P, 0.5, F, s));
-    HIP_TRY(ctx, launch_dot3((int64_t)nn, P, sa.H, F, B.partial.as<double>(), scal + 0, s));
-    HIP_TRY(ctx, hipMemcpyAsync(host_scal.data(), scal, sizeof(double) * 24, hipMemcpyDeviceToHost, s));
-    std::vector<double> pd(10 * (size_t)nat), eps(L);
+    if ((st = fock_energy()) != AM1G_OK) return res->status = st;
+    HIP_TRY(ctx, hipMemcpyAsync(host_scal.data(), scal, sizeof(double) * 64, hipMemcpyDeviceToHost, s));
+    std::vector<double> pd(10 * (size_t)nat), eps[2];
     HIP_TRY(ctx, hipMemcpyAsync(pd.data(), sa.pdiag, sizeof(double) * 10 * nat, hipMemcpyDeviceToHost, s));
-    HIP_TRY(ctx, hipMemcpyAsync(eps.data(), d.eps, sizeof(double) * L, hipMemcpyDeviceToHost, s));
+    for (int c = 0; c < nch; ++c) {
+        eps[c].resize(L);
+        HIP_TRY(ctx, hipMemcpyAsync(eps[c].data(), dch[c].eps, sizeof(double) * L, hipMemcpyDeviceToHost, s));
+    }
     HIP_TRY(ctx, hipStreamSynchronize(s));
-    const double electronic = 0.5 * host_scal[0];
+    const double electronic = e_of(host_scal);
     const double ecore = host_scal[20];
     double eiso = 0.0, eheat = 0.0;
     for (int a = 0; a < nat; ++a) {
         eiso += tab[zs[a]].e_isol;
         eheat += tab[zs[a]].eheat_ev;
-        // Mulliken population from the packed diagonal block (entries 0, 2, 5, 9 are the diagonal)
+        // Mulliken population from the packed diagonal block of the total density (entries 0,
+        // 2, 5, 9 are the diagonal)
         double pop = pd[10 * a];
         if (zs[a] != 1) pop += pd[10 * a + 2] + pd[10 * a + 5] + pd[10 * a + 9];
         if (res->charges) res->charges[a] = tab[zs[a]].core - pop;
     }
-    double gap = INFINITY;
-    for (int c = 0; c < n_sub; ++c) {
-        double below = -INFINITY, above = INFINITY;
-        for (int64_t i = sub_ao[c]; i < sub_ao[c + 1]; ++i) {
-            if (eps[i] <= mu) below = std::max(below, eps[i]);
-            else above = std::min(above, eps[i]);
+    double gap = INFINITY;  // smallest_gap per channel (divide_conquer.rs:659-683), the smaller one
+    for (int c = 0; c < nch; ++c)
+        for (int q = 0; q < n_sub; ++q) {
+            double below = -INFINITY, above = INFINITY;
+            for (int64_t i = sub_ao[q]; i < sub_ao[q + 1]; ++i) {
+                if (eps[c][i] <= mu[c]) below = std::max(below, eps[c][i]);
+                else above = std::min(above, eps[c][i]);
+            }
+            if (std::isfinite(below) && std::isfinite(above)) gap = std::min(gap, above - below);
         }
-        if (std::isfinite(below) && std::isfinite(above)) gap = std::min(gap, above - below);
-    }
-    if (res->density) HIP_TRY(ctx, hipMemcpy(res->density, P, sizeof(double) * nn, hipMemcpyDeviceToHost));
+    if (res->density) HIP_TRY(ctx, hipMemcpy(res->density, Pt, sizeof(double) * nn, hipMemcpyDeviceToHost));
     res->iterations = iterations;
     res->converged = converged;
-    res->unrestricted = 0;
+    res->unrestricted = uhf ? 1 : 0;
     res->n_subsystems = n_sub;
     res->largest_subsystem_aos = largest;
     res->electronic_ev = electronic;
     res->core_ev = ecore;
     res->total_ev = electronic + ecore;
     res->heat_of_formation_kcal = (electronic + ecore - eiso + eheat) * kEvToKcal;
-    res->fermi_ev = mu;
-    res->entropy_ev = ts;
+    res->fermi_ev = mu[0];
+    res->fermi_beta_ev = uhf ? mu[1] : mu[0];
+    res->entropy_ev = ts[0] + (uhf ? ts[1] : 0.0);
     res->homo_lumo_gap_ev = std::isfinite(gap) ? gap : 0.0;
-    res->diagonalization_work = diag_work;
+    res->diagonalization_work = diag_work * nch;
     res->retained_pattern = npat;
     for (int k = 0; k < 5; ++k) res->phase_ms[k] = phase[k];
     ctx->last_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();

@@ -140,6 +140,10 @@ __global__ __launch_bounds__(256) void k_density_update(int n, const TT* Pn, dou
     block_partials<1>(v, partial);
 }
 
+__global__ __launch_bounds__(256) void k_add(int64_t len, const double* x, const double* y, double* z) {
+    for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < len; k += (int64_t)gridDim.x * 256) z[k] = x[k] + y[k];
+}
+
 __global__ __launch_bounds__(256) void k_to_f32(int64_t len, const double* x, float* y) {
     for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < len; k += (int64_t)gridDim.x * 256)
         y[k] = (float)x[k];
@@ -207,6 +211,11 @@ hipError_t launch_density_update(int n, const void* Pn, bool f32, double* P, dou
     else
         hipLaunchKernelGGL(k_density_update<double>, grid, dim3(256), 0, st, n, (const double*)Pn, P, partial);
     hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, st, (int64_t)grid.x * grid.y, 1, partial, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_add(int64_t len, const double* x, const double* y, double* z, hipStream_t st) {
+    hipLaunchKernelGGL(k_add, dim3((unsigned)std::min<int64_t>(nblk(len, 256), 65536)), dim3(256), 0, st, len, x, y, z);
     return hipGetLastError();
 }
 

@@ -111,6 +111,7 @@ hipError_t launch_combine(int n, const double* hf, const int* ring, const double
 hipError_t launch_density_update(int n, const void* Pn, bool f32, double* P, double* partial, double* out,
                                  hipStream_t st);
 hipError_t launch_to_f32(int64_t len, const double* x, float* y, hipStream_t st);
+hipError_t launch_add(int64_t len, const double* x, const double* y, double* z, hipStream_t st);
 hipError_t launch_atom_props(const SysArgs& a, const double* P, double* out, hipStream_t st);
 hipError_t launch_sum(int64_t n, const double* x, double* partial, double* out, hipStream_t st);
 

@@ -125,7 +125,9 @@ typedef struct {
     int status, iterations, converged, unrestricted;
     int n_subsystems, largest_subsystem_aos;
     double electronic_ev, core_ev, total_ev, heat_of_formation_kcal;
-    double fermi_ev, entropy_ev, homo_lumo_gap_ev;
+    double fermi_ev;                 /* chemical potential (alpha channel when unrestricted) */
+    double fermi_beta_ev;            /* beta channel's chemical potential (= fermi_ev for RHF) */
+    double entropy_ev, homo_lumo_gap_ev;
     double diagonalization_work;     /* sum over subsystems of n^3 */
     long long retained_pattern;      /* AO pairs of the retained density pattern (upper triangle) */
     double phase_ms[5];              /* Fock, subsystem eigensolves, Fermi level, assembly, DIIS */
@@ -135,8 +137,9 @@ typedef struct {
 
 void am1g_dc_defaults(am1g_dc_options* op);
 
-/* Divide-and-conquer SCF on the GPU: run_divide_conquer (src/divide_conquer.rs:406-572), closed
- * shell. All subsystem eigensolves of an iteration run as one batched launch. */
+/* Divide-and-conquer SCF on the GPU: run_divide_conquer (src/divide_conquer.rs:406-572), RHF and
+ * UHF (dc_loop_restricted / dc_loop_unrestricted, one chemical potential per spin). All subsystem
+ * eigensolves of an iteration run as one batched launch per spin channel. */
 int am1g_dc(am1g_ctx* ctx, const am1g_request* rq, const am1g_dc_options* op, am1g_dc_result* res);
 
 /* ---------------------------------------------------------------- serving

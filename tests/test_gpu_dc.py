@@ -75,10 +75,38 @@ def test_repeat_call_reuses_buffers_and_is_deterministic(engine):
 def test_error_paths(engine):
     z, x = M.water_chain(3)
     assert engine.dc(z, x, buffer_radius=0.0)["status"] == 1
+    assert engine.dc(z, x, multiplicity=2)["status"] == 1  # even electron count, doublet
+    assert engine.dc(np.array([2, 1, 1]), x[:3])["status"] == 2
+
+
+def _radical_chain(n, drop=4):
+    z, x = M.water_chain(n)
     keep = np.ones(len(z), dtype=bool)
-    keep[4] = False
-    r = engine.dc(z[keep], x[keep], multiplicity=2)
-    assert r["status"] == 1 and "unrestricted" in r["message"]
+    keep[drop] = False  # one hydrogen less: a neutral doublet
+    return z[keep], x[keep]
+
+
+def test_open_shell_full_buffer_matches_uhf(engine):
+    # tests/divide_conquer.rs:305-353: two chemical potentials reproduce the UHF doublet
+    z, x = _radical_chain(3)
+    full = engine.scf(z, x, multiplicity=2, e_tol=1e-11, p_tol=1e-10, full=False)
+    kw = dict(multiplicity=2, core_size=3, buffer_radius=500.0, kt=0.0, e_tol=1e-11, p_tol=1e-10, max_scf=400)
+    g = engine.dc(z, x, **kw)
+    assert g["unrestricted"] and g["converged"]
+    assert abs(g["total_ev"] - full["total_ev"]) < 1e-6
+    _compare(g, O.dc(z, x, **kw))
+
+
+@pytest.mark.parametrize("kt", [0.0, 0.05])
+def test_open_shell_partial_buffer(engine, kt):
+    z, x = _radical_chain(6)
+    kw = dict(multiplicity=2, core_size=6, buffer_radius=9.0, kt=kt)
+    g = engine.dc(z, x, **kw)
+    o = O.dc(z, x, **kw)
+    assert g["unrestricted"]
+    _compare(g, o)
+    assert abs(g["fermi_beta_ev"] - o["fermi_beta_ev"]) <= 1e-5
+    assert abs(g["charges"].sum()) < 1e-8
 
 
 def test_water2000_full_size_against_oracle_fixture(engine):
