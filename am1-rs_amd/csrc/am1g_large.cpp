@@ -17,8 +17,9 @@
 // Device buffers of the large-system SCF, kept on the context so repeated calls reuse them.
 struct LargeBuffers {
     DevBuf A, evals, Ct;                       // eigensolver entry point
-    DevBuf H, F, P, T0, T1, Pn, f32a, f32b, f32c;  // dense n x n (FP32 staging for mixed precision)
+    DevBuf H, F, P, T0, T1, Pn, f32a, f32b;    // dense n x n (FP32 staging for mixed precision)
     DevBuf hist_f, hist_e, hist_d, ring, coef, partial, scal, pdiag, props;
+    DevBuf Fb, Pb, Pt, Tb, Ctb, evals_b, hist_fb, hist_eb;  // beta channel (UHF)
 };
 
 int am1g_large_setup(am1g_ctx* ctx) {
@@ -57,16 +58,279 @@ double ms_since(Clock::time_point& t) {
 
 }  // namespace
 
+// Unrestricted SCF of one large molecule (uhf_loop, scf.rs:719-858): F_s = H + J(Pa + Pb) - K(P_s),
+// E = 1/2 (Pt.H + Pa.Fa + Pb.Fb), CDIIS on the stacked error [Ea ; Eb] with one coefficient set
+// for both spins whenever DIIS is on, the divergence check after the extrapolation, and the sum
+// of the two channels' rms density changes. The system tables, pair pool, H_core and the
+// core-core energy are already on the device.
+static int large_uhf(am1g_ctx* ctx, const am1g_request& rq, const MolDesc& md, const SysArgs& sa, LargeBuffers& L,
+                     double ecore, double* phase, Clock::time_point& t_ph, am1g_result* res) {
+    hipStream_t s = ctx->stream;
+    const int nat = md.nat, n = md.nao, na = md.n_alpha, nb = md.n_beta;
+    const size_t nn = (size_t)n * n;
+    const DevElem* tab = ctx->host_tab + md.method * kMaxZ;
+    const int64_t len_s = (int64_t)n * (n + 1) / 2, len_a = (int64_t)n * (n - 1) / 2;
+    const bool diis = md.use_diis != 0;
+    HIP_TRY(ctx, L.F.reserve(sizeof(double) * nn));
+    HIP_TRY(ctx, L.Fb.reserve(sizeof(double) * nn));
+    HIP_TRY(ctx, L.P.reserve(sizeof(double) * nn));
+    HIP_TRY(ctx, L.Pb.reserve(sizeof(double) * nn));
+    HIP_TRY(ctx, L.Pt.reserve(sizeof(double) * nn));
+    HIP_TRY(ctx, L.T0.reserve(sizeof(double) * nn));
+    HIP_TRY(ctx, L.T1.reserve(sizeof(double) * nn));
+    HIP_TRY(ctx, L.Tb.reserve(sizeof(double) * nn));
+    HIP_TRY(ctx, L.Pn.reserve(sizeof(double) * nn));
+    HIP_TRY(ctx, L.Ct.reserve(sizeof(double) * nn));
+    HIP_TRY(ctx, L.Ctb.reserve(sizeof(double) * nn));
+    HIP_TRY(ctx, L.evals.reserve(sizeof(double) * n));
+    HIP_TRY(ctx, L.evals_b.reserve(sizeof(double) * n));
+    HIP_TRY(ctx, L.hist_f.reserve(sizeof(double) * kDiisDepth * len_s));
+    HIP_TRY(ctx, L.hist_fb.reserve(sizeof(double) * kDiisDepth * len_s));
+    HIP_TRY(ctx, L.hist_e.reserve(sizeof(double) * kDiisDepth * std::max<int64_t>(len_a, 1)));
+    HIP_TRY(ctx, L.hist_eb.reserve(sizeof(double) * kDiisDepth * std::max<int64_t>(len_a, 1)));
+    if (md.precision == 1) {
+        HIP_TRY(ctx, L.f32a.reserve(sizeof(float) * nn));
+        HIP_TRY(ctx, L.f32b.reserve(sizeof(float) * nn));
+    }
+    double* H = sa.H;
+    double* F[2] = {L.F.as<double>(), L.Fb.as<double>()};
+    double* P[2] = {L.P.as<double>(), L.Pb.as<double>()};
+    double* Ct[2] = {L.Ct.as<double>(), L.Ctb.as<double>()};
+    double* eps[2] = {L.evals.as<double>(), L.evals_b.as<double>()};
+    double* hf[2] = {L.hist_f.as<double>(), L.hist_fb.as<double>()};
+    double* he[2] = {L.hist_e.as<double>(), L.hist_eb.as<double>()};
+    double* comb[2] = {L.T1.as<double>(), L.Tb.as<double>()};
+    double* Pt = L.Pt.as<double>();
+    double* T0 = L.T0.as<double>();
+    double* Pn = L.Pn.as<double>();
+    double* partial = L.partial.as<double>();
+    double* scal = L.scal.as<double>();
+    int* dring = L.ring.as<int>();
+    double* dcoef = L.coef.as<double>();
+    const int nocc[2] = {na, nb};
+    // SAD guess split by the alpha / beta electron fractions (scf.rs:730-740)
+    const double ntot = (double)std::max(na + nb, 1);
+    for (int c = 0; c < 2; ++c) {
+        HIP_TRY(ctx, hipMemsetAsync(P[c], 0, sizeof(double) * nn, s));
+        HIP_TRY(ctx, launch_sad(sa, (c == 0 ? na : nb) / ntot, P[c], s));
+    }
+    // scal layout: 0 sum Pt.H, 1/2 sum Ps.Fs, 3/4 sum E_s^2, 5..12 / 13..20 DIIS dots, 21/22 sum dP^2
+    double h_scal[64];
+    double B[kDiisDepth * kDiisDepth] = {0};
+    int hcount = 0, head = 0, ring[kDiisDepth];
+    double e_old = 0.0, residual = INFINITY;
+    int iterations = 0, converged = 0, status = AM1G_OK, fp32_iters = 0;
+    bool fp32_phase = md.precision == 1;
+    double fp32_best = INFINITY;
+    int fp32_stall = 0;
+    const double one = 1.0, zero = 0.0;
+    const float onef = 1.0f, zerof = 0.0f;
+    std::string eig_err;
+    auto fock_energy = [&]() -> int {
+        HIP_TRY(ctx, launch_add((int64_t)nn, P[0], P[1], Pt, s));
+        HIP_TRY(ctx, launch_fock(sa, Pt, P[0], 1.0, F[0], s));
+        HIP_TRY(ctx, launch_fock(sa, Pt, P[1], 1.0, F[1], s));
+        HIP_TRY(ctx, launch_dot3((int64_t)nn, Pt, H, nullptr, partial, scal + 0, s));
+        HIP_TRY(ctx, launch_dot3((int64_t)nn, P[0], F[0], nullptr, partial, scal + 1, s));
+        HIP_TRY(ctx, launch_dot3((int64_t)nn, P[1], F[1], nullptr, partial, scal + 2, s));
+        return AM1G_OK;
+    };
+    int st;
+    for (int iter = 0; iter < md.max_scf; ++iter) {
+        iterations = iter + 1;
+        if ((st = fock_energy()) != AM1G_OK) return res->status = st;
+        phase[1] += ms_since(t_ph);
+        const bool low = fp32_phase;
+        const int slot = diis ? (hcount < kDiisDepth ? (head + hcount) % kDiisDepth : head) : 0;
+        for (int c = 0; c < 2; ++c) {
+            if (rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, P[c], n, F[c], n,
+                              &zero, T0, n) != rocblas_status_success)
+                return res->status = am1g_set_error(ctx, AM1G_DEVICE_ERROR, "rocBLAS dgemm failed");
+            HIP_TRY(ctx, launch_push(n, T0, false, F[c], P[c], hf[c] + slot * len_s, he[c] + slot * len_a, nullptr,
+                                     partial, scal + 3 + c, s));
+        }
+        int m = 0;
+        if (diis) {
+            if (hcount < kDiisDepth) {
+                hcount++;
+            } else {
+                head = (head + 1) % kDiisDepth;
+                for (int i = 0; i < kDiisDepth - 1; ++i)
+                    for (int j = 0; j < kDiisDepth - 1; ++j) B[i * kDiisDepth + j] = B[(i + 1) * kDiisDepth + j + 1];
+            }
+            for (int i = 0; i < hcount; ++i) ring[i] = (head + i) % kDiisDepth;
+            m = hcount;
+            HIP_TRY(ctx, hipMemcpyAsync(dring, ring, sizeof(int) * m, hipMemcpyHostToDevice, s));
+            for (int c = 0; c < 2; ++c)
+                HIP_TRY(ctx, launch_diis_dots(n, hf[c], he[c], nullptr, dring, m, m - 1, partial, scal + 5 + 8 * c, s));
+        }
+        HIP_TRY(ctx, hipMemcpyAsync(h_scal, scal, sizeof(double) * 24, hipMemcpyDeviceToHost, s));
+        HIP_TRY(ctx, hipStreamSynchronize(s));
+        const double e_elec = 0.5 * (h_scal[0] + h_scal[1] + h_scal[2]);
+        const double err = std::sqrt(2.0 * (h_scal[3] + h_scal[4]));
+        const double* Fuse[2] = {F[0], F[1]};
+        if (diis) {
+            const int newest = m - 1;
+            for (int i = 0; i < m; ++i) {
+                const double bi = 2.0 * (h_scal[5 + i] + h_scal[13 + i]);
+                B[newest * kDiisDepth + i] = bi;
+                B[i * kDiisDepth + newest] = bi;
+            }
+            if (m >= 2) {
+                double Bm[(kDiisDepth + 1) * (kDiisDepth + 1)], rhs[kDiisDepth + 1], c[kDiisDepth + 1];
+                const int dim = m + 1;
+                for (int i = 0; i < m; ++i) {
+                    for (int j = 0; j < m; ++j) Bm[i * dim + j] = B[i * kDiisDepth + j];
+                    Bm[i * dim + m] = -1.0;
+                    Bm[m * dim + i] = -1.0;
+                    rhs[i] = 0.0;
+                }
+                Bm[m * dim + m] = 0.0;
+                rhs[m] = -1.0;
+                if (solve_bordered(Bm, rhs, dim, c)) {
+                    HIP_TRY(ctx, hipMemcpyAsync(dcoef, c, sizeof(double) * m, hipMemcpyHostToDevice, s));
+                    for (int q = 0; q < 2; ++q) {
+                        HIP_TRY(ctx, launch_combine(n, hf[q], dring, dcoef, m, comb[q], partial, scal + 23, s));
+                        Fuse[q] = comb[q];
+                    }
+                    HIP_TRY(ctx, hipStreamSynchronize(s));  // c is a stack array
+                }
+            }
+        }
+        if (!std::isfinite(e_elec) || !std::isfinite(err)) {
+            status = AM1G_SCF_DIVERGED;
+            break;
+        }
+        phase[2] += ms_since(t_ph);
+        for (int c = 0; c < 2; ++c) {
+            hipError_t e = eig_large(*ctx->eig, ctx->blas, n, const_cast<double*>(Fuse[c]), eps[c], Ct[c], s, eig_err);
+            if (e != hipSuccess) {
+                if (eig_err.empty())
+                    return res->status = am1g_set_error(ctx, AM1G_DEVICE_ERROR, "HIP error %s in the eigensolver",
+                                                        hipGetErrorString(e));
+                status = AM1G_LINEAR_ALGEBRA;
+            }
+        }
+        phase[3] += ms_since(t_ph);
+        if (status != AM1G_OK) break;
+        if (low) ++fp32_iters;
+        double dsum = 0.0;
+        for (int c = 0; c < 2; ++c) {
+            if (nocc[c] > 0 && low) {
+                float* Cf = L.f32a.as<float>();
+                float* Pf = L.f32b.as<float>();
+                HIP_TRY(ctx, launch_to_f32((int64_t)n * nocc[c], Ct[c], Cf, s));
+                if (rocblas_sgemm(ctx->blas, rocblas_operation_none, rocblas_operation_transpose, n, n, nocc[c], &onef,
+                                  Cf, n, Cf, n, &zerof, Pf, n) != rocblas_status_success)
+                    return res->status = am1g_set_error(ctx, AM1G_DEVICE_ERROR, "rocBLAS sgemm failed");
+                HIP_TRY(ctx, launch_density_update(n, Pf, true, P[c], partial, scal + 21 + c, s));
+            } else {
+                if (nocc[c] > 0) {
+                    if (rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_transpose, n, n, nocc[c],
+                                      &one, Ct[c], n, Ct[c], n, &zero, Pn, n) != rocblas_status_success)
+                        return res->status = am1g_set_error(ctx, AM1G_DEVICE_ERROR, "rocBLAS dgemm failed");
+                } else {
+                    HIP_TRY(ctx, hipMemsetAsync(Pn, 0, sizeof(double) * nn, s));
+                }
+                HIP_TRY(ctx, launch_density_update(n, Pn, false, P[c], partial, scal + 21 + c, s));
+            }
+            HIP_TRY(ctx, hipMemcpyAsync(h_scal + 21 + c, scal + 21 + c, sizeof(double), hipMemcpyDeviceToHost, s));
+            HIP_TRY(ctx, hipStreamSynchronize(s));
+            dsum += std::sqrt(h_scal[21 + c] / (double)std::max<size_t>(1, nn));
+        }
+        phase[4] += ms_since(t_ph);
+        const double de = std::fabs(e_elec - e_old);
+        e_old = e_elec;
+        residual = err;
+        if (fp32_phase) {
+            if (err < fp32_best) {
+                fp32_best = err;
+                fp32_stall = 0;
+            } else {
+                ++fp32_stall;
+            }
+            if (err <= 1.0e-4 || fp32_stall >= 2) fp32_phase = false;
+        }
+        if (iter > 0 && de < md.e_tol && (dsum < md.p_tol || err < 1.0e-7) && !low) {
+            converged = 1;
+            break;
+        }
+    }
+    // ---- final energy, populations, dipole, both channels' orbitals (scf.rs:327-468)
+    if ((st = fock_energy()) != AM1G_OK) return res->status = st;
+    HIP_TRY(ctx, launch_atom_props(sa, Pt, L.props.as<double>(), s));
+    std::vector<double> props(4 * (size_t)nat), heps(n), hepsb(n);
+    HIP_TRY(ctx, hipMemcpyAsync(h_scal, scal, sizeof(double) * 24, hipMemcpyDeviceToHost, s));
+    HIP_TRY(ctx, hipMemcpyAsync(props.data(), L.props.p, sizeof(double) * 4 * nat, hipMemcpyDeviceToHost, s));
+    HIP_TRY(ctx, hipMemcpyAsync(heps.data(), eps[0], sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    HIP_TRY(ctx, hipMemcpyAsync(hepsb.data(), eps[1], sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    HIP_TRY(ctx, hipStreamSynchronize(s));
+    const double electronic = 0.5 * (h_scal[0] + h_scal[1] + h_scal[2]);
+    MolResult r;
+    std::memset(&r, 0, sizeof(r));
+    double dip[3] = {0, 0, 0}, eiso = 0.0, eheat = 0.0;
+    for (int a = 0; a < nat; ++a) {
+        const DevElem& el = tab[rq.z[a]];
+        const double q = el.core - props[4 * a];
+        if (res->charges) res->charges[a] = q;
+        for (int k = 0; k < 3; ++k) dip[k] += rq.xyz_bohr[3 * a + k] * q;
+        if (el.norb == 4)
+            for (int k = 0; k < 3; ++k) dip[k] += -2.0 * el.dd * props[4 * a + 1 + k];
+        eiso += el.e_isol;
+        eheat += el.eheat_ev;
+    }
+    r.status = (status == AM1G_OK && !converged) ? AM1G_SCF_NOT_CONVERGED : status;
+    r.iterations = iterations;
+    r.converged = converged;
+    r.nao = n;
+    r.electronic_ev = electronic;
+    r.core_ev = ecore;
+    r.total_ev = electronic + ecore;
+    r.heat_of_formation_kcal = (electronic + ecore - eiso + eheat) * kEvToKcal;
+    r.homo_ev = na >= 1 ? heps[na - 1] : NAN;
+    r.lumo_ev = na < n ? heps[na] : NAN;
+    r.homo_beta_ev = nb >= 1 ? hepsb[nb - 1] : NAN;
+    r.lumo_beta_ev = nb < n ? hepsb[nb] : NAN;
+    for (int k = 0; k < 3; ++k) r.dipole[k] = dip[k] * kAuDipoleToDebye;
+    r.residual = residual;
+    r.fp32_iterations = fp32_iters;
+    am1g_fill_result(r, md, *res);
+    if (res->mo_energies) std::memcpy(res->mo_energies, heps.data(), sizeof(double) * n);
+    if (res->mo_energies_beta) std::memcpy(res->mo_energies_beta, hepsb.data(), sizeof(double) * n);
+    if (res->density) HIP_TRY(ctx, hipMemcpy(res->density, Pt, sizeof(double) * nn, hipMemcpyDeviceToHost));
+    std::vector<double> tmp;
+    auto columns = [&](const double* ct, double* out) -> int {
+        tmp.resize(nn);
+        HIP_TRY(ctx, hipMemcpy(tmp.data(), ct, sizeof(double) * nn, hipMemcpyDeviceToHost));
+        for (int mu = 0; mu < n; ++mu)
+            for (int k = 0; k < n; ++k) out[(size_t)mu * n + k] = tmp[(size_t)k * n + mu];
+        return AM1G_OK;
+    };
+    if (res->mo_coeff && (st = columns(Ct[0], res->mo_coeff)) != AM1G_OK) return st;
+    if (res->mo_coeff_beta && (st = columns(Ct[1], res->mo_coeff_beta)) != AM1G_OK) return st;
+    if (res->spin_density) {
+        std::vector<double> pb(nn);
+        HIP_TRY(ctx, hipMemcpy(res->spin_density, P[0], sizeof(double) * nn, hipMemcpyDeviceToHost));
+        HIP_TRY(ctx, hipMemcpy(pb.data(), P[1], sizeof(double) * nn, hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < nn; ++k) res->spin_density[k] -= pb[k];
+    }
+    phase[5] += ms_since(t_ph);
+    for (int k = 0; k < 6; ++k) res->phase_ms[k] = phase[k];
+    res->eigen_sweeps = 0;
+    if (status == AM1G_LINEAR_ALGEBRA) am1g_set_error(ctx, status, "%s", eig_err.c_str());
+    else if (status == AM1G_SCF_DIVERGED) am1g_set_error(ctx, status, "SCF diverged after %d iterations", iterations);
+    else if (r.status == AM1G_SCF_NOT_CONVERGED)
+        am1g_set_error(ctx, r.status, "SCF did not converge in %d iterations (residual %.3e)", iterations, residual);
+    return r.status;
+}
+
 // Restricted SCF of one molecule on the whole device (rhf_loop, scf.rs:557-716): grid Fock
 // kernels, library GEMMs for the commutator and the density (FP32 while the commutator norm is
 // above 1e-4 in mixed precision), packed DIIS histories on the device with the small CDIIS and
 // A-DIIS solves on the host, and the large-order eigensolver.
 int am1g_large_scf(am1g_ctx* ctx, const am1g_request& rq, const Prepared& prep, am1g_result* res) {
     const MolDesc& md0 = prep.md;
-    if (prep.uhf)
-        return res->status = am1g_set_error(ctx, AM1G_INVALID_INPUT,
-                                            "unrestricted SCF above 256 atoms or basis functions is not available "
-                                            "on the GPU path");
+    const bool uhf = prep.uhf;
     int st = am1g_large_setup(ctx);
     if (st != AM1G_OK) return res->status = st;
     LargeBuffers& L = *ctx->large;
@@ -185,9 +449,15 @@ int am1g_large_scf(am1g_ctx* ctx, const am1g_request& rq, const Prepared& prep, 
     HIP_TRY(ctx, launch_pairs(ba, s));
     HIP_TRY(ctx, launch_hcore_diag(sa, s));
     HIP_TRY(ctx, launch_sum(npairs, ba.ecore_pair, partial, scal + 30, s));
+    double h_scal[64];
+    if (uhf) {
+        HIP_TRY(ctx, hipMemcpyAsync(h_scal, scal + 30, sizeof(double), hipMemcpyDeviceToHost, s));
+        HIP_TRY(ctx, hipStreamSynchronize(s));
+        phase[0] += ms_since(t_ph);
+        return large_uhf(ctx, rq, md, sa, L, h_scal[0], phase, t_ph, res);
+    }
     HIP_TRY(ctx, hipMemsetAsync(P, 0, sizeof(double) * nn, s));
     HIP_TRY(ctx, launch_sad(sa, 1.0, P, s));
-    double h_scal[64];
     HIP_TRY(ctx, hipMemcpyAsync(h_scal, scal, sizeof(double) * 64, hipMemcpyDeviceToHost, s));
     HIP_TRY(ctx, hipStreamSynchronize(s));
     const double ecore = h_scal[30];

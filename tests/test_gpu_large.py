@@ -67,10 +67,40 @@ def test_water_cluster_100_takes_the_large_path(engine):
     assert np.abs(rm["charges"] - o["charges"]).max() <= 1e-6
 
 
-def test_large_uhf_is_refused(engine, forced):
+def test_methyl_radical_uhf(engine, forced):
+    # scf.rs:1372-1388
     z, x = M.parse(M.METHYL)
     r = engine.scf(z, x, multiplicity=2)
-    assert r["status"] == 1 and "unrestricted" in r["message"]
+    o = O.scf(z, x, multiplicity=2)
+    assert r["unrestricted"]
+    _compare(r, o)
+    assert np.abs(r["mo_energies_beta"] - o["mo_energies_beta"]).max() < 1e-6
+    assert abs(np.trace(r["spin_density"]) - 1.0) < 1e-9
+
+
+def test_forced_uhf_singlet_equals_rhf(engine, forced):
+    # scf.rs:1391-1418
+    z, x = M.parse(M.WATER)
+    u = engine.scf(z, x, reference="uhf")
+    r = engine.scf(z, x)
+    assert u["unrestricted"] and abs(u["total_ev"] - r["total_ev"]) < 1e-6
+    _compare(u, O.scf(z, x, reference="uhf"))
+
+
+def test_water_cluster_radical_uhf_large(engine):
+    """A 300-atom doublet (100 waters less one hydrogen): UHF on the large path, FP64 and mixed."""
+    from am1g import geometry as G
+
+    mol = G.water_cluster((5, 5, 4), seed=4)
+    keep = np.ones(len(mol.z), dtype=bool)
+    keep[2] = False
+    z, x = mol.z[keep], mol.xyz_bohr[keep]
+    o = O.scf(z, x, multiplicity=2, full=False)
+    r = engine.scf(z, x, multiplicity=2, full=False)
+    assert r["unrestricted"]
+    _compare(r, o)
+    rm = engine.scf(z, x, multiplicity=2, full=False, precision="mixed")
+    assert rm["status"] == 0 and abs(rm["total_ev"] - o["total_ev"]) <= 1e-6
 
 
 def test_water512_full_size_against_oracle_fixture(engine):
