@@ -44,6 +44,19 @@ def scf_flops(nao, n_heavy, n_h, nocc):
     return fock + (10.0 / 3.0) * n ** 3 + 2.0 * n ** 3 + 2.0 * n * n * nocc + 16.0 * n * n
 
 
+def pmc_traffic():
+    """HBM bytes per batched launch from the committed FETCH_SIZE / WRITE_SIZE profile of the same
+    workload (counters scaled by a copy-kernel calibration), or None when it is absent."""
+    path = os.path.join(ROOT, "profiles", "k_scf_batch_chains4096_pmc.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def percentile(x, q):
     return float(np.percentile(np.asarray(x, dtype=float), q)) if len(x) else None
 
@@ -169,7 +182,7 @@ def main():
         flops += (r["iterations"] + 1) * scf_flops(m.nao, nh, nH, nocc)  # +1: final Fock/energy pass
     achieved = flops / kernel_s / 1e12
     roofline = {"bound": "mfma", "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / FP64_PEAK_TFLOPS, 6), "traffic": None,
+                "frac": round(achieved / FP64_PEAK_TFLOPS, 6), "traffic": pmc_traffic(),
                 "kernel": "k_scf_batch", "flops_per_launch": flops}
 
     # ---------------------------------------------------------------- plain path on a stratified sample
