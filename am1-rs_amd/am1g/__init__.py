@@ -99,6 +99,7 @@ class ServeOptions(ctypes.Structure):
         ("lanes", ctypes.c_int),
         ("max_batch", ctypes.c_int),
         ("memory_budget_mb", ctypes.c_double),
+        ("max_wait_ms", ctypes.c_double),
     ]
 
 
@@ -424,7 +425,7 @@ class Server:
     completion)."""
 
     def __init__(self, device: int = 0, lanes: int | None = None, max_batch: int | None = None,
-                 memory_budget_mb: float | None = None):
+                 memory_budget_mb: float | None = None, max_wait_ms: float | None = None):
         self._L = load()
         op = ServeOptions()
         self._L.am1g_serve_defaults(ctypes.byref(op))
@@ -434,6 +435,8 @@ class Server:
             op.max_batch = max_batch
         if memory_budget_mb is not None:
             op.memory_budget_mb = memory_budget_mb
+        if max_wait_ms is not None:
+            op.max_wait_ms = max_wait_ms
         self._sv = ctypes.c_void_p()
         st = self._L.am1g_serve_create(int(device), ctypes.byref(op), ctypes.byref(self._sv))
         if st != 0:

@@ -15,6 +15,7 @@
 #include <mutex>
 #include <numeric>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/am1g.h"
@@ -246,6 +247,14 @@ int am1g_element_info(int z, int method, double* out) {
 }
 
 int am1g_scf_batch(am1g_ctx* ctx, int n, const am1g_request* rqs, am1g_result* results) {
+    return am1g_scf_batch_stream(ctx, n, rqs, results, std::function<void(int)>());
+}
+
+}  // extern "C"
+
+int am1g_scf_batch_stream(am1g_ctx* ctx, int n, const am1g_request* rqs, am1g_result* results,
+                          const std::function<void(int)>& on_done) {
+    const bool streaming = (bool)on_done;
     if (!ctx) return am1g_set_error(nullptr, AM1G_INVALID_INPUT, "null context");
     if (n < 0 || (n > 0 && (!rqs || !results))) return am1g_set_error(ctx, AM1G_INVALID_INPUT, "bad batch arguments");
     std::lock_guard<std::mutex> lock(ctx->mu);
@@ -267,6 +276,11 @@ int am1g_scf_batch(am1g_ctx* ctx, int n, const am1g_request* rqs, am1g_result* r
         ok_idx.push_back(i);
     }
     const int m = (int)ok_idx.size();
+    if (streaming)  // refused requests are final at once
+        for (int i = 0, j = 0; i < n; ++i) {
+            if (j < m && ok_idx[j] == i) ++j;
+            else on_done(i);
+        }
     if (m == 0) return AM1G_OK;
     // pool layout
     std::vector<MolDesc> descs(m);
@@ -350,9 +364,10 @@ int am1g_scf_batch(am1g_ctx* ctx, int n, const am1g_request* rqs, am1g_result* r
     HIP_TRY(ctx, hipMemsetAsync(ctx->queue.p, 0, sizeof(int) * 4, st));
 
     BatchArgs a;
+    std::memset(&a, 0, sizeof(a));
     a.mols = ctx->mols.as<MolDesc>();
     a.n_mols = m;
-    a.lds_n = 128;
+    a.lds_n = 192;
     a.z = ctx->z.as<int>();
     a.xyz = ctx->xyz.as<double>();
     a.elems = ctx->elems.as<DevElem>();
@@ -375,10 +390,63 @@ int am1g_scf_batch(am1g_ctx* ctx, int n, const am1g_request* rqs, am1g_result* r
     int precision = 0;
     for (int j = 0; j < m; ++j) precision |= descs[j].precision;
     int grid = std::min(m, ctx->cus);
+    if (streaming) {
+        HIP_TRY(ctx, ctx->h_done.reserve(sizeof(int) * m));
+        HIP_TRY(ctx, ctx->h_results.reserve(sizeof(MolResult) * m));
+        HIP_TRY(ctx, ctx->h_charges.reserve(sizeof(double) * atom_off));
+        HIP_TRY(ctx, ctx->h_mo.reserve(sizeof(double) * ao_off));
+        HIP_TRY(ctx, ctx->h_mob.reserve(sizeof(double) * ao_off));
+        std::memset(ctx->h_done.p, 0, sizeof(int) * m);
+        a.done = ctx->h_done.as<int>();
+        a.results_h = ctx->h_results.as<MolResult>();
+        a.charges_h = ctx->h_charges.as<double>();
+        a.mo_e_h = ctx->h_mo.as<double>();
+        a.mo_eb_h = ctx->h_mob.as<double>();
+    }
     HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
     HIP_TRY(ctx, launch_pairs(a, st));
     HIP_TRY(ctx, launch_scf_batch(a, precision, grid, lds, st));
     HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
+    if (streaming) {
+        // hand out each molecule as its done flag appears, while the launch runs
+        std::vector<char> reported(m, 0);
+        int left = m;
+        const int* done = ctx->h_done.as<int>();
+        const MolResult* hr = ctx->h_results.as<MolResult>();
+        auto publish = [&](int j) {
+            const MolDesc& md = descs[j];
+            am1g_result& r = results[ok_idx[j]];
+            am1g_fill_result(hr[j], md, r);
+            r.latency_ms = (double)(hr[j].t_done - hr[j].t_start) / 1.0e5;
+            for (int k = 0; k < kPhases; ++k) r.phase_ms[k] = (double)hr[j].phase[k] / 1.0e5;
+            r.eigen_sweeps = hr[j].eigen_sweeps;
+            if (r.charges) std::memcpy(r.charges, ctx->h_charges.as<double>() + md.atom_off, sizeof(double) * md.nat);
+            if (r.mo_energies) std::memcpy(r.mo_energies, ctx->h_mo.as<double>() + md.ao_off, sizeof(double) * md.nao);
+            if (r.mo_energies_beta && md.uhf)
+                std::memcpy(r.mo_energies_beta, ctx->h_mob.as<double>() + md.ao_off, sizeof(double) * md.nao);
+            on_done(ok_idx[j]);
+        };
+        for (;;) {
+            const hipError_t q = hipStreamQuery(st);
+            if (q != hipSuccess && q != hipErrorNotReady)
+                return am1g_set_error(ctx, AM1G_DEVICE_ERROR, "HIP error %s in the batched launch", hipGetErrorString(q));
+            for (int j = 0; j < m; ++j)
+                if (!reported[j] && __atomic_load_n(done + j, __ATOMIC_ACQUIRE)) {
+                    reported[j] = 1;
+                    --left;
+                    publish(j);
+                }
+            if (left == 0 || q == hipSuccess) break;
+            std::this_thread::sleep_for(std::chrono::microseconds(50));
+        }
+        HIP_TRY(ctx, hipStreamSynchronize(st));
+        for (int j = 0; j < m; ++j)  // the launch has ended: every flag is set
+            if (!reported[j]) publish(j);
+        float ms = 0.f;
+        HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        ctx->last_ms = ms;
+        return AM1G_OK;
+    }
 
     std::vector<MolResult> res(m);
     std::vector<double> mo(ao_off), mob(ao_off), q(atom_off);
@@ -432,6 +500,8 @@ int am1g_scf_batch(am1g_ctx* ctx, int n, const am1g_request* rqs, am1g_result* r
     }
     return AM1G_OK;
 }
+
+extern "C" {
 
 int am1g_scf(am1g_ctx* ctx, const am1g_request* rq, am1g_result* res) {
     if (!ctx || !rq || !res) return am1g_set_error(ctx, AM1G_INVALID_INPUT, "null argument");

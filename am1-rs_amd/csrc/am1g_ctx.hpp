@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cstdarg>
 #include <cstdio>
+#include <functional>
 #include <mutex>
 #include <string>
 
@@ -14,6 +15,29 @@
 #include "am1g_launch.hpp"
 
 using namespace am1g;  // internal header of the library
+
+// Grow-only pinned host buffer (device-visible), for results streamed out of a running launch.
+struct HostBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    hipError_t reserve(size_t need) {
+        if (need <= bytes) return hipSuccess;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        bytes = 0;
+        size_t want = std::max(need, (size_t)4096);
+        hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
+        if (e == hipSuccess) bytes = want;
+        return e;
+    }
+    template <typename T>
+    T* as() const {
+        return static_cast<T*>(p);
+    }
+    ~HostBuf() {
+        if (p) (void)hipHostFree(p);
+    }
+};
 
 // Grow-only device buffer.
 struct DevBuf {
@@ -49,6 +73,7 @@ struct am1g_ctx {
     DevElem host_tab[kMethods * kMaxZ];
     DevBuf elems, mols, z, xyz, mats, wpool, hist, mo_e, mo_eb, charges, results, queue, order;
     DevBuf alist, aoff, acls, pair_off, ecore_pair;
+    HostBuf h_done, h_results, h_charges, h_mo, h_mob;  // streaming outputs of the batched launch
     struct DcBuffers* dc = nullptr;
     struct LargeBuffers* large = nullptr;  // one large system: dense matrices, DIIS history
     EigWork* eig = nullptr;                // large-order eigensolver workspace
@@ -75,6 +100,11 @@ struct Prepared {
 int am1g_set_error(am1g_ctx* ctx, int code, const char* fmt, ...);
 int am1g_prepare(am1g_ctx* ctx, const am1g_request& rq, Prepared& out, std::string& msg, int max_ao);
 void am1g_dc_release(am1g_ctx* ctx);
+// Batched launch that hands each molecule's result to on_done(request index) as soon as the
+// molecule finishes on the device (charges and orbital energies filled), before the launch ends.
+// Density and orbital matrices are not streamed.
+int am1g_scf_batch_stream(am1g_ctx* ctx, int n, const am1g_request* rqs, am1g_result* results,
+                          const std::function<void(int)>& on_done);
 // Scalars of a device-side MolResult into the C-ABI result.
 void am1g_fill_result(const MolResult& r, const MolDesc& md, am1g_result& out);
 void am1g_large_release(am1g_ctx* ctx);

@@ -35,6 +35,14 @@ struct BatchArgs {
     const int64_t* pair_off;  // [n_mols + 1] first global pair index of each molecule
     int64_t n_pairs;
     double* ecore_pair;  // core-core repulsion of each pair, summed per molecule in fixed order
+    // Streaming (optional, all null otherwise): as each molecule finishes, its result, charges and
+    // orbital energies are copied into host-visible memory and done[m] is set with system scope,
+    // so the host can hand results out while the launch still runs.
+    int* done;
+    MolResult* results_h;
+    double* charges_h;
+    double* mo_e_h;
+    double* mo_eb_h;
 };
 
 hipError_t launch_scf_batch(const BatchArgs& a, int precision, int grid, size_t lds_bytes, hipStream_t stream);
@@ -79,7 +87,7 @@ struct DcArgs {
     double cap;     // 2 restricted, 1 per spin channel
     double tol_abs; // Jacobi tolerance floor
     int* status;    // [n_sub] eigensolver status (sweeps, or -1)
-    double* red;    // reduction scratch for the Fermi search: 8 + 512 * 16 doubles
+    double* red;    // reduction scratch for the Fermi search: 8 + 256 * 31 doubles
 };
 
 hipError_t launch_hcore_diag(const SysArgs& a, hipStream_t st);

@@ -333,7 +333,7 @@ __device__ __noinline__ void mol_fock(const MolDesc& md, const DevElem* el, cons
 // The eigensolver keeps A (packed lower triangle) there and, while it fits, the eigenvector
 // matrix too; the GEMMs use the whole region as panel scratch when the eigensolver is idle.
 constexpr int kLdsDoubles = 19200;
-constexpr int kALds = 128;  // largest order diagonalized with A in LDS
+constexpr int kALds = 192;  // largest order diagonalized with A in LDS (n(n+1)/2 <= kLdsDoubles)
 
 // P = w * C_occ C_occ^T from transposed eigenvectors Ct (row k = MO k): P = w Ct_occ^T Ct_occ.
 template <typename T>
@@ -511,6 +511,11 @@ __device__ __noinline__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolS
     int iterations = 0, converged = 0, status = kOk;
     bool have_c = false;
     int fp32_iters = 0, total_sweeps = 0;
+    // mixed precision: FP32 GEMMs until ||[F,P]|| < 1e-4, or until the FP32-measured error stops
+    // improving for two iterations (its rounding floor); FP64 from then on
+    bool fp32_phase = (PREC == 1) && md.precision == 1;
+    double fp32_best = INFINITY;
+    int fp32_stall = 0;
     for (int iter = 0; iter < md.max_scf; ++iter) {
         iterations = iter + 1;
         mol_fock(md, el, ms, wpool, H, P, P, 0.5, F);
@@ -521,7 +526,7 @@ __device__ __noinline__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolS
         double e_elec = 0.5 * wg_sum(pe, s);
         phase_mark(pc, 1);
         // error = FP - (FP)^T; F and P are symmetric so FP = F^T P
-        bool low = (PREC == 1) && md.precision == 1 && (residual > 1.0e-4);
+        const bool low = fp32_phase;
         if (low) {
             wg_gemm_any<float, true, false>(nao, nao, nao, 1.0, F, nao, P, nao, T0, nao, lds, kLdsDoubles);
             ++fp32_iters;
@@ -702,6 +707,15 @@ __device__ __noinline__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolS
         double de = fabs(e_elec - e_old);
         e_old = e_elec;
         residual = err;
+        if (fp32_phase) {
+            if (err < fp32_best) {
+                fp32_best = err;
+                fp32_stall = 0;
+            } else {
+                ++fp32_stall;
+            }
+            if (err <= 1.0e-4 || fp32_stall >= 2) fp32_phase = false;
+        }
         if (iter > 0 && de < md.e_tol && (dp < md.p_tol || err < 1.0e-7) && !low) {
             converged = 1;
             break;
@@ -805,6 +819,9 @@ __device__ __noinline__ void scf_uhf(const MolDesc& md, const BatchArgs& g, MolS
     double e_old = 0.0, residual = INFINITY, err = INFINITY;
     int iterations = 0, converged = 0, status = kOk, fp32_iters = 0, total_sweeps = 0;
     bool have_c = false;
+    bool fp32_phase = (PREC == 1) && md.precision == 1;
+    double fp32_best = INFINITY;
+    int fp32_stall = 0;
     for (int iter = 0; iter < md.max_scf; ++iter) {
         iterations = iter + 1;
         for (size_t idx = t; idx < nn; idx += kWg) Pt[idx] = Ps[0][idx] + Ps[1][idx];
@@ -817,7 +834,7 @@ __device__ __noinline__ void scf_uhf(const MolDesc& md, const BatchArgs& g, MolS
             pe += Pt[idx] * H[idx] + Ps[0][idx] * Fs[0][idx] + Ps[1][idx] * Fs[1][idx];
         const double e_elec = 0.5 * wg_sum(pe, s);
         phase_mark(pc, 1);
-        const bool low = (PREC == 1) && md.precision == 1 && (residual > 1.0e-4);
+        const bool low = fp32_phase;
         if (low) ++fp32_iters;
         int slot = 0;
         if (diis) slot = ms.hcount < kDiisDepth ? (ms.head + ms.hcount) % kDiisDepth : ms.head;
@@ -947,6 +964,15 @@ __device__ __noinline__ void scf_uhf(const MolDesc& md, const BatchArgs& g, MolS
         const double de = fabs(e_elec - e_old);
         e_old = e_elec;
         residual = err;
+        if (fp32_phase) {
+            if (err < fp32_best) {
+                fp32_best = err;
+                fp32_stall = 0;
+            } else {
+                ++fp32_stall;
+            }
+            if (err <= 1.0e-4 || fp32_stall >= 2) fp32_phase = false;
+        }
         if (iter > 0 && de < md.e_tol && (dp < md.p_tol || err < 1.0e-7) && !low) {
             converged = 1;
             break;
@@ -1032,6 +1058,19 @@ __global__ __launch_bounds__(kWg) void k_scf_batch(BatchArgs g) {
             g.results[mi] = res;
         }
         __syncthreads();
+        if (g.done) {
+            // publish this molecule to the host while the launch goes on
+            for (int a = threadIdx.x; a < md.nat; a += kWg)
+                g.charges_h[md.atom_off + a] = g.charges[md.atom_off + a];
+            for (int k = threadIdx.x; k < md.nao; k += kWg) {
+                g.mo_e_h[md.ao_off + k] = g.mo_e[md.ao_off + k];
+                if (md.uhf) g.mo_eb_h[md.ao_off + k] = g.mo_e_beta[md.ao_off + k];
+            }
+            if (threadIdx.x == 0) g.results_h[mi] = res;
+            __threadfence_system();
+            __syncthreads();
+            if (threadIdx.x == 0) __hip_atomic_store(g.done + mi, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
     }
 }
 

@@ -3,8 +3,10 @@
 // Each lane is one host thread that owns one am1g context (its own HIP stream and device
 // pools). A free lane takes every queued request of the head request's precision that fits the
 // device-memory budget, up to max_batch, and runs them as one persistent batched launch; other
-// lanes keep admitting newly arrived requests meanwhile, so launches overlap on the device.
-// Requests are copied at submission, so callers may free their buffers at once.
+// lanes keep admitting newly arrived requests meanwhile, so launches overlap on the device. Each
+// molecule's result is handed out as soon as that molecule finishes (the launch streams results
+// into pinned host memory), not when the whole launch ends. Requests are copied at submission,
+// so callers may free their buffers at once.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -77,6 +79,14 @@ struct am1g_server {
         for (;;) {
             cv_work.wait(lk, [&] { return stop || !queue.empty(); });
             if (stop) return;
+            if (op.max_wait_ms > 0.0) {  // batching window from the oldest queued request
+                const auto until = jobs.at(queue.front()).submitted +
+                                   std::chrono::duration_cast<Clock::duration>(
+                                       std::chrono::duration<double, std::milli>(op.max_wait_ms));
+                cv_work.wait_until(lk, until, [&] { return stop || (int)queue.size() >= op.max_batch; });
+                if (stop) return;
+                if (queue.empty()) continue;
+            }
             // admit: head request's precision, in arrival order, within the memory budget
             std::vector<long long> take;
             size_t bytes = 0;
@@ -114,19 +124,34 @@ struct am1g_server {
                 res[k].mo_energies_beta = j.mob.data();
             }
             lk.unlock();
-            const int rc = am1g_scf_batch(ctx, (int)take.size(), rqs.data(), res.data());
-            const std::string msg = rc == AM1G_OK ? std::string() : std::string(am1g_last_error(ctx));
-            const auto now = Clock::now();
-            lk.lock();
-            admitted -= bytes;
-            st.running -= (long long)take.size();
-            for (size_t k = 0; k < take.size(); ++k) {
+            // each molecule is handed out as soon as it finishes on the device
+            auto on_done = [&](int k) {
+                const auto now = Clock::now();
+                std::lock_guard<std::mutex> g(mu);
                 Job& j = jobs.at(take[k]);
                 j.res = res[k];
-                if (rc != AM1G_OK) j.res.status = rc;
                 j.state = JobState::kDone;
                 j.done = now;
                 st.completed++;
+                st.running--;
+                cv_done.notify_all();
+            };
+            const int rc = am1g_scf_batch_stream(ctx, (int)take.size(), rqs.data(), res.data(), on_done);
+            const std::string msg = rc == AM1G_OK ? std::string() : std::string(am1g_last_error(ctx));
+            const auto now = Clock::now();
+            lk.lock();
+            admitted -= bytes;  // the lane's device pools are free again
+            for (size_t k = 0; k < take.size(); ++k) {
+                auto it = jobs.find(take[k]);
+                if (it == jobs.end()) continue;  // handed out and already collected by poll
+                Job& j = it->second;
+                if (j.state == JobState::kDone) continue;
+                j.res = res[k];
+                j.res.status = rc != AM1G_OK ? rc : AM1G_DEVICE_ERROR;
+                j.state = JobState::kDone;
+                j.done = now;
+                st.completed++;
+                st.running--;
             }
             if (rc != AM1G_OK) err = msg;
             cv_done.notify_all();
@@ -141,6 +166,7 @@ void am1g_serve_defaults(am1g_serve_options* op) {
     op->lanes = 4;
     op->max_batch = 512;
     op->memory_budget_mb = 32768.0;
+    op->max_wait_ms = 0.0;
 }
 
 int am1g_serve_create(int device, const am1g_serve_options* op, am1g_server** out) {
@@ -149,9 +175,10 @@ int am1g_serve_create(int device, const am1g_serve_options* op, am1g_server** ou
     am1g_serve_options o;
     am1g_serve_defaults(&o);
     if (op) o = *op;
-    if (o.lanes < 1 || o.lanes > 16 || o.max_batch < 1 || !(o.memory_budget_mb > 0.0))
+    if (o.lanes < 1 || o.lanes > 16 || o.max_batch < 1 || !(o.memory_budget_mb > 0.0) || !(o.max_wait_ms >= 0.0))
         return am1g_set_error(nullptr, AM1G_INVALID_INPUT,
-                              "serving needs 1..16 lanes, max_batch >= 1 and a positive memory budget");
+                              "serving needs 1..16 lanes, max_batch >= 1, a positive memory budget and a "
+                              "non-negative batching window");
     am1g_server* sv = new am1g_server();
     sv->op = o;
     for (int l = 0; l < o.lanes; ++l) {
@@ -226,13 +253,21 @@ int am1g_serve_poll(am1g_server* sv, long long ticket, double wait_ms, am1g_resu
         sv->err = "unknown ticket";
         return AM1G_INVALID_INPUT;
     }
-    auto ready = [&] { return sv->jobs.at(ticket).state == JobState::kDone || sv->jobs.at(ticket).cancelled; };
+    auto ready = [&] {
+        auto f = sv->jobs.find(ticket);
+        return f == sv->jobs.end() || f->second.state == JobState::kDone || f->second.cancelled;
+    };
     if (!ready()) {
         if (wait_ms < 0) sv->cv_done.wait(lk, ready);
         else if (wait_ms > 0)
             sv->cv_done.wait_for(lk, std::chrono::duration<double, std::milli>(wait_ms), ready);
     }
-    Job& j = sv->jobs.at(ticket);
+    it = sv->jobs.find(ticket);
+    if (it == sv->jobs.end()) {  // collected meanwhile by another caller
+        sv->err = "unknown ticket";
+        return AM1G_INVALID_INPUT;
+    }
+    Job& j = it->second;
     if (j.cancelled && j.state == JobState::kQueued) {
         sv->jobs.erase(ticket);
         return AM1G_CANCELLED;

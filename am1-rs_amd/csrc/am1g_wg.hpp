@@ -232,7 +232,10 @@ __device__ __forceinline__ void rr_pair(int m, int r, int k, int& p, int& q) {
 // rounding floor). The SCF passes tol_abs = 1e-7 x its current commutator error: an intermediate
 // Fock matrix needs its eigenvectors only to well inside the error the next iteration removes.
 // Returns the number of sweeps, or -1 when max_sweeps did not suffice.
-constexpr int kBlkPer = 5;   // blocks per thread: ceil(64*65/2 / kWg) for n <= 128
+// Rotation blocks per thread: ceil(h(h+1)/2 / kWg) with h = n/2 pairs; 5 covers n <= 128 and 10
+// covers n <= 192 (the larger count costs registers, so it is a separate instantiation).
+constexpr int kBlkSmall = 5;
+constexpr int kBlkLarge = 10;
 
 template <bool PACKED>
 __device__ __forceinline__ size_t jidx(int i, int j, int lda) {
@@ -271,8 +274,9 @@ __device__ __forceinline__ void rotate_rows(double* __restrict__ xp, double* __r
     }
 }
 
-__device__ __noinline__ int wg_jacobi_lds(int n, int v_off, int v_rows, double* __restrict__ Vg, WgScratch& s,
-                                          int max_sweeps, double tol_abs) {
+template <int kBlkPer>
+__device__ __noinline__ int wg_jacobi_lds_t(int n, int v_off, int v_rows, double* __restrict__ Vg, WgScratch& s,
+                                            int max_sweeps, double tol_abs) {
     const int t = threadIdx.x;
     const int m = (n + 1) & ~1;
     const int h = m >> 1;
@@ -381,7 +385,13 @@ __device__ __noinline__ int wg_jacobi_lds(int n, int v_off, int v_rows, double* 
     return -1;
 }
 
-// Jacobi for orders above 128 (global memory, full storage): same algorithm, slots decoded on
+__device__ __forceinline__ int wg_jacobi_lds(int n, int v_off, int v_rows, double* __restrict__ Vg, WgScratch& s,
+                                             int max_sweeps, double tol_abs) {
+    if (n <= 128) return wg_jacobi_lds_t<kBlkSmall>(n, v_off, v_rows, Vg, s, max_sweeps, tol_abs);
+    return wg_jacobi_lds_t<kBlkLarge>(n, v_off, v_rows, Vg, s, max_sweeps, tol_abs);
+}
+
+// Jacobi for orders above 192 (global memory, full storage): same algorithm, slots decoded on
 // the fly because a thread owns more of them than the register budget of the fast path holds.
 __device__ __noinline__ int wg_jacobi_big(int n, double* A, int lda, double* Vt, int ldv, WgScratch& s, int max_sweeps,
                                           double tol_abs) {

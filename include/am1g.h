@@ -156,6 +156,8 @@ typedef struct {
     int lanes;                /* concurrent launches (default 4, at most 16) */
     int max_batch;            /* molecules per launch (default 512) */
     double memory_budget_mb;  /* device memory the running launches may hold (default 32768) */
+    double max_wait_ms;       /* batching window: a free lane waits this long after the first queued
+                                 request for more to arrive before launching (default 0) */
 } am1g_serve_options;
 
 typedef struct {

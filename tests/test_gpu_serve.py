@@ -105,3 +105,23 @@ def test_memory_budget_bounds_admission():
     dense_mb = sum(6 * 8 * (4 * int((m.z != 1).sum()) + int((m.z == 1).sum())) ** 2 for m in mols) / 2**20
     assert stats["launches"] >= int(dense_mb / 2.0)
     assert stats["launches"] > 1
+
+
+def test_results_stream_out_before_the_launch_ends():
+    """One launch holding a 40-water cluster (240 basis functions) and 40 small chains: the
+    chains are handed out as each finishes, long before the cluster does."""
+    rng = np.random.default_rng(12)
+    big = G.small_water_cluster(40, rng)
+    small = G.chain_batch(40, seed=13)
+    sv = am1g.Server(0, lanes=1, max_batch=64, max_wait_ms=500.0)
+    try:
+        tb = sv.submit(big.z, big.xyz_bohr)
+        ts = [sv.submit(m.z, m.xyz_bohr) for m in small]
+        rs = [sv.poll(t, wait_ms=-1) for t in ts]
+        rb = sv.poll(tb, wait_ms=-1)
+        stats = sv.stats()
+    finally:
+        sv.close()
+    assert stats["launches"] == 1
+    assert rb["status"] == 0 and all(r["status"] == 0 for r in rs)
+    assert max(r["latency_ms"] for r in rs) < rb["latency_ms"]
