@@ -506,9 +506,9 @@ extern "C" {
 int am1g_scf(am1g_ctx* ctx, const am1g_request* rq, am1g_result* res) {
     if (!ctx || !rq || !res) return am1g_set_error(ctx, AM1G_INVALID_INPUT, "null argument");
     auto w0 = std::chrono::steady_clock::now();
-    // Molecules beyond one workgroup (more than 256 atoms or basis functions) take the
-    // large-system path; AM1G_FORCE_LARGE_PATH=1 sends every molecule there (for testing it on
-    // sizes the oracle finishes quickly).
+    // Molecules above kRouteLargeAo basis functions (or beyond one workgroup's atom tables) take
+    // the large-system path; AM1G_FORCE_LARGE_PATH=1 sends every molecule there (for testing it
+    // on sizes the oracle finishes quickly).
     Prepared prep;
     std::string msg;
     int st = am1g_prepare(ctx, *rq, prep, msg, INT_MAX);
@@ -519,7 +519,7 @@ int am1g_scf(am1g_ctx* ctx, const am1g_request* rq, am1g_result* res) {
         return am1g_set_error(ctx, st, "%s", msg.c_str());
     }
     const char* force = std::getenv("AM1G_FORCE_LARGE_PATH");
-    if (prep.md.nao > kMaxBatchAo || rq->n_atoms > kMaxAtoms || (force && force[0] == '1')) {
+    if (prep.md.nao > kRouteLargeAo || rq->n_atoms > kMaxAtoms || (force && force[0] == '1')) {
         std::lock_guard<std::mutex> lock(ctx->mu);
         HIP_TRY(ctx, hipSetDevice(ctx->device));
         st = am1g_large_scf(ctx, *rq, prep, res);

@@ -30,6 +30,9 @@ int am1g_large_setup(am1g_ctx* ctx) {
         }
         if (rocblas_set_stream(ctx->blas, ctx->stream) != rocblas_status_success)
             return am1g_set_error(ctx, AM1G_DEVICE_ERROR, "rocBLAS stream binding failed");
+        // no split-K atomics: a request gives the same bits on every call and every lane
+        if (rocblas_set_atomics_mode(ctx->blas, rocblas_atomics_not_allowed) != rocblas_status_success)
+            return am1g_set_error(ctx, AM1G_DEVICE_ERROR, "rocBLAS atomics mode failed");
     }
     if (!ctx->eig) ctx->eig = new EigWork();
     if (!ctx->large) ctx->large = new LargeBuffers();

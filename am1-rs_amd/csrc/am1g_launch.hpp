@@ -9,6 +9,13 @@ namespace am1g {
 constexpr int kMaxAtoms = 256;
 // Largest basis the per-molecule kernel accepts: the Jacobi round holds at most 128 rotations.
 constexpr int kMaxBatchAo = 256;
+// A single molecule with more basis functions than this runs faster on the grid-level kernels
+// (the whole device on one molecule) than in one workgroup. Measured on water clusters, one
+// workgroup against the grid path: 14 / 19 ms at 102 functions, 23 / 23 ms at 114, 36 / 26 ms
+// at 126, 44 / 23 ms at 132, 391 / 38 ms at 240. Inside a full batch the workgroup path still
+// gives more molecules per second (it holds 1/256 of the device), so am1g_scf_batch does not
+// route; am1g_scf and the server do, for latency.
+constexpr int kRouteLargeAo = 128;
 // Dynamic LDS of the per-molecule kernel (the static part is the molecule tables).
 constexpr int kLdsBytes = 19200 * 8;
 

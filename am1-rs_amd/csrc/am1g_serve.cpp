@@ -5,8 +5,11 @@
 // device-memory budget, up to max_batch, and runs them as one persistent batched launch; other
 // lanes keep admitting newly arrived requests meanwhile, so launches overlap on the device. Each
 // molecule's result is handed out as soon as that molecule finishes (the launch streams results
-// into pinned host memory), not when the whole launch ends. Requests are copied at submission,
-// so callers may free their buffers at once.
+// into pinned host memory), not when the whole launch ends. A molecule that am1g_scf would send
+// to the grid-level path (more than kRouteLargeAo basis functions) is never batched: a lane takes
+// it alone, when it reaches the head of the queue, and runs it through am1g_scf, so every
+// request follows the same route, and gives the same result, as it would alone. Requests are
+// copied at submission, so callers may free their buffers at once.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -35,11 +38,14 @@ struct Job {
     Clock::time_point submitted, done;
     am1g_result res;
     std::vector<double> charges, mo, mob;
+    bool large = false;  // runs alone on the grid-level path
 };
 
-// Device bytes one request holds inside a batched launch: dense matrices, pair pool, DIIS
-// history and per-AO / per-atom arrays (the pool layout of am1g_scf_batch).
-size_t job_bytes(const am1g_request& rq) {
+// Device bytes one request holds while it runs: inside a batched launch, its dense matrices,
+// pair pool, DIIS history and per-AO / per-atom arrays (the pool layout of am1g_scf_batch); on
+// the grid-level path, about 13 dense matrices, the two packed DIIS histories of 8 entries each
+// and the eigensolver's workspace (36 nao x nao doubles in all, beside the pair pool).
+size_t job_bytes(const am1g_request& rq, bool large) {
     int64_t nao = 0, nh = 0, nH = 0;
     for (int a = 0; a < rq.n_atoms; ++a) {
         if (rq.z[a] == 1) {
@@ -50,6 +56,9 @@ size_t job_bytes(const am1g_request& rq) {
             ++nh;
         }
     }
+    if (large)
+        return (size_t)(36 * nao * nao + w_pool_size((int)nh, (int)nH) + (int64_t)rq.n_atoms * (rq.n_atoms + 8)) *
+               sizeof(double);
     const bool uhf = rq.reference == AM1G_REF_UHF || rq.multiplicity > 1;
     const int64_t doubles = (int64_t)(uhf ? kMatSlotsUhf : kMatSlotsRhf) * nao * nao + w_pool_size((int)nh, (int)nH) +
                             hist_size((int)nao, uhf) + 2 * nao + (int64_t)rq.n_atoms * (rq.n_atoms + 8);
@@ -72,6 +81,44 @@ struct am1g_server {
     am1g_serve_stats st{};
     std::string err;
 
+    // Runs the queued request `t` alone through am1g_scf on `ctx`; called and returns with `lk` held.
+    void run_large(std::unique_lock<std::mutex>& lk, am1g_ctx* ctx, long long t) {
+        queue.erase(std::remove(queue.begin(), queue.end(), t), queue.end());
+        Job& j = jobs.at(t);
+        j.state = JobState::kRunning;
+        const size_t bytes = j.bytes;
+        admitted += bytes;
+        peak = std::max(peak, admitted);
+        st.launches++;
+        st.queued = (long long)queue.size();
+        st.running++;
+        const am1g_request rq = j.rq;
+        am1g_result res;
+        std::memset(&res, 0, sizeof(res));
+        res.charges = j.charges.data();
+        res.mo_energies = j.mo.data();
+        res.mo_energies_beta = j.mob.data();
+        lk.unlock();
+        const int rc = am1g_scf(ctx, &rq, &res);
+        const std::string msg = rc == AM1G_OK ? std::string() : std::string(am1g_last_error(ctx));
+        const auto now = Clock::now();
+        lk.lock();
+        admitted -= bytes;
+        auto it = jobs.find(t);  // cancelled jobs stay until polled, so the ticket is still here
+        if (it != jobs.end()) {
+            Job& d = it->second;
+            d.res = res;
+            if (rc != AM1G_OK && d.res.status == AM1G_OK) d.res.status = rc;
+            d.state = JobState::kDone;
+            d.done = now;
+        }
+        st.completed++;
+        st.running--;
+        if (rc != AM1G_OK) err = msg;
+        cv_done.notify_all();
+        cv_work.notify_all();
+    }
+
     void lane_loop(int lane) {
         am1g_ctx* ctx = ctxs[lane];
         const size_t budget = (size_t)(op.memory_budget_mb * 1048576.0);
@@ -79,13 +126,24 @@ struct am1g_server {
         for (;;) {
             cv_work.wait(lk, [&] { return stop || !queue.empty(); });
             if (stop) return;
+            // a grid-level request at the head runs alone, with no batching window
+            if (jobs.at(queue.front()).large) {
+                Job& j = jobs.at(queue.front());
+                const bool alone = admitted == 0;
+                if (!alone && admitted + j.bytes > budget) {
+                    cv_work.wait_for(lk, std::chrono::milliseconds(2));
+                    continue;
+                }
+                run_large(lk, ctx, queue.front());
+                continue;
+            }
             if (op.max_wait_ms > 0.0) {  // batching window from the oldest queued request
                 const auto until = jobs.at(queue.front()).submitted +
                                    std::chrono::duration_cast<Clock::duration>(
                                        std::chrono::duration<double, std::milli>(op.max_wait_ms));
                 cv_work.wait_until(lk, until, [&] { return stop || (int)queue.size() >= op.max_batch; });
                 if (stop) return;
-                if (queue.empty()) continue;
+                if (queue.empty() || jobs.at(queue.front()).large) continue;
             }
             // admit: head request's precision, in arrival order, within the memory budget
             std::vector<long long> take;
@@ -93,7 +151,7 @@ struct am1g_server {
             const int prec = jobs.at(queue.front()).rq.precision;
             for (auto it = queue.begin(); it != queue.end() && (int)take.size() < op.max_batch;) {
                 Job& j = jobs.at(*it);
-                if (j.rq.precision != prec) {
+                if (j.rq.precision != prec || j.large) {
                     ++it;
                     continue;
                 }
@@ -221,9 +279,10 @@ int am1g_serve_submit(am1g_server* sv, const am1g_request* rq, long long* ticket
     j.rq = *rq;
     j.rq.z = j.z.data();
     j.rq.xyz_bohr = j.xyz.data();
-    j.bytes = job_bytes(j.rq);
     int nao = 0;
     for (uint8_t zz : j.z) nao += zz == 1 ? 1 : 4;
+    j.large = nao > kRouteLargeAo || rq->n_atoms > kMaxAtoms;
+    j.bytes = job_bytes(j.rq, j.large);
     j.charges.assign(rq->n_atoms, 0.0);
     j.mo.assign(nao, 0.0);
     j.mob.assign(nao, 0.0);

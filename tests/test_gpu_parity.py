@@ -92,9 +92,13 @@ def test_water_cluster_32(engine):
     from am1g import geometry as G
 
     mol = G.water_cluster_config(0)
-    r = engine.scf(mol.z, mol.xyz_bohr, accelerator="cdiis")
     o = O.scf(mol.z, mol.xyz_bohr, accelerator="cdiis")
+    # 192 basis functions: the plain path sends it to the grid-level kernels, the batched
+    # launch diagonalizes it inside one workgroup; both agree with the oracle
+    r = engine.scf(mol.z, mol.xyz_bohr, accelerator="cdiis")
     _compare(r, o)
+    rb = engine.scf_batch([(mol.z, mol.xyz_bohr)], accelerator="cdiis", full=True)[0]
+    _compare(rb, o)
 
 
 def test_batch_matches_oracle_and_is_batch_invariant(engine):

@@ -108,10 +108,10 @@ def test_memory_budget_bounds_admission():
 
 
 def test_results_stream_out_before_the_launch_ends():
-    """One launch holding a 40-water cluster (240 basis functions) and 40 small chains: the
-    chains are handed out as each finishes, long before the cluster does."""
+    """One launch holding a 21-water cluster (126 basis functions, the largest that is batched)
+    and 40 small chains: the chains are handed out as each finishes, before the cluster is."""
     rng = np.random.default_rng(12)
-    big = G.small_water_cluster(40, rng)
+    big = G.small_water_cluster(21, rng)
     small = G.chain_batch(40, seed=13)
     sv = am1g.Server(0, lanes=1, max_batch=64, max_wait_ms=500.0)
     try:
@@ -125,3 +125,37 @@ def test_results_stream_out_before_the_launch_ends():
     assert stats["launches"] == 1
     assert rb["status"] == 0 and all(r["status"] == 0 for r in rs)
     assert max(r["latency_ms"] for r in rs) < rb["latency_ms"]
+
+
+def test_large_requests_run_alone_on_the_grid_path(engine):
+    """Water clusters above 128 basis functions are never batched: each runs alone through the
+    plain entry point's grid-level route, so its result is the one ``Engine.scf`` gives, while
+    the chains submitted around it are batched as usual."""
+    rng = np.random.default_rng(21)
+    bigs = [G.small_water_cluster(k, rng) for k in (24, 30)]
+    small = G.chain_batch(30, seed=22)
+    sv = am1g.Server(0, lanes=2, max_batch=16)
+    try:
+        tb = [sv.submit(bigs[0].z, bigs[0].xyz_bohr)]
+        ts = [sv.submit(m.z, m.xyz_bohr) for m in small[:15]]
+        tb.append(sv.submit(bigs[1].z, bigs[1].xyz_bohr, precision="mixed"))
+        ts += [sv.submit(m.z, m.xyz_bohr) for m in small[15:]]
+        rb = [sv.poll(t, wait_ms=-1) for t in tb]
+        rs = [sv.poll(t, wait_ms=-1) for t in ts]
+        stats = sv.stats()
+    finally:
+        sv.close()
+    assert all(r["status"] == 0 for r in rb + rs)
+    assert stats["completed"] == 32 and stats["launches"] >= 4  # two alone, at least two batches
+    for m, r, kw in zip(bigs, rb, ({}, {"precision": "mixed"})):
+        alone = engine.scf(m.z, m.xyz_bohr, full=False, **kw)
+        assert r["total_ev"] == alone["total_ev"]
+        assert np.array_equal(r["charges"], alone["charges"])
+        assert r["iterations"] == alone["iterations"]
+    o = O.scf(bigs[0].z, bigs[0].xyz_bohr, full=False)
+    assert abs(rb[0]["total_ev"] - o["total_ev"]) <= 1e-6
+    o1 = O.scf(bigs[1].z, bigs[1].xyz_bohr, full=False)
+    assert abs(rb[1]["total_ev"] - o1["total_ev"]) <= 1e-6
+    alone = engine.scf_batch([(m.z, m.xyz_bohr) for m in small])
+    for a, b in zip(rs, alone):
+        assert a["total_ev"] == b["total_ev"]
