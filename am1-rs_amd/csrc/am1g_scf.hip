@@ -17,6 +17,7 @@
 #include "am1g_pair.hpp"
 #include "am1g_launch.hpp"
 #include "am1g_wg.hpp"
+#include "am1g_tri.hpp"
 
 namespace am1g {
 
@@ -345,9 +346,11 @@ __device__ void mol_density(int nao, int nocc, double w, const double* Ct, doubl
     wg_gemm_any<T, true, false>(nao, nao, nocc, w, Ct, nao, Ct, nao, P, nao, lds, kLdsDoubles);
 }
 
-// One spin channel's eigen-step: A = Ct F Ct^T (warm start in the previous orbitals, or F itself
-// on the first pass), Jacobi, sort ascending. Ct is replaced by the sorted eigenvectors
-// (transposed) and eps by the sorted eigenvalues.
+// One spin channel's eigen-step. Orders up to kTriMax: the tridiagonal method (wg_eigh_tri).
+// Larger orders, and the rare matrix whose close eigenvalues that method cannot separate: A =
+// Ct F Ct^T (warm start in the previous orbitals, or F itself on the first pass), Jacobi, sort
+// ascending. Ct is replaced by the sorted eigenvectors (transposed) and eps by the sorted
+// eigenvalues. Returns the Jacobi sweep count (1 for the tridiagonal method), or -1.
 template <typename T>
 __device__ int mol_eigen(int nao, const double* Fuse, double* Ct, double* T0, double* T1, double* lds, int lds_n,
                          bool warm, double* eps, WgScratch& s, double tol_abs) {
@@ -359,6 +362,10 @@ __device__ int mol_eigen(int nao, const double* Fuse, double* Ct, double* T0, do
     // eigenvector rows [0, v_rows) beside A in LDS, the rest in place in Ct
     const int v_rows = a_lds ? (int)min((size_t)n, ((size_t)kLdsDoubles - tri) / (size_t)n) : 0;
     const bool v_lds = a_lds && v_rows == n;
+    if (n <= kTriMax && n <= lds_n && tri_lds_doubles(n) + n <= kLdsDoubles) {
+        // direct method (tridiagonalization); it declines only for unresolved close eigenvalues
+        if (wg_eigh_tri(n, Fuse, Ct, eps, T1, lds, kLdsDoubles, s) == 0) return 1;
+    }
     const double* src = Fuse;
     if (warm) {
         // T0 = Ct F ; T1 = T0 Ct^T = the Fock matrix in the previous orbital basis

@@ -199,19 +199,44 @@ __device__ __forceinline__ void wg_gemm_any(int M, int N, int K, double alpha, c
         wg_gemm<T, TA, TB>(M, N, K, alpha, A, lda, B, ldb, C, ldc);
 }
 
-// Round-robin partner schedule for m (even) players: round r, slot k -> (p, q), p < q.
+// Round-robin partner schedule for m (even) players, L = m - 1 rounds per sweep: round r in
+// [0, L), slot k in [0, m/2) -> (p, q), p < q. Slot 0 pairs player 0 with r + 1; slot k > 0
+// pairs players x + 1 and y + 1 with x = r + k and y = r - k (mod L). Both operands of each
+// wrap are below 2L, so one conditional subtraction replaces the modulo.
+__device__ __forceinline__ int rr_wrap(int x, int L) { return x >= L ? x - L : x; }
+
 __device__ __forceinline__ void rr_pair(int m, int r, int k, int& p, int& q) {
     const int L = m - 1;
     int a, b;
     if (k == 0) {
         a = 0;
-        b = (r % L) + 1;
+        b = r + 1;
     } else {
-        a = ((r + k) % L) + 1;
-        b = ((r - k + L) % L) + 1;
+        a = rr_wrap(r + k, L) + 1;
+        b = rr_wrap(r + L - k, L) + 1;
     }
     p = min(a, b);
     q = max(a, b);
+}
+
+// 1/x and 1/sqrt(x) for positive normal x: the hardware estimates refined by two Newton steps
+// each (y <- y + y(1 - xy); y <- y + y(1 - xy^2)/2), which leaves them within a few units in the
+// last place, ample for rotation parameters (their orthogonality is restored by c = 1/sqrt(1+t^2)).
+__device__ __forceinline__ double fast_rcp(double x) {
+    double y = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-x, y, 1.0);
+    return fma(y, e, y);
+}
+
+__device__ __forceinline__ double fast_rsqrt(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    double h = 0.5 * x;
+    double e = fma(-h * y, y, 0.5);
+    y = fma(y, e, y);
+    e = fma(-h * y, y, 0.5);
+    return fma(y, e, y);
 }
 
 // Parallel cyclic Jacobi on a symmetric n x n matrix, rotating the rows of Vt (Vt = V^T, row k =
@@ -243,14 +268,23 @@ __device__ __forceinline__ size_t jidx(int i, int j, int lda) {
     return (size_t)i * lda + j;
 }
 
-// One Jacobi rotation angle for the 2x2 block (app, apq; apq, aqq): (cos, sin), or (1, 0)
-// when |apq| <= tau.
+// One Jacobi rotation for the 2x2 block (app, apq; apq, aqq): (cos, sin), or (1, 0) when
+// |apq| <= tau. With d = aqq - app, the tangent of the smaller rotation is
+// t = sign(d) 2 apq / (|d| + sqrt(d^2 + 4 apq^2)) (sign(0) = sign(apq), so t = +1 at d = 0), the
+// same number as sign(theta) / (|theta| + sqrt(theta^2 + 1)) with theta = d / (2 apq), without
+// the division by apq; c = 1 / sqrt(1 + t^2), s = t c.
 __device__ __forceinline__ double2 jacobi_angle(double app, double apq, double aqq, double tau) {
     double2 r = make_double2(1.0, 0.0);
     if (fabs(apq) > tau) {
-        const double theta = (aqq - app) / (2.0 * apq);
-        const double tt = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-        r.x = 1.0 / sqrt(tt * tt + 1.0);
+        const double d = aqq - app;
+        const double b = 2.0 * apq;
+        const double w = fma(d, d, b * b);
+        // the estimates need a normal, finite argument; outside 1e-280..1e280 use the library
+        const double root = (w > 1.0e-280 && w < 1.0e280) ? w * fast_rsqrt(w) : sqrt(w);
+        const double num = d > 0.0 ? b : (d < 0.0 ? -b : fabs(b));
+        const double den = fabs(d) + root;
+        const double tt = (den > 1.0e-280 && den < 1.0e280) ? num * fast_rcp(den) : num / den;
+        r.x = fast_rsqrt(fma(tt, tt, 1.0));  // 1 + t^2 lies in [1, 2]
         r.y = tt * r.x;
     }
     return r;

@@ -109,13 +109,14 @@ def test_memory_budget_bounds_admission():
 
 def test_results_stream_out_before_the_launch_ends():
     """One launch holding a 21-water cluster (126 basis functions, the largest that is batched)
-    and 40 small chains: the chains are handed out as each finishes, before the cluster is."""
+    held for exactly 200 iterations (zero tolerances, so it ends unconverged) and 40 small
+    chains: the chains are handed out as each finishes, long before the cluster is."""
     rng = np.random.default_rng(12)
     big = G.small_water_cluster(21, rng)
     small = G.chain_batch(40, seed=13)
     sv = am1g.Server(0, lanes=1, max_batch=64, max_wait_ms=500.0)
     try:
-        tb = sv.submit(big.z, big.xyz_bohr)
+        tb = sv.submit(big.z, big.xyz_bohr, e_tol=0.0, p_tol=0.0, max_scf=200)
         ts = [sv.submit(m.z, m.xyz_bohr) for m in small]
         rs = [sv.poll(t, wait_ms=-1) for t in ts]
         rb = sv.poll(tb, wait_ms=-1)
@@ -123,7 +124,8 @@ def test_results_stream_out_before_the_launch_ends():
     finally:
         sv.close()
     assert stats["launches"] == 1
-    assert rb["status"] == 0 and all(r["status"] == 0 for r in rs)
+    assert rb["status"] == 4 and rb["iterations"] == 200  # ScfNotConverged
+    assert all(r["status"] == 0 for r in rs)
     assert max(r["latency_ms"] for r in rs) < rb["latency_ms"]
 
 
