@@ -1,17 +1,18 @@
 // Workgroup symmetric eigensolver for orders up to kTriMax, one 512-thread workgroup per matrix:
-//   1. Householder tridiagonalization in LDS. Step k builds reflector k from column k (every
-//      wavefront computes it, so no barrier is needed to share it) and then, in one pass over
-//      the trailing rows, applies the previous step's rank-2 update and forms the product for
-//      the new reflector: one barrier and one read-modify-write of the trailing block per step.
-//      The reflectors go to global scratch for step 5.
+//   1. Householder tridiagonalization with the matrix in registers: four threads own a row
+//      (32 columns each, in pairs), so only vectors go through LDS. Step k builds reflector k
+//      from row k, which its owners published in the previous step (every wavefront computes
+//      the reflector, so no barrier is needed to share it), and then each owner applies the
+//      previous step's rank-2 update to its row and forms its entry of the product for the new
+//      reflector: one barrier per step. The reflectors go to global scratch for step 5.
 //   2. The tridiagonal T is split where an off-diagonal entry is below 1e-10 ||T||.
 //   3. Eigenvalues of each unreduced block by multisection on Sturm counts (signs of the
 //      leading principal minors), a group of threads per eigenvalue, two points per thread.
 //   4. Eigenvectors of T from the twisted factorization at each eigenvalue (one thread per
-//      eigenvector, O(n)); runs of eigenvalues closer than 1e-6 ||T|| inside one block take
-//      distinct twist indices and are re-orthogonalized by Gram-Schmidt.
-//   5. Sort, and back transformation x = H_0 ... H_{n-3} z row by row, the reflectors staged
-//      through LDS.
+//      eigenvector, O(n), in LDS); runs of eigenvalues closer than 1e-6 ||T|| inside one block
+//      take distinct twist indices and are re-orthogonalized by Gram-Schmidt.
+//   5. Sort, and back transformation x = H_0 ... H_{n-3} z with each row in registers and all
+//      reflectors in LDS (no barriers between reflectors).
 // It replaces the Jacobi rounds of the reference eigensolver (linalg.rs) for the orders that fit
 // in LDS: a Jacobi sweep needs n-1 barrier-separated rounds over the whole matrix; this needs
 // about n barriers in all. The per-thread sequential loops load their operands four at a time,
@@ -27,47 +28,18 @@ constexpr int kTriMax = 128;
 
 __device__ __forceinline__ int tri_ld(int n) { return n | 1; }  // odd row stride: rows spread over LDS banks
 
-// LDS layout for order n (in doubles): the matrix (n x ld, later the eigenvectors of T), eleven
-// n-vectors (d, e, tau, lam, two reflectors, two products, block starts and ends as ints), then
-// the (d_i, e_{i-1}^2) pairs at an even offset.
-__host__ __device__ __forceinline__ int tri_de_offset(int n) { return (n * (n | 1) + 10 * n + 1) & ~1; }
+// LDS layout for order n (in doubles): an n x ld area (the eigenvectors of T, then the
+// reflectors), thirteen n-vectors (d, e, tau, lam, two reflectors, two products, two published
+// rows, block starts and ends as ints; the vector pairs start at even offsets), then the
+// (d_i, e_{i-1}^2) pairs at an even offset.
+__host__ __device__ __forceinline__ int tri_vec_offset(int n) { return (n * (n | 1) + 4 * n + 1) & ~1; }
+__host__ __device__ __forceinline__ int tri_vec_stride(int n) { return (n + 1) & ~1; }
+__host__ __device__ __forceinline__ int tri_de_offset(int n) {
+    return (tri_vec_offset(n) + 6 * tri_vec_stride(n) + n + 1) & ~1;
+}
 __host__ __device__ __forceinline__ int tri_lds_doubles(int n) { return tri_de_offset(n) + 2 * n; }
 
 __device__ __forceinline__ double tri_guard(double q, double pivmin) { return fabs(q) < pivmin ? -pivmin : q; }
-
-// ---------------------------------------------------------------------------------------------
-// Step 1 helper: row r of the trailing block, columns c, c+4, ... < n. Applies the previous
-// reflector's update a = A - v_r w_c - w_r v_c (w = p - K v) and returns sum_c a * vn_c.
-__device__ __forceinline__ double tri_fused_row(double* __restrict__ row, const double* __restrict__ vo,
-                                                const double* __restrict__ po, const double* __restrict__ vn,
-                                                double K, double vr, double wr, int c, int n) {
-    double s0 = 0.0, s1 = 0.0;
-    for (; c + 12 < n; c += 16) {
-        double x[4], v[4], p[4], u[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            x[q] = row[c + 4 * q];
-            v[q] = vo[c + 4 * q];
-            p[q] = po[c + 4 * q];
-            u[q] = vn[c + 4 * q];
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const double w = fma(-K, v[q], p[q]);
-            const double a = fma(-vr, w, fma(-wr, v[q], x[q]));
-            row[c + 4 * q] = a;
-            if (q & 1) s1 = fma(a, u[q], s1);
-            else s0 = fma(a, u[q], s0);
-        }
-    }
-    for (; c < n; c += 4) {
-        const double v = vo[c], w = fma(-K, v, po[c]);
-        const double a = fma(-vr, w, fma(-wr, v, row[c]));
-        row[c] = a;
-        s0 = fma(a, vn[c], s0);
-    }
-    return s0 + s1;
-}
 
 // ---------------------------------------------------------------------------------------------
 // Step 3 helpers: sign counts of the leading principal minors p_i = (d_i - x) p_{i-1} -
@@ -256,70 +228,47 @@ __device__ __forceinline__ bool tri_vector(double* __restrict__ z, const double2
 }
 
 // ---------------------------------------------------------------------------------------------
-// Step 5 helpers: z . v over c, c+4, ... < n, and z -= f v over the same columns.
-__device__ __forceinline__ double tri_dot4(const double* __restrict__ z, const double* __restrict__ v, int c, int n) {
-    double s0 = 0.0, s1 = 0.0;
-    for (; c + 12 < n; c += 16) {
-        double a[4], b[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            a[q] = z[c + 4 * q];
-            b[q] = v[c + 4 * q];
-        }
-        s0 = fma(a[0], b[0], s0);
-        s1 = fma(a[1], b[1], s1);
-        s0 = fma(a[2], b[2], s0);
-        s1 = fma(a[3], b[3], s1);
-    }
-    for (; c < n; c += 4) s0 = fma(z[c], v[c], s0);
-    return s0 + s1;
-}
-
-__device__ __forceinline__ void tri_axpy4(double* __restrict__ z, const double* __restrict__ v, double f, int c,
-                                          int n) {
-    for (; c + 12 < n; c += 16) {
-        double a[4], b[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            a[q] = z[c + 4 * q];
-            b[q] = v[c + 4 * q];
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) z[c + 4 * q] = fma(-f, b[q], a[q]);
-    }
-    for (; c < n; c += 4) z[c] = fma(-f, v[c], z[c]);
-}
-
-// ---------------------------------------------------------------------------------------------
 // Eigen-decomposition of the symmetric n x n matrix F (global, row-major; symmetrized on load).
 // On success writes the eigenvalues ascending to eps and the eigenvectors as the rows of Ct (row
 // k belongs to eps[k]) and returns 0. Returns -1, with Ct and eps untouched, when a run of close
 // eigenvalues could not be separated or a non-finite number appeared; the caller then uses
-// another method. Vg: n*n doubles of global scratch (the reflectors). lds: lds_cap doubles, at
-// least tri_lds_doubles(n) + n.
+// another method. Vg: n*n doubles of global scratch (the reflectors). lds: tri_lds_doubles(n)
+// doubles (lds_cap is checked by the caller).
 __device__ __noinline__ int wg_eigh_tri(int n, const double* __restrict__ F, double* __restrict__ Ct,
                                         double* __restrict__ eps_out, double* __restrict__ Vg,
                                         double* __restrict__ lds, int lds_cap, WgScratch& s) {
     const int t = threadIdx.x, lane = t & 63;
-    const int ld = tri_ld(n);
-    double* A = lds;
+    const int ld = tri_ld(n), sv = tri_vec_stride(n);
+    double* A = lds;               // eigenvectors of T (step 4), reflectors (step 5)
     double* d = A + n * ld;
     double* e = d + n;
     double* tau = e + n;
     double* lam = tau + n;
-    double* vb = lam + n;          // two reflectors (by parity), indexed by absolute row
-    double* pb = vb + 2 * n;       // their products p = tau A v
-    int* bs = (int*)(pb + 2 * n);  // block start of each row
-    int* be = bs + n;              // block end (exclusive)
-    int* rk = (int*)vb;            // rank of each eigenvalue (vb is free after step 1)
+    double* vb = lds + tri_vec_offset(n);  // two reflectors (by parity), indexed by absolute row
+    double* pb = vb + 2 * sv;              // their products p = tau A v
+    double* pub = pb + 2 * sv;             // row k of the matrix, published for step k (by parity)
+    int* bs = (int*)(pub + 2 * sv);        // block start of each row
+    int* be = bs + n;                      // block end (exclusive)
+    int* rk = (int*)vb;                    // rank of each eigenvalue (vb is free after step 1)
     double2* de = (double2*)(lds + tri_de_offset(n));  // (d_i, e_{i-1}^2)
-    const int rg = t >> 2, rl = t & 3;  // row-parallel passes: four threads per row
-
-    for (int idx = t; idx < n * n; idx += kWg) {
-        const int r = idx / n, c = idx - r * n;
-        A[r * ld + c] = 0.5 * (F[idx] + F[c * n + r]);
+    const int rg = t >> 2, rl = t & 3;  // row rg is owned by four threads; rl owns columns
+                                        // 2 rl + 8 i + {0, 1}, i = 0..15
+    double zr[32];                      // the owned part of row rg
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int c = 2 * rl + 8 * i;
+        zr[2 * i] = (rg < n && c < n) ? 0.5 * (F[(size_t)rg * n + c] + F[(size_t)c * n + rg]) : 0.0;
+        zr[2 * i + 1] = (rg < n && c + 1 < n) ? 0.5 * (F[(size_t)rg * n + c + 1] + F[(size_t)(c + 1) * n + rg]) : 0.0;
     }
-    for (int idx = t; idx < 4 * n; idx += kWg) vb[idx] = 0.0;  // both parities of v and p
+    for (int idx = t; idx < 4 * sv; idx += kWg) vb[idx] = 0.0;  // both parities of v and p
+    if (rg == 0) {  // row 0 is published for step 0
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int c = 2 * rl + 8 * i;
+            if (c < n) pub[c] = zr[2 * i];
+            if (c + 1 < n) pub[c + 1] = zr[2 * i + 1];
+        }
+    }
     if (t == 0) {
         s.bcast[2] = 0.0;  // failure flag
         s.bcast[3] = 0.0;  // close eigenvalues present
@@ -330,80 +279,131 @@ __device__ __noinline__ int wg_eigh_tri(int n, const double* __restrict__ F, dou
     int par = 0;        // parity of the reflector built in this step; the previous one is par ^ 1
     double tauo = 0.0;  // tau of the previous reflector (0 before the first)
     for (int k = 0; k + 2 < n; ++k) {
-        const double* vo = vb + (par ^ 1) * n;
-        const double* po = pb + (par ^ 1) * n;
-        double* vn = vb + par * n;
-        double* pn = pb + par * n;
+        const double* vo = vb + (par ^ 1) * sv;
+        const double* po = pb + (par ^ 1) * sv;
+        double* vn = vb + par * sv;
+        double* pn = pb + par * sv;
+        const double* rowk = pub + (k & 1) * sv;
         // K = (tau/2) p.v of the previous reflector (its support starts at row k)
         double pv = 0.0;
         for (int i = k + lane; i < n; i += 64) pv = fma(po[i], vo[i], pv);
-        for (int off = 32; off > 0; off >>= 1) pv += __shfl_xor(pv, off, 64);
-        const double Ko = 0.5 * tauo * pv;
-        // column k after the previous update: x_r = A[r][k] - v_r w_k - w_r v_k (rows k + lane,
-        // k + 64 + lane; lane 0 holds the diagonal, lane 1 the reflector's leading entry)
+        const double Ko = 0.5 * tauo * wave_sum(pv);
+        // column k (= row k) after the previous update: x_r = A[k][r] - v_r w_k - w_r v_k for
+        // r >= k, rows lane and 64 + lane
         const double vk = vo[k], wk = fma(-Ko, vk, po[k]);
-        const int ra = k + lane, rb = k + 64 + lane;
+        const int ra = lane, rb = 64 + lane;
         double xa = 0.0, xb = 0.0;
-        if (ra < n) {
+        if (ra >= k && ra < n) {
             const double v = vo[ra];
-            xa = fma(-v, wk, fma(-fma(-Ko, v, po[ra]), vk, A[ra * ld + k]));
+            xa = fma(-v, wk, fma(-fma(-Ko, v, po[ra]), vk, rowk[ra]));
         }
-        if (rb < n) {
+        if (rb >= k && rb < n) {
             const double v = vo[rb];
-            xb = fma(-v, wk, fma(-fma(-Ko, v, po[rb]), vk, A[rb * ld + k]));
+            xb = fma(-v, wk, fma(-fma(-Ko, v, po[rb]), vk, rowk[rb]));
         }
-        const double dk = __shfl(xa, 0, 64), x0 = __shfl(xa, 1, 64);
+        // the diagonal entry and the reflector's leading entry, by the same formula
+        const double dk = fma(-vk, wk, fma(-fma(-Ko, vk, po[k]), vk, rowk[k]));
+        const double v1 = vo[k + 1];
+        const double x0 = fma(-v1, wk, fma(-fma(-Ko, v1, po[k + 1]), vk, rowk[k + 1]));
         double sig = ra >= k + 2 ? xa * xa : 0.0;
-        sig = fma(xb, xb, sig);
-        for (int off = 32; off > 0; off >>= 1) sig += __shfl_xor(sig, off, 64);
+        if (rb >= k + 2) sig = fma(xb, xb, sig);
+        sig = wave_sum(sig);
         double tk = 0.0, beta = x0, scale = 0.0;
         if (sig > 0.0) {
             beta = -copysign(sqrt(fma(x0, x0, sig)), x0);
             tk = (beta - x0) / beta;
             scale = 1.0 / (x0 - beta);
         }
-        const double va = ra == k + 1 ? 1.0 : (ra >= k + 2 ? xa * scale : 0.0), vbv = xb * scale;
-        // every wavefront writes the same reflector, so each reads back what it wrote
-        if (ra < n) vn[ra] = va;
-        if (rb < n) vn[rb] = vbv;
+        const double va = ra == k + 1 ? 1.0 : (ra >= k + 2 ? xa * scale : 0.0);
+        const double vbv = rb == k + 1 ? 1.0 : (rb >= k + 2 ? xb * scale : 0.0);
+        // every wavefront writes the same reflector (zero above its support), so each reads back
+        // what it wrote; rows <= k of the new product are cleared the same way
+        if (ra < n) {
+            vn[ra] = va;
+            if (ra <= k) pn[ra] = 0.0;
+        }
+        if (rb < n) {
+            vn[rb] = vbv;
+            if (rb <= k) pn[rb] = 0.0;
+        }
         if (t < 64) {
             if (ra > k && ra < n) Vg[(size_t)k * n + ra] = va;
-            if (rb < n) Vg[(size_t)k * n + rb] = vbv;
+            if (rb > k && rb < n) Vg[(size_t)k * n + rb] = vbv;
             if (lane == 0) {
                 d[k] = dk;
                 e[k] = beta;
                 tau[k] = tk;
             }
         }
-        // rows k+1.. of the trailing block: previous update, and p = tau A v for this reflector
-        const int r = k + 1 + rg;
-        if (r < n) {
-            const double vr = vo[r], wr = fma(-Ko, vr, po[r]);
-            double acc = tri_fused_row(A + r * ld, vo, po, vn, Ko, vr, wr, k + 1 + rl, n);
+        // owned rows k+1..: previous update (zero on the columns it does not touch, since the
+        // previous v and p vanish there) and this reflector's product p_r = tau sum_c a_rc v_c
+        if (rg > k && rg < n) {
+            const double vr = vo[rg], wr = fma(-Ko, vr, po[rg]);
+            double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                const int c = 2 * rl + 8 * i;  // even, so the pairs are 16-byte aligned
+                if (c >= n) continue;          // (c + 1 <= n lies inside the vector stride)
+                const double2 v2 = *(const double2*)(vo + c);
+                const double2 p2 = *(const double2*)(po + c);
+                const double2 u2 = *(const double2*)(vn + c);
+                const double a0 = fma(-vr, fma(-Ko, v2.x, p2.x), fma(-wr, v2.x, zr[2 * i]));
+                const double a1 = fma(-vr, fma(-Ko, v2.y, p2.y), fma(-wr, v2.y, zr[2 * i + 1]));
+                zr[2 * i] = a0;
+                zr[2 * i + 1] = a1;
+                s0 = fma(a0, u2.x, s0);
+                s1 = fma(a1, u2.y, s1);
+            }
+            double acc = s0 + s1;
             acc += __shfl_xor(acc, 1, 64);
             acc += __shfl_xor(acc, 2, 64);
-            if (rl == 0) pn[r] = tk * acc;
+            if (rl == 0) pn[rg] = tk * acc;
+            if (rg == k + 1) {  // publish row k+1 for the next step
+                double* nxt = pub + ((k + 1) & 1) * sv;
+#pragma unroll
+                for (int i = 0; i < 16; ++i) {
+                    const int c = 2 * rl + 8 * i;
+                    if (c < n) nxt[c] = zr[2 * i];
+                    if (c + 1 < n) nxt[c + 1] = zr[2 * i + 1];
+                }
+            }
         }
         tauo = tk;
         par ^= 1;
         __syncthreads();
     }
-    if (t == 0) {  // the last 2 x 2 block, after the last reflector's update
-        const double* vo = vb + (par ^ 1) * n;
-        const double* po = pb + (par ^ 1) * n;
+    // the last 2 x 2 block (rows and columns n-2, n-1) after the last reflector's update
+    {
+        const double* vo = vb + (par ^ 1) * sv;
+        const double* po = pb + (par ^ 1) * sv;
         const int k0 = n >= 2 ? n - 2 : 0;
         double pv = 0.0;
-        for (int i = k0; i < n; ++i) pv = fma(po[i], vo[i], pv);
-        const double Ko = 0.5 * tauo * pv;
-        auto upd = [&](int r, int c) {
-            const double wr = fma(-Ko, vo[r], po[r]), wc = fma(-Ko, vo[c], po[c]);
-            return fma(-vo[r], wc, fma(-wr, vo[c], A[r * ld + c]));
-        };
-        if (n >= 2) {
-            d[n - 2] = upd(n - 2, n - 2);
-            e[n - 2] = upd(n - 1, n - 2);
+        for (int i = k0 + lane; i < n; i += 64) pv = fma(po[i], vo[i], pv);
+        const double Ko = 0.5 * tauo * wave_sum(pv);
+        if (rg >= k0 && rg < n) {
+            const double vr = vo[rg], wr = fma(-Ko, vr, po[rg]);
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+#pragma unroll
+                for (int b = 0; b < 2; ++b) {
+                    const int c = 2 * rl + 8 * i + b;
+                    if (c >= k0 && c < n) {
+                        const double a = fma(-vr, fma(-Ko, vo[c], po[c]), fma(-wr, vo[c], zr[2 * i + b]));
+                        s.bcast[8 + 2 * (rg - k0) + (c - k0)] = a;
+                    }
+                }
+            }
         }
-        d[n - 1] = upd(n - 1, n - 1);
+    }
+    __syncthreads();
+    if (t == 0) {
+        if (n >= 2) {
+            d[n - 2] = s.bcast[8];
+            e[n - 2] = s.bcast[10];  // row n-1, column n-2
+            d[n - 1] = s.bcast[11];
+        } else {
+            d[0] = s.bcast[8];
+        }
         e[n - 1] = 0.0;
     }
     __syncthreads();
@@ -576,35 +576,51 @@ __device__ __noinline__ int wg_eigh_tri(int n, const double* __restrict__ F, dou
         }
     }
     __syncthreads();
-    // back transformation x^T = z^T H_{n-3} ... H_0, four threads per row; the reflectors pass
-    // through the LDS left over beside the matrix, cb at a time
-    double* vst = lds + tri_lds_doubles(n);
-    const int cb = (lds_cap - tri_lds_doubles(n)) / n;
-    for (int khi = n - 3; khi >= 0; khi -= cb) {
-        const int klo = max(0, khi - cb + 1);
-        for (int idx = t; idx < (khi - klo + 1) * n; idx += kWg) {
-            const int q = idx / n, c = idx - q * n, k = klo + q;
-            vst[idx] = c > k ? Vg[(size_t)k * n + c] : 0.0;
-        }
-        __syncthreads();
-        if (rg < n) {
-            double* z = A + rg * ld;
-            for (int k = khi; k >= klo; --k) {
-                const double tk = tau[k];
-                if (tk == 0.0) continue;
-                const double* v = vst + (k - klo) * n;
-                double dot = tri_dot4(z, v, k + 1 + rl, n);
-                dot += __shfl_xor(dot, 1, 64);
-                dot += __shfl_xor(dot, 2, 64);
-                tri_axpy4(z, v, tk * dot, k + 1 + rl, n);
+    // back transformation x^T = z^T H_{n-3} ... H_0: row rg of the eigenvectors of T into the
+    // owners' registers, then all reflectors into the freed LDS area (stride sv)
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int c = 2 * rl + 8 * i;
+        zr[2 * i] = (rg < n && c < n) ? A[rg * ld + c] : 0.0;
+        zr[2 * i + 1] = (rg < n && c + 1 < n) ? A[rg * ld + c + 1] : 0.0;
+    }
+    __syncthreads();
+    for (int idx = t; idx < (n - 2) * sv; idx += kWg) {
+        const int k = idx / sv, c = idx - k * sv;
+        A[idx] = (c > k && c < n) ? Vg[(size_t)k * n + c] : 0.0;
+    }
+    __syncthreads();
+    if (rg < n) {
+        for (int k = n - 3; k >= 0; --k) {
+            const double tk = tau[k];
+            if (tk == 0.0) continue;
+            const double* v = A + k * sv;
+            double2 vv[16];
+            double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                const int c = 2 * rl + 8 * i;
+                vv[i] = c < n ? *(const double2*)(v + c) : make_double2(0.0, 0.0);
+                s0 = fma(zr[2 * i], vv[i].x, s0);
+                s1 = fma(zr[2 * i + 1], vv[i].y, s1);
+            }
+            double dot = s0 + s1;
+            dot += __shfl_xor(dot, 1, 64);
+            dot += __shfl_xor(dot, 2, 64);
+            const double f = tk * dot;
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                zr[2 * i] = fma(-f, vv[i].x, zr[2 * i]);
+                zr[2 * i + 1] = fma(-f, vv[i].y, zr[2 * i + 1]);
             }
         }
-        __syncthreads();
-    }
-    if (rg < n) {
-        const double* z = A + rg * ld;
         double* dst = Ct + (size_t)rk[rg] * n;
-        for (int c = rl; c < n; c += 4) dst[c] = z[c];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int c = 2 * rl + 8 * i;
+            if (c < n) dst[c] = zr[2 * i];
+            if (c + 1 < n) dst[c + 1] = zr[2 * i + 1];
+        }
     }
     __syncthreads();
     return 0;

@@ -19,6 +19,28 @@ struct WgScratch {
     int2 pq[128];         // and their index pairs (q = -1: padding slot of an odd order)
 };
 
+// Data-parallel-primitive moves of a double within rows of 16 lanes (no LDS traffic); CTRL is
+// the DPP control word (quad permutations 0xB1 = [1,0,3,2] and 0x4E = [2,3,0,1], row mirrors
+// 0x141 = lanes 0-7 reversed in each half row and 0x140 = each row of 16 reversed).
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+// Sum over the 64 lanes of a wavefront, the same bits in every lane (each step adds a lane's
+// value to its partner's, and a + b == b + a): four row steps on DPP, two across rows.
+__device__ __forceinline__ double wave_sum(double v) {
+    v += dpp_move<0xB1>(v);
+    v += dpp_move<0x4E>(v);
+    v += dpp_move<0x141>(v);
+    v += dpp_move<0x140>(v);
+    v += __shfl_xor(v, 16, 64);
+    v += __shfl_xor(v, 32, 64);
+    return v;
+}
+
 // Deterministic sum over the workgroup: fixed-shape tree over thread slots.
 __device__ __forceinline__ double wg_sum(double v, WgScratch& s) {
     const int t = threadIdx.x;
