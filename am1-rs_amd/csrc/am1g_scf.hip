@@ -440,6 +440,25 @@ __device__ __forceinline__ int64_t tri_pos(int i, int j, int n) {  // upper incl
     return (int64_t)i * n - (int64_t)i * (i - 1) / 2 + (j - i);
 }
 
+// Row and column of entry p of the packed upper triangle of order n (diagonal included,
+// row-major): the row from a single-precision estimate, corrected exactly in integers.
+__device__ __forceinline__ void tri_rowcol(int64_t p, int n, int& i, int& j) {
+    const float b = 2.0f * (float)n + 1.0f;
+    int r = (int)((b - sqrtf(fmaxf(b * b - 8.0f * (float)p, 0.0f))) * 0.5f);
+    r = max(0, min(r, n - 1));
+    while (r > 0 && tri_pos(r, r, n) > p) --r;
+    while (r + 1 < n && tri_pos(r + 1, r + 1, n) <= p) ++r;
+    i = r;
+    j = r + (int)(p - tri_pos(r, r, n));
+}
+
+// The same for the strict upper triangle of order n (the packed error vectors), which is laid
+// out like the full upper triangle of order n - 1 with columns shifted by one.
+__device__ __forceinline__ void stri_rowcol(int64_t p, int n, int& i, int& j) {
+    tri_rowcol(p, n - 1, i, j);
+    j += 1;
+}
+
 // ---------------------------------------------------------------------------------------------
 // DIIS bookkeeping for the RHF loop. History slots hold packed F (n(n+1)/2), packed error
 // (strict upper triangle, n(n-1)/2) and packed D (n(n+1)/2).
@@ -548,17 +567,19 @@ __device__ __noinline__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolS
         double* hf = dv.f + (int64_t)slot * dv.len_s;
         double* he = dv.e + (int64_t)slot * dv.len_a;
         double* hd = dv.d + (int64_t)slot * dv.len_s;
+        for (int64_t pk = t; pk < dv.len_s; pk += kWg) {
+            int i, j;
+            tri_rowcol(pk, nao, i, j);
+            hf[pk] = F[(size_t)i * nao + j];
+            if (need_d) hd[pk] = P[(size_t)i * nao + j];
+        }
         double en = 0.0;
-        for (int64_t idx = t; idx < (int64_t)nn; idx += kWg) {
-            int i = (int)(idx / nao), j = (int)(idx - (int64_t)i * nao);
-            if (j < i) continue;
-            hf[tri_pos(i, j, nao)] = F[idx];
-            if (need_d) hd[tri_pos(i, j, nao)] = P[idx];
-            if (j > i) {
-                double e = T0[idx] - T0[(size_t)j * nao + i];
-                he[(int64_t)i * nao - (int64_t)i * (i + 1) / 2 + (j - i - 1)] = e;
-                en += e * e;
-            }
+        for (int64_t pk = t; pk < dv.len_a; pk += kWg) {
+            int i, j;
+            stri_rowcol(pk, nao, i, j);
+            const double e = T0[(size_t)i * nao + j] - T0[(size_t)j * nao + i];
+            he[pk] = e;
+            en += e * e;
         }
         double err = sqrt(2.0 * wg_sum(en, s));
         __syncthreads();
@@ -584,34 +605,45 @@ __device__ __noinline__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolS
             double part[3 * kDiisDepth];
 #pragma unroll
             for (int q = 0; q < 3 * kDiisDepth; ++q) part[q] = 0.0;
-            const double* en_p = dv.e + (int64_t)ms.ring[newest] * dv.len_a;
-            for (int64_t idx = t; idx < dv.len_a; idx += kWg) {
-                double x = en_p[idx];
+            const double* ep[kDiisDepth];
+            const double* fp[kDiisDepth];
+            const double* dq[kDiisDepth];
+#pragma unroll
+            for (int i = 0; i < kDiisDepth; ++i) {
+                const int64_t r = ms.ring[i < m ? i : newest];
+                ep[i] = dv.e + r * dv.len_a;
+                fp[i] = dv.f + r * dv.len_s;
+                dq[i] = dv.d + r * dv.len_s;
+            }
+            const double* en_p = ep[newest];
+            for (int64_t pk = t; pk < dv.len_a; pk += kWg) {
+                const double x = en_p[pk];
 #pragma unroll
                 for (int i = 0; i < kDiisDepth; ++i)
-                    if (i < m) part[i] += x * dv.e[(int64_t)ms.ring[i] * dv.len_a + idx];
+                    if (i < m) part[i] = fma(x, ep[i][pk], part[i]);
             }
             if (need_d) {
-                const double* dn = dv.d + (int64_t)ms.ring[newest] * dv.len_s;
-                const double* fn = dv.f + (int64_t)ms.ring[newest] * dv.len_s;
-                for (int64_t idx = t; idx < dv.len_s; idx += kWg) {
-                    // diagonal entries of the packed triangle count once, the rest twice
-                    int i = 0;
-                    {
-                        // row of idx in the upper triangle: the row start is tri_pos(i, i)
-                        double nn2 = (double)(2 * nao + 1);
-                        i = (int)floor((nn2 - sqrt(nn2 * nn2 - 8.0 * (double)idx)) * 0.5);
-                        if (i < 0) i = 0;
-                        while (i > 0 && tri_pos(i, i, nao) > idx) --i;
-                        while (i + 1 < nao && tri_pos(i + 1, i + 1, nao) <= idx) ++i;
-                    }
-                    double wgt = (tri_pos(i, i, nao) == idx) ? 1.0 : 2.0;
-                    double dnv = dn[idx] * wgt, fnv = fn[idx] * wgt;
+                // <D, F> over the matrix: every packed entry stands for two matrix entries except
+                // the diagonal ones, so sum all packed entries, take half the diagonal off, double
+                const double* dn = dq[newest];
+                const double* fn = fp[newest];
+                for (int64_t pk = t; pk < dv.len_s; pk += kWg) {
+                    const double dnv = dn[pk], fnv = fn[pk];
 #pragma unroll
                     for (int j = 0; j < kDiisDepth; ++j) {
                         if (j >= m) continue;
-                        part[kDiisDepth + j] += dnv * dv.f[(int64_t)ms.ring[j] * dv.len_s + idx];
-                        part[2 * kDiisDepth + j] += fnv * dv.d[(int64_t)ms.ring[j] * dv.len_s + idx];
+                        part[kDiisDepth + j] = fma(dnv, fp[j][pk], part[kDiisDepth + j]);
+                        part[2 * kDiisDepth + j] = fma(fnv, dq[j][pk], part[2 * kDiisDepth + j]);
+                    }
+                }
+                for (int i = t; i < nao; i += kWg) {
+                    const int64_t pk = tri_pos(i, i, nao);
+                    const double dnv = -0.5 * dn[pk], fnv = -0.5 * fn[pk];
+#pragma unroll
+                    for (int j = 0; j < kDiisDepth; ++j) {
+                        if (j >= m) continue;
+                        part[kDiisDepth + j] = fma(dnv, fp[j][pk], part[kDiisDepth + j]);
+                        part[2 * kDiisDepth + j] = fma(fnv, dq[j][pk], part[2 * kDiisDepth + j]);
                     }
                 }
             }
@@ -624,8 +656,8 @@ __device__ __noinline__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolS
                 }
                 if (need_d)
                     for (int j = 0; j < m; ++j) {
-                        ms.G[newest * kDiisDepth + j] = tot[kDiisDepth + j];  // <D_new, F_j>
-                        ms.G[j * kDiisDepth + newest] = tot[2 * kDiisDepth + j];  // <D_j, F_new>
+                        ms.G[newest * kDiisDepth + j] = 2.0 * tot[kDiisDepth + j];      // <D_new, F_j>
+                        ms.G[j * kDiisDepth + newest] = 2.0 * tot[2 * kDiisDepth + j];  // <D_j, F_new>
                     }
             }
             __syncthreads();
@@ -673,12 +705,22 @@ __device__ __noinline__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolS
         if (ms.flags[0]) {
             // T1 = sum_i c_i F_i (unpacked); refuse a non-finite combination
             double bad = 0.0;
-            for (int64_t idx = t; idx < (int64_t)nn; idx += kWg) {
-                int i = (int)(idx / nao), j = (int)(idx - (int64_t)i * nao);
-                int64_t pk = (i <= j) ? tri_pos(i, j, nao) : tri_pos(j, i, nao);
+            const double* fq[kDiisDepth];
+            double cq[kDiisDepth];
+#pragma unroll
+            for (int q = 0; q < kDiisDepth; ++q) {
+                fq[q] = dv.f + (int64_t)ms.ring[q < m ? q : m - 1] * dv.len_s;
+                cq[q] = q < m ? ms.coef[q] : 0.0;
+            }
+            for (int64_t pk = t; pk < dv.len_s; pk += kWg) {
+                int i, j;
+                tri_rowcol(pk, nao, i, j);
                 double acc = 0.0;
-                for (int q = 0; q < m; ++q) acc += ms.coef[q] * dv.f[(int64_t)ms.ring[q] * dv.len_s + pk];
-                T1[idx] = acc;
+#pragma unroll
+                for (int q = 0; q < kDiisDepth; ++q)
+                    if (q < m) acc = fma(cq[q], fq[q][pk], acc);
+                T1[(size_t)i * nao + j] = acc;
+                T1[(size_t)j * nao + i] = acc;
                 if (!isfinite(acc)) bad = 1.0;
             }
             bad = wg_max(bad, s);
@@ -854,17 +896,18 @@ __device__ __noinline__ void scf_uhf(const MolDesc& md, const BatchArgs& g, MolS
             __syncthreads();
             double* hf = dv[c].f + (int64_t)slot * dv[c].len_s;
             double* he = dv[c].e + (int64_t)slot * dv[c].len_a;
-            for (int64_t idx = t; idx < (int64_t)nn; idx += kWg) {
-                int i = (int)(idx / nao), j = (int)(idx - (int64_t)i * nao);
-                if (j <= i) continue;
-                double e = T0[idx] - T0[(size_t)j * nao + i];
+            for (int64_t pk = t; pk < dv[c].len_a; pk += kWg) {
+                int i, j;
+                stri_rowcol(pk, nao, i, j);
+                const double e = T0[(size_t)i * nao + j] - T0[(size_t)j * nao + i];
                 en += e * e;
-                if (diis) he[(int64_t)i * nao - (int64_t)i * (i + 1) / 2 + (j - i - 1)] = e;
+                if (diis) he[pk] = e;
             }
             if (diis)
-                for (int64_t idx = t; idx < (int64_t)nn; idx += kWg) {
-                    int i = (int)(idx / nao), j = (int)(idx - (int64_t)i * nao);
-                    if (j >= i) hf[tri_pos(i, j, nao)] = Fs[c][idx];
+                for (int64_t pk = t; pk < dv[c].len_s; pk += kWg) {
+                    int i, j;
+                    tri_rowcol(pk, nao, i, j);
+                    hf[pk] = Fs[c][(size_t)i * nao + j];
                 }
             __syncthreads();
         }
@@ -887,12 +930,15 @@ __device__ __noinline__ void scf_uhf(const MolDesc& md, const BatchArgs& g, MolS
 #pragma unroll
             for (int q = 0; q < kDiisDepth; ++q) part[q] = 0.0;
             for (int c = 0; c < 2; ++c) {
-                const double* en_p = dv[c].e + (int64_t)ms.ring[newest] * dv[c].len_a;
-                for (int64_t idx = t; idx < dv[c].len_a; idx += kWg) {
-                    double x = en_p[idx];
+                const double* ep[kDiisDepth];
+#pragma unroll
+                for (int i = 0; i < kDiisDepth; ++i) ep[i] = dv[c].e + (int64_t)ms.ring[i < m ? i : newest] * dv[c].len_a;
+                const double* en_p = ep[newest];
+                for (int64_t pk = t; pk < dv[c].len_a; pk += kWg) {
+                    const double x = en_p[pk];
 #pragma unroll
                     for (int i = 0; i < kDiisDepth; ++i)
-                        if (i < m) part[i] += x * dv[c].e[(int64_t)ms.ring[i] * dv[c].len_a + idx];
+                        if (i < m) part[i] = fma(x, ep[i][pk], part[i]);
                 }
             }
             double tot[kDiisDepth];
@@ -925,14 +971,24 @@ __device__ __noinline__ void scf_uhf(const MolDesc& md, const BatchArgs& g, MolS
             if (ms.flags[0]) {
                 // combined Focks into T0 (alpha) and T1 (beta)
                 double* outs[2] = {T0, T1};
-                for (int c = 0; c < 2; ++c)
-                    for (int64_t idx = t; idx < (int64_t)nn; idx += kWg) {
-                        int i = (int)(idx / nao), j = (int)(idx - (int64_t)i * nao);
-                        int64_t pk = (i <= j) ? tri_pos(i, j, nao) : tri_pos(j, i, nao);
+                double cq[kDiisDepth];
+#pragma unroll
+                for (int q = 0; q < kDiisDepth; ++q) cq[q] = q < m ? ms.coef[q] : 0.0;
+                for (int c = 0; c < 2; ++c) {
+                    const double* fq[kDiisDepth];
+#pragma unroll
+                    for (int q = 0; q < kDiisDepth; ++q) fq[q] = dv[c].f + (int64_t)ms.ring[q < m ? q : m - 1] * dv[c].len_s;
+                    for (int64_t pk = t; pk < dv[c].len_s; pk += kWg) {
+                        int i, j;
+                        tri_rowcol(pk, nao, i, j);
                         double acc = 0.0;
-                        for (int q = 0; q < m; ++q) acc += ms.coef[q] * dv[c].f[(int64_t)ms.ring[q] * dv[c].len_s + pk];
-                        outs[c][idx] = acc;
+#pragma unroll
+                        for (int q = 0; q < kDiisDepth; ++q)
+                            if (q < m) acc = fma(cq[q], fq[q][pk], acc);
+                        outs[c][(size_t)i * nao + j] = acc;
+                        outs[c][(size_t)j * nao + i] = acc;
                     }
+                }
                 __syncthreads();
                 // the eigen-step needs T0/T1 as scratch: the extrapolated Focks replace F
                 for (int c = 0; c < 2; ++c)

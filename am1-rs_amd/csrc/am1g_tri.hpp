@@ -1,6 +1,6 @@
 // Workgroup symmetric eigensolver for orders up to kTriMax, one 512-thread workgroup per matrix:
-//   1. Householder tridiagonalization with the matrix in registers: four threads own a row
-//      (32 columns each, in pairs), so only vectors go through LDS. Step k builds reflector k
+//   1. Householder tridiagonalization with the matrix in registers: eight threads own two rows
+//      (16 columns each, in pairs), so only vectors go through LDS. Step k builds reflector k
 //      from row k, which its owners published in the previous step (every wavefront computes
 //      the reflector, so no barrier is needed to share it), and then each owner applies the
 //      previous step's rank-2 update to its row and forms its entry of the product for the new
@@ -42,13 +42,14 @@ __host__ __device__ __forceinline__ int tri_lds_doubles(int n) { return tri_de_o
 __device__ __forceinline__ double tri_guard(double q, double pivmin) { return fabs(q) < pivmin ? -pivmin : q; }
 
 // ---------------------------------------------------------------------------------------------
-// Step 3 helpers: sign counts of the leading principal minors p_i = (d_i - x) p_{i-1} -
-// e_{i-1}^2 p_{i-2} of T - x. A zero minor counts as a change (the sign it takes in the limit of
-// a pivot -> -0, the convention of the pivot recurrence q_i = p_i / p_{i-1}).
+// Step 3 helpers: sign changes along the leading principal minors p_i = (d_i - x) p_{i-1} -
+// e_{i-1}^2 p_{i-2} of T - x, counted from the sign bits. A zero minor reads as positive; inside
+// an unreduced block (e_{i-1} != 0) the next minor is then -e^2 p_{i-1}, so the pair of steps
+// counts exactly one change, as the pivot recurrence q_i = p_i / p_{i-1} does with a zero pivot
+// replaced by -0.
 __device__ __forceinline__ void tri_step(double2 q, double x, double& pm, double& pc, int& cnt) {
-    double pn = fma(q.x - x, pc, -q.y * pm);
-    if (pn == 0.0) pn = -pc * 0x1p-60;
-    cnt += (pn < 0.0) != (pc < 0.0);
+    const double pn = fma(q.x - x, pc, -q.y * pm);
+    cnt += (int)((unsigned)(__double2hiint(pn) ^ __double2hiint(pc)) >> 31);
     pm = pc;
     pc = pn;
 }
@@ -65,8 +66,6 @@ __device__ __forceinline__ void tri_rescale(double& pm, double& pc) {
 __device__ __forceinline__ void tri_count2(const double2* __restrict__ de, int b0, int b1, double xa, double xb,
                                            int& ca, int& cb) {
     double am = 1.0, ac = de[b0].x - xa, bm = 1.0, bc = de[b0].x - xb;
-    if (ac == 0.0) ac = -0x1p-60;
-    if (bc == 0.0) bc = -0x1p-60;
     int na = ac < 0.0, nb = bc < 0.0;
     int i = b0 + 1;
     for (; i + 4 <= b1; i += 4) {
@@ -251,22 +250,29 @@ __device__ __noinline__ int wg_eigh_tri(int n, const double* __restrict__ F, dou
     int* be = bs + n;                      // block end (exclusive)
     int* rk = (int*)vb;                    // rank of each eigenvalue (vb is free after step 1)
     double2* de = (double2*)(lds + tri_de_offset(n));  // (d_i, e_{i-1}^2)
-    const int rg = t >> 2, rl = t & 3;  // row rg is owned by four threads; rl owns columns
-                                        // 2 rl + 8 i + {0, 1}, i = 0..15
-    double zr[32];                      // the owned part of row rg
+    // Register blocking: a group of eight threads owns rows 2g and 2g + 1 (g = t / 8); thread q
+    // of the group owns columns 2q + 16j + {0, 1}, j = 0..7, of both rows, in zr[b][2j + h].
+    const int g8 = t >> 3, q8 = t & 7;
+    double zr[2][16];
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        const int c = 2 * rl + 8 * i;
-        zr[2 * i] = (rg < n && c < n) ? 0.5 * (F[(size_t)rg * n + c] + F[(size_t)c * n + rg]) : 0.0;
-        zr[2 * i + 1] = (rg < n && c + 1 < n) ? 0.5 * (F[(size_t)rg * n + c + 1] + F[(size_t)(c + 1) * n + rg]) : 0.0;
+    for (int b = 0; b < 2; ++b) {
+        const int r = 2 * g8 + b;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int c = 2 * q8 + 16 * j + h;
+                zr[b][2 * j + h] = (r < n && c < n) ? 0.5 * (F[(size_t)r * n + c] + F[(size_t)c * n + r]) : 0.0;
+            }
+        }
     }
     for (int idx = t; idx < 4 * sv; idx += kWg) vb[idx] = 0.0;  // both parities of v and p
-    if (rg == 0) {  // row 0 is published for step 0
+    if (g8 == 0) {  // row 0 is published for step 0
 #pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            const int c = 2 * rl + 8 * i;
-            if (c < n) pub[c] = zr[2 * i];
-            if (c + 1 < n) pub[c + 1] = zr[2 * i + 1];
+        for (int j = 0; j < 8; ++j) {
+            const int c = 2 * q8 + 16 * j;
+            if (c < n) pub[c] = zr[0][2 * j];
+            if (c + 1 < n) pub[c + 1] = zr[0][2 * j + 1];
         }
     }
     if (t == 0) {
@@ -278,6 +284,9 @@ __device__ __noinline__ int wg_eigh_tri(int n, const double* __restrict__ F, dou
     // ---- 1. tridiagonalization
     int par = 0;        // parity of the reflector built in this step; the previous one is par ^ 1
     double tauo = 0.0;  // tau of the previous reflector (0 before the first)
+    double vown[16];    // the previous reflector on the owned columns
+#pragma unroll
+    for (int u = 0; u < 16; ++u) vown[u] = 0.0;
     for (int k = 0; k + 2 < n; ++k) {
         const double* vo = vb + (par ^ 1) * sv;
         const double* po = pb + (par ^ 1) * sv;
@@ -337,35 +346,55 @@ __device__ __noinline__ int wg_eigh_tri(int n, const double* __restrict__ F, dou
         }
         // owned rows k+1..: previous update (zero on the columns it does not touch, since the
         // previous v and p vanish there) and this reflector's product p_r = tau sum_c a_rc v_c
-        if (rg > k && rg < n) {
-            const double vr = vo[rg], wr = fma(-Ko, vr, po[rg]);
-            double s0 = 0.0, s1 = 0.0;
+        const int r0 = 2 * g8, r1 = r0 + 1;
+        const bool on0 = r0 > k && r0 < n, on1 = r1 > k && r1 < n;
+        const double vr0 = vo[r0 < n ? r0 : 0], wr0 = fma(-Ko, vr0, po[r0 < n ? r0 : 0]);
+        const double vr1 = vo[r1 < n ? r1 : 0], wr1 = fma(-Ko, vr1, po[r1 < n ? r1 : 0]);
+        double s0 = 0.0, s1 = 0.0;
 #pragma unroll
-            for (int i = 0; i < 16; ++i) {
-                const int c = 2 * rl + 8 * i;  // even, so the pairs are 16-byte aligned
-                if (c >= n) continue;          // (c + 1 <= n lies inside the vector stride)
-                const double2 v2 = *(const double2*)(vo + c);
-                const double2 p2 = *(const double2*)(po + c);
-                const double2 u2 = *(const double2*)(vn + c);
-                const double a0 = fma(-vr, fma(-Ko, v2.x, p2.x), fma(-wr, v2.x, zr[2 * i]));
-                const double a1 = fma(-vr, fma(-Ko, v2.y, p2.y), fma(-wr, v2.y, zr[2 * i + 1]));
-                zr[2 * i] = a0;
-                zr[2 * i + 1] = a1;
-                s0 = fma(a0, u2.x, s0);
-                s1 = fma(a1, u2.y, s1);
+        for (int j = 0; j < 8; ++j) {
+            const int c = 2 * q8 + 16 * j;  // even, so the pairs are 16-byte aligned
+            if (c >= n) continue;           // (c + 1 <= n lies inside the vector stride)
+            const double2 p2 = *(const double2*)(po + c);
+            const double2 u2 = *(const double2*)(vn + c);
+            const double w0 = fma(-Ko, vown[2 * j], p2.x), w1 = fma(-Ko, vown[2 * j + 1], p2.y);
+            const double a00 = fma(-vr0, w0, fma(-wr0, vown[2 * j], zr[0][2 * j]));
+            const double a01 = fma(-vr0, w1, fma(-wr0, vown[2 * j + 1], zr[0][2 * j + 1]));
+            const double a10 = fma(-vr1, w0, fma(-wr1, vown[2 * j], zr[1][2 * j]));
+            const double a11 = fma(-vr1, w1, fma(-wr1, vown[2 * j + 1], zr[1][2 * j + 1]));
+            if (on0) {
+                zr[0][2 * j] = a00;
+                zr[0][2 * j + 1] = a01;
             }
-            double acc = s0 + s1;
-            acc += __shfl_xor(acc, 1, 64);
-            acc += __shfl_xor(acc, 2, 64);
-            if (rl == 0) pn[rg] = tk * acc;
-            if (rg == k + 1) {  // publish row k+1 for the next step
-                double* nxt = pub + ((k + 1) & 1) * sv;
+            if (on1) {
+                zr[1][2 * j] = a10;
+                zr[1][2 * j + 1] = a11;
+            }
+            s0 = fma(a00, u2.x, fma(a01, u2.y, s0));
+            s1 = fma(a10, u2.x, fma(a11, u2.y, s1));
+            vown[2 * j] = u2.x;  // this reflector is the next step's previous one
+            vown[2 * j + 1] = u2.y;
+        }
+        s0 += dpp_move<0xB1>(s0);  // sums over the group of eight
+        s1 += dpp_move<0xB1>(s1);
+        s0 += dpp_move<0x4E>(s0);
+        s1 += dpp_move<0x4E>(s1);
+        s0 += dpp_move<0x141>(s0);
+        s1 += dpp_move<0x141>(s1);
+        if (q8 == 0) {
+            if (on0) pn[r0] = tk * s0;
+            if (on1) pn[r1] = tk * s1;
+        }
+        if (r0 == k + 1 || r1 == k + 1) {  // publish row k+1 for the next step
+            const int b = r1 == k + 1;
+            double* nxt = pub + ((k + 1) & 1) * sv;
 #pragma unroll
-                for (int i = 0; i < 16; ++i) {
-                    const int c = 2 * rl + 8 * i;
-                    if (c < n) nxt[c] = zr[2 * i];
-                    if (c + 1 < n) nxt[c + 1] = zr[2 * i + 1];
-                }
+            for (int j = 0; j < 8; ++j) {
+                const int c = 2 * q8 + 16 * j;
+                const double z0 = b ? zr[1][2 * j] : zr[0][2 * j];
+                const double z1 = b ? zr[1][2 * j + 1] : zr[0][2 * j + 1];
+                if (c < n) nxt[c] = z0;
+                if (c + 1 < n) nxt[c + 1] = z1;
             }
         }
         tauo = tk;
@@ -380,16 +409,20 @@ __device__ __noinline__ int wg_eigh_tri(int n, const double* __restrict__ F, dou
         double pv = 0.0;
         for (int i = k0 + lane; i < n; i += 64) pv = fma(po[i], vo[i], pv);
         const double Ko = 0.5 * tauo * wave_sum(pv);
-        if (rg >= k0 && rg < n) {
-            const double vr = vo[rg], wr = fma(-Ko, vr, po[rg]);
 #pragma unroll
-            for (int i = 0; i < 16; ++i) {
+        for (int b = 0; b < 2; ++b) {
+            const int r = 2 * g8 + b;
+            if (r >= k0 && r < n) {
+                const double vr = vo[r], wr = fma(-Ko, vr, po[r]);
 #pragma unroll
-                for (int b = 0; b < 2; ++b) {
-                    const int c = 2 * rl + 8 * i + b;
-                    if (c >= k0 && c < n) {
-                        const double a = fma(-vr, fma(-Ko, vo[c], po[c]), fma(-wr, vo[c], zr[2 * i + b]));
-                        s.bcast[8 + 2 * (rg - k0) + (c - k0)] = a;
+                for (int j = 0; j < 8; ++j) {
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        const int c = 2 * q8 + 16 * j + h;
+                        if (c >= k0 && c < n) {
+                            const double a = fma(-vr, fma(-Ko, vo[c], po[c]), fma(-wr, vo[c], zr[b][2 * j + h]));
+                            s.bcast[8 + 2 * (r - k0) + (c - k0)] = a;
+                        }
                     }
                 }
             }
@@ -576,13 +609,19 @@ __device__ __noinline__ int wg_eigh_tri(int n, const double* __restrict__ F, dou
         }
     }
     __syncthreads();
-    // back transformation x^T = z^T H_{n-3} ... H_0: row rg of the eigenvectors of T into the
-    // owners' registers, then all reflectors into the freed LDS area (stride sv)
+    // back transformation x^T = z^T H_{n-3} ... H_0: rows 2g, 2g + 1 of the eigenvectors of T
+    // into the owners' registers, then all reflectors into the freed LDS area (stride sv)
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        const int c = 2 * rl + 8 * i;
-        zr[2 * i] = (rg < n && c < n) ? A[rg * ld + c] : 0.0;
-        zr[2 * i + 1] = (rg < n && c + 1 < n) ? A[rg * ld + c + 1] : 0.0;
+    for (int b = 0; b < 2; ++b) {
+        const int r = 2 * g8 + b;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int c = 2 * q8 + 16 * j + h;
+                zr[b][2 * j + h] = (r < n && c < n) ? A[r * ld + c] : 0.0;
+            }
+        }
     }
     __syncthreads();
     for (int idx = t; idx < (n - 2) * sv; idx += kWg) {
@@ -590,36 +629,46 @@ __device__ __noinline__ int wg_eigh_tri(int n, const double* __restrict__ F, dou
         A[idx] = (c > k && c < n) ? Vg[(size_t)k * n + c] : 0.0;
     }
     __syncthreads();
-    if (rg < n) {
+    if (2 * g8 < n) {
         for (int k = n - 3; k >= 0; --k) {
             const double tk = tau[k];
             if (tk == 0.0) continue;
             const double* v = A + k * sv;
-            double2 vv[16];
+            double2 vv[8];
             double s0 = 0.0, s1 = 0.0;
 #pragma unroll
-            for (int i = 0; i < 16; ++i) {
-                const int c = 2 * rl + 8 * i;
-                vv[i] = c < n ? *(const double2*)(v + c) : make_double2(0.0, 0.0);
-                s0 = fma(zr[2 * i], vv[i].x, s0);
-                s1 = fma(zr[2 * i + 1], vv[i].y, s1);
+            for (int j = 0; j < 8; ++j) {
+                const int c = 2 * q8 + 16 * j;
+                vv[j] = c < n ? *(const double2*)(v + c) : make_double2(0.0, 0.0);
+                s0 = fma(zr[0][2 * j], vv[j].x, fma(zr[0][2 * j + 1], vv[j].y, s0));
+                s1 = fma(zr[1][2 * j], vv[j].x, fma(zr[1][2 * j + 1], vv[j].y, s1));
             }
-            double dot = s0 + s1;
-            dot += __shfl_xor(dot, 1, 64);
-            dot += __shfl_xor(dot, 2, 64);
-            const double f = tk * dot;
+            s0 += dpp_move<0xB1>(s0);
+            s1 += dpp_move<0xB1>(s1);
+            s0 += dpp_move<0x4E>(s0);
+            s1 += dpp_move<0x4E>(s1);
+            s0 += dpp_move<0x141>(s0);
+            s1 += dpp_move<0x141>(s1);
+            const double f0 = tk * s0, f1 = tk * s1;
 #pragma unroll
-            for (int i = 0; i < 16; ++i) {
-                zr[2 * i] = fma(-f, vv[i].x, zr[2 * i]);
-                zr[2 * i + 1] = fma(-f, vv[i].y, zr[2 * i + 1]);
+            for (int j = 0; j < 8; ++j) {
+                zr[0][2 * j] = fma(-f0, vv[j].x, zr[0][2 * j]);
+                zr[0][2 * j + 1] = fma(-f0, vv[j].y, zr[0][2 * j + 1]);
+                zr[1][2 * j] = fma(-f1, vv[j].x, zr[1][2 * j]);
+                zr[1][2 * j + 1] = fma(-f1, vv[j].y, zr[1][2 * j + 1]);
             }
         }
-        double* dst = Ct + (size_t)rk[rg] * n;
 #pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            const int c = 2 * rl + 8 * i;
-            if (c < n) dst[c] = zr[2 * i];
-            if (c + 1 < n) dst[c + 1] = zr[2 * i + 1];
+        for (int b = 0; b < 2; ++b) {
+            const int r = 2 * g8 + b;
+            if (r >= n) continue;
+            double* dst = Ct + (size_t)rk[r] * n;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int c = 2 * q8 + 16 * j;
+                if (c < n) dst[c] = zr[b][2 * j];
+                if (c + 1 < n) dst[c + 1] = zr[b][2 * j + 1];
+            }
         }
     }
     __syncthreads();
