@@ -238,6 +238,7 @@ __device__ __noinline__ void mol_fock(const MolDesc& md, const DevElem* el, cons
                 acc -= scale * Ps[(size_t)(ox + la) * nao + ox + si] * one_centre(mu, la, nu, si, ex);
             }
         // two-centre Coulomb from every partner's diagonal density block
+#pragma unroll 4
         for (int y = 0; y < nat; ++y) {
             if (y == x) continue;
             BlockRef br = block_of(ms, wpool, x, y);
@@ -567,6 +568,7 @@ __device__ __noinline__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolS
         double* hf = dv.f + (int64_t)slot * dv.len_s;
         double* he = dv.e + (int64_t)slot * dv.len_a;
         double* hd = dv.d + (int64_t)slot * dv.len_s;
+#pragma unroll 4
         for (int64_t pk = t; pk < dv.len_s; pk += kWg) {
             int i, j;
             tri_rowcol(pk, nao, i, j);
@@ -574,6 +576,7 @@ __device__ __noinline__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolS
             if (need_d) hd[pk] = P[(size_t)i * nao + j];
         }
         double en = 0.0;
+#pragma unroll 4
         for (int64_t pk = t; pk < dv.len_a; pk += kWg) {
             int i, j;
             stri_rowcol(pk, nao, i, j);
@@ -616,6 +619,7 @@ __device__ __noinline__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolS
                 dq[i] = dv.d + r * dv.len_s;
             }
             const double* en_p = ep[newest];
+#pragma unroll 2
             for (int64_t pk = t; pk < dv.len_a; pk += kWg) {
                 const double x = en_p[pk];
 #pragma unroll
@@ -627,6 +631,7 @@ __device__ __noinline__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolS
                 // the diagonal ones, so sum all packed entries, take half the diagonal off, double
                 const double* dn = dq[newest];
                 const double* fn = fp[newest];
+#pragma unroll 2
                 for (int64_t pk = t; pk < dv.len_s; pk += kWg) {
                     const double dnv = dn[pk], fnv = fn[pk];
 #pragma unroll
@@ -712,6 +717,7 @@ __device__ __noinline__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolS
                 fq[q] = dv.f + (int64_t)ms.ring[q < m ? q : m - 1] * dv.len_s;
                 cq[q] = q < m ? ms.coef[q] : 0.0;
             }
+#pragma unroll 2
             for (int64_t pk = t; pk < dv.len_s; pk += kWg) {
                 int i, j;
                 tri_rowcol(pk, nao, i, j);
@@ -896,6 +902,7 @@ __device__ __noinline__ void scf_uhf(const MolDesc& md, const BatchArgs& g, MolS
             __syncthreads();
             double* hf = dv[c].f + (int64_t)slot * dv[c].len_s;
             double* he = dv[c].e + (int64_t)slot * dv[c].len_a;
+#pragma unroll 4
             for (int64_t pk = t; pk < dv[c].len_a; pk += kWg) {
                 int i, j;
                 stri_rowcol(pk, nao, i, j);
@@ -934,6 +941,7 @@ __device__ __noinline__ void scf_uhf(const MolDesc& md, const BatchArgs& g, MolS
 #pragma unroll
                 for (int i = 0; i < kDiisDepth; ++i) ep[i] = dv[c].e + (int64_t)ms.ring[i < m ? i : newest] * dv[c].len_a;
                 const double* en_p = ep[newest];
+#pragma unroll 2
                 for (int64_t pk = t; pk < dv[c].len_a; pk += kWg) {
                     const double x = en_p[pk];
 #pragma unroll
@@ -978,6 +986,7 @@ __device__ __noinline__ void scf_uhf(const MolDesc& md, const BatchArgs& g, MolS
                     const double* fq[kDiisDepth];
 #pragma unroll
                     for (int q = 0; q < kDiisDepth; ++q) fq[q] = dv[c].f + (int64_t)ms.ring[q < m ? q : m - 1] * dv[c].len_s;
+#pragma unroll 2
                     for (int64_t pk = t; pk < dv[c].len_s; pk += kWg) {
                         int i, j;
                         tri_rowcol(pk, nao, i, j);

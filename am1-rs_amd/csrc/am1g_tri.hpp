@@ -317,11 +317,16 @@ __device__ __noinline__ int wg_eigh_tri(int n, const double* __restrict__ F, dou
         double sig = ra >= k + 2 ? xa * xa : 0.0;
         if (rb >= k + 2) sig = fma(xb, xb, sig);
         sig = wave_sum(sig);
+        // beta = -sign(x0) |x|, tau = (beta - x0) / beta = 1 + |x0| / |x|, v = x / (x0 - beta);
+        // 1/|x| and the last reciprocal from refined hardware estimates (a few ulps)
         double tk = 0.0, beta = x0, scale = 0.0;
         if (sig > 0.0) {
-            beta = -copysign(sqrt(fma(x0, x0, sig)), x0);
-            tk = (beta - x0) / beta;
-            scale = 1.0 / (x0 - beta);
+            const double nx2 = fma(x0, x0, sig);
+            const double rn = (nx2 > 1.0e-280 && nx2 < 1.0e280) ? fast_rsqrt(nx2) : 1.0 / sqrt(nx2);
+            beta = -copysign(nx2 * rn, x0);
+            tk = fma(fabs(x0), rn, 1.0);
+            const double den = x0 - beta;
+            scale = (fabs(den) > 1.0e-280 && fabs(den) < 1.0e280) ? fast_rcp(den) : 1.0 / den;
         }
         const double va = ra == k + 1 ? 1.0 : (ra >= k + 2 ? xa * scale : 0.0);
         const double vbv = rb == k + 1 ? 1.0 : (rb >= k + 2 ? xb * scale : 0.0);
@@ -624,6 +629,7 @@ __device__ __noinline__ int wg_eigh_tri(int n, const double* __restrict__ F, dou
         }
     }
     __syncthreads();
+#pragma unroll 8
     for (int idx = t; idx < (n - 2) * sv; idx += kWg) {
         const int k = idx / sv, c = idx - k * sv;
         A[idx] = (c > k && c < n) ? Vg[(size_t)k * n + c] : 0.0;

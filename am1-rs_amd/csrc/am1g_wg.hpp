@@ -29,16 +29,21 @@ __device__ __forceinline__ double dpp_move(double v) {
     return __hiloint2double(hi, lo);
 }
 
-// Sum over the 64 lanes of a wavefront, the same bits in every lane (each step adds a lane's
-// value to its partner's, and a + b == b + a): four row steps on DPP, two across rows.
+__device__ __forceinline__ double read_lane(double v, int l) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                            __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+
+// Sum over the 64 lanes of a wavefront (all lanes active), the same bits in every lane: four
+// DPP steps give every lane its row of 16's sum (each step adds a lane's value to its
+// partner's, and a + b == b + a), then the four row sums are read as scalars and added in a
+// fixed order.
 __device__ __forceinline__ double wave_sum(double v) {
     v += dpp_move<0xB1>(v);
     v += dpp_move<0x4E>(v);
     v += dpp_move<0x141>(v);
     v += dpp_move<0x140>(v);
-    v += __shfl_xor(v, 16, 64);
-    v += __shfl_xor(v, 32, 64);
-    return v;
+    return (read_lane(v, 0) + read_lane(v, 16)) + (read_lane(v, 32) + read_lane(v, 48));
 }
 
 // Deterministic sum over the workgroup: fixed-shape tree over thread slots.
@@ -138,10 +143,11 @@ __device__ __noinline__ void wg_gemm_staged(int M, int N, int K, double alpha, c
     const int t = threadIdx.x;
     const int tm = (M + 3) >> 2, tn = (N + 3) >> 2, ntile = tm * tn;
     const int Mp = tm * 4, Np = tn * 4;
-    int kp = sm_doubles / (Mp + Np);
-    kp = kp > 32 ? 32 : kp;
-    double* As = sm;
-    double* Bs = sm + kp * Mp;
+    // panels are staged in the arithmetic type (a float panel takes half the space and traffic)
+    int kp = (int)(sm_doubles * (sizeof(double) / sizeof(T))) / (Mp + Np);
+    kp = kp > 64 ? 64 : kp;
+    T* As = (T*)sm;
+    T* Bs = As + kp * Mp;
     T acc[2][4][4];
 #pragma unroll
     for (int q = 0; q < 2; ++q)
@@ -161,7 +167,7 @@ __device__ __noinline__ void wg_gemm_staged(int M, int N, int K, double alpha, c
                 kk = idx - i * kc;
             }
             int ic = min(i, M - 1);
-            As[kk * Mp + i] = TA ? A[(size_t)(k0 + kk) * lda + ic] : A[(size_t)ic * lda + k0 + kk];
+            As[kk * Mp + i] = (T)(TA ? A[(size_t)(k0 + kk) * lda + ic] : A[(size_t)ic * lda + k0 + kk]);
         }
         for (int idx = t; idx < kc * Np; idx += kWg) {
             int kk, j;
@@ -173,7 +179,7 @@ __device__ __noinline__ void wg_gemm_staged(int M, int N, int K, double alpha, c
                 j = idx - kk * Np;
             }
             int jc = min(j, N - 1);
-            Bs[kk * Np + j] = TB ? B[(size_t)jc * ldb + k0 + kk] : B[(size_t)(k0 + kk) * ldb + jc];
+            Bs[kk * Np + j] = (T)(TB ? B[(size_t)jc * ldb + k0 + kk] : B[(size_t)(k0 + kk) * ldb + jc]);
         }
         __syncthreads();
 #pragma unroll
@@ -184,9 +190,9 @@ __device__ __noinline__ void wg_gemm_staged(int M, int N, int K, double alpha, c
                 for (int kk = 0; kk < kc; ++kk) {
                     T av[4], bv[4];
 #pragma unroll
-                    for (int a = 0; a < 4; ++a) av[a] = (T)As[kk * Mp + i0 + a];
+                    for (int a = 0; a < 4; ++a) av[a] = As[kk * Mp + i0 + a];
 #pragma unroll
-                    for (int b = 0; b < 4; ++b) bv[b] = (T)Bs[kk * Np + j0 + b];
+                    for (int b = 0; b < 4; ++b) bv[b] = Bs[kk * Np + j0 + b];
 #pragma unroll
                     for (int a = 0; a < 4; ++a)
 #pragma unroll
@@ -210,12 +216,117 @@ __device__ __noinline__ void wg_gemm_staged(int M, int N, int K, double alpha, c
     }
 }
 
-// Dispatch: LDS-staged when the output fits two tiles per thread, register/global otherwise.
+// Matrix-core GEMM for M, N <= 128: C = alpha * opA (M x K) * opB (K x N) on 16 x 16 output
+// tiles, each an accumulation of 16x16x4 MFMA products (FP64, or FP32 with FP32 accumulation);
+// wavefront w owns tiles w, w + 8, ... (row-major over the tile grid). K-panels of both operands
+// are staged through LDS in the arithmetic type, zero-padded to whole tiles and to a multiple
+// of four in K. Operand lanes: A(i, k) in lane i + 16k, B(k, j) in lane j + 16k; result lane l,
+// register r: row l/16 + 4r (FP64) or 4(l/16) + r (FP32), column l%16.
+typedef double mf64x4 __attribute__((ext_vector_type(4)));
+typedef float mf32x4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct MfmaAcc;
+template <>
+struct MfmaAcc<double> {
+    typedef mf64x4 V;
+    static __device__ __forceinline__ V mma(double a, double b, V c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int l, int r) { return (l >> 4) + 4 * r; }
+};
+template <>
+struct MfmaAcc<float> {
+    typedef mf32x4 V;
+    static __device__ __forceinline__ V mma(float a, float b, V c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int l, int r) { return 4 * (l >> 4) + r; }
+};
+
+template <typename T, bool TA, bool TB>
+__device__ __noinline__ void wg_gemm_mfma(int M, int N, int K, double alpha, const double* __restrict__ A, int lda,
+                                          const double* __restrict__ B, int ldb, double* __restrict__ C, int ldc,
+                                          double* sm, int sm_doubles) {
+    typedef typename MfmaAcc<T>::V V;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int TM = (M + 15) >> 4, TN = (N + 15) >> 4, ntile = TM * TN;
+    const int Mp = TM * 16, Np = TN * 16;
+    int kp = (int)(sm_doubles * (sizeof(double) / sizeof(T))) / (Mp + Np);
+    kp = (kp > 64 ? 64 : kp) & ~3;
+    T* As = (T*)sm;
+    T* Bs = As + kp * Mp;
+    V acc[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc[u] = V{0, 0, 0, 0};
+    for (int k0 = 0; k0 < K; k0 += kp) {
+        const int kc = min(kp, K - k0), kc4 = (kc + 3) & ~3;
+#pragma unroll 8
+        for (int idx = t; idx < kc4 * Mp; idx += kWg) {
+            int kk, i;
+            if (TA) {
+                kk = idx / Mp;
+                i = idx - kk * Mp;
+            } else {
+                i = idx / kc4;
+                kk = idx - i * kc4;
+            }
+            As[kk * Mp + i] = (kk < kc && i < M) ? (T)(TA ? A[(size_t)(k0 + kk) * lda + i] : A[(size_t)i * lda + k0 + kk])
+                                               : (T)0;
+        }
+#pragma unroll 8
+        for (int idx = t; idx < kc4 * Np; idx += kWg) {
+            int kk, j;
+            if (TB) {
+                j = idx / kc4;
+                kk = idx - j * kc4;
+            } else {
+                kk = idx / Np;
+                j = idx - kk * Np;
+            }
+            Bs[kk * Np + j] = (kk < kc && j < N) ? (T)(TB ? B[(size_t)j * ldb + k0 + kk] : B[(size_t)(k0 + kk) * ldb + j])
+                                               : (T)0;
+        }
+        __syncthreads();
+        for (int kk = 0; kk < kc4; kk += 4) {
+            const int ka = (kk + (lane >> 4));
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int tile = w + 8 * u;
+                if (tile < ntile) {
+                    const int ti = tile / TN, tj = tile - ti * TN;
+                    const T a = As[ka * Mp + ti * 16 + (lane & 15)];
+                    const T b = Bs[ka * Np + tj * 16 + (lane & 15)];
+                    acc[u] = MfmaAcc<T>::mma(a, b, acc[u]);
+                }
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        const int tile = w + 8 * u;
+        if (tile < ntile) {
+            const int ti = tile / TN, tj = tile - ti * TN;
+            const int col = tj * 16 + (lane & 15);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = ti * 16 + MfmaAcc<T>::row(lane, r);
+                if (row < M && col < N) C[(size_t)row * ldc + col] = alpha * (double)acc[u][r];
+            }
+        }
+    }
+}
+
+// Dispatch: matrix cores when the output has at most 8 x 8 tiles of 16, LDS-staged register
+// tiles when it fits two 4 x 4 tiles per thread, register/global otherwise.
 template <typename T, bool TA, bool TB>
 __device__ __forceinline__ void wg_gemm_any(int M, int N, int K, double alpha, const double* A, int lda,
                                             const double* B, int ldb, double* C, int ldc, double* sm,
                                             int sm_doubles) {
-    if (M <= 128 && N <= 128 && ((M + 3) / 4) * ((N + 3) / 4) <= 2 * kWg)
+    if (M <= 128 && N <= 128)
+        wg_gemm_mfma<T, TA, TB>(M, N, K, alpha, A, lda, B, ldb, C, ldc, sm, sm_doubles);
+    else if (M <= 128 && N <= 128 && ((M + 3) / 4) * ((N + 3) / 4) <= 2 * kWg)
         wg_gemm_staged<T, TA, TB>(M, N, K, alpha, A, lda, B, ldb, C, ldc, sm, sm_doubles);
     else
         wg_gemm<T, TA, TB>(M, N, K, alpha, A, lda, B, ldb, C, ldc);

@@ -220,14 +220,19 @@ def main():
         slow_rate, slow_n = 20.0, 60
         fast = G.serving_stream(args.serve_requests, args.serve_rate)
         slow = G.serving_stream(slow_n, slow_rate)
+        mid_rate = 200.0
+        mid = G.serving_stream(args.serve_requests, mid_rate)
         serving = {
             "plain": serve_stream(am1g, slow, slow_rate, lanes=1, max_batch=1, precision="fp64"),
             "batched_mixed_same_load": serve_stream(am1g, slow, slow_rate, lanes=4, max_batch=512,
                                                     precision="mixed"),
+            "batched_mixed_200": serve_stream(am1g, mid, mid_rate, lanes=4, max_batch=512, precision="mixed"),
             "batched_mixed": serve_stream(am1g, fast, args.serve_rate, lanes=4, max_batch=512, precision="mixed"),
             "note": "AM1-BCC charges are not computed (the SCF each such request needs is); "
                     "latency = submit to completion on the host clock",
         }
+        log(f"serving at {mid_rate:g}/s: {serving['batched_mixed_200']['scfs_per_s']} SCFs/s "
+            f"p50 {serving['batched_mixed_200']['p50_ms']:.1f} ms p99 {serving['batched_mixed_200']['p99_ms']:.1f} ms")
         log(f"serving: plain {serving['plain']['scfs_per_s']} SCFs/s p99 {serving['plain']['p99_ms']:.1f} ms; "
             f"batched mixed at {args.serve_rate:g}/s: {serving['batched_mixed']['scfs_per_s']} SCFs/s "
             f"p99 {serving['batched_mixed']['p99_ms']:.1f} ms")
