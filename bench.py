@@ -101,7 +101,19 @@ def main():
     ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=1000)
     ap.add_argument("--serve-rate", type=float, default=2000.0)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only copy 1 GiB device to device (calibrates the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        n = 256 * 1024 * 1024  # 2^28 float32 = 1 GiB
+        x = torch.ones(n, device="cuda")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copied_bytes": n * 4}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (every box in this setup has one); --gpus {args.gpus} "
               "is not supported. Run with --gpus 1.", file=sys.stderr)

@@ -99,11 +99,13 @@ void am1g_request_defaults(am1g_request* rq);
 int am1g_create(int device, am1g_ctx** out);
 void am1g_destroy(am1g_ctx* ctx);
 
-/* Plain path: one molecule, one launch sequence per SCF phase. Replaces run_am1
+/* Plain path: one molecule. Up to 128 basis functions (and 256 atoms) one workgroup runs the
+ * whole SCF; larger molecules run on grid-level kernels over the whole device. Replaces run_am1
  * (src/scf.rs:294-298). */
 int am1g_scf(am1g_ctx* ctx, const am1g_request* rq, am1g_result* res);
 
-/* Batched path: n independent molecules of any size in one persistent launch. Each result is
+/* Batched path: n independent molecules (each at most 256 atoms and 256 basis functions) in one
+ * persistent launch, one workgroup per molecule. Each result is
  * the same as the molecule's am1g_scf result within the stated tolerance. Returns AM1G_OK when
  * the launch ran; per-molecule outcomes are in results[i].status. */
 int am1g_scf_batch(am1g_ctx* ctx, int n, const am1g_request* rqs, am1g_result* results);
