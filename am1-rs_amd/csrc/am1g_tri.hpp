@@ -33,7 +33,9 @@ __device__ __forceinline__ int tri_ld(int n) { return n | 1; }  // odd row strid
 // rows, block starts and ends as ints; the vector pairs start at even offsets), then the
 // (d_i, e_{i-1}^2) pairs at an even offset.
 __host__ __device__ __forceinline__ int tri_vec_offset(int n) { return (n * (n | 1) + 4 * n + 1) & ~1; }
-__host__ __device__ __forceinline__ int tri_vec_stride(int n) { return (n + 1) & ~1; }
+// Every vector has kTriMax entries (zero beyond n), so the register-blocked passes load all of
+// their columns without bounds tests and the compiler can issue the loads together.
+__host__ __device__ __forceinline__ int tri_vec_stride(int) { return kTriMax; }
 __host__ __device__ __forceinline__ int tri_de_offset(int n) {
     return (tri_vec_offset(n) + 6 * tri_vec_stride(n) + n + 1) & ~1;
 }
@@ -356,10 +358,10 @@ __device__ __noinline__ int wg_eigh_tri(int n, const double* __restrict__ F, dou
         const double vr0 = vo[r0 < n ? r0 : 0], wr0 = fma(-Ko, vr0, po[r0 < n ? r0 : 0]);
         const double vr1 = vo[r1 < n ? r1 : 0], wr1 = fma(-Ko, vr1, po[r1 < n ? r1 : 0]);
         double s0 = 0.0, s1 = 0.0;
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const int c = 2 * q8 + 16 * j;  // even, so the pairs are 16-byte aligned
-            if (c >= n) continue;           // (c + 1 <= n lies inside the vector stride)
+        // columns 2 q8 + 16 j: the pairs are 16-byte aligned; the second half of the columns only
+        // when n > 64, so each half's loads are issued together
+        auto fused = [&](int j) {
+            const int c = 2 * q8 + 16 * j;
             const double2 p2 = *(const double2*)(po + c);
             const double2 u2 = *(const double2*)(vn + c);
             const double w0 = fma(-Ko, vown[2 * j], p2.x), w1 = fma(-Ko, vown[2 * j + 1], p2.y);
@@ -379,6 +381,12 @@ __device__ __noinline__ int wg_eigh_tri(int n, const double* __restrict__ F, dou
             s1 = fma(a10, u2.x, fma(a11, u2.y, s1));
             vown[2 * j] = u2.x;  // this reflector is the next step's previous one
             vown[2 * j + 1] = u2.y;
+        };
+#pragma unroll
+        for (int j = 0; j < 4; ++j) fused(j);
+        if (n > 64) {
+#pragma unroll
+            for (int j = 4; j < 8; ++j) fused(j);
         }
         s0 += dpp_move<0xB1>(s0);  // sums over the group of eight
         s1 += dpp_move<0xB1>(s1);
@@ -396,10 +404,8 @@ __device__ __noinline__ int wg_eigh_tri(int n, const double* __restrict__ F, dou
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
                 const int c = 2 * q8 + 16 * j;
-                const double z0 = b ? zr[1][2 * j] : zr[0][2 * j];
-                const double z1 = b ? zr[1][2 * j + 1] : zr[0][2 * j + 1];
-                if (c < n) nxt[c] = z0;
-                if (c + 1 < n) nxt[c + 1] = z1;
+                *(double2*)(nxt + c) = b ? make_double2(zr[1][2 * j], zr[1][2 * j + 1])
+                                         : make_double2(zr[0][2 * j], zr[0][2 * j + 1]);
             }
         }
         tauo = tk;
@@ -615,7 +621,9 @@ __device__ __noinline__ int wg_eigh_tri(int n, const double* __restrict__ F, dou
     }
     __syncthreads();
     // back transformation x^T = z^T H_{n-3} ... H_0: rows 2g, 2g + 1 of the eigenvectors of T
-    // into the owners' registers, then all reflectors into the freed LDS area (stride sv)
+    // into the owners' registers, then all reflectors into the freed LDS area (even stride sr;
+    // (n - 2) sr <= n ld)
+    const int sr = (n + 1) & ~1;
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
         const int r = 2 * g8 + b;
@@ -630,8 +638,8 @@ __device__ __noinline__ int wg_eigh_tri(int n, const double* __restrict__ F, dou
     }
     __syncthreads();
 #pragma unroll 8
-    for (int idx = t; idx < (n - 2) * sv; idx += kWg) {
-        const int k = idx / sv, c = idx - k * sv;
+    for (int idx = t; idx < (n - 2) * sr; idx += kWg) {
+        const int k = idx / sr, c = idx - k * sr;
         A[idx] = (c > k && c < n) ? Vg[(size_t)k * n + c] : 0.0;
     }
     __syncthreads();
@@ -639,15 +647,25 @@ __device__ __noinline__ int wg_eigh_tri(int n, const double* __restrict__ F, dou
         for (int k = n - 3; k >= 0; --k) {
             const double tk = tau[k];
             if (tk == 0.0) continue;
-            const double* v = A + k * sv;
+            const double* v = A + k * sr;
             double2 vv[8];
             double s0 = 0.0, s1 = 0.0;
+            // columns >= n hold zeros in the rows and in the owners' registers; beyond the row
+            // stride the load is clamped into the row and its value discarded
 #pragma unroll
-            for (int j = 0; j < 8; ++j) {
+            for (int j = 0; j < 8; ++j) vv[j] = make_double2(0.0, 0.0);
+            auto dot = [&](int j) {
                 const int c = 2 * q8 + 16 * j;
-                vv[j] = c < n ? *(const double2*)(v + c) : make_double2(0.0, 0.0);
+                const double2 ld2 = *(const double2*)(v + min(c, sr - 2));
+                vv[j] = c < n ? ld2 : make_double2(0.0, 0.0);
                 s0 = fma(zr[0][2 * j], vv[j].x, fma(zr[0][2 * j + 1], vv[j].y, s0));
                 s1 = fma(zr[1][2 * j], vv[j].x, fma(zr[1][2 * j + 1], vv[j].y, s1));
+            };
+#pragma unroll
+            for (int j = 0; j < 4; ++j) dot(j);
+            if (n > 64) {
+#pragma unroll
+                for (int j = 4; j < 8; ++j) dot(j);
             }
             s0 += dpp_move<0xB1>(s0);
             s1 += dpp_move<0xB1>(s1);
