@@ -671,10 +671,11 @@ __device__ __noinline__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolS
             status = kScfDiverged;
             break;
         }
-        // extrapolation coefficients (thread 0), scf.rs:627-653
-        if (t == 0) {
-            int use = 0;
-            if (accel == 1 || (accel == 2 && err <= md.adiis_switch)) {
+        // extrapolation coefficients, scf.rs:627-653: CDIIS by thread 0, A-DIIS on wavefront 0
+        const bool cdiis = accel == 1 || (accel == 2 && err <= md.adiis_switch);
+        if (cdiis) {
+            if (t == 0) {
+                int use = 0;
                 if (m >= 2) {
                     double Bm[(kDiisDepth + 1) * (kDiisDepth + 1)], rhs[kDiisDepth + 1], x[kDiisDepth + 1];
                     int dim = m + 1;
@@ -695,15 +696,13 @@ __device__ __noinline__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolS
                         }
                     }
                 }
-            } else if (accel == 2) {
-                if (m >= 2) {
-                    double c[kDiisDepth];
-                    adiis_solve(ms.G, m, c);
-                    use = 1;
-                    for (int i = 0; i < m; ++i) ms.coef[i] = c[i];
-                }
+                ms.flags[0] = use;
             }
-            ms.flags[0] = use;
+        } else if (accel == 2 && m >= 2) {
+            if (t < 64) adiis_solve_wave(ms.G, m, ms.coef);
+            if (t == 0) ms.flags[0] = 1;
+        } else if (t == 0) {
+            ms.flags[0] = 0;
         }
         __syncthreads();
         const double* Fuse = F;

@@ -747,4 +747,73 @@ __host__ __device__ inline void adiis_solve(const double* g, int m, double* c) {
     }
 }
 
+// A-DIIS coefficients on one wavefront: the projected-gradient iteration of adiis_solve with
+// lanes 0..m-1 carrying one coefficient each (called by all 64 lanes of one wavefront). Each
+// step's matrix-vector product, simplex projection (ranks instead of a sort; ties keep the lower
+// index first, as the stable sort does) and convergence test read the other lanes' values as
+// scalars, so every lane takes the same decisions. Writes c_out[0..m).
+__device__ inline void adiis_solve_wave(const double* g, int m, double* c_out) {
+    const int lane = threadIdx.x & 63;
+    const int last = m - 1;
+    const bool act = lane < m;
+    const int k = act ? lane : 0;
+    const double gll = g[last * kDiisDepth + last];
+    const double dk = g[k * kDiisDepth + last] - gll;
+    double Mk[kDiisDepth];
+    double row = 0.0;
+#pragma unroll
+    for (int j = 0; j < kDiisDepth; ++j) {
+        double v = 0.0;
+        if (j < m) {
+            const double skj = g[k * kDiisDepth + j] - g[k * kDiisDepth + last] - g[last * kDiisDepth + j] + gll;
+            const double sjk = g[j * kDiisDepth + k] - g[j * kDiisDepth + last] - g[last * kDiisDepth + k] + gll;
+            v = skj + sjk;
+        }
+        Mk[j] = v;
+        row += fabs(v);
+    }
+    double l = 1.0e-12;
+    for (int i = 0; i < m; ++i) l = fmax(l, read_lane(row, i));
+    const double lr = 1.0 / l;
+    double ck = (act && k == last) ? 1.0 : 0.0;
+    for (int it = 0; it < 400; ++it) {
+        double cj[kDiisDepth];
+#pragma unroll
+        for (int j = 0; j < kDiisDepth; ++j) cj[j] = j < m ? read_lane(ck, j) : 0.0;
+        double acc = 2.0 * dk;
+#pragma unroll
+        for (int j = 0; j < kDiisDepth; ++j)
+            if (j < m) acc += Mk[j] * cj[j];
+        const double trial = ck - lr * acc;
+        int rank = 0;
+        double css = 0.0;
+#pragma unroll
+        for (int j = 0; j < kDiisDepth; ++j) {
+            if (j >= m) continue;
+            const double tj = read_lane(trial, j);
+            const bool above = tj > trial || (tj == trial && j < k);
+            rank += above;
+            if (above || j == k) css += tj;
+        }
+        const double tt = (css - 1.0) / (double)(rank + 1);
+        const int key = (act && trial - tt > 0.0) ? rank : -1;
+        // theta: tt at the largest rank whose condition holds (0 when none holds)
+        double theta = 0.0;
+        int best = -1;
+        for (int j = 0; j < m; ++j) {
+            const int rj = __builtin_amdgcn_readlane(key, j);
+            if (rj > best) {
+                best = rj;
+                theta = read_lane(tt, j);
+            }
+        }
+        const double out = act ? fmax(trial - theta, 0.0) : 0.0;
+        double delta = 0.0;
+        for (int j = 0; j < m; ++j) delta += fabs(read_lane(out, j) - cj[j]);
+        ck = out;
+        if (delta < 1.0e-12) break;
+    }
+    if (act) c_out[k] = ck;
+}
+
 }  // namespace am1g
