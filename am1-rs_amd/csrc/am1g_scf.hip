@@ -40,6 +40,9 @@ struct MolShared {
 // Packed index k (0..9) of a 4-orbital block -> orbital pair (row >= col).
 __device__ __constant__ int kPackRow[10] = {0, 1, 1, 2, 2, 2, 3, 3, 3, 3};
 __device__ __constant__ int kPackCol[10] = {0, 0, 1, 0, 1, 2, 0, 1, 2, 3};
+// The same pairs as compile-time constants, for fully unrolled loops.
+constexpr int kPackRowH[10] = {0, 1, 1, 2, 2, 2, 3, 3, 3, 3};
+constexpr int kPackColH[10] = {0, 0, 1, 0, 1, 2, 0, 1, 2, 3};
 
 // Where the integral block of atoms x and y lives, and whether x is its bra (row) atom.
 struct BlockRef {
@@ -213,12 +216,33 @@ __device__ double mol_setup(const MolDesc& md, int mi, const BatchArgs& g, MolSh
 // (fock.rs:249-511). Diagonal atom blocks: one thread per (atom, packed orbital pair), summing
 // over partner atoms in index order. Off-diagonal blocks: one thread per pair.
 __device__ __noinline__ void mol_fock(const MolDesc& md, const DevElem* el, const MolShared& ms, const double* wpool,
-                         const double* H, const double* Pt, const double* Ps, double scale, double* F) {
+                         const double* H, const double* Pt, const double* Ps, double scale, double* F,
+                         double* pdg) {
     const int t = threadIdx.x;
     const int nat = md.nat, nao = md.nao;
     const int nh = ms.nh, nH = ms.nH;
     const int n_items = nh * 10 + nH;
-    for (int it = t; it < n_items; it += kWg) {
+    // every atom's diagonal density block, packed with the off-diagonal entries doubled (pdg:
+    // nat * 10 doubles of shared scratch), read by all the atom's partners below
+    for (int idx = t; idx < nat * 10; idx += kWg) {
+        const int y = idx / 10, q = idx - y * 10;
+        const int oy = ms.off[y];
+        double v = 0.0;
+        if (ms.z[y] != 1) {
+            const int l = kPackRow[q], c = kPackCol[q];
+            v = Pt[(size_t)(oy + l) * nao + oy + c] * (l == c ? 1.0 : 2.0);
+        } else if (q == 0) {
+            v = Pt[(size_t)oy * nao + oy];
+        }
+        pdg[idx] = v;
+    }
+    __syncthreads();
+    // diagonal blocks, G threads per entry (as many as the workgroup allows, up to 8): the
+    // partners are split among them and their sums combined in a fixed order
+    int G = 1;
+    while (G < 8 && n_items * (2 * G) <= kWg) G *= 2;
+    const int gl = t & (G - 1);
+    for (int it = t / G; it < n_items; it += kWg / G) {
         int x, k;
         if (it < nh * 10) {
             x = ms.heavy[it / 10];
@@ -230,34 +254,36 @@ __device__ __noinline__ void mol_fock(const MolDesc& md, const DevElem* el, cons
         const DevElem& ex = el[ms.z[x]];
         const int ox = ms.off[x], nx = ex.norb;
         const int mu = kPackRow[k], nu = kPackCol[k];
-        double acc = H[(size_t)(ox + mu) * nao + ox + nu];
-        // one-centre
-        for (int la = 0; la < nx; ++la)
-            for (int si = 0; si < nx; ++si) {
-                acc += Pt[(size_t)(ox + la) * nao + ox + si] * one_centre(mu, nu, la, si, ex);
-                acc -= scale * Ps[(size_t)(ox + la) * nao + ox + si] * one_centre(mu, la, nu, si, ex);
-            }
+        double acc = 0.0;
         // two-centre Coulomb from every partner's diagonal density block
 #pragma unroll 4
-        for (int y = 0; y < nat; ++y) {
+        for (int y = gl; y < nat; y += G) {
             if (y == x) continue;
             BlockRef br = block_of(ms, wpool, x, y);
-            const int oy = ms.off[y];
+            const double* py = pdg + y * 10;
             if (ms.z[y] == 1) {
                 double wv = br.x_is_bra ? br.w[k * br.wcols] : br.w[k];
-                acc += Pt[(size_t)oy * nao + oy] * wv;
+                acc += py[0] * wv;
             } else {
 #pragma unroll
                 for (int q = 0; q < 10; ++q) {
-                    int l = kPackRow[q], c = kPackCol[q];
-                    double pv = Pt[(size_t)(oy + l) * nao + oy + c] * (l == c ? 1.0 : 2.0);
                     double wv = br.x_is_bra ? br.w[k * br.wcols + q] : br.w[q * br.wcols + k];
-                    acc += pv * wv;
+                    acc += py[q] * wv;
                 }
             }
         }
-        F[(size_t)(ox + mu) * nao + ox + nu] = acc;
-        F[(size_t)(ox + nu) * nao + ox + mu] = acc;
+        for (int off = 1; off < G; off <<= 1) acc += __shfl_xor(acc, off, 64);
+        if (gl == 0) {
+            acc += H[(size_t)(ox + mu) * nao + ox + nu];
+            // one-centre
+            for (int la = 0; la < nx; ++la)
+                for (int si = 0; si < nx; ++si) {
+                    acc += Pt[(size_t)(ox + la) * nao + ox + si] * one_centre(mu, nu, la, si, ex);
+                    acc -= scale * Ps[(size_t)(ox + la) * nao + ox + si] * one_centre(mu, la, nu, si, ex);
+                }
+            F[(size_t)(ox + mu) * nao + ox + nu] = acc;
+            F[(size_t)(ox + nu) * nao + ox + mu] = acc;
+        }
     }
     // exchange blocks, one pair per thread, grouped by class
     const int64_t n_hh = (int64_t)nh * (nh - 1) / 2, n_hH = (int64_t)nh * nH, n_HH = (int64_t)nH * (nH - 1) / 2;
@@ -276,23 +302,29 @@ __device__ __noinline__ void mol_fock(const MolDesc& md, const DevElem* el, cons
                 acc[u * 4 + v] = 0.0;
             }
         // K[mu][la] = sum_{nu,si} Ps[nu][si] (mu nu | la si): walk the 10 packed bra rows once,
-        // each serving (mu, nu) and, off the diagonal, (nu, mu)
-        for (int kb = 0; kb < 10; ++kb) {
-            const int mu = kPackRow[kb], nu = kPackCol[kb];
-            double row[10];
+        // each serving (mu, nu) and, off the diagonal, (nu, mu). The rows are loaded five at a
+        // time, so each half of the block costs one memory round trip.
 #pragma unroll
-            for (int q = 0; q < 10; ++q) row[q] = w[kb * 10 + q];
+        for (int half = 0; half < 2; ++half) {
+            double rows[50];
 #pragma unroll
-            for (int la = 0; la < 4; ++la) {
-                double x = 0.0, y = 0.0;
+            for (int q = 0; q < 50; ++q) rows[q] = w[half * 50 + q];
 #pragma unroll
-                for (int si = 0; si < 4; ++si) {
-                    const double wv = row[pack4(la, si)];
-                    x += pab[nu * 4 + si] * wv;
-                    y += pab[mu * 4 + si] * wv;
+            for (int r = 0; r < 5; ++r) {
+                const int kb = half * 5 + r;
+                const int mu = kPackRowH[kb], nu = kPackColH[kb];
+#pragma unroll
+                for (int la = 0; la < 4; ++la) {
+                    double x = 0.0, y = 0.0;
+#pragma unroll
+                    for (int si = 0; si < 4; ++si) {
+                        const double wv = rows[r * 10 + pack4(la, si)];
+                        x += pab[nu * 4 + si] * wv;
+                        y += pab[mu * 4 + si] * wv;
+                    }
+                    acc[mu * 4 + la] += x;
+                    if (mu != nu) acc[nu * 4 + la] += y;
                 }
-                acc[mu * 4 + la] += x;
-                if (mu != nu) acc[nu * 4 + la] += y;
             }
         }
 #pragma unroll
@@ -545,7 +577,7 @@ __device__ __noinline__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolS
     int fp32_stall = 0;
     for (int iter = 0; iter < md.max_scf; ++iter) {
         iterations = iter + 1;
-        mol_fock(md, el, ms, wpool, H, P, P, 0.5, F);
+        mol_fock(md, el, ms, wpool, H, P, P, 0.5, F, lds);
         __syncthreads();
         // E = 1/2 sum P (H + F)
         double pe = 0.0;
@@ -778,7 +810,7 @@ __device__ __noinline__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolS
     }
     __syncthreads();
     // final energy from the final density (scf.rs:698-701)
-    mol_fock(md, el, ms, wpool, H, P, P, 0.5, F);
+    mol_fock(md, el, ms, wpool, H, P, P, 0.5, F, lds);
     __syncthreads();
     double pe = 0.0;
     for (size_t idx = t; idx < nn; idx += kWg) pe += P[idx] * (H[idx] + F[idx]);
@@ -880,8 +912,8 @@ __device__ __noinline__ void scf_uhf(const MolDesc& md, const BatchArgs& g, MolS
         iterations = iter + 1;
         for (size_t idx = t; idx < nn; idx += kWg) Pt[idx] = Ps[0][idx] + Ps[1][idx];
         __syncthreads();
-        mol_fock(md, el, ms, wpool, H, Pt, Ps[0], 1.0, Fs[0]);
-        mol_fock(md, el, ms, wpool, H, Pt, Ps[1], 1.0, Fs[1]);
+        mol_fock(md, el, ms, wpool, H, Pt, Ps[0], 1.0, Fs[0], lds);
+        mol_fock(md, el, ms, wpool, H, Pt, Ps[1], 1.0, Fs[1], lds);
         __syncthreads();
         double pe = 0.0;
         for (size_t idx = t; idx < nn; idx += kWg)
@@ -1053,8 +1085,8 @@ __device__ __noinline__ void scf_uhf(const MolDesc& md, const BatchArgs& g, MolS
     __syncthreads();
     for (size_t idx = t; idx < nn; idx += kWg) Pt[idx] = Ps[0][idx] + Ps[1][idx];
     __syncthreads();
-    mol_fock(md, el, ms, wpool, H, Pt, Ps[0], 1.0, Fs[0]);
-    mol_fock(md, el, ms, wpool, H, Pt, Ps[1], 1.0, Fs[1]);
+    mol_fock(md, el, ms, wpool, H, Pt, Ps[0], 1.0, Fs[0], lds);
+    mol_fock(md, el, ms, wpool, H, Pt, Ps[1], 1.0, Fs[1], lds);
     __syncthreads();
     double pe = 0.0;
     for (size_t idx = t; idx < nn; idx += kWg) pe += Pt[idx] * H[idx] + Ps[0][idx] * Fs[0][idx] + Ps[1][idx] * Fs[1][idx];
