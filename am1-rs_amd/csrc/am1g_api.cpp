@@ -368,6 +368,7 @@ int am1g_scf_batch_stream(am1g_ctx* ctx, int n, const am1g_request* rqs, am1g_re
     a.mols = ctx->mols.as<MolDesc>();
     a.n_mols = m;
     a.lds_n = 192;
+    a.tri_max = eigen_tri_max();
     a.z = ctx->z.as<int>();
     a.xyz = ctx->xyz.as<double>();
     a.elems = ctx->elems.as<DevElem>();

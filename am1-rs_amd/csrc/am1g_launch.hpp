@@ -23,6 +23,7 @@ struct BatchArgs {
     const MolDesc* mols;
     int n_mols;
     int lds_n;  // matrices up to this order are diagonalized in LDS, larger ones in global memory
+    int tri_max;  // orders up to this take the tridiagonal method, larger ones Jacobi
     const int* z;
     const double* xyz;
     const DevElem* elems;  // [kMethods][kMaxZ]
@@ -53,6 +54,9 @@ struct BatchArgs {
 };
 
 hipError_t launch_scf_batch(const BatchArgs& a, int precision, int grid, size_t lds_bytes, hipStream_t stream);
+// Largest order for the tridiagonal eigensolver: kTriMax, or 0 when AM1G_EIGEN_JACOBI=1 (every
+// eigen-step then takes the Jacobi solver; for testing that solver on the same molecules).
+int eigen_tri_max();
 hipError_t launch_pairs(const BatchArgs& a, hipStream_t stream);
 void build_element_tables(DevElem* tab);
 
@@ -78,6 +82,7 @@ struct SysArgs {
 struct DcArgs {
     int n_sub;
     int lds_n;
+    int tri_max;             // orders up to this take the tridiagonal method, larger ones Jacobi
     const int* sub_n;        // [n_sub] AO count
     const int64_t* sub_ao;   // [n_sub + 1] offset into ao_list / level arrays
     const int64_t* sub_mat;  // [n_sub] offset of the subsystem's n*n matrices

@@ -48,3 +48,28 @@ def water_chain(n):
             z.append(zz)
             x.append((base + np.array(d)) * ANGSTROM_TO_BOHR)
     return np.array(z, dtype=np.int32), np.array(x)
+
+
+def benzene(cc=1.39, ch=1.09):
+    """D6h benzene (doubly degenerate e1g / e2u orbitals)."""
+    import math
+    lines = []
+    for k in range(6):
+        a = math.pi / 3.0 * k
+        lines.append(f"C {cc * math.cos(a):.10f} {cc * math.sin(a):.10f} 0.0")
+        lines.append(f"H {(cc + ch) * math.cos(a):.10f} {(cc + ch) * math.sin(a):.10f} 0.0")
+    return "\n".join(lines)
+
+
+def cubane(cc=1.57, ch=1.09):
+    """Oh cubane C8H8 (triply degenerate t orbitals)."""
+    import math
+    h = cc / 2.0
+    s = ch / math.sqrt(3.0)
+    lines = []
+    for x in (-1, 1):
+        for y in (-1, 1):
+            for z in (-1, 1):
+                lines.append(f"C {x * h:.10f} {y * h:.10f} {z * h:.10f}")
+                lines.append(f"H {x * (h + s):.10f} {y * (h + s):.10f} {z * (h + s):.10f}")
+    return "\n".join(lines)
