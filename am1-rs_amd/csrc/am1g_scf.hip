@@ -384,10 +384,11 @@ __device__ void mol_density(int nao, int nocc, double w, const double* Ct, doubl
 // Larger orders, and the rare matrix whose close eigenvalues that method cannot separate: A =
 // Ct F Ct^T (warm start in the previous orbitals, or F itself on the first pass), Jacobi, sort
 // ascending. Ct is replaced by the sorted eigenvectors (transposed) and eps by the sorted
-// eigenvalues. Returns the Jacobi sweep count (0 for the tridiagonal method), or -1.
+// eigenvalues. tol_abs: Jacobi's rotation threshold; bracket: half-width around the previous
+// eigenvalues for a warm eigenvalue search (tridiagonal method). Returns the Jacobi sweep count (0 for the tridiagonal method), or -1.
 template <typename T>
 __device__ int mol_eigen(int nao, const double* Fuse, double* Ct, double* T0, double* T1, double* lds, int lds_n,
-                         int tri_max, bool warm, double* eps, WgScratch& s, double tol_abs) {
+                         int tri_max, bool warm, double* eps, WgScratch& s, double tol_abs, double bracket) {
     const int t = threadIdx.x;
     const int n = nao;
     const size_t nn = (size_t)n * n;
@@ -398,7 +399,7 @@ __device__ int mol_eigen(int nao, const double* Fuse, double* Ct, double* T0, do
     const bool v_lds = a_lds && v_rows == n;
     if (n <= tri_max && n <= kTriMax && n <= lds_n && tri_lds_doubles(n) + n <= kLdsDoubles) {
         // direct method (tridiagonalization); it declines only for unresolved close eigenvalues
-        if (wg_eigh_tri(n, Fuse, Ct, eps, T1, lds, kLdsDoubles, s) == 0) return 0;
+        if (wg_eigh_tri(n, Fuse, Ct, eps, T1, lds, kLdsDoubles, s, warm ? bracket : 0.0) == 0) return 0;
     }
     const double* src = Fuse;
     if (warm) {
@@ -771,7 +772,8 @@ __device__ __noinline__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolS
         }
         __syncthreads();
         phase_mark(pc, 2);
-        int sweeps = mol_eigen<double>(nao, Fuse, Ct, T0, T1, lds, g.lds_n, g.tri_max, have_c, eps, s, 1.0e-7 * err);
+        int sweeps = mol_eigen<double>(nao, Fuse, Ct, T0, T1, lds, g.lds_n, g.tri_max, have_c, eps, s, 1.0e-7 * err,
+                                       100.0 * err);
         phase_mark(pc, 3);
         total_sweeps += sweeps;
         have_c = true;
@@ -1043,7 +1045,8 @@ __device__ __noinline__ void scf_uhf(const MolDesc& md, const BatchArgs& g, MolS
         }
         phase_mark(pc, 2);
         for (int c = 0; c < 2; ++c) {
-            int sw = mol_eigen<double>(nao, Fuse[c], Cs[c], T0, T1, lds, g.lds_n, g.tri_max, have_c, eps[c], s, 1.0e-7 * err);
+            int sw = mol_eigen<double>(nao, Fuse[c], Cs[c], T0, T1, lds, g.lds_n, g.tri_max, have_c, eps[c], s,
+                                      1.0e-7 * err, 100.0 * err);
             if (sw < 0) status = kLinearAlgebra;
             else total_sweeps += sw;
         }
@@ -1526,7 +1529,7 @@ __global__ __launch_bounds__(kWg) void k_dc_eigen(DcArgs d, const double* F, int
     }
     __syncthreads();
     const int sw = mol_eigen<double>(n, Fs, Ct, d.t0 + d.sub_mat[sub], d.t1 + d.sub_mat[sub], lds_dyn, d.lds_n, d.tri_max,
-                                     d.warm != 0, d.eps + d.sub_ao[sub], s, d.tol_abs);
+                                     d.warm != 0, d.eps + d.sub_ao[sub], s, d.tol_abs, 0.0);
     const unsigned char* core = d.ao_core + d.sub_ao[sub];
     for (int i = t / 16; i < n; i += kWg / 16) {
         double acc = 0.0;

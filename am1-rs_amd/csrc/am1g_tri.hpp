@@ -234,10 +234,15 @@ __device__ __forceinline__ bool tri_vector(double* __restrict__ z, const double2
 // k belongs to eps[k]) and returns 0. Returns -1, with Ct and eps untouched, when a run of close
 // eigenvalues could not be separated or a non-finite number appeared; the caller then uses
 // another method. Vg: n*n doubles of global scratch (the reflectors). lds: tri_lds_doubles(n)
-// doubles (lds_cap is checked by the caller).
+// doubles (lds_cap is checked by the caller). Eigenvalues are resolved to rounding level (a
+// looser stop moved the SCF trajectory enough to change converged orbital energies by about
+// 1e-6 eV). bracket > 0: eps holds
+// the eigenvalues of a nearby matrix (the previous SCF iteration's), and the search for
+// eigenvalue j starts from [eps_j - bracket, eps_j + bracket] when Sturm counts confirm that it
+// contains exactly that one.
 __device__ __noinline__ int wg_eigh_tri(int n, const double* __restrict__ F, double* __restrict__ Ct,
                                         double* __restrict__ eps_out, double* __restrict__ Vg,
-                                        double* __restrict__ lds, int lds_cap, WgScratch& s) {
+                                        double* __restrict__ lds, int lds_cap, WgScratch& s, double bracket) {
     const int t = threadIdx.x, lane = t & 63;
     const int ld = tri_ld(n), sv = tri_vec_stride(n);
     double* A = lds;               // eigenvectors of T (step 4), reflectors (step 5)
@@ -484,44 +489,98 @@ __device__ __noinline__ int wg_eigh_tri(int n, const double* __restrict__ F, dou
             glo = fmin(glo, __shfl_xor(glo, off, 64));
             ghi = fmax(ghi, __shfl_xor(ghi, off, 64));
         }
+        // blocks: bs[j] = the last i <= j with e[i-1] = 0 (else 0), a running maximum over 64-entry
+        // chunks; be[j] = the first i > j with e[i-1] = 0 (else n), a running minimum from the end
+        int carry = 0;
+        for (int c = 0; c < n; c += 64) {
+            const int j = c + lane;
+            int v = (j > 0 && j < n && e[j - 1] == 0.0) ? j : 0;
+            for (int off = 1; off < 64; off <<= 1) {
+                const int u = __shfl_up(v, off, 64);
+                if (lane >= off) v = max(v, u);
+            }
+            v = max(v, carry);
+            if (j < n) bs[j] = v;
+            carry = __shfl(v, 63, 64);
+        }
+        const int last_start = carry;
+        carry = n;
+        for (int c = (n - 1) & ~63; c >= 0; c -= 64) {
+            const int j = c + lane;
+            int v = (j + 1 < n && e[j] == 0.0) ? j + 1 : n;
+            for (int off = 1; off < 64; off <<= 1) {
+                const int u = __shfl_down(v, off, 64);
+                if (lane + off < 64) v = min(v, u);
+            }
+            v = min(v, carry);
+            if (j < n) be[j] = v;
+            carry = __shfl(v, 0, 64);
+        }
         if (lane == 0) {
             s.bcast[0] = nrm;
             s.bcast[5] = glo;
             s.bcast[6] = ghi;
-        }
-    }
-    __syncthreads();
-    if (t == 0) {
-        int cur = 0;
-        for (int j = 0; j < n; ++j) {
-            if (j > 0 && e[j - 1] == 0.0) cur = j;
-            bs[j] = cur;
-        }
-        cur = n;
-        for (int j = n - 1; j >= 0; --j) {
-            be[j] = cur;
-            if (j > 0 && e[j - 1] == 0.0) cur = j;
+            s.bcast[7] = last_start == 0 ? 1.0 : 0.0;  // T is one unreduced block
         }
     }
     __syncthreads();
     const double nrm = s.bcast[0];
     const double pivmin = 1.0e-280 * fmax(1.0, nrm * nrm);
 
-    // ---- 3. eigenvalues: G threads per eigenvalue (about four wavefronts busy), two points each
-    // per pass, so each pass divides the interval by 2G + 1
+    // ---- 3. eigenvalues: G threads per eigenvalue (up to all eight wavefronts), two points each
+    // per pass, so each pass divides the interval by 2G + 1.
+    // An unreduced T of order n >= 23 starts with one shared pass: each thread counts at two of
+    // 2 kWg + 1 equal divisions of the Gershgorin interval (the counts go to the A area, free
+    // until step 4), and every eigenvalue takes the division that holds it.
+    const bool one_block = s.bcast[7] != 0.0;
+    const bool shared_pass = one_block && n * ld >= kWg;
+    int* cnt = (int*)A;
+    double glo = s.bcast[5], ghi = s.bcast[6];
+    {
+        const double pad = 4.0 * DBL_EPSILON * fmax(fabs(glo), fabs(ghi)) + pivmin;
+        glo -= pad;
+        ghi += pad;
+    }
+    const double sstep = (ghi - glo) / (double)(2 * kWg + 1);
+    if (shared_pass) {
+        int ca, cb;
+        tri_count2(de, 0, n, glo + (double)(2 * t + 1) * sstep, glo + (double)(2 * t + 2) * sstep, ca, cb);
+        cnt[2 * t] = ca;
+        cnt[2 * t + 1] = cb;
+        __syncthreads();
+    }
     {
         int G = 1;
-        while (G < 8 && n * (2 * G) <= 256) G *= 2;
+        while (G < 8 && n * (2 * G) <= kWg) G *= 2;
         const int j = t / G, l = t - j * G;
         if (j < n) {
             const int b0 = bs[j], b1 = be[j], kk = j - b0;
             double lo = d[b0], hi = d[b0];
             if (b1 - b0 > 1) {
-                lo = s.bcast[5];
-                hi = s.bcast[6];
-                const double pad = 4.0 * DBL_EPSILON * fmax(fabs(lo), fabs(hi)) + pivmin;
-                lo -= pad;
-                hi += pad;
+                lo = glo;
+                hi = ghi;
+                if (shared_pass) {
+                    // cnt[p] is the count at glo + (p + 1) sstep; keep cnt[plo] <= kk < cnt[phi]
+                    int plo = -1, phi = 2 * kWg;
+                    while (phi - plo > 1) {
+                        const int mid = (plo + phi) >> 1;
+                        if (cnt[mid] <= kk) plo = mid;
+                        else phi = mid;
+                    }
+                    lo = glo + (double)(plo + 1) * sstep;
+                    if (phi < 2 * kWg) hi = glo + (double)(phi + 1) * sstep;
+                }
+                // a verified bracket around the previous eigenvalue (one block spanning T, so
+                // the block-local index is the global one), when it saves two passes or more
+                if (bracket > 0.0 && one_block && 2.0 * bracket * (double)((2 * G + 1) * (2 * G + 1)) < hi - lo) {
+                    const double wl = eps_out[j] - bracket, wh = eps_out[j] + bracket;
+                    int cl, ch;
+                    tri_count2(de, 0, n, wl, wh, cl, ch);
+                    if (cl <= kk && ch > kk) {
+                        lo = fmax(lo, wl);
+                        hi = fmin(hi, wh);
+                    }
+                }
                 const double atol = DBL_EPSILON * nrm;
                 for (int it = 0; it < 80; ++it) {
                     const double w = hi - lo;
