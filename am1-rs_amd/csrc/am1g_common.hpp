@@ -99,9 +99,9 @@ AM1G_HD inline int64_t w_pool_size(int n_heavy, int n_h) {
 
 AM1G_HD inline int64_t tri_len(int n) { return (int64_t)n * (n + 1) / 2; }
 
-// DIIS history per molecule: depth x (packed F + packed E + packed D) for each spin channel.
+// DIIS history per molecule: depth x (packed F + packed E) for each spin channel.
 AM1G_HD inline int64_t hist_size(int nao, int uhf) {
-    int64_t per = 3 * tri_len(nao);
+    int64_t per = 2 * tri_len(nao);
     return (int64_t)kDiisDepth * per * (uhf ? 2 : 1) + 64 * 4;
 }
 

@@ -32,6 +32,7 @@ struct MolShared {
     int hyd[kMaxAtoms];    // atom indices of hydrogens, ascending
     double B[kDiisDepth * kDiisDepth];   // CDIIS Gram of the error history (logical order)
     double G[kDiisDepth * kDiisDepth];   // A-DIIS Gram <D_i, F_j>
+    double hd[kDiisDepth];               // A-DIIS <D_i, H>
     double coef[kDiisDepth + 1];
     int ring[kDiisDepth];  // logical -> physical history slot
     int nh, nH, hcount, head;
@@ -495,12 +496,12 @@ __device__ __forceinline__ void stri_rowcol(int64_t p, int n, int& i, int& j) {
 }
 
 // ---------------------------------------------------------------------------------------------
-// DIIS bookkeeping for the RHF loop. History slots hold packed F (n(n+1)/2), packed error
-// (strict upper triangle, n(n-1)/2) and packed D (n(n+1)/2).
+// DIIS bookkeeping for the SCF loops. History slots hold packed F (n(n+1)/2) and the packed
+// error (strict upper triangle, n(n-1)/2). The A-DIIS products with past densities come from
+// F and cached scalars instead of stored densities (see the RHF Gram update).
 struct DiisView {
     double* f;
     double* e;
-    double* d;
     int64_t len_s, len_a;
 };
 
@@ -508,10 +509,9 @@ __device__ __forceinline__ DiisView diis_view(double* hist, int nao, int channel
     DiisView v;
     v.len_s = tri_len(nao);
     v.len_a = (int64_t)nao * (nao - 1) / 2;
-    double* base = hist + (int64_t)channel * kDiisDepth * 3 * v.len_s;
+    double* base = hist + (int64_t)channel * kDiisDepth * 2 * v.len_s;
     v.f = base;
     v.e = base + kDiisDepth * v.len_s;
-    v.d = base + 2 * kDiisDepth * v.len_s;
     return v;
 }
 
@@ -601,13 +601,11 @@ __device__ __noinline__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolS
         else slot = ms.head;  // oldest is replaced
         double* hf = dv.f + (int64_t)slot * dv.len_s;
         double* he = dv.e + (int64_t)slot * dv.len_a;
-        double* hd = dv.d + (int64_t)slot * dv.len_s;
 #pragma unroll 4
         for (int64_t pk = t; pk < dv.len_s; pk += kWg) {
             int i, j;
             tri_rowcol(pk, nao, i, j);
             hf[pk] = F[(size_t)i * nao + j];
-            if (need_d) hd[pk] = P[(size_t)i * nao + j];
         }
         double en = 0.0;
 #pragma unroll 4
@@ -626,31 +624,34 @@ __device__ __noinline__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolS
             } else {
                 ms.head = (ms.head + 1) % kDiisDepth;
                 // shift the cached Gram matrices one step (drop the oldest row and column)
-                for (int i = 0; i < kDiisDepth - 1; ++i)
+                for (int i = 0; i < kDiisDepth - 1; ++i) {
                     for (int j = 0; j < kDiisDepth - 1; ++j) {
                         ms.B[i * kDiisDepth + j] = ms.B[(i + 1) * kDiisDepth + j + 1];
                         ms.G[i * kDiisDepth + j] = ms.G[(i + 1) * kDiisDepth + j + 1];
                     }
+                    ms.hd[i] = ms.hd[i + 1];
+                }
             }
             for (int i = 0; i < ms.hcount; ++i) ms.ring[i] = (ms.head + i) % kDiisDepth;
         }
         __syncthreads();
         const int m = ms.hcount;
         const int newest = m - 1;
-        // new Gram entries: <e_new, e_i> for all i; A-DIIS <D_new, F_j>, <D_i, F_new>
+        // new Gram entries: <e_new, e_i> for all i; for A-DIIS T_j = <D_new, F_j> and
+        // h_new = <D_new, H>. The remaining A-DIIS entries follow without a density history:
+        // F_j = H + G(D_j) with G linear and symmetric (<A, G(B)> = <B, G(A)>), so
+        // <D_j, F_new> = h_j + T_j - h_new.
         {
-            double part[3 * kDiisDepth];
+            double part[2 * kDiisDepth + 1];
 #pragma unroll
-            for (int q = 0; q < 3 * kDiisDepth; ++q) part[q] = 0.0;
+            for (int q = 0; q < 2 * kDiisDepth + 1; ++q) part[q] = 0.0;
             const double* ep[kDiisDepth];
             const double* fp[kDiisDepth];
-            const double* dq[kDiisDepth];
 #pragma unroll
             for (int i = 0; i < kDiisDepth; ++i) {
                 const int64_t r = ms.ring[i < m ? i : newest];
                 ep[i] = dv.e + r * dv.len_a;
                 fp[i] = dv.f + r * dv.len_s;
-                dq[i] = dv.d + r * dv.len_s;
             }
             const double* en_p = ep[newest];
 #pragma unroll 2
@@ -661,43 +662,36 @@ __device__ __noinline__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolS
                     if (i < m) part[i] = fma(x, ep[i][pk], part[i]);
             }
             if (need_d) {
-                // <D, F> over the matrix: every packed entry stands for two matrix entries except
-                // the diagonal ones, so sum all packed entries, take half the diagonal off, double
-                const double* dn = dq[newest];
-                const double* fn = fp[newest];
+                // <D, X> over the matrix from the packed upper triangle: off-diagonal entries
+                // count twice, so the diagonal ones enter at half weight and the sum is doubled
 #pragma unroll 2
                 for (int64_t pk = t; pk < dv.len_s; pk += kWg) {
-                    const double dnv = dn[pk], fnv = fn[pk];
+                    int i, j;
+                    tri_rowcol(pk, nao, i, j);
+                    const size_t ij = (size_t)i * nao + j;
+                    const double dnv = i == j ? 0.5 * P[ij] : P[ij];
+                    part[2 * kDiisDepth] = fma(dnv, H[ij], part[2 * kDiisDepth]);
 #pragma unroll
-                    for (int j = 0; j < kDiisDepth; ++j) {
-                        if (j >= m) continue;
-                        part[kDiisDepth + j] = fma(dnv, fp[j][pk], part[kDiisDepth + j]);
-                        part[2 * kDiisDepth + j] = fma(fnv, dq[j][pk], part[2 * kDiisDepth + j]);
-                    }
-                }
-                for (int i = t; i < nao; i += kWg) {
-                    const int64_t pk = tri_pos(i, i, nao);
-                    const double dnv = -0.5 * dn[pk], fnv = -0.5 * fn[pk];
-#pragma unroll
-                    for (int j = 0; j < kDiisDepth; ++j) {
-                        if (j >= m) continue;
-                        part[kDiisDepth + j] = fma(dnv, fp[j][pk], part[kDiisDepth + j]);
-                        part[2 * kDiisDepth + j] = fma(fnv, dq[j][pk], part[2 * kDiisDepth + j]);
-                    }
+                    for (int q = 0; q < kDiisDepth; ++q)
+                        if (q < m) part[kDiisDepth + q] = fma(dnv, fp[q][pk], part[kDiisDepth + q]);
                 }
             }
-            double tot[3 * kDiisDepth];
-            wg_sum_many(part, 3 * kDiisDepth, lds, tot);  // LDS is free outside the eigensolver
+            double tot[2 * kDiisDepth + 1];
+            wg_sum_many(part, lds, tot);  // LDS is free outside the eigensolver
             if (t == 0) {
                 for (int i = 0; i < m; ++i) {
                     ms.B[newest * kDiisDepth + i] = 2.0 * tot[i];
                     ms.B[i * kDiisDepth + newest] = 2.0 * tot[i];
                 }
-                if (need_d)
+                if (need_d) {
+                    const double hn = 2.0 * tot[2 * kDiisDepth];
+                    ms.hd[newest] = hn;
                     for (int j = 0; j < m; ++j) {
-                        ms.G[newest * kDiisDepth + j] = 2.0 * tot[kDiisDepth + j];      // <D_new, F_j>
-                        ms.G[j * kDiisDepth + newest] = 2.0 * tot[2 * kDiisDepth + j];  // <D_j, F_new>
+                        const double tj = 2.0 * tot[kDiisDepth + j];
+                        ms.G[newest * kDiisDepth + j] = tj;                                      // <D_new, F_j>
+                        ms.G[j * kDiisDepth + newest] = j == newest ? tj : ms.hd[j] + tj - hn;  // <D_j, F_new>
                     }
+                }
             }
             __syncthreads();
         }
@@ -739,9 +733,9 @@ __device__ __noinline__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolS
             ms.flags[0] = 0;
         }
         __syncthreads();
-        const double* Fuse = F;
         if (ms.flags[0]) {
-            // T1 = sum_i c_i F_i (unpacked); refuse a non-finite combination
+            // F <- sum_i c_i F_i (unpacked) in place: F itself is history slot `newest` and is
+            // rebuilt next pass; a non-finite combination puts that slot back
             double bad = 0.0;
             const double* fq[kDiisDepth];
             double cq[kDiisDepth];
@@ -758,18 +752,22 @@ __device__ __noinline__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolS
 #pragma unroll
                 for (int q = 0; q < kDiisDepth; ++q)
                     if (q < m) acc = fma(cq[q], fq[q][pk], acc);
-                T1[(size_t)i * nao + j] = acc;
-                T1[(size_t)j * nao + i] = acc;
+                F[(size_t)i * nao + j] = acc;
+                F[(size_t)j * nao + i] = acc;
                 if (!isfinite(acc)) bad = 1.0;
             }
             bad = wg_max(bad, s);
-            if (bad == 0.0) Fuse = T1;
+            if (bad != 0.0) {
+                const double* fn = fq[m - 1];
+                for (int64_t pk = t; pk < dv.len_s; pk += kWg) {
+                    int i, j;
+                    tri_rowcol(pk, nao, i, j);
+                    F[(size_t)i * nao + j] = fn[pk];
+                    F[(size_t)j * nao + i] = fn[pk];
+                }
+            }
         }
-        // eigen-step; the warm start needs T0/T1 as scratch, so move F_use out of T1 first
-        if (Fuse == T1) {
-            for (size_t idx = t; idx < nn; idx += kWg) F[idx] = T1[idx];  // F is rebuilt next pass
-            Fuse = F;
-        }
+        const double* Fuse = F;
         __syncthreads();
         phase_mark(pc, 2);
         int sweeps = mol_eigen<double>(nao, Fuse, Ct, T0, T1, lds, g.lds_n, g.tri_max, have_c, eps, s, 1.0e-7 * err,
@@ -984,7 +982,7 @@ __device__ __noinline__ void scf_uhf(const MolDesc& md, const BatchArgs& g, MolS
                 }
             }
             double tot[kDiisDepth];
-            wg_sum_many(part, kDiisDepth, lds, tot);
+            wg_sum_many(part, lds, tot);
             if (t == 0) {
                 for (int i = 0; i < m; ++i) {
                     ms.B[newest * kDiisDepth + i] = 2.0 * tot[i];
@@ -1011,8 +1009,8 @@ __device__ __noinline__ void scf_uhf(const MolDesc& md, const BatchArgs& g, MolS
             }
             __syncthreads();
             if (ms.flags[0]) {
-                // combined Focks into T0 (alpha) and T1 (beta)
-                double* outs[2] = {T0, T1};
+                // combined Focks in place of F (each channel's own F is its newest history slot
+                // and is rebuilt next pass)
                 double cq[kDiisDepth];
 #pragma unroll
                 for (int q = 0; q < kDiisDepth; ++q) cq[q] = q < m ? ms.coef[q] : 0.0;
@@ -1028,14 +1026,10 @@ __device__ __noinline__ void scf_uhf(const MolDesc& md, const BatchArgs& g, MolS
 #pragma unroll
                         for (int q = 0; q < kDiisDepth; ++q)
                             if (q < m) acc = fma(cq[q], fq[q][pk], acc);
-                        outs[c][(size_t)i * nao + j] = acc;
-                        outs[c][(size_t)j * nao + i] = acc;
+                        Fs[c][(size_t)i * nao + j] = acc;
+                        Fs[c][(size_t)j * nao + i] = acc;
                     }
                 }
-                __syncthreads();
-                // the eigen-step needs T0/T1 as scratch: the extrapolated Focks replace F
-                for (int c = 0; c < 2; ++c)
-                    for (size_t idx = t; idx < nn; idx += kWg) Fs[c][idx] = (c == 0 ? T0 : T1)[idx];
                 __syncthreads();
             }
         }

@@ -74,17 +74,19 @@ __device__ __forceinline__ double wg_max(double v, WgScratch& s) {
     return tot;
 }
 
-// Several independent sums at once, reduced through `buf` (kWaves * nv doubles of shared
-// scratch). Every thread receives all nv totals in out[0..nv).
-__device__ void wg_sum_many(const double* v, int nv, double* buf, double* out) {
+// Several independent sums at once, reduced through `buf` (kWaves * NV doubles of shared
+// scratch). Every thread receives all NV totals in out.
+template <int NV>
+__device__ __forceinline__ void wg_sum_many(const double (&v)[NV], double* buf, double (&out)[NV]) {
     const int t = threadIdx.x;
-    for (int i = 0; i < nv; ++i) {
-        double x = v[i];
-        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        const double x = wave_sum(v[i]);
         if ((t & 63) == 0) buf[i * kWaves + (t >> 6)] = x;
     }
     __syncthreads();
-    for (int i = 0; i < nv; ++i) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
         double tot = 0.0;
 #pragma unroll
         for (int w = 0; w < kWaves; ++w) tot += buf[i * kWaves + w];
