@@ -581,10 +581,6 @@ __device__ __noinline__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolS
         iterations = iter + 1;
         mol_fock(md, el, ms, wpool, H, P, P, 0.5, F, lds);
         __syncthreads();
-        // E = 1/2 sum P (H + F)
-        double pe = 0.0;
-        for (size_t idx = t; idx < nn; idx += kWg) pe += P[idx] * (H[idx] + F[idx]);
-        double e_elec = 0.5 * wg_sum(pe, s);
         phase_mark(pc, 1);
         // error = FP - (FP)^T; F and P are symmetric so FP = F^T P
         const bool low = fp32_phase;
@@ -595,34 +591,72 @@ __device__ __noinline__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolS
             wg_gemm_any<double, true, false>(nao, nao, nao, 1.0, F, nao, P, nao, T0, nao, lds, kLdsDoubles);
         }
         __syncthreads();
-        // push history: logical slot = count (or overwrite oldest)
-        int slot;
-        if (ms.hcount < kDiisDepth) slot = (ms.head + ms.hcount) % kDiisDepth;
-        else slot = ms.head;  // oldest is replaced
-        double* hf = dv.f + (int64_t)slot * dv.len_s;
-        double* he = dv.e + (int64_t)slot * dv.len_a;
-#pragma unroll 4
-        for (int64_t pk = t; pk < dv.len_s; pk += kWg) {
-            int i, j;
-            tri_rowcol(pk, nao, i, j);
-            hf[pk] = F[(size_t)i * nao + j];
+        // History ring after this push (every thread derives it; the shared copy follows the
+        // sums): the new entry takes logical position m - 1, replacing the oldest when full.
+        const bool full = ms.hcount == kDiisDepth;
+        const int head = full ? (ms.head + 1) % kDiisDepth : ms.head;
+        const int m = full ? kDiisDepth : ms.hcount + 1;
+        const int newest = m - 1;
+        const double* ep[kDiisDepth];
+        const double* fp[kDiisDepth];
+#pragma unroll
+        for (int i = 0; i < kDiisDepth; ++i) {
+            const int64_t r = (head + (i < m ? i : newest)) % kDiisDepth;
+            ep[i] = dv.e + r * dv.len_a;
+            fp[i] = dv.f + r * dv.len_s;
         }
-        double en = 0.0;
-#pragma unroll 4
+        double* he = dv.e + (int64_t)((head + newest) % kDiisDepth) * dv.len_a;
+        double* hf = dv.f + (int64_t)((head + newest) % kDiisDepth) * dv.len_s;
+        // Two passes over the packed triangles push the new entry and form, in one reduction:
+        // part[q] = <e_new, e_q> (CDIIS Gram row; q = newest gives ||e||^2), and for A-DIIS
+        // part[8 + q] = <D_new, F_q>, part[16] = <D_new, H>; part[17] = E_elec = 1/2 sum P (H + F).
+        // A matrix product <X, Y> over the packed upper triangle counts off-diagonal entries
+        // twice, so diagonal ones enter at half weight and the sum is doubled. The remaining
+        // A-DIIS entries need no density history: F_j = H + G(D_j) with G linear and symmetric
+        // (<A, G(B)> = <B, G(A)>), so <D_j, F_new> = h_j + <D_new, F_j> - h_new.
+        double part[2 * kDiisDepth + 2];
+#pragma unroll
+        for (int q = 0; q < 2 * kDiisDepth + 2; ++q) part[q] = 0.0;
+#pragma unroll 2
         for (int64_t pk = t; pk < dv.len_a; pk += kWg) {
             int i, j;
             stri_rowcol(pk, nao, i, j);
             const double e = T0[(size_t)i * nao + j] - T0[(size_t)j * nao + i];
             he[pk] = e;
-            en += e * e;
+#pragma unroll
+            for (int q = 0; q < kDiisDepth; ++q) {
+                if (q < newest) part[q] = fma(e, ep[q][pk], part[q]);
+                else if (q == newest) part[q] = fma(e, e, part[q]);
+            }
         }
-        double err = sqrt(2.0 * wg_sum(en, s));
-        __syncthreads();
+#pragma unroll 2
+        for (int64_t pk = t; pk < dv.len_s; pk += kWg) {
+            int i, j;
+            tri_rowcol(pk, nao, i, j);
+            const size_t ij = (size_t)i * nao + j;
+            const double f = F[ij], h = H[ij];
+            const double pw = i == j ? 0.5 * P[ij] : P[ij];
+            hf[pk] = f;
+            part[2 * kDiisDepth + 1] = fma(pw, h + f, part[2 * kDiisDepth + 1]);
+            if (need_d) {
+                part[2 * kDiisDepth] = fma(pw, h, part[2 * kDiisDepth]);
+#pragma unroll
+                for (int q = 0; q < kDiisDepth; ++q) {
+                    if (q < newest) part[kDiisDepth + q] = fma(pw, fp[q][pk], part[kDiisDepth + q]);
+                    else if (q == newest) part[kDiisDepth + q] = fma(pw, f, part[kDiisDepth + q]);
+                }
+            }
+        }
+        double tot[2 * kDiisDepth + 2];
+        wg_sum_many(part, lds, tot);  // LDS is free outside the eigensolver
+        double bnn = 0.0;
+#pragma unroll
+        for (int q = 0; q < kDiisDepth; ++q)
+            if (q == newest) bnn = tot[q];
+        const double err = sqrt(2.0 * bnn);
+        const double e_elec = tot[2 * kDiisDepth + 1];
         if (t == 0) {
-            if (ms.hcount < kDiisDepth) {
-                ms.hcount++;
-            } else {
-                ms.head = (ms.head + 1) % kDiisDepth;
+            if (full) {
                 // shift the cached Gram matrices one step (drop the oldest row and column)
                 for (int i = 0; i < kDiisDepth - 1; ++i) {
                     for (int j = 0; j < kDiisDepth - 1; ++j) {
@@ -632,69 +666,24 @@ __device__ __noinline__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolS
                     ms.hd[i] = ms.hd[i + 1];
                 }
             }
-            for (int i = 0; i < ms.hcount; ++i) ms.ring[i] = (ms.head + i) % kDiisDepth;
-        }
-        __syncthreads();
-        const int m = ms.hcount;
-        const int newest = m - 1;
-        // new Gram entries: <e_new, e_i> for all i; for A-DIIS T_j = <D_new, F_j> and
-        // h_new = <D_new, H>. The remaining A-DIIS entries follow without a density history:
-        // F_j = H + G(D_j) with G linear and symmetric (<A, G(B)> = <B, G(A)>), so
-        // <D_j, F_new> = h_j + T_j - h_new.
-        {
-            double part[2 * kDiisDepth + 1];
-#pragma unroll
-            for (int q = 0; q < 2 * kDiisDepth + 1; ++q) part[q] = 0.0;
-            const double* ep[kDiisDepth];
-            const double* fp[kDiisDepth];
-#pragma unroll
-            for (int i = 0; i < kDiisDepth; ++i) {
-                const int64_t r = ms.ring[i < m ? i : newest];
-                ep[i] = dv.e + r * dv.len_a;
-                fp[i] = dv.f + r * dv.len_s;
-            }
-            const double* en_p = ep[newest];
-#pragma unroll 2
-            for (int64_t pk = t; pk < dv.len_a; pk += kWg) {
-                const double x = en_p[pk];
-#pragma unroll
-                for (int i = 0; i < kDiisDepth; ++i)
-                    if (i < m) part[i] = fma(x, ep[i][pk], part[i]);
+            ms.hcount = m;
+            ms.head = head;
+            for (int i = 0; i < m; ++i) ms.ring[i] = (head + i) % kDiisDepth;
+            for (int i = 0; i < m; ++i) {
+                ms.B[newest * kDiisDepth + i] = 2.0 * tot[i];
+                ms.B[i * kDiisDepth + newest] = 2.0 * tot[i];
             }
             if (need_d) {
-                // <D, X> over the matrix from the packed upper triangle: off-diagonal entries
-                // count twice, so the diagonal ones enter at half weight and the sum is doubled
-#pragma unroll 2
-                for (int64_t pk = t; pk < dv.len_s; pk += kWg) {
-                    int i, j;
-                    tri_rowcol(pk, nao, i, j);
-                    const size_t ij = (size_t)i * nao + j;
-                    const double dnv = i == j ? 0.5 * P[ij] : P[ij];
-                    part[2 * kDiisDepth] = fma(dnv, H[ij], part[2 * kDiisDepth]);
-#pragma unroll
-                    for (int q = 0; q < kDiisDepth; ++q)
-                        if (q < m) part[kDiisDepth + q] = fma(dnv, fp[q][pk], part[kDiisDepth + q]);
+                const double hn = 2.0 * tot[2 * kDiisDepth];
+                ms.hd[newest] = hn;
+                for (int j = 0; j < m; ++j) {
+                    const double tj = 2.0 * tot[kDiisDepth + j];
+                    ms.G[newest * kDiisDepth + j] = tj;                                      // <D_new, F_j>
+                    ms.G[j * kDiisDepth + newest] = j == newest ? tj : ms.hd[j] + tj - hn;  // <D_j, F_new>
                 }
             }
-            double tot[2 * kDiisDepth + 1];
-            wg_sum_many(part, lds, tot);  // LDS is free outside the eigensolver
-            if (t == 0) {
-                for (int i = 0; i < m; ++i) {
-                    ms.B[newest * kDiisDepth + i] = 2.0 * tot[i];
-                    ms.B[i * kDiisDepth + newest] = 2.0 * tot[i];
-                }
-                if (need_d) {
-                    const double hn = 2.0 * tot[2 * kDiisDepth];
-                    ms.hd[newest] = hn;
-                    for (int j = 0; j < m; ++j) {
-                        const double tj = 2.0 * tot[kDiisDepth + j];
-                        ms.G[newest * kDiisDepth + j] = tj;                                      // <D_new, F_j>
-                        ms.G[j * kDiisDepth + newest] = j == newest ? tj : ms.hd[j] + tj - hn;  // <D_j, F_new>
-                    }
-                }
-            }
-            __syncthreads();
         }
+        __syncthreads();
         if (!isfinite(e_elec) || !isfinite(err)) {
             status = kScfDiverged;
             break;
