@@ -363,42 +363,45 @@ __device__ __noinline__ int wg_eigh_tri(int n, const double* __restrict__ F, dou
         const double vr0 = vo[r0 < n ? r0 : 0], wr0 = fma(-Ko, vr0, po[r0 < n ? r0 : 0]);
         const double vr1 = vo[r1 < n ? r1 : 0], wr1 = fma(-Ko, vr1, po[r1 < n ? r1 : 0]);
         double s0 = 0.0, s1 = 0.0;
-        // columns 2 q8 + 16 j: the pairs are 16-byte aligned; the second half of the columns only
-        // when n > 64, so each half's loads are issued together
-        auto fused = [&](int j) {
-            const int c = 2 * q8 + 16 * j;
-            const double2 p2 = *(const double2*)(po + c);
-            const double2 u2 = *(const double2*)(vn + c);
-            const double w0 = fma(-Ko, vown[2 * j], p2.x), w1 = fma(-Ko, vown[2 * j + 1], p2.y);
-            const double a00 = fma(-vr0, w0, fma(-wr0, vown[2 * j], zr[0][2 * j]));
-            const double a01 = fma(-vr0, w1, fma(-wr0, vown[2 * j + 1], zr[0][2 * j + 1]));
-            const double a10 = fma(-vr1, w0, fma(-wr1, vown[2 * j], zr[1][2 * j]));
-            const double a11 = fma(-vr1, w1, fma(-wr1, vown[2 * j + 1], zr[1][2 * j + 1]));
-            if (on0) {
+        // Rows and columns below k are finished: there the previous v and p vanish, so the update
+        // leaves them unchanged and the new reflector adds nothing to the product. A wavefront
+        // whose rows are all <= k skips the work (row k is not read again), and so does every
+        // quarter of the columns that lies below k (both tests are uniform across a wavefront).
+        if (16 * (t >> 6) + 15 > k && 16 * (t >> 6) < n) {
+            // columns 2 q8 + 16 j: the pairs are 16-byte aligned; each quarter's loads are
+            // issued together
+            auto fused = [&](int j) {
+                const int c = 2 * q8 + 16 * j;
+                const double2 p2 = *(const double2*)(po + c);
+                const double2 u2 = *(const double2*)(vn + c);
+                const double w0 = fma(-Ko, vown[2 * j], p2.x), w1 = fma(-Ko, vown[2 * j + 1], p2.y);
+                const double a00 = fma(-vr0, w0, fma(-wr0, vown[2 * j], zr[0][2 * j]));
+                const double a01 = fma(-vr0, w1, fma(-wr0, vown[2 * j + 1], zr[0][2 * j + 1]));
+                const double a10 = fma(-vr1, w0, fma(-wr1, vown[2 * j], zr[1][2 * j]));
+                const double a11 = fma(-vr1, w1, fma(-wr1, vown[2 * j + 1], zr[1][2 * j + 1]));
                 zr[0][2 * j] = a00;
                 zr[0][2 * j + 1] = a01;
-            }
-            if (on1) {
                 zr[1][2 * j] = a10;
                 zr[1][2 * j + 1] = a11;
+                s0 = fma(a00, u2.x, fma(a01, u2.y, s0));
+                s1 = fma(a10, u2.x, fma(a11, u2.y, s1));
+                vown[2 * j] = u2.x;  // this reflector is the next step's previous one
+                vown[2 * j + 1] = u2.y;
+            };
+#pragma unroll
+            for (int jq = 0; jq < 4; ++jq) {
+                if (32 * jq + 32 > k && 32 * jq < n) {
+                    fused(2 * jq);
+                    fused(2 * jq + 1);
+                }
             }
-            s0 = fma(a00, u2.x, fma(a01, u2.y, s0));
-            s1 = fma(a10, u2.x, fma(a11, u2.y, s1));
-            vown[2 * j] = u2.x;  // this reflector is the next step's previous one
-            vown[2 * j + 1] = u2.y;
-        };
-#pragma unroll
-        for (int j = 0; j < 4; ++j) fused(j);
-        if (n > 64) {
-#pragma unroll
-            for (int j = 4; j < 8; ++j) fused(j);
+            s0 += dpp_move<0xB1>(s0);  // sums over the group of eight
+            s1 += dpp_move<0xB1>(s1);
+            s0 += dpp_move<0x4E>(s0);
+            s1 += dpp_move<0x4E>(s1);
+            s0 += dpp_move<0x141>(s0);
+            s1 += dpp_move<0x141>(s1);
         }
-        s0 += dpp_move<0xB1>(s0);  // sums over the group of eight
-        s1 += dpp_move<0xB1>(s1);
-        s0 += dpp_move<0x4E>(s0);
-        s1 += dpp_move<0x4E>(s1);
-        s0 += dpp_move<0x141>(s0);
-        s1 += dpp_move<0x141>(s1);
         if (q8 == 0) {
             if (on0) pn[r0] = tk * s0;
             if (on1) pn[r1] = tk * s1;
