@@ -617,7 +617,7 @@ __device__ __noinline__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolS
         double part[2 * kDiisDepth + 2];
 #pragma unroll
         for (int q = 0; q < 2 * kDiisDepth + 2; ++q) part[q] = 0.0;
-#pragma unroll 2
+#pragma unroll 4
         for (int64_t pk = t; pk < dv.len_a; pk += kWg) {
             int i, j;
             stri_rowcol(pk, nao, i, j);
@@ -629,7 +629,7 @@ __device__ __noinline__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolS
                 else if (q == newest) part[q] = fma(e, e, part[q]);
             }
         }
-#pragma unroll 2
+#pragma unroll 4
         for (int64_t pk = t; pk < dv.len_s; pk += kWg) {
             int i, j;
             tri_rowcol(pk, nao, i, j);
@@ -647,6 +647,17 @@ __device__ __noinline__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolS
                 }
             }
         }
+        // The cached Gram entries move one step when the oldest entry is dropped: wavefront 0
+        // reads them here, before the reduction's barriers, and writes them after, lane l
+        // moving entry (l / 8, l % 8) from (l / 8 + 1, l % 8 + 1). Only stores follow the
+        // barriers, so no lane reads an entry another lane rewrites.
+        const int gi = (t & 63) >> 3, gj = t & 7;
+        const bool gmove = full && t < 64 && gi < kDiisDepth - 1 && gj < kDiisDepth - 1;
+        const double bsh = gmove ? ms.B[(gi + 1) * kDiisDepth + gj + 1] : 0.0;
+        const double gsh = gmove ? ms.G[(gi + 1) * kDiisDepth + gj + 1] : 0.0;
+        double hsh[kDiisDepth];  // <D_j, H> in the new logical order
+#pragma unroll
+        for (int j = 0; j < kDiisDepth; ++j) hsh[j] = full ? (j + 1 < kDiisDepth ? ms.hd[j + 1] : 0.0) : ms.hd[j];
         double tot[2 * kDiisDepth + 2];
         wg_sum_many(part, lds, tot);  // LDS is free outside the eigensolver
         double bnn = 0.0;
@@ -655,31 +666,25 @@ __device__ __noinline__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolS
             if (q == newest) bnn = tot[q];
         const double err = sqrt(2.0 * bnn);
         const double e_elec = tot[2 * kDiisDepth + 1];
+        if (gmove) {
+            ms.B[gi * kDiisDepth + gj] = bsh;
+            ms.G[gi * kDiisDepth + gj] = gsh;
+        }
         if (t == 0) {
-            if (full) {
-                // shift the cached Gram matrices one step (drop the oldest row and column)
-                for (int i = 0; i < kDiisDepth - 1; ++i) {
-                    for (int j = 0; j < kDiisDepth - 1; ++j) {
-                        ms.B[i * kDiisDepth + j] = ms.B[(i + 1) * kDiisDepth + j + 1];
-                        ms.G[i * kDiisDepth + j] = ms.G[(i + 1) * kDiisDepth + j + 1];
-                    }
-                    ms.hd[i] = ms.hd[i + 1];
-                }
-            }
             ms.hcount = m;
             ms.head = head;
-            for (int i = 0; i < m; ++i) ms.ring[i] = (head + i) % kDiisDepth;
-            for (int i = 0; i < m; ++i) {
+            const double hn = 2.0 * tot[2 * kDiisDepth];
+#pragma unroll
+            for (int i = 0; i < kDiisDepth; ++i) {
+                if (i >= m) continue;
+                ms.ring[i] = (head + i) % kDiisDepth;
                 ms.B[newest * kDiisDepth + i] = 2.0 * tot[i];
                 ms.B[i * kDiisDepth + newest] = 2.0 * tot[i];
-            }
-            if (need_d) {
-                const double hn = 2.0 * tot[2 * kDiisDepth];
-                ms.hd[newest] = hn;
-                for (int j = 0; j < m; ++j) {
-                    const double tj = 2.0 * tot[kDiisDepth + j];
-                    ms.G[newest * kDiisDepth + j] = tj;                                      // <D_new, F_j>
-                    ms.G[j * kDiisDepth + newest] = j == newest ? tj : ms.hd[j] + tj - hn;  // <D_j, F_new>
+                if (need_d) {
+                    const double ti = 2.0 * tot[kDiisDepth + i];
+                    ms.hd[i] = i == newest ? hn : hsh[i];
+                    ms.G[newest * kDiisDepth + i] = ti;                                      // <D_new, F_i>
+                    ms.G[i * kDiisDepth + newest] = i == newest ? ti : hsh[i] + ti - hn;  // <D_i, F_new>
                 }
             }
         }
@@ -691,29 +696,9 @@ __device__ __noinline__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolS
         // extrapolation coefficients, scf.rs:627-653: CDIIS by thread 0, A-DIIS on wavefront 0
         const bool cdiis = accel == 1 || (accel == 2 && err <= md.adiis_switch);
         if (cdiis) {
-            if (t == 0) {
-                int use = 0;
-                if (m >= 2) {
-                    double Bm[(kDiisDepth + 1) * (kDiisDepth + 1)], rhs[kDiisDepth + 1], x[kDiisDepth + 1];
-                    int dim = m + 1;
-                    for (int i = 0; i < m; ++i) {
-                        for (int j = 0; j < m; ++j) Bm[i * dim + j] = ms.B[i * kDiisDepth + j];
-                        Bm[i * dim + m] = -1.0;
-                        Bm[m * dim + i] = -1.0;
-                        rhs[i] = 0.0;
-                    }
-                    Bm[m * dim + m] = 0.0;
-                    rhs[m] = -1.0;
-                    if (solve_bordered(Bm, rhs, dim, x)) {
-                        double wsum = 0.0;
-                        for (int i = 0; i < m; ++i) wsum += fabs(x[i]);
-                        if (isfinite(wsum)) {
-                            use = 1;
-                            for (int i = 0; i < m; ++i) ms.coef[i] = x[i];
-                        }
-                    }
-                }
-                ms.flags[0] = use;
+            if (t < 64) {
+                const bool use = m >= 2 && cdiis_solve_wave(ms.B, m, ms.coef);
+                if (t == 0) ms.flags[0] = use ? 1 : 0;
             }
         } else if (accel == 2 && m >= 2) {
             if (t < 64) adiis_solve_wave(ms.G, m, ms.coef);
@@ -733,7 +718,7 @@ __device__ __noinline__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolS
                 fq[q] = dv.f + (int64_t)ms.ring[q < m ? q : m - 1] * dv.len_s;
                 cq[q] = q < m ? ms.coef[q] : 0.0;
             }
-#pragma unroll 2
+#pragma unroll 4
             for (int64_t pk = t; pk < dv.len_s; pk += kWg) {
                 int i, j;
                 tri_rowcol(pk, nao, i, j);

@@ -530,8 +530,9 @@ __device__ __noinline__ int wg_eigh_tri(int n, const double* __restrict__ F, dou
     const double nrm = s.bcast[0];
     const double pivmin = 1.0e-280 * fmax(1.0, nrm * nrm);
 
-    // ---- 3. eigenvalues: G threads per eigenvalue (up to all eight wavefronts), two points each
-    // per pass, so each pass divides the interval by 2G + 1.
+    // ---- 3. eigenvalues: G threads per eigenvalue, two points each per pass, so each pass
+    // divides the interval by 2G + 1. Up to four wavefronts (one per SIMD): the counts are
+    // issue-bound, and more points per pass cost more issue than the passes they save.
     // An unreduced T of order n >= 23 starts with one shared pass: each thread counts at two of
     // 2 kWg + 1 equal divisions of the Gershgorin interval (the counts go to the A area, free
     // until step 4), and every eigenvalue takes the division that holds it.
@@ -554,7 +555,7 @@ __device__ __noinline__ int wg_eigh_tri(int n, const double* __restrict__ F, dou
     }
     {
         int G = 1;
-        while (G < 8 && n * (2 * G) <= kWg) G *= 2;
+        while (G < 8 && n * (2 * G) <= kWg / 2) G *= 2;
         const int j = t / G, l = t - j * G;
         if (j < n) {
             const int b0 = bs[j], b1 = be[j], kk = j - b0;

@@ -749,6 +749,87 @@ __host__ __device__ inline void adiis_solve(const double* g, int m, double* c) {
     }
 }
 
+// CDIIS coefficients on one wavefront: the bordered system [B -1; -1 0] [c; lambda] = [0; -1] of
+// solve_bordered (Gaussian elimination with partial pivoting, the first largest pivot winning,
+// then back substitution) with lane r holding row r, so each column's eliminations run in
+// parallel and every lane takes the same pivot decisions. B: logical Gram matrix (row stride
+// kDiisDepth), m >= 2 entries. Returns false when a pivot falls below 1e-12 or the coefficients
+// are not finite; otherwise lane 0 writes c_out[0..m). Called by all 64 lanes of one wavefront.
+__device__ inline bool cdiis_solve_wave(const double* B, int m, double* c_out) {
+    constexpr int kD = kDiisDepth + 1;
+    const int lane = threadIdx.x & 63;
+    const int dim = m + 1;
+    double a[kD];
+#pragma unroll
+    for (int j = 0; j < kD; ++j) {
+        double v = 0.0;
+        if (lane < m && j < m) v = B[lane * kDiisDepth + j];
+        else if ((lane < m && j == m) || (lane == m && j < m)) v = -1.0;
+        a[j] = v;
+    }
+    double rhs = lane == m ? -1.0 : 0.0;
+    bool ok = true;  // (no early exits: the loops must unroll so that a[] stays in registers)
+#pragma unroll
+    for (int col = 0; col < kD; ++col) {
+        if (col >= dim || !ok) continue;
+        double best = -1.0;
+        int piv = col;
+        for (int row = col; row < dim; ++row) {
+            const double v = fabs(read_lane(a[col], row));
+            if (v > best) {
+                best = v;
+                piv = row;
+            }
+        }
+        if (best < 1.0e-12) {
+            ok = false;
+            continue;
+        }
+        if (piv != col) {
+            const int src = lane == col ? piv : (lane == piv ? col : lane);
+#pragma unroll
+            for (int j = col; j < kD; ++j) a[j] = __shfl(a[j], src, 64);
+            rhs = __shfl(rhs, src, 64);
+        }
+        double prow[kD];
+#pragma unroll
+        for (int j = col; j < kD; ++j) prow[j] = read_lane(a[j], col);
+        const double prhs = read_lane(rhs, col);
+        if (lane > col && lane < dim) {
+            const double factor = a[col] / prow[col];
+            if (factor != 0.0) {
+#pragma unroll
+                for (int j = col; j < kD; ++j) a[j] -= factor * prow[j];
+                rhs -= factor * prhs;
+            }
+        }
+    }
+    double x[kD];
+#pragma unroll
+    for (int j = 0; j < kD; ++j) x[j] = 0.0;
+#pragma unroll
+    for (int col = kD - 1; col >= 0; --col) {
+        if (col >= dim || !ok) continue;
+        double sum = rhs;
+#pragma unroll
+        for (int j = col + 1; j < kD; ++j)
+            if (j < dim) sum -= a[j] * x[j];
+        x[col] = read_lane(sum / a[col], col);
+    }
+    if (!ok) return false;
+    double wsum = 0.0;
+#pragma unroll
+    for (int i = 0; i < kDiisDepth; ++i)
+        if (i < m) wsum += fabs(x[i]);
+    if (!isfinite(wsum)) return false;
+    if (lane == 0) {
+#pragma unroll
+        for (int i = 0; i < kDiisDepth; ++i)
+            if (i < m) c_out[i] = x[i];
+    }
+    return true;
+}
+
 // A-DIIS coefficients on one wavefront: the projected-gradient iteration of adiis_solve with
 // lanes 0..m-1 carrying one coefficient each (called by all 64 lanes of one wavefront). Each
 // step's matrix-vector product, simplex projection (ranks instead of a sort; ties keep the lower
