@@ -221,74 +221,106 @@ __device__ __noinline__ void mol_fock(const MolDesc& md, const DevElem* el, cons
                          const double* H, const double* Pt, const double* Ps, double scale, double* F,
                          double* pdg) {
     const int t = threadIdx.x;
-    const int nat = md.nat, nao = md.nao;
+    const int nao = md.nao;
     const int nh = ms.nh, nH = ms.nH;
     const int n_items = nh * 10 + nH;
-    // every atom's diagonal density block, packed with the off-diagonal entries doubled (pdg:
-    // nat * 10 doubles of shared scratch), read by all the atom's partners below
-    for (int idx = t; idx < nat * 10; idx += kWg) {
-        const int y = idx / 10, q = idx - y * 10;
-        const int oy = ms.off[y];
-        double v = 0.0;
-        if (ms.z[y] != 1) {
+    const int64_t n_hh = (int64_t)nh * (nh - 1) / 2, n_hH = (int64_t)nh * nH, n_HH = (int64_t)nH * (nH - 1) / 2;
+    // every atom's diagonal density block (pdg: n_items doubles of shared scratch): heavy atoms'
+    // 10 packed entries with the off-diagonal ones doubled, in heavy-list order, then one entry
+    // per hydrogen; read by all of the atom's partners below
+    for (int idx = t; idx < n_items; idx += kWg) {
+        double v;
+        if (idx < nh * 10) {
+            const int y = ms.heavy[idx / 10], q = idx % 10, oy = ms.off[y];
             const int l = kPackRow[q], c = kPackCol[q];
             v = Pt[(size_t)(oy + l) * nao + oy + c] * (l == c ? 1.0 : 2.0);
-        } else if (q == 0) {
+        } else {
+            const int oy = ms.off[ms.hyd[idx - nh * 10]];
             v = Pt[(size_t)oy * nao + oy];
         }
         pdg[idx] = v;
     }
     __syncthreads();
+    const double* pdh = pdg + nh * 10;
+    const double* wHh = wpool + n_hh * 100;  // heavy-hydrogen blocks (10 each), heavy atom as bra
+    const double* wHH = wHh + n_hH * 10;     // hydrogen-hydrogen (1 each)
     // diagonal blocks, G threads per entry (as many as the workgroup allows, up to 8): the
-    // partners are split among them and their sums combined in a fixed order
+    // partners are split among them and their sums combined in a fixed order. Partners are
+    // walked by class (heavy atoms below and above, then hydrogens), so each loop has a fixed
+    // shape and its loads issue together.
     int G = 1;
     while (G < 8 && n_items * (2 * G) <= kWg) G *= 2;
     const int gl = t & (G - 1);
     for (int it = t / G; it < n_items; it += kWg / G) {
         int x, k;
+        double acc = 0.0;
         if (it < nh * 10) {
-            x = ms.heavy[it / 10];
+            const int ix = it / 10;
+            x = ms.heavy[ix];
             k = it % 10;
+#pragma unroll 4
+            for (int iy = gl; iy < ix; iy += G) {  // x is the ket: column k of the block
+                const double* w = wpool + tri_index(iy, ix, nh) * 100 + k;
+                const double* py = pdg + iy * 10;
+#pragma unroll
+                for (int q = 0; q < 10; ++q) acc += py[q] * w[q * 10];
+            }
+#pragma unroll 4
+            for (int iy = ix + 1 + gl; iy < nh; iy += G) {  // x is the bra: row k
+                const double* w = wpool + tri_index(ix, iy, nh) * 100 + k * 10;
+                const double* py = pdg + iy * 10;
+#pragma unroll
+                for (int q = 0; q < 10; ++q) acc += py[q] * w[q];
+            }
+            const double* w = wHh + (int64_t)ix * nH * 10 + k;
+#pragma unroll 4
+            for (int m = gl; m < nH; m += G) acc += pdh[m] * w[m * 10];
         } else {
-            x = ms.hyd[it - nh * 10];
+            const int mx = it - nh * 10;
+            x = ms.hyd[mx];
             k = 0;
+#pragma unroll 4
+            for (int iy = gl; iy < nh; iy += G) {
+                const double* w = wHh + ((int64_t)iy * nH + mx) * 10;
+                const double* py = pdg + iy * 10;
+#pragma unroll
+                for (int q = 0; q < 10; ++q) acc += py[q] * w[q];
+            }
+#pragma unroll 4
+            for (int my = gl; my < mx; my += G) acc += pdh[my] * wHH[tri_index(my, mx, nH)];
+#pragma unroll 4
+            for (int my = mx + 1 + gl; my < nH; my += G) acc += pdh[my] * wHH[tri_index(mx, my, nH)];
         }
         const DevElem& ex = el[ms.z[x]];
         const int ox = ms.off[x], nx = ex.norb;
         const int mu = kPackRow[k], nu = kPackCol[k];
-        double acc = 0.0;
-        // two-centre Coulomb from every partner's diagonal density block
-#pragma unroll 4
-        for (int y = gl; y < nat; y += G) {
-            if (y == x) continue;
-            BlockRef br = block_of(ms, wpool, x, y);
-            const double* py = pdg + y * 10;
-            if (ms.z[y] == 1) {
-                double wv = br.x_is_bra ? br.w[k * br.wcols] : br.w[k];
-                acc += py[0] * wv;
-            } else {
-#pragma unroll
-                for (int q = 0; q < 10; ++q) {
-                    double wv = br.x_is_bra ? br.w[k * br.wcols + q] : br.w[q * br.wcols + k];
-                    acc += py[q] * wv;
-                }
-            }
-        }
         for (int off = 1; off < G; off <<= 1) acc += __shfl_xor(acc, off, 64);
         if (gl == 0) {
             acc += H[(size_t)(ox + mu) * nao + ox + nu];
-            // one-centre
-            for (int la = 0; la < nx; ++la)
-                for (int si = 0; si < nx; ++si) {
-                    acc += Pt[(size_t)(ox + la) * nao + ox + si] * one_centre(mu, nu, la, si, ex);
-                    acc -= scale * Ps[(size_t)(ox + la) * nao + ox + si] * one_centre(mu, la, nu, si, ex);
+            // one-centre (the atom's density block is loaded at once)
+            if (nx == 4) {
+                double pt[16], ps[16];
+#pragma unroll
+                for (int q = 0; q < 16; ++q) {
+                    pt[q] = Pt[(size_t)(ox + (q >> 2)) * nao + ox + (q & 3)];
+                    ps[q] = Ps[(size_t)(ox + (q >> 2)) * nao + ox + (q & 3)];
                 }
+#pragma unroll
+                for (int la = 0; la < 4; ++la)
+#pragma unroll
+                    for (int si = 0; si < 4; ++si) {
+                        acc += pt[la * 4 + si] * one_centre(mu, nu, la, si, ex);
+                        acc -= scale * ps[la * 4 + si] * one_centre(mu, la, nu, si, ex);
+                    }
+            } else {
+                acc += Pt[(size_t)ox * nao + ox] * one_centre(mu, nu, 0, 0, ex);
+                acc -= scale * Ps[(size_t)ox * nao + ox] * one_centre(mu, 0, nu, 0, ex);
+            }
             F[(size_t)(ox + mu) * nao + ox + nu] = acc;
             F[(size_t)(ox + nu) * nao + ox + mu] = acc;
         }
     }
     // exchange blocks, one pair per thread, grouped by class
-    const int64_t n_hh = (int64_t)nh * (nh - 1) / 2, n_hH = (int64_t)nh * nH, n_HH = (int64_t)nH * (nH - 1) / 2;
     for (int64_t k = t; k < n_hh; k += kWg) {
         int i, j;
         tri_decode(k, nh, i, j);
