@@ -368,12 +368,8 @@ __device__ __noinline__ int wg_eigh_tri(int n, const double* __restrict__ F, dou
         // whose rows are all <= k skips the work (row k is not read again), and so does every
         // quarter of the columns that lies below k (both tests are uniform across a wavefront).
         if (16 * (t >> 6) + 15 > k && 16 * (t >> 6) < n) {
-            // columns 2 q8 + 16 j: the pairs are 16-byte aligned; each quarter's loads are
-            // issued together
-            auto fused = [&](int j) {
-                const int c = 2 * q8 + 16 * j;
-                const double2 p2 = *(const double2*)(po + c);
-                const double2 u2 = *(const double2*)(vn + c);
+            // columns 2 q8 + 16 j: the pairs are 16-byte aligned
+            auto fused = [&](int j, double2 p2, double2 u2, double& acc0, double& acc1) {
                 const double w0 = fma(-Ko, vown[2 * j], p2.x), w1 = fma(-Ko, vown[2 * j + 1], p2.y);
                 const double a00 = fma(-vr0, w0, fma(-wr0, vown[2 * j], zr[0][2 * j]));
                 const double a01 = fma(-vr0, w1, fma(-wr0, vown[2 * j + 1], zr[0][2 * j + 1]));
@@ -383,18 +379,36 @@ __device__ __noinline__ int wg_eigh_tri(int n, const double* __restrict__ F, dou
                 zr[0][2 * j + 1] = a01;
                 zr[1][2 * j] = a10;
                 zr[1][2 * j + 1] = a11;
-                s0 = fma(a00, u2.x, fma(a01, u2.y, s0));
-                s1 = fma(a10, u2.x, fma(a11, u2.y, s1));
+                acc0 = fma(a00, u2.x, fma(a01, u2.y, acc0));
+                acc1 = fma(a10, u2.x, fma(a11, u2.y, acc1));
                 vown[2 * j] = u2.x;  // this reflector is the next step's previous one
                 vown[2 * j + 1] = u2.y;
             };
+            // each half's loads are issued together, ahead of its quarters' arithmetic (one LDS
+            // round trip per half); odd column blocks accumulate separately (shorter chains)
+            double u0 = 0.0, u1 = 0.0;
 #pragma unroll
-            for (int jq = 0; jq < 4; ++jq) {
-                if (32 * jq + 32 > k && 32 * jq < n) {
-                    fused(2 * jq);
-                    fused(2 * jq + 1);
+            for (int hf = 0; hf < 2; ++hf) {
+                if (64 * hf + 64 > k && 64 * hf < n) {
+                    double2 p2[4], u2[4];
+#pragma unroll
+                    for (int jj = 0; jj < 4; ++jj) {
+                        const int c = 2 * q8 + 16 * (4 * hf + jj);
+                        p2[jj] = *(const double2*)(po + c);
+                        u2[jj] = *(const double2*)(vn + c);
+                    }
+#pragma unroll
+                    for (int qq = 0; qq < 2; ++qq) {
+                        const int jq = 2 * hf + qq;
+                        if (32 * jq + 32 > k && 32 * jq < n) {
+                            fused(2 * jq, p2[2 * qq], u2[2 * qq], s0, s1);
+                            fused(2 * jq + 1, p2[2 * qq + 1], u2[2 * qq + 1], u0, u1);
+                        }
+                    }
                 }
             }
+            s0 += u0;
+            s1 += u1;
             s0 += dpp_move<0xB1>(s0);  // sums over the group of eight
             s1 += dpp_move<0xB1>(s1);
             s0 += dpp_move<0x4E>(s0);
@@ -708,28 +722,36 @@ __device__ __noinline__ int wg_eigh_tri(int n, const double* __restrict__ F, dou
     __syncthreads();
     if (2 * g8 < n) {
         for (int k = n - 3; k >= 0; --k) {
+            // tau = 0 (a column that needed no reflection) makes the step a no-op
             const double tk = tau[k];
-            if (tk == 0.0) continue;
             const double* v = A + k * sr;
+            // All loads first, so they cost one LDS round trip: columns >= n hold zeros in the
+            // rows and in the owners' registers; beyond the row stride the load is clamped into
+            // the row and its value replaced by zero.
             double2 vv[8];
-            double s0 = 0.0, s1 = 0.0;
-            // columns >= n hold zeros in the rows and in the owners' registers; beyond the row
-            // stride the load is clamped into the row and its value discarded
 #pragma unroll
-            for (int j = 0; j < 8; ++j) vv[j] = make_double2(0.0, 0.0);
-            auto dot = [&](int j) {
+            for (int j = 0; j < 8; ++j) {
                 const int c = 2 * q8 + 16 * j;
                 const double2 ld2 = *(const double2*)(v + min(c, sr - 2));
                 vv[j] = c < n ? ld2 : make_double2(0.0, 0.0);
-                s0 = fma(zr[0][2 * j], vv[j].x, fma(zr[0][2 * j + 1], vv[j].y, s0));
-                s1 = fma(zr[1][2 * j], vv[j].x, fma(zr[1][2 * j + 1], vv[j].y, s1));
+            }
+            // The reflector vanishes on columns <= k, so column quarters that lie there are
+            // skipped (a uniform test); even and odd column blocks accumulate separately
+            // (shorter dependency chains).
+            double s0 = 0.0, s1 = 0.0, u0 = 0.0, u1 = 0.0;
+            auto dot = [&](int j, double& a0, double& a1) {
+                a0 = fma(zr[0][2 * j], vv[j].x, fma(zr[0][2 * j + 1], vv[j].y, a0));
+                a1 = fma(zr[1][2 * j], vv[j].x, fma(zr[1][2 * j + 1], vv[j].y, a1));
             };
 #pragma unroll
-            for (int j = 0; j < 4; ++j) dot(j);
-            if (n > 64) {
-#pragma unroll
-                for (int j = 4; j < 8; ++j) dot(j);
+            for (int jq = 0; jq < 4; ++jq) {
+                if (32 * jq + 31 > k && 32 * jq < n) {
+                    dot(2 * jq, s0, s1);
+                    dot(2 * jq + 1, u0, u1);
+                }
             }
+            s0 += u0;
+            s1 += u1;
             s0 += dpp_move<0xB1>(s0);
             s1 += dpp_move<0xB1>(s1);
             s0 += dpp_move<0x4E>(s0);
@@ -738,11 +760,16 @@ __device__ __noinline__ int wg_eigh_tri(int n, const double* __restrict__ F, dou
             s1 += dpp_move<0x141>(s1);
             const double f0 = tk * s0, f1 = tk * s1;
 #pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                zr[0][2 * j] = fma(-f0, vv[j].x, zr[0][2 * j]);
-                zr[0][2 * j + 1] = fma(-f0, vv[j].y, zr[0][2 * j + 1]);
-                zr[1][2 * j] = fma(-f1, vv[j].x, zr[1][2 * j]);
-                zr[1][2 * j + 1] = fma(-f1, vv[j].y, zr[1][2 * j + 1]);
+            for (int jq = 0; jq < 4; ++jq) {
+                if (32 * jq + 31 > k && 32 * jq < n) {
+#pragma unroll
+                    for (int j = 2 * jq; j < 2 * jq + 2; ++j) {
+                        zr[0][2 * j] = fma(-f0, vv[j].x, zr[0][2 * j]);
+                        zr[0][2 * j + 1] = fma(-f0, vv[j].y, zr[0][2 * j + 1]);
+                        zr[1][2 * j] = fma(-f1, vv[j].x, zr[1][2 * j]);
+                        zr[1][2 * j + 1] = fma(-f1, vv[j].y, zr[1][2 * j + 1]);
+                    }
+                }
             }
         }
 #pragma unroll
