@@ -91,6 +91,47 @@ __device__ __forceinline__ void tri_count2(const double2* __restrict__ de, int b
     cb = nb;
 }
 
+// tri_count2 that also returns the last minors: p_{b1-1}(x) = f * 2^e for both points (the
+// characteristic polynomial of the block at x, up to its sign convention), for the
+// regula-falsi steps of step 3.
+__device__ __forceinline__ void tri_eval2(const double2* __restrict__ de, int b0, int b1, double xa, double xb,
+                                          int& ca, int& cb, double& fa, int& ea, double& fb, int& eb) {
+    double am = 1.0, ac = de[b0].x - xa, bm = 1.0, bc = de[b0].x - xb;
+    int na = ac < 0.0, nb = bc < 0.0;
+    int sa = 0, sb = 0;
+    int i = b0 + 1;
+    for (; i + 4 <= b1; i += 4) {
+        double2 q[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) q[u] = de[i + u];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            tri_step(q[u], xa, am, ac, na);
+            tri_step(q[u], xb, bm, bc, nb);
+        }
+        int ex;
+        frexp(ac, &ex);
+        ac = ldexp(ac, -ex);
+        am = ldexp(am, -ex);
+        sa += ex;
+        frexp(bc, &ex);
+        bc = ldexp(bc, -ex);
+        bm = ldexp(bm, -ex);
+        sb += ex;
+    }
+    for (; i < b1; ++i) {
+        const double2 q = de[i];
+        tri_step(q, xa, am, ac, na);
+        tri_step(q, xb, bm, bc, nb);
+    }
+    ca = na;
+    cb = nb;
+    fa = ac;
+    ea = sa;
+    fb = bc;
+    eb = sb;
+}
+
 // ---------------------------------------------------------------------------------------------
 // Step 4: eigenvector of the block [b0, b1) at eigenvalue lm into z (normalized), through the
 // twisted factorization T - lm = N_r D_r N_r^T: z_r = 1, z_i = -(e_i / D+_i) z_{i+1} below the
@@ -577,6 +618,11 @@ __device__ __noinline__ int wg_eigh_tri(int n, const double* __restrict__ F, dou
             if (b1 - b0 > 1) {
                 lo = glo;
                 hi = ghi;
+                // Bracket state: block-local counts at the ends (clo <= kk < chi) and, once known,
+                // the characteristic polynomial there (f * 2^e).
+                int clo = 0, chi = b1 - b0, elo = 0, ehi = 0;
+                double flo = 0.0, fhi = 0.0;
+                bool klo = false, khi = false;
                 if (shared_pass) {
                     // cnt[p] is the count at glo + (p + 1) sstep; keep cnt[plo] <= kk < cnt[phi]
                     int plo = -1, phi = 2 * kWg;
@@ -585,40 +631,104 @@ __device__ __noinline__ int wg_eigh_tri(int n, const double* __restrict__ F, dou
                         if (cnt[mid] <= kk) plo = mid;
                         else phi = mid;
                     }
-                    lo = glo + (double)(plo + 1) * sstep;
-                    if (phi < 2 * kWg) hi = glo + (double)(phi + 1) * sstep;
+                    if (plo >= 0) {
+                        lo = glo + (double)(plo + 1) * sstep;
+                        clo = cnt[plo];
+                    }
+                    if (phi < 2 * kWg) {
+                        hi = glo + (double)(phi + 1) * sstep;
+                        chi = cnt[phi];
+                    }
                 }
                 // a verified bracket around the previous eigenvalue (one block spanning T, so
                 // the block-local index is the global one), when it saves two passes or more
                 if (bracket > 0.0 && one_block && 2.0 * bracket * (double)((2 * G + 1) * (2 * G + 1)) < hi - lo) {
                     const double wl = eps_out[j] - bracket, wh = eps_out[j] + bracket;
-                    int cl, ch;
-                    tri_count2(de, 0, n, wl, wh, cl, ch);
+                    int cl, ch, el, eh;
+                    double fl, fh;
+                    tri_eval2(de, 0, n, wl, wh, cl, ch, fl, el, fh, eh);
                     if (cl <= kk && ch > kk) {
-                        lo = fmax(lo, wl);
-                        hi = fmin(hi, wh);
+                        if (wl > lo) {
+                            lo = wl;
+                            clo = cl;
+                            flo = fl;
+                            elo = el;
+                            klo = true;
+                        }
+                        if (wh < hi) {
+                            hi = wh;
+                            chi = ch;
+                            fhi = fh;
+                            ehi = eh;
+                            khi = true;
+                        }
                     }
                 }
+                // Passes: multisection while the bracket holds more than this eigenvalue or an
+                // end's polynomial value is unknown; then regula falsi: the secant point xr of
+                // the polynomial and points within +-eps of it, eps from the error the previous
+                // secant point turned out to have (error ~ K w^2). Counts decide every move, so
+                // the bracket always holds the eigenvalue; a pass that fails to halve it makes
+                // the next one a multisection.
                 const double atol = DBL_EPSILON * nrm;
-                for (int it = 0; it < 80; ++it) {
+                double xr_prev = 0.0, w_prev = 0.0;
+                bool have_prev = false, force = false;
+                for (int it = 0; it < 120; ++it) {
                     const double w = hi - lo;
                     if (!(w > fmax(2.0 * DBL_EPSILON * fmax(fabs(lo), fabs(hi)), atol))) break;
-                    const double step = w / (double)(2 * G + 1);
-                    const double xa = lo + (double)(2 * l + 1) * step, xb = lo + (double)(2 * l + 2) * step;
-                    int ca, cb;
-                    tri_count2(de, b0, b1, xa, xb, ca, cb);
-                    // a point at or below the kk-th eigenvalue raises lo, one above it lowers hi
-                    double nlo = -INFINITY, nhi = INFINITY;
-                    if (ca <= kk) nlo = xa;
-                    else nhi = xa;
-                    if (cb <= kk) nlo = fmax(nlo, xb);
-                    else nhi = fmin(nhi, xb);
-                    for (int off = 1; off < G; off <<= 1) {
-                        nlo = fmax(nlo, __shfl_xor(nlo, off, 64));
-                        nhi = fmin(nhi, __shfl_xor(nhi, off, 64));
+                    const bool rf = chi - clo == 1 && klo && khi && !force;
+                    double xa, xb;
+                    if (rf) {
+                        const double r = ldexp(fhi / flo, max(-1000, min(1000, ehi - elo)));
+                        const double tiny = fmax(1.0e-9 * w, 0.25 * atol);
+                        double xr = lo + w / (1.0 - r);
+                        xr = fmin(fmax(xr, lo + tiny), hi - tiny);
+                        double ep = w * (1.0 / 64.0);
+                        if (have_prev) ep = 8.0 * fabs(xr - xr_prev) * (w / w_prev) * (w / w_prev);
+                        ep = fmax(ep, 0.5 * atol);
+                        const double sp = 2.0 * ep / (double)(2 * G - 1);
+                        xa = fmin(fmax(xr - ep + (double)(2 * l) * sp, lo + tiny), hi - tiny);
+                        xb = fmin(fmax(xr - ep + (double)(2 * l + 1) * sp, lo + tiny), hi - tiny);
+                        xr_prev = xr;
+                        w_prev = w;
+                        have_prev = true;
+                    } else {
+                        const double step = w / (double)(2 * G + 1);
+                        xa = lo + (double)(2 * l + 1) * step;
+                        xb = lo + (double)(2 * l + 2) * step;
                     }
-                    lo = fmax(lo, nlo);
-                    hi = fmin(hi, nhi);
+                    int ca, cb, ea, eb;
+                    double fa, fb;
+                    tri_eval2(de, b0, b1, xa, xb, ca, cb, fa, ea, fb, eb);
+                    // a point at or below the kk-th eigenvalue raises lo, one above it lowers hi
+                    double nlo = -INFINITY, nhi = INFINITY, nflo = 0.0, nfhi = 0.0;
+                    int nclo = 0, nchi = 0, nelo = 0, nehi = 0;
+                    if (ca <= kk) {
+                        nlo = xa; nclo = ca; nflo = fa; nelo = ea;
+                    } else {
+                        nhi = xa; nchi = ca; nfhi = fa; nehi = ea;
+                    }
+                    if (cb <= kk) {
+                        if (xb > nlo) { nlo = xb; nclo = cb; nflo = fb; nelo = eb; }
+                    } else if (xb < nhi) {
+                        nhi = xb; nchi = cb; nfhi = fb; nehi = eb;
+                    }
+                    for (int off = 1; off < G; off <<= 1) {
+                        const double olo = __shfl_xor(nlo, off, 64), ohi = __shfl_xor(nhi, off, 64);
+                        const double oflo = __shfl_xor(nflo, off, 64), ofhi = __shfl_xor(nfhi, off, 64);
+                        const int oclo = __shfl_xor(nclo, off, 64), ochi = __shfl_xor(nchi, off, 64);
+                        const int oelo = __shfl_xor(nelo, off, 64), oehi = __shfl_xor(nehi, off, 64);
+                        if (olo > nlo) { nlo = olo; nclo = oclo; nflo = oflo; nelo = oelo; }
+                        if (ohi < nhi) { nhi = ohi; nchi = ochi; nfhi = ofhi; nehi = oehi; }
+                    }
+                    if (nlo > lo) {
+                        lo = nlo; clo = nclo; flo = nflo; elo = nelo; klo = true;
+                    }
+                    if (nhi < hi) {
+                        hi = nhi; chi = nchi; fhi = nfhi; ehi = nehi; khi = true;
+                    }
+                    force = rf && hi - lo > 0.5 * w;
+                    if (force) have_prev = false;
                 }
             }
             if (l == 0) lam[j] = 0.5 * (lo + hi);
