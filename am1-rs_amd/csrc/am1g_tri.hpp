@@ -641,8 +641,9 @@ __device__ __noinline__ int wg_eigh_tri(int n, const double* __restrict__ F, dou
                     }
                 }
                 // a verified bracket around the previous eigenvalue (one block spanning T, so
-                // the block-local index is the global one), when it saves two passes or more
-                if (bracket > 0.0 && one_block && 2.0 * bracket * (double)((2 * G + 1) * (2 * G + 1)) < hi - lo) {
+                // the block-local index is the global one) when it is the narrower one: its
+                // evaluation also gives the polynomial at both ends, so regula falsi can start
+                if (bracket > 0.0 && one_block && 2.0 * bracket < hi - lo) {
                     const double wl = eps_out[j] - bracket, wh = eps_out[j] + bracket;
                     int cl, ch, el, eh;
                     double fl, fh;
