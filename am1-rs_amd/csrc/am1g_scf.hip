@@ -632,13 +632,13 @@ __device__ __noinline__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolS
         const double* ep[kDiisDepth];
         const double* fp[kDiisDepth];
 #pragma unroll
-        for (int i = 0; i < kDiisDepth; ++i) {
-            const int64_t r = (head + (i < m ? i : newest)) % kDiisDepth;
+        for (int i = 0; i < kDiisDepth; ++i) {  // past the history: the oldest slot (read, unused)
+            const int64_t r = (head + (i < newest ? i : 0)) % kDiisDepth;
             ep[i] = dv.e + r * dv.len_a;
             fp[i] = dv.f + r * dv.len_s;
         }
-        double* he = dv.e + (int64_t)((head + newest) % kDiisDepth) * dv.len_a;
-        double* hf = dv.f + (int64_t)((head + newest) % kDiisDepth) * dv.len_s;
+        double* __restrict__ he = dv.e + (int64_t)((head + newest) % kDiisDepth) * dv.len_a;
+        double* __restrict__ hf = dv.f + (int64_t)((head + newest) % kDiisDepth) * dv.len_s;
         // Two passes over the packed triangles push the new entry and form, in one reduction:
         // part[q] = <e_new, e_q> (CDIIS Gram row; q = newest gives ||e||^2), and for A-DIIS
         // part[8 + q] = <D_new, F_q>, part[16] = <D_new, H>; part[17] = E_elec = 1/2 sum P (H + F).
@@ -655,10 +655,12 @@ __device__ __noinline__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolS
             stri_rowcol(pk, nao, i, j);
             const double e = T0[(size_t)i * nao + j] - T0[(size_t)j * nao + i];
             he[pk] = e;
+            // every slot pointer is valid, so all loads issue without branches; the ones past
+            // the history are discarded by the selects
 #pragma unroll
             for (int q = 0; q < kDiisDepth; ++q) {
-                if (q < newest) part[q] = fma(e, ep[q][pk], part[q]);
-                else if (q == newest) part[q] = fma(e, e, part[q]);
+                const double hq = ep[q][pk];
+                part[q] = fma(e, q < newest ? hq : (q == newest ? e : 0.0), part[q]);
             }
         }
 #pragma unroll 4
@@ -674,8 +676,8 @@ __device__ __noinline__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolS
                 part[2 * kDiisDepth] = fma(pw, h, part[2 * kDiisDepth]);
 #pragma unroll
                 for (int q = 0; q < kDiisDepth; ++q) {
-                    if (q < newest) part[kDiisDepth + q] = fma(pw, fp[q][pk], part[kDiisDepth + q]);
-                    else if (q == newest) part[kDiisDepth + q] = fma(pw, f, part[kDiisDepth + q]);
+                    const double fq = fp[q][pk];
+                    part[kDiisDepth + q] = fma(pw, q < newest ? fq : (q == newest ? f : 0.0), part[kDiisDepth + q]);
                 }
             }
         }
@@ -756,8 +758,10 @@ __device__ __noinline__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolS
                 tri_rowcol(pk, nao, i, j);
                 double acc = 0.0;
 #pragma unroll
-                for (int q = 0; q < kDiisDepth; ++q)
-                    if (q < m) acc = fma(cq[q], fq[q][pk], acc);
+                for (int q = 0; q < kDiisDepth; ++q) {
+                    const double v = fq[q][pk];  // (slots past the history repeat the newest)
+                    acc = fma(cq[q], q < m ? v : 0.0, acc);
+                }
                 F[(size_t)i * nao + j] = acc;
                 F[(size_t)j * nao + i] = acc;
                 if (!isfinite(acc)) bad = 1.0;
