@@ -547,6 +547,30 @@ __device__ __forceinline__ DiisView diis_view(double* hist, int nao, int channel
     return v;
 }
 
+// P <- Pn over n entries; returns this thread's part of sum (Pn - P)^2. Eight entries per thread
+// per round, all loaded before any store, so each round costs one memory round trip.
+__device__ __forceinline__ double density_update(const double* __restrict__ Pn, double* __restrict__ P, size_t n) {
+    constexpr int U = 8;
+    double dd = 0.0;
+    for (size_t base = threadIdx.x; base < n; base += (size_t)U * kWg) {
+        double a[U], b[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const size_t idx = base + (size_t)u * kWg;
+            a[u] = idx < n ? Pn[idx] : 0.0;
+            b[u] = idx < n ? P[idx] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const size_t idx = base + (size_t)u * kWg;
+            const double x = a[u] - b[u];
+            dd += x * x;
+            if (idx < n) P[idx] = a[u];
+        }
+    }
+    return dd;
+}
+
 // Per-phase device clock, kept by thread 0 (phases: 0 setup, 1 Fock + energy, 2 commutator +
 // DIIS, 3 eigensolve, 4 density + convergence, 5 final energy and properties).
 struct PhaseClock {
@@ -752,19 +776,32 @@ __device__ __noinline__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolS
                 fq[q] = dv.f + (int64_t)ms.ring[q < m ? q : m - 1] * dv.len_s;
                 cq[q] = q < m ? ms.coef[q] : 0.0;
             }
-#pragma unroll 4
-            for (int64_t pk = t; pk < dv.len_s; pk += kWg) {
-                int i, j;
-                tri_rowcol(pk, nao, i, j);
-                double acc = 0.0;
+            // four entries per thread per round, every load before any store (F does not alias
+            // the history, but the compiler cannot know it); slots past the history repeat the
+            // newest and are discarded by the selects
+            constexpr int U = 4;
+            for (int64_t base = t; base < dv.len_s; base += (int64_t)U * kWg) {
+                double v[U][kDiisDepth];
 #pragma unroll
-                for (int q = 0; q < kDiisDepth; ++q) {
-                    const double v = fq[q][pk];  // (slots past the history repeat the newest)
-                    acc = fma(cq[q], q < m ? v : 0.0, acc);
+                for (int u = 0; u < U; ++u) {
+                    const int64_t pk = min(base + (int64_t)u * kWg, dv.len_s - 1);
+#pragma unroll
+                    for (int q = 0; q < kDiisDepth; ++q) v[u][q] = fq[q][pk];
                 }
-                F[(size_t)i * nao + j] = acc;
-                F[(size_t)j * nao + i] = acc;
-                if (!isfinite(acc)) bad = 1.0;
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int64_t pk = base + (int64_t)u * kWg;
+                    double acc = 0.0;
+#pragma unroll
+                    for (int q = 0; q < kDiisDepth; ++q) acc = fma(cq[q], q < m ? v[u][q] : 0.0, acc);
+                    if (pk < dv.len_s) {
+                        int i, j;
+                        tri_rowcol(pk, nao, i, j);
+                        F[(size_t)i * nao + j] = acc;
+                        F[(size_t)j * nao + i] = acc;
+                        if (!isfinite(acc)) bad = 1.0;
+                    }
+                }
             }
             bad = wg_max(bad, s);
             if (bad != 0.0) {
@@ -793,12 +830,7 @@ __device__ __noinline__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolS
         if (low) mol_density<float>(nao, nocc, 2.0, Ct, T0, lds);
         else mol_density<double>(nao, nocc, 2.0, Ct, T0, lds);
         __syncthreads();
-        double dd = 0.0;
-        for (size_t idx = t; idx < nn; idx += kWg) {
-            double x = T0[idx] - P[idx];
-            dd += x * x;
-            P[idx] = T0[idx];
-        }
+        const double dd = density_update(T0, P, nn);
         double dp = sqrt(wg_sum(dd, s) / (double)max((size_t)1, nn));
         phase_mark(pc, 4);
         double de = fabs(e_elec - e_old);
@@ -1062,12 +1094,7 @@ __device__ __noinline__ void scf_uhf(const MolDesc& md, const BatchArgs& g, MolS
             if (low) mol_density<float>(nao, nocc[c], 1.0, Cs[c], T0, lds);
             else mol_density<double>(nao, nocc[c], 1.0, Cs[c], T0, lds);
             __syncthreads();
-            double dd = 0.0;
-            for (size_t idx = t; idx < nn; idx += kWg) {
-                double x = T0[idx] - Ps[c][idx];
-                dd += x * x;
-                Ps[c][idx] = T0[idx];
-            }
+            const double dd = density_update(T0, Ps[c], nn);
             dsum[c] = sqrt(wg_sum(dd, s) / (double)max((size_t)1, nn));
         }
         const double dp = dsum[0] + dsum[1];
