@@ -369,31 +369,63 @@ __device__ __noinline__ void mol_fock(const MolDesc& md, const DevElem* el, cons
                 F[(size_t)(ob + la) * nao + oa + mu] = H[(size_t)(ob + la) * nao + oa + mu] - acc[mu * 4 + la];
             }
     }
-    for (int64_t k = t; k < n_hH; k += kWg) {
-        int i = (int)(k / nH), m = (int)(k - (int64_t)i * nH);
-        int a = ms.heavy[i], b = ms.hyd[m];
-        int oa = ms.off[a], ob = ms.off[b];
-        const double* w = wpool + n_hh * 100 + k * 10;
-        double pab[4];
+    // heavy-hydrogen and hydrogen-hydrogen blocks: a few pairs per thread per round, every load
+    // of a round issued before its stores (for the compiler F may alias the inputs, so loads
+    // after a store would wait for it)
+    for (int64_t k0 = t; k0 < n_hH; k0 += 2 * kWg) {
+        double pab[2][4], wv[2][10], hv[2][8];
+        int oa[2], ob[2];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) pab[u] = scale * Ps[(size_t)(oa + u) * nao + ob];
+        for (int u = 0; u < 2; ++u) {
+            const int64_t k = min(k0 + (int64_t)u * kWg, n_hH - 1);
+            const int i = (int)(k / nH), m = (int)(k - (int64_t)i * nH);
+            oa[u] = ms.off[ms.heavy[i]];
+            ob[u] = ms.off[ms.hyd[m]];
+            const double* w = wpool + n_hh * 100 + k * 10;
 #pragma unroll
-        for (int mu = 0; mu < 4; ++mu) {
-            double acc = 0.0;
+            for (int q = 0; q < 10; ++q) wv[u][q] = w[q];
 #pragma unroll
-            for (int nu = 0; nu < 4; ++nu) acc += pab[nu] * w[pack4(mu, nu)];
-            F[(size_t)(oa + mu) * nao + ob] = H[(size_t)(oa + mu) * nao + ob] - acc;
-            F[(size_t)ob * nao + oa + mu] = H[(size_t)ob * nao + oa + mu] - acc;
+            for (int v = 0; v < 4; ++v) {
+                pab[u][v] = scale * Ps[(size_t)(oa[u] + v) * nao + ob[u]];
+                hv[u][v] = H[(size_t)(oa[u] + v) * nao + ob[u]];
+                hv[u][4 + v] = H[(size_t)ob[u] * nao + oa[u] + v];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            if (k0 + (int64_t)u * kWg >= n_hH) continue;
+#pragma unroll
+            for (int mu = 0; mu < 4; ++mu) {
+                double acc = 0.0;
+#pragma unroll
+                for (int nu = 0; nu < 4; ++nu) acc += pab[u][nu] * wv[u][pack4(mu, nu)];
+                F[(size_t)(oa[u] + mu) * nao + ob[u]] = hv[u][mu] - acc;
+                F[(size_t)ob[u] * nao + oa[u] + mu] = hv[u][4 + mu] - acc;
+            }
         }
     }
-    for (int64_t k = t; k < n_HH; k += kWg) {
-        int i, j;
-        tri_decode(k, nH, i, j);
-        int a = ms.hyd[i], b = ms.hyd[j];
-        int oa = ms.off[a], ob = ms.off[b];
-        double acc = scale * Ps[(size_t)oa * nao + ob] * wpool[n_hh * 100 + n_hH * 10 + k];
-        F[(size_t)oa * nao + ob] = H[(size_t)oa * nao + ob] - acc;
-        F[(size_t)ob * nao + oa] = H[(size_t)ob * nao + oa] - acc;
+    for (int64_t k0 = t; k0 < n_HH; k0 += 4 * kWg) {
+        double pv[4], wv[4], h1[4], h2[4];
+        int oa[4], ob[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int64_t k = min(k0 + (int64_t)u * kWg, n_HH - 1);
+            int i, j;
+            tri_decode(k, nH, i, j);
+            oa[u] = ms.off[ms.hyd[i]];
+            ob[u] = ms.off[ms.hyd[j]];
+            pv[u] = Ps[(size_t)oa[u] * nao + ob[u]];
+            wv[u] = wpool[n_hh * 100 + n_hH * 10 + k];
+            h1[u] = H[(size_t)oa[u] * nao + ob[u]];
+            h2[u] = H[(size_t)ob[u] * nao + oa[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (k0 + (int64_t)u * kWg >= n_HH) continue;
+            const double acc = scale * pv[u] * wv[u];
+            F[(size_t)oa[u] * nao + ob[u]] = h1[u] - acc;
+            F[(size_t)ob[u] * nao + oa[u]] = h2[u] - acc;
+        }
     }
 }
 
