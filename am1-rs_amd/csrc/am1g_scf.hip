@@ -787,7 +787,7 @@ __device__ __noinline__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolS
         const bool cdiis = accel == 1 || (accel == 2 && err <= md.adiis_switch);
         if (cdiis) {
             if (t < 64) {
-                const bool use = m >= 2 && cdiis_solve_wave(ms.B, m, ms.coef);
+                const bool use = m >= 2 && cdiis_solve_wave(ms.B, m, ms.coef, true);
                 if (t == 0) ms.flags[0] = use ? 1 : 0;
             }
         } else if (accel == 2 && m >= 2) {
@@ -1027,16 +1027,20 @@ __device__ __noinline__ void scf_uhf(const MolDesc& md, const BatchArgs& g, MolS
         err = sqrt(2.0 * wg_sum(en, s));
         const double* Fuse[2] = {Fs[0], Fs[1]};
         if (diis) {
+            // ring update; when full the cached Gram matrix moves one step: wavefront 0 reads the
+            // moved entries (lane l: entry (l / 8 + 1, l % 8 + 1)) before the barrier and writes
+            // them after it
+            const bool full = ms.hcount == kDiisDepth;
+            const int gi = (t & 63) >> 3, gj = t & 7;
+            const bool gmove = full && t < 64 && gi < kDiisDepth - 1 && gj < kDiisDepth - 1;
+            const double bsh = gmove ? ms.B[(gi + 1) * kDiisDepth + gj + 1] : 0.0;
+            __syncthreads();
             if (t == 0) {
-                if (ms.hcount < kDiisDepth) {
-                    ms.hcount++;
-                } else {
-                    ms.head = (ms.head + 1) % kDiisDepth;
-                    for (int i = 0; i < kDiisDepth - 1; ++i)
-                        for (int j = 0; j < kDiisDepth - 1; ++j) ms.B[i * kDiisDepth + j] = ms.B[(i + 1) * kDiisDepth + j + 1];
-                }
+                if (!full) ms.hcount++;
+                else ms.head = (ms.head + 1) % kDiisDepth;
                 for (int i = 0; i < ms.hcount; ++i) ms.ring[i] = (ms.head + i) % kDiisDepth;
             }
+            if (gmove) ms.B[gi * kDiisDepth + gj] = bsh;
             __syncthreads();
             const int m = ms.hcount, newest = m - 1;
             double part[kDiisDepth];
@@ -1047,39 +1051,32 @@ __device__ __noinline__ void scf_uhf(const MolDesc& md, const BatchArgs& g, MolS
 #pragma unroll
                 for (int i = 0; i < kDiisDepth; ++i) ep[i] = dv[c].e + (int64_t)ms.ring[i < m ? i : newest] * dv[c].len_a;
                 const double* en_p = ep[newest];
-#pragma unroll 2
+                // every slot pointer is valid (past the history: the newest), so all loads issue
+                // without branches and the selects discard the extra ones
+#pragma unroll 4
                 for (int64_t pk = t; pk < dv[c].len_a; pk += kWg) {
                     const double x = en_p[pk];
 #pragma unroll
-                    for (int i = 0; i < kDiisDepth; ++i)
-                        if (i < m) part[i] = fma(x, ep[i][pk], part[i]);
+                    for (int i = 0; i < kDiisDepth; ++i) {
+                        const double v = ep[i][pk];
+                        part[i] = fma(x, i < m ? v : 0.0, part[i]);
+                    }
                 }
             }
             double tot[kDiisDepth];
             wg_sum_many(part, lds, tot);
             if (t == 0) {
-                for (int i = 0; i < m; ++i) {
+#pragma unroll
+                for (int i = 0; i < kDiisDepth; ++i) {
+                    if (i >= m) continue;
                     ms.B[newest * kDiisDepth + i] = 2.0 * tot[i];
                     ms.B[i * kDiisDepth + newest] = 2.0 * tot[i];
                 }
-                int use = 0;
-                if (m >= 2) {
-                    double Bm[(kDiisDepth + 1) * (kDiisDepth + 1)], rhs[kDiisDepth + 1], x[kDiisDepth + 1];
-                    const int dim = m + 1;
-                    for (int i = 0; i < m; ++i) {
-                        for (int j = 0; j < m; ++j) Bm[i * dim + j] = ms.B[i * kDiisDepth + j];
-                        Bm[i * dim + m] = -1.0;
-                        Bm[m * dim + i] = -1.0;
-                        rhs[i] = 0.0;
-                    }
-                    Bm[m * dim + m] = 0.0;
-                    rhs[m] = -1.0;
-                    if (solve_bordered(Bm, rhs, dim, x)) {
-                        use = 1;
-                        for (int i = 0; i < m; ++i) ms.coef[i] = x[i];
-                    }
-                }
-                ms.flags[0] = use;
+            }
+            __syncthreads();
+            if (t < 64) {  // CDIIS on one wavefront (the serial solve's pivoting; any finite or not)
+                const bool use = m >= 2 && cdiis_solve_wave(ms.B, m, ms.coef, false);
+                if (t == 0) ms.flags[0] = use ? 1 : 0;
             }
             __syncthreads();
             if (ms.flags[0]) {
@@ -1092,16 +1089,28 @@ __device__ __noinline__ void scf_uhf(const MolDesc& md, const BatchArgs& g, MolS
                     const double* fq[kDiisDepth];
 #pragma unroll
                     for (int q = 0; q < kDiisDepth; ++q) fq[q] = dv[c].f + (int64_t)ms.ring[q < m ? q : m - 1] * dv[c].len_s;
-#pragma unroll 2
-                    for (int64_t pk = t; pk < dv[c].len_s; pk += kWg) {
-                        int i, j;
-                        tri_rowcol(pk, nao, i, j);
-                        double acc = 0.0;
+                    // four entries per thread per round, every load before any store
+                    constexpr int U = 4;
+                    for (int64_t base = t; base < dv[c].len_s; base += (int64_t)U * kWg) {
+                        double v[U][kDiisDepth];
 #pragma unroll
-                        for (int q = 0; q < kDiisDepth; ++q)
-                            if (q < m) acc = fma(cq[q], fq[q][pk], acc);
-                        Fs[c][(size_t)i * nao + j] = acc;
-                        Fs[c][(size_t)j * nao + i] = acc;
+                        for (int u = 0; u < U; ++u) {
+                            const int64_t pk = min(base + (int64_t)u * kWg, dv[c].len_s - 1);
+#pragma unroll
+                            for (int q = 0; q < kDiisDepth; ++q) v[u][q] = fq[q][pk];
+                        }
+#pragma unroll
+                        for (int u = 0; u < U; ++u) {
+                            const int64_t pk = base + (int64_t)u * kWg;
+                            if (pk >= dv[c].len_s) continue;
+                            double acc = 0.0;
+#pragma unroll
+                            for (int q = 0; q < kDiisDepth; ++q) acc = fma(cq[q], q < m ? v[u][q] : 0.0, acc);
+                            int i, j;
+                            tri_rowcol(pk, nao, i, j);
+                            Fs[c][(size_t)i * nao + j] = acc;
+                            Fs[c][(size_t)j * nao + i] = acc;
+                        }
                     }
                 }
                 __syncthreads();

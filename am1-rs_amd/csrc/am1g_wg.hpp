@@ -753,9 +753,10 @@ __host__ __device__ inline void adiis_solve(const double* g, int m, double* c) {
 // solve_bordered (Gaussian elimination with partial pivoting, the first largest pivot winning,
 // then back substitution) with lane r holding row r, so each column's eliminations run in
 // parallel and every lane takes the same pivot decisions. B: logical Gram matrix (row stride
-// kDiisDepth), m >= 2 entries. Returns false when a pivot falls below 1e-12 or the coefficients
-// are not finite; otherwise lane 0 writes c_out[0..m). Called by all 64 lanes of one wavefront.
-__device__ inline bool cdiis_solve_wave(const double* B, int m, double* c_out) {
+// kDiisDepth), m >= 2 entries. Returns false when a pivot falls below 1e-12 or, with
+// check_finite (the RHF loop's rule), when the coefficients are not finite; otherwise lane 0
+// writes c_out[0..m). Called by all 64 lanes of one wavefront.
+__device__ inline bool cdiis_solve_wave(const double* B, int m, double* c_out, bool check_finite) {
     constexpr int kD = kDiisDepth + 1;
     const int lane = threadIdx.x & 63;
     const int dim = m + 1;
@@ -821,7 +822,7 @@ __device__ inline bool cdiis_solve_wave(const double* B, int m, double* c_out) {
 #pragma unroll
     for (int i = 0; i < kDiisDepth; ++i)
         if (i < m) wsum += fabs(x[i]);
-    if (!isfinite(wsum)) return false;
+    if (check_finite && !isfinite(wsum)) return false;
     if (lane == 0) {
 #pragma unroll
         for (int i = 0; i < kDiisDepth; ++i)
