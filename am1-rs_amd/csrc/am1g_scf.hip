@@ -1297,60 +1297,87 @@ hipError_t launch_scf_batch(const BatchArgs& a, int precision, int grid, size_t 
 // reductions (the plain path and divide-and-conquer), plus the divide-and-conquer stages.
 namespace am1g {
 
-__device__ __forceinline__ BlockRef block_of_g(const SysArgs& a, int x, int y) {
-    const int nh = a.md.n_heavy, nH = a.md.n_h;
-    const int64_t hh = (int64_t)nh * (nh - 1) / 2, hH = (int64_t)nh * nH;
-    const bool hx = a.z[x] != 1, hy = a.z[y] != 1;
-    BlockRef r;
-    const int cx = a.acls[x], cy = a.acls[y];
-    if (hx && hy) {
-        r.w = a.wpool + tri_index(min(cx, cy), max(cx, cy), nh) * 100;
-        r.wcols = 10;
-        r.x_is_bra = cx < cy;
-    } else if (hx != hy) {
-        const int hi = hx ? cx : cy, mi = hx ? cy : cx;
-        r.w = a.wpool + hh * 100 + ((int64_t)hi * nH + mi) * 10;
-        r.wcols = 1;
-        r.x_is_bra = hx;
-    } else {
-        r.w = a.wpool + hh * 100 + hH * 10 + tri_index(min(cx, cy), max(cx, cy), nH);
-        r.wcols = 1;
-        r.x_is_bra = cx < cy;
-    }
-    return r;
-}
 
 
 // Partner sums of the diagonal blocks, one wavefront per atom x: lanes stride over the partners
 // y, each lane accumulates its partners' contributions to the 10 (heavy) or 1 (hydrogen) packed
 // entries, then a fixed butterfly sums the lanes. With `pd` null the partner term is the core
 // attraction -Z_y (mu nu | s_y s_y); otherwise the Coulomb term sum_q P_y[q] (mu nu | q_y).
+// Partners are walked by class (heavy atoms below x's index, above it, then hydrogens), so the
+// lanes of a wavefront take the same branch and each loop has one shape.
 __device__ __forceinline__ void diag_partner_sum(const SysArgs& a, int x, const double* pd_all, double (&acc)[10]) {
     const DevElem* el = a.elems + a.md.method * kMaxZ;
     const int lane = threadIdx.x & 63;
-    const int nk = a.z[x] == 1 ? 1 : 10;
+    const int nh = a.md.n_heavy, nH = a.md.n_h;
+    const int64_t n_hh = (int64_t)nh * (nh - 1) / 2, n_hH = (int64_t)nh * nH;
+    const double* wHh = a.wpool + n_hh * 100;  // heavy-hydrogen blocks, heavy atom as bra
+    const double* wHH = wHh + n_hH * 10;       // hydrogen-hydrogen scalars
+    const int cx = a.acls[x];
 #pragma unroll
     for (int k = 0; k < 10; ++k) acc[k] = 0.0;
-    for (int y = lane; y < a.md.nat; y += 64) {
-        if (y == x) continue;
-        const BlockRef br = block_of_g(a, x, y);
-        if (!pd_all || a.z[y] == 1) {
+    if (a.z[x] != 1) {
+        // heavy partners below x (x is the ket: column k of each block) and above (the bra: row k)
+        for (int iy = lane; iy < cx; iy += 64) {
+            const int y = a.alist[iy];
+            const double* w = a.wpool + tri_index(iy, cx, nh) * 100;
+            if (!pd_all) {
+                const double c = -el[a.z[y]].core;
+#pragma unroll
+                for (int k = 0; k < 10; ++k) acc[k] += c * w[k];
+            } else {
+#pragma unroll
+                for (int q = 0; q < 10; ++q) {
+                    const double pq = pd_all[y * 10 + q];
+#pragma unroll
+                    for (int k = 0; k < 10; ++k) acc[k] += pq * w[q * 10 + k];
+                }
+            }
+        }
+        for (int iy = cx + 1 + lane; iy < nh; iy += 64) {
+            const int y = a.alist[iy];
+            const double* w = a.wpool + tri_index(cx, iy, nh) * 100;
+            if (!pd_all) {
+                const double c = -el[a.z[y]].core;
+#pragma unroll
+                for (int k = 0; k < 10; ++k) acc[k] += c * w[k * 10];
+            } else {
+                double pd[10];
+#pragma unroll
+                for (int q = 0; q < 10; ++q) pd[q] = pd_all[y * 10 + q];
+#pragma unroll
+                for (int k = 0; k < 10; ++k) {
+                    double t = 0.0;
+#pragma unroll
+                    for (int q = 0; q < 10; ++q) t += pd[q] * w[k * 10 + q];
+                    acc[k] += t;
+                }
+            }
+        }
+        const double* wx = wHh + (int64_t)cx * nH * 10;
+        for (int m = lane; m < nH; m += 64) {
+            const int y = a.alist[nh + m];
             const double c = pd_all ? pd_all[y * 10] : -el[a.z[y]].core;
 #pragma unroll
-            for (int k = 0; k < 10; ++k)
-                if (k < nk) acc[k] += c * (br.x_is_bra ? br.w[k * br.wcols] : br.w[k]);
-        } else {
-            double pd[10];
-#pragma unroll
-            for (int q = 0; q < 10; ++q) pd[q] = pd_all[y * 10 + q];
-#pragma unroll
-            for (int k = 0; k < 10; ++k) {
-                if (k >= nk) continue;
+            for (int k = 0; k < 10; ++k) acc[k] += c * wx[(int64_t)m * 10 + k];
+        }
+    } else {
+        for (int iy = lane; iy < nh; iy += 64) {
+            const int y = a.alist[iy];
+            const double* w = wHh + ((int64_t)iy * nH + cx) * 10;
+            if (!pd_all) {
+                acc[0] += -el[a.z[y]].core * w[0];
+            } else {
                 double t = 0.0;
 #pragma unroll
-                for (int q = 0; q < 10; ++q) t += pd[q] * (br.x_is_bra ? br.w[k * br.wcols + q] : br.w[q * br.wcols + k]);
-                acc[k] += t;
+                for (int q = 0; q < 10; ++q) t += pd_all[y * 10 + q] * w[q];
+                acc[0] += t;
             }
+        }
+        for (int m = lane; m < nH; m += 64) {
+            if (m == cx) continue;
+            const int y = a.alist[nh + m];
+            const double c = pd_all ? pd_all[y * 10] : -el[a.z[y]].core;
+            acc[0] += c * wHH[tri_index(min(m, cx), max(m, cx), nH)];
         }
     }
 #pragma unroll
