@@ -783,7 +783,7 @@ __device__ __noinline__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolS
             status = kScfDiverged;
             break;
         }
-        // extrapolation coefficients, scf.rs:627-653: CDIIS by thread 0, A-DIIS on wavefront 0
+        // extrapolation coefficients, scf.rs:627-653: CDIIS and A-DIIS each on wavefront 0
         const bool cdiis = accel == 1 || (accel == 2 && err <= md.adiis_switch);
         if (cdiis) {
             if (t < 64) {
