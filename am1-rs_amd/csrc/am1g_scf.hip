@@ -1452,7 +1452,10 @@ __global__ __launch_bounds__(256) void k_fock_diag(SysArgs a, const double* Pt, 
     F[(size_t)(ox + nu) * nao + ox + mu] = f;
 }
 
-// Fock off-diagonal blocks: H - K, one atom pair per thread.
+// Fock off-diagonal blocks: H - K, one atom pair per thread. Each pair writes its block in the
+// upper triangle only (atom AO ranges are disjoint, so a block lies wholly on one side), reading
+// P and H there as well; k_sym_lower then copies the upper triangle to the lower in coalesced
+// tiles. Writing both triangles here cost a strided pass over the matrix per side.
 __global__ __launch_bounds__(256) void k_fock_off(SysArgs a, const double* Ps, double scale, double* F) {
     const int nh = a.md.n_heavy, nH = a.md.n_h, nao = a.md.nao;
     const int64_t n_hh = (int64_t)nh * (nh - 1) / 2, n_hH = (int64_t)nh * nH, n_HH = (int64_t)nH * (nH - 1) / 2;
@@ -1462,13 +1465,19 @@ __global__ __launch_bounds__(256) void k_fock_off(SysArgs a, const double* Ps, d
             int i, j;
             tri_decode(p, nh, i, j);
             const int oa = a.aoff[a.alist[i]], ob = a.aoff[a.alist[j]];
+            const bool up = oa < ob;
+            // entry (mu, la) of the block (rows on atom i, columns on atom j) in the upper triangle
+            auto at = [&](int mu, int la) -> size_t {
+                return up ? (size_t)(oa + mu) * nao + ob + la : (size_t)(ob + la) * nao + oa + mu;
+            };
             const double* w = a.wpool + p * 100;
-            double pab[16], acc[16];
+            double pab[16], acc[16], hv[16];
 #pragma unroll
             for (int u = 0; u < 4; ++u)
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
-                    pab[u * 4 + v] = scale * Ps[(size_t)(oa + u) * nao + ob + v];
+                    pab[u * 4 + v] = scale * Ps[at(u, v)];
+                    hv[u * 4 + v] = a.H[at(u, v)];
                     acc[u * 4 + v] = 0.0;
                 }
             for (int kb = 0; kb < 10; ++kb) {
@@ -1492,33 +1501,62 @@ __global__ __launch_bounds__(256) void k_fock_off(SysArgs a, const double* Ps, d
 #pragma unroll
             for (int mu = 0; mu < 4; ++mu)
 #pragma unroll
-                for (int la = 0; la < 4; ++la) {
-                    F[(size_t)(oa + mu) * nao + ob + la] = a.H[(size_t)(oa + mu) * nao + ob + la] - acc[mu * 4 + la];
-                    F[(size_t)(ob + la) * nao + oa + mu] = a.H[(size_t)(ob + la) * nao + oa + mu] - acc[mu * 4 + la];
-                }
+                for (int la = 0; la < 4; ++la) F[at(mu, la)] = hv[mu * 4 + la] - acc[mu * 4 + la];
         } else if (p < n_hh + n_hH) {
             const int64_t k = p - n_hh;
             const int oa = a.aoff[a.alist[(int)(k / nH)]], ob = a.aoff[a.alist[nh + (int)(k % nH)]];
+            const bool up = oa < ob;
             const double* w = a.wpool + n_hh * 100 + k * 10;
-            double pab[4];
+            double pab[4], hv[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) pab[u] = scale * Ps[(size_t)(oa + u) * nao + ob];
+            for (int u = 0; u < 4; ++u) {
+                const size_t ix = up ? (size_t)(oa + u) * nao + ob : (size_t)ob * nao + oa + u;
+                pab[u] = scale * Ps[ix];
+                hv[u] = a.H[ix];
+            }
 #pragma unroll
             for (int mu = 0; mu < 4; ++mu) {
                 double acc = 0.0;
 #pragma unroll
                 for (int nu = 0; nu < 4; ++nu) acc += pab[nu] * w[pack4(mu, nu)];
-                F[(size_t)(oa + mu) * nao + ob] = a.H[(size_t)(oa + mu) * nao + ob] - acc;
-                F[(size_t)ob * nao + oa + mu] = a.H[(size_t)ob * nao + oa + mu] - acc;
+                F[up ? (size_t)(oa + mu) * nao + ob : (size_t)ob * nao + oa + mu] = hv[mu] - acc;
             }
         } else {
             int i, j;
             tri_decode(p - n_hh - n_hH, nH, i, j);
             const int oa = a.aoff[a.alist[nh + i]], ob = a.aoff[a.alist[nh + j]];
-            const double acc = scale * Ps[(size_t)oa * nao + ob] * a.wpool[p - n_hh - n_hH + n_hh * 100 + n_hH * 10];
-            F[(size_t)oa * nao + ob] = a.H[(size_t)oa * nao + ob] - acc;
-            F[(size_t)ob * nao + oa] = a.H[(size_t)ob * nao + oa] - acc;
+            const size_t ix = oa < ob ? (size_t)oa * nao + ob : (size_t)ob * nao + oa;
+            const double acc = scale * Ps[ix] * a.wpool[p - n_hh - n_hH + n_hh * 100 + n_hH * 10];
+            F[ix] = a.H[ix] - acc;
         }
+    }
+}
+
+// F[i][j] = F[j][i] for i > j, in 32 x 32 tiles through LDS (grid: the tiles on and below the
+// diagonal, tile (bi, bj) with bi >= bj; 32 x 8 threads, four rows each).
+__global__ __launch_bounds__(256) void k_sym_lower(int n, double* F) {
+    __shared__ double tile[32][33];
+    const int nt = (n + 31) / 32;
+    // linear block index -> (bi, bj), bj <= bi
+    const int64_t lb = blockIdx.x;
+    int bi = (int)((sqrt(8.0 * (double)lb + 1.0) - 1.0) * 0.5);
+    while ((int64_t)bi * (bi + 1) / 2 > lb) --bi;
+    while ((int64_t)(bi + 1) * (bi + 2) / 2 <= lb) ++bi;
+    const int bj = (int)(lb - (int64_t)bi * (bi + 1) / 2);
+    if (bi >= nt) return;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    // read the upper tile (rows of block bj, columns of block bi)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int row = bj * 32 + ty + 8 * r, col = bi * 32 + tx;
+        if (row < n && col < n && row < col) tile[ty + 8 * r][tx] = F[(size_t)row * n + col];
+    }
+    __syncthreads();
+    // write its transpose (rows of block bi, columns of block bj) below the diagonal
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int row = bi * 32 + ty + 8 * r, col = bj * 32 + tx;
+        if (row < n && col < n && row > col) F[(size_t)row * n + col] = tile[tx][ty + 8 * r];
     }
 }
 
@@ -1949,6 +1987,8 @@ hipError_t launch_fock(const SysArgs& a, const double* Pt, const double* Ps, dou
         int64_t b = blocks_for(np);
         if (b > 65536) b = 65536;
         hipLaunchKernelGGL(k_fock_off, dim3((unsigned)b), dim3(256), 0, st, a, Ps, scale, F);
+        const int64_t nt = ((int64_t)a.md.nao + 31) / 32;
+        hipLaunchKernelGGL(k_sym_lower, dim3((unsigned)(nt * (nt + 1) / 2)), dim3(256), 0, st, a.md.nao, F);
     }
     return hipGetLastError();
 }
