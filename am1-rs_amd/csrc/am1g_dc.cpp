@@ -77,6 +77,65 @@ bool lu_solve(std::vector<double> a, std::vector<double> b, int n, std::vector<d
     return true;
 }
 
+// Uniform grid of cubic cells of side at least `cell` (the buffer radius) over the atoms, so the
+// atoms within that distance of an atom are found among the 27 cells around it. The side grows
+// when the grid would have more than 8 cells per atom (a very large radius ends in one cell).
+struct CellGrid {
+    double lo[3], side;
+    int dim[3];
+    std::vector<int> start, atoms;  // atoms of cell k: atoms[start[k] .. start[k + 1])
+    int cell_of(const double* p) const {
+        int c[3];
+        for (int k = 0; k < 3; ++k) c[k] = std::min(dim[k] - 1, std::max(0, (int)((p[k] - lo[k]) / side)));
+        return (c[2] * dim[1] + c[1]) * dim[0] + c[0];
+    }
+    CellGrid(const double* xyz, int nat, double cell) {
+        double hi[3];
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = INFINITY;
+            hi[k] = -INFINITY;
+        }
+        for (int a = 0; a < nat; ++a)
+            for (int k = 0; k < 3; ++k) {
+                lo[k] = std::min(lo[k], xyz[3 * a + k]);
+                hi[k] = std::max(hi[k], xyz[3 * a + k]);
+            }
+        side = std::max(cell, 1.0e-6);
+        for (;;) {
+            double cells = 1.0;
+            for (int k = 0; k < 3; ++k) {
+                dim[k] = (int)std::min(1.0e6, std::floor((hi[k] - lo[k]) / side) + 1.0);
+                cells *= dim[k];
+            }
+            if (cells <= 8.0 * std::max(nat, 1)) break;
+            side *= 2.0;
+        }
+        const int nc = dim[0] * dim[1] * dim[2];
+        start.assign(nc + 1, 0);
+        std::vector<int> cid(nat);
+        for (int a = 0; a < nat; ++a) {
+            cid[a] = cell_of(xyz + 3 * a);
+            ++start[cid[a] + 1];
+        }
+        for (int k = 0; k < nc; ++k) start[k + 1] += start[k];
+        atoms.resize(nat);
+        std::vector<int> fill(start.begin(), start.end() - 1);
+        for (int a = 0; a < nat; ++a) atoms[fill[cid[a]]++] = a;  // ascending within a cell
+    }
+    // calls f(b) for every atom b in the 27 cells around point p
+    template <typename Fn>
+    void around(const double* p, Fn&& f) const {
+        int c[3];
+        for (int k = 0; k < 3; ++k) c[k] = std::min(dim[k] - 1, std::max(0, (int)((p[k] - lo[k]) / side)));
+        for (int z = std::max(0, c[2] - 1); z <= std::min(dim[2] - 1, c[2] + 1); ++z)
+            for (int y = std::max(0, c[1] - 1); y <= std::min(dim[1] - 1, c[1] + 1); ++y)
+                for (int x = std::max(0, c[0] - 1); x <= std::min(dim[0] - 1, c[0] + 1); ++x) {
+                    const int k = (z * dim[1] + y) * dim[0] + x;
+                    for (int q = start[k]; q < start[k + 1]; ++q) f(atoms[q]);
+                }
+    }
+};
+
 }  // namespace
 
 void am1g_dc_release(am1g_ctx* ctx) {
@@ -230,14 +289,27 @@ int am1g_dc(am1g_ctx* ctx, const am1g_request* rq, const am1g_dc_options* op, am
     int64_t mat_total = 0;
     double diag_work = 0.0;
     int largest = 0;
+    // subsystem c: its core atoms and every atom within the buffer radius of one of them, in
+    // ascending atom order (neighbours from a cell grid instead of all atom pairs)
+    const CellGrid grid(xyz.data(), nat, op->buffer_radius);
+    std::vector<int> stamp(nat, -1), cand;
+    std::vector<char> in_core(nat, 0);
     for (int c = 0; c < n_sub; ++c) {
-        std::vector<char> in_core(nat, 0);
         for (int a : cores[c]) {
             in_core[a] = 1;
             core_sub[a] = c;
         }
+        cand.clear();
+        for (int a : cores[c])
+            grid.around(xyz.data() + 3 * a, [&](int j) {
+                if (stamp[j] != c) {
+                    stamp[j] = c;
+                    cand.push_back(j);
+                }
+            });
+        std::sort(cand.begin(), cand.end());
         int n = 0;
-        for (int j = 0; j < nat; ++j) {
+        for (int j : cand) {
             bool take = in_core[j];
             for (size_t q = 0; !take && q < cores[c].size(); ++q) take = d2(cores[c][q], j) <= r2;
             if (!take) continue;
@@ -258,6 +330,7 @@ int am1g_dc(am1g_ctx* ctx, const am1g_request* rq, const am1g_dc_options* op, am
         mat_total += (int64_t)n * n;
         diag_work += (double)n * n * n;
         largest = std::max(largest, n);
+        for (int a : cores[c]) in_core[a] = 0;
     }
     for (int a = 0; a < nat; ++a)
         if (core_sub[a] < 0) return res->status = am1g_set_error(ctx, AM1G_INVALID_INPUT, "atom %d has no core", a);
@@ -265,9 +338,13 @@ int am1g_dc(am1g_ctx* ctx, const am1g_request* rq, const am1g_dc_options* op, am
     // retained pattern: AO pairs (r <= c) of atom pairs within the buffer radius
     std::vector<int64_t> pat;
     std::vector<double> mult;
-    for (int a = 0; a < nat; ++a)
-        for (int b = a; b < nat; ++b) {
-            if (d2(a, b) > r2) continue;
+    std::vector<int> nbr;
+    for (int a = 0; a < nat; ++a) {
+        nbr.clear();
+        grid.around(xyz.data() + 3 * a, [&](int b) {
+            if (b >= a && d2(a, b) <= r2) nbr.push_back(b);
+        });
+        for (int b : nbr) {
             for (int i = 0; i < tab[zs[a]].norb; ++i)
                 for (int j = 0; j < tab[zs[b]].norb; ++j) {
                     int r = aoff[a] + i, c = aoff[b] + j;
@@ -276,6 +353,7 @@ int am1g_dc(am1g_ctx* ctx, const am1g_request* rq, const am1g_dc_options* op, am
                     pat.push_back((int64_t)r * nao + c);
                 }
         }
+    }
     std::sort(pat.begin(), pat.end());
     pat.erase(std::unique(pat.begin(), pat.end()), pat.end());
     mult.resize(pat.size());
