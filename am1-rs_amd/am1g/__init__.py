@@ -11,6 +11,7 @@ library or a GPU is missing, loading fails with an error that says so.
 from __future__ import annotations
 
 import ctypes
+import time
 import math
 import os
 import subprocess
@@ -374,7 +375,10 @@ class Engine:
                 opts.update(mol[2])
             rqs[i] = self._request(z, xyz, keep, **opts)
             arrays.append(self._attach(rss[i], len(z), full, keep))
+        t0 = time.perf_counter()
         st = self._L.am1g_scf_batch(self._ctx, n, rqs, rss)
+        # the C call alone (host tables, copies, launches, result fill): what a native caller waits
+        self.last_call_ms = (time.perf_counter() - t0) * 1e3
         if st != 0:
             raise RuntimeError(f"am1g_scf_batch failed ({STATUS_NAMES.get(st, st)}): {self.last_error()}")
         return [self._to_dict(rss[i], arrays[i], full) for i in range(n)]

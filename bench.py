@@ -133,12 +133,13 @@ def main():
     for w in range(args.warmup):
         eng.scf_batch(ms)
         log(f"warmup {w + 1}/{args.warmup}: kernel {eng.last_kernel_ms():.1f} ms")
-    walls, kernels = [], []
+    walls, calls, kernels = [], [], []
     res = None
     for s in range(args.steps):
         t = time.perf_counter()
         res = eng.scf_batch(ms)
         walls.append(time.perf_counter() - t)
+        calls.append(eng.last_call_ms)
         kernels.append(eng.last_kernel_ms())
         log(f"step {s + 1}/{args.steps}: kernel {kernels[-1]:.1f} ms, wall {walls[-1] * 1e3:.1f} ms")
     kernel_s = float(np.mean(kernels)) / 1e3
@@ -293,7 +294,9 @@ def main():
                     "converged_verified": n_ok, "max_abs_dE_ev": max_de,
                     "mean_iterations": float(np.mean([r["iterations"] for r in res])),
                     "wall_ms_per_step": round(wall_s * 1e3, 3),
-                    "wall_inclusive_scfs_per_s": round(n_ok / wall_s, 2)},
+                    "wall_inclusive_scfs_per_s": round(n_ok / wall_s, 2),
+                    "c_call_ms_per_step": round(float(np.mean(calls)), 3),
+                    "c_call_inclusive_scfs_per_s": round(n_ok / (float(np.mean(calls)) / 1e3), 2)},
         "plain": {"value": round(plain_rate, 2), "sample": len(idx), "p50_ms": percentile(plain_ms, 50),
                   "p99_ms": percentile(plain_ms, 99)},
         "batched_over_plain": round(value / plain_rate, 2) if plain_rate > 0 else None,
