@@ -8,6 +8,7 @@
 #include <functional>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "../../include/am1g.h"
 #include "am1g_common.hpp"
@@ -72,7 +73,7 @@ struct am1g_ctx {
     std::string err;
     DevElem host_tab[kMethods * kMaxZ];
     DevBuf elems, mols, z, xyz, mats, wpool, hist, mo_e, mo_eb, charges, results, queue, order;
-    DevBuf alist, aoff, acls, pair_off, ecore_pair;
+    DevBuf alist, aoff, acls, later, pair_off, ecore_pair;
     HostBuf h_done, h_results, h_charges, h_mo, h_mob;  // streaming outputs of the batched launch
     struct DcBuffers* dc = nullptr;
     struct LargeBuffers* large = nullptr;  // one large system: dense matrices, DIIS history
@@ -118,4 +119,22 @@ int am1g_large_scf(am1g_ctx* ctx, const am1g_request& rq, const Prepared& prep, 
         hipError_t e_ = (expr);                                                                               \
         if (e_ != hipSuccess)                                                                                 \
             return am1g_set_error(ctx, AM1G_DEVICE_ERROR, "HIP error %s at %s", hipGetErrorString(e_), #expr); \
-    } while (0)
+    } while (0)// Per class-list entry, where the other class's partners later in AO order begin: for heavy atom
+// j (entry j), the first hydrogen-list index whose AO offset exceeds j's; for hydrogen m (entry
+// n_heavy + m), the first heavy-list index whose AO offset exceeds m's. Both lists keep the input
+// order, so AO offsets grow along each.
+inline std::vector<int> later_partners(const std::vector<int>& alist, const std::vector<int>& aoff, int nh) {
+    const int nat = (int)alist.size(), nH = nat - nh;
+    std::vector<int> out(nat);
+    for (int j = 0, m = 0; j < nh; ++j) {
+        while (m < nH && aoff[alist[nh + m]] < aoff[alist[j]]) ++m;
+        out[j] = m;
+    }
+    for (int m = 0, j = 0; m < nH; ++m) {
+        while (j < nh && aoff[alist[j]] < aoff[alist[nh + m]]) ++j;
+        out[nh + m] = j;
+    }
+    return out;
+}
+
+

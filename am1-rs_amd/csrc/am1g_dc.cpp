@@ -206,6 +206,7 @@ int am1g_dc(am1g_ctx* ctx, const am1g_request* rq, const am1g_dc_options* op, am
     HIP_TRY(ctx, ctx->alist.reserve(sizeof(int) * nat));
     HIP_TRY(ctx, ctx->aoff.reserve(sizeof(int) * nat));
     HIP_TRY(ctx, ctx->acls.reserve(sizeof(int) * nat));
+    HIP_TRY(ctx, ctx->later.reserve(sizeof(int) * nat));
     HIP_TRY(ctx, ctx->pair_off.reserve(sizeof(int64_t) * 2));
     HIP_TRY(ctx, ctx->ecore_pair.reserve(sizeof(double) * std::max<int64_t>(npairs, 1)));
     HIP_TRY(ctx, ctx->wpool.reserve(sizeof(double) * std::max<int64_t>(w_pool_size(md.n_heavy, md.n_h), 1)));
@@ -216,6 +217,8 @@ int am1g_dc(am1g_ctx* ctx, const am1g_request* rq, const am1g_dc_options* op, am
     HIP_TRY(ctx, hipMemcpyAsync(ctx->alist.p, alist.data(), sizeof(int) * nat, hipMemcpyHostToDevice, s));
     HIP_TRY(ctx, hipMemcpyAsync(ctx->aoff.p, aoff.data(), sizeof(int) * nat, hipMemcpyHostToDevice, s));
     HIP_TRY(ctx, hipMemcpyAsync(ctx->acls.p, acls.data(), sizeof(int) * nat, hipMemcpyHostToDevice, s));
+    const std::vector<int> later = later_partners(alist, aoff, md.n_heavy);
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->later.p, later.data(), sizeof(int) * nat, hipMemcpyHostToDevice, s));
     HIP_TRY(ctx, hipMemcpyAsync(ctx->pair_off.p, pair_off.data(), sizeof(int64_t) * 2, hipMemcpyHostToDevice, s));
 
     BatchArgs ba;
@@ -242,6 +245,7 @@ int am1g_dc(am1g_ctx* ctx, const am1g_request* rq, const am1g_dc_options* op, am
     sa.alist = ba.alist;
     sa.aoff = ba.aoff;
     sa.acls = ba.acls;
+    sa.later = ctx->later.as<int>();
     sa.elems = ba.elems;
     sa.wpool = ba.wpool;
     sa.H = ba.mats;
@@ -668,9 +672,10 @@ int am1g_dc(am1g_ctx* ctx, const am1g_request* rq, const am1g_dc_options* op, am
         eiso += tab[zs[a]].e_isol;
         eheat += tab[zs[a]].eheat_ev;
         // Mulliken population from the packed diagonal block of the total density (entries 0,
-        // 2, 5, 9 are the diagonal)
-        double pop = pd[10 * a];
-        if (zs[a] != 1) pop += pd[10 * a + 2] + pd[10 * a + 5] + pd[10 * a + 9];
+        // 2, 5, 9 are the diagonal; rows in class order, heavy atoms first)
+        const double* row = pd.data() + 10 * (size_t)(zs[a] == 1 ? md.n_heavy + acls[a] : acls[a]);
+        double pop = row[0];
+        if (zs[a] != 1) pop += row[2] + row[5] + row[9];
         if (res->charges) res->charges[a] = tab[zs[a]].core - pop;
     }
     double gap = INFINITY;  // smallest_gap per channel (divide_conquer.rs:659-683), the smaller one

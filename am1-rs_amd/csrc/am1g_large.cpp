@@ -373,6 +373,7 @@ int am1g_large_scf(am1g_ctx* ctx, const am1g_request& rq, const Prepared& prep, 
     HIP_TRY(ctx, ctx->alist.reserve(sizeof(int) * nat));
     HIP_TRY(ctx, ctx->aoff.reserve(sizeof(int) * nat));
     HIP_TRY(ctx, ctx->acls.reserve(sizeof(int) * nat));
+    HIP_TRY(ctx, ctx->later.reserve(sizeof(int) * nat));
     HIP_TRY(ctx, ctx->pair_off.reserve(sizeof(int64_t) * 2));
     HIP_TRY(ctx, ctx->ecore_pair.reserve(sizeof(double) * std::max<int64_t>(npairs, 1)));
     HIP_TRY(ctx, ctx->wpool.reserve(sizeof(double) * std::max<int64_t>(w_pool_size(md.n_heavy, md.n_h), 1)));
@@ -383,6 +384,8 @@ int am1g_large_scf(am1g_ctx* ctx, const am1g_request& rq, const Prepared& prep, 
     HIP_TRY(ctx, hipMemcpyAsync(ctx->alist.p, alist.data(), sizeof(int) * nat, hipMemcpyHostToDevice, s));
     HIP_TRY(ctx, hipMemcpyAsync(ctx->aoff.p, aoff.data(), sizeof(int) * nat, hipMemcpyHostToDevice, s));
     HIP_TRY(ctx, hipMemcpyAsync(ctx->acls.p, acls.data(), sizeof(int) * nat, hipMemcpyHostToDevice, s));
+    const std::vector<int> later = later_partners(alist, aoff, md.n_heavy);
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->later.p, later.data(), sizeof(int) * nat, hipMemcpyHostToDevice, s));
     HIP_TRY(ctx, hipMemcpyAsync(ctx->pair_off.p, pair_off.data(), sizeof(int64_t) * 2, hipMemcpyHostToDevice, s));
     BatchArgs ba;
     std::memset(&ba, 0, sizeof(ba));
@@ -407,6 +410,7 @@ int am1g_large_scf(am1g_ctx* ctx, const am1g_request& rq, const Prepared& prep, 
     sa.alist = ba.alist;
     sa.aoff = ba.aoff;
     sa.acls = ba.acls;
+    sa.later = ctx->later.as<int>();
     sa.elems = ba.elems;
     sa.wpool = ba.wpool;
     sa.H = L.H.as<double>();

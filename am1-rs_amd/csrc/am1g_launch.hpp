@@ -72,10 +72,12 @@ struct SysArgs {
     const int* alist;    // heavy atoms then hydrogens
     const int* aoff;     // AO offset per atom
     const int* acls;     // index inside the heavy / hydrogen list
+    const int* later;    // [nat] class order: where the other class's partners later in AO order begin
     const DevElem* elems;
     const double* wpool;
     double* H;           // [nao][nao]
-    double* pdiag;       // [nat][10] packed diagonal density blocks (off-diagonal entries doubled)
+    double* pdiag;       // [nat][10] packed diagonal density blocks (off-diagonal entries doubled),
+                         // rows in class order: heavy atoms, then hydrogens
 };
 
 // Divide-and-conquer subsystem tables (all device pointers).

@@ -1299,148 +1299,26 @@ namespace am1g {
 
 
 
-// Partner sums of the diagonal blocks, one wavefront per atom x: lanes stride over the partners
-// y, each lane accumulates its partners' contributions to the 10 (heavy) or 1 (hydrogen) packed
-// entries, then a fixed butterfly sums the lanes. With `pd` null the partner term is the core
-// attraction -Z_y (mu nu | s_y s_y); otherwise the Coulomb term sum_q P_y[q] (mu nu | q_y).
-// Partners are walked by class (heavy atoms below x's index, above it, then hydrogens), so the
-// lanes of a wavefront take the same branch and each loop has one shape.
-__device__ __forceinline__ void diag_partner_sum(const SysArgs& a, int x, const double* pd_all, double (&acc)[10]) {
-    const DevElem* el = a.elems + a.md.method * kMaxZ;
-    const int lane = threadIdx.x & 63;
-    const int nh = a.md.n_heavy, nH = a.md.n_h;
-    const int64_t n_hh = (int64_t)nh * (nh - 1) / 2, n_hH = (int64_t)nh * nH;
-    const double* wHh = a.wpool + n_hh * 100;  // heavy-hydrogen blocks, heavy atom as bra
-    const double* wHH = wHh + n_hH * 10;       // hydrogen-hydrogen scalars
-    const int cx = a.acls[x];
-#pragma unroll
-    for (int k = 0; k < 10; ++k) acc[k] = 0.0;
-    if (a.z[x] != 1) {
-        // heavy partners below x (x is the ket: column k of each block) and above (the bra: row k)
-        for (int iy = lane; iy < cx; iy += 64) {
-            const int y = a.alist[iy];
-            const double* w = a.wpool + tri_index(iy, cx, nh) * 100;
-            if (!pd_all) {
-                const double c = -el[a.z[y]].core;
-#pragma unroll
-                for (int k = 0; k < 10; ++k) acc[k] += c * w[k];
-            } else {
-#pragma unroll
-                for (int q = 0; q < 10; ++q) {
-                    const double pq = pd_all[y * 10 + q];
-#pragma unroll
-                    for (int k = 0; k < 10; ++k) acc[k] += pq * w[q * 10 + k];
-                }
-            }
-        }
-        for (int iy = cx + 1 + lane; iy < nh; iy += 64) {
-            const int y = a.alist[iy];
-            const double* w = a.wpool + tri_index(cx, iy, nh) * 100;
-            if (!pd_all) {
-                const double c = -el[a.z[y]].core;
-#pragma unroll
-                for (int k = 0; k < 10; ++k) acc[k] += c * w[k * 10];
-            } else {
-                double pd[10];
-#pragma unroll
-                for (int q = 0; q < 10; ++q) pd[q] = pd_all[y * 10 + q];
-#pragma unroll
-                for (int k = 0; k < 10; ++k) {
-                    double t = 0.0;
-#pragma unroll
-                    for (int q = 0; q < 10; ++q) t += pd[q] * w[k * 10 + q];
-                    acc[k] += t;
-                }
-            }
-        }
-        const double* wx = wHh + (int64_t)cx * nH * 10;
-        for (int m = lane; m < nH; m += 64) {
-            const int y = a.alist[nh + m];
-            const double c = pd_all ? pd_all[y * 10] : -el[a.z[y]].core;
-#pragma unroll
-            for (int k = 0; k < 10; ++k) acc[k] += c * wx[(int64_t)m * 10 + k];
-        }
-    } else {
-        for (int iy = lane; iy < nh; iy += 64) {
-            const int y = a.alist[iy];
-            const double* w = wHh + ((int64_t)iy * nH + cx) * 10;
-            if (!pd_all) {
-                acc[0] += -el[a.z[y]].core * w[0];
-            } else {
-                double t = 0.0;
-#pragma unroll
-                for (int q = 0; q < 10; ++q) t += pd_all[y * 10 + q] * w[q];
-                acc[0] += t;
-            }
-        }
-        for (int m = lane; m < nH; m += 64) {
-            if (m == cx) continue;
-            const int y = a.alist[nh + m];
-            const double c = pd_all ? pd_all[y * 10] : -el[a.z[y]].core;
-            acc[0] += c * wHH[tri_index(min(m, cx), max(m, cx), nH)];
-        }
-    }
-#pragma unroll
-    for (int k = 0; k < 10; ++k)
-        for (int off = 32; off > 0; off >>= 1) acc[k] += __shfl_xor(acc[k], off, 64);
-}
-
-// H_core diagonal blocks: U + sum over partners y of (-Z_y)(mu nu | s_y s_y).
-__global__ __launch_bounds__(256) void k_hcore_diag(SysArgs a) {
-    const int x = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (x >= a.md.nat) return;
-    double acc[10];
-    diag_partner_sum(a, x, nullptr, acc);
-    const int k = threadIdx.x & 63;
+// Diagonal blocks of H_core and of the Fock matrix. Each block needs a sum over every partner y
+// of y's 10 packed factors (pdiag, rows in class order: heavy atoms, then hydrogens) against the
+// pair's integrals: the Coulomb term
+// sum_q P_y[q] (mu nu | q_y) for the Fock matrix, or the core attraction -Z_y (mu nu | s_y s_y)
+// when pdiag holds (-Z_y, 0, ..., 0). Workgroups 0..n_heavy-1 take one heavy atom each (its class
+// index); the rest take 8 consecutive hydrogens of the hydrogen list. Lanes read consecutive
+// doubles of the pool, so one load instruction covers a partner's whole 100-entry block (heavy
+// atom) or the 8 hydrogens' 80 contiguous entries against one heavy partner (hydrogen group);
+// the four wavefronts split the partners and meet in LDS.
+__device__ __forceinline__ void diag_block_store(const SysArgs& a, int x, int k, double v, bool fock,
+                                                 const double* Pt, const double* Ps, double scale, double* F) {
     const DevElem& ex = a.elems[a.md.method * kMaxZ + a.z[x]];
-    if (k >= (ex.norb == 4 ? 10 : 1)) return;
-    double v = acc[0];
-#pragma unroll
-    for (int q = 1; q < 10; ++q)
-        if (q == k) v = acc[q];
-    const int nao = a.md.nao;
+    const int nx = ex.norb, nao = a.md.nao, ox = a.aoff[x];
     const int mu = kPackRow[k], nu = kPackCol[k];
-    const double u = (mu == nu) ? (mu == 0 ? ex.uss : ex.upp) : 0.0;
-    const int o = a.aoff[x];
-    a.H[(size_t)(o + mu) * nao + o + nu] = u + v;
-    a.H[(size_t)(o + nu) * nao + o + mu] = u + v;
-}
-
-// Packed diagonal density blocks, off-diagonal entries doubled (the 2 of the symmetric sum).
-__global__ __launch_bounds__(256) void k_pack_diag(SysArgs a, const double* Pt) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= a.md.nat) return;
-    const int nao = a.md.nao, o = a.aoff[x];
-    if (a.z[x] == 1) {
-        a.pdiag[x * 10] = Pt[(size_t)o * nao + o];
+    if (!fock) {
+        const double u = (mu == nu) ? (mu == 0 ? ex.uss : ex.upp) : 0.0;
+        a.H[(size_t)(ox + mu) * nao + ox + nu] = u + v;
+        a.H[(size_t)(ox + nu) * nao + ox + mu] = u + v;
         return;
     }
-#pragma unroll
-    for (int q = 0; q < 10; ++q) {
-        const int l = kPackRow[q], c = kPackCol[q];
-        a.pdiag[x * 10 + q] = Pt[(size_t)(o + l) * nao + o + c] * (l == c ? 1.0 : 2.0);
-    }
-}
-
-// Fock diagonal blocks: H + one-centre terms + Coulomb from every partner's diagonal block,
-// one wavefront per atom.
-__global__ __launch_bounds__(256) void k_fock_diag(SysArgs a, const double* Pt, const double* Ps, double scale,
-                                                   double* F) {
-    const int x = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (x >= a.md.nat) return;
-    double acc[10];
-    diag_partner_sum(a, x, a.pdiag, acc);
-    const int k = threadIdx.x & 63;
-    const DevElem& ex = a.elems[a.md.method * kMaxZ + a.z[x]];
-    const int nx = ex.norb;
-    if (k >= (nx == 4 ? 10 : 1)) return;
-    double v = acc[0];
-#pragma unroll
-    for (int q = 1; q < 10; ++q)
-        if (q == k) v = acc[q];
-    const int nao = a.md.nao;
-    const int ox = a.aoff[x];
-    const int mu = kPackRow[k], nu = kPackCol[k];
     double oc = 0.0;
     for (int la = 0; la < nx; ++la)
         for (int si = 0; si < nx; ++si) {
@@ -1452,83 +1330,281 @@ __global__ __launch_bounds__(256) void k_fock_diag(SysArgs a, const double* Pt, 
     F[(size_t)(ox + nu) * nao + ox + mu] = f;
 }
 
-// Fock off-diagonal blocks: H - K, one atom pair per thread. Each pair writes its block in the
-// upper triangle only (atom AO ranges are disjoint, so a block lies wholly on one side), reading
-// P and H there as well; k_sym_lower then copies the upper triangle to the lower in coalesced
-// tiles. Writing both triangles here cost a strided pass over the matrix per side.
-__global__ __launch_bounds__(256) void k_fock_off(SysArgs a, const double* Ps, double scale, double* F) {
+template <bool kFock>
+__global__ __launch_bounds__(256) void k_diag_blocks(SysArgs a, const double* Pt, const double* Ps, double scale,
+                                                     double* F) {
+    __shared__ double red[4][260];
+    const int nh = a.md.n_heavy, nH = a.md.n_h;
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t n_hh = (int64_t)nh * (nh - 1) / 2, n_hH = (int64_t)nh * nH;
+    const double* wHh = a.wpool + n_hh * 100;  // heavy-hydrogen blocks, heavy atom as bra
+    const double* wHH = wHh + n_hH * 10;       // hydrogen-hydrogen scalars
+    const double* pd = a.pdiag;
+    const double* pdH = pd + (size_t)nh * 10;  // hydrogens' rows
+    const int* al = a.alist;
+    double* r = red[w];
+    if ((int)blockIdx.x < nh) {
+        const int cx = blockIdx.x;
+        // Block entry e = 10 * (bra pair) + (ket pair). Lane holds e0 = lane and e1 = 64 + lane
+        // (lanes below 36; the others repeat e0 and are dropped). With x as the ket (partners
+        // below x in the heavy list) the factor index is e / 10 and the target e % 10; with x as
+        // the bra (partners above) the reverse.
+        const int e0 = lane, e1 = lane < 36 ? 64 + lane : lane;
+        const int qb0 = e0 / 10, qb1 = e1 / 10, qa0 = e0 % 10, qa1 = e1 % 10;
+        double b0 = 0.0, b1 = 0.0, u0 = 0.0, u1 = 0.0, h = 0.0;
+#pragma unroll 4
+        for (int iy = w; iy < cx; iy += 4) {
+            const double* blk = a.wpool + tri_index(iy, cx, nh) * 100;
+            const double* py = pd + (size_t)iy * 10;
+            b0 += py[qb0] * blk[e0];
+            b1 += py[qb1] * blk[e1];
+        }
+#pragma unroll 4
+        for (int iy = cx + 1 + w; iy < nh; iy += 4) {
+            const double* blk = a.wpool + tri_index(cx, iy, nh) * 100;
+            const double* py = pd + (size_t)iy * 10;
+            u0 += py[qa0] * blk[e0];
+            u1 += py[qa1] * blk[e1];
+        }
+        // hydrogen partners: x's heavy-hydrogen row is nH * 10 contiguous doubles, entry
+        // 10 m + k; lanes below 60 take 6 partners per load, lane -> (m offset lane / 10, k lane % 10)
+        const double* wx = wHh + (int64_t)cx * nH * 10;
+        if (lane < 60) {
+            const int lm = lane / 10, lk = lane % 10;
+#pragma unroll 4
+            for (int m = w * 6 + lm; m < nH; m += 24) h += pdH[(size_t)m * 10] * wx[(int64_t)m * 10 + lk];
+        }
+        if (lane >= 36) b1 = u1 = 0.0;
+        r[lane] = b0;
+        r[100 + lane] = u0;
+        if (lane < 36) {
+            r[64 + lane] = b1;
+            r[164 + lane] = u1;
+        }
+        if (lane < 60) r[200 + lane] = h;
+        __syncthreads();
+        if (threadIdx.x < 40) {
+            const int k = threadIdx.x % 10;
+            const double* s = red[threadIdx.x / 10];
+            double v = 0.0;
+#pragma unroll
+            for (int q = 0; q < 10; ++q) v += s[q * 10 + k] + s[100 + k * 10 + q];
+#pragma unroll
+            for (int g = 0; g < 6; ++g) v += s[200 + g * 10 + k];
+            v += __shfl(v, threadIdx.x + 10, 64);
+            v += __shfl(v, threadIdx.x + 20, 64);
+            const int x = al[cx];
+            const int nk = a.elems[a.md.method * kMaxZ + a.z[x]].norb == 4 ? 10 : 1;
+            if (threadIdx.x < nk) diag_block_store(a, x, k, v, kFock, Pt, Ps, scale, F);
+        }
+        return;
+    }
+    // hydrogen group: class indices cx0 .. cx0 + gsz - 1
+    const int cx0 = ((int)blockIdx.x - nh) * 8;
+    const int gsz = min(8, nH - cx0);
+    // heavy partners: the group's entries against heavy iy are 10 * gsz contiguous doubles,
+    // entry 10 j + q (hydrogen cx0 + j, factor q of the heavy partner)
+    const bool v0 = lane < 10 * gsz, v1 = lane < 16 && 64 + lane < 10 * gsz;
+    const int e0 = v0 ? lane : 0, e1 = v1 ? 64 + lane : 0;
+    const int q0 = e0 % 10, q1 = e1 % 10;
+    double s0 = 0.0, s1 = 0.0, hh = 0.0;
+#pragma unroll 4
+    for (int iy = w; iy < nh; iy += 4) {
+        const double* seg = wHh + ((int64_t)iy * nH + cx0) * 10;
+        const double* py = pd + (size_t)iy * 10;
+        s0 += py[q0] * seg[e0];
+        s1 += py[q1] * seg[e1];
+    }
+    // hydrogen partners: lane -> (hydrogen cx0 + (lane & 7), partner m = base + (lane >> 3));
+    // partners below the group read 8 consecutive entries of row m, partners above it 8
+    // consecutive entries of each of the 8 rows
+    const int j = lane & 7, x = cx0 + j;
+    if (nH > 1) {
+#pragma unroll 4
+        for (int mb = w * 8; mb < nH; mb += 32) {
+            const int m = mb + (lane >> 3);
+            const bool ok = j < gsz && m < nH && m != x;
+            const int64_t t = ok ? tri_index(min(m, x), max(m, x), nH) : 0;
+            const double c = pdH[(size_t)(ok ? m : cx0) * 10];
+            const double g = wHH[t];
+            hh += ok ? c * g : 0.0;
+        }
+    }
+    if (!v0) s0 = 0.0;
+    if (!v1) s1 = 0.0;
+    r[lane] = s0;
+    if (lane < 16) r[64 + lane] = s1;
+    r[80 + lane] = hh;
+    __syncthreads();
+    if (threadIdx.x < 32) {
+        const int jj = threadIdx.x & 7;
+        const double* s = red[threadIdx.x >> 3];
+        double v = 0.0;
+#pragma unroll
+        for (int q = 0; q < 10; ++q) v += s[jj * 10 + q];
+#pragma unroll
+        for (int g = 0; g < 8; ++g) v += s[80 + g * 8 + jj];
+        v += __shfl(v, threadIdx.x + 8, 64);
+        v += __shfl(v, threadIdx.x + 16, 64);
+        if (threadIdx.x < gsz) diag_block_store(a, al[nh + cx0 + jj], 0, v, kFock, Pt, Ps, scale, F);
+    }
+}
+
+// Row of atom x in pdiag: heavy atoms first, then hydrogens, each in class order.
+__device__ __forceinline__ double* pdiag_row(const SysArgs& a, int x) {
+    return a.pdiag + (size_t)(a.z[x] == 1 ? a.md.n_heavy + a.acls[x] : a.acls[x]) * 10;
+}
+
+// Partner factors of the core attraction: (-Z_y, 0, ..., 0) per atom.
+__global__ __launch_bounds__(256) void k_pack_core(SysArgs a) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= a.md.nat) return;
+    const double c = -a.elems[a.md.method * kMaxZ + a.z[x]].core;
+    double* row = pdiag_row(a, x);
+#pragma unroll
+    for (int q = 0; q < 10; ++q) row[q] = q == 0 ? c : 0.0;
+}
+
+// Packed diagonal density blocks, off-diagonal entries doubled (the 2 of the symmetric sum).
+__global__ __launch_bounds__(256) void k_pack_diag(SysArgs a, const double* Pt) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= a.md.nat) return;
+    const int nao = a.md.nao, o = a.aoff[x];
+    double* row = pdiag_row(a, x);
+    if (a.z[x] == 1) {
+        row[0] = Pt[(size_t)o * nao + o];
+        return;
+    }
+#pragma unroll
+    for (int q = 0; q < 10; ++q) {
+        const int l = kPackRow[q], c = kPackCol[q];
+        row[q] = Pt[(size_t)(o + l) * nao + o + c] * (l == c ? 1.0 : 2.0);
+    }
+}
+
+// Off-diagonal Fock blocks, H - K, written in the upper triangle only (atom AO ranges are
+// disjoint, so a block lies wholly on one side); k_sym_lower then copies the upper triangle to
+// the lower in coalesced tiles. P and H are read at the same upper positions. One workgroup per
+// row atom walks that atom's matrix rows left to right, so every cache line of P, H and F in the
+// rows is fetched once. Both class lists keep the input order, so AO offsets grow along each and
+// the partners later in AO order are a suffix of each list (`later` gives where it begins).
+constexpr int kHhPairs = 16;
+
+// Orders one wavefront's LDS accesses around this point (lanes read what other lanes wrote).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Rows of heavy atom ci (workgroup ci). The later heavy partners are taken 16 at a time, one chunk
+// per wavefront: their 100-entry integral blocks (contiguous in the pool) are staged through LDS
+// with coalesced loads, lane (pair lane / 4, row mu = lane % 4) forms row mu of the exchange
+// block, K(mu, la) = sum_{nu, si} P(nu, si) (mu nu | la si), and writes F(mu, la) = H - K. The
+// hydrogens whose columns lie among the chunk's follow at once, one lane per hydrogen (4 x 1
+// blocks).
+__global__ __launch_bounds__(256) void k_fock_rows_heavy(SysArgs a, const double* Ps, double scale, double* F) {
+    __shared__ double stage[4][kHhPairs * 100];
     const int nh = a.md.n_heavy, nH = a.md.n_h, nao = a.md.nao;
-    const int64_t n_hh = (int64_t)nh * (nh - 1) / 2, n_hH = (int64_t)nh * nH, n_HH = (int64_t)nH * (nH - 1) / 2;
-    const int64_t total = n_hh + n_hH + n_HH;
-    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < total; p += (int64_t)gridDim.x * blockDim.x) {
-        if (p < n_hh) {
-            int i, j;
-            tri_decode(p, nh, i, j);
-            const int oa = a.aoff[a.alist[i]], ob = a.aoff[a.alist[j]];
-            const bool up = oa < ob;
-            // entry (mu, la) of the block (rows on atom i, columns on atom j) in the upper triangle
-            auto at = [&](int mu, int la) -> size_t {
-                return up ? (size_t)(oa + mu) * nao + ob + la : (size_t)(ob + la) * nao + oa + mu;
-            };
-            const double* w = a.wpool + p * 100;
-            double pab[16], acc[16], hv[16];
+    const int64_t n_hh = (int64_t)nh * (nh - 1) / 2;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    double* st = stage[wv];
+    const int pr = lane >> 2, mu = lane & 3;
+    const int ci = blockIdx.x, oa = a.aoff[a.alist[ci]];
+    const size_t r0 = (size_t)oa * nao;
+    const double* wi = a.wpool + n_hh * 100 + (int64_t)ci * nH * 10;  // ci's heavy-hydrogen row
+    const int nj = nh - 1 - ci;
+    const int nck = max(1, (nj + kHhPairs - 1) / kHhPairs);
+    for (int k = wv; k < nck; k += 4) {
+        const int j0 = ci + 1 + k * kHhPairs, j1 = min(j0 + kHhPairs, nh);
+        if (j1 > j0) {
+            const int nd = (j1 - j0) * 100;
+            const double* src = a.wpool + tri_index(ci, j0, nh) * 100;
+            double v[kHhPairs * 100 / 64];
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
+            for (int t = 0; t < kHhPairs * 100 / 64; ++t) {
+                const int e = lane + 64 * t;
+                v[t] = src[e < nd ? e : 0];
+            }
+            wave_lds_sync();  // the previous chunk's LDS reads come first
 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    pab[u * 4 + v] = scale * Ps[at(u, v)];
-                    hv[u * 4 + v] = a.H[at(u, v)];
-                    acc[u * 4 + v] = 0.0;
-                }
-            for (int kb = 0; kb < 10; ++kb) {
-                const int mu = kPackRow[kb], nu = kPackCol[kb];
-                double row[10];
+            for (int t = 0; t < kHhPairs * 100 / 64; ++t) {
+                const int e = lane + 64 * t;
+                if (e < nd) st[e] = v[t];
+            }
+            wave_lds_sync();
+            if (pr < j1 - j0) {
+                const int ob = a.aoff[a.alist[j0 + pr]];
+                double pab[16], hv[4], kk[4];
 #pragma unroll
-                for (int q = 0; q < 10; ++q) row[q] = w[kb * 10 + q];
+                for (int nu = 0; nu < 4; ++nu)
+#pragma unroll
+                    for (int si = 0; si < 4; ++si) pab[nu * 4 + si] = scale * Ps[r0 + (size_t)nu * nao + ob + si];
 #pragma unroll
                 for (int la = 0; la < 4; ++la) {
-                    double xx = 0.0, yy = 0.0;
-#pragma unroll
-                    for (int si = 0; si < 4; ++si) {
-                        const double wv = row[pack4(la, si)];
-                        xx += pab[nu * 4 + si] * wv;
-                        yy += pab[mu * 4 + si] * wv;
-                    }
-                    acc[mu * 4 + la] += xx;
-                    if (mu != nu) acc[nu * 4 + la] += yy;
+                    hv[la] = a.H[r0 + (size_t)mu * nao + ob + la];
+                    kk[la] = 0.0;
                 }
+                const double* blk = st + pr * 100;
+#pragma unroll
+                for (int nu = 0; nu < 4; ++nu) {
+                    const double* row = blk + pack4(mu, nu) * 10;
+                    double r[10];
+#pragma unroll
+                    for (int q = 0; q < 10; ++q) r[q] = row[q];
+#pragma unroll
+                    for (int la = 0; la < 4; ++la)
+#pragma unroll
+                        for (int si = 0; si < 4; ++si) kk[la] += pab[nu * 4 + si] * r[pack4(la, si)];
+                }
+#pragma unroll
+                for (int la = 0; la < 4; ++la) F[r0 + (size_t)mu * nao + ob + la] = hv[la] - kk[la];
             }
-#pragma unroll
-            for (int mu = 0; mu < 4; ++mu)
-#pragma unroll
-                for (int la = 0; la < 4; ++la) F[at(mu, la)] = hv[mu * 4 + la] - acc[mu * 4 + la];
-        } else if (p < n_hh + n_hH) {
-            const int64_t k = p - n_hh;
-            const int oa = a.aoff[a.alist[(int)(k / nH)]], ob = a.aoff[a.alist[nh + (int)(k % nH)]];
-            const bool up = oa < ob;
-            const double* w = a.wpool + n_hh * 100 + k * 10;
+        }
+        // hydrogens after heavy j0 - 1 (ci itself for the first chunk) and before heavy j1
+        const int m0 = a.later[j0 - 1], m1 = j1 >= nh ? nH : a.later[j1 - 1];
+        for (int m = m0 + lane; m < m1; m += 64) {
+            const int ob = a.aoff[a.alist[nh + m]];
+            const double* w = wi + (int64_t)m * 10;
             double pab[4], hv[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                const size_t ix = up ? (size_t)(oa + u) * nao + ob : (size_t)ob * nao + oa + u;
-                pab[u] = scale * Ps[ix];
-                hv[u] = a.H[ix];
+                pab[u] = scale * Ps[r0 + (size_t)u * nao + ob];
+                hv[u] = a.H[r0 + (size_t)u * nao + ob];
             }
 #pragma unroll
-            for (int mu = 0; mu < 4; ++mu) {
+            for (int u = 0; u < 4; ++u) {
                 double acc = 0.0;
 #pragma unroll
-                for (int nu = 0; nu < 4; ++nu) acc += pab[nu] * w[pack4(mu, nu)];
-                F[up ? (size_t)(oa + mu) * nao + ob : (size_t)ob * nao + oa + mu] = hv[mu] - acc;
+                for (int nu = 0; nu < 4; ++nu) acc += pab[nu] * w[pack4(u, nu)];
+                F[r0 + (size_t)u * nao + ob] = hv[u] - acc;
             }
-        } else {
-            int i, j;
-            tri_decode(p - n_hh - n_hH, nH, i, j);
-            const int oa = a.aoff[a.alist[nh + i]], ob = a.aoff[a.alist[nh + j]];
-            const size_t ix = oa < ob ? (size_t)oa * nao + ob : (size_t)ob * nao + oa;
-            const double acc = scale * Ps[ix] * a.wpool[p - n_hh - n_hH + n_hh * 100 + n_hH * 10];
-            F[ix] = a.H[ix] - acc;
         }
+    }
+}
+
+// Row of hydrogen cm (workgroup cm): its heavy partners later in AO order, lane per (partner,
+// orbital), the 1 x 4 blocks; then its hydrogen partners later in the list.
+__global__ __launch_bounds__(256) void k_fock_rows_h(SysArgs a, const double* Ps, double scale, double* F) {
+    const int nh = a.md.n_heavy, nH = a.md.n_h, nao = a.md.nao;
+    const int64_t n_hh = (int64_t)nh * (nh - 1) / 2, n_hH = (int64_t)nh * nH;
+    const double* wHh = a.wpool + n_hh * 100;
+    const double* wHH = wHh + n_hH * 10;
+    const int cm = blockIdx.x, ob = a.aoff[a.alist[nh + cm]];
+    const size_t row = (size_t)ob * nao;
+    const int mu = threadIdx.x & 3;
+    for (int i = a.later[nh + cm] + (threadIdx.x >> 2); i < nh; i += 64) {
+        const int oa = a.aoff[a.alist[i]];
+        const double* w = wHh + ((int64_t)i * nH + cm) * 10;
+        double acc = 0.0;
+#pragma unroll
+        for (int nu = 0; nu < 4; ++nu) acc += scale * Ps[row + oa + nu] * w[pack4(mu, nu)];
+        F[row + oa + mu] = a.H[row + oa + mu] - acc;
+    }
+    const double* wm = wHH + tri_index(cm, cm + 1, nH) - (cm + 1);  // entry of partner m > cm: wm[m]
+    for (int m = cm + 1 + threadIdx.x; m < nH; m += 256) {
+        const size_t ix = row + a.aoff[a.alist[nh + m]];
+        F[ix] = a.H[ix] - scale * Ps[ix] * wm[m];
     }
 }
 
@@ -1975,18 +2051,22 @@ __global__ __launch_bounds__(256) void k_dc_update(int64_t npat, const int64_t* 
 static inline unsigned blocks_for(int64_t n, int per = 256) { return (unsigned)((n + per - 1) / per); }
 
 hipError_t launch_hcore_diag(const SysArgs& a, hipStream_t st) {
-    if (a.md.nat > 0) hipLaunchKernelGGL(k_hcore_diag, dim3(blocks_for(a.md.nat, 4)), dim3(256), 0, st, a);
+    if (a.md.nat > 0) {
+        hipLaunchKernelGGL(k_pack_core, dim3(blocks_for(a.md.nat)), dim3(256), 0, st, a);
+        hipLaunchKernelGGL(k_diag_blocks<false>, dim3(a.md.n_heavy + (a.md.n_h + 7) / 8), dim3(256), 0, st, a,
+                           nullptr, nullptr, 0.0, nullptr);
+    }
     return hipGetLastError();
 }
 
 hipError_t launch_fock(const SysArgs& a, const double* Pt, const double* Ps, double scale, double* F, hipStream_t st) {
     hipLaunchKernelGGL(k_pack_diag, dim3(blocks_for(a.md.nat)), dim3(256), 0, st, a, Pt);
-    hipLaunchKernelGGL(k_fock_diag, dim3(blocks_for(a.md.nat, 4)), dim3(256), 0, st, a, Pt, Ps, scale, F);
-    const int64_t np = (int64_t)a.md.nat * (a.md.nat - 1) / 2;
-    if (np > 0) {
-        int64_t b = blocks_for(np);
-        if (b > 65536) b = 65536;
-        hipLaunchKernelGGL(k_fock_off, dim3((unsigned)b), dim3(256), 0, st, a, Ps, scale, F);
+    hipLaunchKernelGGL(k_diag_blocks<true>, dim3(a.md.n_heavy + (a.md.n_h + 7) / 8), dim3(256), 0, st, a, Pt, Ps,
+                       scale, F);
+    const int64_t nh = a.md.n_heavy, nH = a.md.n_h;
+    if (nh > 0) hipLaunchKernelGGL(k_fock_rows_heavy, dim3((unsigned)nh), dim3(256), 0, st, a, Ps, scale, F);
+    if (nH > 0) hipLaunchKernelGGL(k_fock_rows_h, dim3((unsigned)nH), dim3(256), 0, st, a, Ps, scale, F);
+    if (a.md.nat > 1) {
         const int64_t nt = ((int64_t)a.md.nao + 31) / 32;
         hipLaunchKernelGGL(k_sym_lower, dim3((unsigned)(nt * (nt + 1) / 2)), dim3(256), 0, st, a.md.nao, F);
     }
