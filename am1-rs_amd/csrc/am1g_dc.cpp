@@ -251,7 +251,7 @@ int am1g_dc(am1g_ctx* ctx, const am1g_request* rq, const am1g_dc_options* op, am
     sa.H = ba.mats;
     sa.pdiag = B.pdiag.as<double>();
 
-    HIP_TRY(ctx, B.partial.reserve(sizeof(double) * 1024 * (kDiisDepth + 1)));
+    HIP_TRY(ctx, B.partial.reserve(sizeof(double) * std::max<int64_t>(1024 * (kDiisDepth + 1), 2 * fock_energy_parts(md))));
     HIP_TRY(ctx, B.scal.reserve(sizeof(double) * 64));
     double* scal = B.scal.as<double>();
     auto t_start = std::chrono::steady_clock::now();
@@ -470,18 +470,15 @@ int am1g_dc(am1g_ctx* ctx, const am1g_request* rq, const am1g_dc_options* op, am
     auto fock_energy = [&]() -> int {
         if (uhf) {
             HIP_TRY(ctx, launch_add((int64_t)nn, P[0], P[1], Pt, s));
-            HIP_TRY(ctx, launch_fock(sa, Pt, P[0], 1.0, F[0], s));
-            HIP_TRY(ctx, launch_fock(sa, Pt, P[1], 1.0, F[1], s));
-            HIP_TRY(ctx, launch_dot3((int64_t)nn, Pt, sa.H, nullptr, partial, scal + 0, s));
-            HIP_TRY(ctx, launch_dot3((int64_t)nn, P[0], F[0], nullptr, partial, scal + 3, s));
-            HIP_TRY(ctx, launch_dot3((int64_t)nn, P[1], F[1], nullptr, partial, scal + 4, s));
+            HIP_TRY(ctx, launch_fock(sa, Pt, P[0], 1.0, F[0], s, partial, scal + 3));
+            HIP_TRY(ctx, launch_fock(sa, Pt, P[1], 1.0, F[1], s, partial, scal + 4));
         } else {
-            HIP_TRY(ctx, launch_fock(sa, P[0], P[0], 0.5, F[0], s));
-            HIP_TRY(ctx, launch_dot3((int64_t)nn, P[0], sa.H, F[0], partial, scal + 0, s));
+            HIP_TRY(ctx, launch_fock(sa, P[0], P[0], 0.5, F[0], s, partial, scal + 0));
         }
         return AM1G_OK;
     };
-    auto e_of = [&](const std::vector<double>& h) { return uhf ? 0.5 * (h[0] + h[3] + h[4]) : 0.5 * h[0]; };
+    // E_elec = 1/2 sum P (H + F) closed shell, 1/2 sum_s P_s (H + F_s) open shell
+    auto e_of = [&](const std::vector<double>& h) { return uhf ? 0.5 * (h[3] + h[4]) : 0.5 * h[0]; };
 
     // ------------------------------------------------------------------ SCF loop (divide_conquer.rs:1033-1254)
     for (int iter = 0; iter < op->max_scf; ++iter) {

@@ -95,7 +95,6 @@ static int large_uhf(am1g_ctx* ctx, const am1g_request& rq, const MolDesc& md, c
         HIP_TRY(ctx, L.f32a.reserve(sizeof(float) * nn));
         HIP_TRY(ctx, L.f32b.reserve(sizeof(float) * nn));
     }
-    double* H = sa.H;
     double* F[2] = {L.F.as<double>(), L.Fb.as<double>()};
     double* P[2] = {L.P.as<double>(), L.Pb.as<double>()};
     double* Ct[2] = {L.Ct.as<double>(), L.Ctb.as<double>()};
@@ -131,11 +130,8 @@ static int large_uhf(am1g_ctx* ctx, const am1g_request& rq, const MolDesc& md, c
     std::string eig_err;
     auto fock_energy = [&]() -> int {
         HIP_TRY(ctx, launch_add((int64_t)nn, P[0], P[1], Pt, s));
-        HIP_TRY(ctx, launch_fock(sa, Pt, P[0], 1.0, F[0], s));
-        HIP_TRY(ctx, launch_fock(sa, Pt, P[1], 1.0, F[1], s));
-        HIP_TRY(ctx, launch_dot3((int64_t)nn, Pt, H, nullptr, partial, scal + 0, s));
-        HIP_TRY(ctx, launch_dot3((int64_t)nn, P[0], F[0], nullptr, partial, scal + 1, s));
-        HIP_TRY(ctx, launch_dot3((int64_t)nn, P[1], F[1], nullptr, partial, scal + 2, s));
+        HIP_TRY(ctx, launch_fock(sa, Pt, P[0], 1.0, F[0], s, partial, scal + 1));
+        HIP_TRY(ctx, launch_fock(sa, Pt, P[1], 1.0, F[1], s, partial, scal + 2));
         return AM1G_OK;
     };
     int st;
@@ -169,7 +165,7 @@ static int large_uhf(am1g_ctx* ctx, const am1g_request& rq, const MolDesc& md, c
         }
         HIP_TRY(ctx, hipMemcpyAsync(h_scal, scal, sizeof(double) * 24, hipMemcpyDeviceToHost, s));
         HIP_TRY(ctx, hipStreamSynchronize(s));
-        const double e_elec = 0.5 * (h_scal[0] + h_scal[1] + h_scal[2]);
+        const double e_elec = 0.5 * (h_scal[1] + h_scal[2]);
         const double err = std::sqrt(2.0 * (h_scal[3] + h_scal[4]));
         const double* Fuse[2] = {F[0], F[1]};
         if (diis) {
@@ -268,7 +264,7 @@ static int large_uhf(am1g_ctx* ctx, const am1g_request& rq, const MolDesc& md, c
     HIP_TRY(ctx, hipMemcpyAsync(heps.data(), eps[0], sizeof(double) * n, hipMemcpyDeviceToHost, s));
     HIP_TRY(ctx, hipMemcpyAsync(hepsb.data(), eps[1], sizeof(double) * n, hipMemcpyDeviceToHost, s));
     HIP_TRY(ctx, hipStreamSynchronize(s));
-    const double electronic = 0.5 * (h_scal[0] + h_scal[1] + h_scal[2]);
+    const double electronic = 0.5 * (h_scal[1] + h_scal[2]);
     MolResult r;
     std::memset(&r, 0, sizeof(r));
     double dip[3] = {0, 0, 0}, eiso = 0.0, eheat = 0.0;
@@ -430,14 +426,14 @@ int am1g_large_scf(am1g_ctx* ctx, const am1g_request& rq, const Prepared& prep, 
     if (need_d) HIP_TRY(ctx, L.hist_d.reserve(sizeof(double) * kDiisDepth * len_s));
     HIP_TRY(ctx, L.ring.reserve(sizeof(int) * kDiisDepth));
     HIP_TRY(ctx, L.coef.reserve(sizeof(double) * kDiisDepth));
-    HIP_TRY(ctx, L.partial.reserve(sizeof(double) * std::max<int64_t>(large_partials(n), 4096)));
+    const int64_t n_part = std::max<int64_t>({large_partials(n), 4096, 2 * fock_energy_parts(md)});
+    HIP_TRY(ctx, L.partial.reserve(sizeof(double) * n_part));
     HIP_TRY(ctx, L.scal.reserve(sizeof(double) * 64));
     HIP_TRY(ctx, L.props.reserve(sizeof(double) * 4 * nat));
     if (md.precision == 1) {
         HIP_TRY(ctx, L.f32a.reserve(sizeof(float) * nn));
         HIP_TRY(ctx, L.f32b.reserve(sizeof(float) * nn));
     }
-    double* H = L.H.as<double>();
     double* F = L.F.as<double>();
     double* P = L.P.as<double>();
     double* T0 = L.T0.as<double>();
@@ -488,8 +484,7 @@ int am1g_large_scf(am1g_ctx* ctx, const am1g_request& rq, const Prepared& prep, 
     std::string eig_err;
     for (int iter = 0; iter < md.max_scf; ++iter) {
         iterations = iter + 1;
-        HIP_TRY(ctx, launch_fock(sa, P, P, 0.5, F, s));
-        HIP_TRY(ctx, launch_dot3((int64_t)nn, P, H, F, partial, scal + 0, s));
+        HIP_TRY(ctx, launch_fock(sa, P, P, 0.5, F, s, partial, scal + 0));
         phase[1] += ms_since(t_ph);
         const bool low = fp32_phase;
         // commutator FP - PF: column-major P*F read row-major is F*P
@@ -620,8 +615,7 @@ int am1g_large_scf(am1g_ctx* ctx, const am1g_request& rq, const Prepared& prep, 
         }
     }
     // ---- final energy, populations, dipole (scf.rs:327-468)
-    HIP_TRY(ctx, launch_fock(sa, P, P, 0.5, F, s));
-    HIP_TRY(ctx, launch_dot3((int64_t)nn, P, H, F, partial, scal + 0, s));
+    HIP_TRY(ctx, launch_fock(sa, P, P, 0.5, F, s, partial, scal + 0));
     HIP_TRY(ctx, launch_atom_props(sa, P, L.props.as<double>(), s));
     std::vector<double> props(4 * (size_t)nat), heps(n);
     HIP_TRY(ctx, hipMemcpyAsync(h_scal, scal, sizeof(double) * 30, hipMemcpyDeviceToHost, s));

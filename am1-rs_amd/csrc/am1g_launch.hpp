@@ -105,7 +105,14 @@ struct DcArgs {
 };
 
 hipError_t launch_hcore_diag(const SysArgs& a, hipStream_t st);
-hipError_t launch_fock(const SysArgs& a, const double* Pt, const double* Ps, double scale, double* F, hipStream_t st);
+// Fock matrix F of the total density Pt and spin density Ps (exchange scaled by `scale`). With
+// e_out given, also writes sum Ps (H + F) there, from per-workgroup partials in epart
+// (2 * fock_energy_parts doubles).
+hipError_t launch_fock(const SysArgs& a, const double* Pt, const double* Ps, double scale, double* F, hipStream_t st,
+                       double* epart = nullptr, double* e_out = nullptr);
+inline int64_t fock_energy_parts(const MolDesc& md) {
+    return 2 * (int64_t)md.n_heavy + (md.n_h + 7) / 8 + md.n_h;
+}
 hipError_t launch_dot3(int64_t n, const double* a, const double* b, const double* c, double* partial, double* out,
                        hipStream_t st);
 hipError_t launch_dc_eigen(const DcArgs& d, const double* F, int nao, hipStream_t st);

@@ -1299,6 +1299,55 @@ namespace am1g {
 
 
 
+// Compensated (sum, error) pairs. The large reductions of one big system (E_core over ~1e7
+// pairs, Tr P(H+F) over ~1e8 entries) are +-1e8 eV terms that cancel to ~1e6 eV; plain FP64
+// summation in any order moves the result by ~1e-5 eV at 6000 atoms, so these sums carry their
+// rounding error (Neumaier within a thread, error-free TwoSum when combining) and are accurate to
+// a few ulps of the result whatever the grid shape.
+struct KSum {
+    double s = 0.0, c = 0.0;
+};
+
+__device__ __forceinline__ void ksum_add(KSum& k, double x) {
+    const double t = k.s + x;
+    k.c += fabs(k.s) >= fabs(x) ? (k.s - t) + x : (x - t) + k.s;
+    k.s = t;
+}
+
+__device__ __forceinline__ KSum ksum_join(KSum a, KSum b) {
+    KSum r;
+    r.s = a.s + b.s;
+    const double bp = r.s - a.s;
+    r.c = a.c + b.c + ((a.s - (r.s - bp)) + (b.s - bp));
+    return r;
+}
+
+__device__ __forceinline__ KSum ksum_block(KSum v) {  // 256 threads, fixed order; result in thread 0
+    __shared__ double sh[2][4];
+    for (int off = 32; off > 0; off >>= 1) {
+        KSum o;
+        o.s = __shfl_xor(v.s, off, 64);
+        o.c = __shfl_xor(v.c, off, 64);
+        v = ksum_join(v, o);
+    }
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) {
+        sh[0][threadIdx.x >> 6] = v.s;
+        sh[1][threadIdx.x >> 6] = v.c;
+    }
+    __syncthreads();
+    KSum r;
+    if (threadIdx.x == 0) {
+        KSum w[4];
+        for (int q = 0; q < 4; ++q) {
+            w[q].s = sh[0][q];
+            w[q].c = sh[1][q];
+        }
+        r = ksum_join(ksum_join(w[0], w[1]), ksum_join(w[2], w[3]));
+    }
+    return r;
+}
+
 // Diagonal blocks of H_core and of the Fock matrix. Each block needs a sum over every partner y
 // of y's 10 packed factors (pdiag, rows in class order: heavy atoms, then hydrogens) against the
 // pair's integrals: the Coulomb term
@@ -1308,8 +1357,10 @@ namespace am1g {
 // doubles of the pool, so one load instruction covers a partner's whole 100-entry block (heavy
 // atom) or the 8 hydrogens' 80 contiguous entries against one heavy partner (hydrogen group);
 // the four wavefronts split the partners and meet in LDS.
-__device__ __forceinline__ void diag_block_store(const SysArgs& a, int x, int k, double v, bool fock,
-                                                 const double* Pt, const double* Ps, double scale, double* F) {
+// Stores entry k of x's diagonal block and returns its share of sum Ps (H + F) over the whole
+// matrix (both (mu, nu) and (nu, mu); zero for H_core).
+__device__ __forceinline__ double diag_block_store(const SysArgs& a, int x, int k, double v, bool fock,
+                                                   const double* Pt, const double* Ps, double scale, double* F) {
     const DevElem& ex = a.elems[a.md.method * kMaxZ + a.z[x]];
     const int nx = ex.norb, nao = a.md.nao, ox = a.aoff[x];
     const int mu = kPackRow[k], nu = kPackCol[k];
@@ -1317,7 +1368,7 @@ __device__ __forceinline__ void diag_block_store(const SysArgs& a, int x, int k,
         const double u = (mu == nu) ? (mu == 0 ? ex.uss : ex.upp) : 0.0;
         a.H[(size_t)(ox + mu) * nao + ox + nu] = u + v;
         a.H[(size_t)(ox + nu) * nao + ox + mu] = u + v;
-        return;
+        return 0.0;
     }
     double oc = 0.0;
     for (int la = 0; la < nx; ++la)
@@ -1325,14 +1376,24 @@ __device__ __forceinline__ void diag_block_store(const SysArgs& a, int x, int k,
             oc += Pt[(size_t)(ox + la) * nao + ox + si] * one_centre(mu, nu, la, si, ex);
             oc -= scale * Ps[(size_t)(ox + la) * nao + ox + si] * one_centre(mu, la, nu, si, ex);
         }
-    const double f = a.H[(size_t)(ox + mu) * nao + ox + nu] + oc + v;
+    const double h = a.H[(size_t)(ox + mu) * nao + ox + nu], f = h + oc + v;
     F[(size_t)(ox + mu) * nao + ox + nu] = f;
     F[(size_t)(ox + nu) * nao + ox + mu] = f;
+    return (mu == nu ? 1.0 : 2.0) * Ps[(size_t)(ox + mu) * nao + ox + nu] * (h + f);
+}
+
+// Per-workgroup partial of sum Ps (H + F): the workgroup's compensated sum at slot `slot`.
+__device__ __forceinline__ void energy_part(const KSum& v, double* epart, int64_t slot) {
+    const KSum r = ksum_block(v);
+    if (threadIdx.x == 0) {
+        epart[2 * slot] = r.s;
+        epart[2 * slot + 1] = r.c;
+    }
 }
 
 template <bool kFock>
 __global__ __launch_bounds__(256) void k_diag_blocks(SysArgs a, const double* Pt, const double* Ps, double scale,
-                                                     double* F) {
+                                                     double* F, double* epart) {
     __shared__ double red[4][260];
     const int nh = a.md.n_heavy, nH = a.md.n_h;
     const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -1343,6 +1404,7 @@ __global__ __launch_bounds__(256) void k_diag_blocks(SysArgs a, const double* Pt
     const double* pdH = pd + (size_t)nh * 10;  // hydrogens' rows
     const int* al = a.alist;
     double* r = red[w];
+    KSum ev;
     if ((int)blockIdx.x < nh) {
         const int cx = blockIdx.x;
         // Block entry e = 10 * (bra pair) + (ket pair). Lane holds e0 = lane and e1 = 64 + lane
@@ -1395,59 +1457,61 @@ __global__ __launch_bounds__(256) void k_diag_blocks(SysArgs a, const double* Pt
             v += __shfl(v, threadIdx.x + 20, 64);
             const int x = al[cx];
             const int nk = a.elems[a.md.method * kMaxZ + a.z[x]].norb == 4 ? 10 : 1;
-            if (threadIdx.x < nk) diag_block_store(a, x, k, v, kFock, Pt, Ps, scale, F);
+            if (threadIdx.x < nk) ksum_add(ev, diag_block_store(a, x, k, v, kFock, Pt, Ps, scale, F));
         }
-        return;
-    }
-    // hydrogen group: class indices cx0 .. cx0 + gsz - 1
-    const int cx0 = ((int)blockIdx.x - nh) * 8;
-    const int gsz = min(8, nH - cx0);
-    // heavy partners: the group's entries against heavy iy are 10 * gsz contiguous doubles,
-    // entry 10 j + q (hydrogen cx0 + j, factor q of the heavy partner)
-    const bool v0 = lane < 10 * gsz, v1 = lane < 16 && 64 + lane < 10 * gsz;
-    const int e0 = v0 ? lane : 0, e1 = v1 ? 64 + lane : 0;
-    const int q0 = e0 % 10, q1 = e1 % 10;
-    double s0 = 0.0, s1 = 0.0, hh = 0.0;
+    } else {
+        // hydrogen group: class indices cx0 .. cx0 + gsz - 1
+        const int cx0 = ((int)blockIdx.x - nh) * 8;
+        const int gsz = min(8, nH - cx0);
+        // heavy partners: the group's entries against heavy iy are 10 * gsz contiguous doubles,
+        // entry 10 j + q (hydrogen cx0 + j, factor q of the heavy partner)
+        const bool v0 = lane < 10 * gsz, v1 = lane < 16 && 64 + lane < 10 * gsz;
+        const int e0 = v0 ? lane : 0, e1 = v1 ? 64 + lane : 0;
+        const int q0 = e0 % 10, q1 = e1 % 10;
+        double s0 = 0.0, s1 = 0.0, hh = 0.0;
 #pragma unroll 4
-    for (int iy = w; iy < nh; iy += 4) {
-        const double* seg = wHh + ((int64_t)iy * nH + cx0) * 10;
-        const double* py = pd + (size_t)iy * 10;
-        s0 += py[q0] * seg[e0];
-        s1 += py[q1] * seg[e1];
-    }
-    // hydrogen partners: lane -> (hydrogen cx0 + (lane & 7), partner m = base + (lane >> 3));
-    // partners below the group read 8 consecutive entries of row m, partners above it 8
-    // consecutive entries of each of the 8 rows
-    const int j = lane & 7, x = cx0 + j;
-    if (nH > 1) {
+        for (int iy = w; iy < nh; iy += 4) {
+            const double* seg = wHh + ((int64_t)iy * nH + cx0) * 10;
+            const double* py = pd + (size_t)iy * 10;
+            s0 += py[q0] * seg[e0];
+            s1 += py[q1] * seg[e1];
+        }
+        // hydrogen partners: lane -> (hydrogen cx0 + (lane & 7), partner m = base + (lane >> 3));
+        // partners below the group read 8 consecutive entries of row m, partners above it 8
+        // consecutive entries of each of the 8 rows
+        const int j = lane & 7, x = cx0 + j;
+        if (nH > 1) {
 #pragma unroll 4
-        for (int mb = w * 8; mb < nH; mb += 32) {
-            const int m = mb + (lane >> 3);
-            const bool ok = j < gsz && m < nH && m != x;
-            const int64_t t = ok ? tri_index(min(m, x), max(m, x), nH) : 0;
-            const double c = pdH[(size_t)(ok ? m : cx0) * 10];
-            const double g = wHH[t];
-            hh += ok ? c * g : 0.0;
+            for (int mb = w * 8; mb < nH; mb += 32) {
+                const int m = mb + (lane >> 3);
+                const bool ok = j < gsz && m < nH && m != x;
+                const int64_t t = ok ? tri_index(min(m, x), max(m, x), nH) : 0;
+                const double c = pdH[(size_t)(ok ? m : cx0) * 10];
+                const double g = wHH[t];
+                hh += ok ? c * g : 0.0;
+            }
+        }
+        if (!v0) s0 = 0.0;
+        if (!v1) s1 = 0.0;
+        r[lane] = s0;
+        if (lane < 16) r[64 + lane] = s1;
+        r[80 + lane] = hh;
+        __syncthreads();
+        if (threadIdx.x < 32) {
+            const int jj = threadIdx.x & 7;
+            const double* s = red[threadIdx.x >> 3];
+            double v = 0.0;
+#pragma unroll
+            for (int q = 0; q < 10; ++q) v += s[jj * 10 + q];
+#pragma unroll
+            for (int g = 0; g < 8; ++g) v += s[80 + g * 8 + jj];
+            v += __shfl(v, threadIdx.x + 8, 64);
+            v += __shfl(v, threadIdx.x + 16, 64);
+            if (threadIdx.x < gsz)
+                ksum_add(ev, diag_block_store(a, al[nh + cx0 + jj], 0, v, kFock, Pt, Ps, scale, F));
         }
     }
-    if (!v0) s0 = 0.0;
-    if (!v1) s1 = 0.0;
-    r[lane] = s0;
-    if (lane < 16) r[64 + lane] = s1;
-    r[80 + lane] = hh;
-    __syncthreads();
-    if (threadIdx.x < 32) {
-        const int jj = threadIdx.x & 7;
-        const double* s = red[threadIdx.x >> 3];
-        double v = 0.0;
-#pragma unroll
-        for (int q = 0; q < 10; ++q) v += s[jj * 10 + q];
-#pragma unroll
-        for (int g = 0; g < 8; ++g) v += s[80 + g * 8 + jj];
-        v += __shfl(v, threadIdx.x + 8, 64);
-        v += __shfl(v, threadIdx.x + 16, 64);
-        if (threadIdx.x < gsz) diag_block_store(a, al[nh + cx0 + jj], 0, v, kFock, Pt, Ps, scale, F);
-    }
+    if (kFock && epart) energy_part(ev, epart, blockIdx.x);
 }
 
 // Row of atom x in pdiag: heavy atoms first, then hydrogens, each in class order.
@@ -1503,7 +1567,8 @@ __device__ __forceinline__ void wave_lds_sync() {
 // block, K(mu, la) = sum_{nu, si} P(nu, si) (mu nu | la si), and writes F(mu, la) = H - K. The
 // hydrogens whose columns lie among the chunk's follow at once, one lane per hydrogen (4 x 1
 // blocks).
-__global__ __launch_bounds__(256) void k_fock_rows_heavy(SysArgs a, const double* Ps, double scale, double* F) {
+__global__ __launch_bounds__(256) void k_fock_rows_heavy(SysArgs a, const double* Ps, double scale, double* F,
+                                                         double* epart, int64_t slot0) {
     __shared__ double stage[4][kHhPairs * 100];
     const int nh = a.md.n_heavy, nH = a.md.n_h, nao = a.md.nao;
     const int64_t n_hh = (int64_t)nh * (nh - 1) / 2;
@@ -1515,6 +1580,7 @@ __global__ __launch_bounds__(256) void k_fock_rows_heavy(SysArgs a, const double
     const double* wi = a.wpool + n_hh * 100 + (int64_t)ci * nH * 10;  // ci's heavy-hydrogen row
     const int nj = nh - 1 - ci;
     const int nck = max(1, (nj + kHhPairs - 1) / kHhPairs);
+    KSum ev;
     for (int k = wv; k < nck; k += 4) {
         const int j0 = ci + 1 + k * kHhPairs, j1 = min(j0 + kHhPairs, nh);
         if (j1 > j0) {
@@ -1539,7 +1605,7 @@ __global__ __launch_bounds__(256) void k_fock_rows_heavy(SysArgs a, const double
 #pragma unroll
                 for (int nu = 0; nu < 4; ++nu)
 #pragma unroll
-                    for (int si = 0; si < 4; ++si) pab[nu * 4 + si] = scale * Ps[r0 + (size_t)nu * nao + ob + si];
+                    for (int si = 0; si < 4; ++si) pab[nu * 4 + si] = Ps[r0 + (size_t)nu * nao + ob + si];
 #pragma unroll
                 for (int la = 0; la < 4; ++la) {
                     hv[la] = a.H[r0 + (size_t)mu * nao + ob + la];
@@ -1558,7 +1624,11 @@ __global__ __launch_bounds__(256) void k_fock_rows_heavy(SysArgs a, const double
                         for (int si = 0; si < 4; ++si) kk[la] += pab[nu * 4 + si] * r[pack4(la, si)];
                 }
 #pragma unroll
-                for (int la = 0; la < 4; ++la) F[r0 + (size_t)mu * nao + ob + la] = hv[la] - kk[la];
+                for (int la = 0; la < 4; ++la) {
+                    const double f = hv[la] - scale * kk[la];
+                    F[r0 + (size_t)mu * nao + ob + la] = f;
+                    ksum_add(ev, 2.0 * pab[mu * 4 + la] * (hv[la] + f));
+                }
             }
         }
         // hydrogens after heavy j0 - 1 (ci itself for the first chunk) and before heavy j1
@@ -1569,7 +1639,7 @@ __global__ __launch_bounds__(256) void k_fock_rows_heavy(SysArgs a, const double
             double pab[4], hv[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                pab[u] = scale * Ps[r0 + (size_t)u * nao + ob];
+                pab[u] = Ps[r0 + (size_t)u * nao + ob];
                 hv[u] = a.H[r0 + (size_t)u * nao + ob];
             }
 #pragma unroll
@@ -1577,15 +1647,19 @@ __global__ __launch_bounds__(256) void k_fock_rows_heavy(SysArgs a, const double
                 double acc = 0.0;
 #pragma unroll
                 for (int nu = 0; nu < 4; ++nu) acc += pab[nu] * w[pack4(u, nu)];
-                F[r0 + (size_t)u * nao + ob] = hv[u] - acc;
+                const double f = hv[u] - scale * acc;
+                F[r0 + (size_t)u * nao + ob] = f;
+                ksum_add(ev, 2.0 * pab[u] * (hv[u] + f));
             }
         }
     }
+    if (epart) energy_part(ev, epart, slot0 + blockIdx.x);
 }
 
 // Row of hydrogen cm (workgroup cm): its heavy partners later in AO order, lane per (partner,
 // orbital), the 1 x 4 blocks; then its hydrogen partners later in the list.
-__global__ __launch_bounds__(256) void k_fock_rows_h(SysArgs a, const double* Ps, double scale, double* F) {
+__global__ __launch_bounds__(256) void k_fock_rows_h(SysArgs a, const double* Ps, double scale, double* F,
+                                                     double* epart, int64_t slot0) {
     const int nh = a.md.n_heavy, nH = a.md.n_h, nao = a.md.nao;
     const int64_t n_hh = (int64_t)nh * (nh - 1) / 2, n_hH = (int64_t)nh * nH;
     const double* wHh = a.wpool + n_hh * 100;
@@ -1593,19 +1667,25 @@ __global__ __launch_bounds__(256) void k_fock_rows_h(SysArgs a, const double* Ps
     const int cm = blockIdx.x, ob = a.aoff[a.alist[nh + cm]];
     const size_t row = (size_t)ob * nao;
     const int mu = threadIdx.x & 3;
+    KSum ev;
     for (int i = a.later[nh + cm] + (threadIdx.x >> 2); i < nh; i += 64) {
         const int oa = a.aoff[a.alist[i]];
         const double* w = wHh + ((int64_t)i * nH + cm) * 10;
         double acc = 0.0;
 #pragma unroll
-        for (int nu = 0; nu < 4; ++nu) acc += scale * Ps[row + oa + nu] * w[pack4(mu, nu)];
-        F[row + oa + mu] = a.H[row + oa + mu] - acc;
+        for (int nu = 0; nu < 4; ++nu) acc += Ps[row + oa + nu] * w[pack4(mu, nu)];
+        const double h = a.H[row + oa + mu], f = h - scale * acc;
+        F[row + oa + mu] = f;
+        ksum_add(ev, 2.0 * Ps[row + oa + mu] * (h + f));
     }
     const double* wm = wHH + tri_index(cm, cm + 1, nH) - (cm + 1);  // entry of partner m > cm: wm[m]
     for (int m = cm + 1 + threadIdx.x; m < nH; m += 256) {
         const size_t ix = row + a.aoff[a.alist[nh + m]];
-        F[ix] = a.H[ix] - scale * Ps[ix] * wm[m];
+        const double p = Ps[ix], h = a.H[ix], f = h - scale * p * wm[m];
+        F[ix] = f;
+        ksum_add(ev, 2.0 * p * (h + f));
     }
+    if (epart) energy_part(ev, epart, slot0 + blockIdx.x);
 }
 
 // F[i][j] = F[j][i] for i > j, in 32 x 32 tiles through LDS (grid: the tiles on and below the
@@ -1638,55 +1718,6 @@ __global__ __launch_bounds__(256) void k_sym_lower(int n, double* F) {
 
 // Deterministic two-stage sums: fixed-size grid of block partials, then one block in order.
 constexpr int kRedBlocks = 1024;
-
-// Compensated (sum, error) pairs. The large reductions of one big system (E_core over ~1e7
-// pairs, Tr P(H+F) over ~1e8 entries) are +-1e8 eV terms that cancel to ~1e6 eV; plain FP64
-// summation in any order moves the result by ~1e-5 eV at 6000 atoms, so these sums carry their
-// rounding error (Neumaier within a thread, error-free TwoSum when combining) and are accurate to
-// a few ulps of the result whatever the grid shape.
-struct KSum {
-    double s = 0.0, c = 0.0;
-};
-
-__device__ __forceinline__ void ksum_add(KSum& k, double x) {
-    const double t = k.s + x;
-    k.c += fabs(k.s) >= fabs(x) ? (k.s - t) + x : (x - t) + k.s;
-    k.s = t;
-}
-
-__device__ __forceinline__ KSum ksum_join(KSum a, KSum b) {
-    KSum r;
-    r.s = a.s + b.s;
-    const double bp = r.s - a.s;
-    r.c = a.c + b.c + ((a.s - (r.s - bp)) + (b.s - bp));
-    return r;
-}
-
-__device__ __forceinline__ KSum ksum_block(KSum v) {  // 256 threads, fixed order; result in thread 0
-    __shared__ double sh[2][4];
-    for (int off = 32; off > 0; off >>= 1) {
-        KSum o;
-        o.s = __shfl_xor(v.s, off, 64);
-        o.c = __shfl_xor(v.c, off, 64);
-        v = ksum_join(v, o);
-    }
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) {
-        sh[0][threadIdx.x >> 6] = v.s;
-        sh[1][threadIdx.x >> 6] = v.c;
-    }
-    __syncthreads();
-    KSum r;
-    if (threadIdx.x == 0) {
-        KSum w[4];
-        for (int q = 0; q < 4; ++q) {
-            w[q].s = sh[0][q];
-            w[q].c = sh[1][q];
-        }
-        r = ksum_join(ksum_join(w[0], w[1]), ksum_join(w[2], w[3]));
-    }
-    return r;
-}
 
 // Block partials of sum a * (b + c) (b, c optional), as (sum, error) pairs.
 __global__ __launch_bounds__(256) void k_dot3_partial(int64_t n, const double* a, const double* b, const double* c,
@@ -2054,22 +2085,26 @@ hipError_t launch_hcore_diag(const SysArgs& a, hipStream_t st) {
     if (a.md.nat > 0) {
         hipLaunchKernelGGL(k_pack_core, dim3(blocks_for(a.md.nat)), dim3(256), 0, st, a);
         hipLaunchKernelGGL(k_diag_blocks<false>, dim3(a.md.n_heavy + (a.md.n_h + 7) / 8), dim3(256), 0, st, a,
-                           nullptr, nullptr, 0.0, nullptr);
+                           nullptr, nullptr, 0.0, nullptr, nullptr);
     }
     return hipGetLastError();
 }
 
-hipError_t launch_fock(const SysArgs& a, const double* Pt, const double* Ps, double scale, double* F, hipStream_t st) {
+hipError_t launch_fock(const SysArgs& a, const double* Pt, const double* Ps, double scale, double* F, hipStream_t st,
+                       double* epart, double* e_out) {
+    const int64_t nh = a.md.n_heavy, nH = a.md.n_h, ndiag = nh + (nH + 7) / 8;
+    if (!e_out) epart = nullptr;
     hipLaunchKernelGGL(k_pack_diag, dim3(blocks_for(a.md.nat)), dim3(256), 0, st, a, Pt);
-    hipLaunchKernelGGL(k_diag_blocks<true>, dim3(a.md.n_heavy + (a.md.n_h + 7) / 8), dim3(256), 0, st, a, Pt, Ps,
-                       scale, F);
-    const int64_t nh = a.md.n_heavy, nH = a.md.n_h;
-    if (nh > 0) hipLaunchKernelGGL(k_fock_rows_heavy, dim3((unsigned)nh), dim3(256), 0, st, a, Ps, scale, F);
-    if (nH > 0) hipLaunchKernelGGL(k_fock_rows_h, dim3((unsigned)nH), dim3(256), 0, st, a, Ps, scale, F);
+    hipLaunchKernelGGL(k_diag_blocks<true>, dim3((unsigned)ndiag), dim3(256), 0, st, a, Pt, Ps, scale, F, epart);
+    if (nh > 0)
+        hipLaunchKernelGGL(k_fock_rows_heavy, dim3((unsigned)nh), dim3(256), 0, st, a, Ps, scale, F, epart, ndiag);
+    if (nH > 0)
+        hipLaunchKernelGGL(k_fock_rows_h, dim3((unsigned)nH), dim3(256), 0, st, a, Ps, scale, F, epart, ndiag + nh);
     if (a.md.nat > 1) {
         const int64_t nt = ((int64_t)a.md.nao + 31) / 32;
         hipLaunchKernelGGL(k_sym_lower, dim3((unsigned)(nt * (nt + 1) / 2)), dim3(256), 0, st, a.md.nao, F);
     }
+    if (e_out) hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(256), 0, st, (int)fock_energy_parts(a.md), epart, e_out);
     return hipGetLastError();
 }
 
