@@ -251,7 +251,8 @@ int am1g_dc(am1g_ctx* ctx, const am1g_request* rq, const am1g_dc_options* op, am
     sa.H = ba.mats;
     sa.pdiag = B.pdiag.as<double>();
 
-    HIP_TRY(ctx, B.partial.reserve(sizeof(double) * std::max<int64_t>(1024 * (kDiisDepth + 1), 2 * fock_energy_parts(md))));
+    HIP_TRY(ctx, B.partial.reserve(sizeof(double) * std::max<int64_t>({1024 * (kDiisDepth + 1), 2 * fock_energy_parts(md),
+                                                                       fermi_scratch_doubles()})));
     HIP_TRY(ctx, B.scal.reserve(sizeof(double) * 64));
     double* scal = B.scal.as<double>();
     auto t_start = std::chrono::steady_clock::now();
@@ -433,7 +434,7 @@ int am1g_dc(am1g_ctx* ctx, const am1g_request* rq, const am1g_dc_options* op, am
     d0.t1 = B.t1.as<double>();
     d0.cap = uhf ? 1.0 : 2.0;
     d0.tol_abs = 0.0;
-    d0.red = B.partial.as<double>();  // 1024 * 9 doubles >= 8 + 256 * 31
+    d0.red = B.partial.as<double>();  // at least fermi_scratch_doubles()
     DcArgs dch[2];
     for (int c = 0; c < 2; ++c) {
         dch[c] = d0;

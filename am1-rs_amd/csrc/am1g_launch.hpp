@@ -101,7 +101,7 @@ struct DcArgs {
     double cap;     // 2 restricted, 1 per spin channel
     double tol_abs; // Jacobi tolerance floor
     int* status;    // [n_sub] eigensolver status (sweeps, or -1)
-    double* red;    // reduction scratch for the Fermi search: 8 + 256 * 31 doubles
+    double* red;    // reduction scratch for the Fermi search: fermi_scratch_doubles()
 };
 
 hipError_t launch_hcore_diag(const SysArgs& a, hipStream_t st);
@@ -116,6 +116,7 @@ inline int64_t fock_energy_parts(const MolDesc& md) {
 hipError_t launch_dot3(int64_t n, const double* a, const double* b, const double* c, double* partial, double* out,
                        hipStream_t st);
 hipError_t launch_dc_eigen(const DcArgs& d, const double* F, int nao, hipStream_t st);
+int64_t fermi_scratch_doubles();  // size of DcArgs::red
 hipError_t launch_fermi(const DcArgs& d, int64_t levels, double n_elec, double kt, double* out, hipStream_t st);
 hipError_t launch_dc_density(const DcArgs& d, hipStream_t st);
 hipError_t launch_dc_gather(const DcArgs& d, int64_t npat, const int64_t* pat, const int* ao_atom,

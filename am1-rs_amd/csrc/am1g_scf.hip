@@ -1799,11 +1799,14 @@ __device__ __forceinline__ double fermi_dirac(double x) {  // fermi.rs:110-119
 // five bisection steps (a complete binary tree of intervals) over all levels at once, and one
 // thread then replays those five steps exactly as the reference takes them; the replay also
 // stops when the interval no longer shrinks, which is where 200 reference steps end up.
-// State in red[0..4): lo, hi, done, mu, steps; block partials from red[8].
+// State in red[0..4): lo, hi, done, mu, steps; block partials from red[8], midpoint-major.
 constexpr int kFermiDepth = 5;
 constexpr int kFermiK = (1 << kFermiDepth) - 1;
-constexpr int kFermiBlocks = 256;
-constexpr int kFermiPasses = 40;
+constexpr int kFermiBlocks = 1024;
+constexpr int kFermiPasses = 40;  // 200 steps
+constexpr int kFermiFirst = 12;   // passes launched before the host looks at the done flag
+
+int64_t fermi_scratch_doubles() { return 8 + (int64_t)kFermiBlocks * kFermiK; }
 
 __device__ __forceinline__ void fermi_tree(double lo, double hi, double (&mid)[kFermiK]) {
     double l[kFermiK], h[kFermiK];
@@ -1821,12 +1824,35 @@ __device__ __forceinline__ void fermi_tree(double lo, double hi, double (&mid)[k
     }
 }
 
-__global__ __launch_bounds__(kWg) void k_fermi_bounds(DcArgs d, int64_t L, double kt) {
-    __shared__ WgScratch s;
+// Block partials of the lowest and highest level (red[8 + 2 b], red[9 + 2 b]).
+__global__ __launch_bounds__(256) void k_fermi_minmax(DcArgs d, int64_t L) {
+    __shared__ double sh[2][4];
     double lo = INFINITY, hi = -INFINITY;
-    for (int64_t i = threadIdx.x; i < L; i += kWg) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < L; i += (int64_t)gridDim.x * 256) {
         lo = fmin(lo, d.eps[i]);
         hi = fmax(hi, d.eps[i]);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        lo = fmin(lo, __shfl_xor(lo, off, 64));
+        hi = fmax(hi, __shfl_xor(hi, off, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        sh[0][threadIdx.x >> 6] = lo;
+        sh[1][threadIdx.x >> 6] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        d.red[8 + 2 * blockIdx.x] = fmin(fmin(sh[0][0], sh[0][1]), fmin(sh[0][2], sh[0][3]));
+        d.red[9 + 2 * blockIdx.x] = fmax(fmax(sh[1][0], sh[1][1]), fmax(sh[1][2], sh[1][3]));
+    }
+}
+
+__global__ __launch_bounds__(kWg) void k_fermi_bounds(DcArgs d, double kt) {
+    __shared__ WgScratch s;
+    double lo = INFINITY, hi = -INFINITY;
+    for (int b = threadIdx.x; b < kFermiBlocks; b += kWg) {
+        lo = fmin(lo, d.red[8 + 2 * b]);
+        hi = fmax(hi, d.red[9 + 2 * b]);
     }
     lo = -wg_max(-lo, s) - 50.0 * kt - 1.0;
     hi = wg_max(hi, s) + 50.0 * kt + 1.0;
@@ -1842,14 +1868,25 @@ __global__ __launch_bounds__(kWg) void k_fermi_bounds(DcArgs d, int64_t L, doubl
 __global__ __launch_bounds__(256) void k_fermi_count(DcArgs d, int64_t L, double kt) {
     if (d.red[2] != 0.0) return;
     double mu[kFermiK], c[kFermiK];
-    fermi_tree(d.red[0], d.red[1], mu);
+    const double lo = d.red[0], hi = d.red[1];
+    fermi_tree(lo, hi, mu);
 #pragma unroll
     for (int q = 0; q < kFermiK; ++q) c[q] = 0.0;
+    double below = 0.0;  // weight of the levels below every midpoint by more than 40 kT
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < L; i += (int64_t)gridDim.x * 256) {
         const double w = d.cap * d.proj[i], e = d.eps[i];
+        // every midpoint lies in [lo, hi] and rounded subtraction and division are monotone, so
+        // beyond these bounds fermi_dirac is exactly 0 (above) or 1 (below) for all of them
+        if ((e - hi) / kt > 40.0) continue;
+        if ((e - lo) / kt < -40.0) {
+            below += w;
+            continue;
+        }
 #pragma unroll
         for (int q = 0; q < kFermiK; ++q) c[q] += w * fermi_dirac((e - mu[q]) / kt);
     }
+#pragma unroll
+    for (int q = 0; q < kFermiK; ++q) c[q] += below;
     __shared__ double sh[4][kFermiK];
 #pragma unroll
     for (int q = 0; q < kFermiK; ++q) {
@@ -1860,23 +1897,32 @@ __global__ __launch_bounds__(256) void k_fermi_count(DcArgs d, int64_t L, double
     __syncthreads();
     double* part = d.red + 8;
     if (threadIdx.x < kFermiK)
-        part[blockIdx.x * kFermiK + threadIdx.x] =
+        part[threadIdx.x * kFermiBlocks + blockIdx.x] =
             (sh[0][threadIdx.x] + sh[1][threadIdx.x]) + (sh[2][threadIdx.x] + sh[3][threadIdx.x]);
 }
 
-__global__ __launch_bounds__(256) void k_fermi_pick(DcArgs d, double n_elec) {
+constexpr int kPickGroups = 32;  // block partials per midpoint summed by one thread: kFermiBlocks / 32
+static_assert(kFermiBlocks % kPickGroups == 0 && kPickGroups * kFermiK <= 1024, "pick layout");
+__global__ __launch_bounds__(1024) void k_fermi_pick(DcArgs d, double n_elec) {
     if (d.red[2] != 0.0) return;
     __shared__ double tot[kFermiK];
-    __shared__ double sh[4];
+    __shared__ double sh[kPickGroups][kFermiK];
     const double* part = d.red + 8;
-    for (int q = 0; q < kFermiK; ++q) {
+    // thread (q, g) sums the g-th slice of midpoint q's block partials; then thread q the slices
+    if (threadIdx.x < kPickGroups * kFermiK) {
+        const int q = threadIdx.x % kFermiK, g = threadIdx.x / kFermiK;
+        const double* p = part + (size_t)q * kFermiBlocks + g * (kFermiBlocks / kPickGroups);
         double v = 0.0;
-        for (int b = threadIdx.x; b < kFermiBlocks; b += 256) v += part[b * kFermiK + q];
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-        __syncthreads();
-        if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
-        __syncthreads();
-        if (threadIdx.x == 0) tot[q] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+#pragma unroll
+        for (int b = 0; b < kFermiBlocks / kPickGroups; ++b) v += p[b];
+        sh[g][q] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < kFermiK) {
+        const int q = threadIdx.x;
+        double v = 0.0;
+        for (int g = 0; g < kPickGroups; ++g) v += sh[g][q];
+        tot[q] = v;
     }
     __syncthreads();
     if (threadIdx.x != 0) return;
@@ -2133,10 +2179,20 @@ hipError_t launch_dc_eigen(const DcArgs& d, const double* F, int nao, hipStream_
 }
 
 hipError_t launch_fermi(const DcArgs& d, int64_t levels, double n_elec, double kt, double* out, hipStream_t st) {
-    hipLaunchKernelGGL(k_fermi_bounds, dim3(1), dim3(kWg), 0, st, d, levels, kt);
+    hipLaunchKernelGGL(k_fermi_minmax, dim3(kFermiBlocks), dim3(256), 0, st, d, levels);
+    hipLaunchKernelGGL(k_fermi_bounds, dim3(1), dim3(kWg), 0, st, d, kt);
     for (int pass = 0; pass < kFermiPasses; ++pass) {
+        if (pass == kFermiFirst) {
+            // 60 steps end nearly every search (the interval reaches the spacing of doubles
+            // near mu); look at the done flag before launching the rest
+            double done = 0.0;
+            hipError_t e = hipMemcpyAsync(&done, d.red + 2, sizeof(double), hipMemcpyDeviceToHost, st);
+            if (e == hipSuccess) e = hipStreamSynchronize(st);
+            if (e != hipSuccess) return e;
+            if (done != 0.0) break;
+        }
         hipLaunchKernelGGL(k_fermi_count, dim3(kFermiBlocks), dim3(256), 0, st, d, levels, kt);
-        hipLaunchKernelGGL(k_fermi_pick, dim3(1), dim3(256), 0, st, d, n_elec);
+        hipLaunchKernelGGL(k_fermi_pick, dim3(1), dim3(1024), 0, st, d, n_elec);
     }
     hipLaunchKernelGGL(k_fermi_final, dim3(kFermiBlocks), dim3(256), 0, st, d, levels, kt);
     hipLaunchKernelGGL(k_fermi_out, dim3(1), dim3(256), 0, st, d, kt, out);
