@@ -250,6 +250,10 @@ int am1g_dc(am1g_ctx* ctx, const am1g_request* rq, const am1g_dc_options* op, am
     sa.wpool = ba.wpool;
     sa.H = ba.mats;
     sa.pdiag = B.pdiag.as<double>();
+    HIP_TRY(ctx, B.tcol.reserve(sizeof(double) * 10 * std::max<int64_t>(1, (int64_t)md.n_heavy * (md.n_heavy - 1) / 2)));
+    HIP_TRY(ctx, B.rsum.reserve(sizeof(double) * 10 * std::max(1, md.n_heavy)));
+    sa.tcol = B.tcol.as<double>();
+    sa.rsum = B.rsum.as<double>();
 
     HIP_TRY(ctx, B.partial.reserve(sizeof(double) * std::max<int64_t>({1024 * (kDiisDepth + 1), 2 * fock_energy_parts(md),
                                                                        fermi_scratch_doubles()})));

@@ -18,7 +18,7 @@
 struct LargeBuffers {
     DevBuf A, evals, Ct;                       // eigensolver entry point
     DevBuf H, F, P, T0, T1, Pn, f32a, f32b;    // dense n x n (FP32 staging for mixed precision)
-    DevBuf hist_f, hist_e, hist_d, ring, coef, partial, scal, pdiag, props;
+    DevBuf hist_f, hist_e, hist_d, ring, coef, partial, scal, pdiag, tcol, rsum, props;
     DevBuf Fb, Pb, Pt, Tb, Ctb, evals_b, hist_fb, hist_eb;  // beta channel (UHF)
 };
 
@@ -411,6 +411,10 @@ int am1g_large_scf(am1g_ctx* ctx, const am1g_request& rq, const Prepared& prep, 
     sa.wpool = ba.wpool;
     sa.H = L.H.as<double>();
     sa.pdiag = L.pdiag.as<double>();
+    HIP_TRY(ctx, L.tcol.reserve(sizeof(double) * 10 * std::max<int64_t>(1, (int64_t)md.n_heavy * (md.n_heavy - 1) / 2)));
+    HIP_TRY(ctx, L.rsum.reserve(sizeof(double) * 10 * std::max(1, md.n_heavy)));
+    sa.tcol = L.tcol.as<double>();
+    sa.rsum = L.rsum.as<double>();
 
     const bool need_d = md.accel == 2;
     const int64_t len_s = (int64_t)n * (n + 1) / 2, len_a = (int64_t)n * (n - 1) / 2;

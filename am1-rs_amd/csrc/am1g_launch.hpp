@@ -78,6 +78,9 @@ struct SysArgs {
     double* H;           // [nao][nao]
     double* pdiag;       // [nat][10] packed diagonal density blocks (off-diagonal entries doubled),
                          // rows in class order: heavy atoms, then hydrogens
+    double* tcol;        // [n_hh][10] heavy pair (i < j): Coulomb of i's diagonal block on j's, at
+                         // j (j - 1) / 2 + i (each column contiguous)
+    double* rsum;        // [n_heavy][10] Coulomb on heavy atom i from its heavy partners j > i
 };
 
 // Divide-and-conquer subsystem tables (all device pointers).

@@ -1407,54 +1407,74 @@ __global__ __launch_bounds__(256) void k_diag_blocks(SysArgs a, const double* Pt
     KSum ev;
     if ((int)blockIdx.x < nh) {
         const int cx = blockIdx.x;
-        // Block entry e = 10 * (bra pair) + (ket pair). Lane holds e0 = lane and e1 = 64 + lane
-        // (lanes below 36; the others repeat e0 and are dropped). With x as the ket (partners
-        // below x in the heavy list) the factor index is e / 10 and the target e % 10; with x as
-        // the bra (partners above) the reverse.
-        const int e0 = lane, e1 = lane < 36 ? 64 + lane : lane;
-        const int qb0 = e0 / 10, qb1 = e1 / 10, qa0 = e0 % 10, qa1 = e1 % 10;
-        double b0 = 0.0, b1 = 0.0, u0 = 0.0, u1 = 0.0, h = 0.0;
-#pragma unroll 4
-        for (int iy = w; iy < cx; iy += 4) {
-            const double* blk = a.wpool + tri_index(iy, cx, nh) * 100;
-            const double* py = pd + (size_t)iy * 10;
-            b0 += py[qb0] * blk[e0];
-            b1 += py[qb1] * blk[e1];
-        }
-#pragma unroll 4
-        for (int iy = cx + 1 + w; iy < nh; iy += 4) {
-            const double* blk = a.wpool + tri_index(cx, iy, nh) * 100;
-            const double* py = pd + (size_t)iy * 10;
-            u0 += py[qa0] * blk[e0];
-            u1 += py[qa1] * blk[e1];
-        }
         // hydrogen partners: x's heavy-hydrogen row is nH * 10 contiguous doubles, entry
         // 10 m + k; lanes below 60 take 6 partners per load, lane -> (m offset lane / 10, k lane % 10)
+        const int lm = lane / 10, lk = lane % 10;
         const double* wx = wHh + (int64_t)cx * nH * 10;
+        double h = 0.0;
         if (lane < 60) {
-            const int lm = lane / 10, lk = lane % 10;
 #pragma unroll 4
             for (int m = w * 6 + lm; m < nH; m += 24) h += pdH[(size_t)m * 10] * wx[(int64_t)m * 10 + lk];
         }
-        if (lane >= 36) b1 = u1 = 0.0;
-        r[lane] = b0;
-        r[100 + lane] = u0;
-        if (lane < 36) {
-            r[64 + lane] = b1;
-            r[164 + lane] = u1;
-        }
         if (lane < 60) r[200 + lane] = h;
+        if constexpr (kFock) {
+            // heavy partners: k_fock_rows_heavy has left the terms from partners above x in
+            // rsum[x] and those from partners below x in x's column of tcol (cx rows of 10
+            // contiguous doubles), read here like the hydrogen row
+            const double* tc = a.tcol + (int64_t)cx * (cx - 1) / 2 * 10;
+            double c = 0.0;
+            if (lane < 60) {
+#pragma unroll 4
+                for (int m = w * 6 + lm; m < cx; m += 24) c += tc[(int64_t)m * 10 + lk];
+                r[lane] = c;
+            }
+        } else {
+            // Block entry e = 10 * (bra pair) + (ket pair). Lane holds e0 = lane and e1 = 64 + lane
+            // (lanes below 36; the others repeat e0 and are dropped). With x as the ket (partners
+            // below x in the heavy list) the factor index is e / 10 and the target e % 10; with x
+            // as the bra (partners above) the reverse.
+            const int e0 = lane, e1 = lane < 36 ? 64 + lane : lane;
+            const int qb0 = e0 / 10, qb1 = e1 / 10, qa0 = e0 % 10, qa1 = e1 % 10;
+            double b0 = 0.0, b1 = 0.0, u0 = 0.0, u1 = 0.0;
+#pragma unroll 4
+            for (int iy = w; iy < cx; iy += 4) {
+                const double* blk = a.wpool + tri_index(iy, cx, nh) * 100;
+                const double* py = pd + (size_t)iy * 10;
+                b0 += py[qb0] * blk[e0];
+                b1 += py[qb1] * blk[e1];
+            }
+#pragma unroll 4
+            for (int iy = cx + 1 + w; iy < nh; iy += 4) {
+                const double* blk = a.wpool + tri_index(cx, iy, nh) * 100;
+                const double* py = pd + (size_t)iy * 10;
+                u0 += py[qa0] * blk[e0];
+                u1 += py[qa1] * blk[e1];
+            }
+            if (lane >= 36) b1 = u1 = 0.0;
+            r[lane] = b0;
+            r[100 + lane] = u0;
+            if (lane < 36) {
+                r[64 + lane] = b1;
+                r[164 + lane] = u1;
+            }
+        }
         __syncthreads();
         if (threadIdx.x < 40) {
             const int k = threadIdx.x % 10;
             const double* s = red[threadIdx.x / 10];
             double v = 0.0;
+            if constexpr (kFock) {
 #pragma unroll
-            for (int q = 0; q < 10; ++q) v += s[q * 10 + k] + s[100 + k * 10 + q];
+                for (int g = 0; g < 6; ++g) v += s[g * 10 + k] + s[200 + g * 10 + k];
+            } else {
 #pragma unroll
-            for (int g = 0; g < 6; ++g) v += s[200 + g * 10 + k];
+                for (int q = 0; q < 10; ++q) v += s[q * 10 + k] + s[100 + k * 10 + q];
+#pragma unroll
+                for (int g = 0; g < 6; ++g) v += s[200 + g * 10 + k];
+            }
             v += __shfl(v, threadIdx.x + 10, 64);
             v += __shfl(v, threadIdx.x + 20, 64);
+            if constexpr (kFock) v += a.rsum[cx * 10 + k];
             const int x = al[cx];
             const int nk = a.elems[a.md.method * kMaxZ + a.z[x]].norb == 4 ? 10 : 1;
             if (threadIdx.x < nk) ksum_add(ev, diag_block_store(a, x, k, v, kFock, Pt, Ps, scale, F));
@@ -1566,7 +1586,10 @@ __device__ __forceinline__ void wave_lds_sync() {
 // with coalesced loads, lane (pair lane / 4, row mu = lane % 4) forms row mu of the exchange
 // block, K(mu, la) = sum_{nu, si} P(nu, si) (mu nu | la si), and writes F(mu, la) = H - K. The
 // hydrogens whose columns lie among the chunk's follow at once, one lane per hydrogen (4 x 1
-// blocks).
+// blocks). From the staged block the wavefront also takes both atoms' Coulomb terms for their
+// diagonal blocks: ci's (row k of the block against j's packed factors, summed over the row into
+// rsum[ci]) and j's (column k against ci's factors, into tcol), so the diagonal-block kernel
+// need not read the heavy pair blocks again.
 __global__ __launch_bounds__(256) void k_fock_rows_heavy(SysArgs a, const double* Ps, double scale, double* F,
                                                          double* epart, int64_t slot0) {
     __shared__ double stage[4][kHhPairs * 100];
@@ -1580,6 +1603,9 @@ __global__ __launch_bounds__(256) void k_fock_rows_heavy(SysArgs a, const double
     const double* wi = a.wpool + n_hh * 100 + (int64_t)ci * nH * 10;  // ci's heavy-hydrogen row
     const int nj = nh - 1 - ci;
     const int nck = max(1, (nj + kHhPairs - 1) / kHhPairs);
+    double pci[10], arow[3] = {0.0, 0.0, 0.0};  // ci's factors; lane's terms for k = mu, mu + 4, mu + 8
+#pragma unroll
+    for (int q = 0; q < 10; ++q) pci[q] = a.pdiag[(size_t)ci * 10 + q];
     KSum ev;
     for (int k = wv; k < nck; k += 4) {
         const int j0 = ci + 1 + k * kHhPairs, j1 = min(j0 + kHhPairs, nh);
@@ -1629,6 +1655,25 @@ __global__ __launch_bounds__(256) void k_fock_rows_heavy(SysArgs a, const double
                     F[r0 + (size_t)mu * nao + ob + la] = f;
                     ksum_add(ev, 2.0 * pab[mu * 4 + la] * (hv[la] + f));
                 }
+                const int j = j0 + pr;
+                double pj[10];
+#pragma unroll
+                for (int q = 0; q < 10; ++q) pj[q] = a.pdiag[(size_t)j * 10 + q];
+                double* tj = a.tcol + ((int64_t)j * (j - 1) / 2 + ci) * 10;
+#pragma unroll 1
+                for (int t = 0; t < 3; ++t) {
+                    const int kq = mu + 4 * t;
+                    if (kq < 10) {
+                        double sr = 0.0, sc = 0.0;
+#pragma unroll
+                        for (int q = 0; q < 10; ++q) {
+                            sr += pj[q] * blk[kq * 10 + q];
+                            sc += pci[q] * blk[q * 10 + kq];
+                        }
+                        arow[t] += sr;  // t is uniform in the loop; arow stays in registers
+                        tj[kq] = sc;
+                    }
+                }
             }
         }
         // hydrogens after heavy j0 - 1 (ci itself for the first chunk) and before heavy j1
@@ -1652,6 +1697,20 @@ __global__ __launch_bounds__(256) void k_fock_rows_heavy(SysArgs a, const double
                 ksum_add(ev, 2.0 * pab[u] * (hv[u] + f));
             }
         }
+    }
+    // rsum[ci][k]: lanes l with l % 4 == k % 4 hold k's terms in arow[k / 4]
+    __syncthreads();
+    double* flat = &stage[0][0];
+#pragma unroll
+    for (int t = 0; t < 3; ++t) flat[wv * 192 + lane * 3 + t] = arow[t];
+    __syncthreads();
+    if (threadIdx.x < 40) {
+        const int kq = threadIdx.x % 10, g = threadIdx.x / 10;
+        double v = 0.0;
+        for (int p = 0; p < 16; ++p) v += flat[g * 192 + (p * 4 + kq % 4) * 3 + kq / 4];
+        v += __shfl(v, threadIdx.x + 10, 64);
+        v += __shfl(v, threadIdx.x + 20, 64);
+        if (threadIdx.x < 10) a.rsum[(size_t)ci * 10 + kq] = v;
     }
     if (epart) energy_part(ev, epart, slot0 + blockIdx.x);
 }
@@ -2141,11 +2200,12 @@ hipError_t launch_fock(const SysArgs& a, const double* Pt, const double* Ps, dou
     const int64_t nh = a.md.n_heavy, nH = a.md.n_h, ndiag = nh + (nH + 7) / 8;
     if (!e_out) epart = nullptr;
     hipLaunchKernelGGL(k_pack_diag, dim3(blocks_for(a.md.nat)), dim3(256), 0, st, a, Pt);
-    hipLaunchKernelGGL(k_diag_blocks<true>, dim3((unsigned)ndiag), dim3(256), 0, st, a, Pt, Ps, scale, F, epart);
     if (nh > 0)
         hipLaunchKernelGGL(k_fock_rows_heavy, dim3((unsigned)nh), dim3(256), 0, st, a, Ps, scale, F, epart, ndiag);
     if (nH > 0)
         hipLaunchKernelGGL(k_fock_rows_h, dim3((unsigned)nH), dim3(256), 0, st, a, Ps, scale, F, epart, ndiag + nh);
+    // after k_fock_rows_heavy, which leaves the heavy partners' terms in rsum and tcol
+    hipLaunchKernelGGL(k_diag_blocks<true>, dim3((unsigned)ndiag), dim3(256), 0, st, a, Pt, Ps, scale, F, epart);
     if (a.md.nat > 1) {
         const int64_t nt = ((int64_t)a.md.nao + 31) / 32;
         hipLaunchKernelGGL(k_sym_lower, dim3((unsigned)(nt * (nt + 1) / 2)), dim3(256), 0, st, a.md.nao, F);
