@@ -150,6 +150,92 @@ __global__ __launch_bounds__(256) void k_tridiag_next(int n, int c, double* A, d
     reflect_row(n, r, A, d, e, tau, vb + (size_t)(r & 1) * n, red);
 }
 
+// k_tridiag_next for m = n - c - 1 <= 4096 with 1024 threads: each thread keeps its entries
+// q = t + 1024 u of v_c, p_c and row c+1 in registers, so the kernel makes one pass over global
+// memory and two block reductions (the 256-thread version makes five dependent passes).
+constexpr int kNextT = 1024, kNextU = 4;
+
+__device__ __forceinline__ double block_sum_1024(double v, double* red) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    __syncthreads();  // red may still be read from the previous sum
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double tot = 0.0;
+#pragma unroll
+    for (int w = 0; w < kNextT / 64; ++w) tot += red[w];
+    return tot;
+}
+
+__global__ __launch_bounds__(kNextT) void k_tridiag_next_wide(int n, int c, double* A, double* d, double* e,
+                                                              double* tau, double* vb, double* wb) {
+    __shared__ double red[kNextT / 64];
+    __shared__ double x0;
+    const int m = n - c - 1, t = threadIdx.x, r = c + 1;
+    const double* vc = vb + (size_t)(c & 1) * n;
+    double* pc = wb + (size_t)(c & 1) * n;
+    double* rowr = A + (size_t)r * n + r;  // rowr[q] = A[r][r + q], q < m
+    double vq[kNextU], pq[kNextU], aq[kNextU];
+    const double v0 = vc[0], p0 = pc[0];  // read before the reduction's barriers; pc[0] is rewritten
+    double pv = 0.0;
+#pragma unroll
+    for (int u = 0; u < kNextU; ++u) {
+        const int q = t + kNextT * u;
+        vq[u] = q < m ? vc[q] : 0.0;
+        pq[u] = q < m ? pc[q] : 0.0;
+        aq[u] = q < m ? rowr[q] : 0.0;
+        pv += pq[u] * vq[u];
+    }
+    const double alpha = 0.5 * tau[c] * block_sum_1024(pv, red);
+    // w_c = p_c - alpha v_c (the next column's deferred update reads it), then the rank-2 update
+    // of column c on row c+1; every thread forms w_c[0] itself
+    const double w0 = p0 - alpha * v0;
+    double ss = 0.0;
+#pragma unroll
+    for (int u = 0; u < kNextU; ++u) {
+        const int q = t + kNextT * u;
+        if (q < m) {
+            pq[u] -= alpha * vq[u];
+            pc[q] = pq[u];
+            aq[u] -= v0 * pq[u] + w0 * vq[u];
+            if (q >= 2) ss += aq[u] * aq[u];
+        }
+    }
+    if (t == 1) x0 = aq[0];  // A[r][r+1], the reflector's leading entry
+    if (t == 1 && r == n - 2) d[n - 1] = A[(size_t)(n - 1) * n + n - 1] - 2.0 * vq[0] * pq[0];
+    ss = block_sum_1024(ss, red);
+    // reflector of row r, as reflect_row
+    const double ax = x0;
+    double beta, tt, scale;
+    if (ss == 0.0) {
+        beta = ax;
+        tt = 0.0;
+        scale = 0.0;
+    } else {
+        beta = -copysign(sqrt(ax * ax + ss), ax);
+        tt = (beta - ax) / beta;
+        scale = 1.0 / (ax - beta);
+    }
+    double* vr = vb + (size_t)(r & 1) * n;
+#pragma unroll
+    for (int u = 0; u < kNextU; ++u) {
+        const int q = t + kNextT * u;
+        if (q < m) {
+            double val = aq[u];
+            if (q >= 2) {
+                val *= scale;
+                vr[q - 1] = val;
+            }
+            rowr[q] = val;
+        }
+    }
+    if (t == 0) {
+        vr[0] = 1.0;
+        d[r] = aq[0];
+        e[r] = beta;
+        tau[r] = tt;
+    }
+}
+
 // ---------------------------------------------------------------- divide and conquer
 // Leaves: the tridiagonal block [s, s + m) with the split-modified diagonal, diagonalized by the
 // workgroup Jacobi solver; eigenvectors go to the columns of Q (column-major, ld n) in ascending
@@ -427,7 +513,10 @@ hipError_t eig_large(EigWork& w, rocblas_handle blas, int n, double* A, double* 
         hipLaunchKernelGGL(k_reflect0, dim3(1), dim3(256), 0, st, n, A, d, e, tau, v);
         for (int c = 0; c + 2 < n; ++c) {
             hipLaunchKernelGGL(k_tridiag_rows, dim3(blocks(n - c - 1, 4)), dim3(256), 0, st, n, c, A, tau, v, p);
-            hipLaunchKernelGGL(k_tridiag_next, dim3(1), dim3(256), 0, st, n, c, A, d, e, tau, v, p);
+            if (n - c - 1 <= kNextT * kNextU)
+                hipLaunchKernelGGL(k_tridiag_next_wide, dim3(1), dim3(kNextT), 0, st, n, c, A, d, e, tau, v, p);
+            else
+                hipLaunchKernelGGL(k_tridiag_next, dim3(1), dim3(256), 0, st, n, c, A, d, e, tau, v, p);
         }
     }
     EIG_TRY(hipGetLastError());

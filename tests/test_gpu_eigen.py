@@ -38,6 +38,14 @@ def test_random_symmetric(engine, n):
     _check(engine, 0.5 * (b + b.T))
 
 
+def test_order_above_the_register_resident_reflector_step(engine):
+    # the first columns of a 4113-order matrix (more than 4096 trailing rows) take the
+    # 256-thread next-reflector kernel, the rest the register-resident 1024-thread one
+    rng = np.random.default_rng(4113)
+    b = rng.normal(size=(4113, 4113))
+    _check(engine, 0.5 * (b + b.T))
+
+
 def test_exactly_degenerate_spectrum(engine):
     # block diagonal with repeated blocks, rotated: every eigenvalue 4-fold degenerate
     rng = np.random.default_rng(3)
