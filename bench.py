@@ -103,6 +103,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=2000.0)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 1 GiB device to device (calibrates the FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--dc-probe", action="store_true",
+                    help="only the 2000-water divide-and-conquer SCF of the large section (for counter runs)")
     args = ap.parse_args()
     if args.copy_probe:
         import torch
@@ -121,6 +123,15 @@ def main():
 
     import am1g
     from am1g import geometry as G
+
+    if args.dc_probe:
+        eng = am1g.Engine(0)
+        w2000 = G.water_cluster_config(3)
+        rd = eng.dc(w2000.z, w2000.xyz_bohr, core_of_atom=np.arange(len(w2000.z)) // 3,
+                    buffer_radius=4.0 / 0.529167, kt=1000.0 * 8.617333262e-5)
+        print(json.dumps({"dc_status": rd["status"], "iterations": rd["iterations"],
+                          "wall_ms": round(rd["wall_ms"], 1)}), flush=True)
+        return
 
     t0 = time.time()
     mols = G.chain_batch(args.molecules, seed=1)
