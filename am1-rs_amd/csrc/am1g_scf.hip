@@ -1350,13 +1350,14 @@ __device__ __forceinline__ KSum ksum_block(KSum v) {  // 256 threads, fixed orde
 
 // Diagonal blocks of H_core and of the Fock matrix. Each block needs a sum over every partner y
 // of y's 10 packed factors (pdiag, rows in class order: heavy atoms, then hydrogens) against the
-// pair's integrals: the Coulomb term
-// sum_q P_y[q] (mu nu | q_y) for the Fock matrix, or the core attraction -Z_y (mu nu | s_y s_y)
-// when pdiag holds (-Z_y, 0, ..., 0). Workgroups 0..n_heavy-1 take one heavy atom each (its class
-// index); the rest take 8 consecutive hydrogens of the hydrogen list. Lanes read consecutive
-// doubles of the pool, so one load instruction covers a partner's whole 100-entry block (heavy
-// atom) or the 8 hydrogens' 80 contiguous entries against one heavy partner (hydrogen group);
-// the four wavefronts split the partners and meet in LDS.
+// pair's integrals: the Coulomb term sum_q P_y[q] (mu nu | q_y) for the Fock matrix, or the core
+// attraction -Z_y (mu nu | s_y s_y) when pdiag holds (-Z_y, 0, ..., 0). Workgroups
+// 0..n_heavy-1 take one heavy atom each (its class index); the rest take 8 consecutive hydrogens
+// of the hydrogen list. Lanes read consecutive doubles of the pool, so one load instruction
+// covers a partner's whole 100-entry block (heavy atom) or the 8 hydrogens' 80 contiguous
+// entries against one heavy partner (hydrogen group); the four wavefronts split the partners and
+// meet in LDS. For the Fock matrix a heavy atom's heavy partners are not read here:
+// k_fock_rows_heavy, which stages those blocks anyway, leaves their terms in rsum and tcol.
 // Stores entry k of x's diagonal block and returns its share of sum Ps (H + F) over the whole
 // matrix (both (mu, nu) and (nu, mu); zero for H_core).
 __device__ __forceinline__ double diag_block_store(const SysArgs& a, int x, int k, double v, bool fock,
