@@ -1862,7 +1862,7 @@ __device__ __forceinline__ double fermi_dirac(double x) {  // fermi.rs:110-119
 // State in red[0..4): lo, hi, done, mu, steps; block partials from red[8], midpoint-major.
 constexpr int kFermiDepth = 5;
 constexpr int kFermiK = (1 << kFermiDepth) - 1;
-constexpr int kFermiBlocks = 1024;
+constexpr int kFermiBlocks = 256;
 constexpr int kFermiPasses = 40;  // 200 steps
 constexpr int kFermiFirst = 12;   // passes launched before the host looks at the done flag
 
