@@ -2072,12 +2072,23 @@ __global__ __launch_bounds__(kWg) void k_dc_density(DcArgs d) {
     const double* Ct = d.cbuf + d.sub_mat[sub];
     const double* f = d.frac + d.sub_ao[sub];
     double* T0 = d.t0 + d.sub_mat[sub];
+    // levels come in ascending order and f falls with the level, so the rows that contribute are
+    // a prefix: the product runs over the rows up to the last one with f > 1e-14 (the rest are
+    // exact zeros)
+    __shared__ int k_used;
+    if (t == 0) k_used = 1;
+    __syncthreads();
+    int last = 0;
+    for (int i = t; i < n; i += kWg)
+        if (f[i] > 1.0e-14) last = i + 1;
+    if (last > 0) atomicMax(&k_used, last);
     for (int idx = t; idx < n * n; idx += kWg) {
         const int i = idx / n;
         T0[idx] = f[i] > 1.0e-14 ? Ct[idx] * sqrt(d.cap * f[i]) : 0.0;
     }
     __syncthreads();
-    wg_gemm_any<double, true, false>(n, n, n, 1.0, T0, n, T0, n, d.fbuf + d.sub_mat[sub], n, lds_dyn, kLdsDoubles);
+    wg_gemm_any<double, true, false>(n, n, k_used, 1.0, T0, n, T0, n, d.fbuf + d.sub_mat[sub], n, lds_dyn,
+                                     kLdsDoubles);
 }
 
 __device__ __forceinline__ int local_ao(const DcArgs& d, int sub, int g) {
