@@ -3,7 +3,7 @@
 ``single_point`` mirrors am1-rs's Python entry point ``am1_rs.native.single_point``
 (python/am1_rs/native.py:52-97 -> src/python.rs:147-199): coordinates in Angstrom, result dict
 in eV and Hartree, every error raised as ``ValueError`` with the engine's message
-(python.rs:27-29). ``single_point_batch`` runs many molecules in one persistent GPU launch.
+(python.rs:27-29). ``Engine.scf_batch`` runs many molecules in one persistent GPU launch.
 
 The engine is ``libam1g.so`` built from ``csrc/`` for gfx950. There is no CPU fallback: when the
 library or a GPU is missing, loading fails with an error that says so.
@@ -266,7 +266,11 @@ class Engine:
                  adiis_switch=0.1, precision="fp64"):
         rq = Request()
         self._L.am1g_request_defaults(ctypes.byref(rq))
-        za = np.ascontiguousarray(z, dtype=np.uint8)
+        zi = np.asarray(z)
+        if zi.size and (not np.issubdtype(zi.dtype, np.integer) or zi.min() < 1 or zi.max() > 255):
+            # an out-of-range atomic number would wrap in the uint8 cast and name another element
+            raise ValueError(f"atomic numbers must be integers in 1..255, got {zi.ravel().tolist()[:8]}")
+        za = np.ascontiguousarray(zi, dtype=np.uint8)
         xa = np.ascontiguousarray(xyz_bohr, dtype=np.float64).reshape(-1, 3)
         keep.extend([za, xa])
         rq.n_atoms = len(za)
@@ -514,36 +518,69 @@ def engine() -> Engine:
     return _default_engine
 
 
-def single_point(symbols_or_z, positions_angstrom, charge=0.0, multiplicity=1, method="am1", max_scf=200,
-                 e_tol=1e-8, p_tol=1e-7, use_diis=True, accelerator="adiis_cdiis", adiis_switch=0.1,
-                 reference="auto", precision="fp64"):
-    """``am1_rs.native.single_point`` counterpart: Angstrom in, the reference's result-dict keys out
-    (python.rs:166-198; energies in eV and Hartree). Errors raise ``ValueError``."""
-    z = [SYMBOLS[s] if isinstance(s, str) else int(s) for s in symbols_or_z]
-    xyz = np.asarray(positions_angstrom, dtype=np.float64).reshape(-1, 3) * ANGSTROM_TO_BOHR
-    r = engine().scf(z, xyz, full=True, charge=charge, multiplicity=multiplicity, method=method,
-                     max_scf=max_scf, e_tol=e_tol, p_tol=p_tol, use_diis=use_diis, accelerator=accelerator,
-                     adiis_switch=adiis_switch, reference=reference, precision=precision)
+def _as_numbers(numbers):
+    out = []
+    for v in numbers:
+        if isinstance(v, str):
+            if v not in SYMBOLS:
+                raise ValueError(f"unknown element symbol {v!r}")
+            out.append(SYMBOLS[v])
+        else:
+            iv = int(v)
+            if iv != v or not 1 <= iv <= 255:
+                raise ValueError(f"atomic number {v!r} is not an integer in 1..255")
+            out.append(iv)
+    return out
+
+
+def single_point(numbers, positions, charge=0.0, multiplicity=1, reference="auto", method="am1",
+                 electric_field=None) -> dict:
+    """``am1_rs.native.single_point`` counterpart (python/am1_rs/native.py:52-97 ->
+    src/python.rs:147-199): same signature and positional order, positions in Angstrom, and the
+    same result keys: ``method``, ``energy_hartree``, ``energy_ev``, ``heat_of_formation_kcal``,
+    ``electronic_ev``, ``core_ev``, ``charges``, ``dipole_debye``, ``dipole_magnitude``,
+    ``homo_ev``, ``lumo_ev``, ``homo_beta_ev``, ``lumo_beta_ev``, ``field_nuclear_ev``,
+    ``converged``, ``iterations``, ``unrestricted``. ``electric_field`` is three floats in atomic
+    units (Hartree per e*Bohr) or None. Every error, non-convergence included, raises
+    ``ValueError`` (python.rs:27-29)."""
+    z = _as_numbers(numbers)
+    pos = np.asarray(positions, dtype=np.float64)
+    if pos.ndim != 2 or pos.shape[1] != 3:
+        raise ValueError("each position must have 3 components")
+    if len(z) != pos.shape[0]:
+        raise ValueError("numbers and positions length mismatch")
+    if electric_field is not None:
+        f = np.asarray(electric_field, dtype=float).reshape(-1)
+        if f.size != 3:
+            raise ValueError("electric_field must have three components")
+        raise ValueError("an external electric field is not supported by this engine")
+    if str(method).lower() not in METHODS:
+        raise ValueError(f"unknown method {method!r} (expected 'am1' or 'rm1')")
+    if str(reference).lower() not in REFERENCES:
+        raise ValueError(f"unknown reference {reference!r} (expected 'auto', 'rhf' or 'uhf')")
+    r = engine().scf(z, pos * ANGSTROM_TO_BOHR, full=False, charge=float(charge), multiplicity=max(1, int(multiplicity)),
+                     method=str(method).lower(), reference=str(reference).lower())
     if r["status"] == 4:
         raise ValueError(f"SCF did not converge in {r['iterations']} iterations "
                          f"(residual {r['residual']:.3e})")
     if r["status"] != 0:
         raise ValueError(r.get("message", r["status_name"]))
     return {
-        "energy": r["total_ev"] / AM1_EV,
+        "method": str(method).lower(),
+        "energy_hartree": r["total_ev"] / AM1_EV,
         "energy_ev": r["total_ev"],
-        "electronic_energy": r["electronic_ev"] / AM1_EV,
-        "electronic_energy_ev": r["electronic_ev"],
-        "core_repulsion": r["core_ev"] / AM1_EV,
-        "core_repulsion_ev": r["core_ev"],
         "heat_of_formation_kcal": r["heat_of_formation_kcal"],
+        "electronic_ev": r["electronic_ev"],
+        "core_ev": r["core_ev"],
         "charges": r["charges"].tolist(),
         "dipole_debye": r["dipole_debye"].tolist(),
-        "dipole_magnitude_debye": r["dipole_magnitude"],
+        "dipole_magnitude": r["dipole_magnitude"],
         "homo_ev": r["homo_ev"],
         "lumo_ev": r["lumo_ev"],
-        "mo_energies_ev": r["mo_energies"].tolist(),
-        "iterations": r["iterations"],
+        "homo_beta_ev": r["homo_beta_ev"],
+        "lumo_beta_ev": r["lumo_beta_ev"],
+        "field_nuclear_ev": r.get("field_nuclear_ev", 0.0),
         "converged": r["converged"],
+        "iterations": r["iterations"],
         "unrestricted": r["unrestricted"],
     }

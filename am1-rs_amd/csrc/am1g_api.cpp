@@ -25,15 +25,17 @@
 
 using namespace am1g;
 
+// Error text of the last failing call made on this thread. Each thread owns its copy, so threads
+// that share a context never read a buffer another thread is rewriting.
 thread_local std::string g_tls_error;
 
 int am1g_set_error(am1g_ctx* ctx, int code, const char* fmt, ...) {
+    (void)ctx;
     char buf[512];
     va_list ap;
     va_start(ap, fmt);
     std::vsnprintf(buf, sizeof(buf), fmt, ap);
     va_end(ap);
-    if (ctx) ctx->err = buf;
     g_tls_error = buf;
     return code;
 }
@@ -229,7 +231,10 @@ void am1g_destroy(am1g_ctx* ctx) {
     delete ctx;
 }
 
-const char* am1g_last_error(const am1g_ctx* ctx) { return ctx ? ctx->err.c_str() : g_tls_error.c_str(); }
+const char* am1g_last_error(const am1g_ctx* ctx) {
+    (void)ctx;
+    return g_tls_error.c_str();
+}
 
 double am1g_last_kernel_ms(const am1g_ctx* ctx) { return ctx ? ctx->last_ms : 0.0; }
 

@@ -70,7 +70,6 @@ struct am1g_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
     std::mutex mu;
-    std::string err;
     DevElem host_tab[kMethods * kMaxZ];
     DevBuf elems, mols, z, xyz, mats, wpool, hist, mo_e, mo_eb, charges, results, queue, order;
     DevBuf alist, aoff, acls, later, pair_off, ecore_pair;

@@ -665,6 +665,10 @@ __device__ __noinline__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolS
     bool fp32_phase = (PREC == 1) && md.precision == 1;
     double fp32_best = INFINITY;
     int fp32_stall = 0;
+    // iterations run with FP64 GEMMs; convergence needs two of them, so that the energy change and
+    // the density change compare two FP64 densities (after an FP32 phase as well)
+    int fp64_iters = 0;
+    int loose_iters = 0;  // FP64 iterations that met the p_tol test
     for (int iter = 0; iter < md.max_scf; ++iter) {
         iterations = iter + 1;
         mol_fock(md, el, ms, wpool, H, P, P, 0.5, F, lds);
@@ -672,6 +676,7 @@ __device__ __noinline__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolS
         phase_mark(pc, 1);
         // error = FP - (FP)^T; F and P are symmetric so FP = F^T P
         const bool low = fp32_phase;
+        if (!low) ++fp64_iters;
         if (low) {
             wg_gemm_any<float, true, false>(nao, nao, nao, 1.0, F, nao, P, nao, T0, nao, lds, kLdsDoubles);
             ++fp32_iters;
@@ -877,7 +882,14 @@ __device__ __noinline__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolS
             }
             if (err <= 1.0e-4 || fp32_stall >= 2) fp32_phase = false;
         }
-        if (iter > 0 && de < md.e_tol && (dp < md.p_tol || err < 1.0e-7) && !low) {
+        // scf.rs:693 (RHF), :820 (UHF). A mixed-precision run takes another trajectory than the FP64 one, and the
+        // density-change test at p_tol stops up to ~1e-6 e short of the fixed point, so it ends by
+        // a test 100x tighter; should that one stall (a near-degenerate open shell can hold the
+        // error on a plateau), ten iterations after the p_tol test is met end the run as well.
+        const bool conv_loose = de < md.e_tol && (dp < md.p_tol || err < 1.0e-7);
+        const bool conv_tight = de < md.e_tol && (dp < 1.0e-2 * md.p_tol || err < 1.0e-7);
+        if (conv_loose && fp64_iters >= 2) ++loose_iters;
+        if (fp64_iters >= 2 && (md.precision == 1 ? (conv_tight || loose_iters > 10) : conv_loose)) {
             converged = 1;
             break;
         }
@@ -983,6 +995,10 @@ __device__ __noinline__ void scf_uhf(const MolDesc& md, const BatchArgs& g, MolS
     bool fp32_phase = (PREC == 1) && md.precision == 1;
     double fp32_best = INFINITY;
     int fp32_stall = 0;
+    // iterations run with FP64 GEMMs; convergence needs two of them, so that the energy change and
+    // the density change compare two FP64 densities (after an FP32 phase as well)
+    int fp64_iters = 0;
+    int loose_iters = 0;  // FP64 iterations that met the p_tol test
     for (int iter = 0; iter < md.max_scf; ++iter) {
         iterations = iter + 1;
         for (size_t idx = t; idx < nn; idx += kWg) Pt[idx] = Ps[0][idx] + Ps[1][idx];
@@ -996,6 +1012,7 @@ __device__ __noinline__ void scf_uhf(const MolDesc& md, const BatchArgs& g, MolS
         const double e_elec = 0.5 * wg_sum(pe, s);
         phase_mark(pc, 1);
         const bool low = fp32_phase;
+        if (!low) ++fp64_iters;
         if (low) ++fp32_iters;
         int slot = 0;
         if (diis) slot = ms.hcount < kDiisDepth ? (ms.head + ms.hcount) % kDiisDepth : ms.head;
@@ -1152,7 +1169,14 @@ __device__ __noinline__ void scf_uhf(const MolDesc& md, const BatchArgs& g, MolS
             }
             if (err <= 1.0e-4 || fp32_stall >= 2) fp32_phase = false;
         }
-        if (iter > 0 && de < md.e_tol && (dp < md.p_tol || err < 1.0e-7) && !low) {
+        // scf.rs:693 (RHF), :820 (UHF). A mixed-precision run takes another trajectory than the FP64 one, and the
+        // density-change test at p_tol stops up to ~1e-6 e short of the fixed point, so it ends by
+        // a test 100x tighter; should that one stall (a near-degenerate open shell can hold the
+        // error on a plateau), ten iterations after the p_tol test is met end the run as well.
+        const bool conv_loose = de < md.e_tol && (dp < md.p_tol || err < 1.0e-7);
+        const bool conv_tight = de < md.e_tol && (dp < 1.0e-2 * md.p_tol || err < 1.0e-7);
+        if (conv_loose && fp64_iters >= 2) ++loose_iters;
+        if (fp64_iters >= 2 && (md.precision == 1 ? (conv_tight || loose_iters > 10) : conv_loose)) {
             converged = 1;
             break;
         }

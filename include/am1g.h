@@ -192,7 +192,9 @@ const char* am1g_serve_last_error(const am1g_server* sv);
  * back-transformation; the same solver serves the SCF of systems above 256 basis functions. */
 int am1g_symmetric_eigen(am1g_ctx* ctx, int n, const double* a, double* eigenvalues, double* eigenvectors);
 
-/* Text of the last error on this context (the Display text of the matching Am1Error). */
+/* Text of the last error raised on the calling thread by any am1g_* call that takes a context
+ * (the Display text of the matching Am1Error). Thread-local: concurrent callers sharing a
+ * context each read their own message. */
 const char* am1g_last_error(const am1g_ctx* ctx);
 
 /* Derived element parameters as the library uses them (for parity checks):

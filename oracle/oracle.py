@@ -221,15 +221,20 @@ def scf(z, xyz_bohr, full=True, **kw):
 
 
 def scf_many(molecules, threads=0, **kw):
-    """Independent SCFs, OpenMP over molecules. ``molecules`` = list of (z, xyz_bohr)."""
+    """Independent SCFs, OpenMP over molecules. ``molecules`` = list of (z, xyz_bohr) or
+    (z, xyz_bohr, per_molecule_kwargs)."""
     L = lib()
     keep = []
     n = len(molecules)
     rqs = (_Request * n)()
     rss = (_Result * n)()
     charges = []
-    for i, (z, xyz) in enumerate(molecules):
-        rqs[i] = _make_request(z, xyz, keep=keep, **kw)
+    for i, mol in enumerate(molecules):
+        z, xyz = mol[0], mol[1]
+        opts = dict(kw)
+        if len(mol) > 2 and mol[2]:
+            opts.update(mol[2])
+        rqs[i] = _make_request(z, xyz, keep=keep, **opts)
         q = np.zeros(len(z))
         charges.append(q)
         rss[i].charges = _dp(q)

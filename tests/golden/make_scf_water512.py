@@ -3,7 +3,12 @@
 A-DIIS -> CDIIS, e_tol 1e-8 eV, p_tol 1e-7). The values come from the CPU restatement (oracle/),
 not from the reference; the GPU test compares the large-system path against them at full size.
 
-Run from the repository root: OMP_NUM_THREADS=8 python tests/golden/make_scf_water512.py
+With --tight it writes tests/golden/scf_water512_tight.npz instead: the same SCF with p_tol
+1e-9, the density-change test of a mixed-precision run (100x the default), which ends within
+~1e-8 e of the fixed point. Mixed-precision runs, whose trajectory differs from the FP64 one, are
+checked against that.
+
+Run from the repository root: OMP_NUM_THREADS=8 python tests/golden/make_scf_water512.py [--tight]
 """
 import os
 import sys
@@ -23,12 +28,13 @@ def main():
     threads = int(os.environ.get("OMP_NUM_THREADS", "8"))
     O.set_lapack(True, threads)
     mol = G.water_cluster_config(2)
+    fixed = "--tight" in sys.argv[1:]
     t = time.time()
-    r = O.scf(mol.z, mol.xyz_bohr, full=True)
+    r = O.scf(mol.z, mol.xyz_bohr, full=True, **({"p_tol": 1e-9} if fixed else {}))
     dt = time.time() - t
     print({k: r[k] for k in ("status", "iterations", "total_ev", "homo_ev", "lumo_ev")}, f"{dt:.1f} s", flush=True)
     assert r["status"] == 0, r.get("message")
-    np.savez_compressed(os.path.join(os.path.dirname(os.path.abspath(__file__)), "scf_water512.npz"),
+    np.savez_compressed(os.path.join(os.path.dirname(os.path.abspath(__file__)), "scf_water512_tight.npz" if fixed else "scf_water512.npz"),
                         total_ev=r["total_ev"], electronic_ev=r["electronic_ev"], core_ev=r["core_ev"],
                         heat_of_formation_kcal=r["heat_of_formation_kcal"], homo_ev=r["homo_ev"],
                         lumo_ev=r["lumo_ev"], iterations=r["iterations"], charges=r["charges"],

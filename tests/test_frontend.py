@@ -1,0 +1,83 @@
+"""Python front end: ``am1g.single_point`` keeps am1-rs's ``native.single_point`` contract
+(python/am1_rs/native.py:52-97, src/python.rs:147-199): signature and positional order, the
+result keys the reference's own contract test requires (tests/test_result_contract.py:42-55),
+``ValueError`` for every bad input (python.rs:27-29)."""
+import inspect
+
+import numpy as np
+import pytest
+
+import am1g
+from oracle import oracle as O
+
+# tests/test_result_contract.py:42-55: the molecular energy report
+ENERGY_KEYS = {"energy_ev", "energy_hartree", "electronic_ev", "core_ev", "heat_of_formation_kcal", "homo_ev",
+               "lumo_ev", "dipole_debye", "dipole_magnitude", "charges", "iterations", "unrestricted"}
+# python.rs:166-198: every key single_point sets
+ALL_KEYS = ENERGY_KEYS | {"method", "homo_beta_ev", "lumo_beta_ev", "field_nuclear_ev", "converged"}
+
+WATER_NUMBERS = [8, 1, 1]
+WATER_POSITIONS = [[0.0, 0.0, 0.1173], [0.0, 0.7572, -0.4692], [0.0, -0.7572, -0.4692]]
+
+
+def test_signature_matches_the_reference():
+    params = list(inspect.signature(am1g.single_point).parameters.values())
+    assert [p.name for p in params] == ["numbers", "positions", "charge", "multiplicity", "reference", "method",
+                                        "electric_field"]
+    assert [p.default for p in params[2:]] == [0.0, 1, "auto", "am1", None]
+
+
+@pytest.mark.parametrize("numbers", [[262, 1, 1], [0, 1, 1], [8.5, 1, 1], [-1, 1, 1]])
+def test_out_of_range_atomic_numbers_raise(numbers):
+    # 262 would wrap to 6 (carbon) in a plain uint8 cast
+    with pytest.raises(ValueError):
+        am1g.single_point(numbers, WATER_POSITIONS)
+
+
+def test_shape_errors_raise():
+    with pytest.raises(ValueError):
+        am1g.single_point([8, 1], WATER_POSITIONS)
+    with pytest.raises(ValueError):
+        am1g.single_point(WATER_NUMBERS, [[0.0, 0.0], [1.0, 0.0], [0.0, 1.0]])
+    with pytest.raises(ValueError):
+        am1g.single_point(WATER_NUMBERS, WATER_POSITIONS, electric_field=[0.0, 0.0])
+    with pytest.raises(ValueError):
+        am1g.single_point(WATER_NUMBERS, WATER_POSITIONS, method="pm3")
+
+
+def test_engine_request_rejects_wrapping_numbers():
+    eng = am1g.Engine.__new__(am1g.Engine)
+    eng._L = am1g.load()
+    with pytest.raises(ValueError):
+        eng._request(np.array([262, 1, 1]), np.zeros((3, 3)), [])
+
+
+@pytest.mark.gpu
+def test_single_point_keys_and_values():
+    r = am1g.single_point(WATER_NUMBERS, WATER_POSITIONS)
+    assert ENERGY_KEYS <= set(r)
+    assert ALL_KEYS <= set(r)
+    xyz = np.array(WATER_POSITIONS) * am1g.ANGSTROM_TO_BOHR
+    e = am1g.engine().scf(WATER_NUMBERS, xyz, full=False)
+    assert r["energy_ev"] == e["total_ev"]
+    assert r["energy_hartree"] == pytest.approx(e["total_ev"] / 27.21, rel=1e-15)
+    assert r["electronic_ev"] == e["electronic_ev"] and r["core_ev"] == e["core_ev"]
+    assert r["charges"] == e["charges"].tolist()
+    assert r["dipole_magnitude"] == pytest.approx(float(np.linalg.norm(r["dipole_debye"])), rel=1e-14)
+    assert r["method"] == "am1" and r["converged"] and not r["unrestricted"]
+    assert r["homo_beta_ev"] is None and r["lumo_beta_ev"] is None and r["field_nuclear_ev"] == 0.0
+    o = O.scf(WATER_NUMBERS, xyz, full=False)
+    assert abs(r["energy_ev"] - o["total_ev"]) <= 1e-6
+
+
+@pytest.mark.gpu
+def test_single_point_positional_reference_and_method():
+    # positional order (numbers, positions, charge, multiplicity, reference, method)
+    methyl = [[0.0, 0.0, 0.0], [1.079, 0.0, 0.0], [-0.5395, 0.9344, 0.0], [-0.5395, -0.9344, 0.0]]
+    r = am1g.single_point([6, 1, 1, 1], methyl, 0.0, 2, "uhf", "rm1")
+    assert r["unrestricted"] and r["method"] == "rm1"
+    assert r["homo_beta_ev"] is not None
+    o = O.scf([6, 1, 1, 1], np.array(methyl) * am1g.ANGSTROM_TO_BOHR, full=False, multiplicity=2, reference="uhf",
+              method="rm1")
+    assert abs(r["energy_ev"] - o["total_ev"]) <= 1e-6
+    assert abs(r["homo_beta_ev"] - o["homo_beta_ev"]) <= 1e-6

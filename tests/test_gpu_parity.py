@@ -118,9 +118,22 @@ def test_batch_matches_oracle_and_is_batch_invariant(engine):
         assert alone["iterations"] == rb[i]["iterations"]
 
 
+def _within_contract(r, o):
+    """Energy, charges and frontier orbitals of one SCF against the oracle's (no iteration check)."""
+    ok = r["status"] == 0 and o["status"] == 0 and abs(r["total_ev"] - o["total_ev"]) <= E_TOL
+    ok = ok and np.abs(r["charges"] - o["charges"]).max() <= Q_TOL
+    for k in ("homo_ev", "lumo_ev"):
+        ok = ok and (o[k] is None or abs(r[k] - o[k]) <= ORB_TOL)
+    return bool(ok)
+
+
 def test_full_size_batch_properties(engine):
-    """4096 chains in one launch: every SCF converges, every neutral molecule's charges sum to
-    zero, and a sample of 48 agrees with the oracle."""
+    """4096 chains in one launch, FP64 and mixed precision: every SCF converges, every neutral
+    molecule's charges sum to zero, and every molecule agrees with the oracle in energy, charges
+    and HOMO/LUMO. FP64 follows the oracle's trajectory (same options, iteration count within
+    one). Mixed precision takes another trajectory; with the default density-change test the
+    oracle itself stops up to ~1.1e-6 e short of the fixed point, so a mixed run ends by a test
+    100x tighter (p_tol 1e-9) and is held to the oracle run with that same test."""
     from am1g import geometry as G
 
     mols = G.chain_batch(4096, seed=1)
@@ -128,10 +141,15 @@ def test_full_size_batch_properties(engine):
     rb = engine.scf_batch(ms)
     assert all(r["status"] == 0 for r in rb)
     assert max(abs(r["charges"].sum()) for r in rb) < 1e-8
-    idx = np.random.default_rng(0).choice(len(ms), 48, replace=False)
-    ob = O.scf_many([ms[i] for i in idx], threads=8)
-    for i, o in zip(idx, ob):
-        _compare(rb[i], o)
+    ob = O.scf_many(ms, threads=16)
+    bad = [i for i in range(len(ms)) if not _within_contract(rb[i], ob[i])]
+    assert not bad, f"{len(bad)} FP64 molecules outside the contract, first {bad[:5]}"
+    assert max(abs(a["iterations"] - b["iterations"]) for a, b in zip(rb, ob)) <= 1
+    rm = engine.scf_batch(ms, precision="mixed")
+    assert sum(r["fp32_iterations"] for r in rm) > 0
+    of = O.scf_many(ms, threads=16, p_tol=1e-9)
+    bad = [i for i in range(len(ms)) if not _within_contract(rm[i], of[i])]
+    assert not bad, f"{len(bad)} mixed-precision molecules outside the contract, first {bad[:5]}"
 
 
 # ------------------------------------------------------------------------------ UHF
