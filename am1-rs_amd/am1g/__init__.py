@@ -160,7 +160,8 @@ DC_PHASES = ["fock", "eigen", "fermi", "assembly", "diis"]
 EXPORTS = ["am1g_request_defaults", "am1g_create", "am1g_destroy", "am1g_scf", "am1g_scf_batch",
            "am1g_dc_defaults", "am1g_dc", "am1g_serve_defaults", "am1g_serve_create", "am1g_serve_destroy",
            "am1g_serve_submit", "am1g_serve_poll", "am1g_serve_cancel", "am1g_serve_get_stats",
-           "am1g_serve_last_error", "am1g_symmetric_eigen", "am1g_last_error", "am1g_element_info", "am1g_last_kernel_ms"]
+           "am1g_serve_last_error", "am1g_symmetric_eigen", "am1g_last_error", "am1g_element_info", "am1g_last_kernel_ms",
+           "am1g_scf_batch_noalloc", "am1g_noalloc_capacity_bytes"]
 
 _lib = None
 
@@ -184,6 +185,10 @@ def load():
     L.am1g_destroy.argtypes = [ctypes.c_void_p]
     L.am1g_scf.argtypes = [ctypes.c_void_p, ctypes.POINTER(Request), ctypes.POINTER(Result)]
     L.am1g_scf_batch.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(Request), ctypes.POINTER(Result)]
+    L.am1g_scf_batch_noalloc.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(Request),
+                                         ctypes.POINTER(Result)]
+    L.am1g_noalloc_capacity_bytes.argtypes = []
+    L.am1g_noalloc_capacity_bytes.restype = ctypes.c_longlong
     L.am1g_dc_defaults.argtypes = [ctypes.POINTER(DcOptions)]
     L.am1g_dc.argtypes = [ctypes.c_void_p, ctypes.POINTER(Request), ctypes.POINTER(DcOptions),
                           ctypes.POINTER(DcResult)]
@@ -364,9 +369,10 @@ class Engine:
             d["message"] = self.last_error()
         return d
 
-    def scf_batch(self, molecules, full: bool = False, **kw) -> list:
-        """Many SCFs in one persistent launch (``am1g_scf_batch``). ``molecules`` is a list of
-        ``(z, xyz_bohr)`` or ``(z, xyz_bohr, per_molecule_kwargs)``."""
+    def scf_batch(self, molecules, full: bool = False, noalloc: bool = False, **kw) -> list:
+        """Many SCFs in one persistent launch (``am1g_scf_batch``; ``am1g_scf_batch_noalloc`` with
+        ``noalloc``). ``molecules`` is a list of ``(z, xyz_bohr)`` or
+        ``(z, xyz_bohr, per_molecule_kwargs)``."""
         keep = []
         n = len(molecules)
         rqs = (Request * n)()
@@ -380,7 +386,8 @@ class Engine:
             rqs[i] = self._request(z, xyz, keep, **opts)
             arrays.append(self._attach(rss[i], len(z), full, keep))
         t0 = time.perf_counter()
-        st = self._L.am1g_scf_batch(self._ctx, n, rqs, rss)
+        entry = self._L.am1g_scf_batch_noalloc if noalloc else self._L.am1g_scf_batch
+        st = entry(self._ctx, n, rqs, rss)
         # the C call alone (host tables, copies, launches, result fill): what a native caller waits
         self.last_call_ms = (time.perf_counter() - t0) * 1e3
         if st != 0:

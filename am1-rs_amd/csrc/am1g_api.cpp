@@ -178,6 +178,9 @@ void am1g_fill_result(const MolResult& r, const MolDesc& md, am1g_result& out) {
     out.fp32_iterations = (int)r.fp32_iterations;
 }
 
+static int batch_impl(am1g_ctx* ctx, int n, const am1g_request* rqs, am1g_result* results,
+                      const std::function<void(int)>& on_done, bool noalloc);
+
 extern "C" {
 
 void am1g_request_defaults(am1g_request* rq) {
@@ -210,6 +213,8 @@ int am1g_create(int device, am1g_ctx** out) {
     HIP_TRY(ctx, hipGetDeviceProperties(&prop, device));
     ctx->cus = prop.multiProcessorCount;
     HIP_TRY(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    // device address of this GPU's scratch array for am1g_scf_batch_noalloc (once per device)
+    HIP_TRY(ctx, scratch_init(device));
     HIP_TRY(ctx, hipEventCreate(&ctx->ev0));
     HIP_TRY(ctx, hipEventCreate(&ctx->ev1));
     build_element_tables(ctx->host_tab);
@@ -255,11 +260,25 @@ int am1g_scf_batch(am1g_ctx* ctx, int n, const am1g_request* rqs, am1g_result* r
     return am1g_scf_batch_stream(ctx, n, rqs, results, std::function<void(int)>());
 }
 
+int am1g_scf_batch_noalloc(am1g_ctx* ctx, int n, const am1g_request* rqs, am1g_result* results) {
+    return batch_impl(ctx, n, rqs, results, std::function<void(int)>(), true);
+}
+
+long long am1g_noalloc_capacity_bytes(void) { return (long long)kScratchBytes; }
+
 }  // extern "C"
 
 int am1g_scf_batch_stream(am1g_ctx* ctx, int n, const am1g_request* rqs, am1g_result* results,
                           const std::function<void(int)>& on_done) {
+    return batch_impl(ctx, n, rqs, results, on_done, false);
+}
+
+// The batched launch. Its device pools come from the context's grow-only buffers, or with
+// `noalloc` from the per-device scratch array (no allocation at all; sizes beyond it are refused).
+static int batch_impl(am1g_ctx* ctx, int n, const am1g_request* rqs, am1g_result* results,
+                      const std::function<void(int)>& on_done, bool noalloc) {
     const bool streaming = (bool)on_done;
+    if (noalloc && streaming) return am1g_set_error(ctx, AM1G_INVALID_INPUT, "streaming needs the allocating entry point");
     if (!ctx) return am1g_set_error(nullptr, AM1G_INVALID_INPUT, "null context");
     if (n < 0 || (n > 0 && (!rqs || !results))) return am1g_set_error(ctx, AM1G_INVALID_INPUT, "bad batch arguments");
     std::lock_guard<std::mutex> lock(ctx->mu);
@@ -339,59 +358,92 @@ int am1g_scf_batch_stream(am1g_ctx* ctx, int n, const am1g_request* rqs, am1g_re
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return descs[a].nao > descs[b].nao; });
 
-    HIP_TRY(ctx, ctx->mols.reserve(sizeof(MolDesc) * m));
-    HIP_TRY(ctx, ctx->z.reserve(sizeof(int) * zs.size()));
-    HIP_TRY(ctx, ctx->xyz.reserve(sizeof(double) * xyz.size()));
-    HIP_TRY(ctx, ctx->mats.reserve(sizeof(double) * mat_off));
-    HIP_TRY(ctx, ctx->wpool.reserve(sizeof(double) * std::max<int64_t>(w_off, 1)));
-    HIP_TRY(ctx, ctx->hist.reserve(sizeof(double) * hist_off));
-    HIP_TRY(ctx, ctx->mo_e.reserve(sizeof(double) * ao_off));
-    HIP_TRY(ctx, ctx->mo_eb.reserve(sizeof(double) * ao_off));
-    HIP_TRY(ctx, ctx->charges.reserve(sizeof(double) * atom_off));
-    HIP_TRY(ctx, ctx->results.reserve(sizeof(MolResult) * m));
-    HIP_TRY(ctx, ctx->queue.reserve(sizeof(int) * 4));
-    HIP_TRY(ctx, ctx->order.reserve(sizeof(int) * m));
-    HIP_TRY(ctx, ctx->alist.reserve(sizeof(int) * alist.size()));
-    HIP_TRY(ctx, ctx->aoff.reserve(sizeof(int) * aoff.size()));
-    HIP_TRY(ctx, ctx->acls.reserve(sizeof(int) * acls.size()));
-    HIP_TRY(ctx, ctx->pair_off.reserve(sizeof(int64_t) * pair_off.size()));
-    HIP_TRY(ctx, ctx->ecore_pair.reserve(sizeof(double) * std::max<int64_t>(pair_off.back(), 1)));
+    // device pools: (buffer, bytes) in one fixed order
+    struct Pool {
+        DevBuf* buf;
+        size_t bytes;
+        void* p;
+    };
+    Pool pools[] = {
+        {&ctx->mols, sizeof(MolDesc) * m, nullptr},
+        {&ctx->z, sizeof(int) * zs.size(), nullptr},
+        {&ctx->xyz, sizeof(double) * xyz.size(), nullptr},
+        {&ctx->mats, sizeof(double) * mat_off, nullptr},
+        {&ctx->wpool, sizeof(double) * std::max<int64_t>(w_off, 1), nullptr},
+        {&ctx->hist, sizeof(double) * hist_off, nullptr},
+        {&ctx->mo_e, sizeof(double) * ao_off, nullptr},
+        {&ctx->mo_eb, sizeof(double) * ao_off, nullptr},
+        {&ctx->charges, sizeof(double) * atom_off, nullptr},
+        {&ctx->results, sizeof(MolResult) * m, nullptr},
+        {&ctx->queue, sizeof(int) * 4, nullptr},
+        {&ctx->order, sizeof(int) * m, nullptr},
+        {&ctx->alist, sizeof(int) * alist.size(), nullptr},
+        {&ctx->aoff, sizeof(int) * aoff.size(), nullptr},
+        {&ctx->acls, sizeof(int) * acls.size(), nullptr},
+        {&ctx->pair_off, sizeof(int64_t) * pair_off.size(), nullptr},
+        {&ctx->ecore_pair, sizeof(double) * std::max<int64_t>(pair_off.back(), 1), nullptr},
+    };
+    constexpr int kPools = (int)(sizeof(pools) / sizeof(pools[0]));
+    enum { pMols, pZ, pXyz, pMats, pW, pHist, pMo, pMob, pQ, pRes, pQueue, pOrder, pAlist, pAoff, pAcls, pPairOff,
+           pEcore };
+    if (noalloc) {
+        char* base = static_cast<char*>(scratch_ptr(ctx->device));
+        if (!base)
+            return am1g_set_error(ctx, AM1G_DEVICE_ERROR, "device scratch of device %d is not set up", ctx->device);
+        size_t used = 0;
+        for (int k = 0; k < kPools; ++k) used = ((used + 255) & ~(size_t)255) + pools[k].bytes;
+        if (used > kScratchBytes)
+            return am1g_set_error(ctx, AM1G_INVALID_INPUT,
+                                  "batch needs %.1f MiB of device memory; the allocation-free entry point has %.1f MiB",
+                                  used / 1048576.0, kScratchBytes / 1048576.0);
+        used = 0;
+        for (int k = 0; k < kPools; ++k) {
+            used = (used + 255) & ~(size_t)255;
+            pools[k].p = base + used;
+            used += pools[k].bytes;
+        }
+    } else {
+        for (int k = 0; k < kPools; ++k) {
+            HIP_TRY(ctx, pools[k].buf->reserve(pools[k].bytes));
+            pools[k].p = pools[k].buf->p;
+        }
+    }
     hipStream_t st = ctx->stream;
-    HIP_TRY(ctx, hipMemcpyAsync(ctx->mols.p, descs.data(), sizeof(MolDesc) * m, hipMemcpyHostToDevice, st));
-    HIP_TRY(ctx, hipMemcpyAsync(ctx->z.p, zs.data(), sizeof(int) * zs.size(), hipMemcpyHostToDevice, st));
-    HIP_TRY(ctx, hipMemcpyAsync(ctx->xyz.p, xyz.data(), sizeof(double) * xyz.size(), hipMemcpyHostToDevice, st));
-    HIP_TRY(ctx, hipMemcpyAsync(ctx->order.p, order.data(), sizeof(int) * m, hipMemcpyHostToDevice, st));
-    HIP_TRY(ctx, hipMemcpyAsync(ctx->alist.p, alist.data(), sizeof(int) * alist.size(), hipMemcpyHostToDevice, st));
-    HIP_TRY(ctx, hipMemcpyAsync(ctx->aoff.p, aoff.data(), sizeof(int) * aoff.size(), hipMemcpyHostToDevice, st));
-    HIP_TRY(ctx, hipMemcpyAsync(ctx->acls.p, acls.data(), sizeof(int) * acls.size(), hipMemcpyHostToDevice, st));
-    HIP_TRY(ctx, hipMemcpyAsync(ctx->pair_off.p, pair_off.data(), sizeof(int64_t) * pair_off.size(),
+    HIP_TRY(ctx, hipMemcpyAsync(pools[pMols].p, descs.data(), sizeof(MolDesc) * m, hipMemcpyHostToDevice, st));
+    HIP_TRY(ctx, hipMemcpyAsync(pools[pZ].p, zs.data(), sizeof(int) * zs.size(), hipMemcpyHostToDevice, st));
+    HIP_TRY(ctx, hipMemcpyAsync(pools[pXyz].p, xyz.data(), sizeof(double) * xyz.size(), hipMemcpyHostToDevice, st));
+    HIP_TRY(ctx, hipMemcpyAsync(pools[pOrder].p, order.data(), sizeof(int) * m, hipMemcpyHostToDevice, st));
+    HIP_TRY(ctx, hipMemcpyAsync(pools[pAlist].p, alist.data(), sizeof(int) * alist.size(), hipMemcpyHostToDevice, st));
+    HIP_TRY(ctx, hipMemcpyAsync(pools[pAoff].p, aoff.data(), sizeof(int) * aoff.size(), hipMemcpyHostToDevice, st));
+    HIP_TRY(ctx, hipMemcpyAsync(pools[pAcls].p, acls.data(), sizeof(int) * acls.size(), hipMemcpyHostToDevice, st));
+    HIP_TRY(ctx, hipMemcpyAsync(pools[pPairOff].p, pair_off.data(), sizeof(int64_t) * pair_off.size(),
                                 hipMemcpyHostToDevice, st));
-    HIP_TRY(ctx, hipMemsetAsync(ctx->queue.p, 0, sizeof(int) * 4, st));
+    HIP_TRY(ctx, launch_zero_ints(static_cast<int*>(pools[pQueue].p), 4, st));
 
     BatchArgs a;
     std::memset(&a, 0, sizeof(a));
-    a.mols = ctx->mols.as<MolDesc>();
+    a.mols = static_cast<MolDesc*>(pools[pMols].p);
     a.n_mols = m;
     a.lds_n = 192;
     a.tri_max = eigen_tri_max();
-    a.z = ctx->z.as<int>();
-    a.xyz = ctx->xyz.as<double>();
+    a.z = static_cast<int*>(pools[pZ].p);
+    a.xyz = static_cast<double*>(pools[pXyz].p);
     a.elems = ctx->elems.as<DevElem>();
-    a.mats = ctx->mats.as<double>();
-    a.wpool = ctx->wpool.as<double>();
-    a.hist = ctx->hist.as<double>();
-    a.mo_e = ctx->mo_e.as<double>();
-    a.mo_e_beta = ctx->mo_eb.as<double>();
-    a.charges = ctx->charges.as<double>();
-    a.results = ctx->results.as<MolResult>();
-    a.queue = ctx->queue.as<int>();
-    a.order = ctx->order.as<int>();
-    a.alist = ctx->alist.as<int>();
-    a.aoff = ctx->aoff.as<int>();
-    a.acls = ctx->acls.as<int>();
-    a.pair_off = ctx->pair_off.as<int64_t>();
+    a.mats = static_cast<double*>(pools[pMats].p);
+    a.wpool = static_cast<double*>(pools[pW].p);
+    a.hist = static_cast<double*>(pools[pHist].p);
+    a.mo_e = static_cast<double*>(pools[pMo].p);
+    a.mo_e_beta = static_cast<double*>(pools[pMob].p);
+    a.charges = static_cast<double*>(pools[pQ].p);
+    a.results = static_cast<MolResult*>(pools[pRes].p);
+    a.queue = static_cast<int*>(pools[pQueue].p);
+    a.order = static_cast<int*>(pools[pOrder].p);
+    a.alist = static_cast<int*>(pools[pAlist].p);
+    a.aoff = static_cast<int*>(pools[pAoff].p);
+    a.acls = static_cast<int*>(pools[pAcls].p);
+    a.pair_off = static_cast<int64_t*>(pools[pPairOff].p);
     a.n_pairs = pair_off.back();
-    a.ecore_pair = ctx->ecore_pair.as<double>();
+    a.ecore_pair = static_cast<double*>(pools[pEcore].p);
     size_t lds = (size_t)kLdsBytes;
     int precision = 0;
     for (int j = 0; j < m; ++j) precision |= descs[j].precision;
@@ -456,10 +508,10 @@ int am1g_scf_batch_stream(am1g_ctx* ctx, int n, const am1g_request* rqs, am1g_re
 
     std::vector<MolResult> res(m);
     std::vector<double> mo(ao_off), mob(ao_off), q(atom_off);
-    HIP_TRY(ctx, hipMemcpyAsync(res.data(), ctx->results.p, sizeof(MolResult) * m, hipMemcpyDeviceToHost, st));
-    HIP_TRY(ctx, hipMemcpyAsync(mo.data(), ctx->mo_e.p, sizeof(double) * ao_off, hipMemcpyDeviceToHost, st));
-    HIP_TRY(ctx, hipMemcpyAsync(q.data(), ctx->charges.p, sizeof(double) * atom_off, hipMemcpyDeviceToHost, st));
-    HIP_TRY(ctx, hipMemcpyAsync(mob.data(), ctx->mo_eb.p, sizeof(double) * ao_off, hipMemcpyDeviceToHost, st));
+    HIP_TRY(ctx, hipMemcpyAsync(res.data(), pools[pRes].p, sizeof(MolResult) * m, hipMemcpyDeviceToHost, st));
+    HIP_TRY(ctx, hipMemcpyAsync(mo.data(), pools[pMo].p, sizeof(double) * ao_off, hipMemcpyDeviceToHost, st));
+    HIP_TRY(ctx, hipMemcpyAsync(q.data(), pools[pQ].p, sizeof(double) * atom_off, hipMemcpyDeviceToHost, st));
+    HIP_TRY(ctx, hipMemcpyAsync(mob.data(), pools[pMob].p, sizeof(double) * ao_off, hipMemcpyDeviceToHost, st));
     HIP_TRY(ctx, hipStreamSynchronize(st));
     float ms = 0.f;
     HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
@@ -482,10 +534,10 @@ int am1g_scf_batch_stream(am1g_ctx* ctx, int n, const am1g_request* rqs, am1g_re
         const size_t nn = (size_t)md.nao * md.nao;
         if (r.density) {
             const int slot = md.uhf ? kPt : kP;
-            HIP_TRY(ctx, hipMemcpy(r.density, ctx->mats.as<double>() + md.mat_off + slot * nn, nn * sizeof(double),
+            HIP_TRY(ctx, hipMemcpy(r.density, static_cast<double*>(pools[pMats].p) + md.mat_off + slot * nn, nn * sizeof(double),
                                    hipMemcpyDeviceToHost));
         }
-        const double* M = ctx->mats.as<double>() + md.mat_off;
+        const double* M = static_cast<double*>(pools[pMats].p) + md.mat_off;
         // eigenvectors are stored transposed (row k = MO k); the result holds MOs as columns
         auto columns = [&](int slot, double* out) -> hipError_t {
             mat.resize(nn);

@@ -54,6 +54,16 @@ struct BatchArgs {
 };
 
 hipError_t launch_scf_batch(const BatchArgs& a, int precision, int grid, size_t lds_bytes, hipStream_t stream);
+// Size of the per-device scratch array of the allocation-free batched entry point.
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+constexpr int kMaxDevices = 64;
+// Looks up the device address of the scratch array on `device`, which must be current; once per
+// device, later calls return at once.
+hipError_t scratch_init(int device);
+// Device address of the scratch array on `device` (null before scratch_init succeeded there).
+void* scratch_ptr(int device);
+// Zeroes n ints with a kernel (no memset node, so the sequence stays capturable).
+hipError_t launch_zero_ints(int* p, int n, hipStream_t stream);
 // Largest order for the tridiagonal eigensolver: kTriMax, or 0 when AM1G_EIGEN_JACOBI=1 (every
 // eigen-step then takes the Jacobi solver; for testing that solver on the same molecules).
 int eigen_tri_max();

@@ -110,6 +110,20 @@ int am1g_scf(am1g_ctx* ctx, const am1g_request* rq, am1g_result* res);
  * the launch ran; per-molecule outcomes are in results[i].status. */
 int am1g_scf_batch(am1g_ctx* ctx, int n, const am1g_request* rqs, am1g_result* results);
 
+/* Allocation-free batched path. No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). Same kernels and results as am1g_scf_batch, but every device buffer of the
+ * launch comes from one fixed scratch array per GPU (am1g_noalloc_capacity_bytes(), 64 MiB),
+ * reserved when the library's device code is loaded and held until the process ends; the call
+ * makes no hipMalloc / hipFree. A batch whose buffers exceed the array is refused with
+ * AM1G_INVALID_INPUT and a message giving both sizes (as a guide, about 25 molecules of 128
+ * basis functions or 90 of 70). The array keeps the last call's contents; calls that overlap on
+ * one GPU (from two contexts or two threads without a shared context) race on it. Density and
+ * orbital matrices are returned as with am1g_scf_batch. */
+int am1g_scf_batch_noalloc(am1g_ctx* ctx, int n, const am1g_request* rqs, am1g_result* results);
+/* Bytes of the per-GPU scratch array of am1g_scf_batch_noalloc. */
+long long am1g_noalloc_capacity_bytes(void);
+
 /* Divide-and-conquer controls: DcOptions (src/divide_conquer.rs:66-119). */
 typedef struct {
     int core_size;            /* target atoms per core when core_of_atom is NULL (default 12) */
