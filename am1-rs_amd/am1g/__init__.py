@@ -54,6 +54,7 @@ class Request(ctypes.Structure):
         ("p_tol", ctypes.c_double),
         ("adiis_switch", ctypes.c_double),
         ("precision", ctypes.c_int),
+        ("electric_field", ctypes.POINTER(ctypes.c_double)),
     ]
 
 
@@ -80,6 +81,7 @@ class Result(ctypes.Structure):
         ("lumo_beta_ev", ctypes.c_double),
         ("dipole_debye", ctypes.c_double * 3),
         ("dipole_magnitude", ctypes.c_double),
+        ("field_nuclear_ev", ctypes.c_double),
         ("residual", ctypes.c_double),
         ("fp32_iterations", ctypes.c_int),
         ("latency_ms", ctypes.c_double),
@@ -268,7 +270,8 @@ class Engine:
     # -- request marshalling -------------------------------------------------------------
     def _request(self, z, xyz_bohr, keep, charge=0.0, multiplicity=1, method="am1", reference="auto",
                  accelerator="adiis_cdiis", use_diis=True, max_scf=200, e_tol=1e-8, p_tol=1e-7,
-                 adiis_switch=0.1, precision="fp64"):
+                 adiis_switch=0.1, precision="fp64", electric_field=None):
+        """``electric_field``: three floats in eV per (e*Bohr) (Am1Options::electric_field) or None."""
         rq = Request()
         self._L.am1g_request_defaults(ctypes.byref(rq))
         zi = np.asarray(z)
@@ -292,6 +295,12 @@ class Engine:
         rq.p_tol = float(p_tol)
         rq.adiis_switch = float(adiis_switch)
         rq.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        if electric_field is not None:
+            fa = np.ascontiguousarray(electric_field, dtype=np.float64).reshape(-1)
+            if fa.size != 3:
+                raise ValueError("electric_field must have three components")
+            keep.append(fa)
+            rq.electric_field = fa.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
         return rq
 
     @staticmethod
@@ -340,6 +349,7 @@ class Engine:
             "lumo_beta_ev": rs.lumo_beta_ev if rs.has_lumo_beta else None,
             "dipole_debye": np.array(list(rs.dipole_debye)),
             "dipole_magnitude": rs.dipole_magnitude,
+            "field_nuclear_ev": rs.field_nuclear_ev,
             "residual": rs.residual,
             "fp32_iterations": rs.fp32_iterations,
             "latency_ms": rs.latency_ms,
@@ -556,17 +566,18 @@ def single_point(numbers, positions, charge=0.0, multiplicity=1, reference="auto
         raise ValueError("each position must have 3 components")
     if len(z) != pos.shape[0]:
         raise ValueError("numbers and positions length mismatch")
+    field = None
     if electric_field is not None:
         f = np.asarray(electric_field, dtype=float).reshape(-1)
         if f.size != 3:
             raise ValueError("electric_field must have three components")
-        raise ValueError("an external electric field is not supported by this engine")
+        field = f * AM1_EV  # atomic units (Hartree per e*Bohr) -> eV per e*Bohr (python.rs:55-70)
     if str(method).lower() not in METHODS:
         raise ValueError(f"unknown method {method!r} (expected 'am1' or 'rm1')")
     if str(reference).lower() not in REFERENCES:
         raise ValueError(f"unknown reference {reference!r} (expected 'auto', 'rhf' or 'uhf')")
     r = engine().scf(z, pos * ANGSTROM_TO_BOHR, full=False, charge=float(charge), multiplicity=max(1, int(multiplicity)),
-                     method=str(method).lower(), reference=str(reference).lower())
+                     method=str(method).lower(), reference=str(reference).lower(), electric_field=field)
     if r["status"] == 4:
         raise ValueError(f"SCF did not converge in {r['iterations']} iterations "
                          f"(residual {r['residual']:.3e})")
@@ -586,7 +597,7 @@ def single_point(numbers, positions, charge=0.0, multiplicity=1, reference="auto
         "lumo_ev": r["lumo_ev"],
         "homo_beta_ev": r["homo_beta_ev"],
         "lumo_beta_ev": r["lumo_beta_ev"],
-        "field_nuclear_ev": r.get("field_nuclear_ev", 0.0),
+        "field_nuclear_ev": r["field_nuclear_ev"],
         "converged": r["converged"],
         "iterations": r["iterations"],
         "unrestricted": r["unrestricted"],

@@ -143,6 +143,20 @@ int am1g_prepare(am1g_ctx* ctx, const am1g_request& rq, Prepared& out, std::stri
     md.e_tol = rq.e_tol;
     md.p_tol = rq.p_tol;
     md.adiis_switch = rq.adiis_switch;
+    if (rq.electric_field) {
+        // the nuclear half of -mu.F (dipole.rs:146-153); the electronic half enters H_core
+        double nuc = 0.0;
+        for (int k = 0; k < 3; ++k) {
+            if (!std::isfinite(rq.electric_field[k])) {
+                msg = "non-finite electric field";
+                return AM1G_INVALID_INPUT;
+            }
+            md.field[k] = rq.electric_field[k];
+        }
+        for (int i = 0; i < rq.n_atoms; ++i)
+            for (int k = 0; k < 3; ++k) nuc += tab[rq.z[i]].core * rq.xyz_bohr[3 * i + k] * md.field[k];
+        md.field_nuc = -nuc;
+    }
     out.nat = rq.n_atoms;
     out.uhf = uhf;
     return AM1G_OK;
@@ -174,6 +188,7 @@ void am1g_fill_result(const MolResult& r, const MolDesc& md, am1g_result& out) {
     out.has_lumo_beta = md.uhf && std::isfinite(r.lumo_beta_ev);
     for (int k = 0; k < 3; ++k) out.dipole_debye[k] = r.dipole[k];
     out.dipole_magnitude = std::sqrt(r.dipole[0] * r.dipole[0] + r.dipole[1] * r.dipole[1] + r.dipole[2] * r.dipole[2]);
+    out.field_nuclear_ev = md.field_nuc;
     out.residual = r.residual;
     out.fp32_iterations = (int)r.fp32_iterations;
 }

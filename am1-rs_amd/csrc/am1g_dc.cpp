@@ -164,6 +164,9 @@ int am1g_dc(am1g_ctx* ctx, const am1g_request* rq, const am1g_dc_options* op, am
     res->converged = 0;
     if (!(op->buffer_radius > 0.0))
         return res->status = am1g_set_error(ctx, AM1G_INVALID_INPUT, "divide-and-conquer buffer_radius must be positive");
+    if (rq->electric_field)
+        return res->status =
+                   am1g_set_error(ctx, AM1G_INVALID_INPUT, "an external electric field is not supported by am1g_dc");
     Prepared prep;
     std::string msg;
     int st = am1g_prepare(ctx, *rq, prep, msg, INT_MAX);

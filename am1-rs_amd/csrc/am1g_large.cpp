@@ -296,8 +296,8 @@ static int large_uhf(am1g_ctx* ctx, const am1g_request& rq, const MolDesc& md, c
     r.nao = n;
     r.electronic_ev = electronic;
     r.core_ev = ecore;
-    r.total_ev = electronic + ecore;
-    r.heat_of_formation_kcal = (electronic + ecore - eiso + eheat) * kEvToKcal;
+    r.total_ev = electronic + ecore + md.field_nuc;
+    r.heat_of_formation_kcal = (electronic + ecore + md.field_nuc - eiso + eheat) * kEvToKcal;
     r.homo_ev = na >= 1 ? heps[na - 1] : NAN;
     r.lumo_ev = na < n ? heps[na] : NAN;
     r.homo_beta_ev = nb >= 1 ? hepsb[nb - 1] : NAN;
@@ -670,8 +670,8 @@ int am1g_large_scf(am1g_ctx* ctx, const am1g_request& rq, const Prepared& prep, 
     r.nao = n;
     r.electronic_ev = electronic;
     r.core_ev = ecore;
-    r.total_ev = electronic + ecore;
-    r.heat_of_formation_kcal = (electronic + ecore - eiso + eheat) * kEvToKcal;
+    r.total_ev = electronic + ecore + md.field_nuc;
+    r.heat_of_formation_kcal = (electronic + ecore + md.field_nuc - eiso + eheat) * kEvToKcal;
     r.homo_ev = nocc >= 1 ? heps[nocc - 1] : NAN;
     r.lumo_ev = nocc < n ? heps[nocc] : NAN;
     r.homo_beta_ev = NAN;

@@ -205,6 +205,11 @@ __device__ double mol_setup(const MolDesc& md, int mi, const BatchArgs& g, MolSh
             acc += -el[ms.z[y]].core * wv;
         }
         double u = (mu == nu) ? (mu == 0 ? ex.uss : ex.upp) : 0.0;
+        // uniform external field F . M (dipole.rs:126-144): F . R_x on the diagonal, F_alpha dd on
+        // the s-p_alpha entries (mu = 1 + alpha, nu = 0 in the packed order)
+        const double* rx = g.xyz + (md.atom_off + x) * 3;
+        if (mu == nu) u += md.field[0] * rx[0] + md.field[1] * rx[1] + md.field[2] * rx[2];
+        else if (nu == 0) u += md.field[mu - 1] * ex.dd;
         int o = ms.off[x];
         H[(size_t)(o + mu) * nao + o + nu] = u + acc;
         H[(size_t)(o + nu) * nao + o + mu] = u + acc;
@@ -931,8 +936,8 @@ __device__ __noinline__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolS
         res.nao = nao;
         res.electronic_ev = electronic;
         res.core_ev = ecore;
-        res.total_ev = electronic + ecore;
-        res.heat_of_formation_kcal = (electronic + ecore - eiso + eheat) * kEvToKcal;
+        res.total_ev = electronic + ecore + md.field_nuc;
+        res.heat_of_formation_kcal = (electronic + ecore + md.field_nuc - eiso + eheat) * kEvToKcal;
         res.homo_ev = nocc >= 1 ? eps[nocc - 1] : NAN;
         res.lumo_ev = nocc < nao ? eps[nocc] : NAN;
         res.homo_beta_ev = NAN;
@@ -1218,8 +1223,8 @@ __device__ __noinline__ void scf_uhf(const MolDesc& md, const BatchArgs& g, MolS
         res.nao = nao;
         res.electronic_ev = electronic;
         res.core_ev = ecore;
-        res.total_ev = electronic + ecore;
-        res.heat_of_formation_kcal = (electronic + ecore - eiso + eheat) * kEvToKcal;
+        res.total_ev = electronic + ecore + md.field_nuc;
+        res.heat_of_formation_kcal = (electronic + ecore + md.field_nuc - eiso + eheat) * kEvToKcal;
         res.homo_ev = na >= 1 ? eps[0][na - 1] : NAN;
         res.lumo_ev = na < nao ? eps[0][na] : NAN;
         res.homo_beta_ev = nb >= 1 ? eps[1][nb - 1] : NAN;
@@ -2268,11 +2273,30 @@ __global__ __launch_bounds__(256) void k_dc_update(int64_t npat, const int64_t* 
 // ---------------------------------------------------------------- launchers
 static inline unsigned blocks_for(int64_t n, int per = 256) { return (unsigned)((n + per - 1) / per); }
 
+// Uniform external field on H_core (dipole.rs:126-144): F . R_a on each AO's diagonal entry and
+// F_alpha dd on the s-p_alpha entries of each heavy atom. One thread per atom.
+__global__ void k_field_hcore(SysArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.md.nat) return;
+    const int nao = a.md.nao, o = a.aoff[i];
+    const DevElem& e = a.elems[a.md.method * kMaxZ + a.z[i]];
+    const double* r = a.xyz + 3 * (size_t)i;
+    const double fr = a.md.field[0] * r[0] + a.md.field[1] * r[1] + a.md.field[2] * r[2];
+    for (int k = 0; k < e.norb; ++k) a.H[(size_t)(o + k) * nao + o + k] += fr;
+    if (e.norb == 4)
+        for (int k = 0; k < 3; ++k) {
+            a.H[(size_t)o * nao + o + 1 + k] += a.md.field[k] * e.dd;
+            a.H[(size_t)(o + 1 + k) * nao + o] += a.md.field[k] * e.dd;
+        }
+}
+
 hipError_t launch_hcore_diag(const SysArgs& a, hipStream_t st) {
     if (a.md.nat > 0) {
         hipLaunchKernelGGL(k_pack_core, dim3(blocks_for(a.md.nat)), dim3(256), 0, st, a);
         hipLaunchKernelGGL(k_diag_blocks<false>, dim3(a.md.n_heavy + (a.md.n_h + 7) / 8), dim3(256), 0, st, a,
                            nullptr, nullptr, 0.0, nullptr, nullptr);
+        if (a.md.field[0] != 0.0 || a.md.field[1] != 0.0 || a.md.field[2] != 0.0)
+            hipLaunchKernelGGL(k_field_hcore, dim3(blocks_for(a.md.nat)), dim3(256), 0, st, a);
     }
     return hipGetLastError();
 }

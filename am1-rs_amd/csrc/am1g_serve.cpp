@@ -32,6 +32,7 @@ struct Job {
     am1g_request rq;
     std::vector<uint8_t> z;
     std::vector<double> xyz;
+    double field[3] = {0.0, 0.0, 0.0};  // the request's own copy of its external field
     size_t bytes = 0;
     JobState state = JobState::kQueued;
     bool cancelled = false;
@@ -279,6 +280,8 @@ int am1g_serve_submit(am1g_server* sv, const am1g_request* rq, long long* ticket
     j.rq = *rq;
     j.rq.z = j.z.data();
     j.rq.xyz_bohr = j.xyz.data();
+    if (rq->electric_field)
+        for (int k = 0; k < 3; ++k) j.field[k] = rq->electric_field[k];
     int nao = 0;
     for (uint8_t zz : j.z) nao += zz == 1 ? 1 : 4;
     j.large = nao > kRouteLargeAo || rq->n_atoms > kMaxAtoms;
@@ -295,6 +298,7 @@ int am1g_serve_submit(am1g_server* sv, const am1g_request* rq, long long* ticket
         slot = std::move(j);
         slot.rq.z = slot.z.data();  // the vectors moved with their heap buffers; re-point anyway
         slot.rq.xyz_bohr = slot.xyz.data();
+        slot.rq.electric_field = rq->electric_field ? slot.field : nullptr;
         sv->queue.push_back(t);
         sv->st.submitted++;
         sv->st.queued = (long long)sv->queue.size();

@@ -61,6 +61,9 @@ typedef struct {
     double p_tol;            /* default 1e-7 */
     double adiis_switch;     /* default 0.1 */
     int precision;           /* AM1G_PRECISION_*; FP32 GEMM phase until the DIIS error < 1e-4 */
+    const double* electric_field; /* [3] uniform external field in eV per (e*Bohr), or NULL for none
+                                     (Am1Options::electric_field, scf.rs:141-153; am1g_scf and
+                                     am1g_scf_batch; am1g_dc refuses a field) */
 } am1g_request;
 
 /* Result of one SCF. Pointer members are caller-provided outputs; NULL skips that output. */
@@ -73,6 +76,7 @@ typedef struct {
     double homo_ev, lumo_ev, homo_beta_ev, lumo_beta_ev;
     double dipole_debye[3];
     double dipole_magnitude;
+    double field_nuclear_ev;  /* -F . sum_a Z_a R_a, part of total_ev (Am1Result::field_nuclear_ev) */
     double residual;          /* ||[F,P]|| of the last iteration */
     int fp32_iterations;      /* iterations run with FP32 GEMMs (mixed precision) */
     double latency_ms;        /* batched: device time from the launch's first molecule start to this

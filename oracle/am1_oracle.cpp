@@ -1584,6 +1584,7 @@ struct OracleRequest {
     int max_scf;
     double e_tol, p_tol, adiis_switch;
     int use_diis;
+    const double* electric_field;  // [3] eV per (e*Bohr), or NULL for no field
 };
 
 struct OracleResult {
@@ -1593,6 +1594,7 @@ struct OracleResult {
     double homo_ev, lumo_ev, homo_beta_ev, lumo_beta_ev;  // NaN when absent
     double dipole_debye[3];
     double residual;
+    double field_nuclear_ev;  // -F . sum_a Z_a R_a (0 without a field)
     // optional outputs (may be NULL)
     double* charges;       // [n_atoms]
     double* mo_energies;   // [nao] (alpha for UHF)
@@ -1722,11 +1724,34 @@ int oracle_adiis_simplex(int m, const double* d, const double* s, double* c) {
     return 0;
 }
 
+// Uniform external field F (eV per e*Bohr), dipole.rs:126-153: H_core += sum_alpha F_alpha M_alpha with
+// the dipole operator M_alpha = R_alpha on every AO's diagonal and dd on the s-p_alpha pair of each
+// heavy atom (dipole.rs:20-60); the nuclear half -F . sum_a Z_a R_a joins the total (scf.rs:375-381).
+static double add_field(const System& s, const double* field, Mat& H) {
+    if (!field) return 0.0;
+    const ParamSet& ps = *s.ps;
+    double nuc = 0.0;
+    for (int i = 0; i < s.nat; ++i) {
+        const Elem& e = ps.el[s.z[i]];
+        const int o = s.off[i];
+        double fr = 0.0;
+        for (int k = 0; k < 3; ++k) fr += field[k] * s.xyz[3 * i + k];
+        for (int k = 0; k < s.norb[i]; ++k) H(o + k, o + k) += fr;
+        if (s.norb[i] == 4)
+            for (int k = 0; k < 3; ++k) {
+                H(o, o + 1 + k) += field[k] * e.dd;
+                H(o + 1 + k, o) += field[k] * e.dd;
+            }
+        nuc += e.core * fr;
+    }
+    return -nuc;
+}
+
 static void fill_common(const System& s, OracleResult* res, const Mat& density, double electronic, double core) {
     const ParamSet& ps = *s.ps;
     res->electronic_ev = electronic;
     res->core_ev = core;
-    res->total_ev = electronic + core;
+    res->total_ev = electronic + core + res->field_nuclear_ev;
     double eiso = 0.0, eheat = 0.0;
     for (int i = 0; i < s.nat; ++i) {
         eiso += ps.el[s.z[i]].e_isol;
@@ -1753,6 +1778,7 @@ static void fill_common(const System& s, OracleResult* res, const Mat& density, 
 // run_am1 (scf.rs:294-468) with rhf_loop (:557-716) and uhf_loop (:719-858).
 int oracle_scf(const OracleRequest* rq, OracleResult* res) {
     res->message[0] = 0;
+    res->field_nuclear_ev = 0.0;
     res->homo_ev = res->lumo_ev = res->homo_beta_ev = res->lumo_beta_ev = NAN;
     res->iterations = 0;
     res->converged = 0;
@@ -1792,6 +1818,7 @@ int oracle_scf(const OracleRequest* rq, OracleResult* res) {
     Mat H;
     std::vector<Pair> pairs;
     build_core(s, H, pairs);
+    res->field_nuclear_ev = add_field(s, rq->electric_field, H);
     double core_ev = core_core_energy(s);
     const int max_diis = 8;
     int accel = rq->use_diis ? rq->accelerator : 0;

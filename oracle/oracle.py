@@ -40,6 +40,7 @@ class _Request(ctypes.Structure):
         ("p_tol", ctypes.c_double),
         ("adiis_switch", ctypes.c_double),
         ("use_diis", ctypes.c_int),
+        ("electric_field", ctypes.POINTER(ctypes.c_double)),
     ]
 
 
@@ -62,6 +63,7 @@ class _Result(ctypes.Structure):
         ("lumo_beta_ev", ctypes.c_double),
         ("dipole_debye", ctypes.c_double * 3),
         ("residual", ctypes.c_double),
+        ("field_nuclear_ev", ctypes.c_double),
         ("charges", ctypes.POINTER(ctypes.c_double)),
         ("mo_energies", ctypes.POINTER(ctypes.c_double)),
         ("density", ctypes.POINTER(ctypes.c_double)),
@@ -141,7 +143,9 @@ def _ip(a):
 
 def _make_request(z, xyz_bohr, charge=0.0, multiplicity=1, method="am1", reference="auto",
                   accelerator="adiis_cdiis", max_scf=200, e_tol=1e-8, p_tol=1e-7, adiis_switch=0.1,
-                  use_diis=True, keep=None):
+                  use_diis=True, electric_field=None, keep=None):
+    """``electric_field``: three floats in eV per (e*Bohr), the unit of Am1Options::electric_field
+    (scf.rs:141-153), or None."""
     z = np.ascontiguousarray(z, dtype=np.int32)
     xyz = np.ascontiguousarray(xyz_bohr, dtype=np.float64).reshape(-1, 3)
     if keep is not None:
@@ -160,6 +164,11 @@ def _make_request(z, xyz_bohr, charge=0.0, multiplicity=1, method="am1", referen
     rq.p_tol = float(p_tol)
     rq.adiis_switch = float(adiis_switch)
     rq.use_diis = 1 if use_diis else 0
+    if electric_field is not None:
+        f = np.ascontiguousarray(electric_field, dtype=np.float64).reshape(3)
+        if keep is not None:
+            keep.append(f)
+        rq.electric_field = _dp(f)
     return rq
 
 
@@ -183,6 +192,7 @@ def _result_dict(rs, extra):
         "lumo_beta_ev": None if math.isnan(rs.lumo_beta_ev) else rs.lumo_beta_ev,
         "dipole_debye": np.array(list(rs.dipole_debye)),
         "residual": rs.residual,
+        "field_nuclear_ev": rs.field_nuclear_ev,
     }
     out.update(extra)
     return out

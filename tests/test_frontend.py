@@ -81,3 +81,39 @@ def test_single_point_positional_reference_and_method():
               method="rm1")
     assert abs(r["energy_ev"] - o["total_ev"]) <= 1e-6
     assert abs(r["homo_beta_ev"] - o["homo_beta_ev"]) <= 1e-6
+
+
+@pytest.mark.gpu
+def test_single_point_electric_field():
+    """A field in atomic units (python.rs:55-70) through single_point against the oracle with the
+    same field in eV per (e*Bohr)."""
+    f_au = np.array([0.002, -0.001, 0.003])
+    r = am1g.single_point(WATER_NUMBERS, WATER_POSITIONS, electric_field=f_au)
+    xyz = np.array(WATER_POSITIONS) * am1g.ANGSTROM_TO_BOHR
+    o = O.scf(WATER_NUMBERS, xyz, full=False, electric_field=f_au * 27.21)
+    assert abs(r["energy_ev"] - o["total_ev"]) <= 1e-6
+    assert abs(r["field_nuclear_ev"] - o["field_nuclear_ev"]) <= 1e-12
+    assert np.abs(np.array(r["charges"]) - o["charges"]).max() <= 1e-6
+    assert abs(r["homo_ev"] - o["homo_ev"]) <= 1e-6
+    z0 = am1g.single_point(WATER_NUMBERS, WATER_POSITIONS)
+    assert abs(z0["energy_ev"] - r["energy_ev"]) > 1e-3  # the field does something
+
+
+@pytest.mark.gpu
+def test_electric_field_large_path_and_batch(engine, monkeypatch):
+    """The field in the batched kernel (beside a field-free copy of the molecule in the same
+    launch) and on the grid-level path, against the oracle."""
+    from tests import molecules as M
+
+    z, x = M.parse(M.FORMALDEHYDE)
+    f = np.array([0.05, 0.02, -0.04])
+    o = O.scf(z, x, full=False, electric_field=f)
+    rb = engine.scf_batch([(z, x, {"electric_field": f}), (z, x)])
+    assert abs(rb[0]["total_ev"] - o["total_ev"]) <= 1e-6
+    assert np.abs(rb[0]["charges"] - o["charges"]).max() <= 1e-6
+    assert rb[0]["field_nuclear_ev"] == pytest.approx(o["field_nuclear_ev"], abs=1e-12)
+    assert rb[1]["field_nuclear_ev"] == 0.0
+    monkeypatch.setenv("AM1G_FORCE_LARGE_PATH", "1")
+    rl = engine.scf(z, x, full=False, electric_field=f)
+    assert abs(rl["total_ev"] - o["total_ev"]) <= 1e-6
+    assert np.abs(rl["charges"] - o["charges"]).max() <= 1e-6

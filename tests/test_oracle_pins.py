@@ -320,3 +320,41 @@ def test_planted_mistake_is_caught():
     e_ok = 0.5 * np.sum(P * (H + F))
     e_wrong = 0.5 * np.sum(P * (H + F_wrong))
     assert abs(e_ok - e_wrong) > 1.0
+
+
+# ------------------------------------------------------------------------------ external field
+def test_zero_field_changes_nothing():
+    # tests/external_field.rs:48-55
+    z, x = M.parse(M.WATER)
+    bare = O.scf(z, x, full=False)
+    zero = O.scf(z, x, full=False, electric_field=[0.0, 0.0, 0.0])
+    assert abs(bare["total_ev"] - zero["total_ev"]) < 1e-12
+    assert zero["field_nuclear_ev"] == 0.0
+
+
+@pytest.mark.parametrize("name", ["water", "formaldehyde"])
+def test_field_derivative_is_minus_the_dipole(name):
+    """tests/external_field.rs:61-90: dE/dF_alpha = -mu_alpha (Hellmann-Feynman; E is variational
+    in P). Central differences of the oracle's energy against its own dipole, in e*Bohr; this
+    fixes the sign and size of both the electronic (H_core) and the nuclear halves."""
+    z, x = M.parse(M.WATER if name == "water" else M.FORMALDEHYDE)
+    tight = dict(full=False, e_tol=1e-12, p_tol=1e-11)
+    r0 = O.scf(z, x, **tight)
+    mu = r0["dipole_debye"] / 2.541746473  # e*Bohr
+    h = 1e-4  # eV per (e*Bohr)
+    for a in range(3):
+        f = np.zeros(3)
+        f[a] = h
+        ep = O.scf(z, x, electric_field=f, **tight)["total_ev"]
+        em = O.scf(z, x, electric_field=-f, **tight)["total_ev"]
+        assert (ep - em) / (2 * h) == pytest.approx(-mu[a], abs=2e-6)
+
+
+def test_field_nuclear_term():
+    # -F . sum_a Z_a R_a (dipole.rs:146-153), with core charges 6 (O) and 1 (H)
+    z, x = M.parse(M.WATER)
+    f = np.array([0.3, -0.2, 0.1])
+    r = O.scf(z, x, full=False, electric_field=f)
+    zc = np.array([6.0 if v == 8 else 1.0 for v in z])
+    assert r["field_nuclear_ev"] == pytest.approx(-f @ (zc[:, None] * x).sum(axis=0), rel=1e-14)
+    assert r["total_ev"] == pytest.approx(r["electronic_ev"] + r["core_ev"] + r["field_nuclear_ev"], abs=1e-9)
