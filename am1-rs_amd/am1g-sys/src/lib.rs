@@ -42,6 +42,7 @@ pub mod raw {
         pub adiis_switch: c_double,
         pub precision: c_int,
         pub electric_field: *const c_double,
+        pub bcc: c_int,
     }
 
     #[repr(C)]
@@ -74,6 +75,7 @@ pub mod raw {
         pub latency_ms: c_double,
         pub phase_ms: [c_double; 6],
         pub eigen_sweeps: c_int,
+        pub bcc_warnings: c_int,
         pub charges: *mut c_double,
         pub mo_energies: *mut c_double,
         pub mo_energies_beta: *mut c_double,
@@ -81,6 +83,7 @@ pub mod raw {
         pub mo_coeff: *mut c_double,
         pub mo_coeff_beta: *mut c_double,
         pub spin_density: *mut c_double,
+        pub bcc_charges: *mut c_double,
     }
 
     #[repr(C)]
@@ -175,6 +178,9 @@ pub mod raw {
         pub fn am1g_last_error(ctx: *const am1g_ctx) -> *const c_char;
         pub fn am1g_element_info(z: c_int, method: c_int, out: *mut c_double) -> c_int;
         pub fn am1g_last_kernel_ms(ctx: *const am1g_ctx) -> c_double;
+        pub fn am1g_bcc_charges(n_atoms: c_int, z: *const u8, xyz_bohr: *const c_double, mulliken: *const c_double,
+                                charges: *mut c_double, atom_types: *mut c_int, n_warnings: *mut c_int,
+                                warnings: *mut c_char, warnings_len: c_int) -> c_int;
     }
 }
 

@@ -55,6 +55,7 @@ class Request(ctypes.Structure):
         ("adiis_switch", ctypes.c_double),
         ("precision", ctypes.c_int),
         ("electric_field", ctypes.POINTER(ctypes.c_double)),
+        ("bcc", ctypes.c_int),
     ]
 
 
@@ -87,6 +88,7 @@ class Result(ctypes.Structure):
         ("latency_ms", ctypes.c_double),
         ("phase_ms", ctypes.c_double * 6),
         ("eigen_sweeps", ctypes.c_int),
+        ("bcc_warnings", ctypes.c_int),
         ("charges", ctypes.POINTER(ctypes.c_double)),
         ("mo_energies", ctypes.POINTER(ctypes.c_double)),
         ("mo_energies_beta", ctypes.POINTER(ctypes.c_double)),
@@ -94,6 +96,7 @@ class Result(ctypes.Structure):
         ("mo_coeff", ctypes.POINTER(ctypes.c_double)),
         ("mo_coeff_beta", ctypes.POINTER(ctypes.c_double)),
         ("spin_density", ctypes.POINTER(ctypes.c_double)),
+        ("bcc_charges", ctypes.POINTER(ctypes.c_double)),
     ]
 
 
@@ -163,7 +166,7 @@ EXPORTS = ["am1g_request_defaults", "am1g_create", "am1g_destroy", "am1g_scf", "
            "am1g_dc_defaults", "am1g_dc", "am1g_serve_defaults", "am1g_serve_create", "am1g_serve_destroy",
            "am1g_serve_submit", "am1g_serve_poll", "am1g_serve_cancel", "am1g_serve_get_stats",
            "am1g_serve_last_error", "am1g_symmetric_eigen", "am1g_last_error", "am1g_element_info", "am1g_last_kernel_ms",
-           "am1g_scf_batch_noalloc", "am1g_noalloc_capacity_bytes"]
+           "am1g_scf_batch_noalloc", "am1g_noalloc_capacity_bytes", "am1g_bcc_charges"]
 
 _lib = None
 
@@ -189,6 +192,10 @@ def load():
     L.am1g_scf_batch.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(Request), ctypes.POINTER(Result)]
     L.am1g_scf_batch_noalloc.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(Request),
                                          ctypes.POINTER(Result)]
+    L.am1g_bcc_charges.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(ctypes.c_double),
+                                   ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                   ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), ctypes.c_char_p,
+                                   ctypes.c_int]
     L.am1g_noalloc_capacity_bytes.argtypes = []
     L.am1g_noalloc_capacity_bytes.restype = ctypes.c_longlong
     L.am1g_dc_defaults.argtypes = [ctypes.POINTER(DcOptions)]
@@ -270,7 +277,7 @@ class Engine:
     # -- request marshalling -------------------------------------------------------------
     def _request(self, z, xyz_bohr, keep, charge=0.0, multiplicity=1, method="am1", reference="auto",
                  accelerator="adiis_cdiis", use_diis=True, max_scf=200, e_tol=1e-8, p_tol=1e-7,
-                 adiis_switch=0.1, precision="fp64", electric_field=None):
+                 adiis_switch=0.1, precision="fp64", electric_field=None, bcc=False):
         """``electric_field``: three floats in eV per (e*Bohr) (Am1Options::electric_field) or None."""
         rq = Request()
         self._L.am1g_request_defaults(ctypes.byref(rq))
@@ -295,6 +302,7 @@ class Engine:
         rq.p_tol = float(p_tol)
         rq.adiis_switch = float(adiis_switch)
         rq.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        rq.bcc = 1 if bcc else 0
         if electric_field is not None:
             fa = np.ascontiguousarray(electric_field, dtype=np.float64).reshape(-1)
             if fa.size != 3:
@@ -306,9 +314,11 @@ class Engine:
     @staticmethod
     def _attach(rs, n_atoms, full, keep):
         q = np.zeros(n_atoms)
-        keep.append(q)
+        qb = np.zeros(n_atoms)
+        keep.extend([q, qb])
         rs.charges = q.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
-        arrays = {"charges": q}
+        rs.bcc_charges = qb.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        arrays = {"charges": q, "bcc": qb}
         if full:
             nao_max = 4 * n_atoms
             mo = np.zeros(nao_max)
@@ -356,6 +366,8 @@ class Engine:
             "phase_ms": dict(zip(PHASES, list(rs.phase_ms))),
             "eigen_sweeps": rs.eigen_sweeps,
             "charges": arrays["charges"].copy(),
+            "bcc_charges": arrays["bcc"].copy() if "bcc" in arrays else None,
+            "bcc_warnings": rs.bcc_warnings,
         }
         if full and rs.status in (0, 4):
             n = rs.nao
@@ -498,17 +510,19 @@ class Server:
         rs = Result()
         n = self._n_atoms[ticket]
         q = np.zeros(n)
+        qb = np.zeros(n)
         mo = np.zeros(4 * n)
         mob = np.zeros(4 * n)
-        keep.extend([q, mo, mob])
+        keep.extend([q, qb, mo, mob])
         rs.charges = q.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        rs.bcc_charges = qb.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
         rs.mo_energies = mo.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
         rs.mo_energies_beta = mob.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
         st = self._L.am1g_serve_poll(self._sv, int(ticket), float(wait_ms), ctypes.byref(rs))
         if st == PENDING:
             return None
         del self._n_atoms[ticket]
-        d = Engine._to_dict(rs, {"charges": q}, False)
+        d = Engine._to_dict(rs, {"charges": q, "bcc": qb}, False)
         d["status"] = st
         d["status_name"] = STATUS_NAMES.get(st, str(st))
         if st in (0, 4):
@@ -602,3 +616,46 @@ def single_point(numbers, positions, charge=0.0, multiplicity=1, reference="auto
         "iterations": r["iterations"],
         "unrestricted": r["unrestricted"],
     }
+
+
+def bcc_correct(numbers, xyz_bohr, mulliken) -> dict:
+    """AM1-BCC corrections (``am1g_bcc_charges``, host code) applied to given AM1 Mulliken charges:
+    ``charges``, ``atom_types`` (antechamber codes as strings, "0" = none) and ``warnings``."""
+    L = load()
+    z = np.ascontiguousarray(_as_numbers(numbers), dtype=np.uint8)
+    x = np.ascontiguousarray(xyz_bohr, dtype=np.float64).reshape(-1, 3)
+    q = np.ascontiguousarray(mulliken, dtype=np.float64).reshape(-1)
+    if not (len(z) == len(x) == len(q)):
+        raise ValueError("numbers, positions and charges differ in length")
+    out = np.zeros(len(z))
+    types = np.zeros(len(z), dtype=np.int32)
+    nw = ctypes.c_int(0)
+    buf = ctypes.create_string_buffer(8192)
+    st = L.am1g_bcc_charges(len(z), z.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                            x.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                            q.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                            out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                            types.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), ctypes.byref(nw), buf, len(buf))
+    if st != 0:
+        raise ValueError(f"am1g_bcc_charges failed ({STATUS_NAMES.get(st, st)})")
+    warnings = [w for w in buf.value.decode(errors="replace").split("\n") if w]
+    return {"charges": out.tolist(), "atom_types": [str(int(t)) for t in types], "warnings": warnings}
+
+
+def am1_bcc(numbers, positions, charge=0.0, multiplicity=1) -> dict:
+    """``am1_rs.native.am1_bcc`` counterpart (python/am1_rs/native.py:834-857, src/python.rs:1785-1815):
+    the AM1 SCF on the GPU for the Mulliken charges, then the AM1-BCC corrections. Positions in
+    Angstrom. Returns ``charges``, ``mulliken``, ``atom_types`` and ``warnings``."""
+    z = _as_numbers(numbers)
+    pos = np.asarray(positions, dtype=np.float64)
+    if pos.ndim != 2 or pos.shape[1] != 3 or pos.shape[0] != len(z):
+        raise ValueError("numbers and positions length mismatch")
+    xyz = pos * ANGSTROM_TO_BOHR
+    r = engine().scf(z, xyz, full=False, charge=float(charge), multiplicity=max(1, int(multiplicity)), method="am1")
+    if r["status"] == 4:
+        raise ValueError(f"SCF did not converge in {r['iterations']} iterations (residual {r['residual']:.3e})")
+    if r["status"] != 0:
+        raise ValueError(r.get("message", r["status_name"]))
+    out = bcc_correct(z, xyz, r["charges"])
+    out["mulliken"] = r["charges"].tolist()
+    return out

@@ -61,6 +61,11 @@ int am1g_prepare(am1g_ctx* ctx, const am1g_request& rq, Prepared& out, std::stri
         msg = "unknown method";
         return AM1G_INVALID_INPUT;
     }
+    if (rq.bcc && rq.method != AM1G_METHOD_AM1) {
+        // the bond charge corrections were fitted to AM1 Mulliken charges (native.py:841-842)
+        msg = "AM1-BCC charges need the AM1 method";
+        return AM1G_INVALID_INPUT;
+    }
     const DevElem* tab = ctx->host_tab + rq.method * kMaxZ;
     int nao = 0, nh = 0, nH = 0;
     double n_elec = 0.0;
@@ -165,6 +170,13 @@ int am1g_prepare(am1g_ctx* ctx, const am1g_request& rq, Prepared& out, std::stri
 namespace {
 
 }  // namespace
+
+void am1g_apply_bcc(const am1g_request& rq, const double* mulliken, am1g_result& r) {
+    if (!rq.bcc || !r.bcc_charges || !mulliken || (r.status != AM1G_OK && r.status != AM1G_SCF_NOT_CONVERGED)) return;
+    int nw = 0;
+    am1g_bcc_charges(rq.n_atoms, rq.z, rq.xyz_bohr, mulliken, r.bcc_charges, nullptr, &nw, nullptr, 0);
+    r.bcc_warnings = nw;
+}
 
 void am1g_fill_result(const MolResult& r, const MolDesc& md, am1g_result& out) {
     out.status = r.status;
@@ -494,6 +506,7 @@ static int batch_impl(am1g_ctx* ctx, int n, const am1g_request* rqs, am1g_result
             for (int k = 0; k < kPhases; ++k) r.phase_ms[k] = (double)hr[j].phase[k] / 1.0e5;
             r.eigen_sweeps = hr[j].eigen_sweeps;
             if (r.charges) std::memcpy(r.charges, ctx->h_charges.as<double>() + md.atom_off, sizeof(double) * md.nat);
+            am1g_apply_bcc(rqs[ok_idx[j]], ctx->h_charges.as<double>() + md.atom_off, r);
             if (r.mo_energies) std::memcpy(r.mo_energies, ctx->h_mo.as<double>() + md.ao_off, sizeof(double) * md.nao);
             if (r.mo_energies_beta && md.uhf)
                 std::memcpy(r.mo_energies_beta, ctx->h_mob.as<double>() + md.ao_off, sizeof(double) * md.nao);
@@ -543,6 +556,7 @@ static int batch_impl(am1g_ctx* ctx, int n, const am1g_request* rqs, am1g_result
         for (int k = 0; k < kPhases; ++k) r.phase_ms[k] = (double)res[j].phase[k] / 1.0e5;
         r.eigen_sweeps = res[j].eigen_sweeps;
         if (r.charges) std::memcpy(r.charges, q.data() + md.atom_off, sizeof(double) * md.nat);
+        am1g_apply_bcc(rqs[i], q.data() + md.atom_off, r);
         if (r.mo_energies) std::memcpy(r.mo_energies, mo.data() + md.ao_off, sizeof(double) * md.nao);
         if (r.mo_energies_beta && md.uhf)
             std::memcpy(r.mo_energies_beta, mob.data() + md.ao_off, sizeof(double) * md.nao);
@@ -596,6 +610,7 @@ int am1g_scf(am1g_ctx* ctx, const am1g_request* rq, am1g_result* res) {
         std::lock_guard<std::mutex> lock(ctx->mu);
         HIP_TRY(ctx, hipSetDevice(ctx->device));
         st = am1g_large_scf(ctx, *rq, prep, res);
+        am1g_apply_bcc(*rq, res->charges, *res);
         res->latency_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
         ctx->last_ms = res->latency_ms;
         return st;

@@ -107,6 +107,9 @@ int am1g_scf_batch_stream(am1g_ctx* ctx, int n, const am1g_request* rqs, am1g_re
                           const std::function<void(int)>& on_done);
 // Scalars of a device-side MolResult into the C-ABI result.
 void am1g_fill_result(const MolResult& r, const MolDesc& md, am1g_result& out);
+// AM1-BCC charges into r.bcc_charges from the given Mulliken charges when the request asks for
+// them and the SCF produced a result.
+void am1g_apply_bcc(const am1g_request& rq, const double* mulliken, am1g_result& r);
 void am1g_large_release(am1g_ctx* ctx);
 // Creates the context's rocBLAS handle and eigensolver workspace on first use.
 int am1g_large_setup(am1g_ctx* ctx);

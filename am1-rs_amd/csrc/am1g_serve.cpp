@@ -38,7 +38,7 @@ struct Job {
     bool cancelled = false;
     Clock::time_point submitted, done;
     am1g_result res;
-    std::vector<double> charges, mo, mob;
+    std::vector<double> charges, mo, mob, bcc;
     bool large = false;  // runs alone on the grid-level path
 };
 
@@ -99,6 +99,7 @@ struct am1g_server {
         res.charges = j.charges.data();
         res.mo_energies = j.mo.data();
         res.mo_energies_beta = j.mob.data();
+        res.bcc_charges = j.bcc.empty() ? nullptr : j.bcc.data();
         lk.unlock();
         const int rc = am1g_scf(ctx, &rq, &res);
         const std::string msg = rc == AM1G_OK ? std::string() : std::string(am1g_last_error(ctx));
@@ -181,6 +182,7 @@ struct am1g_server {
                 res[k].charges = j.charges.data();
                 res[k].mo_energies = j.mo.data();
                 res[k].mo_energies_beta = j.mob.data();
+                res[k].bcc_charges = j.bcc.empty() ? nullptr : j.bcc.data();
             }
             lk.unlock();
             // each molecule is handed out as soon as it finishes on the device
@@ -287,6 +289,7 @@ int am1g_serve_submit(am1g_server* sv, const am1g_request* rq, long long* ticket
     j.large = nao > kRouteLargeAo || rq->n_atoms > kMaxAtoms;
     j.bytes = job_bytes(j.rq, j.large);
     j.charges.assign(rq->n_atoms, 0.0);
+    if (rq->bcc) j.bcc.assign(rq->n_atoms, 0.0);
     j.mo.assign(nao, 0.0);
     j.mob.assign(nao, 0.0);
     j.submitted = Clock::now();
@@ -345,6 +348,7 @@ int am1g_serve_poll(am1g_server* sv, long long ticket, double wait_ms, am1g_resu
         double* coeff = out->mo_coeff;
         double* coeff_b = out->mo_coeff_beta;
         double* spin = out->spin_density;
+        double* qbcc = out->bcc_charges;
         *out = j.res;
         out->charges = q;
         out->mo_energies = mo;
@@ -353,9 +357,11 @@ int am1g_serve_poll(am1g_server* sv, long long ticket, double wait_ms, am1g_resu
         out->mo_coeff = coeff;
         out->mo_coeff_beta = coeff_b;
         out->spin_density = spin;
+        out->bcc_charges = qbcc;
         out->latency_ms = std::chrono::duration<double, std::milli>(j.done - j.submitted).count();
         if (j.cancelled) out->status = AM1G_CANCELLED;
         if (q) std::memcpy(q, j.charges.data(), sizeof(double) * j.charges.size());
+        if (qbcc && !j.bcc.empty()) std::memcpy(qbcc, j.bcc.data(), sizeof(double) * j.bcc.size());
         if (mo && j.res.nao > 0) std::memcpy(mo, j.mo.data(), sizeof(double) * j.res.nao);
         if (mob && j.res.nao > 0 && j.res.unrestricted) std::memcpy(mob, j.mob.data(), sizeof(double) * j.res.nao);
     }

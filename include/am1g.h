@@ -64,6 +64,9 @@ typedef struct {
     const double* electric_field; /* [3] uniform external field in eV per (e*Bohr), or NULL for none
                                      (Am1Options::electric_field, scf.rs:141-153; am1g_scf and
                                      am1g_scf_batch; am1g_dc refuses a field) */
+    int bcc;                 /* nonzero: also AM1-BCC charges from this SCF's Mulliken charges into
+                                am1g_result.bcc_charges (am1_bcc_charges, bcc/mod.rs:99-150); AM1
+                                method only, and the charges output must be given */
 } am1g_request;
 
 /* Result of one SCF. Pointer members are caller-provided outputs; NULL skips that output. */
@@ -85,6 +88,7 @@ typedef struct {
                                  commutator + DIIS, eigensolve, density + convergence, final
                                  energy and properties (am1-rs AM1_TIMING phases, timing.rs:34-85) */
     int eigen_sweeps;         /* Jacobi sweeps summed over all iterations (diagnostic) */
+    int bcc_warnings;         /* AM1-BCC: perception or parameter warnings (am1g_bcc_charges) */
     double* charges;          /* [n_atoms] Mulliken charges */
     double* mo_energies;      /* [nao] (alpha for UHF) */
     double* mo_energies_beta; /* [nao] UHF only */
@@ -92,6 +96,7 @@ typedef struct {
     double* mo_coeff;         /* [nao*nao] orbitals as columns (alpha for UHF) */
     double* mo_coeff_beta;    /* [nao*nao] UHF only: beta orbitals as columns (BetaOrbitals::coeff) */
     double* spin_density;     /* [nao*nao] UHF only: P_alpha - P_beta (Am1Result::spin_density) */
+    double* bcc_charges;      /* [n_atoms] AM1-BCC charges when the request set bcc */
 } am1g_result;
 
 /* Fill a request with am1-rs's Am1Options defaults (scf.rs:174-197). */
@@ -209,6 +214,18 @@ const char* am1g_serve_last_error(const am1g_server* sv);
  * reference. Householder tridiagonalization, tridiagonal divide and conquer and a blocked
  * back-transformation; the same solver serves the SCF of systems above 256 basis functions. */
 int am1g_symmetric_eigen(am1g_ctx* ctx, int n, const double* a, double* eigenvalues, double* eigenvectors);
+
+/* AM1-BCC partial charges (am1_bcc_charges, src/bcc/mod.rs:99-150) from AM1 Mulliken charges:
+ * the molecular graph is perceived from the geometry (src/topology.rs: bonds by covalent radii,
+ * rings, aromaticity, Kekule structure, delocalized groups), every atom gets its antechamber BCC
+ * atom type (ATOMTYPE_BCC.DEF rules, src/bcc/atomtype.rs) and every bond its correction from
+ * BCCPARM.DAT (+delta onto the first atom, -delta onto the second). Host code, no device.
+ * charges[n_atoms] receives the corrected charges; atom_types[n_atoms] (optional) the type codes
+ * (11..91, 0 = none); *n_warnings (optional) the number of warnings, whose texts go,
+ * newline-separated, into warnings[warnings_len] (optional). AM1G_INVALID_INPUT for bad
+ * arguments. */
+int am1g_bcc_charges(int n_atoms, const uint8_t* z, const double* xyz_bohr, const double* mulliken,
+                     double* charges, int* atom_types, int* n_warnings, char* warnings, int warnings_len);
 
 /* Text of the last error raised on the calling thread by any am1g_* call that takes a context
  * (the Display text of the matching Am1Error). Thread-local: concurrent callers sharing a
