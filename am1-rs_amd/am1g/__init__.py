@@ -166,7 +166,7 @@ EXPORTS = ["am1g_request_defaults", "am1g_create", "am1g_destroy", "am1g_scf", "
            "am1g_dc_defaults", "am1g_dc", "am1g_serve_defaults", "am1g_serve_create", "am1g_serve_destroy",
            "am1g_serve_submit", "am1g_serve_poll", "am1g_serve_cancel", "am1g_serve_get_stats",
            "am1g_serve_last_error", "am1g_symmetric_eigen", "am1g_last_error", "am1g_element_info", "am1g_last_kernel_ms",
-           "am1g_scf_batch_noalloc", "am1g_noalloc_capacity_bytes", "am1g_bcc_charges"]
+           "am1g_scf_batch_noalloc", "am1g_noalloc_capacity_bytes", "am1g_bcc_charges", "am1g_gradient"]
 
 _lib = None
 
@@ -196,6 +196,8 @@ def load():
                                    ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                    ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), ctypes.c_char_p,
                                    ctypes.c_int]
+    L.am1g_gradient.argtypes = [ctypes.c_void_p, ctypes.POINTER(Request), ctypes.POINTER(Result),
+                                ctypes.POINTER(ctypes.c_double)]
     L.am1g_noalloc_capacity_bytes.argtypes = []
     L.am1g_noalloc_capacity_bytes.restype = ctypes.c_longlong
     L.am1g_dc_defaults.argtypes = [ctypes.POINTER(DcOptions)]
@@ -387,6 +389,21 @@ class Engine:
         arrays = self._attach(rs, rq.n_atoms, full, keep)
         self._L.am1g_scf(self._ctx, ctypes.byref(rq), ctypes.byref(rs))
         d = self._to_dict(rs, arrays, full)
+        if rs.status not in (0, 4):
+            d["message"] = self.last_error()
+        return d
+
+    def gradient(self, z, xyz_bohr, **kw) -> dict:
+        """SCF plus nuclear gradient (``am1g_gradient``): the ``scf`` dict (without matrices) with
+        ``gradient`` [n][3] in eV/Bohr."""
+        keep = []
+        rq = self._request(z, xyz_bohr, keep, **kw)
+        rs = Result()
+        arrays = self._attach(rs, rq.n_atoms, False, keep)
+        g = np.zeros((rq.n_atoms, 3))
+        self._L.am1g_gradient(self._ctx, ctypes.byref(rq), ctypes.byref(rs), g.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        d = self._to_dict(rs, arrays, False)
+        d["gradient"] = g
         if rs.status not in (0, 4):
             d["message"] = self.last_error()
         return d
@@ -659,3 +676,53 @@ def am1_bcc(numbers, positions, charge=0.0, multiplicity=1) -> dict:
     out = bcc_correct(z, xyz, r["charges"])
     out["mulliken"] = r["charges"].tolist()
     return out
+
+
+def gradient(numbers, positions, charge=0.0, multiplicity=1, reference="auto", method="am1",
+             electric_field=None) -> dict:
+    """``am1_rs.native.gradient`` counterpart (src/python.rs:202-275): the SCF and its nuclear
+    gradient in one call, positions in Angstrom. Keys as the reference: ``energy_hartree``,
+    ``energy_ev``, ``heat_of_formation_kcal``, ``electronic_ev``, ``core_ev``, ``homo_ev``,
+    ``lumo_ev``, ``gradient_hartree_per_bohr``, ``gradient_ev_per_angstrom``, ``charges``,
+    ``dipole_debye``, ``dipole_magnitude``, ``max_gradient_hartree_per_bohr``, ``converged``,
+    ``iterations``, ``unrestricted``, ``method``. Errors raise ``ValueError``."""
+    z = _as_numbers(numbers)
+    pos = np.asarray(positions, dtype=np.float64)
+    if pos.ndim != 2 or pos.shape[1] != 3 or pos.shape[0] != len(z):
+        raise ValueError("numbers and positions length mismatch")
+    field = None
+    if electric_field is not None:
+        f = np.asarray(electric_field, dtype=float).reshape(-1)
+        if f.size != 3:
+            raise ValueError("electric_field must have three components")
+        field = f * AM1_EV
+    if str(method).lower() not in METHODS:
+        raise ValueError(f"unknown method {method!r} (expected 'am1' or 'rm1')")
+    if str(reference).lower() not in REFERENCES:
+        raise ValueError(f"unknown reference {reference!r} (expected 'auto', 'rhf' or 'uhf')")
+    r = engine().gradient(z, pos * ANGSTROM_TO_BOHR, charge=float(charge), multiplicity=max(1, int(multiplicity)),
+                          method=str(method).lower(), reference=str(reference).lower(), electric_field=field)
+    if r["status"] == 4:
+        raise ValueError(f"SCF did not converge in {r['iterations']} iterations (residual {r['residual']:.3e})")
+    if r["status"] != 0:
+        raise ValueError(r.get("message", r["status_name"]))
+    g = r["gradient"]  # eV/Bohr
+    return {
+        "energy_hartree": r["total_ev"] / AM1_EV,
+        "energy_ev": r["total_ev"],
+        "heat_of_formation_kcal": r["heat_of_formation_kcal"],
+        "electronic_ev": r["electronic_ev"],
+        "core_ev": r["core_ev"],
+        "homo_ev": r["homo_ev"],
+        "lumo_ev": r["lumo_ev"],
+        "gradient_hartree_per_bohr": (g / AM1_EV).tolist(),
+        "gradient_ev_per_angstrom": (g * ANGSTROM_TO_BOHR).tolist(),
+        "charges": r["charges"].tolist(),
+        "dipole_debye": r["dipole_debye"].tolist(),
+        "dipole_magnitude": r["dipole_magnitude"],
+        "max_gradient_hartree_per_bohr": float(np.abs(g).max()) / AM1_EV if g.size else 0.0,
+        "converged": r["converged"],
+        "iterations": r["iterations"],
+        "unrestricted": r["unrestricted"],
+        "method": str(method).lower(),
+    }

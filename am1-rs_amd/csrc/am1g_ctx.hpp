@@ -77,6 +77,7 @@ struct am1g_ctx {
     struct DcBuffers* dc = nullptr;
     struct LargeBuffers* large = nullptr;  // one large system: dense matrices, DIIS history
     EigWork* eig = nullptr;                // large-order eigensolver workspace
+    DevBuf grad_buf;                       // nuclear gradient: tables, densities, pair terms
     rocblas_handle blas = nullptr;         // created on first use, bound to `stream`
 };
 

@@ -339,10 +339,11 @@ __device__ void local_heavy_heavy(const DevElem& ea, const DevElem& eb, double r
 #undef SQ2
 }
 
-// Rotated heavy-heavy block written straight to global memory, w[pack_a * 10 + pack_b]
-// (integrals.rs:515-569 in axis/projector form). Fully unrolled: every branch is resolved at
-// compile time, so one pair costs the same straight-line code for every orientation.
-__device__ void heavy_heavy_block(const DevElem& ea, const DevElem& eb, const double d[3], double* __restrict__ w) {
+// Rotated heavy-heavy block (integrals.rs:515-569 in axis/projector form), each entry handed to
+// sink(pack_a * 10 + pack_b, value). Fully unrolled: every branch is resolved at compile time,
+// so one pair costs the same straight-line code for every orientation.
+template <class Sink>
+__device__ __forceinline__ void heavy_heavy_visit(const DevElem& ea, const DevElem& eb, const double d[3], Sink&& sink) {
     double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
     double n[3] = {-d[0] / r, -d[1] / r, -d[2] / r};
     double p[3][3];
@@ -382,8 +383,13 @@ __device__ void heavy_heavy_block(const DevElem& ea, const DevElem& eb, const do
                             ri[19] * (n[k] * (p[l][q] * n[m] + p[l][m] * n[q]) + n[l] * (p[k][q] * n[m] + p[k][m] * n[q])) +
                             ri[20] * p[k][l] * p[m][q] + ri[21] * (p[k][m] * p[l][q] + p[k][q] * p[l][m]);
                     }
-                    w[pack4(kk, ll) * 10 + pack4(mm, nn)] = v;
+                    sink(pack4(kk, ll) * 10 + pack4(mm, nn), v);
                 }
+}
+
+// The rotated heavy-heavy block written straight to global memory, w[pack_a * 10 + pack_b].
+__device__ void heavy_heavy_block(const DevElem& ea, const DevElem& eb, const double d[3], double* __restrict__ w) {
+    heavy_heavy_visit(ea, eb, d, [&](int idx, double v) { w[idx] = v; });
 }
 
 // Core-core repulsion of one atom pair, eV (repulsion.rs:144-188); za/zb are atomic numbers.

@@ -215,6 +215,14 @@ const char* am1g_serve_last_error(const am1g_server* sv);
  * back-transformation; the same solver serves the SCF of systems above 256 basis functions. */
 int am1g_symmetric_eigen(am1g_ctx* ctx, int n, const double* a, double* eigenvalues, double* eigenvectors);
 
+/* Nuclear gradient of the SCF energy: closed_form_gradient (src/gradient.rs:173-254) behind the
+ * Python gradient (src/python.rs:202-275). Runs the am1g_scf SCF (res as there), then on the
+ * device the derivative of every atom pair's energy at the converged densities (resonance,
+ * electron-core, two-electron and core-core terms; central differences of the pair's own
+ * integrals, the scheme of analytic_gradient, gradient.rs:41-90), RHF and UHF, plus -F q_a for
+ * an external field. gradient[n_atoms][3] in eV/Bohr. */
+int am1g_gradient(am1g_ctx* ctx, const am1g_request* rq, am1g_result* res, double* gradient);
+
 /* AM1-BCC partial charges (am1_bcc_charges, src/bcc/mod.rs:99-150) from AM1 Mulliken charges:
  * the molecular graph is perceived from the geometry (src/topology.rs: bonds by covalent radii,
  * rings, aromaticity, Kekule structure, delocalized groups), every atom gets its antechamber BCC

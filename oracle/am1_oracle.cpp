@@ -1602,6 +1602,8 @@ struct OracleResult {
     double* mo_coeff;      // [nao*nao]
     double* mo_energies_beta;  // [nao] UHF only
     char message[256];
+    double* density_alpha;  // [nao*nao] optional: alpha density (RHF: half the total)
+    double* density_beta;   // [nao*nao] optional: beta density
 };
 
 int oracle_set_lapack(const char* path, int threads) {
@@ -1893,6 +1895,9 @@ int oracle_scf(const OracleRequest* rq, OracleResult* res) {
         res->converged = converged;
         res->residual = residual;
         fill_common(s, res, P, electronic, core_ev);
+        for (double* out : {res->density_alpha, res->density_beta})
+            if (out)
+                for (size_t k = 0; k < P.d.size(); ++k) out[k] = 0.5 * P.d[k];
         if (n_alpha >= 1) res->homo_ev = eps[n_alpha - 1];
         if (n_alpha < nao) res->lumo_ev = eps[n_alpha];
         if (res->mo_energies) std::copy(eps.begin(), eps.end(), res->mo_energies);
@@ -1963,6 +1968,8 @@ int oracle_scf(const OracleRequest* rq, OracleResult* res) {
         res->converged = converged;
         res->residual = residual;
         fill_common(s, res, Pt, electronic, core_ev);
+        if (res->density_alpha) std::copy(Pa.d.begin(), Pa.d.end(), res->density_alpha);
+        if (res->density_beta) std::copy(Pb.d.begin(), Pb.d.end(), res->density_beta);
         if (n_alpha >= 1) res->homo_ev = ea[n_alpha - 1];
         if (n_alpha < nao) res->lumo_ev = ea[n_alpha];
         if (n_beta >= 1) res->homo_beta_ev = eb[n_beta - 1];
@@ -1978,6 +1985,66 @@ int oracle_scf(const OracleRequest* rq, OracleResult* res) {
 }
 
 // Many independent SCFs, OpenMP over molecules (the CPU baseline's batched form).
+// Energy at fixed alpha and beta densities for the geometry xyz (eV): the electronic energy
+// 1/2 (Pt.H + Pa.Fa + Pb.Fb) with F_s = H + J(Pt) - K(P_s) (RHF: 1/2 Pt.(H + F) with
+// F = H + J(P) - K(P / 2), the same number), plus core-core and the field's nuclear term
+// (gradient.rs:15-35 energy_at_fixed_density).
+static double energy_fixed_density(const OracleRequest* rq, const double* xyz, const Mat& Pa, const Mat& Pb,
+                                   bool& ok) {
+    System s;
+    ok = build_system(s, rq->n_atoms, rq->z, xyz, rq->method);
+    if (!ok) return 0.0;
+    Mat H;
+    std::vector<Pair> pairs;
+    build_core(s, H, pairs);
+    const double fnuc = add_field(s, rq->electric_field, H);
+    Mat Pt(s.nao), Fa, Fb;
+    for (size_t k = 0; k < Pt.d.size(); ++k) Pt.d[k] = Pa.d[k] + Pb.d[k];
+    build_fock(s, H, pairs, Pt, Pa, 1.0, Fa);
+    build_fock(s, H, pairs, Pt, Pb, 1.0, Fb);
+    return 0.5 * (frob(Pt, H) + frob(Pa, Fa) + frob(Pb, Fb)) + core_core_energy(s) + fnuc;
+}
+
+// Nuclear gradient (eV/Bohr, [n_atoms][3]) of the converged SCF: the energy at the converged
+// densities differentiated along every coordinate, the scheme of analytic_gradient
+// (gradient.rs:41-90) with the core-core term differentiated the same way; the five-point central
+// difference (8 (E(+h) - E(-h)) - (E(+2h) - E(-2h))) / 12h, whose error is O(h^4).
+int oracle_gradient(const OracleRequest* rq, OracleResult* res, double h, double* grad) {
+    const int nat = rq->n_atoms;
+    int nao = 0;
+    {
+        System s;
+        if (!build_system(s, nat, rq->z, rq->xyz_bohr, rq->method)) return oracle_scf(rq, res);
+        nao = s.nao;
+    }
+    Mat Pa(nao), Pb(nao);
+    double* keep_a = res->density_alpha;
+    double* keep_b = res->density_beta;
+    res->density_alpha = Pa.d.data();
+    res->density_beta = Pb.d.data();
+    int st = oracle_scf(rq, res);
+    res->density_alpha = keep_a;
+    res->density_beta = keep_b;
+    if (st != ST_OK) return st;
+    if (keep_a) std::copy(Pa.d.begin(), Pa.d.end(), keep_a);
+    if (keep_b) std::copy(Pb.d.begin(), Pb.d.end(), keep_b);
+    std::vector<double> x(rq->xyz_bohr, rq->xyz_bohr + 3 * (size_t)nat);
+    for (int a = 0; a < nat; ++a)
+        for (int k = 0; k < 3; ++k) {
+            bool ok = true;
+            const double x0 = x[3 * a + k];
+            double e[4];
+            const double off[4] = {h, -h, 2.0 * h, -2.0 * h};
+            for (int q = 0; q < 4; ++q) {
+                x[3 * a + k] = x0 + off[q];
+                e[q] = energy_fixed_density(rq, x.data(), Pa, Pb, ok);
+            }
+            x[3 * a + k] = x0;
+            grad[3 * a + k] = (8.0 * (e[0] - e[1]) - (e[2] - e[3])) / (12.0 * h);
+        }
+    return ST_OK;
+}
+
 int oracle_scf_many(int n, const OracleRequest* rqs, OracleResult* res, int threads) {
 #ifdef _OPENMP
     int prev_nested = omp_get_max_active_levels();

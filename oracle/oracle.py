@@ -70,6 +70,8 @@ class _Result(ctypes.Structure):
         ("mo_coeff", ctypes.POINTER(ctypes.c_double)),
         ("mo_energies_beta", ctypes.POINTER(ctypes.c_double)),
         ("message", ctypes.c_char * 256),
+        ("density_alpha", ctypes.POINTER(ctypes.c_double)),
+        ("density_beta", ctypes.POINTER(ctypes.c_double)),
     ]
 
 
@@ -114,6 +116,7 @@ def lib(use_lapack: bool = True, threads: int = 0):
         L.oracle_eigen.argtypes = [ctypes.c_int, dp, dp, dp]
         L.oracle_diis_solve.argtypes = [ctypes.c_int, dp, dp]
         L.oracle_adiis_simplex.argtypes = [ctypes.c_int, dp, dp, dp]
+        L.oracle_gradient.argtypes = [ctypes.POINTER(_Request), ctypes.POINTER(_Result), ctypes.c_double, dp]
         _lib = L
         if use_lapack:
             path = _find_openblas()
@@ -250,6 +253,23 @@ def scf_many(molecules, threads=0, **kw):
         rss[i].charges = _dp(q)
     L.oracle_scf_many(n, rqs, rss, int(threads))
     return [_result_dict(rss[i], {"charges": charges[i]}) for i in range(n)]
+
+
+def gradient(z, xyz_bohr, step=1.0e-4, **kw):
+    """Nuclear gradient (eV/Bohr, [n][3]) of the converged SCF by central differences of the energy
+    at the converged densities (``oracle_gradient``), with the SCF's result dict."""
+    L = lib()
+    keep = []
+    rq = _make_request(z, xyz_bohr, keep=keep, **kw)
+    n = len(z)
+    rs = _Result()
+    charges = np.zeros(n)
+    rs.charges = _dp(charges)
+    g = np.zeros((n, 3))
+    L.oracle_gradient(ctypes.byref(rq), ctypes.byref(rs), float(step), _dp(g))
+    out = _result_dict(rs, {"charges": charges})
+    out["gradient"] = g
+    return out
 
 
 def element(z, method="am1"):
