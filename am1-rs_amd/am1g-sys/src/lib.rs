@@ -178,6 +178,8 @@ pub mod raw {
         pub fn am1g_last_error(ctx: *const am1g_ctx) -> *const c_char;
         pub fn am1g_element_info(z: c_int, method: c_int, out: *mut c_double) -> c_int;
         pub fn am1g_last_kernel_ms(ctx: *const am1g_ctx) -> c_double;
+        pub fn am1g_set_tunable(ctx: *mut am1g_ctx, name: *const c_char, value: c_double) -> c_int;
+        pub fn am1g_get_tunable(ctx: *const am1g_ctx, name: *const c_char, value: *mut c_double) -> c_int;
         pub fn am1g_gradient(ctx: *mut am1g_ctx, rq: *const am1g_request, res: *mut am1g_result,
                              gradient: *mut c_double) -> c_int;
         pub fn am1g_bcc_charges(n_atoms: c_int, z: *const u8, xyz_bohr: *const c_double, mulliken: *const c_double,

@@ -166,7 +166,8 @@ EXPORTS = ["am1g_request_defaults", "am1g_create", "am1g_destroy", "am1g_scf", "
            "am1g_dc_defaults", "am1g_dc", "am1g_serve_defaults", "am1g_serve_create", "am1g_serve_destroy",
            "am1g_serve_submit", "am1g_serve_poll", "am1g_serve_cancel", "am1g_serve_get_stats",
            "am1g_serve_last_error", "am1g_symmetric_eigen", "am1g_last_error", "am1g_element_info", "am1g_last_kernel_ms",
-           "am1g_scf_batch_noalloc", "am1g_noalloc_capacity_bytes", "am1g_bcc_charges", "am1g_gradient"]
+           "am1g_scf_batch_noalloc", "am1g_noalloc_capacity_bytes", "am1g_bcc_charges", "am1g_gradient",
+           "am1g_set_tunable", "am1g_get_tunable"]
 
 _lib = None
 
@@ -196,6 +197,8 @@ def load():
                                    ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                    ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), ctypes.c_char_p,
                                    ctypes.c_int]
+    L.am1g_set_tunable.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_double]
+    L.am1g_get_tunable.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double)]
     L.am1g_gradient.argtypes = [ctypes.c_void_p, ctypes.POINTER(Request), ctypes.POINTER(Result),
                                 ctypes.POINTER(ctypes.c_double)]
     L.am1g_noalloc_capacity_bytes.argtypes = []
@@ -262,6 +265,22 @@ class Engine:
 
     def last_kernel_ms(self) -> float:
         return float(self._L.am1g_last_kernel_ms(self._ctx))
+
+    TUNABLES = ("route_large_ao", "jacobi_lds_n", "eig_graphs")
+
+    def tunables(self) -> dict:
+        """The context's device-dependent launch choices (``am1g_get_tunable``)."""
+        out = {}
+        for k in self.TUNABLES:
+            v = ctypes.c_double()
+            self._L.am1g_get_tunable(self._ctx, k.encode(), ctypes.byref(v))
+            out[k] = v.value
+        return out
+
+    def set_tunable(self, name: str, value: float) -> None:
+        st = self._L.am1g_set_tunable(self._ctx, name.encode(), float(value))
+        if st != 0:
+            raise ValueError(self.last_error())
 
     def eigh(self, a):
         """``symmetric_eigen`` on the device: (eigenvalues ascending, eigenvectors as columns)."""

@@ -1,0 +1,145 @@
+"""Autotuner for the device-dependent launch choices of libam1g (``am1g_set_tunable``).
+
+Measures on the current GPU, through the C ABI, and writes the table the library reads at
+``am1g_create`` (``tuned.txt`` beside ``libam1g.so``, or the file ``AM1G_TUNE_FILE`` names):
+
+* ``route_large_ao``: the size from which one molecule runs faster on the grid-level kernels
+  (the whole device) than in one workgroup of the batched kernel. Water clusters of 16 to 24
+  molecules (96 to 144 basis functions) are timed both ways; the crossover is the smallest size
+  from which the grid path wins at every larger size measured.
+* ``jacobi_lds_n``: whether the batched Jacobi solver (129 to 256 basis functions) should keep
+  its matrix in LDS up to 192 (the LDS capacity) or work in global memory; timed on a batch of
+  clusters of 140 to 190 basis functions.
+* ``eig_graphs``: whether replaying the large eigensolver's tridiagonalization as a HIP graph
+  pays; timed on a 1200-order eigensolve.
+
+Usage: ``python -m am1g.tune [path]``. Each choice is a size- or machine-keyed constant; the
+measured times are written as comments beside it.
+"""
+from __future__ import annotations
+
+import os
+import sys
+import time
+
+import numpy as np
+
+import am1g
+from am1g import geometry as G
+
+_TRI_MAX = 128  # largest order the batched kernel diagonalizes by the tridiagonal method
+
+
+def _time_scf(eng, mol, reps=3):
+    best = float("inf")
+    for _ in range(reps):
+        r = eng.scf(mol.z, mol.xyz_bohr, full=False)
+        if r["status"] != 0:
+            raise RuntimeError(r.get("message", r["status_name"]))
+        best = min(best, r["latency_ms"])
+    return best
+
+
+def tune_route(eng, log):
+    sizes = list(range(16, 25))
+    rows = []
+    for k in sizes:
+        mol = G.small_water_cluster(k, np.random.default_rng(100 + k))
+        eng.set_tunable("route_large_ao", 100000)  # one workgroup
+        t_wg = _time_scf(eng, mol)
+        eng.set_tunable("route_large_ao", 0)  # grid-level kernels
+        t_grid = _time_scf(eng, mol)
+        rows.append((mol.nao, t_wg, t_grid))
+        log(f"route: {mol.nao} AOs: workgroup {t_wg:.2f} ms, grid {t_grid:.2f} ms")
+    route = rows[-1][0]
+    for i in range(len(rows)):
+        if all(t_grid < t_wg for _, t_wg, t_grid in rows[i:]):
+            # The workgroup path changes solver above _TRI_MAX functions (tridiagonal QL up to
+            # it, Jacobi above), which is where its time jumps: a crossover found between two
+            # measured sizes that bracket that switch is placed at the switch itself.
+            lo = rows[i - 1][0] if i > 0 else rows[i][0] - 1
+            route = _TRI_MAX if lo <= _TRI_MAX < rows[i][0] else lo
+            break
+    eng.set_tunable("route_large_ao", route)
+    return route, rows
+
+
+def tune_jacobi_lds(eng, log):
+    mols = [G.small_water_cluster(k, np.random.default_rng(200 + k)) for k in (24, 26, 28, 30, 32) for _ in range(8)]
+    batch = [(m.z, m.xyz_bohr) for m in mols]
+    times = {}
+    for lds_n in (0, 192):
+        eng.set_tunable("jacobi_lds_n", lds_n)
+        eng.scf_batch(batch)
+        ts = []
+        for _ in range(2):
+            eng.scf_batch(batch)
+            ts.append(eng.last_kernel_ms())
+        times[lds_n] = min(ts)
+        log(f"jacobi_lds_n {lds_n}: batch of {len(batch)} clusters {times[lds_n]:.1f} ms")
+    best = min(times, key=times.get)
+    eng.set_tunable("jacobi_lds_n", best)
+    return best, times
+
+
+def tune_graphs(eng, log):
+    n = 1200
+    rng = np.random.default_rng(5)
+    a = rng.standard_normal((n, n))
+    a = a + a.T
+    times = {}
+    for g in (0, 1):
+        eng.set_tunable("eig_graphs", g)
+        eng.eigh(a)
+        ts = []
+        for _ in range(3):
+            t = time.perf_counter()
+            eng.eigh(a)
+            ts.append((time.perf_counter() - t) * 1e3)
+        times[g] = min(ts)
+        log(f"eig_graphs {g}: {n}-order eigensolve {times[g]:.1f} ms")
+    best = 1 if times[1] < 0.98 * times[0] else 0
+    eng.set_tunable("eig_graphs", best)
+    return best, times
+
+
+def _device_name(device: int) -> str:
+    try:
+        import torch
+
+        props = torch.cuda.get_device_properties(device)
+        return f"{props.name} ({props.gcnArchName})"
+    except Exception as exc:  # noqa: BLE001 - the name is informational only
+        return f"device {device} (name unavailable: {type(exc).__name__})"
+
+
+def autotune(path: str | None = None, device: int = 0, log=print) -> dict:
+    """Measure every tunable on `device` and write the table to `path` (default: tuned.txt
+    beside the library). Returns the chosen values."""
+    path = path or os.path.join(os.path.dirname(am1g.LIB_PATH), "tuned.txt")
+    os.environ["AM1G_TUNE_FILE"] = ""  # measure from the defaults, not from an earlier table
+    eng = am1g.Engine(device)
+    try:
+        route, rows = tune_route(eng, log)
+        lds_n, jt = tune_jacobi_lds(eng, log)
+        graphs, gt = tune_graphs(eng, log)
+    finally:
+        eng.close()
+    name = _device_name(device)
+    lines = [f"# libam1g tunables measured by am1g.tune on {name}, {time.strftime('%Y-%m-%d')}",
+             "# route_large_ao: one molecule, best of 3 (ms): AOs workgroup grid"]
+    lines += [f"#   {nao} {a:.2f} {b:.2f}" for nao, a, b in rows]
+    lines.append(f"route_large_ao {route}")
+    lines.append("# jacobi_lds_n: batched Jacobi on 40 clusters of 144-192 AOs (ms): " +
+                 ", ".join(f"{k}: {v:.1f}" for k, v in jt.items()))
+    lines.append(f"jacobi_lds_n {lds_n}")
+    lines.append("# eig_graphs: 1200-order eigensolve (ms): " + ", ".join(f"{k}: {v:.1f}" for k, v in gt.items()))
+    lines.append(f"eig_graphs {graphs}")
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    return {"route_large_ao": route, "jacobi_lds_n": lds_n, "eig_graphs": graphs, "path": path}
+
+
+if __name__ == "__main__":
+    out = autotune(sys.argv[1] if len(sys.argv) > 1 else None)
+    print(out)

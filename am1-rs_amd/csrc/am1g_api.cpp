@@ -12,6 +12,9 @@
 #include <cstdlib>
 #include <climits>
 #include <cstring>
+#include <dlfcn.h>
+#include <fstream>
+#include <sstream>
 #include <mutex>
 #include <numeric>
 #include <string>
@@ -38,6 +41,52 @@ int am1g_set_error(am1g_ctx* ctx, int code, const char* fmt, ...) {
     va_end(ap);
     g_tls_error = buf;
     return code;
+}
+
+namespace {
+bool set_tunable(Tunables& t, const std::string& name, double v) {
+    if (name == "route_large_ao") t.route_large_ao = (int)v;
+    else if (name == "jacobi_lds_n") t.jacobi_lds_n = std::min(192, std::max(0, (int)v));
+    else if (name == "eig_graphs") t.eig_graphs = v != 0.0;
+    else return false;
+    return true;
+}
+
+bool get_tunable(const Tunables& t, const std::string& name, double& v) {
+    if (name == "route_large_ao") v = t.route_large_ao;
+    else if (name == "jacobi_lds_n") v = t.jacobi_lds_n;
+    else if (name == "eig_graphs") v = t.eig_graphs;
+    else return false;
+    return true;
+}
+}  // namespace
+
+void am1g_load_tunables(am1g_ctx* ctx) {
+    std::string path;
+    if (const char* env = std::getenv("AM1G_TUNE_FILE")) {
+        path = env;
+    } else {
+        Dl_info info;
+        if (dladdr(reinterpret_cast<void*>(&am1g_load_tunables), &info) && info.dli_fname) {
+            path = info.dli_fname;
+            const size_t slash = path.find_last_of('/');
+            path = (slash == std::string::npos ? std::string() : path.substr(0, slash + 1)) + "tuned.txt";
+        }
+    }
+    if (path.empty()) return;
+    std::ifstream in(path);
+    if (!in) return;
+    std::string line;
+    bool any = false;
+    while (std::getline(in, line)) {
+        const size_t hash = line.find('#');
+        if (hash != std::string::npos) line.resize(hash);
+        std::istringstream ls(line);
+        std::string name;
+        double v;
+        if (ls >> name >> v) any = set_tunable(ctx->tun, name, v) || any;
+    }
+    if (any) ctx->tune_source = path;
 }
 
 // run_am1's input checks (scf.rs:300-366) plus the element checks of Basis::build.
@@ -74,11 +123,6 @@ int am1g_prepare(am1g_ctx* ctx, const am1g_request& rq, Prepared& out, std::stri
         if (zz <= 0 || zz >= kMaxZ || !tab[zz].valid) {
             std::snprintf(buf, sizeof(buf), "%s has no parameters for element Z=%d",
                           rq.method == 0 ? "AM1" : "RM1", zz);
-            msg = buf;
-            return AM1G_ELEMENT_NOT_PARAMETERIZED;
-        }
-        if (tab[zz].n > 3) {
-            std::snprintf(buf, sizeof(buf), "element Z=%d needs the numeric n>=4 Slater overlap, not provided", zz);
             msg = buf;
             return AM1G_ELEMENT_NOT_PARAMETERIZED;
         }
@@ -242,6 +286,7 @@ int am1g_create(int device, am1g_ctx** out) {
     HIP_TRY(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     // device address of this GPU's scratch array for am1g_scf_batch_noalloc (once per device)
     HIP_TRY(ctx, scratch_init(device));
+    am1g_load_tunables(ctx);
     HIP_TRY(ctx, hipEventCreate(&ctx->ev0));
     HIP_TRY(ctx, hipEventCreate(&ctx->ev1));
     build_element_tables(ctx->host_tab);
@@ -285,6 +330,21 @@ int am1g_element_info(int z, int method, double* out) {
 
 int am1g_scf_batch(am1g_ctx* ctx, int n, const am1g_request* rqs, am1g_result* results) {
     return am1g_scf_batch_stream(ctx, n, rqs, results, std::function<void(int)>());
+}
+
+int am1g_set_tunable(am1g_ctx* ctx, const char* name, double value) {
+    if (!ctx || !name) return am1g_set_error(ctx, AM1G_INVALID_INPUT, "null argument");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (!set_tunable(ctx->tun, name, value)) return am1g_set_error(ctx, AM1G_INVALID_INPUT, "unknown tunable %s", name);
+    if (ctx->eig) ctx->eig->use_graphs = ctx->tun.eig_graphs != 0;
+    return AM1G_OK;
+}
+
+int am1g_get_tunable(const am1g_ctx* ctx, const char* name, double* value) {
+    if (!ctx || !name || !value) return am1g_set_error(nullptr, AM1G_INVALID_INPUT, "null argument");
+    if (!get_tunable(ctx->tun, name, *value))
+        return am1g_set_error(nullptr, AM1G_INVALID_INPUT, "unknown tunable %s", name);
+    return AM1G_OK;
 }
 
 int am1g_scf_batch_noalloc(am1g_ctx* ctx, int n, const am1g_request* rqs, am1g_result* results) {
@@ -451,7 +511,7 @@ static int batch_impl(am1g_ctx* ctx, int n, const am1g_request* rqs, am1g_result
     std::memset(&a, 0, sizeof(a));
     a.mols = static_cast<MolDesc*>(pools[pMols].p);
     a.n_mols = m;
-    a.lds_n = 192;
+    a.lds_n = ctx->tun.jacobi_lds_n;
     a.tri_max = eigen_tri_max();
     a.z = static_cast<int*>(pools[pZ].p);
     a.xyz = static_cast<double*>(pools[pXyz].p);
@@ -606,7 +666,7 @@ int am1g_scf(am1g_ctx* ctx, const am1g_request* rq, am1g_result* res) {
         return am1g_set_error(ctx, st, "%s", msg.c_str());
     }
     const char* force = std::getenv("AM1G_FORCE_LARGE_PATH");
-    if (prep.md.nao > kRouteLargeAo || rq->n_atoms > kMaxAtoms || (force && force[0] == '1')) {
+    if (prep.md.nao > ctx->tun.route_large_ao || rq->n_atoms > kMaxAtoms || (force && force[0] == '1')) {
         std::lock_guard<std::mutex> lock(ctx->mu);
         HIP_TRY(ctx, hipSetDevice(ctx->device));
         st = am1g_large_scf(ctx, *rq, prep, res);

@@ -63,8 +63,18 @@ struct DevBuf {
     }
 };
 
+// Launch choices that depend on the device rather than on the inputs; defaults here, replaced at
+// context creation by the table the autotuner (am1g.tune) measured, when one is found.
+struct Tunables {
+    int route_large_ao = kRouteLargeAo;  // single molecules above this many AOs take the grid path
+    int jacobi_lds_n = 192;              // batched Jacobi (129-256 AOs) keeps orders up to this in LDS
+    int eig_graphs = 1;                  // replay the large eigensolver's tridiagonalization as a graph
+};
+
 struct am1g_ctx {
     int device = 0;
+    Tunables tun;
+    std::string tune_source;  // the table the tunables came from (empty: defaults)
     int cus = 256;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -99,6 +109,9 @@ struct Prepared {
 
 
 int am1g_set_error(am1g_ctx* ctx, int code, const char* fmt, ...);
+// Reads the tunables table: the file AM1G_TUNE_FILE names (empty: none), else tuned.txt beside the
+// library; lines "name value", '#' starts a comment. Unknown names are ignored.
+void am1g_load_tunables(am1g_ctx* ctx);
 int am1g_prepare(am1g_ctx* ctx, const am1g_request& rq, Prepared& out, std::string& msg, int max_ao);
 void am1g_dc_release(am1g_ctx* ctx);
 // Batched launch that hands each molecule's result to on_done(request index) as soon as the

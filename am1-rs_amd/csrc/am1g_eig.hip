@@ -506,10 +506,9 @@ hipError_t eig_large(EigWork& w, rocblas_handle blas, int n, double* A, double* 
     double* p = w.pbuf.as<double>();
     double* Q = w.Q.as<double>();
 
-    // ---- 1. tridiagonalization
-    if (n == 1) {
-        EIG_TRY(hipMemcpyAsync(d, A, sizeof(double), hipMemcpyDeviceToDevice, st));
-    } else {
+    // ---- 1. tridiagonalization: 2 n dependent kernels; captured once per order and buffer set
+    // as a hipGraph and replayed (kernels only, so the graph holds no memset or copy nodes)
+    auto tridiag_launches = [&]() {
         hipLaunchKernelGGL(k_reflect0, dim3(1), dim3(256), 0, st, n, A, d, e, tau, v);
         for (int c = 0; c + 2 < n; ++c) {
             hipLaunchKernelGGL(k_tridiag_rows, dim3(blocks(n - c - 1, 4)), dim3(256), 0, st, n, c, A, tau, v, p);
@@ -518,6 +517,38 @@ hipError_t eig_large(EigWork& w, rocblas_handle blas, int n, double* A, double* 
             else
                 hipLaunchKernelGGL(k_tridiag_next, dim3(1), dim3(256), 0, st, n, c, A, d, e, tau, v, p);
         }
+    };
+    if (n == 1) {
+        EIG_TRY(hipMemcpyAsync(d, A, sizeof(double), hipMemcpyDeviceToDevice, st));
+    } else if (!w.use_graphs || n < 64) {
+        tridiag_launches();
+    } else {
+        const void* key[6] = {A, d, e, tau, v, p};
+        TridiagGraph* hit = nullptr;
+        for (auto& g : w.graphs)
+            if (g.n == n && std::equal(key, key + 6, g.key)) hit = &g;
+        if (!hit) {
+            hipGraph_t graph = nullptr;
+            EIG_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            tridiag_launches();
+            const hipError_t ec = hipStreamEndCapture(st, &graph);
+            if (ec != hipSuccess) return ec;
+            hipGraphExec_t exec = nullptr;
+            const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (ei != hipSuccess) return ei;
+            if (w.graphs.size() >= EigWork::kMaxGraphs) {
+                (void)hipGraphExecDestroy(w.graphs.front().exec);
+                w.graphs.erase(w.graphs.begin());
+            }
+            TridiagGraph g;
+            g.n = n;
+            std::copy(key, key + 6, g.key);
+            g.exec = exec;
+            w.graphs.push_back(g);
+            hit = &w.graphs.back();
+        }
+        EIG_TRY(hipGraphLaunch(hit->exec, st));
     }
     EIG_TRY(hipGetLastError());
     w.h_d.resize(n);

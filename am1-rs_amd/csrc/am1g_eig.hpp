@@ -29,6 +29,14 @@ struct EigBuf {
     }
 };
 
+// A captured launch sequence of the tridiagonalization (2 n kernels) for one order and one set of
+// buffers, replayed as one graph launch.
+struct TridiagGraph {
+    int n = 0;
+    const void* key[6] = {};  // A, d, e, tau, v, p
+    hipGraphExec_t exec = nullptr;
+};
+
 // Device and host workspace, grown on demand and kept between calls.
 struct EigWork {
     EigBuf d, e, tau, vbuf, pbuf, scal;       // tridiagonal and Householder vectors
@@ -37,6 +45,13 @@ struct EigWork {
     EigBuf V, Tm, X, Y, G;                    // back-transformation blocks
     EigBuf rot, idx, flt, lam;                // merge tables
     std::vector<double> h_d, h_e, h_lam;
+    bool use_graphs = true;                   // replay the tridiagonalization as a hipGraph
+    std::vector<TridiagGraph> graphs;         // at most kMaxGraphs, oldest dropped first
+    static constexpr size_t kMaxGraphs = 4;
+    ~EigWork() {
+        for (auto& g : graphs)
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    }
 };
 
 namespace am1g {

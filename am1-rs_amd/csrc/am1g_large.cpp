@@ -35,6 +35,7 @@ int am1g_large_setup(am1g_ctx* ctx) {
             return am1g_set_error(ctx, AM1G_DEVICE_ERROR, "rocBLAS atomics mode failed");
     }
     if (!ctx->eig) ctx->eig = new EigWork();
+    ctx->eig->use_graphs = ctx->tun.eig_graphs != 0;
     if (!ctx->large) ctx->large = new LargeBuffers();
     return AM1G_OK;
 }

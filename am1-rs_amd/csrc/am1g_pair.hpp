@@ -59,9 +59,66 @@ __device__ __forceinline__ void aux_b(double x, double* b) {
     }
 }
 
-// Local overlaps (s111, s211, s121, s221, s222) for the ordered pair with ea.n >= eb.n.
-// Returns false for principal quantum numbers above 3.
+#include "../data/gauss_legendre.inc"
+
+// Overlap of two normalized Slater functions (n_a, l_a, m) and (n_b, l_b, m), l <= 1 and m = 0
+// (sigma) or 1 (pi), at distance r, by quadrature in prolate spheroidal coordinates (the general
+// path of overlap_numeric.rs:25-105 for principal quantum numbers n >= 4): 48 Gauss-Legendre
+// points in xi on [1, 1 + 60 / max(p, 0.05)], p = (za + zb) r / 2, and 40 in eta on [-1, 1];
+// r_a = r (xi + eta) / 2, r_b = r (xi - eta) / 2; Jacobian (r / 2)^3 (xi^2 - eta^2).
+__device__ double sto_overlap_numeric(int na, double za, int la, int m, int nb, double zb, int lb, double r) {
+    double c;
+    if (la == 0 && lb == 0) c = 0.5;
+    else if (la != lb && m == 0) c = 0.86602540378443865;  // sqrt(3) / 2
+    else if (la == 1 && lb == 1) c = m == 0 ? 1.5 : 0.75;
+    else return 0.0;
+    auto norm = [](int n, double z) {
+        double f = 1.0;
+        for (int k = 2; k <= 2 * n; ++k) f *= k;
+        return pow(2.0 * z, n + 0.5) / sqrt(f);
+    };
+    const double h = 0.5 * r;
+    const double p = (za + zb) * h;
+    const double xi_max = 1.0 + 60.0 / fmax(p, 0.05);
+    const double mid = 0.5 * (1.0 + xi_max), half = 0.5 * (xi_max - 1.0);
+    double sum = 0.0;
+    for (int i = 0; i < 48; ++i) {
+        const double xi = mid + half * kGlXiNode[i];
+        const double wx = half * kGlXiWeight[i];
+        for (int j = 0; j < 40; ++j) {
+            const double eta = kGlEtaNode[j];
+            const double ra = h * (xi + eta), rb = h * (xi - eta);
+            if (ra <= 0.0 || rb <= 0.0) continue;
+            double ang = 1.0;
+            if (la == 1) {
+                const double ca = (1.0 + xi * eta) / (xi + eta);
+                ang *= m == 0 ? ca : sqrt(fmax(0.0, 1.0 - ca * ca));
+            }
+            if (lb == 1) {
+                const double cb = (xi * eta - 1.0) / (xi - eta);
+                ang *= m == 0 ? cb : sqrt(fmax(0.0, 1.0 - cb * cb));
+            }
+            double radial = exp(-za * ra - zb * rb);
+            for (int k = 1; k < na; ++k) radial *= ra;
+            for (int k = 1; k < nb; ++k) radial *= rb;
+            sum += radial * (wx * kGlEtaWeight[j] * ang * (xi * xi - eta * eta));
+        }
+    }
+    return h * h * h * sum * norm(na, za) * norm(nb, zb) * c;
+}
+
+// Local overlaps (s111, s211, s121, s221, s222) for the ordered pair with ea.n >= eb.n: the
+// closed forms for n <= 3 (overlap.rs:120-296), the quadrature above for n >= 4
+// (overlap_numeric.rs:8-24: ss, p_sigma|s, -(s|p_sigma), -(p_sigma|p_sigma), p_pi|p_pi).
 __device__ bool overlap_locals(const DevElem& ea, const DevElem& eb, double r, double s[5]) {
+    if (ea.n > 3 || eb.n > 3) {
+        s[0] = sto_overlap_numeric(ea.n, ea.zs, 0, 0, eb.n, eb.zs, 0, r);
+        s[1] = sto_overlap_numeric(ea.n, ea.zp, 1, 0, eb.n, eb.zs, 0, r);
+        s[2] = -sto_overlap_numeric(ea.n, ea.zs, 0, 0, eb.n, eb.zp, 1, r);
+        s[3] = -sto_overlap_numeric(ea.n, ea.zp, 1, 0, eb.n, eb.zp, 1, r);
+        s[4] = sto_overlap_numeric(ea.n, ea.zp, 1, 1, eb.n, eb.zp, 1, r);
+        return true;
+    }
     const double sq3 = 1.7320508075688772;
     const double za0 = ea.zs, za1 = ea.zp, zb0 = eb.zs, zb1 = eb.zp;
     s[0] = s[1] = s[2] = s[3] = s[4] = 0.0;

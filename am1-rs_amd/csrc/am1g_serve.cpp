@@ -286,7 +286,7 @@ int am1g_serve_submit(am1g_server* sv, const am1g_request* rq, long long* ticket
         for (int k = 0; k < 3; ++k) j.field[k] = rq->electric_field[k];
     int nao = 0;
     for (uint8_t zz : j.z) nao += zz == 1 ? 1 : 4;
-    j.large = nao > kRouteLargeAo || rq->n_atoms > kMaxAtoms;
+    j.large = nao > sv->ctxs[0]->tun.route_large_ao || rq->n_atoms > kMaxAtoms;
     j.bytes = job_bytes(j.rq, j.large);
     j.charges.assign(rq->n_atoms, 0.0);
     if (rq->bcc) j.bcc.assign(rq->n_atoms, 0.0);

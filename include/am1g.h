@@ -215,6 +215,17 @@ const char* am1g_serve_last_error(const am1g_server* sv);
  * back-transformation; the same solver serves the SCF of systems above 256 basis functions. */
 int am1g_symmetric_eigen(am1g_ctx* ctx, int n, const double* a, double* eigenvalues, double* eigenvectors);
 
+/* Device-dependent launch choices of a context, measured by the project's autotuner (am1g.tune)
+ * and read at am1g_create from tuned.txt beside the library (or the file AM1G_TUNE_FILE names):
+ *   route_large_ao  single molecules above this many basis functions run on the grid-level path
+ *                   (am1g_scf, am1g_gradient, the server); default 128
+ *   jacobi_lds_n    the batched Jacobi solver (129-256 basis functions) works in LDS up to this
+ *                   order, in global memory above it; default 192 (at most 192)
+ *   eig_graphs      1: the large eigensolver's tridiagonalization replays as a HIP graph; default 1
+ * No reference counterpart. AM1G_INVALID_INPUT for an unknown name. */
+int am1g_set_tunable(am1g_ctx* ctx, const char* name, double value);
+int am1g_get_tunable(const am1g_ctx* ctx, const char* name, double* value);
+
 /* Nuclear gradient of the SCF energy: closed_form_gradient (src/gradient.rs:173-254) behind the
  * Python gradient (src/python.rs:202-275). Runs the am1g_scf SCF (res as there), then on the
  * device the derivative of every atom pair's energy at the converged densities (resonance,

@@ -24,6 +24,7 @@
 #include <cstdio>
 #include <cstring>
 #include <dlfcn.h>
+#include <mutex>
 #include <string>
 #include <vector>
 #ifdef _OPENMP
@@ -267,8 +268,104 @@ void bintgs(double x, double b[13]) {  // overlap.rs:455-488
     }
 }
 
-// overlap.rs:120-296 (n <= 3 closed forms). Returns false for n >= 4.
+// Gauss-Legendre nodes and weights of order n on [-1, 1] (Newton on P_n from the Chebyshev-like
+// first guess), for the numeric Slater overlap below.
+static void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w) {
+    x.assign(n, 0.0);
+    w.assign(n, 0.0);
+    for (int i = 0; i < (n + 1) / 2; ++i) {
+        double t = std::cos(M_PI * (i + 0.75) / (n + 0.5));
+        double dp = 0.0;
+        for (int it = 0; it < 100; ++it) {
+            double p0 = 1.0, p1 = t;
+            for (int k = 2; k <= n; ++k) {
+                const double p2 = ((2.0 * k - 1.0) * t * p1 - (k - 1.0) * p0) / k;
+                p0 = p1;
+                p1 = p2;
+            }
+            dp = n * (t * p1 - p0) / (t * t - 1.0);
+            const double dx = -p1 / dp;
+            t += dx;
+            if (std::fabs(dx) < 1.0e-15) break;
+        }
+        {
+            double p0 = 1.0, p1 = t;
+            for (int k = 2; k <= n; ++k) {
+                const double p2 = ((2.0 * k - 1.0) * t * p1 - (k - 1.0) * p0) / k;
+                p0 = p1;
+                p1 = p2;
+            }
+            dp = n * (t * p1 - p0) / (t * t - 1.0);
+        }
+        const double wt = 2.0 / ((1.0 - t * t) * dp * dp);
+        x[i] = -t;
+        x[n - 1 - i] = t;
+        w[i] = wt;
+        w[n - 1 - i] = wt;
+    }
+}
+
+// Overlap of two normalized Slater functions n_a l_a m / n_b l_b m (l <= 1; m = 0 sigma, 1 pi)
+// at distance r, by quadrature in prolate spheroidal coordinates: xi on [1, 1 + 60 / max(p, 0.05)]
+// with p = (za + zb) r / 2 (48-point Gauss-Legendre), eta on [-1, 1] (40 points), with
+// r_a = r (xi + eta) / 2, r_b = r (xi - eta) / 2, the angular factors cos(theta) / sin(theta)
+// of each centre and the Jacobian (r / 2)^3 (xi^2 - eta^2) (overlap_numeric.rs:25-105).
+static double overlap_sto_numeric(int na, double za, int la, int m, int nb, double zb, int lb, double r) {
+    static std::vector<double> xt, xw, et, ew;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        gauss_legendre(48, xt, xw);
+        gauss_legendre(40, et, ew);
+    });
+    double c;
+    if (la == 0 && lb == 0) c = 0.5;
+    else if ((la == 1) != (lb == 1) && m == 0) c = std::sqrt(3.0) / 2.0;
+    else if (la == 1 && lb == 1) c = m == 0 ? 1.5 : 0.75;
+    else return 0.0;
+    auto fact = [](int n) {
+        double f = 1.0;
+        for (int k = 2; k <= n; ++k) f *= k;
+        return f;
+    };
+    const double norm_a = std::pow(2.0 * za, na + 0.5) / std::sqrt(fact(2 * na));
+    const double norm_b = std::pow(2.0 * zb, nb + 0.5) / std::sqrt(fact(2 * nb));
+    const double h = 0.5 * r;
+    const double p = (za + zb) * h;
+    const double xi_max = 1.0 + 60.0 / std::max(p, 0.05);
+    const double mid = 0.5 * (1.0 + xi_max), half = 0.5 * (xi_max - 1.0);
+    double sum = 0.0;
+    for (size_t i = 0; i < xt.size(); ++i) {
+        const double xi = mid + half * xt[i];
+        for (size_t j = 0; j < et.size(); ++j) {
+            const double eta = et[j];
+            const double ra = h * (xi + eta), rb = h * (xi - eta);
+            if (ra <= 0.0 || rb <= 0.0) continue;
+            const double ca = (1.0 + xi * eta) / (xi + eta), cb = (xi * eta - 1.0) / (xi - eta);
+            const double ang_a = la == 0 ? 1.0 : (m == 0 ? ca : std::sqrt(std::max(0.0, 1.0 - ca * ca)));
+            const double ang_b = lb == 0 ? 1.0 : (m == 0 ? cb : std::sqrt(std::max(0.0, 1.0 - cb * cb)));
+            const double radial = std::pow(ra, na - 1) * std::pow(rb, nb - 1) * std::exp(-za * ra - zb * rb);
+            sum += radial * (half * xw[i] * ew[j] * ang_a * ang_b * (xi * xi - eta * eta));
+        }
+    }
+    return h * h * h * sum * norm_a * norm_b * c;
+}
+
+// The five local overlaps (s111, s211, s121, s221, s222) by quadrature, ea the higher-n atom:
+// ss, p_sigma(a)|s(b), -(s(a)|p_sigma(b)), -(p_sigma|p_sigma), p_pi|p_pi (overlap_numeric.rs:8-24).
+void overlap_locals_numeric(const Elem& ea, const Elem& eb, double r, double s[5]) {
+    s[0] = overlap_sto_numeric(ea.n, ea.zs, 0, 0, eb.n, eb.zs, 0, r);
+    s[1] = overlap_sto_numeric(ea.n, ea.zp, 1, 0, eb.n, eb.zs, 0, r);
+    s[2] = -overlap_sto_numeric(ea.n, ea.zs, 0, 0, eb.n, eb.zp, 1, r);
+    s[3] = -overlap_sto_numeric(ea.n, ea.zp, 1, 0, eb.n, eb.zp, 1, r);
+    s[4] = overlap_sto_numeric(ea.n, ea.zp, 1, 1, eb.n, eb.zp, 1, r);
+}
+
+// overlap.rs:120-296: the n <= 3 closed forms, the quadrature for n >= 4.
 bool overlap_locals(const Elem& ea, const Elem& eb, double r, double s[5]) {
+    if (ea.n > 3 || eb.n > 3) {
+        overlap_locals_numeric(ea, eb, r, s);
+        return true;
+    }
     int jcall;
     if (ea.n == 1 && eb.n == 1) jcall = 2;
     else if (ea.n == 2 && eb.n == 1) jcall = 3;
@@ -761,11 +858,6 @@ bool build_system(System& s, int nat, const int* z, const double* xyz, int metho
         if (z[i] <= 0 || z[i] > 86 || !s.ps->el[z[i]].valid) {
             s.status = ST_ELEMENT;
             s.err = "element not parameterized: Z=" + std::to_string(z[i]);
-            return false;
-        }
-        if (s.ps->el[z[i]].n > 3) {
-            s.status = ST_ELEMENT;
-            s.err = "element Z=" + std::to_string(z[i]) + " needs the numeric (n>=4) STO overlap";
             return false;
         }
         s.off[i] = s.nao;
@@ -1639,6 +1731,16 @@ int oracle_element(int z, int method, double* out) {
 }
 
 void oracle_bintgs(double x, double* b) { bintgs(x, b); }
+// Local overlaps (s111, s211, s121, s221, s222) of the ordered pair (za, zb), za the higher n,
+// by the n >= 4 quadrature whatever the n (to compare it with the closed forms).
+int oracle_overlap_locals(int za, int zb, double r, int method, int numeric, double* out) {
+    const ParamSet& ps = params_for(method);
+    if (numeric) {
+        overlap_locals_numeric(ps.el[za], ps.el[zb], r, out);
+        return 0;
+    }
+    return overlap_locals(ps.el[za], ps.el[zb], r, out) ? 0 : ST_ELEMENT;
+}
 void oracle_aintgs(double x, double* a) { aintgs(x, a); }
 
 // Overlap block of atoms (zi at pi) and (zj at pj), row = orbitals of i.

@@ -116,6 +116,7 @@ def lib(use_lapack: bool = True, threads: int = 0):
         L.oracle_eigen.argtypes = [ctypes.c_int, dp, dp, dp]
         L.oracle_diis_solve.argtypes = [ctypes.c_int, dp, dp]
         L.oracle_adiis_simplex.argtypes = [ctypes.c_int, dp, dp, dp]
+        L.oracle_overlap_locals.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, dp]
         L.oracle_gradient.argtypes = [ctypes.POINTER(_Request), ctypes.POINTER(_Result), ctypes.c_double, dp]
         _lib = L
         if use_lapack:
@@ -253,6 +254,16 @@ def scf_many(molecules, threads=0, **kw):
         rss[i].charges = _dp(q)
     L.oracle_scf_many(n, rqs, rss, int(threads))
     return [_result_dict(rss[i], {"charges": charges[i]}) for i in range(n)]
+
+
+def overlap_locals(za, zb, r, method="am1", numeric=False):
+    """The five local overlaps of the ordered pair (za the higher n) at distance r (Bohr): the
+    closed forms, or with ``numeric`` the n >= 4 quadrature."""
+    out = np.zeros(5)
+    st = lib().oracle_overlap_locals(int(za), int(zb), float(r), METHODS[method], 1 if numeric else 0, _dp(out))
+    if st:
+        raise ValueError("no closed form for this pair")
+    return out
 
 
 def gradient(z, xyz_bohr, step=1.0e-4, **kw):
