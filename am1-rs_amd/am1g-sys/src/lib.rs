@@ -182,6 +182,9 @@ pub mod raw {
         pub fn am1g_get_tunable(ctx: *const am1g_ctx, name: *const c_char, value: *mut c_double) -> c_int;
         pub fn am1g_gradient(ctx: *mut am1g_ctx, rq: *const am1g_request, res: *mut am1g_result,
                              gradient: *mut c_double) -> c_int;
+        pub fn am1g_core_hamiltonian(ctx: *mut am1g_ctx, rq: *const am1g_request, h_core: *mut c_double,
+                                     core_ev: *mut c_double, pair_blocks: *mut c_double,
+                                     pair_blocks_len: c_longlong) -> c_int;
         pub fn am1g_bcc_charges(n_atoms: c_int, z: *const u8, xyz_bohr: *const c_double, mulliken: *const c_double,
                                 charges: *mut c_double, atom_types: *mut c_int, n_warnings: *mut c_int,
                                 warnings: *mut c_char, warnings_len: c_int) -> c_int;

@@ -167,7 +167,7 @@ EXPORTS = ["am1g_request_defaults", "am1g_create", "am1g_destroy", "am1g_scf", "
            "am1g_serve_submit", "am1g_serve_poll", "am1g_serve_cancel", "am1g_serve_get_stats",
            "am1g_serve_last_error", "am1g_symmetric_eigen", "am1g_last_error", "am1g_element_info", "am1g_last_kernel_ms",
            "am1g_scf_batch_noalloc", "am1g_noalloc_capacity_bytes", "am1g_bcc_charges", "am1g_gradient",
-           "am1g_set_tunable", "am1g_get_tunable"]
+           "am1g_set_tunable", "am1g_get_tunable", "am1g_core_hamiltonian"]
 
 _lib = None
 
@@ -201,6 +201,9 @@ def load():
     L.am1g_get_tunable.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double)]
     L.am1g_gradient.argtypes = [ctypes.c_void_p, ctypes.POINTER(Request), ctypes.POINTER(Result),
                                 ctypes.POINTER(ctypes.c_double)]
+    L.am1g_core_hamiltonian.argtypes = [ctypes.c_void_p, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_double),
+                                        ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                        ctypes.c_longlong]
     L.am1g_noalloc_capacity_bytes.argtypes = []
     L.am1g_noalloc_capacity_bytes.restype = ctypes.c_longlong
     L.am1g_dc_defaults.argtypes = [ctypes.POINTER(DcOptions)]
@@ -426,6 +429,43 @@ class Engine:
         if rs.status not in (0, 4):
             d["message"] = self.last_error()
         return d
+
+    def core_hamiltonian(self, z, xyz_bohr, **kw) -> dict:
+        """The device's core Hamiltonian (``am1g_core_hamiltonian``): ``h_core`` [nao][nao],
+        ``core_ev`` and ``pairs``, a dict {(a, b): w} of every pair's two-electron block with atom
+        indices in input order, a the heavy atom (or the lower index of two like atoms) and w of
+        shape (10, 10), (10, 1) or (1, 1)."""
+        keep = []
+        rq = self._request(z, xyz_bohr, keep, **kw)
+        zz = np.asarray(z, dtype=np.int64).ravel()
+        heavy = [a for a in range(len(zz)) if zz[a] != 1]
+        hyd = [a for a in range(len(zz)) if zz[a] == 1]
+        nh, nH = len(heavy), len(hyd)
+        nao = 4 * nh + nH
+        w_len = 100 * nh * (nh - 1) // 2 + 10 * nh * nH + nH * (nH - 1) // 2
+        H = np.zeros((nao, nao))
+        w = np.zeros(max(w_len, 1))
+        ec = ctypes.c_double(0.0)
+        dp = ctypes.POINTER(ctypes.c_double)
+        st = self._L.am1g_core_hamiltonian(self._ctx, ctypes.byref(rq), H.ctypes.data_as(dp), ctypes.byref(ec),
+                                           w.ctypes.data_as(dp), len(w))
+        if st != 0:
+            raise ValueError(f"{STATUS_NAMES.get(st, st)}: {self.last_error()}")
+        pairs = {}
+        k = 0
+        for i in range(nh):
+            for j in range(i + 1, nh):
+                pairs[(heavy[i], heavy[j])] = w[k:k + 100].reshape(10, 10).copy()
+                k += 100
+        for i in range(nh):
+            for j in range(nH):
+                pairs[(heavy[i], hyd[j])] = w[k:k + 10].reshape(10, 1).copy()
+                k += 10
+        for i in range(nH):
+            for j in range(i + 1, nH):
+                pairs[(hyd[i], hyd[j])] = w[k:k + 1].reshape(1, 1).copy()
+                k += 1
+        return {"h_core": H, "core_ev": ec.value, "pairs": pairs}
 
     def scf_batch(self, molecules, full: bool = False, noalloc: bool = False, **kw) -> list:
         """Many SCFs in one persistent launch (``am1g_scf_batch``; ``am1g_scf_batch_noalloc`` with

@@ -340,21 +340,16 @@ static int large_uhf(am1g_ctx* ctx, const am1g_request& rq, const MolDesc& md, c
 // kernels, library GEMMs for the commutator and the density (FP32 while the commutator norm is
 // above 1e-4 in mixed precision), packed DIIS histories on the device with the small CDIIS and
 // A-DIIS solves on the host, and the large-order eigensolver.
-int am1g_large_scf(am1g_ctx* ctx, const am1g_request& rq, const Prepared& prep, am1g_result* res) {
-    const MolDesc& md0 = prep.md;
-    const bool uhf = prep.uhf;
-    int st = am1g_large_setup(ctx);
-    if (st != AM1G_OK) return res->status = st;
-    LargeBuffers& L = *ctx->large;
+// The core Hamiltonian of one large molecule on the device (build_core_with_neighbors,
+// hamiltonian.rs:211-366; core_core_energy_with_neighbors, repulsion.rs:38-56): atom tables, the
+// pair pool (k_pairs), H_core in L.H and the core-core energy in L.scal[30] (queued on the
+// context's stream, not yet synchronized). Fills `sa` for the kernels that follow and reserves
+// L.partial and L.scal for the SCF.
+static int large_core(am1g_ctx* ctx, const am1g_request& rq, const MolDesc& md, LargeBuffers& L, SysArgs& sa) {
     hipStream_t s = ctx->stream;
-    MolDesc md = md0;
-    const int nat = md.nat, n = md.nao, nocc = md.n_alpha;
+    const int nat = md.nat, n = md.nao;
     const size_t nn = (size_t)n * n;
     const DevElem* tab = ctx->host_tab + md.method * kMaxZ;
-    auto t_all = Clock::now(), t_ph = t_all;
-    double phase[6] = {0, 0, 0, 0, 0, 0};
-
-    // ---- atom tables, pair integrals, H_core, core-core energy
     std::vector<int> zs(nat), alist, aoff(nat), acls(nat);
     {
         int off = 0, nh = 0, nH = 0;
@@ -412,7 +407,6 @@ int am1g_large_scf(am1g_ctx* ctx, const am1g_request& rq, const Prepared& prep, 
     ba.n_pairs = npairs;
     ba.ecore_pair = ctx->ecore_pair.as<double>();
     HIP_TRY(ctx, L.pdiag.reserve(sizeof(double) * 10 * nat));
-    SysArgs sa;
     sa.md = md;
     sa.z = ba.z;
     sa.xyz = ba.xyz;
@@ -429,6 +423,33 @@ int am1g_large_scf(am1g_ctx* ctx, const am1g_request& rq, const Prepared& prep, 
     sa.tcol = L.tcol.as<double>();
     sa.rsum = L.rsum.as<double>();
 
+    const int64_t n_part = std::max<int64_t>({large_partials(n), 4096, 2 * fock_energy_parts(md)});
+    HIP_TRY(ctx, L.partial.reserve(sizeof(double) * n_part));
+    HIP_TRY(ctx, L.scal.reserve(sizeof(double) * 64));
+    HIP_TRY(ctx, launch_pairs(ba, s));
+    HIP_TRY(ctx, launch_hcore_diag(sa, s));
+    HIP_TRY(ctx, launch_sum(npairs, ba.ecore_pair, L.partial.as<double>(), L.scal.as<double>() + 30, s));
+    return AM1G_OK;
+}
+
+int am1g_large_scf(am1g_ctx* ctx, const am1g_request& rq, const Prepared& prep, am1g_result* res) {
+    const MolDesc& md0 = prep.md;
+    const bool uhf = prep.uhf;
+    int st = am1g_large_setup(ctx);
+    if (st != AM1G_OK) return res->status = st;
+    LargeBuffers& L = *ctx->large;
+    hipStream_t s = ctx->stream;
+    MolDesc md = md0;
+    const int nat = md.nat, n = md.nao, nocc = md.n_alpha;
+    const size_t nn = (size_t)n * n;
+    const DevElem* tab = ctx->host_tab + md.method * kMaxZ;
+    auto t_all = Clock::now(), t_ph = t_all;
+    double phase[6] = {0, 0, 0, 0, 0, 0};
+
+    // ---- atom tables, pair integrals, H_core, core-core energy
+    SysArgs sa;
+    st = large_core(ctx, rq, md, L, sa);
+    if (st != AM1G_OK) return res->status = st;
     const bool need_d = md.accel == 2;
     const int64_t len_s = (int64_t)n * (n + 1) / 2, len_a = (int64_t)n * (n - 1) / 2;
     HIP_TRY(ctx, L.F.reserve(sizeof(double) * nn));
@@ -443,9 +464,6 @@ int am1g_large_scf(am1g_ctx* ctx, const am1g_request& rq, const Prepared& prep, 
     if (need_d) HIP_TRY(ctx, L.hist_d.reserve(sizeof(double) * kDiisDepth * len_s));
     HIP_TRY(ctx, L.ring.reserve(sizeof(int) * kDiisDepth));
     HIP_TRY(ctx, L.coef.reserve(sizeof(double) * kDiisDepth));
-    const int64_t n_part = std::max<int64_t>({large_partials(n), 4096, 2 * fock_energy_parts(md)});
-    HIP_TRY(ctx, L.partial.reserve(sizeof(double) * n_part));
-    HIP_TRY(ctx, L.scal.reserve(sizeof(double) * 64));
     HIP_TRY(ctx, L.props.reserve(sizeof(double) * 4 * nat));
     if (md.precision == 1) {
         HIP_TRY(ctx, L.f32a.reserve(sizeof(float) * nn));
@@ -466,9 +484,6 @@ int am1g_large_scf(am1g_ctx* ctx, const am1g_request& rq, const Prepared& prep, 
     int* dring = L.ring.as<int>();
     double* dcoef = L.coef.as<double>();
     // scal layout: 0 sum P(H+F), 1 sum E^2, 2..25 DIIS dots, 26 non-finite count, 27 sum dP^2, 30 E_core
-    HIP_TRY(ctx, launch_pairs(ba, s));
-    HIP_TRY(ctx, launch_hcore_diag(sa, s));
-    HIP_TRY(ctx, launch_sum(npairs, ba.ecore_pair, partial, scal + 30, s));
     double h_scal[64];
     if (uhf) {
         HIP_TRY(ctx, hipMemcpyAsync(h_scal, scal + 30, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -656,7 +671,7 @@ int am1g_large_scf(am1g_ctx* ctx, const am1g_request& rq, const Prepared& prep, 
     std::memset(&r, 0, sizeof(r));
     double dip[3] = {0, 0, 0}, eiso = 0.0, eheat = 0.0;
     for (int a = 0; a < nat; ++a) {
-        const DevElem& el = tab[zs[a]];
+        const DevElem& el = tab[rq.z[a]];
         const double q = el.core - props[4 * a];
         if (res->charges) res->charges[a] = q;
         for (int k = 0; k < 3; ++k) dip[k] += rq.xyz_bohr[3 * a + k] * q;
@@ -700,6 +715,38 @@ int am1g_large_scf(am1g_ctx* ctx, const am1g_request& rq, const Prepared& prep, 
 }
 
 extern "C" {
+
+int am1g_core_hamiltonian(am1g_ctx* ctx, const am1g_request* rq, double* h_core, double* core_ev,
+                          double* pair_blocks, long long pair_blocks_len) {
+    if (!ctx || !rq) return am1g_set_error(ctx, AM1G_INVALID_INPUT, "null argument");
+    Prepared prep;
+    std::string msg;
+    int st = am1g_prepare(ctx, *rq, prep, msg, INT_MAX);
+    if (st != AM1G_OK) return am1g_set_error(ctx, st, "%s", msg.c_str());
+    const MolDesc& md = prep.md;
+    const int64_t w_len = w_pool_size(md.n_heavy, md.n_h);
+    if (pair_blocks && pair_blocks_len < w_len)
+        return am1g_set_error(ctx, AM1G_INVALID_INPUT, "pair_blocks holds %lld doubles; this molecule needs %lld",
+                              pair_blocks_len, (long long)w_len);
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    st = am1g_large_setup(ctx);
+    if (st != AM1G_OK) return st;
+    LargeBuffers& L = *ctx->large;
+    SysArgs sa;
+    st = large_core(ctx, *rq, md, L, sa);
+    if (st != AM1G_OK) return st;
+    hipStream_t s = ctx->stream;
+    const size_t nn = (size_t)md.nao * md.nao;
+    if (h_core) HIP_TRY(ctx, hipMemcpyAsync(h_core, L.H.p, sizeof(double) * nn, hipMemcpyDeviceToHost, s));
+    double ec = 0.0;
+    HIP_TRY(ctx, hipMemcpyAsync(&ec, L.scal.as<double>() + 30, sizeof(double), hipMemcpyDeviceToHost, s));
+    if (pair_blocks && w_len > 0)
+        HIP_TRY(ctx, hipMemcpyAsync(pair_blocks, ctx->wpool.p, sizeof(double) * w_len, hipMemcpyDeviceToHost, s));
+    HIP_TRY(ctx, hipStreamSynchronize(s));
+    if (core_ev) *core_ev = ec;
+    return AM1G_OK;
+}
 
 int am1g_symmetric_eigen(am1g_ctx* ctx, int n, const double* a, double* eigenvalues, double* eigenvectors) {
     if (!ctx || n < 0 || (n > 0 && (!a || !eigenvalues || !eigenvectors)))

@@ -246,6 +246,20 @@ int am1g_gradient(am1g_ctx* ctx, const am1g_request* rq, am1g_result* res, doubl
 int am1g_bcc_charges(int n_atoms, const uint8_t* z, const double* xyz_bohr, const double* mulliken,
                      double* charges, int* atom_types, int* n_warnings, char* warnings, int warnings_len);
 
+/* The core Hamiltonian of one molecule as the device builds it: build_core_with_neighbors
+ * (src/hamiltonian.rs:211-366, CoreHamiltonian{h_core, pairs}) and
+ * core_core_energy_with_neighbors (src/repulsion.rs:38-56). h_core[nao][nao] (row-major; NULL =
+ * skip) receives H_core, including an external field's dipole term; *core_ev (NULL = skip) the
+ * core-core repulsion in eV; pair_blocks[pair_blocks_len] (NULL = skip) the two-electron pair
+ * integrals (PairIntegral::w, src/hamiltonian.rs:58-83) grouped by pair class, each pair
+ * oriented heavy atom first with b - a as the separation: heavy-heavy pairs (i < j over the heavy
+ * atoms in input order) as 10x10 row-major blocks, then heavy-hydrogen pairs (heavy-major, each
+ * heavy atom against every hydrogen in input order) as 10 values, then hydrogen-hydrogen pairs
+ * (i < j) as one value; it needs 100 hh + 10 hH + HH doubles. For per-block checks of the
+ * pair-integral kernel. AM1G_INVALID_INPUT for a short pair_blocks buffer. */
+int am1g_core_hamiltonian(am1g_ctx* ctx, const am1g_request* rq, double* h_core, double* core_ev,
+                          double* pair_blocks, long long pair_blocks_len);
+
 /* Text of the last error raised on the calling thread by any am1g_* call that takes a context
  * (the Display text of the matching Am1Error). Thread-local: concurrent callers sharing a
  * context each read their own message. */
