@@ -43,6 +43,7 @@ pub mod raw {
         pub precision: c_int,
         pub electric_field: *const c_double,
         pub bcc: c_int,
+        pub multipole_cutoff: c_double,
     }
 
     #[repr(C)]
@@ -262,6 +263,8 @@ pub struct ScfOptions {
     pub mixed_precision: bool,
     /// eV per (e*Bohr)
     pub electric_field: Option<[f64; 3]>,
+    /// Bohr; pairs farther apart are net-charge monopoles (Am1Options::multipole_cutoff)
+    pub multipole_cutoff: Option<f64>,
 }
 
 impl Default for ScfOptions {
@@ -279,6 +282,7 @@ impl Default for ScfOptions {
             adiis_switch: 0.1,
             mixed_precision: false,
             electric_field: None,
+            multipole_cutoff: None,
         }
     }
 }
@@ -357,6 +361,7 @@ impl Context {
         rq.adiis_switch = o.adiis_switch;
         rq.precision = o.mixed_precision as i32;
         rq.electric_field = field.as_ref().map_or(std::ptr::null(), |f| f.as_ptr());
+        rq.multipole_cutoff = o.multipole_cutoff.unwrap_or(0.0);
         let mut q = vec![0.0; z.len()];
         let mut eps = vec![0.0; nao];
         let mut eps_b = vec![0.0; nao];

@@ -56,6 +56,7 @@ class Request(ctypes.Structure):
         ("precision", ctypes.c_int),
         ("electric_field", ctypes.POINTER(ctypes.c_double)),
         ("bcc", ctypes.c_int),
+        ("multipole_cutoff", ctypes.c_double),
     ]
 
 
@@ -301,8 +302,9 @@ class Engine:
     # -- request marshalling -------------------------------------------------------------
     def _request(self, z, xyz_bohr, keep, charge=0.0, multiplicity=1, method="am1", reference="auto",
                  accelerator="adiis_cdiis", use_diis=True, max_scf=200, e_tol=1e-8, p_tol=1e-7,
-                 adiis_switch=0.1, precision="fp64", electric_field=None, bcc=False):
-        """``electric_field``: three floats in eV per (e*Bohr) (Am1Options::electric_field) or None."""
+                 adiis_switch=0.1, precision="fp64", electric_field=None, bcc=False, multipole_cutoff=None):
+        """``electric_field``: three floats in eV per (e*Bohr) (Am1Options::electric_field) or None.
+        ``multipole_cutoff``: Bohr (Am1Options::multipole_cutoff), None for every pair exact."""
         rq = Request()
         self._L.am1g_request_defaults(ctypes.byref(rq))
         zi = np.asarray(z)
@@ -327,6 +329,7 @@ class Engine:
         rq.adiis_switch = float(adiis_switch)
         rq.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
         rq.bcc = 1 if bcc else 0
+        rq.multipole_cutoff = float(multipole_cutoff) if multipole_cutoff is not None else 0.0
         if electric_field is not None:
             fa = np.ascontiguousarray(electric_field, dtype=np.float64).reshape(-1)
             if fa.size != 3:

@@ -192,6 +192,8 @@ int am1g_prepare(am1g_ctx* ctx, const am1g_request& rq, Prepared& out, std::stri
     md.e_tol = rq.e_tol;
     md.p_tol = rq.p_tol;
     md.adiis_switch = rq.adiis_switch;
+    // a cutoff that is not > 0 (NaN included) keeps every pair (farfield.rs:97-104)
+    md.far_cut = rq.multipole_cutoff > 0.0 ? rq.multipole_cutoff : 0.0;
     if (rq.electric_field) {
         // the nuclear half of -mu.F (dipole.rs:146-153); the electronic half enters H_core
         double nuc = 0.0;

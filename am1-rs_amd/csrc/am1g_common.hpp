@@ -51,6 +51,7 @@ struct MolDesc {
     double e_tol, p_tol, adiis_switch;
     double field[3];    // uniform external field, eV per (e*Bohr); zero for none
     double field_nuc;   // its nuclear energy -F . sum_a Z_a R_a (eV)
+    double far_cut;     // far-field cutoff (Bohr): pairs beyond it are net-charge monopoles; 0 = off
     int64_t atom_off;   // into z[] / xyz[] (xyz: 3 per atom)
     int64_t ao_off;     // into per-AO arrays (mo energies)
     int64_t mat_off;    // start of this molecule's dense matrices

@@ -97,6 +97,7 @@ struct DcChannel {  // one spin channel (the only one for a closed shell)
 };
 struct DcBuffers {
     DcChannel ch[2];
+    DevBuf far;  // far-field terms (SysArgs::far)
     DevBuf Pt, pdiag, tcol, rsum, fbuf, t0, t1, status, sub_n, sub_ao, sub_mat, ao_list, ao_core, pat, mult, coef, ao_atom,
         core_sub, partial, scal;
 };

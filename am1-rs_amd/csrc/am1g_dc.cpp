@@ -257,6 +257,11 @@ int am1g_dc(am1g_ctx* ctx, const am1g_request* rq, const am1g_dc_options* op, am
     HIP_TRY(ctx, B.rsum.reserve(sizeof(double) * 10 * std::max(1, md.n_heavy)));
     sa.tcol = B.tcol.as<double>();
     sa.rsum = B.rsum.as<double>();
+    sa.far = nullptr;
+    if (md.far_cut > 0.0) {
+        HIP_TRY(ctx, B.far.reserve(sizeof(double) * (2 * (size_t)nat + 1)));
+        sa.far = B.far.as<double>();
+    }
 
     HIP_TRY(ctx, B.partial.reserve(sizeof(double) * std::max<int64_t>({1024 * (kDiisDepth + 1), 2 * fock_energy_parts(md),
                                                                        fermi_scratch_doubles()})));
@@ -669,8 +674,10 @@ int am1g_dc(am1g_ctx* ctx, const am1g_request* rq, const am1g_dc_options* op, am
         eps[c].resize(L);
         HIP_TRY(ctx, hipMemcpyAsync(eps[c].data(), dch[c].eps, sizeof(double) * L, hipMemcpyDeviceToHost, s));
     }
+    double zv = 0.0;  // sum Z_a V_a of the far field (long_range_energy_term, fock.rs:200-232)
+    if (sa.far) HIP_TRY(ctx, hipMemcpyAsync(&zv, sa.far + 2 * (size_t)nat, sizeof(double), hipMemcpyDeviceToHost, s));
     HIP_TRY(ctx, hipStreamSynchronize(s));
-    const double electronic = e_of(host_scal);
+    const double electronic = e_of(host_scal) + 0.5 * zv;
     const double ecore = host_scal[20];
     double eiso = 0.0, eheat = 0.0;
     for (int a = 0; a < nat; ++a) {

@@ -179,6 +179,26 @@ __global__ __launch_bounds__(128) void k_grad_pairs(GradArgs g) {
         const DevElem& ea = el[za];
         const DevElem& eb = el[zb];
         const int oa = g.aoff[a], ob = g.aoff[b], n = g.md.nao;
+        const double* xa = g.xyz + 3 * (size_t)a;
+        const double* xb = g.xyz + 3 * (size_t)b;
+        {
+            const double d0 = xb[0] - xa[0], d1 = xb[1] - xa[1], d2 = xb[2] - xa[2];
+            const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+            if (g.md.far_cut > 0.0 && r2 > g.md.far_cut * g.md.far_cut) {
+                // a far-field pair: its energy is the monopole Q_A Q_B 27.21 / sqrt(r^2 + eta^2)
+                // at the converged net charges (FarField::gradient, farfield.rs:178-206)
+                double pop_a = 0.0, pop_b = 0.0;
+                for (int k = 0; k < ea.norb; ++k) pop_a += g.pa[(size_t)(oa + k) * (n + 1)] + g.pb[(size_t)(oa + k) * (n + 1)];
+                for (int k = 0; k < eb.norb; ++k) pop_b += g.pa[(size_t)(ob + k) * (n + 1)] + g.pb[(size_t)(ob + k) * (n + 1)];
+                const double eta = ea.rho0 + eb.rho0;
+                const double sr = sqrt(r2 + eta * eta);
+                const double f = -kEv * (ea.core - pop_a) * (eb.core - pop_b) / (sr * sr * sr);
+                g.fpair[3 * p] = f * d0;
+                g.fpair[3 * p + 1] = f * d1;
+                g.fpair[3 * p + 2] = f * d2;
+                continue;
+            }
+        }
         // resonance coefficients: Pt_ij (beta_i + beta_j), both triangles of H_core
         double cres[16];
 #pragma unroll
@@ -193,8 +213,6 @@ __global__ __launch_bounds__(128) void k_grad_pairs(GradArgs g) {
                 cres[i * 4 + j] = v;
             }
         const double* c = g.coef + grad_coef_off(g, p);
-        const double* xa = g.xyz + 3 * (size_t)a;
-        const double* xb = g.xyz + 3 * (size_t)b;
         for (int k = 0; k < 3; ++k) {
             double e[4];
 #pragma unroll 1

@@ -18,7 +18,7 @@
 struct LargeBuffers {
     DevBuf A, evals, Ct;                       // eigensolver entry point
     DevBuf H, F, P, T0, T1, Pn, f32a, f32b;    // dense n x n (FP32 staging for mixed precision)
-    DevBuf hist_f, hist_e, hist_d, ring, coef, partial, scal, pdiag, tcol, rsum, props;
+    DevBuf hist_f, hist_e, hist_d, ring, coef, partial, scal, pdiag, tcol, rsum, props, far;
     DevBuf Fb, Pb, Pt, Tb, Ctb, evals_b, hist_fb, hist_eb;  // beta channel (UHF)
 };
 
@@ -276,8 +276,10 @@ static int large_uhf(am1g_ctx* ctx, const am1g_request& rq, const MolDesc& md, c
     HIP_TRY(ctx, hipMemcpyAsync(props.data(), L.props.p, sizeof(double) * 4 * nat, hipMemcpyDeviceToHost, s));
     HIP_TRY(ctx, hipMemcpyAsync(heps.data(), eps[0], sizeof(double) * n, hipMemcpyDeviceToHost, s));
     HIP_TRY(ctx, hipMemcpyAsync(hepsb.data(), eps[1], sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    double zv = 0.0;  // sum Z_a V_a of the far field (long_range_energy_term, fock.rs:200-232)
+    if (sa.far) HIP_TRY(ctx, hipMemcpyAsync(&zv, sa.far + 2 * (size_t)nat, sizeof(double), hipMemcpyDeviceToHost, s));
     HIP_TRY(ctx, hipStreamSynchronize(s));
-    const double electronic = 0.5 * (h_scal[1] + h_scal[2]);
+    const double electronic = 0.5 * (h_scal[1] + h_scal[2]) + 0.5 * zv;
     MolResult r;
     std::memset(&r, 0, sizeof(r));
     double dip[3] = {0, 0, 0}, eiso = 0.0, eheat = 0.0;
@@ -422,6 +424,11 @@ static int large_core(am1g_ctx* ctx, const am1g_request& rq, const MolDesc& md, 
     HIP_TRY(ctx, L.rsum.reserve(sizeof(double) * 10 * std::max(1, md.n_heavy)));
     sa.tcol = L.tcol.as<double>();
     sa.rsum = L.rsum.as<double>();
+    sa.far = nullptr;
+    if (md.far_cut > 0.0) {
+        HIP_TRY(ctx, L.far.reserve(sizeof(double) * (2 * (size_t)nat + 1)));
+        sa.far = L.far.as<double>();
+    }
 
     const int64_t n_part = std::max<int64_t>({large_partials(n), 4096, 2 * fock_energy_parts(md)});
     HIP_TRY(ctx, L.partial.reserve(sizeof(double) * n_part));
@@ -665,8 +672,10 @@ int am1g_large_scf(am1g_ctx* ctx, const am1g_request& rq, const Prepared& prep, 
     HIP_TRY(ctx, hipMemcpyAsync(h_scal, scal, sizeof(double) * 30, hipMemcpyDeviceToHost, s));
     HIP_TRY(ctx, hipMemcpyAsync(props.data(), L.props.p, sizeof(double) * 4 * nat, hipMemcpyDeviceToHost, s));
     HIP_TRY(ctx, hipMemcpyAsync(heps.data(), eps, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    double zv = 0.0;  // sum Z_a V_a of the far field (long_range_energy_term, fock.rs:200-232)
+    if (sa.far) HIP_TRY(ctx, hipMemcpyAsync(&zv, sa.far + 2 * (size_t)nat, sizeof(double), hipMemcpyDeviceToHost, s));
     HIP_TRY(ctx, hipStreamSynchronize(s));
-    const double electronic = 0.5 * h_scal[0];
+    const double electronic = 0.5 * h_scal[0] + 0.5 * zv;
     MolResult r;
     std::memset(&r, 0, sizeof(r));
     double dip[3] = {0, 0, 0}, eiso = 0.0, eheat = 0.0;

@@ -93,6 +93,8 @@ struct SysArgs {
     double* tcol;        // [n_hh][10] heavy pair (i < j): Coulomb of i's diagonal block on j's, at
                          // j (j - 1) / 2 + i (each column contiguous)
     double* rsum;        // [n_heavy][10] Coulomb on heavy atom i from its heavy partners j > i
+    double* far;         // [2 nat + 1] far-field terms (md.far_cut > 0): per atom -p_a V_a, then
+                         // Z_a V_a, then sum Z_a V_a of the last Fock build; null when off
 };
 
 // Divide-and-conquer subsystem tables (all device pointers).

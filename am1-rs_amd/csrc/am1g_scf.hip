@@ -128,6 +128,33 @@ __global__ __launch_bounds__(256) void k_pairs(BatchArgs g) {
         double d[3] = {xyz[3 * b] - xyz[3 * a], xyz[3 * b + 1] - xyz[3 * a + 1], xyz[3 * b + 2] - xyz[3 * a + 2]};
         double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
         const int oa = aoff[a], ob = aoff[b];
+        if (md.far_cut > 0.0 && r > md.far_cut) {
+            // beyond the far-field cutoff the pair is out of the pair list (neighbors.rs:90-101):
+            // no integrals, no resonance, no core-core term; its net charges interact through
+            // the far-field potential instead
+            if (k_all < n_hh) {
+                double* dst = wpool + k_all * 100;
+                for (int q = 0; q < 100; ++q) dst[q] = 0.0;
+                for (int i2 = 0; i2 < 4; ++i2)
+                    for (int j2 = 0; j2 < 4; ++j2) {
+                        H[(size_t)(oa + i2) * nao + ob + j2] = 0.0;
+                        H[(size_t)(ob + j2) * nao + oa + i2] = 0.0;
+                    }
+            } else if (k_all < n_hh + n_hH) {
+                double* dst = wpool + n_hh * 100 + (k_all - n_hh) * 10;
+                for (int q = 0; q < 10; ++q) dst[q] = 0.0;
+                for (int i2 = 0; i2 < 4; ++i2) {
+                    H[(size_t)(oa + i2) * nao + ob] = 0.0;
+                    H[(size_t)ob * nao + oa + i2] = 0.0;
+                }
+            } else {
+                wpool[n_hh * 100 + n_hH * 10 + (k_all - n_hh - n_hH)] = 0.0;
+                H[(size_t)oa * nao + ob] = 0.0;
+                H[(size_t)ob * nao + oa] = 0.0;
+            }
+            g.ecore_pair[p] = 0.0;
+            continue;
+        }
         double S[16];
         overlap_block(ea, eb, d, S);
         if (k_all < n_hh) {
@@ -221,12 +248,37 @@ __device__ double mol_setup(const MolDesc& md, int mi, const BatchArgs& g, MolSh
     return ecore;
 }
 
+// Far-field potential at atom x from the atoms y = y0, y0 + dy, ... (FarField::potential,
+// farfield.rs:150-176): 27.21 / sqrt(r^2 + (rho0_x + rho0_y)^2) * Q_y summed over the atoms
+// farther than md.far_cut, with the net charge Q_y = Z_y minus y's population on Pt's diagonal
+// (pbc::ewald::net_charges).
+__device__ double far_potential(const MolDesc& md, const DevElem* el, const MolShared& ms, const double* xyz,
+                                const double* Pt, int x, int y0, int dy) {
+    const int nao = md.nao;
+    const double cut2 = md.far_cut * md.far_cut;
+    const double rx = el[ms.z[x]].rho0;
+    const double x0 = xyz[3 * x], x1 = xyz[3 * x + 1], x2 = xyz[3 * x + 2];
+    double acc = 0.0;
+    for (int y = y0; y < md.nat; y += dy) {
+        const double d0 = xyz[3 * y] - x0, d1 = xyz[3 * y + 1] - x1, d2 = xyz[3 * y + 2] - x2;
+        const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+        if (y == x || r2 <= cut2) continue;
+        const DevElem& ey = el[ms.z[y]];
+        const int oy = ms.off[y];
+        double pop = 0.0;
+        for (int k = 0; k < ey.norb; ++k) pop += Pt[(size_t)(oy + k) * nao + oy + k];
+        const double eta = rx + ey.rho0;
+        acc += kEv / sqrt(r2 + eta * eta) * (ey.core - pop);
+    }
+    return acc;
+}
+
 // F = H + one-centre(P, P_spin) + J(P) - K(scale * P_spin) for one spin channel
 // (fock.rs:249-511). Diagonal atom blocks: one thread per (atom, packed orbital pair), summing
 // over partner atoms in index order. Off-diagonal blocks: one thread per pair.
 __device__ __noinline__ void mol_fock(const MolDesc& md, const DevElem* el, const MolShared& ms, const double* wpool,
                          const double* H, const double* Pt, const double* Ps, double scale, double* F,
-                         double* pdg) {
+                         double* pdg, const double* xyz) {
     const int t = threadIdx.x;
     const int nao = md.nao;
     const int nh = ms.nh, nH = ms.nH;
@@ -301,6 +353,8 @@ __device__ __noinline__ void mol_fock(const MolDesc& md, const DevElem* el, cons
         const DevElem& ex = el[ms.z[x]];
         const int ox = ms.off[x], nx = ex.norb;
         const int mu = kPackRow[k], nu = kPackCol[k];
+        // far-field shift -V_x on the atom's diagonal (fock.rs:278-318), partners split the same way
+        if (md.far_cut > 0.0 && mu == nu) acc -= far_potential(md, el, ms, xyz, Pt, x, gl, G);
         for (int off = 1; off < G; off <<= 1) acc += __shfl_xor(acc, off, 64);
         if (gl == 0) {
             acc += H[(size_t)(ox + mu) * nao + ox + nu];
@@ -678,7 +732,7 @@ __device__ __noinline__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolS
     int loose_iters = 0;  // FP64 iterations that met the p_tol test
     for (int iter = 0; iter < md.max_scf; ++iter) {
         iterations = iter + 1;
-        mol_fock(md, el, ms, wpool, H, P, P, 0.5, F, lds);
+        mol_fock(md, el, ms, wpool, H, P, P, 0.5, F, lds, g.xyz + md.atom_off * 3);
         __syncthreads();
         phase_mark(pc, 1);
         // error = FP - (FP)^T; F and P are symmetric so FP = F^T P
@@ -904,17 +958,18 @@ __device__ __noinline__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolS
     }
     __syncthreads();
     // final energy from the final density (scf.rs:698-701)
-    mol_fock(md, el, ms, wpool, H, P, P, 0.5, F, lds);
+    mol_fock(md, el, ms, wpool, H, P, P, 0.5, F, lds, g.xyz + md.atom_off * 3);
     __syncthreads();
     double pe = 0.0;
     for (size_t idx = t; idx < nn; idx += kWg) pe += P[idx] * (H[idx] + F[idx]);
     double electronic = 0.5 * wg_sum(pe, s);
     // Mulliken charges and dipole (scf.rs:400-427)
     double dip[3] = {0.0, 0.0, 0.0};
-    double eiso = 0.0, eheat = 0.0;
+    double eiso = 0.0, eheat = 0.0, ezv = 0.0;
     const double* xyz = g.xyz + md.atom_off * 3;
     for (int a = t; a < nat; a += kWg) {
         const DevElem& e = el[ms.z[a]];
+        if (md.far_cut > 0.0) ezv += e.core * far_potential(md, el, ms, xyz, P, a, 0, 1);
         int o = ms.off[a];
         double pop = 0.0;
         for (int k = 0; k < e.norb; ++k) pop += P[(size_t)(o + k) * nao + o + k];
@@ -929,6 +984,8 @@ __device__ __noinline__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolS
     for (int k = 0; k < 3; ++k) dip[k] = wg_sum(dip[k], s);
     eiso = wg_sum(eiso, s);
     eheat = wg_sum(eheat, s);
+    // the far-field energy term +1/2 sum_a Z_a V_a of the final energy (scf.rs:699-701)
+    if (md.far_cut > 0.0) electronic += 0.5 * wg_sum(ezv, s);
     if (t == 0) {
         res.status = (status == kOk && !converged) ? kScfNotConverged : status;
         res.iterations = iterations;
@@ -1010,8 +1067,8 @@ __device__ __noinline__ void scf_uhf(const MolDesc& md, const BatchArgs& g, MolS
         iterations = iter + 1;
         for (size_t idx = t; idx < nn; idx += kWg) Pt[idx] = Ps[0][idx] + Ps[1][idx];
         __syncthreads();
-        mol_fock(md, el, ms, wpool, H, Pt, Ps[0], 1.0, Fs[0], lds);
-        mol_fock(md, el, ms, wpool, H, Pt, Ps[1], 1.0, Fs[1], lds);
+        mol_fock(md, el, ms, wpool, H, Pt, Ps[0], 1.0, Fs[0], lds, g.xyz + md.atom_off * 3);
+        mol_fock(md, el, ms, wpool, H, Pt, Ps[1], 1.0, Fs[1], lds, g.xyz + md.atom_off * 3);
         __syncthreads();
         double pe = 0.0;
         for (size_t idx = t; idx < nn; idx += kWg)
@@ -1192,16 +1249,17 @@ __device__ __noinline__ void scf_uhf(const MolDesc& md, const BatchArgs& g, MolS
     __syncthreads();
     for (size_t idx = t; idx < nn; idx += kWg) Pt[idx] = Ps[0][idx] + Ps[1][idx];
     __syncthreads();
-    mol_fock(md, el, ms, wpool, H, Pt, Ps[0], 1.0, Fs[0], lds);
-    mol_fock(md, el, ms, wpool, H, Pt, Ps[1], 1.0, Fs[1], lds);
+    mol_fock(md, el, ms, wpool, H, Pt, Ps[0], 1.0, Fs[0], lds, g.xyz + md.atom_off * 3);
+    mol_fock(md, el, ms, wpool, H, Pt, Ps[1], 1.0, Fs[1], lds, g.xyz + md.atom_off * 3);
     __syncthreads();
     double pe = 0.0;
     for (size_t idx = t; idx < nn; idx += kWg) pe += Pt[idx] * H[idx] + Ps[0][idx] * Fs[0][idx] + Ps[1][idx] * Fs[1][idx];
-    const double electronic = 0.5 * wg_sum(pe, s);
-    double dip[3] = {0.0, 0.0, 0.0}, eiso = 0.0, eheat = 0.0;
+    double electronic = 0.5 * wg_sum(pe, s);
+    double dip[3] = {0.0, 0.0, 0.0}, eiso = 0.0, eheat = 0.0, ezv = 0.0;
     const double* xyz = g.xyz + md.atom_off * 3;
     for (int a = t; a < nat; a += kWg) {
         const DevElem& e = el[ms.z[a]];
+        if (md.far_cut > 0.0) ezv += e.core * far_potential(md, el, ms, xyz, Pt, a, 0, 1);
         int o = ms.off[a];
         double pop = 0.0;
         for (int k = 0; k < e.norb; ++k) pop += Pt[(size_t)(o + k) * nao + o + k];
@@ -1216,6 +1274,8 @@ __device__ __noinline__ void scf_uhf(const MolDesc& md, const BatchArgs& g, MolS
     for (int k = 0; k < 3; ++k) dip[k] = wg_sum(dip[k], s);
     eiso = wg_sum(eiso, s);
     eheat = wg_sum(eheat, s);
+    // the far-field energy term +1/2 sum_a Z_a V_a of the final energy (scf.rs:699-701)
+    if (md.far_cut > 0.0) electronic += 0.5 * wg_sum(ezv, s);
     if (t == 0) {
         res.status = (status == kOk && !converged) ? kScfNotConverged : status;
         res.iterations = iterations;
@@ -2301,6 +2361,59 @@ hipError_t launch_hcore_diag(const SysArgs& a, hipStream_t st) {
     return hipGetLastError();
 }
 
+// Far-field shift of one spin channel's Fock matrix (fock.rs:278-318): F_mumu -= V_a on every AO
+// of atom a, with V_a = sum over atoms b farther than the cutoff of 27.21 / sqrt(r^2 + (rho0_a +
+// rho0_b)^2) * Q_b (FarField::potential, farfield.rs:150-176) and the net charges Q_b from the
+// packed diagonal blocks of Pt (k_pack_diag). One thread per atom; also the atom's energy terms
+// -ps_a V_a (ps_a: its population in Ps, the density this channel's energy sum contracts with F)
+// and Z_a V_a.
+__global__ __launch_bounds__(256) void k_far_shift(SysArgs a, const double* Ps, double* F) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nat = a.md.nat, nao = a.md.nao;
+    if (i >= nat) return;
+    const DevElem* el = a.elems + a.md.method * kMaxZ;
+    const double cut2 = a.md.far_cut * a.md.far_cut;
+    const double x0 = a.xyz[3 * (size_t)i], x1 = a.xyz[3 * (size_t)i + 1], x2 = a.xyz[3 * (size_t)i + 2];
+    const DevElem& ei = el[a.z[i]];
+    double v = 0.0;
+    for (int b = 0; b < nat; ++b) {
+        const double d0 = a.xyz[3 * (size_t)b] - x0, d1 = a.xyz[3 * (size_t)b + 1] - x1,
+                     d2 = a.xyz[3 * (size_t)b + 2] - x2;
+        const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+        if (b == i || r2 <= cut2) continue;
+        const DevElem& eb = el[a.z[b]];
+        const double* pd = pdiag_row(a, b);
+        const double pop = a.z[b] == 1 ? pd[0] : pd[0] + pd[2] + pd[5] + pd[9];
+        const double eta = ei.rho0 + eb.rho0;
+        v += kEv / sqrt(r2 + eta * eta) * (eb.core - pop);
+    }
+    const int o = a.aoff[i];
+    double ps = 0.0;
+    for (int k = 0; k < ei.norb; ++k) {
+        F[(size_t)(o + k) * nao + o + k] -= v;
+        ps += Ps[(size_t)(o + k) * nao + o + k];
+    }
+    a.far[i] = -ps * v;
+    a.far[nat + i] = ei.core * v;
+}
+
+// Sums of k_far_shift's per-atom terms in a fixed order: the channel's energy sum gains
+// sum -ps_a V_a (its Fock sum was taken before the shift) and far[2 nat] = sum Z_a V_a, for the
+// final energy's +1/2 sum Z_a V_a (long_range_energy_term, fock.rs:200-232).
+__global__ __launch_bounds__(256) void k_far_energy(int nat, double* far, double* e_out) {
+    KSum pv, zv;
+    for (int i = threadIdx.x; i < nat; i += 256) {
+        ksum_add(pv, far[i]);
+        ksum_add(zv, far[nat + i]);
+    }
+    const KSum rp = ksum_block(pv);
+    const KSum rz = ksum_block(zv);
+    if (threadIdx.x == 0) {
+        if (e_out) e_out[0] += rp.s + rp.c;
+        far[2 * nat] = rz.s + rz.c;
+    }
+}
+
 hipError_t launch_fock(const SysArgs& a, const double* Pt, const double* Ps, double scale, double* F, hipStream_t st,
                        double* epart, double* e_out) {
     const int64_t nh = a.md.n_heavy, nH = a.md.n_h, ndiag = nh + (nH + 7) / 8;
@@ -2317,6 +2430,11 @@ hipError_t launch_fock(const SysArgs& a, const double* Pt, const double* Ps, dou
         hipLaunchKernelGGL(k_sym_lower, dim3((unsigned)(nt * (nt + 1) / 2)), dim3(256), 0, st, a.md.nao, F);
     }
     if (e_out) hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(256), 0, st, (int)fock_energy_parts(a.md), epart, e_out);
+    if (a.md.far_cut > 0.0) {
+        if (!a.far) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(k_far_shift, dim3(blocks_for(a.md.nat)), dim3(256), 0, st, a, Ps, F);
+        hipLaunchKernelGGL(k_far_energy, dim3(1), dim3(256), 0, st, a.md.nat, a.far, e_out);
+    }
     return hipGetLastError();
 }
 

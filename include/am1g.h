@@ -67,6 +67,10 @@ typedef struct {
     int bcc;                 /* nonzero: also AM1-BCC charges from this SCF's Mulliken charges into
                                 am1g_result.bcc_charges (am1_bcc_charges, bcc/mod.rs:99-150); AM1
                                 method only, and the charges output must be given */
+    double multipole_cutoff; /* Bohr; atom pairs farther apart leave the pair list and interact
+                                through their net charges only (Am1Options::multipole_cutoff,
+                                scf.rs:130-140, farfield.rs:80-236); 0 (the default), a negative
+                                value or NaN keeps every pair exact */
 } am1g_request;
 
 /* Result of one SCF. Pointer members are caller-provided outputs; NULL skips that output. */

@@ -835,7 +835,12 @@ struct System {
     int nao = 0;
     int status = 0;
     std::string err;
+    double far_cut = 0.0;  // multipole_cutoff (Bohr); pairs beyond it are monopoles, 0 = off
 };
+
+// A pair beyond the far-field cutoff leaves the pair list (NeighborList::build_screened,
+// neighbors.rs:90-101: kept when r <= cutoff; a cutoff that is not > 0 keeps every pair).
+static inline bool far_pair(const System& s, double r) { return s.far_cut > 0.0 && r > s.far_cut; }
 
 struct Pair {
     int a, b;  // a: heavy first (hamiltonian.rs:289-294)
@@ -909,6 +914,19 @@ void build_core(const System& s, Mat& H, std::vector<Pair>& pairs) {
         for (int x = 0; x < 4; ++x)
             for (int y = 0; y < 4; ++y) sblk[pi * 16 + x * 4 + y] = S[x][y];
       }
+    if (s.far_cut > 0.0) {  // screened pair list, order kept (farfield.rs: the far pairs are monopoles)
+        size_t k = 0;
+        for (size_t pi = 0; pi < np; ++pi)
+            if (!far_pair(s, pairs[pi].r)) {
+                if (k != pi) {
+                    pairs[k] = pairs[pi];
+                    std::copy(&sblk[pi * 16], &sblk[pi * 16] + 16, &sblk[k * 16]);
+                }
+                ++k;
+            }
+        np = k;
+        pairs.resize(np);
+    }
     for (size_t pi = 0; pi < np; ++pi) {
         const Pair& P = pairs[pi];
         const Elem& ea = ps.el[s.z[P.a]];
@@ -941,6 +959,7 @@ double core_core_energy(const System& s) {
             double d[3];
             for (int k = 0; k < 3; ++k) d[k] = s.xyz[3 * j + k] - s.xyz[3 * i + k];
             double r = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+            if (far_pair(s, r)) continue;  // screened out of the pair list
             double sA = r * kA0;
             double rho = ei.rho0 + ej.rho0;
             double gam = kEv / std::sqrt(r * r + rho * rho);
@@ -973,7 +992,50 @@ double oc_two_electron(int a, int b, int c, int d, const Elem& e) {
     return 0.0;
 }
 
-// fock.rs:249-511: F = H + J(p_tot) - K(spin_scale * p_spin), with an in-order scatter.
+// FarField::potential (farfield.rs:150-176): V_a = sum over far b of 27.21 / sqrt(r^2 + (rho0_a +
+// rho0_b)^2) * Q_b with the net charges Q_b = Z_b - sum of P_tot's diagonal on b
+// (pbc::ewald::net_charges), eV per unit charge. Empty when the far field is off.
+std::vector<double> far_potential(const System& s, const Mat& pt) {
+    std::vector<double> v;
+    if (!(s.far_cut > 0.0)) return v;
+    const ParamSet& ps = *s.ps;
+    std::vector<double> q(s.nat);
+    for (int b = 0; b < s.nat; ++b) {
+        double pop = 0.0;
+        for (int k = 0; k < s.norb[b]; ++k) pop += pt(s.off[b] + k, s.off[b] + k);
+        q[b] = ps.el[s.z[b]].core - pop;
+    }
+    v.assign(s.nat, 0.0);
+    const double cut2 = s.far_cut * s.far_cut;
+    for (int a = 0; a < s.nat; ++a) {
+        double acc = 0.0;
+        for (int b = 0; b < s.nat; ++b) {
+            if (b == a) continue;
+            double r2 = 0.0;
+            for (int k = 0; k < 3; ++k) {
+                const double d = s.xyz[3 * b + k] - s.xyz[3 * a + k];
+                r2 += d * d;
+            }
+            if (r2 <= cut2) continue;
+            const double eta = ps.el[s.z[a]].rho0 + ps.el[s.z[b]].rho0;
+            acc += kEv / std::sqrt(r2 + eta * eta) * q[b];
+        }
+        v[a] = acc;
+    }
+    return v;
+}
+
+// long_range_energy_term (fock.rs:200-232): +1/2 sum_a Z_a V_a, added to the final electronic
+// energy only (the loop's energy for the convergence test leaves it out, scf.rs:595 and :701).
+double far_energy(const System& s, const Mat& pt) {
+    const std::vector<double> v = far_potential(s, pt);
+    double e = 0.0;
+    for (size_t a = 0; a < v.size(); ++a) e += s.ps->el[s.z[a]].core * v[a];
+    return 0.5 * e;
+}
+
+// fock.rs:249-511: F = H + J(p_tot) - K(spin_scale * p_spin), with an in-order scatter, and the
+// far-field shift -V_a on each atom's diagonal (fock.rs:278-318, FockBase::CoreHamiltonian).
 void build_fock(const System& s, const Mat& H, const std::vector<Pair>& pairs, const Mat& pt, const Mat& psp,
                 double spin_scale, Mat& F) {
     const ParamSet& ps = *s.ps;
@@ -1043,6 +1105,9 @@ void build_fock(const System& s, const Mat& H, const std::vector<Pair>& pairs, c
                 F(ob + la, oa + mu) += v;
             }
     }
+    const std::vector<double> vfar = far_potential(s, pt);
+    for (size_t a = 0; a < vfar.size(); ++a)
+        for (int k = 0; k < s.norb[a]; ++k) F(s.off[a] + k, s.off[a] + k) -= vfar[a];
 }
 
 // scf.rs:475-494
@@ -1677,6 +1742,7 @@ struct OracleRequest {
     double e_tol, p_tol, adiis_switch;
     int use_diis;
     const double* electric_field;  // [3] eV per (e*Bohr), or NULL for no field
+    double multipole_cutoff;       // Bohr; pairs beyond it are far-field monopoles (0 = off)
 };
 
 struct OracleResult {
@@ -1895,6 +1961,7 @@ int oracle_scf(const OracleRequest* rq, OracleResult* res) {
     if (rq->multiplicity < 1) return fail(ST_INVALID, "multiplicity must be >= 1");
     System s;
     if (!build_system(s, rq->n_atoms, rq->z, rq->xyz_bohr, rq->method)) return fail(s.status, s.err);
+    s.far_cut = rq->multipole_cutoff;
     const ParamSet& ps = *s.ps;
     int nao = s.nao;
     res->nao = nao;
@@ -1992,7 +2059,7 @@ int oracle_scf(const OracleRequest* rq, OracleResult* res) {
             }
         }
         build_fock(s, H, pairs, P, P, 0.5, F);
-        double electronic = 0.5 * (frob(P, H) + frob(P, F));
+        double electronic = 0.5 * (frob(P, H) + frob(P, F)) + far_energy(s, P);
         res->iterations = iterations;
         res->converged = converged;
         res->residual = residual;
@@ -2065,7 +2132,7 @@ int oracle_scf(const OracleRequest* rq, OracleResult* res) {
         for (size_t k = 0; k < Pt.d.size(); ++k) Pt.d[k] = Pa.d[k] + Pb.d[k];
         build_fock(s, H, pairs, Pt, Pa, 1.0, Fa);
         build_fock(s, H, pairs, Pt, Pb, 1.0, Fb);
-        double electronic = 0.5 * (frob(Pt, H) + frob(Pa, Fa) + frob(Pb, Fb));
+        double electronic = 0.5 * (frob(Pt, H) + frob(Pa, Fa) + frob(Pb, Fb)) + far_energy(s, Pt);
         res->iterations = iterations;
         res->converged = converged;
         res->residual = residual;
@@ -2095,6 +2162,7 @@ static double energy_fixed_density(const OracleRequest* rq, const double* xyz, c
                                    bool& ok) {
     System s;
     ok = build_system(s, rq->n_atoms, rq->z, xyz, rq->method);
+    s.far_cut = rq->multipole_cutoff;
     if (!ok) return 0.0;
     Mat H;
     std::vector<Pair> pairs;
@@ -2104,7 +2172,7 @@ static double energy_fixed_density(const OracleRequest* rq, const double* xyz, c
     for (size_t k = 0; k < Pt.d.size(); ++k) Pt.d[k] = Pa.d[k] + Pb.d[k];
     build_fock(s, H, pairs, Pt, Pa, 1.0, Fa);
     build_fock(s, H, pairs, Pt, Pb, 1.0, Fb);
-    return 0.5 * (frob(Pt, H) + frob(Pa, Fa) + frob(Pb, Fb)) + core_core_energy(s) + fnuc;
+    return 0.5 * (frob(Pt, H) + frob(Pa, Fa) + frob(Pb, Fb)) + far_energy(s, Pt) + core_core_energy(s) + fnuc;
 }
 
 // Nuclear gradient (eV/Bohr, [n_atoms][3]) of the converged SCF: the energy at the converged
@@ -2194,6 +2262,7 @@ int oracle_dc(const OracleRequest* rq, const OracleDcOptions* op, OracleDcResult
     if (op->buffer_radius <= 0.0) return fail(ST_INVALID, "divide-and-conquer buffer_radius must be positive");
     System s;
     if (!build_system(s, rq->n_atoms, rq->z, rq->xyz_bohr, rq->method)) return fail(s.status, s.err);
+    s.far_cut = rq->multipole_cutoff;
     const ParamSet& ps = *s.ps;
     Mat H;
     std::vector<Pair> pairs;
@@ -2266,7 +2335,7 @@ int oracle_dc(const OracleRequest* rq, const OracleDcOptions* op, OracleDcResult
         }
         Mat F;
         build_fock(s, H, pairs, P, P, 0.5, F);
-        res->electronic_ev = 0.5 * (frob(P, H) + frob(P, F));
+        res->electronic_ev = 0.5 * (frob(P, H) + frob(P, F)) + far_energy(s, P);
         Ptot = P;
     } else {
         double nt = std::max(na + nb, 1.0);
@@ -2319,7 +2388,7 @@ int oracle_dc(const OracleRequest* rq, const OracleDcOptions* op, OracleDcResult
         Mat Fa, Fb;
         build_fock(s, H, pairs, Pt, Pa, 1.0, Fa);
         build_fock(s, H, pairs, Pt, Pb, 1.0, Fb);
-        res->electronic_ev = 0.5 * (frob(Pt, H) + frob(Pa, Fa) + frob(Pb, Fb));
+        res->electronic_ev = 0.5 * (frob(Pt, H) + frob(Pa, Fa) + frob(Pb, Fb)) + far_energy(s, Pt);
         Ptot = Pt;
     }
     res->iterations = iterations;
@@ -2360,6 +2429,7 @@ int oracle_fill(int n, const double* e, const double* w, double nel, double kt, 
 int oracle_dc_weight_sum(const OracleRequest* rq, const OracleDcOptions* op, double* out) {
     System s;
     if (!build_system(s, rq->n_atoms, rq->z, rq->xyz_bohr, rq->method)) return s.status;
+    s.far_cut = rq->multipole_cutoff;
     std::vector<std::vector<int>> cores;
     if (op->core_of_atom) {
         int nc = 0;

@@ -41,6 +41,7 @@ class _Request(ctypes.Structure):
         ("adiis_switch", ctypes.c_double),
         ("use_diis", ctypes.c_int),
         ("electric_field", ctypes.POINTER(ctypes.c_double)),
+        ("multipole_cutoff", ctypes.c_double),
     ]
 
 
@@ -147,9 +148,10 @@ def _ip(a):
 
 def _make_request(z, xyz_bohr, charge=0.0, multiplicity=1, method="am1", reference="auto",
                   accelerator="adiis_cdiis", max_scf=200, e_tol=1e-8, p_tol=1e-7, adiis_switch=0.1,
-                  use_diis=True, electric_field=None, keep=None):
+                  use_diis=True, electric_field=None, multipole_cutoff=None, keep=None):
     """``electric_field``: three floats in eV per (e*Bohr), the unit of Am1Options::electric_field
-    (scf.rs:141-153), or None."""
+    (scf.rs:141-153), or None. ``multipole_cutoff``: Am1Options::multipole_cutoff (Bohr,
+    scf.rs:130-140), None or a value that is not > 0 for none."""
     z = np.ascontiguousarray(z, dtype=np.int32)
     xyz = np.ascontiguousarray(xyz_bohr, dtype=np.float64).reshape(-1, 3)
     if keep is not None:
@@ -168,6 +170,7 @@ def _make_request(z, xyz_bohr, charge=0.0, multiplicity=1, method="am1", referen
     rq.p_tol = float(p_tol)
     rq.adiis_switch = float(adiis_switch)
     rq.use_diis = 1 if use_diis else 0
+    rq.multipole_cutoff = float(multipole_cutoff) if multipole_cutoff is not None else 0.0
     if electric_field is not None:
         f = np.ascontiguousarray(electric_field, dtype=np.float64).reshape(3)
         if keep is not None:
