@@ -164,9 +164,6 @@ int am1g_dc(am1g_ctx* ctx, const am1g_request* rq, const am1g_dc_options* op, am
     res->converged = 0;
     if (!(op->buffer_radius > 0.0))
         return res->status = am1g_set_error(ctx, AM1G_INVALID_INPUT, "divide-and-conquer buffer_radius must be positive");
-    if (rq->electric_field)
-        return res->status =
-                   am1g_set_error(ctx, AM1G_INVALID_INPUT, "an external electric field is not supported by am1g_dc");
     Prepared prep;
     std::string msg;
     int st = am1g_prepare(ctx, *rq, prep, msg, INT_MAX);
@@ -708,8 +705,9 @@ int am1g_dc(am1g_ctx* ctx, const am1g_request* rq, const am1g_dc_options* op, am
     res->largest_subsystem_aos = largest;
     res->electronic_ev = electronic;
     res->core_ev = ecore;
-    res->total_ev = electronic + ecore;
-    res->heat_of_formation_kcal = (electronic + ecore - eiso + eheat) * kEvToKcal;
+    // the field's nuclear half (divide_conquer.rs:494-500); its electronic half is in H_core
+    res->total_ev = electronic + ecore + md.field_nuc;
+    res->heat_of_formation_kcal = (electronic + ecore + md.field_nuc - eiso + eheat) * kEvToKcal;
     res->fermi_ev = mu[0];
     res->fermi_beta_ev = uhf ? mu[1] : mu[0];
     res->entropy_ev = ts[0] + (uhf ? ts[1] : 0.0);

@@ -62,8 +62,8 @@ typedef struct {
     double adiis_switch;     /* default 0.1 */
     int precision;           /* AM1G_PRECISION_*; FP32 GEMM phase until the DIIS error < 1e-4 */
     const double* electric_field; /* [3] uniform external field in eV per (e*Bohr), or NULL for none
-                                     (Am1Options::electric_field, scf.rs:141-153; am1g_scf and
-                                     am1g_scf_batch; am1g_dc refuses a field) */
+                                     (Am1Options::electric_field, scf.rs:141-153; every SCF
+                                     entry point, am1g_dc included) */
     int bcc;                 /* nonzero: also AM1-BCC charges from this SCF's Mulliken charges into
                                 am1g_result.bcc_charges (am1_bcc_charges, bcc/mod.rs:99-150); AM1
                                 method only, and the charges output must be given */

@@ -2267,6 +2267,7 @@ int oracle_dc(const OracleRequest* rq, const OracleDcOptions* op, OracleDcResult
     Mat H;
     std::vector<Pair> pairs;
     build_core(s, H, pairs);
+    const double fnuc = add_field(s, rq->electric_field, H);  // divide_conquer.rs:494-500
     std::vector<std::vector<int>> cores;
     if (op->core_of_atom) {
         int nc = 0;
@@ -2398,7 +2399,7 @@ int oracle_dc(const OracleRequest* rq, const OracleDcOptions* op, OracleDcResult
     res->entropy_ev = entropy;
     res->gap_ev = gap;
     res->core_ev = core_core_energy(s);
-    res->total_ev = res->electronic_ev + res->core_ev;
+    res->total_ev = res->electronic_ev + res->core_ev + fnuc;
     double eiso = 0.0, eheat = 0.0;
     for (int i = 0; i < s.nat; ++i) {
         eiso += ps.el[s.z[i]].e_isol;

@@ -129,3 +129,15 @@ def test_water2000_full_size_against_oracle_fixture(engine):
     assert np.abs(r["charges"] - g["charges"]).max() <= Q_TOL
     assert abs(r["charges"].sum()) < 1e-8
     assert abs(r["fermi_ev"] - float(g["fermi_ev"])) <= 1e-5
+
+
+def test_external_field(engine):
+    """The field enters the D&C through H_core and its nuclear term (divide_conquer.rs:494-500)."""
+    z, x = M.water_chain(6)
+    kw = dict(core_size=3, buffer_radius=8.0, kt=0.05, e_tol=1e-8, p_tol=1e-7, max_scf=300,
+              electric_field=[0.02, -0.01, 0.03])
+    r = engine.dc(z, x, **kw)
+    o = O.dc(z, x, **kw)
+    assert r["status"] == 0 and o["status"] == 0
+    assert abs(r["total_ev"] - o["total_ev"]) <= 1e-6
+    assert np.abs(np.asarray(r["charges"]) - o["charges"]).max() <= 1e-6

@@ -135,3 +135,18 @@ def test_overfilling_is_an_error_and_empty_or_full_fills():
     assert abs((w * f).sum()) < 1e-10
     f, _, ts = O.fill(e, w, 4.0, 0.1)
     assert abs((w * f).sum() - 4.0) < 1e-10 and abs(ts) < 1e-9
+
+
+def test_full_buffer_with_a_field_reproduces_the_field_scf():
+    # run_divide_conquer puts the field into H_core and adds its nuclear half
+    # (divide_conquer.rs:494-500); the full-buffer identity must then hold with the field on
+    z, x = M.water_chain(4)
+    f = [0.02, -0.01, 0.03]
+    full = O.scf(z, x, e_tol=1e-11, p_tol=1e-10, full=False, electric_field=f)
+    zero = O.scf(z, x, e_tol=1e-11, p_tol=1e-10, full=False)
+    assert abs(full["total_ev"] - zero["total_ev"]) > 1e-3
+    dc = O.dc(z, x, core_size=3, buffer_radius=500.0, kt=0.0, e_tol=1e-11, p_tol=1e-10, max_scf=400,
+              electric_field=f)
+    assert dc["converged"]
+    assert abs(dc["total_ev"] - full["total_ev"]) < 1e-7
+    assert np.abs(dc["charges"] - full["charges"]).max() < 1e-7
