@@ -270,7 +270,7 @@ class Engine:
     def last_kernel_ms(self) -> float:
         return float(self._L.am1g_last_kernel_ms(self._ctx))
 
-    TUNABLES = ("route_large_ao", "jacobi_lds_n", "eig_graphs")
+    TUNABLES = ("route_large_ao", "jacobi_lds_n", "eig_graphs", "serve_large_batch", "tri_max")
 
     def tunables(self) -> dict:
         """The context's device-dependent launch choices (``am1g_get_tunable``)."""

@@ -4,12 +4,15 @@ Measures on the current GPU, through the C ABI, and writes the table the library
 ``am1g_create`` (``tuned.txt`` beside ``libam1g.so``, or the file ``AM1G_TUNE_FILE`` names):
 
 * ``route_large_ao``: the size from which one molecule runs faster on the grid-level kernels
-  (the whole device) than in one workgroup of the batched kernel. Water clusters of 16 to 24
-  molecules (96 to 144 basis functions) are timed both ways; the crossover is the smallest size
+  (the whole device) than in one workgroup of the batched kernel. Water clusters of 16 to 42
+  molecules (96 to 252 basis functions) are timed both ways; the crossover is the smallest size
   from which the grid path wins at every larger size measured.
-* ``jacobi_lds_n``: whether the batched Jacobi solver (129 to 256 basis functions) should keep
-  its matrix in LDS up to 192 (the LDS capacity) or work in global memory; timed on a batch of
-  clusters of 140 to 190 basis functions.
+* ``tri_max``: the largest order the batched kernel diagonalizes by the tridiagonal method
+  (above 128 with the matrix in global memory), Jacobi beyond it; 128 and 256 are timed on a
+  batch of clusters of 144 to 192 basis functions.
+* ``jacobi_lds_n``: whether the batched Jacobi solver (the fallback, and the solver above
+  ``tri_max``) keeps its matrix in LDS up to 192 (the LDS capacity) or works in global memory;
+  timed on the same batch with ``tri_max`` 128.
 * ``eig_graphs``: whether replaying the large eigensolver's tridiagonalization as a HIP graph
   pays; timed on a 1200-order eigensolve.
 
@@ -41,7 +44,7 @@ def _time_scf(eng, mol, reps=3):
 
 
 def tune_route(eng, log):
-    sizes = list(range(16, 25))
+    sizes = list(range(16, 25)) + list(range(26, 43, 2))
     rows = []
     for k in sizes:
         mol = G.small_water_cluster(k, np.random.default_rng(100 + k))
@@ -54,9 +57,10 @@ def tune_route(eng, log):
     route = rows[-1][0]
     for i in range(len(rows)):
         if all(t_grid < t_wg for _, t_wg, t_grid in rows[i:]):
-            # The workgroup path changes solver above _TRI_MAX functions (tridiagonal QL up to
-            # it, Jacobi above), which is where its time jumps: a crossover found between two
-            # measured sizes that bracket that switch is placed at the switch itself.
+            # The workgroup path changes solver above _TRI_MAX functions (the matrix in
+            # registers up to it, in global memory or Jacobi above), which is where its time
+            # jumps: a crossover found between two measured sizes that bracket that switch is
+            # placed at the switch itself.
             lo = rows[i - 1][0] if i > 0 else rows[i][0] - 1
             route = _TRI_MAX if lo <= _TRI_MAX < rows[i][0] else lo
             break
@@ -64,21 +68,44 @@ def tune_route(eng, log):
     return route, rows
 
 
-def tune_jacobi_lds(eng, log):
+def _cluster_batch():
     mols = [G.small_water_cluster(k, np.random.default_rng(200 + k)) for k in (24, 26, 28, 30, 32) for _ in range(8)]
-    batch = [(m.z, m.xyz_bohr) for m in mols]
+    return [(m.z, m.xyz_bohr) for m in mols]
+
+
+def _batch_ms(eng, batch):
+    eng.scf_batch(batch)
+    ts = []
+    for _ in range(2):
+        eng.scf_batch(batch)
+        ts.append(eng.last_kernel_ms())
+    return min(ts)
+
+
+def tune_jacobi_lds(eng, log):
+    batch = _cluster_batch()
     times = {}
+    old_tri = eng.tunables()["tri_max"]
+    eng.set_tunable("tri_max", 128)
     for lds_n in (0, 192):
         eng.set_tunable("jacobi_lds_n", lds_n)
-        eng.scf_batch(batch)
-        ts = []
-        for _ in range(2):
-            eng.scf_batch(batch)
-            ts.append(eng.last_kernel_ms())
-        times[lds_n] = min(ts)
+        times[lds_n] = _batch_ms(eng, batch)
         log(f"jacobi_lds_n {lds_n}: batch of {len(batch)} clusters {times[lds_n]:.1f} ms")
+    eng.set_tunable("tri_max", old_tri)
     best = min(times, key=times.get)
     eng.set_tunable("jacobi_lds_n", best)
+    return best, times
+
+
+def tune_tri_max(eng, log):
+    batch = _cluster_batch()
+    times = {}
+    for tm in (128, 256):
+        eng.set_tunable("tri_max", tm)
+        times[tm] = _batch_ms(eng, batch)
+        log(f"tri_max {tm}: batch of {len(batch)} clusters {times[tm]:.1f} ms")
+    best = min(times, key=times.get)
+    eng.set_tunable("tri_max", best)
     return best, times
 
 
@@ -120,6 +147,7 @@ def autotune(path: str | None = None, device: int = 0, log=print) -> dict:
     os.environ["AM1G_TUNE_FILE"] = ""  # measure from the defaults, not from an earlier table
     eng = am1g.Engine(device)
     try:
+        tri, tt = tune_tri_max(eng, log)
         route, rows = tune_route(eng, log)
         lds_n, jt = tune_jacobi_lds(eng, log)
         graphs, gt = tune_graphs(eng, log)
@@ -130,14 +158,16 @@ def autotune(path: str | None = None, device: int = 0, log=print) -> dict:
              "# route_large_ao: one molecule, best of 3 (ms): AOs workgroup grid"]
     lines += [f"#   {nao} {a:.2f} {b:.2f}" for nao, a, b in rows]
     lines.append(f"route_large_ao {route}")
-    lines.append("# jacobi_lds_n: batched Jacobi on 40 clusters of 144-192 AOs (ms): " +
+    lines.append("# tri_max: batch of 40 clusters of 144-192 AOs (ms): " + ", ".join(f"{k}: {v:.1f}" for k, v in tt.items()))
+    lines.append(f"tri_max {tri}")
+    lines.append("# jacobi_lds_n: batched Jacobi (tri_max 128) on 40 clusters of 144-192 AOs (ms): " +
                  ", ".join(f"{k}: {v:.1f}" for k, v in jt.items()))
     lines.append(f"jacobi_lds_n {lds_n}")
     lines.append("# eig_graphs: 1200-order eigensolve (ms): " + ", ".join(f"{k}: {v:.1f}" for k, v in gt.items()))
     lines.append(f"eig_graphs {graphs}")
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")
-    return {"route_large_ao": route, "jacobi_lds_n": lds_n, "eig_graphs": graphs, "path": path}
+    return {"route_large_ao": route, "tri_max": tri, "jacobi_lds_n": lds_n, "eig_graphs": graphs, "path": path}
 
 
 if __name__ == "__main__":

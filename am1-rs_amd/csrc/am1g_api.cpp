@@ -48,6 +48,8 @@ bool set_tunable(Tunables& t, const std::string& name, double v) {
     if (name == "route_large_ao") t.route_large_ao = (int)v;
     else if (name == "jacobi_lds_n") t.jacobi_lds_n = std::min(192, std::max(0, (int)v));
     else if (name == "eig_graphs") t.eig_graphs = v != 0.0;
+    else if (name == "serve_large_batch") t.serve_large_batch = std::max(1, (int)v);
+    else if (name == "tri_max") t.tri_max = std::min(256, std::max(0, (int)v));
     else return false;
     return true;
 }
@@ -56,6 +58,8 @@ bool get_tunable(const Tunables& t, const std::string& name, double& v) {
     if (name == "route_large_ao") v = t.route_large_ao;
     else if (name == "jacobi_lds_n") v = t.jacobi_lds_n;
     else if (name == "eig_graphs") v = t.eig_graphs;
+    else if (name == "serve_large_batch") v = t.serve_large_batch;
+    else if (name == "tri_max") v = t.tri_max;
     else return false;
     return true;
 }
@@ -514,7 +518,7 @@ static int batch_impl(am1g_ctx* ctx, int n, const am1g_request* rqs, am1g_result
     a.mols = static_cast<MolDesc*>(pools[pMols].p);
     a.n_mols = m;
     a.lds_n = ctx->tun.jacobi_lds_n;
-    a.tri_max = eigen_tri_max();
+    a.tri_max = std::min(eigen_tri_max(), ctx->tun.tri_max);
     a.z = static_cast<int*>(pools[pZ].p);
     a.xyz = static_cast<double*>(pools[pXyz].p);
     a.elems = ctx->elems.as<DevElem>();

@@ -69,6 +69,8 @@ struct Tunables {
     int route_large_ao = kRouteLargeAo;  // single molecules above this many AOs take the grid path
     int jacobi_lds_n = 192;              // batched Jacobi (129-256 AOs) keeps orders up to this in LDS
     int eig_graphs = 1;                  // replay the large eigensolver's tridiagonalization as a graph
+    int serve_large_batch = 2;           // a server lane batches queued grid-path requests from this many
+    int tri_max = 256;                   // batched kernel: tridiagonal method up to this order, Jacobi above
 };
 
 struct am1g_ctx {

@@ -432,7 +432,7 @@ int am1g_dc(am1g_ctx* ctx, const am1g_request* rq, const am1g_dc_options* op, am
     DcArgs d0;
     d0.n_sub = n_sub;
     d0.lds_n = ctx->tun.jacobi_lds_n;
-    d0.tri_max = eigen_tri_max();
+    d0.tri_max = std::min(eigen_tri_max(), ctx->tun.tri_max);
     d0.sub_n = B.sub_n.as<int>();
     d0.sub_ao = B.sub_ao.as<int64_t>();
     d0.sub_mat = B.sub_mat.as<int64_t>();

@@ -11,13 +11,13 @@ constexpr int kMaxAtoms = 256;
 constexpr int kMaxBatchAo = 256;
 // A single molecule with more basis functions than this runs faster on the grid-level kernels
 // (the whole device on one molecule) than in one workgroup. Measured on water clusters, one
-// workgroup against the grid path: 4.9 / 18.6 ms at 102 functions, 6.1 / 25.3 ms at 126,
-// 42.4 / 22.5 ms at 132, 54.6 / 23.6 ms at 144: the workgroup time jumps where its solver
-// changes from the tridiagonal method to Jacobi (above kTriMax = 128). Inside a full batch the
-// workgroup path still gives more molecules per second (it holds 1/256 of the device), so
-// am1g_scf_batch does not route; am1g_scf and the server do, for latency. The context's
-// tunable "route_large_ao" starts at this value and can be re-measured per device.
-constexpr int kRouteLargeAo = 128;
+// workgroup against the grid path: 6.1 / 25.5 ms at 126 functions, 15.8 / 23.1 ms at 132 (the
+// tridiagonal method with the matrix in global memory from 129 on), 28.1 / 31.0 ms at 192,
+// 36.1 / 33.3 ms at 204, 55.0 / 43.5 ms at 252. Inside a full batch the workgroup path still
+// gives more molecules per second (it holds 1/256 of the device), so am1g_scf_batch does not
+// route; am1g_scf and the server do, for latency. The context's tunable "route_large_ao" starts
+// at this value and can be re-measured per device.
+constexpr int kRouteLargeAo = 192;
 // Dynamic LDS of the per-molecule kernel (the static part is the molecule tables).
 constexpr int kLdsBytes = 19200 * 8;
 

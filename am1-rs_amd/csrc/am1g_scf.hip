@@ -526,6 +526,10 @@ __device__ int mol_eigen(int nao, const double* Fuse, double* Ct, double* T0, do
     if (n <= tri_max && n <= kTriMax && n <= lds_n && tri_lds_doubles(n) + n <= kLdsDoubles) {
         // direct method (tridiagonalization); it declines only for unresolved close eigenvalues
         if (wg_eigh_tri(n, Fuse, Ct, eps, T1, lds, kLdsDoubles, s, warm ? bracket : 0.0) == 0) return 0;
+    } else if (n > kTriMax && n <= tri_max && n <= kTriBig) {
+        // the same method with the matrix in global memory (T0: the matrix, then the
+        // eigenvectors of T; T1: the reflectors)
+        if (wg_eigh_tri_g(n, Fuse, Ct, eps, T0, T1, lds, kLdsDoubles, s, warm ? bracket : 0.0) == 0) return 0;
     }
     const double* src = Fuse;
     if (warm) {
@@ -1404,7 +1408,7 @@ hipError_t launch_zero_ints(int* p, int n, hipStream_t stream) {
 
 int eigen_tri_max() {
     const char* v = std::getenv("AM1G_EIGEN_JACOBI");
-    return (v && v[0] == '1') ? 0 : kTriMax;
+    return (v && v[0] == '1') ? 0 : kTriBig;
 }
 
 hipError_t launch_scf_batch(const BatchArgs& a, int precision, int grid, size_t lds_bytes, hipStream_t stream) {

@@ -6,10 +6,12 @@
 // lanes keep admitting newly arrived requests meanwhile, so launches overlap on the device. Each
 // molecule's result is handed out as soon as that molecule finishes (the launch streams results
 // into pinned host memory), not when the whole launch ends. A molecule that am1g_scf would send
-// to the grid-level path (more than kRouteLargeAo basis functions) is never batched: a lane takes
-// it alone, when it reaches the head of the queue, and runs it through am1g_scf, so every
-// request follows the same route, and gives the same result, as it would alone. Requests are
-// copied at submission, so callers may free their buffers at once.
+// to the grid-level path (more than the route_large_ao tunable's basis functions) runs alone
+// through am1g_scf when it reaches the head of the queue, giving the result it would give alone,
+// unless at least serve_large_batch such molecules (each within one workgroup's reach) are queued:
+// then a lane runs all of them in one batched launch, which ends sooner than that many grid-path
+// SCFs one after another. Requests are copied at submission, so callers may free their buffers
+// at once.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -39,7 +41,8 @@ struct Job {
     Clock::time_point submitted, done;
     am1g_result res;
     std::vector<double> charges, mo, mob, bcc;
-    bool large = false;  // runs alone on the grid-level path
+    bool large = false;      // above the grid-path route: runs alone on the grid-level path ...
+    bool batchable = false;  // ... unless enough such requests queue up (within one workgroup's reach)
 };
 
 // Device bytes one request holds while it runs: inside a batched launch, its dense matrices,
@@ -124,36 +127,60 @@ struct am1g_server {
     void lane_loop(int lane) {
         am1g_ctx* ctx = ctxs[lane];
         const size_t budget = (size_t)(op.memory_budget_mb * 1048576.0);
+        // With two lanes or more, the first half of the lanes take only requests that batch
+        // (at most route_large_ao basis functions), so a short request never waits for a lane
+        // that is busy with long grid-path or large-molecule work; the other lanes take both.
+        const bool small_only = op.lanes >= 2 && lane < op.lanes / 2;
+        auto head_of = [&]() -> long long {
+            for (long long q : queue)
+                if (!small_only || !jobs.at(q).large) return q;
+            return -1;
+        };
         std::unique_lock<std::mutex> lk(mu);
         for (;;) {
-            cv_work.wait(lk, [&] { return stop || !queue.empty(); });
+            cv_work.wait(lk, [&] { return stop || head_of() >= 0; });
             if (stop) return;
-            // a grid-level request at the head runs alone, with no batching window
-            if (jobs.at(queue.front()).large) {
-                Job& j = jobs.at(queue.front());
+            const long long head = head_of();
+            // A grid-level request at the head runs alone, with no batching window, unless at
+            // least serve_large_batch of them (of its precision, each within one workgroup's
+            // reach) are queued: then one batched launch takes them all, which finishes sooner
+            // than that many grid-path SCFs one after another.
+            bool large_mode = false;
+            if (jobs.at(head).large) {
+                const Job& h = jobs.at(head);
+                int n_large = 0;
+                if (h.batchable)
+                    for (long long q : queue) {
+                        const Job& x = jobs.at(q);
+                        n_large += x.large && x.batchable && x.rq.precision == h.rq.precision;
+                    }
+                large_mode = h.batchable && n_large >= ctx->tun.serve_large_batch;
+            }
+            if (jobs.at(head).large && !large_mode) {
+                Job& j = jobs.at(head);
                 const bool alone = admitted == 0;
                 if (!alone && admitted + j.bytes > budget) {
                     cv_work.wait_for(lk, std::chrono::milliseconds(2));
                     continue;
                 }
-                run_large(lk, ctx, queue.front());
+                run_large(lk, ctx, head);
                 continue;
             }
-            if (op.max_wait_ms > 0.0) {  // batching window from the oldest queued request
-                const auto until = jobs.at(queue.front()).submitted +
+            if (op.max_wait_ms > 0.0 && !large_mode) {  // batching window from the oldest queued request
+                const auto until = jobs.at(head).submitted +
                                    std::chrono::duration_cast<Clock::duration>(
                                        std::chrono::duration<double, std::milli>(op.max_wait_ms));
                 cv_work.wait_until(lk, until, [&] { return stop || (int)queue.size() >= op.max_batch; });
                 if (stop) return;
-                if (queue.empty() || jobs.at(queue.front()).large) continue;
+                if (head_of() != head) continue;  // taken by another lane meanwhile
             }
             // admit: head request's precision, in arrival order, within the memory budget
             std::vector<long long> take;
             size_t bytes = 0;
-            const int prec = jobs.at(queue.front()).rq.precision;
+            const int prec = jobs.at(head).rq.precision;
             for (auto it = queue.begin(); it != queue.end() && (int)take.size() < op.max_batch;) {
                 Job& j = jobs.at(*it);
-                if (j.rq.precision != prec || j.large) {
+                if (j.rq.precision != prec || j.large != large_mode || (large_mode && !j.batchable)) {
                     ++it;
                     continue;
                 }
@@ -287,6 +314,7 @@ int am1g_serve_submit(am1g_server* sv, const am1g_request* rq, long long* ticket
     int nao = 0;
     for (uint8_t zz : j.z) nao += zz == 1 ? 1 : 4;
     j.large = nao > sv->ctxs[0]->tun.route_large_ao || rq->n_atoms > kMaxAtoms;
+    j.batchable = nao <= kMaxBatchAo && rq->n_atoms <= kMaxAtoms;
     j.bytes = job_bytes(j.rq, j.large);
     j.charges.assign(rq->n_atoms, 0.0);
     if (rq->bcc) j.bcc.assign(rq->n_atoms, 0.0);
@@ -307,7 +335,7 @@ int am1g_serve_submit(am1g_server* sv, const am1g_request* rq, long long* ticket
         sv->st.queued = (long long)sv->queue.size();
         *ticket = t;
     }
-    sv->cv_work.notify_one();
+    sv->cv_work.notify_all();  // lanes differ in what they take, so every idle lane looks
     return AM1G_OK;
 }
 

@@ -269,6 +269,239 @@ __device__ __forceinline__ bool tri_vector(double* __restrict__ z, const double2
     return ok;
 }
 
+// Steps 2 and 3 of the tridiagonal method for T = (d, e) of order n, all in LDS: the norm, the
+// split into unreduced blocks (bs, be: block start and end of each row), the (d_i, e_{i-1}^2)
+// pairs (de) and every eigenvalue into lam (unsorted, block order). cnt: 2 kWg ints of LDS
+// scratch for the shared first pass, used when cnt_ok. Returns ||T|| (max row sum).
+__device__ __forceinline__ double tri_spectrum(int n, double* __restrict__ d, double* __restrict__ e,
+                                               double* __restrict__ lam, int* __restrict__ bs, int* __restrict__ be,
+                                               double2* __restrict__ de, int* __restrict__ cnt, bool cnt_ok,
+                                               const double* __restrict__ eps_out, double bracket, WgScratch& s) {
+    const int t = threadIdx.x, lane = t & 63;
+    // ---- 2. norm, splitting, blocks
+    if (t < 64) {
+        double nrm = 0.0;
+        for (int i = lane; i < n; i += 64) {
+            const double a = fabs(d[i]) + (i > 0 ? fabs(e[i - 1]) : 0.0) + (i + 1 < n ? fabs(e[i]) : 0.0);
+            nrm = fmax(nrm, a);
+        }
+        for (int off = 32; off > 0; off >>= 1) nrm = fmax(nrm, __shfl_xor(nrm, off, 64));
+        // entries below 1e-10 ||T|| are dropped: that moves eigenvalues by less than 1e-10 ||T||,
+        // and it separates the copies of a degenerate eigenvalue that the reduction leaves
+        // coupled only through rounding-level entries
+        const double tol = 1.0e-10 * nrm;
+        for (int i = lane; i < n; i += 64) {
+            double ei = i + 1 < n ? e[i] : 0.0;
+            if (fabs(ei) <= tol) ei = 0.0;
+            e[i] = ei;
+        }
+        for (int i = lane; i < n; i += 64) {  // reads entries the loop above settled (same wavefront)
+            const double ep = i > 0 ? e[i - 1] : 0.0;
+            de[i] = make_double2(d[i], ep * ep);
+        }
+        // Gershgorin bounds of T (valid for every block)
+        double glo = INFINITY, ghi = -INFINITY;
+        for (int i = lane; i < n; i += 64) {
+            const double r = (i > 0 ? fabs(e[i - 1]) : 0.0) + fabs(e[i]);
+            glo = fmin(glo, d[i] - r);
+            ghi = fmax(ghi, d[i] + r);
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            glo = fmin(glo, __shfl_xor(glo, off, 64));
+            ghi = fmax(ghi, __shfl_xor(ghi, off, 64));
+        }
+        // blocks: bs[j] = the last i <= j with e[i-1] = 0 (else 0), a running maximum over 64-entry
+        // chunks; be[j] = the first i > j with e[i-1] = 0 (else n), a running minimum from the end
+        int carry = 0;
+        for (int c = 0; c < n; c += 64) {
+            const int j = c + lane;
+            int v = (j > 0 && j < n && e[j - 1] == 0.0) ? j : 0;
+            for (int off = 1; off < 64; off <<= 1) {
+                const int u = __shfl_up(v, off, 64);
+                if (lane >= off) v = max(v, u);
+            }
+            v = max(v, carry);
+            if (j < n) bs[j] = v;
+            carry = __shfl(v, 63, 64);
+        }
+        const int last_start = carry;
+        carry = n;
+        for (int c = (n - 1) & ~63; c >= 0; c -= 64) {
+            const int j = c + lane;
+            int v = (j + 1 < n && e[j] == 0.0) ? j + 1 : n;
+            for (int off = 1; off < 64; off <<= 1) {
+                const int u = __shfl_down(v, off, 64);
+                if (lane + off < 64) v = min(v, u);
+            }
+            v = min(v, carry);
+            if (j < n) be[j] = v;
+            carry = __shfl(v, 0, 64);
+        }
+        if (lane == 0) {
+            s.bcast[0] = nrm;
+            s.bcast[5] = glo;
+            s.bcast[6] = ghi;
+            s.bcast[7] = last_start == 0 ? 1.0 : 0.0;  // T is one unreduced block
+        }
+    }
+    __syncthreads();
+    const double nrm = s.bcast[0];
+    const double pivmin = 1.0e-280 * fmax(1.0, nrm * nrm);
+
+    // ---- 3. eigenvalues: G threads per eigenvalue, two points each per pass, so each pass
+    // divides the interval by 2G + 1. Up to four wavefronts (one per SIMD): the counts are
+    // issue-bound, and more points per pass cost more issue than the passes they save.
+    // An unreduced T of order n >= 23 starts with one shared pass: each thread counts at two of
+    // 2 kWg + 1 equal divisions of the Gershgorin interval (the counts go to the A area, free
+    // until step 4), and every eigenvalue takes the division that holds it.
+    const bool one_block = s.bcast[7] != 0.0;
+    const bool shared_pass = one_block && cnt_ok;
+    double glo = s.bcast[5], ghi = s.bcast[6];
+    {
+        const double pad = 4.0 * DBL_EPSILON * fmax(fabs(glo), fabs(ghi)) + pivmin;
+        glo -= pad;
+        ghi += pad;
+    }
+    const double sstep = (ghi - glo) / (double)(2 * kWg + 1);
+    if (shared_pass) {
+        int ca, cb;
+        tri_count2(de, 0, n, glo + (double)(2 * t + 1) * sstep, glo + (double)(2 * t + 2) * sstep, ca, cb);
+        cnt[2 * t] = ca;
+        cnt[2 * t + 1] = cb;
+        __syncthreads();
+    }
+    {
+        int G = 1;
+        while (G < 8 && n * (2 * G) <= kWg / 2) G *= 2;
+        const int j = t / G, l = t - j * G;
+        if (j < n) {
+            const int b0 = bs[j], b1 = be[j], kk = j - b0;
+            double lo = d[b0], hi = d[b0];
+            if (b1 - b0 > 1) {
+                lo = glo;
+                hi = ghi;
+                // Bracket state: block-local counts at the ends (clo <= kk < chi) and, once known,
+                // the characteristic polynomial there (f * 2^e).
+                int clo = 0, chi = b1 - b0, elo = 0, ehi = 0;
+                double flo = 0.0, fhi = 0.0;
+                bool klo = false, khi = false;
+                if (shared_pass) {
+                    // cnt[p] is the count at glo + (p + 1) sstep; keep cnt[plo] <= kk < cnt[phi]
+                    int plo = -1, phi = 2 * kWg;
+                    while (phi - plo > 1) {
+                        const int mid = (plo + phi) >> 1;
+                        if (cnt[mid] <= kk) plo = mid;
+                        else phi = mid;
+                    }
+                    if (plo >= 0) {
+                        lo = glo + (double)(plo + 1) * sstep;
+                        clo = cnt[plo];
+                    }
+                    if (phi < 2 * kWg) {
+                        hi = glo + (double)(phi + 1) * sstep;
+                        chi = cnt[phi];
+                    }
+                }
+                // a verified bracket around the previous eigenvalue (one block spanning T, so
+                // the block-local index is the global one) when it is the narrower one: its
+                // evaluation also gives the polynomial at both ends, so regula falsi can start
+                if (bracket > 0.0 && one_block && 2.0 * bracket < hi - lo) {
+                    const double wl = eps_out[j] - bracket, wh = eps_out[j] + bracket;
+                    int cl, ch, el, eh;
+                    double fl, fh;
+                    tri_eval2(de, 0, n, wl, wh, cl, ch, fl, el, fh, eh);
+                    if (cl <= kk && ch > kk) {
+                        if (wl > lo) {
+                            lo = wl;
+                            clo = cl;
+                            flo = fl;
+                            elo = el;
+                            klo = true;
+                        }
+                        if (wh < hi) {
+                            hi = wh;
+                            chi = ch;
+                            fhi = fh;
+                            ehi = eh;
+                            khi = true;
+                        }
+                    }
+                }
+                // Passes: multisection while the bracket holds more than this eigenvalue or an
+                // end's polynomial value is unknown; then regula falsi: the secant point xr of
+                // the polynomial and points within +-eps of it, eps from the error the previous
+                // secant point turned out to have (error ~ K w^2). Counts decide every move, so
+                // the bracket always holds the eigenvalue; a pass that fails to halve it makes
+                // the next one a multisection.
+                const double atol = DBL_EPSILON * nrm;
+                double xr_prev = 0.0, w_prev = 0.0;
+                bool have_prev = false, force = false;
+                for (int it = 0; it < 120; ++it) {
+                    const double w = hi - lo;
+                    if (!(w > fmax(2.0 * DBL_EPSILON * fmax(fabs(lo), fabs(hi)), atol))) break;
+                    const bool rf = chi - clo == 1 && klo && khi && !force;
+                    double xa, xb;
+                    if (rf) {
+                        const double r = ldexp(fhi / flo, max(-1000, min(1000, ehi - elo)));
+                        const double tiny = fmax(1.0e-9 * w, 0.25 * atol);
+                        double xr = lo + w / (1.0 - r);
+                        xr = fmin(fmax(xr, lo + tiny), hi - tiny);
+                        double ep = w * (1.0 / 64.0);
+                        if (have_prev) ep = 8.0 * fabs(xr - xr_prev) * (w / w_prev) * (w / w_prev);
+                        ep = fmax(ep, 0.5 * atol);
+                        const double sp = 2.0 * ep / (double)(2 * G - 1);
+                        xa = fmin(fmax(xr - ep + (double)(2 * l) * sp, lo + tiny), hi - tiny);
+                        xb = fmin(fmax(xr - ep + (double)(2 * l + 1) * sp, lo + tiny), hi - tiny);
+                        xr_prev = xr;
+                        w_prev = w;
+                        have_prev = true;
+                    } else {
+                        const double step = w / (double)(2 * G + 1);
+                        xa = lo + (double)(2 * l + 1) * step;
+                        xb = lo + (double)(2 * l + 2) * step;
+                    }
+                    int ca, cb, ea, eb;
+                    double fa, fb;
+                    tri_eval2(de, b0, b1, xa, xb, ca, cb, fa, ea, fb, eb);
+                    // a point at or below the kk-th eigenvalue raises lo, one above it lowers hi
+                    double nlo = -INFINITY, nhi = INFINITY, nflo = 0.0, nfhi = 0.0;
+                    int nclo = 0, nchi = 0, nelo = 0, nehi = 0;
+                    if (ca <= kk) {
+                        nlo = xa; nclo = ca; nflo = fa; nelo = ea;
+                    } else {
+                        nhi = xa; nchi = ca; nfhi = fa; nehi = ea;
+                    }
+                    if (cb <= kk) {
+                        if (xb > nlo) { nlo = xb; nclo = cb; nflo = fb; nelo = eb; }
+                    } else if (xb < nhi) {
+                        nhi = xb; nchi = cb; nfhi = fb; nehi = eb;
+                    }
+                    for (int off = 1; off < G; off <<= 1) {
+                        const double olo = __shfl_xor(nlo, off, 64), ohi = __shfl_xor(nhi, off, 64);
+                        const double oflo = __shfl_xor(nflo, off, 64), ofhi = __shfl_xor(nfhi, off, 64);
+                        const int oclo = __shfl_xor(nclo, off, 64), ochi = __shfl_xor(nchi, off, 64);
+                        const int oelo = __shfl_xor(nelo, off, 64), oehi = __shfl_xor(nehi, off, 64);
+                        if (olo > nlo) { nlo = olo; nclo = oclo; nflo = oflo; nelo = oelo; }
+                        if (ohi < nhi) { nhi = ohi; nchi = ochi; nfhi = ofhi; nehi = oehi; }
+                    }
+                    if (nlo > lo) {
+                        lo = nlo; clo = nclo; flo = nflo; elo = nelo; klo = true;
+                    }
+                    if (nhi < hi) {
+                        hi = nhi; chi = nchi; fhi = nfhi; ehi = nehi; khi = true;
+                    }
+                    force = rf && hi - lo > 0.5 * w;
+                    if (force) have_prev = false;
+                }
+            }
+            if (l == 0) lam[j] = 0.5 * (lo + hi);
+        }
+    }
+    __syncthreads();
+
+    return nrm;
+}
+
 // ---------------------------------------------------------------------------------------------
 // Eigen-decomposition of the symmetric n x n matrix F (global, row-major; symmetrized on load).
 // On success writes the eigenvalues ascending to eps and the eigenvectors as the rows of Ct (row
@@ -515,227 +748,8 @@ __device__ __noinline__ int wg_eigh_tri(int n, const double* __restrict__ F, dou
     }
     __syncthreads();
 
-    // ---- 2. norm, splitting, blocks
-    if (t < 64) {
-        double nrm = 0.0;
-        for (int i = lane; i < n; i += 64) {
-            const double a = fabs(d[i]) + (i > 0 ? fabs(e[i - 1]) : 0.0) + (i + 1 < n ? fabs(e[i]) : 0.0);
-            nrm = fmax(nrm, a);
-        }
-        for (int off = 32; off > 0; off >>= 1) nrm = fmax(nrm, __shfl_xor(nrm, off, 64));
-        // entries below 1e-10 ||T|| are dropped: that moves eigenvalues by less than 1e-10 ||T||,
-        // and it separates the copies of a degenerate eigenvalue that the reduction leaves
-        // coupled only through rounding-level entries
-        const double tol = 1.0e-10 * nrm;
-        for (int i = lane; i < n; i += 64) {
-            double ei = i + 1 < n ? e[i] : 0.0;
-            if (fabs(ei) <= tol) ei = 0.0;
-            e[i] = ei;
-        }
-        for (int i = lane; i < n; i += 64) {  // reads entries the loop above settled (same wavefront)
-            const double ep = i > 0 ? e[i - 1] : 0.0;
-            de[i] = make_double2(d[i], ep * ep);
-        }
-        // Gershgorin bounds of T (valid for every block)
-        double glo = INFINITY, ghi = -INFINITY;
-        for (int i = lane; i < n; i += 64) {
-            const double r = (i > 0 ? fabs(e[i - 1]) : 0.0) + fabs(e[i]);
-            glo = fmin(glo, d[i] - r);
-            ghi = fmax(ghi, d[i] + r);
-        }
-        for (int off = 32; off > 0; off >>= 1) {
-            glo = fmin(glo, __shfl_xor(glo, off, 64));
-            ghi = fmax(ghi, __shfl_xor(ghi, off, 64));
-        }
-        // blocks: bs[j] = the last i <= j with e[i-1] = 0 (else 0), a running maximum over 64-entry
-        // chunks; be[j] = the first i > j with e[i-1] = 0 (else n), a running minimum from the end
-        int carry = 0;
-        for (int c = 0; c < n; c += 64) {
-            const int j = c + lane;
-            int v = (j > 0 && j < n && e[j - 1] == 0.0) ? j : 0;
-            for (int off = 1; off < 64; off <<= 1) {
-                const int u = __shfl_up(v, off, 64);
-                if (lane >= off) v = max(v, u);
-            }
-            v = max(v, carry);
-            if (j < n) bs[j] = v;
-            carry = __shfl(v, 63, 64);
-        }
-        const int last_start = carry;
-        carry = n;
-        for (int c = (n - 1) & ~63; c >= 0; c -= 64) {
-            const int j = c + lane;
-            int v = (j + 1 < n && e[j] == 0.0) ? j + 1 : n;
-            for (int off = 1; off < 64; off <<= 1) {
-                const int u = __shfl_down(v, off, 64);
-                if (lane + off < 64) v = min(v, u);
-            }
-            v = min(v, carry);
-            if (j < n) be[j] = v;
-            carry = __shfl(v, 0, 64);
-        }
-        if (lane == 0) {
-            s.bcast[0] = nrm;
-            s.bcast[5] = glo;
-            s.bcast[6] = ghi;
-            s.bcast[7] = last_start == 0 ? 1.0 : 0.0;  // T is one unreduced block
-        }
-    }
-    __syncthreads();
-    const double nrm = s.bcast[0];
+    const double nrm = tri_spectrum(n, d, e, lam, bs, be, de, (int*)A, n * ld >= kWg, eps_out, bracket, s);
     const double pivmin = 1.0e-280 * fmax(1.0, nrm * nrm);
-
-    // ---- 3. eigenvalues: G threads per eigenvalue, two points each per pass, so each pass
-    // divides the interval by 2G + 1. Up to four wavefronts (one per SIMD): the counts are
-    // issue-bound, and more points per pass cost more issue than the passes they save.
-    // An unreduced T of order n >= 23 starts with one shared pass: each thread counts at two of
-    // 2 kWg + 1 equal divisions of the Gershgorin interval (the counts go to the A area, free
-    // until step 4), and every eigenvalue takes the division that holds it.
-    const bool one_block = s.bcast[7] != 0.0;
-    const bool shared_pass = one_block && n * ld >= kWg;
-    int* cnt = (int*)A;
-    double glo = s.bcast[5], ghi = s.bcast[6];
-    {
-        const double pad = 4.0 * DBL_EPSILON * fmax(fabs(glo), fabs(ghi)) + pivmin;
-        glo -= pad;
-        ghi += pad;
-    }
-    const double sstep = (ghi - glo) / (double)(2 * kWg + 1);
-    if (shared_pass) {
-        int ca, cb;
-        tri_count2(de, 0, n, glo + (double)(2 * t + 1) * sstep, glo + (double)(2 * t + 2) * sstep, ca, cb);
-        cnt[2 * t] = ca;
-        cnt[2 * t + 1] = cb;
-        __syncthreads();
-    }
-    {
-        int G = 1;
-        while (G < 8 && n * (2 * G) <= kWg / 2) G *= 2;
-        const int j = t / G, l = t - j * G;
-        if (j < n) {
-            const int b0 = bs[j], b1 = be[j], kk = j - b0;
-            double lo = d[b0], hi = d[b0];
-            if (b1 - b0 > 1) {
-                lo = glo;
-                hi = ghi;
-                // Bracket state: block-local counts at the ends (clo <= kk < chi) and, once known,
-                // the characteristic polynomial there (f * 2^e).
-                int clo = 0, chi = b1 - b0, elo = 0, ehi = 0;
-                double flo = 0.0, fhi = 0.0;
-                bool klo = false, khi = false;
-                if (shared_pass) {
-                    // cnt[p] is the count at glo + (p + 1) sstep; keep cnt[plo] <= kk < cnt[phi]
-                    int plo = -1, phi = 2 * kWg;
-                    while (phi - plo > 1) {
-                        const int mid = (plo + phi) >> 1;
-                        if (cnt[mid] <= kk) plo = mid;
-                        else phi = mid;
-                    }
-                    if (plo >= 0) {
-                        lo = glo + (double)(plo + 1) * sstep;
-                        clo = cnt[plo];
-                    }
-                    if (phi < 2 * kWg) {
-                        hi = glo + (double)(phi + 1) * sstep;
-                        chi = cnt[phi];
-                    }
-                }
-                // a verified bracket around the previous eigenvalue (one block spanning T, so
-                // the block-local index is the global one) when it is the narrower one: its
-                // evaluation also gives the polynomial at both ends, so regula falsi can start
-                if (bracket > 0.0 && one_block && 2.0 * bracket < hi - lo) {
-                    const double wl = eps_out[j] - bracket, wh = eps_out[j] + bracket;
-                    int cl, ch, el, eh;
-                    double fl, fh;
-                    tri_eval2(de, 0, n, wl, wh, cl, ch, fl, el, fh, eh);
-                    if (cl <= kk && ch > kk) {
-                        if (wl > lo) {
-                            lo = wl;
-                            clo = cl;
-                            flo = fl;
-                            elo = el;
-                            klo = true;
-                        }
-                        if (wh < hi) {
-                            hi = wh;
-                            chi = ch;
-                            fhi = fh;
-                            ehi = eh;
-                            khi = true;
-                        }
-                    }
-                }
-                // Passes: multisection while the bracket holds more than this eigenvalue or an
-                // end's polynomial value is unknown; then regula falsi: the secant point xr of
-                // the polynomial and points within +-eps of it, eps from the error the previous
-                // secant point turned out to have (error ~ K w^2). Counts decide every move, so
-                // the bracket always holds the eigenvalue; a pass that fails to halve it makes
-                // the next one a multisection.
-                const double atol = DBL_EPSILON * nrm;
-                double xr_prev = 0.0, w_prev = 0.0;
-                bool have_prev = false, force = false;
-                for (int it = 0; it < 120; ++it) {
-                    const double w = hi - lo;
-                    if (!(w > fmax(2.0 * DBL_EPSILON * fmax(fabs(lo), fabs(hi)), atol))) break;
-                    const bool rf = chi - clo == 1 && klo && khi && !force;
-                    double xa, xb;
-                    if (rf) {
-                        const double r = ldexp(fhi / flo, max(-1000, min(1000, ehi - elo)));
-                        const double tiny = fmax(1.0e-9 * w, 0.25 * atol);
-                        double xr = lo + w / (1.0 - r);
-                        xr = fmin(fmax(xr, lo + tiny), hi - tiny);
-                        double ep = w * (1.0 / 64.0);
-                        if (have_prev) ep = 8.0 * fabs(xr - xr_prev) * (w / w_prev) * (w / w_prev);
-                        ep = fmax(ep, 0.5 * atol);
-                        const double sp = 2.0 * ep / (double)(2 * G - 1);
-                        xa = fmin(fmax(xr - ep + (double)(2 * l) * sp, lo + tiny), hi - tiny);
-                        xb = fmin(fmax(xr - ep + (double)(2 * l + 1) * sp, lo + tiny), hi - tiny);
-                        xr_prev = xr;
-                        w_prev = w;
-                        have_prev = true;
-                    } else {
-                        const double step = w / (double)(2 * G + 1);
-                        xa = lo + (double)(2 * l + 1) * step;
-                        xb = lo + (double)(2 * l + 2) * step;
-                    }
-                    int ca, cb, ea, eb;
-                    double fa, fb;
-                    tri_eval2(de, b0, b1, xa, xb, ca, cb, fa, ea, fb, eb);
-                    // a point at or below the kk-th eigenvalue raises lo, one above it lowers hi
-                    double nlo = -INFINITY, nhi = INFINITY, nflo = 0.0, nfhi = 0.0;
-                    int nclo = 0, nchi = 0, nelo = 0, nehi = 0;
-                    if (ca <= kk) {
-                        nlo = xa; nclo = ca; nflo = fa; nelo = ea;
-                    } else {
-                        nhi = xa; nchi = ca; nfhi = fa; nehi = ea;
-                    }
-                    if (cb <= kk) {
-                        if (xb > nlo) { nlo = xb; nclo = cb; nflo = fb; nelo = eb; }
-                    } else if (xb < nhi) {
-                        nhi = xb; nchi = cb; nfhi = fb; nehi = eb;
-                    }
-                    for (int off = 1; off < G; off <<= 1) {
-                        const double olo = __shfl_xor(nlo, off, 64), ohi = __shfl_xor(nhi, off, 64);
-                        const double oflo = __shfl_xor(nflo, off, 64), ofhi = __shfl_xor(nfhi, off, 64);
-                        const int oclo = __shfl_xor(nclo, off, 64), ochi = __shfl_xor(nchi, off, 64);
-                        const int oelo = __shfl_xor(nelo, off, 64), oehi = __shfl_xor(nehi, off, 64);
-                        if (olo > nlo) { nlo = olo; nclo = oclo; nflo = oflo; nelo = oelo; }
-                        if (ohi < nhi) { nhi = ohi; nchi = ochi; nfhi = ofhi; nehi = oehi; }
-                    }
-                    if (nlo > lo) {
-                        lo = nlo; clo = nclo; flo = nflo; elo = nelo; klo = true;
-                    }
-                    if (nhi < hi) {
-                        hi = nhi; chi = nchi; fhi = nfhi; ehi = nehi; khi = true;
-                    }
-                    force = rf && hi - lo > 0.5 * w;
-                    if (force) have_prev = false;
-                }
-            }
-            if (l == 0) lam[j] = 0.5 * (lo + hi);
-        }
-    }
-    __syncthreads();
 
     // ---- 4. eigenvectors of T (row j of the LDS matrix)
     if (t < n) {
@@ -893,6 +907,377 @@ __device__ __noinline__ int wg_eigh_tri(int n, const double* __restrict__ F, dou
                 const int c = 2 * q8 + 16 * j;
                 if (c < n) dst[c] = zr[b][2 * j];
                 if (c + 1 < n) dst[c + 1] = zr[b][2 * j + 1];
+            }
+        }
+    }
+    __syncthreads();
+    return 0;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Orders kTriMax < n <= kTriBig: the same five steps with the matrix in global memory (too large
+// for the registers of one workgroup). A (n x n, row-major, in the caller's scratch) holds the
+// symmetrized matrix during the reduction and then the eigenvectors of T; Vg (n x n) the
+// reflectors. The vectors of the reduction, T, the ranks and the eigenvalue counts live in LDS;
+// the eigenvectors of T are built in LDS a chunk at a time and the reflectors are staged
+// through LDS a block at a time for the back transformation.
+//
+// Step 1 keeps the scheme of the register version: step k builds reflector k from row k, which
+// its owners published in the previous step, and each owner then applies the previous step's
+// rank-2 update to its rows and forms its entry of the new product in the same pass over the
+// rows (one read and one write of the trailing rows per step, one barrier). A group of eight
+// threads owns rows 2g, 2g + 1, 128 + 2g and 129 + 2g; thread q of the group owns columns
+// 2q + 16j + {0, 1}, j = 0..15, so eight consecutive threads cover 128 consecutive bytes of a row.
+constexpr int kTriBig = 256;
+
+// LDS layout (doubles) of the global variant: d, e, tau, lam, two reflectors, two products, two
+// published rows (kTriBig each), block starts and ends and ranks (ints), the (d_i, e_{i-1}^2)
+// pairs, the shared-pass counts (2 kWg ints), then the staging area.
+constexpr int kTgD = 0, kTgE = 256, kTgTau = 512, kTgLam = 768, kTgV = 1024, kTgP = 1536, kTgPub = 2048;
+constexpr int kTgBs = 2560, kTgBe = 2688, kTgRk = 2816, kTgDe = 2944, kTgCnt = 3456;
+constexpr int kTgStage = kTgCnt + kWg;  // 3968
+
+__device__ __noinline__ int wg_eigh_tri_g(int n, const double* __restrict__ F, double* __restrict__ Ct,
+                                          double* __restrict__ eps_out, double* __restrict__ A,
+                                          double* __restrict__ Vg, double* __restrict__ lds, int lds_cap,
+                                          WgScratch& s, double bracket) {
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    double* d = lds + kTgD;
+    double* e = lds + kTgE;
+    double* tau = lds + kTgTau;
+    double* lam = lds + kTgLam;
+    double* vb = lds + kTgV;    // reflectors by parity, indexed by absolute row
+    double* pb = lds + kTgP;    // their products p = tau A v
+    double* pub = lds + kTgPub; // row k of the matrix, published for step k (by parity)
+    int* bs = (int*)(lds + kTgBs);
+    int* be = (int*)(lds + kTgBe);
+    int* rk = (int*)(lds + kTgRk);
+    double2* de = (double2*)(lds + kTgDe);
+    int* cnt = (int*)(lds + kTgCnt);
+    double* stage = lds + kTgStage;
+    const int stage_cap = lds_cap - kTgStage;
+    const int g8 = t >> 3, q8 = t & 7;
+
+    // A = (F + F^T) / 2; row 0 published; vectors cleared
+    for (int idx = t; idx < n * n; idx += kWg) {
+        const int r = idx / n, c = idx - r * n;
+        A[idx] = 0.5 * (F[(size_t)r * n + c] + F[(size_t)c * n + r]);
+    }
+    for (int idx = t; idx < 2 * kTriBig; idx += kWg) {
+        vb[idx] = 0.0;
+        pb[idx] = 0.0;
+    }
+    for (int c = t; c < kTriBig; c += kWg) pub[c] = c < n ? 0.5 * (F[c] + F[(size_t)c * n]) : 0.0;
+    if (t == 0) {
+        s.bcast[2] = 0.0;  // failure flag
+        s.bcast[3] = 0.0;  // close eigenvalues present
+    }
+    __syncthreads();
+
+    // ---- 1. tridiagonalization
+    int par = 0;
+    double tauo = 0.0;
+    for (int k = 0; k + 2 < n; ++k) {
+        const double* vo = vb + (par ^ 1) * kTriBig;
+        const double* po = pb + (par ^ 1) * kTriBig;
+        double* vn = vb + par * kTriBig;
+        double* pn = pb + par * kTriBig;
+        const double* rowk = pub + (k & 1) * kTriBig;
+        double pv = 0.0;
+        for (int i = k + lane; i < n; i += 64) pv = fma(po[i], vo[i], pv);
+        const double Ko = 0.5 * tauo * wave_sum(pv);
+        // column k after the previous update, rows lane + 64 u
+        const double vk = vo[k], wk = fma(-Ko, vk, po[k]);
+        double x[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int r = lane + 64 * u;
+            x[u] = 0.0;
+            if (r >= k && r < n) {
+                const double v = vo[r];
+                x[u] = fma(-v, wk, fma(-fma(-Ko, v, po[r]), vk, rowk[r]));
+            }
+        }
+        const double dk = fma(-vk, wk, fma(-fma(-Ko, vk, po[k]), vk, rowk[k]));
+        const double v1 = vo[k + 1];
+        const double x0 = fma(-v1, wk, fma(-fma(-Ko, v1, po[k + 1]), vk, rowk[k + 1]));
+        double sig = 0.0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (lane + 64 * u >= k + 2) sig = fma(x[u], x[u], sig);
+        sig = wave_sum(sig);
+        double tk = 0.0, beta = x0, scale = 0.0;
+        if (sig > 0.0) {
+            const double nx2 = fma(x0, x0, sig);
+            const double rn = (nx2 > 1.0e-280 && nx2 < 1.0e280) ? fast_rsqrt(nx2) : 1.0 / sqrt(nx2);
+            beta = -copysign(nx2 * rn, x0);
+            tk = fma(fabs(x0), rn, 1.0);
+            const double den = x0 - beta;
+            scale = (fabs(den) > 1.0e-280 && fabs(den) < 1.0e280) ? fast_rcp(den) : 1.0 / den;
+        }
+        // every wavefront writes the same reflector (zero outside its support) and reads back
+        // what it wrote; rows <= k of the new product are cleared the same way
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int r = lane + 64 * u;
+            const double v = r == k + 1 ? 1.0 : (r >= k + 2 && r < n ? x[u] * scale : 0.0);
+            vn[r] = v;
+            if (r <= k) pn[r] = 0.0;
+            if (wv == 0 && r > k && r < n) Vg[(size_t)k * n + r] = v;
+        }
+        if (t == 0) {
+            d[k] = dk;
+            e[k] = beta;
+            tau[k] = tk;
+        }
+        // owned rows k+1..n-1: the previous update and this reflector's product, a half of the
+        // rows (h) and a quarter of the columns (jq) at a time; finished rows and columns (<= k)
+        // are skipped with wavefront-uniform tests
+#pragma unroll 1
+        for (int h = 0; h < 2; ++h) {
+            const int r0 = 2 * g8 + 128 * h, r1 = r0 + 1;
+            if (16 * wv + 128 * h + 15 <= k || 16 * wv + 128 * h >= n) continue;
+            const bool on0 = r0 > k && r0 < n, on1 = r1 > k && r1 < n;
+            const double vr0 = vo[r0 < n ? r0 : 0], wr0 = fma(-Ko, vr0, po[r0 < n ? r0 : 0]);
+            const double vr1 = vo[r1 < n ? r1 : 0], wr1 = fma(-Ko, vr1, po[r1 < n ? r1 : 0]);
+            double* a0 = A + (size_t)(on0 ? r0 : 0) * n;
+            double* a1 = A + (size_t)(on1 ? r1 : 0) * n;
+            const bool pub0 = r0 == k + 1, pub1 = r1 == k + 1;
+            double* nxt = pub + ((k + 1) & 1) * kTriBig;
+            double s0 = 0.0, s1 = 0.0;
+#pragma unroll 1
+            for (int jq = 0; jq < 4; ++jq) {
+                if (64 * jq + 63 <= k || 64 * jq >= n) continue;
+                double av0[8], av1[8], vv[8], pp[8], uu[8];
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const int c = 2 * q8 + 16 * (4 * jq + (i >> 1)) + (i & 1);
+                    const bool cin = c < n;
+                    av0[i] = (on0 && cin) ? a0[c] : 0.0;
+                    av1[i] = (on1 && cin) ? a1[c] : 0.0;
+                    vv[i] = vo[c];
+                    pp[i] = po[c];
+                    uu[i] = vn[c];
+                }
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const int c = 2 * q8 + 16 * (4 * jq + (i >> 1)) + (i & 1);
+                    const double w = fma(-Ko, vv[i], pp[i]);
+                    const double b0 = fma(-vr0, w, fma(-wr0, vv[i], av0[i]));
+                    const double b1 = fma(-vr1, w, fma(-wr1, vv[i], av1[i]));
+                    s0 = fma(b0, uu[i], s0);
+                    s1 = fma(b1, uu[i], s1);
+                    if (c < n) {
+                        if (on0) a0[c] = b0;
+                        if (on1) a1[c] = b1;
+                        if (pub0) nxt[c] = b0;
+                        if (pub1) nxt[c] = b1;
+                    }
+                }
+            }
+            s0 += dpp_move<0xB1>(s0);  // sums over the group of eight
+            s1 += dpp_move<0xB1>(s1);
+            s0 += dpp_move<0x4E>(s0);
+            s1 += dpp_move<0x4E>(s1);
+            s0 += dpp_move<0x141>(s0);
+            s1 += dpp_move<0x141>(s1);
+            if (q8 == 0) {
+                if (on0) pn[r0] = tk * s0;
+                if (on1) pn[r1] = tk * s1;
+            }
+        }
+        tauo = tk;
+        par ^= 1;
+        __syncthreads();
+    }
+    // the last 2 x 2 block (rows and columns n-2, n-1) after the last reflector's update
+    if (t < 4) {
+        const double* vo = vb + (par ^ 1) * kTriBig;
+        const double* po = pb + (par ^ 1) * kTriBig;
+        double pv = 0.0;
+        for (int i = n - 2; i < n; ++i) pv = fma(po[i], vo[i], pv);
+        const double Ko = 0.5 * tauo * pv;
+        const int r = n - 2 + (t >> 1), c = n - 2 + (t & 1);
+        const double vr = vo[r], wr = fma(-Ko, vr, po[r]);
+        s.bcast[8 + t] = fma(-vr, fma(-Ko, vo[c], po[c]), fma(-wr, vo[c], A[(size_t)r * n + c]));
+    }
+    __syncthreads();
+    if (t == 0) {
+        d[n - 2] = s.bcast[8];
+        e[n - 2] = s.bcast[10];  // row n-1, column n-2
+        d[n - 1] = s.bcast[11];
+        e[n - 1] = 0.0;
+    }
+    __syncthreads();
+
+    // ---- 2, 3. split and eigenvalues
+    const double nrm = tri_spectrum(n, d, e, lam, bs, be, de, cnt, true, eps_out, bracket, s);
+    const double pivmin = 1.0e-280 * fmax(1.0, nrm * nrm);
+
+    // ---- 4. eigenvectors of T, built in LDS a chunk of rows at a time, then rows of A
+    const int ld = n | 1;
+    const int chunk = min(kWg, stage_cap / ld);
+    for (int j0 = 0; j0 < n; j0 += chunk) {
+        const int j = j0 + t;
+        if (t < chunk && j < n) {
+            double* z = stage + t * ld;
+            const int b0 = bs[j], b1 = be[j];
+            for (int i = 0; i < b0; ++i) z[i] = 0.0;
+            for (int i = b1; i < n; ++i) z[i] = 0.0;
+            bool ok = true;
+            if (b1 - b0 == 1) {
+                z[b0] = 1.0;
+            } else {
+                int q = 0;
+                while (q < 3 && j - q - 1 >= b0 && lam[j - q] - lam[j - q - 1] < 1.0e-6 * nrm) ++q;
+                ok = tri_vector(z, de, e, b0, b1, lam[j], q, pivmin);
+            }
+            if (!ok) s.bcast[2] = 1.0;
+        }
+        __syncthreads();
+        const int rows = min(chunk, n - j0);
+        for (int idx = t; idx < rows * n; idx += kWg) {
+            const int r = idx / n, c = idx - r * n;
+            A[(size_t)(j0 + r) * n + c] = stage[r * ld + c];
+        }
+        __syncthreads();
+    }
+    // close eigenvalues inside one block: Gram-Schmidt over each run (one wavefront; rare)
+    if (t + 1 < n && bs[t + 1] == bs[t] && lam[t + 1] - lam[t] < 1.0e-6 * nrm) s.bcast[3] = 1.0;
+    __syncthreads();
+    if (s.bcast[3] != 0.0 && t < 64) {
+        const double gap = 1.0e-6 * nrm;
+        for (int j0 = 0; j0 < n;) {
+            int j1 = j0 + 1;
+            while (j1 < n && bs[j1] == bs[j0] && lam[j1] - lam[j1 - 1] < gap) ++j1;
+            if (j1 - j0 > 1) {
+                const int b0 = bs[j0], b1 = be[j0];
+                for (int a = j0 + 1; a < j1; ++a) {
+                    double* za = A + (size_t)a * n;
+                    for (int q = j0; q < a; ++q) {
+                        const double* zq = A + (size_t)q * n;
+                        double dot = 0.0;
+                        for (int i = b0 + lane; i < b1; i += 64) dot = fma(zq[i], za[i], dot);
+                        for (int off = 32; off > 0; off >>= 1) dot += __shfl_xor(dot, off, 64);
+                        for (int i = b0 + lane; i < b1; i += 64) za[i] = fma(-dot, zq[i], za[i]);
+                    }
+                    double ss = 0.0;
+                    for (int i = b0 + lane; i < b1; i += 64) ss = fma(za[i], za[i], ss);
+                    for (int off = 32; off > 0; off >>= 1) ss += __shfl_xor(ss, off, 64);
+                    if (!(ss > 1.0e-6)) {
+                        if (lane == 0) s.bcast[2] = 1.0;
+                        ss = 1.0;
+                    }
+                    const double inv = 1.0 / sqrt(ss);
+                    for (int i = b0 + lane; i < b1; i += 64) za[i] *= inv;
+                }
+            }
+            j0 = j1;
+        }
+    }
+    __syncthreads();
+    const bool failed = s.bcast[2] != 0.0;
+    __syncthreads();
+    if (failed) return -1;
+
+    // ---- 5. ranks (ascending, stable), eigenvalues out
+    for (int i = t / 16; i < n; i += kWg / 16) {
+        const double ei = lam[i];
+        int rank = 0;
+        for (int j = t % 16; j < n; j += 16) rank += (lam[j] < ei) || (lam[j] == ei && j < i);
+        for (int off = 8; off > 0; off >>= 1) rank += __shfl_xor(rank, off, 16);
+        if (t % 16 == 0) {
+            eps_out[rank] = ei;
+            rk[i] = rank;
+        }
+    }
+    __syncthreads();
+    // back transformation x^T = z^T H_{n-3} ... H_0, a half of the rows at a time (rows 2g,
+    // 2g + 1 of the half in the owners' registers); the reflectors go through LDS in blocks,
+    // the last ones first (even stride sr, zero outside each reflector's support)
+    const int sr = (n + 1) & ~1;
+    const int blk = max(1, min(n - 2, stage_cap / sr));
+#pragma unroll 1
+    for (int h = 0; h < 2; ++h) {
+        const int r0 = 2 * g8 + 128 * h, r1 = r0 + 1;
+        double z0[32], z1[32];
+#pragma unroll
+        for (int i = 0; i < 32; ++i) {
+            const int c = 2 * q8 + 16 * (i >> 1) + (i & 1);
+            z0[i] = (r0 < n && c < n) ? A[(size_t)r0 * n + c] : 0.0;
+            z1[i] = (r1 < n && c < n) ? A[(size_t)r1 * n + c] : 0.0;
+        }
+        const bool active = 16 * wv + 128 * h < n;
+#pragma unroll 1
+        for (int kb = n - 3; kb >= 0; kb -= blk) {
+            const int k_lo = max(0, kb - blk + 1);
+            __syncthreads();  // the previous block is consumed
+            for (int idx = t; idx < (kb - k_lo + 1) * sr; idx += kWg) {
+                const int kk = idx / sr, c = idx - kk * sr, k = k_lo + kk;
+                stage[idx] = (c > k && c < n) ? Vg[(size_t)k * n + c] : 0.0;
+            }
+            __syncthreads();
+            if (!active) continue;
+#pragma unroll 1
+            for (int k = kb; k >= k_lo; --k) {
+                const double tk = tau[k];
+                const double* v = stage + (k - k_lo) * sr;
+                double s0 = 0.0, s1 = 0.0, u0 = 0.0, u1 = 0.0;
+#pragma unroll
+                for (int jh = 0; jh < 2; ++jh) {  // columns 0..127, then 128..255
+                    if (128 * jh + 127 <= k || 128 * jh >= n) continue;
+                    double2 vv[8];
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        const int c = 2 * q8 + 16 * (8 * jh + j);
+                        vv[j] = *(const double2*)(v + min(c, sr - 2));
+                        if (c >= n) vv[j] = make_double2(0.0, 0.0);
+                    }
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        const int i = 2 * (8 * jh + j);
+                        double& a0 = (j & 1) ? u0 : s0;
+                        double& a1 = (j & 1) ? u1 : s1;
+                        a0 = fma(z0[i], vv[j].x, fma(z0[i + 1], vv[j].y, a0));
+                        a1 = fma(z1[i], vv[j].x, fma(z1[i + 1], vv[j].y, a1));
+                    }
+                }
+                s0 += u0;
+                s1 += u1;
+                s0 += dpp_move<0xB1>(s0);
+                s1 += dpp_move<0xB1>(s1);
+                s0 += dpp_move<0x4E>(s0);
+                s1 += dpp_move<0x4E>(s1);
+                s0 += dpp_move<0x141>(s0);
+                s1 += dpp_move<0x141>(s1);
+                const double f0 = tk * s0, f1 = tk * s1;
+#pragma unroll
+                for (int jh = 0; jh < 2; ++jh) {
+                    if (128 * jh + 127 <= k || 128 * jh >= n) continue;
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        const int c = 2 * q8 + 16 * (8 * jh + j);
+                        double2 vj = *(const double2*)(v + min(c, sr - 2));
+                        if (c >= n) vj = make_double2(0.0, 0.0);
+                        const int i = 2 * (8 * jh + j);
+                        z0[i] = fma(-f0, vj.x, z0[i]);
+                        z0[i + 1] = fma(-f0, vj.y, z0[i + 1]);
+                        z1[i] = fma(-f1, vj.x, z1[i]);
+                        z1[i + 1] = fma(-f1, vj.y, z1[i + 1]);
+                    }
+                }
+            }
+        }
+        __syncthreads();  // every row of this half is read before any result row is written
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const int r = b ? r1 : r0;
+            if (r >= n) continue;
+            double* dst = Ct + (size_t)rk[r] * n;
+#pragma unroll
+            for (int i = 0; i < 32; ++i) {
+                const int c = 2 * q8 + 16 * (i >> 1) + (i & 1);
+                if (c < n) dst[c] = b ? z1[i] : z0[i];
             }
         }
     }

@@ -130,11 +130,12 @@ def test_results_stream_out_before_the_launch_ends():
 
 
 def test_large_requests_run_alone_on_the_grid_path(engine):
-    """Water clusters above 128 basis functions are never batched: each runs alone through the
-    plain entry point's grid-level route, so its result is the one ``Engine.scf`` gives, while
-    the chains submitted around it are batched as usual."""
+    """A water cluster above the grid-path route (192 basis functions by the committed table)
+    runs alone through the plain entry point's grid-level route when it is the only such request
+    of its precision in the queue, so its result is the one ``Engine.scf`` gives, while the chains
+    submitted around it are batched as usual."""
     rng = np.random.default_rng(21)
-    bigs = [G.small_water_cluster(k, rng) for k in (24, 30)]
+    bigs = [G.small_water_cluster(k, rng) for k in (34, 40)]
     small = G.chain_batch(30, seed=22)
     sv = am1g.Server(0, lanes=2, max_batch=16)
     try:
@@ -161,3 +162,28 @@ def test_large_requests_run_alone_on_the_grid_path(engine):
     alone = engine.scf_batch([(m.z, m.xyz_bohr) for m in small])
     for a, b in zip(rs, alone):
         assert a["total_ev"] == b["total_ev"]
+
+
+def test_many_large_requests_share_one_launch(engine):
+    """When at least serve_large_batch (default 8) grid-path requests are queued, a lane runs them
+    in one batched launch: fewer launches than requests, every result within the contract of the
+    oracle and bit-identical to what that path gives the molecule alone (the grid path or the
+    batched kernel, whichever ran it)."""
+    rng = np.random.default_rng(31)
+    bigs = [G.small_water_cluster(int(k), rng) for k in rng.integers(33, 42, size=12)]
+    sv = am1g.Server(0, lanes=2, max_batch=64)
+    try:
+        ts = [sv.submit(m.z, m.xyz_bohr) for m in bigs]
+        rs = [sv.poll(t, wait_ms=-1) for t in ts]
+        stats = sv.stats()
+    finally:
+        sv.close()
+    assert all(r["status"] == 0 for r in rs)
+    assert stats["completed"] == 12 and stats["launches"] < 12, stats
+    for m, r in zip(bigs, rs):
+        o = O.scf(m.z, m.xyz_bohr, full=False)
+        assert abs(r["total_ev"] - o["total_ev"]) <= 1e-6
+        assert np.abs(r["charges"] - o["charges"]).max() <= 1e-6
+        grid = engine.scf(m.z, m.xyz_bohr, full=False)
+        batched = engine.scf_batch([(m.z, m.xyz_bohr)])[0]
+        assert r["total_ev"] in (grid["total_ev"], batched["total_ev"])
