@@ -270,7 +270,7 @@ class Engine:
     def last_kernel_ms(self) -> float:
         return float(self._L.am1g_last_kernel_ms(self._ctx))
 
-    TUNABLES = ("route_large_ao", "jacobi_lds_n", "eig_graphs", "serve_large_batch", "tri_max")
+    TUNABLES = ("route_large_ao", "jacobi_lds_n", "eig_graphs", "serve_large_batch", "tri_max", "pairs_block")
 
     def tunables(self) -> dict:
         """The context's device-dependent launch choices (``am1g_get_tunable``)."""
@@ -433,11 +433,12 @@ class Engine:
             d["message"] = self.last_error()
         return d
 
-    def core_hamiltonian(self, z, xyz_bohr, **kw) -> dict:
+    def core_hamiltonian(self, z, xyz_bohr, with_h: bool = True, with_pairs: bool = True, **kw) -> dict:
         """The device's core Hamiltonian (``am1g_core_hamiltonian``): ``h_core`` [nao][nao],
         ``core_ev`` and ``pairs``, a dict {(a, b): w} of every pair's two-electron block with atom
         indices in input order, a the heavy atom (or the lower index of two like atoms) and w of
-        shape (10, 10), (10, 1) or (1, 1)."""
+        shape (10, 10), (10, 1) or (1, 1). ``with_h`` / ``with_pairs`` False skip copying those
+        out (``h_core`` / ``pairs`` are then None)."""
         keep = []
         rq = self._request(z, xyz_bohr, keep, **kw)
         zz = np.asarray(z, dtype=np.int64).ravel()
@@ -446,14 +447,17 @@ class Engine:
         nh, nH = len(heavy), len(hyd)
         nao = 4 * nh + nH
         w_len = 100 * nh * (nh - 1) // 2 + 10 * nh * nH + nH * (nH - 1) // 2
-        H = np.zeros((nao, nao))
-        w = np.zeros(max(w_len, 1))
+        H = np.zeros((nao, nao)) if with_h else None
+        w = np.zeros(max(w_len, 1)) if with_pairs else None
         ec = ctypes.c_double(0.0)
         dp = ctypes.POINTER(ctypes.c_double)
-        st = self._L.am1g_core_hamiltonian(self._ctx, ctypes.byref(rq), H.ctypes.data_as(dp), ctypes.byref(ec),
-                                           w.ctypes.data_as(dp), len(w))
+        st = self._L.am1g_core_hamiltonian(self._ctx, ctypes.byref(rq), H.ctypes.data_as(dp) if with_h else None,
+                                           ctypes.byref(ec), w.ctypes.data_as(dp) if with_pairs else None,
+                                           len(w) if with_pairs else 0)
         if st != 0:
             raise ValueError(f"{STATUS_NAMES.get(st, st)}: {self.last_error()}")
+        if not with_pairs:
+            return {"h_core": H, "core_ev": ec.value, "pairs": None}
         pairs = {}
         k = 0
         for i in range(nh):

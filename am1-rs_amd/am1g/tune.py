@@ -15,6 +15,9 @@ Measures on the current GPU, through the C ABI, and writes the table the library
   timed on the same batch with ``tri_max`` 128.
 * ``eig_graphs``: whether replaying the large eigensolver's tridiagonalization as a HIP graph
   pays; timed on a 1200-order eigensolve.
+* ``pairs_block``: the workgroup size of the pair-integral kernel (64, 128 or 256 threads);
+  timed by the core-Hamiltonian build of 2000 waters (18 M atom pairs), which that kernel
+  dominates.
 
 Usage: ``python -m am1g.tune [path]``. Each choice is a size- or machine-keyed constant; the
 measured times are written as comments beside it.
@@ -140,6 +143,24 @@ def _device_name(device: int) -> str:
         return f"device {device} (name unavailable: {type(exc).__name__})"
 
 
+def tune_pairs_block(eng, log):
+    m = G.water_cluster_config(3)
+    times = {}
+    for b in (64, 128, 256):
+        eng.set_tunable("pairs_block", b)
+        eng.core_hamiltonian(m.z, m.xyz_bohr, with_h=False, with_pairs=False)
+        ts = []
+        for _ in range(3):
+            t = time.perf_counter()
+            eng.core_hamiltonian(m.z, m.xyz_bohr, with_h=False, with_pairs=False)
+            ts.append((time.perf_counter() - t) * 1e3)
+        times[b] = min(ts)
+        log(f"pairs_block {b}: core Hamiltonian of 2000 waters {times[b]:.1f} ms")
+    best = min(times, key=times.get)
+    eng.set_tunable("pairs_block", best)
+    return best, times
+
+
 def autotune(path: str | None = None, device: int = 0, log=print) -> dict:
     """Measure every tunable on `device` and write the table to `path` (default: tuned.txt
     beside the library). Returns the chosen values."""
@@ -151,6 +172,7 @@ def autotune(path: str | None = None, device: int = 0, log=print) -> dict:
         route, rows = tune_route(eng, log)
         lds_n, jt = tune_jacobi_lds(eng, log)
         graphs, gt = tune_graphs(eng, log)
+        pblock, pt = tune_pairs_block(eng, log)
     finally:
         eng.close()
     name = _device_name(device)
@@ -165,9 +187,11 @@ def autotune(path: str | None = None, device: int = 0, log=print) -> dict:
     lines.append(f"jacobi_lds_n {lds_n}")
     lines.append("# eig_graphs: 1200-order eigensolve (ms): " + ", ".join(f"{k}: {v:.1f}" for k, v in gt.items()))
     lines.append(f"eig_graphs {graphs}")
+    lines.append("# pairs_block: core Hamiltonian of 2000 waters (ms): " + ", ".join(f"{k}: {v:.1f}" for k, v in pt.items()))
+    lines.append(f"pairs_block {pblock}")
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")
-    return {"route_large_ao": route, "tri_max": tri, "jacobi_lds_n": lds_n, "eig_graphs": graphs, "path": path}
+    return {"route_large_ao": route, "tri_max": tri, "jacobi_lds_n": lds_n, "eig_graphs": graphs, "pairs_block": pblock, "path": path}
 
 
 if __name__ == "__main__":

@@ -50,6 +50,7 @@ bool set_tunable(Tunables& t, const std::string& name, double v) {
     else if (name == "eig_graphs") t.eig_graphs = v != 0.0;
     else if (name == "serve_large_batch") t.serve_large_batch = std::max(1, (int)v);
     else if (name == "tri_max") t.tri_max = std::min(256, std::max(0, (int)v));
+    else if (name == "pairs_block") t.pairs_block = (v == 64.0 || v == 128.0) ? (int)v : 256;
     else return false;
     return true;
 }
@@ -60,6 +61,7 @@ bool get_tunable(const Tunables& t, const std::string& name, double& v) {
     else if (name == "eig_graphs") v = t.eig_graphs;
     else if (name == "serve_large_batch") v = t.serve_large_batch;
     else if (name == "tri_max") v = t.tri_max;
+    else if (name == "pairs_block") v = t.pairs_block;
     else return false;
     return true;
 }
@@ -555,7 +557,7 @@ static int batch_impl(am1g_ctx* ctx, int n, const am1g_request* rqs, am1g_result
         a.mo_eb_h = ctx->h_mob.as<double>();
     }
     HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
-    HIP_TRY(ctx, launch_pairs(a, st));
+    HIP_TRY(ctx, launch_pairs(a, st, ctx->tun.pairs_block));
     HIP_TRY(ctx, launch_scf_batch(a, precision, grid, lds, st));
     HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
     if (streaming) {

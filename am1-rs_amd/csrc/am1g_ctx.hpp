@@ -71,6 +71,7 @@ struct Tunables {
     int eig_graphs = 1;                  // replay the large eigensolver's tridiagonalization as a graph
     int serve_large_batch = 2;           // a server lane batches queued grid-path requests from this many
     int tri_max = 256;                   // batched kernel: tridiagonal method up to this order, Jacobi above
+    int pairs_block = 256;               // threads per workgroup of the pair-integral kernel
 };
 
 struct am1g_ctx {

@@ -69,7 +69,8 @@ hipError_t launch_zero_ints(int* p, int n, hipStream_t stream);
 // Largest order for the tridiagonal eigensolver: kTriMax, or 0 when AM1G_EIGEN_JACOBI=1 (every
 // eigen-step then takes the Jacobi solver; for testing that solver on the same molecules).
 int eigen_tri_max();
-hipError_t launch_pairs(const BatchArgs& a, hipStream_t stream);
+// block: threads per workgroup of k_pairs (64, 128 or 256; the context tunable "pairs_block")
+hipError_t launch_pairs(const BatchArgs& a, hipStream_t stream, int block = 256);
 void build_element_tables(DevElem* tab);
 
 }  // namespace am1g

@@ -1354,11 +1354,12 @@ template __global__ void k_scf_batch<1>(BatchArgs);
 }  // namespace am1g
 
 namespace am1g {
-hipError_t launch_pairs(const BatchArgs& a, hipStream_t stream) {
+hipError_t launch_pairs(const BatchArgs& a, hipStream_t stream, int block) {
     if (a.n_pairs == 0) return hipSuccess;
-    int64_t blocks = (a.n_pairs + 255) / 256;
+    block = block == 64 || block == 128 ? block : 256;  // the kernel's grid-stride loop takes any
+    int64_t blocks = (a.n_pairs + block - 1) / block;
     if (blocks > 65536) blocks = 65536;
-    hipLaunchKernelGGL(k_pairs, dim3((unsigned)blocks), dim3(256), 0, stream, a);
+    hipLaunchKernelGGL(k_pairs, dim3((unsigned)blocks), dim3(block), 0, stream, a);
     return hipGetLastError();
 }
 

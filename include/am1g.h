@@ -221,11 +221,20 @@ int am1g_symmetric_eigen(am1g_ctx* ctx, int n, const double* a, double* eigenval
 
 /* Device-dependent launch choices of a context, measured by the project's autotuner (am1g.tune)
  * and read at am1g_create from tuned.txt beside the library (or the file AM1G_TUNE_FILE names):
- *   route_large_ao  single molecules above this many basis functions run on the grid-level path
- *                   (am1g_scf, am1g_gradient, the server); default 128
- *   jacobi_lds_n    the batched Jacobi solver (129-256 basis functions) works in LDS up to this
- *                   order, in global memory above it; default 192 (at most 192)
- *   eig_graphs      1: the large eigensolver's tridiagonalization replays as a HIP graph; default 1
+ *   route_large_ao     single molecules above this many basis functions run on the grid-level
+ *                      path (am1g_scf, am1g_gradient, the server); default 192
+ *   tri_max            the batched kernel diagonalizes orders up to this by the tridiagonal
+ *                      method (above 128 with the matrix in global memory), larger ones by
+ *                      Jacobi; default 256 (at most 256)
+ *   jacobi_lds_n       the batched Jacobi solver works in LDS up to this order, in global memory
+ *                      above it; default 192 (at most 192)
+ *   eig_graphs         1: the large eigensolver's tridiagonalization replays as a HIP graph;
+ *                      default 1
+ *   serve_large_batch  a server lane runs the queued grid-path requests (within one
+ *                      workgroup's reach) in one batched launch once this many are queued;
+ *                      default 2
+ *   pairs_block        threads per workgroup of the pair-integral kernel (64, 128 or 256);
+ *                      default 256
  * No reference counterpart. AM1G_INVALID_INPUT for an unknown name. */
 int am1g_set_tunable(am1g_ctx* ctx, const char* name, double value);
 int am1g_get_tunable(const am1g_ctx* ctx, const char* name, double* value);
