@@ -480,10 +480,10 @@ int am1g_dc(am1g_ctx* ctx, const am1g_request* rq, const am1g_dc_options* op, am
     auto fock_energy = [&]() -> int {
         if (uhf) {
             HIP_TRY(ctx, launch_add((int64_t)nn, P[0], P[1], Pt, s));
-            HIP_TRY(ctx, launch_fock(sa, Pt, P[0], 1.0, F[0], s, partial, scal + 3));
-            HIP_TRY(ctx, launch_fock(sa, Pt, P[1], 1.0, F[1], s, partial, scal + 4));
+            HIP_TRY(ctx, launch_fock(sa, Pt, P[0], 1.0, F[0], s, partial, scal + 3, false));
+            HIP_TRY(ctx, launch_fock(sa, Pt, P[1], 1.0, F[1], s, partial, scal + 4, false));
         } else {
-            HIP_TRY(ctx, launch_fock(sa, P[0], P[0], 0.5, F[0], s, partial, scal + 0));
+            HIP_TRY(ctx, launch_fock(sa, P[0], P[0], 0.5, F[0], s, partial, scal + 0, false));
         }
         return AM1G_OK;
     };

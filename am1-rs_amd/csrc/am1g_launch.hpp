@@ -126,8 +126,10 @@ hipError_t launch_hcore_diag(const SysArgs& a, hipStream_t st);
 // Fock matrix F of the total density Pt and spin density Ps (exchange scaled by `scale`). With
 // e_out given, also writes sum Ps (H + F) there, from per-workgroup partials in epart
 // (2 * fock_energy_parts doubles).
+// full = false leaves F's lower triangle unwritten (off-diagonal atom blocks are written in the
+// upper triangle only; a reader then takes F[min(i,j)][max(i,j)]).
 hipError_t launch_fock(const SysArgs& a, const double* Pt, const double* Ps, double scale, double* F, hipStream_t st,
-                       double* epart = nullptr, double* e_out = nullptr);
+                       double* epart = nullptr, double* e_out = nullptr, bool full = true);
 inline int64_t fock_energy_parts(const MolDesc& md) {
     return 2 * (int64_t)md.n_heavy + (md.n_h + 7) / 8 + md.n_h;
 }

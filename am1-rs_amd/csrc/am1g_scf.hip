@@ -1967,9 +1967,15 @@ __global__ __launch_bounds__(kWg) void k_dc_eigen(DcArgs d, const double* F, int
     const int* ao = d.ao_list + d.sub_ao[sub];
     double* Fs = d.fbuf + d.sub_mat[sub];
     double* Ct = d.cbuf + d.sub_mat[sub];
+    // F holds its upper triangle (and the diagonal atom blocks): the D&C Fock build skips the
+    // copy to the lower triangle. The subsystem's AO list ascends, so i <= j reads the upper
+    // triangle (row ao[i], consecutive threads along it) and fills both mirror entries.
     for (int idx = t; idx < n * n; idx += kWg) {
         const int i = idx / n, j = idx - i * n;
-        Fs[idx] = F[(size_t)ao[i] * nao + ao[j]];
+        if (j < i) continue;
+        const double v = F[(size_t)ao[i] * nao + ao[j]];
+        Fs[(size_t)i * n + j] = v;
+        Fs[(size_t)j * n + i] = v;
     }
     __syncthreads();
     const int sw = mol_eigen<double>(n, Fs, Ct, d.t0 + d.sub_mat[sub], d.t1 + d.sub_mat[sub], lds_dyn, d.lds_n, d.tri_max,
@@ -2420,7 +2426,7 @@ __global__ __launch_bounds__(256) void k_far_energy(int nat, double* far, double
 }
 
 hipError_t launch_fock(const SysArgs& a, const double* Pt, const double* Ps, double scale, double* F, hipStream_t st,
-                       double* epart, double* e_out) {
+                       double* epart, double* e_out, bool full) {
     const int64_t nh = a.md.n_heavy, nH = a.md.n_h, ndiag = nh + (nH + 7) / 8;
     if (!e_out) epart = nullptr;
     hipLaunchKernelGGL(k_pack_diag, dim3(blocks_for(a.md.nat)), dim3(256), 0, st, a, Pt);
@@ -2430,7 +2436,7 @@ hipError_t launch_fock(const SysArgs& a, const double* Pt, const double* Ps, dou
         hipLaunchKernelGGL(k_fock_rows_h, dim3((unsigned)nH), dim3(256), 0, st, a, Ps, scale, F, epart, ndiag + nh);
     // after k_fock_rows_heavy, which leaves the heavy partners' terms in rsum and tcol
     hipLaunchKernelGGL(k_diag_blocks<true>, dim3((unsigned)ndiag), dim3(256), 0, st, a, Pt, Ps, scale, F, epart);
-    if (a.md.nat > 1) {
+    if (full && a.md.nat > 1) {
         const int64_t nt = ((int64_t)a.md.nao + 31) / 32;
         hipLaunchKernelGGL(k_sym_lower, dim3((unsigned)(nt * (nt + 1) / 2)), dim3(256), 0, st, a.md.nao, F);
     }
