@@ -122,6 +122,7 @@ pub mod raw {
         pub phase_ms: [c_double; 5],
         pub charges: *mut c_double,
         pub density: *mut c_double,
+        pub spin_density: *mut c_double,
     }
 
     #[repr(C)]
@@ -183,6 +184,8 @@ pub mod raw {
         pub fn am1g_get_tunable(ctx: *const am1g_ctx, name: *const c_char, value: *mut c_double) -> c_int;
         pub fn am1g_gradient(ctx: *mut am1g_ctx, rq: *const am1g_request, res: *mut am1g_result,
                              gradient: *mut c_double) -> c_int;
+        pub fn am1g_gradient_density(ctx: *mut am1g_ctx, rq: *const am1g_request, density: *const c_double,
+                                     spin_density: *const c_double, gradient: *mut c_double) -> c_int;
         pub fn am1g_core_hamiltonian(ctx: *mut am1g_ctx, rq: *const am1g_request, h_core: *mut c_double,
                                      core_ev: *mut c_double, pair_blocks: *mut c_double,
                                      pair_blocks_len: c_longlong) -> c_int;

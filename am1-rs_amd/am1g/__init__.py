@@ -157,6 +157,7 @@ class DcResult(ctypes.Structure):
         ("phase_ms", ctypes.c_double * 5),
         ("charges", ctypes.POINTER(ctypes.c_double)),
         ("density", ctypes.POINTER(ctypes.c_double)),
+        ("spin_density", ctypes.POINTER(ctypes.c_double)),
     ]
 
 
@@ -168,7 +169,7 @@ EXPORTS = ["am1g_request_defaults", "am1g_create", "am1g_destroy", "am1g_scf", "
            "am1g_serve_submit", "am1g_serve_poll", "am1g_serve_cancel", "am1g_serve_get_stats",
            "am1g_serve_last_error", "am1g_symmetric_eigen", "am1g_last_error", "am1g_element_info", "am1g_last_kernel_ms",
            "am1g_scf_batch_noalloc", "am1g_noalloc_capacity_bytes", "am1g_bcc_charges", "am1g_gradient",
-           "am1g_set_tunable", "am1g_get_tunable", "am1g_core_hamiltonian"]
+           "am1g_set_tunable", "am1g_get_tunable", "am1g_core_hamiltonian", "am1g_gradient_density"]
 
 _lib = None
 
@@ -205,6 +206,8 @@ def load():
     L.am1g_core_hamiltonian.argtypes = [ctypes.c_void_p, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_double),
                                         ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                         ctypes.c_longlong]
+    L.am1g_gradient_density.argtypes = [ctypes.c_void_p, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_double),
+                                        ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
     L.am1g_noalloc_capacity_bytes.argtypes = []
     L.am1g_noalloc_capacity_bytes.restype = ctypes.c_longlong
     L.am1g_dc_defaults.argtypes = [ctypes.POINTER(DcOptions)]
@@ -519,13 +522,16 @@ class Engine:
         rs = DcResult()
         q = np.zeros(rq.n_atoms)
         rs.charges = q.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
-        P = None
+        P = S = None
         if density:
             nao = sum(1 if int(v) == 1 else 4 for v in np.asarray(z).ravel())
             P = np.zeros(nao * nao)
+            S = np.zeros(nao * nao)
             rs.density = P.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+            rs.spin_density = S.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
         self._L.am1g_dc(self._ctx, ctypes.byref(rq), ctypes.byref(op), ctypes.byref(rs))
-        out = {k: getattr(rs, k) for k, _ in DcResult._fields_ if k not in ("charges", "density", "phase_ms")}
+        out = {k: getattr(rs, k) for k, _ in DcResult._fields_
+               if k not in ("charges", "density", "spin_density", "phase_ms")}
         out["converged"] = bool(rs.converged)
         out["status_name"] = STATUS_NAMES.get(rs.status, str(rs.status))
         out["phase_ms"] = dict(zip(DC_PHASES, list(rs.phase_ms)))
@@ -534,9 +540,25 @@ class Engine:
         if density:
             n = int(round(len(P) ** 0.5))
             out["density"] = P.reshape(n, n)
+            out["spin_density"] = S.reshape(n, n) if rs.unrestricted else None
         if rs.status not in (0, 4):
             out["message"] = self.last_error()
         return out
+
+    def gradient_density(self, z, xyz_bohr, density, spin_density=None, **kw):
+        """Fixed-density nuclear gradient (``am1g_gradient_density``) at a given total density and,
+        for an open shell, spin density P_alpha - P_beta: [n][3] in eV/Bohr."""
+        keep = []
+        rq = self._request(z, xyz_bohr, keep, **kw)
+        p = np.ascontiguousarray(density, dtype=np.float64)
+        sp = None if spin_density is None else np.ascontiguousarray(spin_density, dtype=np.float64)
+        g = np.zeros((rq.n_atoms, 3))
+        dp = ctypes.POINTER(ctypes.c_double)
+        st = self._L.am1g_gradient_density(self._ctx, ctypes.byref(rq), p.ctypes.data_as(dp),
+                                           sp.ctypes.data_as(dp) if sp is not None else None, g.ctypes.data_as(dp))
+        if st != 0:
+            raise ValueError(f"{STATUS_NAMES.get(st, st)}: {self.last_error()}")
+        return g
 
 
 class Server:
@@ -657,23 +679,8 @@ def single_point(numbers, positions, charge=0.0, multiplicity=1, reference="auto
     ``converged``, ``iterations``, ``unrestricted``. ``electric_field`` is three floats in atomic
     units (Hartree per e*Bohr) or None. Every error, non-convergence included, raises
     ``ValueError`` (python.rs:27-29)."""
-    z = _as_numbers(numbers)
-    pos = np.asarray(positions, dtype=np.float64)
-    if pos.ndim != 2 or pos.shape[1] != 3:
-        raise ValueError("each position must have 3 components")
-    if len(z) != pos.shape[0]:
-        raise ValueError("numbers and positions length mismatch")
-    field = None
-    if electric_field is not None:
-        f = np.asarray(electric_field, dtype=float).reshape(-1)
-        if f.size != 3:
-            raise ValueError("electric_field must have three components")
-        field = f * AM1_EV  # atomic units (Hartree per e*Bohr) -> eV per e*Bohr (python.rs:55-70)
-    if str(method).lower() not in METHODS:
-        raise ValueError(f"unknown method {method!r} (expected 'am1' or 'rm1')")
-    if str(reference).lower() not in REFERENCES:
-        raise ValueError(f"unknown reference {reference!r} (expected 'auto', 'rhf' or 'uhf')")
-    r = engine().scf(z, pos * ANGSTROM_TO_BOHR, full=False, charge=float(charge), multiplicity=max(1, int(multiplicity)),
+    z, pos, field = _molecular_input(numbers, positions, method, reference, electric_field)
+    r = engine().scf(z, pos, full=False, charge=float(charge), multiplicity=max(1, int(multiplicity)),
                      method=str(method).lower(), reference=str(reference).lower(), electric_field=field)
     if r["status"] == 4:
         raise ValueError(f"SCF did not converge in {r['iterations']} iterations "
@@ -699,6 +706,151 @@ def single_point(numbers, positions, charge=0.0, multiplicity=1, reference="auto
         "iterations": r["iterations"],
         "unrestricted": r["unrestricted"],
     }
+
+
+def _molecular_input(numbers, positions, method, reference, electric_field):
+    """Validated (z, positions in Bohr, field in eV per e*Bohr or None) for the module-level
+    functions, with the reference front end's checks and errors (python.rs:27-70)."""
+    z = _as_numbers(numbers)
+    pos = np.asarray(positions, dtype=np.float64)
+    if pos.ndim != 2 or pos.shape[1] != 3:
+        raise ValueError("each position must have 3 components")
+    if len(z) != pos.shape[0]:
+        raise ValueError("numbers and positions length mismatch")
+    field = None
+    if electric_field is not None:
+        f = np.asarray(electric_field, dtype=float).reshape(-1)
+        if f.size != 3:
+            raise ValueError("electric_field must have three components")
+        field = f * AM1_EV  # atomic units (Hartree per e*Bohr) -> eV per e*Bohr
+    if str(method).lower() not in METHODS:
+        raise ValueError(f"unknown method {method!r} (expected 'am1' or 'rm1')")
+    if str(reference).lower() not in REFERENCES:
+        raise ValueError(f"unknown reference {reference!r} (expected 'auto', 'rhf' or 'uhf')")
+    return z, pos * ANGSTROM_TO_BOHR, field
+
+
+def orbitals(numbers, positions, charge=0.0, multiplicity=1, reference="auto", method="am1",
+             electric_field=None) -> dict:
+    """``am1_rs`` ``orbitals`` counterpart (src/python.rs:1905-1992): orbital energies in Hartree
+    and eV, the coefficient matrix (rows atomic orbitals, columns orbitals), the occupied count,
+    HOMO/LUMO, both spin channels for an unrestricted run (``beta_*`` keys), the AO labels
+    ``(atom, "s" | "px" | "py" | "pz")`` and the method."""
+    z, pos, field = _molecular_input(numbers, positions, method, reference, electric_field)
+    r = engine().scf(z, pos, full=True, charge=float(charge), multiplicity=max(1, int(multiplicity)),
+                     method=str(method).lower(), reference=str(reference).lower(), electric_field=field)
+    if r["status"] == 4:
+        raise ValueError(f"SCF did not converge in {r['iterations']} iterations (residual {r['residual']:.3e})")
+    if r["status"] != 0:
+        raise ValueError(r.get("message", r["status_name"]))
+    d = {
+        "energies_hartree": (r["mo_energies"] / AM1_EV).tolist(),
+        "energies_ev": r["mo_energies"].tolist(),
+        "coefficients": r["mo_coeff"].tolist(),
+        "n_occupied": r["n_occ"],
+        "homo_ev": r["homo_ev"],
+        "lumo_ev": r["lumo_ev"],
+        "unrestricted": r["unrestricted"],
+    }
+    if r["unrestricted"]:
+        d.update({
+            "beta_energies_hartree": (r["mo_energies_beta"] / AM1_EV).tolist(),
+            "beta_energies_ev": r["mo_energies_beta"].tolist(),
+            "beta_coefficients": r["mo_coeff_beta"].tolist(),
+            "beta_n_occupied": r["n_occ_beta"],
+            "homo_beta_ev": r["homo_beta_ev"],
+            "lumo_beta_ev": r["lumo_beta_ev"],
+        })
+    labels = []
+    for a, zz in enumerate(z):
+        labels.extend([(a, o) for o in (("s",) if int(zz) == 1 else ("s", "px", "py", "pz"))])
+    d["ao_labels"] = labels
+    d["method"] = str(method).lower()
+    return d
+
+
+def _block_nonzero(density, z):
+    """Atom x atom mask of the density blocks that are not identically zero."""
+    nz = np.asarray(density) != 0.0
+    offs = np.concatenate([[0], np.cumsum([1 if int(v) == 1 else 4 for v in z])[:-1]])
+    return np.logical_or.reduceat(np.logical_or.reduceat(nz, offs, axis=0), offs, axis=1)
+
+
+def divide_conquer(numbers, positions, charge=0.0, multiplicity=1, reference="auto", method="am1",
+                   core_size=12, buffer_radius=11.0, smearing_ev=0.05, e_tol=1.0e-7, p_tol=1.0e-6, max_scf=300,
+                   mixing=0.4, gap_warn_ev=0.5, forces=True, multipole_cutoff=None, electric_field=None, cell=None,
+                   pbc=None, realspace_cutoff=40.0, exchange_cutoff=20.0, optimize=False, opt_max_iter=200,
+                   opt_gtol=1.0e-3) -> dict:
+    """``am1_rs`` ``divide_conquer`` counterpart (src/python.rs:448-663): the same signature and
+    defaults (positions in Angstrom, ``buffer_radius`` and ``multipole_cutoff`` in Bohr,
+    ``smearing_ev`` the Fermi-Dirac kT, 0 for aufbau filling) and the same result keys, forces in
+    eV/Angstrom from the fixed-density gradient at the divide-and-conquer density
+    (divide_conquer.rs:1448-1515). Non-convergence raises ``ValueError``. Periodic cells
+    (``cell``/``pbc``) and geometry optimization (``optimize``) are outside the GPU engine and
+    raise ``NotImplementedError``; ``realspace_cutoff``, ``exchange_cutoff``, ``opt_max_iter`` and
+    ``opt_gtol`` only apply to those."""
+    if (cell is None) != (pbc is None):
+        raise ValueError("cell and pbc must be given together")
+    if cell is not None:
+        raise NotImplementedError("periodic divide-and-conquer is not part of the am1g GPU engine")
+    if optimize:
+        raise NotImplementedError("divide-and-conquer geometry optimization is not part of the am1g GPU engine")
+    z, pos, field = _molecular_input(numbers, positions, method, reference, electric_field)
+    eng = engine()
+    kw = dict(charge=float(charge), multiplicity=max(1, int(multiplicity)), method=str(method).lower(),
+              reference=str(reference).lower(), electric_field=field, multipole_cutoff=multipole_cutoff)
+    r = eng.dc(z, pos, core_size=int(core_size), buffer_radius=float(buffer_radius), kt=float(smearing_ev),
+               max_scf=int(max_scf), e_tol=float(e_tol), p_tol=float(p_tol), mixing=float(mixing), density=True, **kw)
+    if r["status"] not in (0, 4):
+        raise ValueError(r.get("message", r["status_name"]))
+    if not r["converged"]:
+        raise ValueError(f"divide-and-conquer SCF did not converge in {r['iterations']} iterations")
+    unrestricted = bool(r["unrestricted"])
+    gap = r["homo_lumo_gap_ev"]
+    warning = None
+    if gap < gap_warn_ev:
+        warning = (f"smallest subsystem gap {gap:.3f} eV is below gap_warn_ev = {gap_warn_ev:.3f} eV: the density "
+                   "of a small-gap system decays slowly with distance, so the buffer may not contain what the "
+                   "cores need; compare with a full SCF or enlarge buffer_radius")
+    blocks = _block_nonzero(r["density"], z)
+    iu = np.triu_indices(len(z), 1)
+    near = np.ones(len(iu[0]), dtype=bool)
+    if multipole_cutoff is not None and multipole_cutoff > 0:
+        dist = np.linalg.norm(pos[iu[1]] - pos[iu[0]], axis=1)
+        near = dist <= multipole_cutoff
+    d = {
+        "energy_ev": r["total_ev"],
+        "energy_hartree": r["total_ev"] / AM1_EV,
+        "free_energy_ev": r["total_ev"] - r["entropy_ev"],
+        "heat_of_formation_kcal": r["heat_of_formation_kcal"],
+        "electronic_ev": r["electronic_ev"],
+        "core_ev": r["core_ev"],
+        "converged": True,
+        "positions_angstrom": (pos / ANGSTROM_TO_BOHR).tolist(),
+    }
+    if forces:
+        g = eng.gradient_density(z, pos, r["density"], r["spin_density"] if unrestricted else None, **kw)
+        d["forces_ev_per_angstrom"] = (-g * ANGSTROM_TO_BOHR).tolist()
+    d.update({
+        "charges": np.asarray(r["charges"]).tolist(),
+        "fermi_energy_ev": r["fermi_ev"],
+        "fermi_energies_ev": [r["fermi_ev"], r["fermi_beta_ev"]] if unrestricted else [r["fermi_ev"]],
+        "entropy_ev": r["entropy_ev"],
+        "iterations": r["iterations"],
+        "unrestricted": unrestricted,
+        "homo_lumo_gap_ev": gap,
+        "small_gap_warning": warning,
+        "subsystems": r["n_subsystems"],
+        "largest_subsystem_aos": r["largest_subsystem_aos"],
+        "diagonalization_work": r["diagonalization_work"],
+        "coulomb_work": int(near.sum()),
+        "exchange_work": int((blocks[iu] & near).sum()),
+        "retained_density_blocks": int(blocks.sum()),
+        "diis_pattern_elements": int(r["retained_pattern"]),
+        "dense_triangle_elements": int(len(r["density"]) * (len(r["density"]) + 1) // 2),
+        "method": str(method).lower(),
+    })
+    return d
 
 
 def bcc_correct(numbers, xyz_bohr, mulliken) -> dict:
@@ -752,21 +904,8 @@ def gradient(numbers, positions, charge=0.0, multiplicity=1, reference="auto", m
     ``lumo_ev``, ``gradient_hartree_per_bohr``, ``gradient_ev_per_angstrom``, ``charges``,
     ``dipole_debye``, ``dipole_magnitude``, ``max_gradient_hartree_per_bohr``, ``converged``,
     ``iterations``, ``unrestricted``, ``method``. Errors raise ``ValueError``."""
-    z = _as_numbers(numbers)
-    pos = np.asarray(positions, dtype=np.float64)
-    if pos.ndim != 2 or pos.shape[1] != 3 or pos.shape[0] != len(z):
-        raise ValueError("numbers and positions length mismatch")
-    field = None
-    if electric_field is not None:
-        f = np.asarray(electric_field, dtype=float).reshape(-1)
-        if f.size != 3:
-            raise ValueError("electric_field must have three components")
-        field = f * AM1_EV
-    if str(method).lower() not in METHODS:
-        raise ValueError(f"unknown method {method!r} (expected 'am1' or 'rm1')")
-    if str(reference).lower() not in REFERENCES:
-        raise ValueError(f"unknown reference {reference!r} (expected 'auto', 'rhf' or 'uhf')")
-    r = engine().gradient(z, pos * ANGSTROM_TO_BOHR, charge=float(charge), multiplicity=max(1, int(multiplicity)),
+    z, pos, field = _molecular_input(numbers, positions, method, reference, electric_field)
+    r = engine().gradient(z, pos, charge=float(charge), multiplicity=max(1, int(multiplicity)),
                           method=str(method).lower(), reference=str(reference).lower(), electric_field=field)
     if r["status"] == 4:
         raise ValueError(f"SCF did not converge in {r['iterations']} iterations (residual {r['residual']:.3e})")

@@ -698,6 +698,12 @@ int am1g_dc(am1g_ctx* ctx, const am1g_request* rq, const am1g_dc_options* op, am
             if (std::isfinite(below) && std::isfinite(above)) gap = std::min(gap, above - below);
         }
     if (res->density) HIP_TRY(ctx, hipMemcpy(res->density, Pt, sizeof(double) * nn, hipMemcpyDeviceToHost));
+    if (res->spin_density && uhf) {  // P_alpha - P_beta
+        std::vector<double> pb(nn);
+        HIP_TRY(ctx, hipMemcpy(res->spin_density, P[0], sizeof(double) * nn, hipMemcpyDeviceToHost));
+        HIP_TRY(ctx, hipMemcpy(pb.data(), P[1], sizeof(double) * nn, hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < nn; ++k) res->spin_density[k] -= pb[k];
+    }
     res->iterations = iterations;
     res->converged = converged;
     res->unrestricted = uhf ? 1 : 0;

@@ -269,52 +269,15 @@ using namespace am1g;
 // Nuclear gradient (closed_form_gradient, gradient.rs:173-254; its Python front end python.rs:
 // 202-275): the SCF of am1g_scf, then the pair derivatives at the converged density, plus the
 // external field's -F q_a (gradient.rs:243-270).
-extern "C" int am1g_gradient(am1g_ctx* ctx, const am1g_request* rq, am1g_result* res, double* gradient) {
-    if (!ctx || !rq || !res || !gradient) return am1g_set_error(ctx, AM1G_INVALID_INPUT, "null argument");
-    Prepared prep;
-    std::string msg;
-    int st = am1g_prepare(ctx, *rq, prep, msg, INT_MAX);
-    if (st != AM1G_OK) {
-        res->status = st;
-        return am1g_set_error(ctx, st, "%s", msg.c_str());
-    }
-    const MolDesc md0 = prep.md;
+// The gradient at fixed alpha and beta densities pab = [Pa | Pb] (host, 2 nao^2 doubles) with the
+// Mulliken charges q (for the field term): pair derivatives on the device, fixed-order sums per
+// atom, then -F q_a. Takes the context lock.
+static int gradient_at(am1g_ctx* ctx, const am1g_request* rq, const MolDesc& md0, const std::vector<double>& pab,
+                       const double* q, double* gradient) {
     const int nat = rq->n_atoms, nao = md0.nao;
     const size_t nn = (size_t)nao * nao;
-    // the SCF, with the total (and spin) density brought back if the caller did not ask for them
-    std::vector<double> ptot, pspin, qbuf;
-    am1g_result r = *res;
-    if (!r.density) {
-        ptot.resize(nn);
-        r.density = ptot.data();
-    }
-    if (prep.uhf && !r.spin_density) {
-        pspin.resize(nn);
-        r.spin_density = pspin.data();
-    }
-    if (!r.charges) {
-        qbuf.resize(nat);
-        r.charges = qbuf.data();
-    }
-    st = am1g_scf(ctx, rq, &r);
-    {
-        double* keep[] = {res->charges, res->density, res->spin_density};
-        *res = r;
-        res->charges = keep[0];
-        res->density = keep[1];
-        res->spin_density = keep[2];
-    }
-    if (st != AM1G_OK) return st;
-    const auto w0 = std::chrono::steady_clock::now();
     std::lock_guard<std::mutex> lock(ctx->mu);
     HIP_TRY(ctx, hipSetDevice(ctx->device));
-    // alpha and beta densities: halves of the total for RHF, (P +- P_spin) / 2 for UHF
-    std::vector<double> pab(2 * nn);
-    for (size_t k = 0; k < nn; ++k) {
-        const double s = prep.uhf ? r.spin_density[k] : 0.0;
-        pab[k] = 0.5 * (r.density[k] + s);
-        pab[nn + k] = 0.5 * (r.density[k] - s);
-    }
     // atom tables: heavy atoms then hydrogens, input order within each (the pair layout of k_pairs)
     const DevElem* tab = ctx->host_tab + md0.method * kMaxZ;
     std::vector<int> zs(nat), aoff(nat), acls(nat), alist;
@@ -381,8 +344,85 @@ extern "C" int am1g_gradient(am1g_ctx* ctx, const am1g_request* rq, am1g_result*
     HIP_TRY(ctx, hipStreamSynchronize(s));
     if (rq->electric_field) {  // -F q_a (dipole.rs:155-162)
         for (int a = 0; a < nat; ++a)
-            for (int k = 0; k < 3; ++k) gradient[3 * a + k] -= rq->electric_field[k] * r.charges[a];
+            for (int k = 0; k < 3; ++k) gradient[3 * a + k] -= rq->electric_field[k] * q[a];
     }
-    res->phase_ms[5] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
     return AM1G_OK;
+}
+
+extern "C" int am1g_gradient(am1g_ctx* ctx, const am1g_request* rq, am1g_result* res, double* gradient) {
+    if (!ctx || !rq || !res || !gradient) return am1g_set_error(ctx, AM1G_INVALID_INPUT, "null argument");
+    Prepared prep;
+    std::string msg;
+    int st = am1g_prepare(ctx, *rq, prep, msg, INT_MAX);
+    if (st != AM1G_OK) {
+        res->status = st;
+        return am1g_set_error(ctx, st, "%s", msg.c_str());
+    }
+    const MolDesc md0 = prep.md;
+    const int nat = rq->n_atoms, nao = md0.nao;
+    const size_t nn = (size_t)nao * nao;
+    // the SCF, with the total (and spin) density brought back if the caller did not ask for them
+    std::vector<double> ptot, pspin, qbuf;
+    am1g_result r = *res;
+    if (!r.density) {
+        ptot.resize(nn);
+        r.density = ptot.data();
+    }
+    if (prep.uhf && !r.spin_density) {
+        pspin.resize(nn);
+        r.spin_density = pspin.data();
+    }
+    if (!r.charges) {
+        qbuf.resize(nat);
+        r.charges = qbuf.data();
+    }
+    st = am1g_scf(ctx, rq, &r);
+    {
+        double* keep[] = {res->charges, res->density, res->spin_density};
+        *res = r;
+        res->charges = keep[0];
+        res->density = keep[1];
+        res->spin_density = keep[2];
+    }
+    if (st != AM1G_OK) return st;
+    const auto w0 = std::chrono::steady_clock::now();
+    // alpha and beta densities: halves of the total for RHF, (P +- P_spin) / 2 for UHF
+    std::vector<double> pab(2 * nn);
+    for (size_t k = 0; k < nn; ++k) {
+        const double s = prep.uhf ? r.spin_density[k] : 0.0;
+        pab[k] = 0.5 * (r.density[k] + s);
+        pab[nn + k] = 0.5 * (r.density[k] - s);
+    }
+    st = gradient_at(ctx, rq, md0, pab, r.charges, gradient);
+    res->phase_ms[5] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
+    return st;
+}
+
+extern "C" int am1g_gradient_density(am1g_ctx* ctx, const am1g_request* rq, const double* density,
+                                     const double* spin_density, double* gradient) {
+    if (!ctx || !rq || !density || !gradient) return am1g_set_error(ctx, AM1G_INVALID_INPUT, "null argument");
+    Prepared prep;
+    std::string msg;
+    const int st = am1g_prepare(ctx, *rq, prep, msg, INT_MAX);
+    if (st != AM1G_OK) return am1g_set_error(ctx, st, "%s", msg.c_str());
+    const MolDesc& md0 = prep.md;
+    const int nat = rq->n_atoms, nao = md0.nao;
+    const size_t nn = (size_t)nao * nao;
+    std::vector<double> pab(2 * nn);
+    for (size_t k = 0; k < nn; ++k) {
+        const double s = spin_density ? spin_density[k] : 0.0;
+        pab[k] = 0.5 * (density[k] + s);
+        pab[nn + k] = 0.5 * (density[k] - s);
+    }
+    // Mulliken charges of the given density, for the field term
+    const DevElem* tab = ctx->host_tab + md0.method * kMaxZ;
+    std::vector<double> q(nat);
+    for (int a = 0, off = 0; a < nat; ++a) {
+        const DevElem& el = tab[rq->z[a]];
+        double pop = 0.0;
+        for (int k = 0; k < el.norb; ++k) pop += density[(size_t)(off + k) * nao + off + k];
+        q[a] = el.core - pop;
+        off += el.norb;
+    }
+    return gradient_at(ctx, rq, md0, pab, q.data(), gradient);
 }

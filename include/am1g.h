@@ -162,6 +162,8 @@ typedef struct {
     double phase_ms[5];              /* Fock, subsystem eigensolves, Fermi level, assembly, DIIS */
     double* charges;                 /* [n_atoms] */
     double* density;                 /* [nao*nao] optional */
+    double* spin_density;            /* [nao*nao] optional, unrestricted runs only: P_alpha - P_beta
+                                        (DcResult::spin_density) */
 } am1g_dc_result;
 
 void am1g_dc_defaults(am1g_dc_options* op);
@@ -246,6 +248,15 @@ int am1g_get_tunable(const am1g_ctx* ctx, const char* name, double* value);
  * integrals, the scheme of analytic_gradient, gradient.rs:41-90), RHF and UHF, plus -F q_a for
  * an external field. gradient[n_atoms][3] in eV/Bohr. */
 int am1g_gradient(am1g_ctx* ctx, const am1g_request* rq, am1g_result* res, double* gradient);
+
+/* Nuclear gradient at given densities (the fixed-density, Hellmann-Feynman gradient):
+ * electronic_gradient_fixed_density(_spin) plus the core-core, far-field and field terms
+ * (src/gradient.rs, src/divide_conquer.rs:1448-1515 divide_conquer_gradient, which applies it to
+ * the divide-and-conquer density). density[nao][nao] is the total density; spin_density (NULL
+ * for a closed shell) is P_alpha - P_beta. gradient[n_atoms][3] in eV/Bohr. The request's
+ * geometry, method, field and multipole_cutoff are used; no SCF is run. */
+int am1g_gradient_density(am1g_ctx* ctx, const am1g_request* rq, const double* density,
+                          const double* spin_density, double* gradient);
 
 /* AM1-BCC partial charges (am1_bcc_charges, src/bcc/mod.rs:99-150) from AM1 Mulliken charges:
  * the molecular graph is perceived from the geometry (src/topology.rs: bonds by covalent radii,

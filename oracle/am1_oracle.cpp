@@ -2215,6 +2215,37 @@ int oracle_gradient(const OracleRequest* rq, OracleResult* res, double h, double
     return ST_OK;
 }
 
+// The same central differences of the energy at given alpha and beta densities (nao x nao each):
+// the fixed-density gradient divide_conquer_gradient (divide_conquer.rs:1448-1515) applies to the
+// divide-and-conquer density.
+int oracle_gradient_density(const OracleRequest* rq, const double* pa, const double* pb, double h, double* grad) {
+    const int nat = rq->n_atoms;
+    int nao = 0;
+    {
+        System s;
+        if (!build_system(s, nat, rq->z, rq->xyz_bohr, rq->method)) return s.status;
+        nao = s.nao;
+    }
+    Mat Pa(nao), Pb(nao);
+    std::copy(pa, pa + (size_t)nao * nao, Pa.d.begin());
+    std::copy(pb, pb + (size_t)nao * nao, Pb.d.begin());
+    std::vector<double> x(rq->xyz_bohr, rq->xyz_bohr + 3 * (size_t)nat);
+    for (int a = 0; a < nat; ++a)
+        for (int k = 0; k < 3; ++k) {
+            bool ok = true;
+            const double x0 = x[3 * a + k];
+            double e[4];
+            const double off[4] = {h, -h, 2.0 * h, -2.0 * h};
+            for (int q = 0; q < 4; ++q) {
+                x[3 * a + k] = x0 + off[q];
+                e[q] = energy_fixed_density(rq, x.data(), Pa, Pb, ok);
+            }
+            x[3 * a + k] = x0;
+            grad[3 * a + k] = (8.0 * (e[0] - e[1]) - (e[2] - e[3])) / (12.0 * h);
+        }
+    return ST_OK;
+}
+
 int oracle_scf_many(int n, const OracleRequest* rqs, OracleResult* res, int threads) {
 #ifdef _OPENMP
     int prev_nested = omp_get_max_active_levels();

@@ -225,6 +225,10 @@ def scf(z, xyz_bohr, full=True, **kw):
         rs.mo_energies_beta = _dp(mob)
         rs.density = _dp(dens)
         rs.mo_coeff = _dp(coeff)
+        da = np.zeros(nao_max * nao_max)
+        db = np.zeros(nao_max * nao_max)
+        rs.density_alpha = _dp(da)
+        rs.density_beta = _dp(db)
     L.oracle_scf(ctypes.byref(rq), ctypes.byref(rs))
     if full:
         nao = rs.nao
@@ -233,6 +237,7 @@ def scf(z, xyz_bohr, full=True, **kw):
             mo_energies_beta=mob[:nao].copy() if rs.unrestricted else None,
             density=dens[: nao * nao].reshape(nao, nao).copy(),
             mo_coeff=coeff[: nao * nao].reshape(nao, nao).copy(),
+            spin_density=(da[: nao * nao] - db[: nao * nao]).reshape(nao, nao) if rs.unrestricted else None,
         )
     return _result_dict(rs, extra)
 
@@ -284,6 +289,26 @@ def gradient(z, xyz_bohr, step=1.0e-4, **kw):
     out = _result_dict(rs, {"charges": charges})
     out["gradient"] = g
     return out
+
+
+def gradient_density(z, xyz_bohr, density, spin_density=None, step=1.0e-4, **kw):
+    """Fixed-density gradient (eV/Bohr, [n][3]) at a given total density and, for an open shell,
+    spin density P_alpha - P_beta (``oracle_gradient_density``)."""
+    L = lib()
+    L.oracle_gradient_density.argtypes = [ctypes.POINTER(_Request), ctypes.POINTER(ctypes.c_double),
+                                          ctypes.POINTER(ctypes.c_double), ctypes.c_double,
+                                          ctypes.POINTER(ctypes.c_double)]
+    keep = []
+    rq = _make_request(z, xyz_bohr, keep=keep, **kw)
+    p = np.ascontiguousarray(density, dtype=np.float64)
+    sp = np.zeros_like(p) if spin_density is None else np.ascontiguousarray(spin_density, dtype=np.float64)
+    pa = np.ascontiguousarray(0.5 * (p + sp))
+    pb = np.ascontiguousarray(0.5 * (p - sp))
+    g = np.zeros((len(z), 3))
+    st = L.oracle_gradient_density(ctypes.byref(rq), _dp(pa), _dp(pb), float(step), _dp(g))
+    if st:
+        raise ValueError(f"oracle_gradient_density status {st}")
+    return g
 
 
 def element(z, method="am1"):

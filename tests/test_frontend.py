@@ -117,3 +117,53 @@ def test_electric_field_large_path_and_batch(engine, monkeypatch):
     rl = engine.scf(z, x, full=False, electric_field=f)
     assert abs(rl["total_ev"] - o["total_ev"]) <= 1e-6
     assert np.abs(rl["charges"] - o["charges"]).max() <= 1e-6
+
+
+def test_orbitals_and_divide_conquer_signatures_match_the_reference():
+    # python.rs:1910 (orbitals) and :448-460 (divide_conquer)
+    p = list(inspect.signature(am1g.orbitals).parameters.values())
+    assert [x.name for x in p] == ["numbers", "positions", "charge", "multiplicity", "reference", "method",
+                                   "electric_field"]
+    p = list(inspect.signature(am1g.divide_conquer).parameters.values())
+    assert [x.name for x in p] == ["numbers", "positions", "charge", "multiplicity", "reference", "method",
+                                   "core_size", "buffer_radius", "smearing_ev", "e_tol", "p_tol", "max_scf", "mixing",
+                                   "gap_warn_ev", "forces", "multipole_cutoff", "electric_field", "cell", "pbc",
+                                   "realspace_cutoff", "exchange_cutoff", "optimize", "opt_max_iter", "opt_gtol"]
+    assert [x.default for x in p[2:]] == [0.0, 1, "auto", "am1", 12, 11.0, 0.05, 1.0e-7, 1.0e-6, 300, 0.4, 0.5, True,
+                                          None, None, None, None, 40.0, 20.0, False, 200, 1.0e-3]
+
+
+def test_divide_conquer_refuses_cells_and_optimization():
+    with pytest.raises(ValueError):  # cell without pbc is ambiguous (python.rs:494-500)
+        am1g.divide_conquer(WATER_NUMBERS, WATER_POSITIONS, cell=[[10, 0, 0], [0, 10, 0], [0, 0, 10]])
+    with pytest.raises(NotImplementedError):
+        am1g.divide_conquer(WATER_NUMBERS, WATER_POSITIONS, cell=[[10, 0, 0], [0, 10, 0], [0, 0, 10]],
+                            pbc=[True, True, True])
+    with pytest.raises(NotImplementedError):
+        am1g.divide_conquer(WATER_NUMBERS, WATER_POSITIONS, optimize=True)
+
+
+def test_oracle_fixed_density_gradient_is_the_scf_gradient_at_its_density():
+    """At the converged SCF density the fixed-density gradient is the SCF gradient (the two
+    oracle routines differentiate the same energy at the same densities)."""
+    from tests import molecules as M
+
+    for name, kw in (("formaldehyde", {}), ("methyl", {"multiplicity": 2})):
+        z, x = M.parse(getattr(M, name.upper()))
+        r = O.scf(z, x, full=True, e_tol=1e-11, p_tol=1e-10, **kw)
+        g = O.gradient(z, x, e_tol=1e-11, p_tol=1e-10, **kw)["gradient"]
+        sp = r.get("spin_density")
+        gd = O.gradient_density(z, x, r["density"], sp, **kw)
+        assert np.abs(gd - g).max() < 1e-9, name
+
+
+def test_oracle_dc_full_buffer_forces_are_the_scf_forces():
+    """divide_conquer_gradient at a full-buffer D&C density (which equals the SCF density,
+    tests/divide_conquer.rs:160-214) is the SCF gradient."""
+    from tests import molecules as M
+
+    z, x = M.water_chain(3)
+    dc = O.dc(z, x, core_size=3, buffer_radius=500.0, kt=0.0, e_tol=1e-11, p_tol=1e-10, max_scf=400, density=True)
+    g_dc = O.gradient_density(z, x, dc["density"])
+    g = O.gradient(z, x, e_tol=1e-11, p_tol=1e-10)["gradient"]
+    assert np.abs(g_dc - g).max() < 1e-6
