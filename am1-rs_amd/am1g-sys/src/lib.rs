@@ -177,6 +177,9 @@ pub mod raw {
         pub fn am1g_serve_last_error(sv: *const am1g_server) -> *const c_char;
         pub fn am1g_symmetric_eigen(ctx: *mut am1g_ctx, n: c_int, a: *const c_double, eigenvalues: *mut c_double,
                                     eigenvectors: *mut c_double) -> c_int;
+        pub fn am1g_matmul(ctx: *mut am1g_ctx, trans_a: c_int, trans_b: c_int, m: c_int, n: c_int, k: c_int,
+                           alpha: c_double, a: *const c_double, b: *const c_double, beta: c_double, c: *mut c_double,
+                           precision: c_int) -> c_int;
         pub fn am1g_last_error(ctx: *const am1g_ctx) -> *const c_char;
         pub fn am1g_element_info(z: c_int, method: c_int, out: *mut c_double) -> c_int;
         pub fn am1g_last_kernel_ms(ctx: *const am1g_ctx) -> c_double;

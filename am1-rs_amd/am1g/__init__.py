@@ -169,7 +169,8 @@ EXPORTS = ["am1g_request_defaults", "am1g_create", "am1g_destroy", "am1g_scf", "
            "am1g_serve_submit", "am1g_serve_poll", "am1g_serve_cancel", "am1g_serve_get_stats",
            "am1g_serve_last_error", "am1g_symmetric_eigen", "am1g_last_error", "am1g_element_info", "am1g_last_kernel_ms",
            "am1g_scf_batch_noalloc", "am1g_noalloc_capacity_bytes", "am1g_bcc_charges", "am1g_gradient",
-           "am1g_set_tunable", "am1g_get_tunable", "am1g_core_hamiltonian", "am1g_gradient_density"]
+           "am1g_set_tunable", "am1g_get_tunable", "am1g_core_hamiltonian", "am1g_gradient_density",
+           "am1g_matmul"]
 
 _lib = None
 
@@ -224,6 +225,9 @@ def load():
     L.am1g_serve_last_error.restype = ctypes.c_char_p
     L.am1g_symmetric_eigen.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                        ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+    L.am1g_matmul.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                              ctypes.c_double, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                              ctypes.c_double, ctypes.POINTER(ctypes.c_double), ctypes.c_int]
     L.am1g_last_error.argtypes = [ctypes.c_void_p]
     L.am1g_last_error.restype = ctypes.c_char_p
     L.am1g_request_defaults.argtypes = [ctypes.POINTER(Request)]
@@ -301,6 +305,25 @@ class Engine:
         if st != 0:
             raise ValueError(f"{STATUS_NAMES.get(st, st)}: {self.last_error()}")
         return w, v
+
+    def matmul(self, a, b, c=None, trans_a=False, trans_b=False, alpha=1.0, beta=0.0, precision="fp64"):
+        """``alpha op(a) op(b) + beta c`` on the device (``Matrix::matmul`` and its transposed and
+        accumulating forms); returns the product as a new array."""
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        m, k = (a.shape[1], a.shape[0]) if trans_a else a.shape
+        kb, n = (b.shape[1], b.shape[0]) if trans_b else b.shape
+        if kb != k:
+            raise ValueError(f"matmul: inner dimensions differ ({k} and {kb})")
+        out = np.zeros((m, n)) if c is None else np.array(c, dtype=np.float64, order="C", copy=True)
+        if out.shape != (m, n):
+            raise ValueError("matmul: c has the wrong shape")
+        dp = ctypes.POINTER(ctypes.c_double)
+        st = self._L.am1g_matmul(self._ctx, int(trans_a), int(trans_b), m, n, k, float(alpha), a.ctypes.data_as(dp),
+                                 b.ctypes.data_as(dp), float(beta), out.ctypes.data_as(dp), PRECISIONS[precision])
+        if st != 0:
+            raise ValueError(f"{STATUS_NAMES.get(st, st)}: {self.last_error()}")
+        return out
 
     # -- request marshalling -------------------------------------------------------------
     def _request(self, z, xyz_bohr, keep, charge=0.0, multiplicity=1, method="am1", reference="auto",

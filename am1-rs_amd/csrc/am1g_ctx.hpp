@@ -91,7 +91,6 @@ struct am1g_ctx {
     struct LargeBuffers* large = nullptr;  // one large system: dense matrices, DIIS history
     EigWork* eig = nullptr;                // large-order eigensolver workspace
     DevBuf grad_buf;                       // nuclear gradient: tables, densities, pair terms
-    rocblas_handle blas = nullptr;         // created on first use, bound to `stream`
 };
 
 // Buffers of the divide-and-conquer driver, kept on the context so repeated calls reuse them.

@@ -15,7 +15,6 @@
 //      three GEMMs per block).
 // The deflation scan and the final ordering of each merge run on the host (O(m) each).
 #include <hip/hip_runtime.h>
-#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <cfloat>
@@ -24,6 +23,7 @@
 
 #include "am1g_common.hpp"
 #include "am1g_eig.hpp"
+#include "am1g_gemm.hpp"
 #include "am1g_wg.hpp"
 
 namespace am1g {
@@ -461,13 +461,6 @@ inline unsigned blocks(int64_t n, int per) { return (unsigned)((n + per - 1) / p
         hipError_t e_ = (x);                         \
         if (e_ != hipSuccess) return e_;             \
     } while (0)
-#define BLAS_TRY(x)                                  \
-    do {                                             \
-        if ((x) != rocblas_status_success) {         \
-            err = "rocBLAS call failed: " #x;        \
-            return hipErrorUnknown;                  \
-        }                                            \
-    } while (0)
 
 struct Node {
     int s, m, m1;
@@ -487,7 +480,7 @@ void split(int s, int m, std::vector<Node>& merges, std::vector<int>& ls, std::v
 
 }  // namespace
 
-hipError_t eig_large(EigWork& w, rocblas_handle blas, int n, double* A, double* evals, double* Ct, hipStream_t st,
+hipError_t eig_large(EigWork& w, int n, double* A, double* evals, double* Ct, hipStream_t st,
                      std::string& err) {
     err.clear();
     if (n < 1) return hipSuccess;
@@ -713,8 +706,7 @@ hipError_t eig_large(EigWork& w, rocblas_handle blas, int n, double* A, double* 
             hipLaunchKernelGGL(k_gather_cols, dim3(blocks(m, 256), k), dim3(256), 0, st, n, Q, s, m, k, icol, W);
             EIG_TRY(hipGetLastError());
             const double one = 1.0, zero = 0.0;
-            BLAS_TRY(rocblas_dgemm(blas, rocblas_operation_none, rocblas_operation_none, m, k, k, &one, W, m, U, k,
-                                   &zero, Qt, m));
+            EIG_TRY(gemm_f64(st, false, false, m, k, k, one, W, m, U, k, zero, Qt, m));
             lamk.resize(k);
             EIG_TRY(hipMemcpyAsync(lamk.data(), dlamk, sizeof(double) * k, hipMemcpyDeviceToHost, st));
             EIG_TRY(hipStreamSynchronize(st));
@@ -781,16 +773,16 @@ hipError_t eig_large(EigWork& w, rocblas_handle blas, int n, double* A, double* 
             const int r0 = k0 + 1, mr = n - r0;
             hipLaunchKernelGGL(k_build_v, dim3(blocks(mr, 256), nb), dim3(256), 0, st, n, k0, nb, A, Vb);
             EIG_TRY(hipGetLastError());
-            BLAS_TRY(rocblas_dgemm(blas, rocblas_operation_transpose, rocblas_operation_none, nb, nb, mr, &one, Vb, mr,
-                                   Vb, mr, &zero, G, nb));
+            EIG_TRY(w.gw.reserve(sizeof(double) * gemm_splitk_elems(nb, nb, mr)));
+            EIG_TRY(gemm_f64(st, true, false, nb, nb, mr, one, Vb, mr, Vb, mr, zero, G, nb, w.gw.as<double>(),
+                             w.gw.bytes / sizeof(double)));
             hipLaunchKernelGGL(k_larft, dim3(1), dim3(256), 0, st, nb, k0, tau, G, T);
             EIG_TRY(hipGetLastError());
-            BLAS_TRY(rocblas_dgemm(blas, rocblas_operation_transpose, rocblas_operation_none, nb, n, mr, &one, Vb, mr,
-                                   Q + r0, n, &zero, X, nb));
-            BLAS_TRY(rocblas_dgemm(blas, rocblas_operation_none, rocblas_operation_none, nb, n, nb, &one, T, nb, X, nb,
-                                   &zero, Y, nb));
-            BLAS_TRY(rocblas_dgemm(blas, rocblas_operation_none, rocblas_operation_none, mr, n, nb, &mone, Vb, mr, Y,
-                                   nb, &one, Q + r0, n));
+            EIG_TRY(w.gw.reserve(sizeof(double) * gemm_splitk_elems(nb, n, mr)));
+            EIG_TRY(gemm_f64(st, true, false, nb, n, mr, one, Vb, mr, Q + r0, n, zero, X, nb, w.gw.as<double>(),
+                             w.gw.bytes / sizeof(double)));
+            EIG_TRY(gemm_f64(st, false, false, nb, n, nb, one, T, nb, X, nb, zero, Y, nb));
+            EIG_TRY(gemm_f64(st, false, false, mr, n, nb, mone, Vb, mr, Y, nb, one, Q + r0, n));
         }
     }
     EIG_TRY(hipMemcpyAsync(evals, w.h_lam.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));

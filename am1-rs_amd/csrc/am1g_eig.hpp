@@ -3,7 +3,6 @@
 // header of the library.
 #pragma once
 #include <hip/hip_runtime.h>
-#include <rocblas/rocblas.h>
 
 #include <string>
 #include <vector>
@@ -44,6 +43,7 @@ struct EigWork {
     EigBuf W, Qt, U, Qn;                      // merge scratch
     EigBuf V, Tm, X, Y, G;                    // back-transformation blocks
     EigBuf rot, idx, flt, lam;                // merge tables
+    EigBuf gw;                                // split-K partial products of the GEMMs
     std::vector<double> h_d, h_e, h_lam;
     bool use_graphs = true;                   // replay the tridiagonalization as a hipGraph
     std::vector<TridiagGraph> graphs;         // at most kMaxGraphs, oldest dropped first
@@ -60,7 +60,7 @@ namespace am1g {
 // evals[0..n) holds the eigenvalues in ascending order and Ct (row-major n x n) holds the
 // eigenvectors as rows (row k = eigenvector of evals[k]). Returns hipSuccess or the first
 // device error; `err` names a numerical failure (then the return is hipErrorUnknown).
-hipError_t eig_large(EigWork& w, rocblas_handle blas, int n, double* A, double* evals, double* Ct, hipStream_t st,
+hipError_t eig_large(EigWork& w, int n, double* A, double* evals, double* Ct, hipStream_t st,
                      std::string& err);
 
 }  // namespace am1g

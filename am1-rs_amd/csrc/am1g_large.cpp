@@ -2,7 +2,6 @@
 // plain-path SCF for molecules beyond one workgroup's reach (more than 256 atoms or basis
 // functions).
 #include <hip/hip_runtime.h>
-#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <chrono>
@@ -12,6 +11,7 @@
 #include <vector>
 
 #include "am1g_ctx.hpp"
+#include "am1g_gemm.hpp"
 #include "am1g_wg.hpp"
 
 // Device buffers of the large-system SCF, kept on the context so repeated calls reuse them.
@@ -20,20 +20,10 @@ struct LargeBuffers {
     DevBuf H, F, P, T0, T1, Pn, f32a, f32b;    // dense n x n (FP32 staging for mixed precision)
     DevBuf hist_f, hist_e, hist_d, ring, coef, partial, scal, pdiag, tcol, rsum, props, far;
     DevBuf Fb, Pb, Pt, Tb, Ctb, evals_b, hist_fb, hist_eb;  // beta channel (UHF)
+    DevBuf mm_a, mm_b, mm_c, mm_w;                         // am1g_matmul operands and split-K partials
 };
 
 int am1g_large_setup(am1g_ctx* ctx) {
-    if (!ctx->blas) {
-        if (rocblas_create_handle(&ctx->blas) != rocblas_status_success) {
-            ctx->blas = nullptr;
-            return am1g_set_error(ctx, AM1G_DEVICE_ERROR, "rocBLAS handle creation failed");
-        }
-        if (rocblas_set_stream(ctx->blas, ctx->stream) != rocblas_status_success)
-            return am1g_set_error(ctx, AM1G_DEVICE_ERROR, "rocBLAS stream binding failed");
-        // no split-K atomics: a request gives the same bits on every call and every lane
-        if (rocblas_set_atomics_mode(ctx->blas, rocblas_atomics_not_allowed) != rocblas_status_success)
-            return am1g_set_error(ctx, AM1G_DEVICE_ERROR, "rocBLAS atomics mode failed");
-    }
     if (!ctx->eig) ctx->eig = new EigWork();
     ctx->eig->use_graphs = ctx->tun.eig_graphs != 0;
     if (!ctx->large) ctx->large = new LargeBuffers();
@@ -45,8 +35,6 @@ void am1g_large_release(am1g_ctx* ctx) {
     ctx->large = nullptr;
     delete ctx->eig;
     ctx->eig = nullptr;
-    if (ctx->blas) (void)rocblas_destroy_handle(ctx->blas);
-    ctx->blas = nullptr;
 }
 
 namespace {
@@ -148,9 +136,7 @@ static int large_uhf(am1g_ctx* ctx, const am1g_request& rq, const MolDesc& md, c
         if (!low) ++fp64_iters;
         const int slot = diis ? (hcount < kDiisDepth ? (head + hcount) % kDiisDepth : head) : 0;
         for (int c = 0; c < 2; ++c) {
-            if (rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, P[c], n, F[c], n,
-                              &zero, T0, n) != rocblas_status_success)
-                return res->status = am1g_set_error(ctx, AM1G_DEVICE_ERROR, "rocBLAS dgemm failed");
+            HIP_TRY(ctx, gemm_f64(s, false, false, n, n, n, one, P[c], n, F[c], n, zero, T0, n));
             HIP_TRY(ctx, launch_push(n, T0, false, F[c], P[c], hf[c] + slot * len_s, he[c] + slot * len_a, nullptr,
                                      partial, scal + 3 + c, s));
         }
@@ -208,7 +194,7 @@ static int large_uhf(am1g_ctx* ctx, const am1g_request& rq, const MolDesc& md, c
         }
         phase[2] += ms_since(t_ph);
         for (int c = 0; c < 2; ++c) {
-            hipError_t e = eig_large(*ctx->eig, ctx->blas, n, const_cast<double*>(Fuse[c]), eps[c], Ct[c], s, eig_err);
+            hipError_t e = eig_large(*ctx->eig, n, const_cast<double*>(Fuse[c]), eps[c], Ct[c], s, eig_err);
             if (e != hipSuccess) {
                 if (eig_err.empty())
                     return res->status = am1g_set_error(ctx, AM1G_DEVICE_ERROR, "HIP error %s in the eigensolver",
@@ -225,15 +211,11 @@ static int large_uhf(am1g_ctx* ctx, const am1g_request& rq, const MolDesc& md, c
                 float* Cf = L.f32a.as<float>();
                 float* Pf = L.f32b.as<float>();
                 HIP_TRY(ctx, launch_to_f32((int64_t)n * nocc[c], Ct[c], Cf, s));
-                if (rocblas_sgemm(ctx->blas, rocblas_operation_none, rocblas_operation_transpose, n, n, nocc[c], &onef,
-                                  Cf, n, Cf, n, &zerof, Pf, n) != rocblas_status_success)
-                    return res->status = am1g_set_error(ctx, AM1G_DEVICE_ERROR, "rocBLAS sgemm failed");
+                HIP_TRY(ctx, gemm_f32(s, false, true, n, n, nocc[c], onef, Cf, n, Cf, n, zerof, Pf, n));
                 HIP_TRY(ctx, launch_density_update(n, Pf, true, P[c], partial, scal + 21 + c, s));
             } else {
                 if (nocc[c] > 0) {
-                    if (rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_transpose, n, n, nocc[c],
-                                      &one, Ct[c], n, Ct[c], n, &zero, Pn, n) != rocblas_status_success)
-                        return res->status = am1g_set_error(ctx, AM1G_DEVICE_ERROR, "rocBLAS dgemm failed");
+                    HIP_TRY(ctx, gemm_f64(s, false, true, n, n, nocc[c], one, Ct[c], n, Ct[c], n, zero, Pn, n));
                 } else {
                     HIP_TRY(ctx, hipMemsetAsync(Pn, 0, sizeof(double) * nn, s));
                 }
@@ -254,7 +236,13 @@ static int large_uhf(am1g_ctx* ctx, const am1g_request& rq, const MolDesc& md, c
             } else {
                 ++fp32_stall;
             }
-            if (err <= 1.0e-4 || fp32_stall >= 2) fp32_phase = false;
+            if (err <= 1.0e-4 || fp32_stall >= 2) {
+                // the FP64 phase starts a fresh history: entries built from FP32-rounded densities
+                // let the extrapolation settle on a density whose own commutator is still ~1e-5
+                fp32_phase = false;
+                hcount = 0;
+                head = 0;
+            }
         }
         // scf.rs:693 (RHF), :820 (UHF). A mixed-precision run takes another trajectory than the FP64 one, and the
         // density-change test at p_tol stops up to ~1e-6 e short of the fixed point, so it ends by
@@ -532,9 +520,7 @@ int am1g_large_scf(am1g_ctx* ctx, const am1g_request& rq, const Prepared& prep, 
         const bool low = fp32_phase;
         if (!low) ++fp64_iters;
         // commutator FP - PF: column-major P*F read row-major is F*P
-        if (rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, P, n, F, n, &zero, T0,
-                          n) != rocblas_status_success)
-            return res->status = am1g_set_error(ctx, AM1G_DEVICE_ERROR, "rocBLAS dgemm failed");
+        HIP_TRY(ctx, gemm_f64(s, false, false, n, n, n, one, P, n, F, n, zero, T0, n));
         const int slot = hcount < kDiisDepth ? (head + hcount) % kDiisDepth : head;
         HIP_TRY(ctx, launch_push(n, T0, false, F, P, hf + slot * len_s, he + slot * len_a,
                                  hd ? hd + slot * len_s : nullptr, partial, scal + 1, s));
@@ -605,7 +591,7 @@ int am1g_large_scf(am1g_ctx* ctx, const am1g_request& rq, const Prepared& prep, 
         }
         phase[2] += ms_since(t_ph);
         // eigensolve (destroys Fuse; F is rebuilt by the next Fock pass)
-        hipError_t e = eig_large(*ctx->eig, ctx->blas, n, Fuse, eps, Ct, s, eig_err);
+        hipError_t e = eig_large(*ctx->eig, n, Fuse, eps, Ct, s, eig_err);
         if (e != hipSuccess) {
             if (!eig_err.empty()) {
                 status = AM1G_LINEAR_ALGEBRA;
@@ -622,14 +608,10 @@ int am1g_large_scf(am1g_ctx* ctx, const am1g_request& rq, const Prepared& prep, 
                 float* Cf = L.f32a.as<float>();
                 float* Pf = L.f32b.as<float>();
                 HIP_TRY(ctx, launch_to_f32((int64_t)n * nocc, Ct, Cf, s));
-                if (rocblas_sgemm(ctx->blas, rocblas_operation_none, rocblas_operation_transpose, n, n, nocc, &twof, Cf,
-                                  n, Cf, n, &zerof, Pf, n) != rocblas_status_success)
-                    return res->status = am1g_set_error(ctx, AM1G_DEVICE_ERROR, "rocBLAS sgemm failed");
+                HIP_TRY(ctx, gemm_f32(s, false, true, n, n, nocc, twof, Cf, n, Cf, n, zerof, Pf, n));
                 HIP_TRY(ctx, launch_density_update(n, Pf, true, P, partial, scal + 27, s));
             } else {
-                if (rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_transpose, n, n, nocc, &two, Ct, n,
-                                  Ct, n, &zero, Pn, n) != rocblas_status_success)
-                    return res->status = am1g_set_error(ctx, AM1G_DEVICE_ERROR, "rocBLAS dgemm failed");
+                HIP_TRY(ctx, gemm_f64(s, false, true, n, n, nocc, two, Ct, n, Ct, n, zero, Pn, n));
                 HIP_TRY(ctx, launch_density_update(n, Pn, false, P, partial, scal + 27, s));
             }
         } else {
@@ -651,7 +633,13 @@ int am1g_large_scf(am1g_ctx* ctx, const am1g_request& rq, const Prepared& prep, 
             } else {
                 ++fp32_stall;
             }
-            if (err <= 1.0e-4 || fp32_stall >= 2) fp32_phase = false;
+            if (err <= 1.0e-4 || fp32_stall >= 2) {
+                // the FP64 phase starts a fresh history: entries built from FP32-rounded densities
+                // let the extrapolation settle on a density whose own commutator is still ~1e-5
+                fp32_phase = false;
+                hcount = 0;
+                head = 0;
+            }
         }
         // scf.rs:693 (RHF), :820 (UHF). A mixed-precision run takes another trajectory than the FP64 one, and the
         // density-change test at p_tol stops up to ~1e-6 e short of the fixed point, so it ends by
@@ -777,7 +765,7 @@ int am1g_symmetric_eigen(am1g_ctx* ctx, int n, const double* a, double* eigenval
     HIP_TRY(ctx, hipMemcpyAsync(L.A.p, a, sizeof(double) * nn, hipMemcpyHostToDevice, s));
     HIP_TRY(ctx, hipEventRecord(ctx->ev0, s));
     std::string err;
-    hipError_t e = eig_large(*ctx->eig, ctx->blas, n, L.A.as<double>(), L.evals.as<double>(), L.Ct.as<double>(), s, err);
+    hipError_t e = eig_large(*ctx->eig, n, L.A.as<double>(), L.evals.as<double>(), L.Ct.as<double>(), s, err);
     if (e != hipSuccess) {
         if (!err.empty()) return am1g_set_error(ctx, AM1G_LINEAR_ALGEBRA, "%s", err.c_str());
         return am1g_set_error(ctx, AM1G_DEVICE_ERROR, "HIP error %s in the eigensolver", hipGetErrorString(e));
@@ -793,6 +781,62 @@ int am1g_symmetric_eigen(am1g_ctx* ctx, int n, const double* a, double* eigenval
     // rows of Ct are eigenvectors; the result holds them as columns
     for (int i = 0; i < n; ++i)
         for (int k = 0; k < n; ++k) eigenvectors[(size_t)i * n + k] = ct[(size_t)k * n + i];
+    return AM1G_OK;
+}
+
+int am1g_matmul(am1g_ctx* ctx, int trans_a, int trans_b, int m, int n, int k, double alpha, const double* a,
+                const double* b, double beta, double* c, int precision) {
+    if (!ctx || m < 0 || n < 0 || k < 0 || (m > 0 && n > 0 && (!c || (k > 0 && (!a || !b)))))
+        return am1g_set_error(ctx, AM1G_INVALID_INPUT, "null argument");
+    if (precision != AM1G_PRECISION_FP64 && precision != AM1G_PRECISION_MIXED)
+        return am1g_set_error(ctx, AM1G_INVALID_INPUT, "matmul: unknown precision %d", precision);
+    if (m == 0 || n == 0) return AM1G_OK;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    int st = am1g_large_setup(ctx);
+    if (st != AM1G_OK) return st;
+    LargeBuffers& L = *ctx->large;
+    hipStream_t s = ctx->stream;
+    const size_t na = (size_t)m * k, nb = (size_t)k * n, nc = (size_t)m * n;
+    // Row-major c = op(a) op(b) is column-major c^T = op(b)^T op(a)^T: the column-major product
+    // takes b first; a row-major operand read column-major is its transpose.
+    const int lda = trans_b ? k : n, ldb = trans_a ? m : k;
+    const size_t wn = gemm_splitk_elems(n, m, k);
+    const bool f32 = precision == AM1G_PRECISION_MIXED;
+    const size_t esz = f32 ? sizeof(float) : sizeof(double);
+    HIP_TRY(ctx, L.mm_a.reserve(esz * std::max<size_t>(na, 1)));
+    HIP_TRY(ctx, L.mm_b.reserve(esz * std::max<size_t>(nb, 1)));
+    HIP_TRY(ctx, L.mm_c.reserve(esz * nc));
+    HIP_TRY(ctx, L.mm_w.reserve(esz * std::max<size_t>(wn, 1)));
+    if (f32) {
+        std::vector<float> ha(na), hb(nb), hc(nc);
+        for (size_t i = 0; i < na; ++i) ha[i] = (float)a[i];
+        for (size_t i = 0; i < nb; ++i) hb[i] = (float)b[i];
+        for (size_t i = 0; i < nc; ++i) hc[i] = beta != 0.0 ? (float)c[i] : 0.0f;
+        if (na) HIP_TRY(ctx, hipMemcpyAsync(L.mm_a.p, ha.data(), esz * na, hipMemcpyHostToDevice, s));
+        if (nb) HIP_TRY(ctx, hipMemcpyAsync(L.mm_b.p, hb.data(), esz * nb, hipMemcpyHostToDevice, s));
+        HIP_TRY(ctx, hipMemcpyAsync(L.mm_c.p, hc.data(), esz * nc, hipMemcpyHostToDevice, s));
+        HIP_TRY(ctx, hipEventRecord(ctx->ev0, s));
+        HIP_TRY(ctx, gemm_f32(s, trans_b != 0, trans_a != 0, n, m, k, (float)alpha, L.mm_b.as<float>(), lda,
+                              L.mm_a.as<float>(), ldb, (float)beta, L.mm_c.as<float>(), n, L.mm_w.as<float>(), wn));
+        HIP_TRY(ctx, hipEventRecord(ctx->ev1, s));
+        HIP_TRY(ctx, hipMemcpyAsync(hc.data(), L.mm_c.p, esz * nc, hipMemcpyDeviceToHost, s));
+        HIP_TRY(ctx, hipStreamSynchronize(s));
+        for (size_t i = 0; i < nc; ++i) c[i] = hc[i];
+    } else {
+        if (na) HIP_TRY(ctx, hipMemcpyAsync(L.mm_a.p, a, esz * na, hipMemcpyHostToDevice, s));
+        if (nb) HIP_TRY(ctx, hipMemcpyAsync(L.mm_b.p, b, esz * nb, hipMemcpyHostToDevice, s));
+        if (beta != 0.0) HIP_TRY(ctx, hipMemcpyAsync(L.mm_c.p, c, esz * nc, hipMemcpyHostToDevice, s));
+        HIP_TRY(ctx, hipEventRecord(ctx->ev0, s));
+        HIP_TRY(ctx, gemm_f64(s, trans_b != 0, trans_a != 0, n, m, k, alpha, L.mm_b.as<double>(), lda,
+                              L.mm_a.as<double>(), ldb, beta, L.mm_c.as<double>(), n, L.mm_w.as<double>(), wn));
+        HIP_TRY(ctx, hipEventRecord(ctx->ev1, s));
+        HIP_TRY(ctx, hipMemcpyAsync(c, L.mm_c.p, esz * nc, hipMemcpyDeviceToHost, s));
+        HIP_TRY(ctx, hipStreamSynchronize(s));
+    }
+    float ms = 0.f;
+    HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ctx->last_ms = ms;
     return AM1G_OK;
 }
 

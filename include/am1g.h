@@ -221,6 +221,17 @@ const char* am1g_serve_last_error(const am1g_server* sv);
  * back-transformation; the same solver serves the SCF of systems above 256 basis functions. */
 int am1g_symmetric_eigen(am1g_ctx* ctx, int n, const double* a, double* eigenvalues, double* eigenvectors);
 
+/* Dense matrix product on the device: Matrix::matmul, transpose_matmul, matmul_transpose and
+ * their _acc forms (src/linalg.rs:92-181), the products of the SCF's commutator and density
+ * steps. c = alpha * op(a) * op(b) + beta * c, all row-major: op(a) is m x k, op(b) k x n, c m x n;
+ * trans_a / trans_b = 1 means the operand is stored transposed (a is k x m, b is n x k). beta = 0
+ * does not read c. AM1G_PRECISION_FP64 runs FP64 matrix cores; AM1G_PRECISION_MIXED rounds the
+ * operands to FP32 and accumulates in FP32 (the mixed-precision density product). The library's
+ * own grid-level GEMM: the large-system SCF and eigensolver use the same kernels.
+ * am1g_last_kernel_ms gives the device time of the product alone. */
+int am1g_matmul(am1g_ctx* ctx, int trans_a, int trans_b, int m, int n, int k, double alpha, const double* a,
+                const double* b, double beta, double* c, int precision);
+
 /* Device-dependent launch choices of a context, measured by the project's autotuner (am1g.tune)
  * and read at am1g_create from tuned.txt beside the library (or the file AM1G_TUNE_FILE names):
  *   route_large_ao     single molecules above this many basis functions run on the grid-level
