@@ -15,6 +15,12 @@ Measures on the current GPU, through the C ABI, and writes the table the library
   timed on the same batch with ``tri_max`` 128.
 * ``eig_graphs``: whether replaying the large eigensolver's tridiagonalization as a HIP graph
   pays; timed on a 1200-order eigensolve.
+* ``tri_blocked_min``: the order from which the large eigensolver's tridiagonalization runs in
+  panels (two grid kernels per column reading the trailing matrix once, a rank-2k matrix-core
+  update per panel) instead of one column at a time (one grid pass that reads and writes the
+  trailing matrix, then a one-workgroup reflector step); orders 600, 1200 and 3072 are timed
+  both ways, and the blocked form is kept from the smallest order from which it wins at every
+  larger order measured (otherwise off).
 * ``pairs_block``: the workgroup size of the pair-integral kernel (64, 128 or 256 threads);
   timed by the core-Hamiltonian build of 2000 waters (18 M atom pairs), which that kernel
   dominates.
@@ -133,6 +139,36 @@ def tune_graphs(eng, log):
     return best, times
 
 
+_BLOCKED_OFF = 1 << 30
+
+
+def tune_blocked(eng, log):
+    rng = np.random.default_rng(6)
+    rows = []
+    for n in (600, 1200, 3072):
+        a = rng.standard_normal((n, n))
+        a = a + a.T
+        t = {}
+        for mode, bmin in (("column", _BLOCKED_OFF), ("blocked", 8)):
+            eng.set_tunable("tri_blocked_min", bmin)
+            eng.eigh(a)
+            ts = []
+            for _ in range(3):
+                eng.eigh(a)
+                ts.append(eng.last_kernel_ms())
+            t[mode] = min(ts)
+        rows.append((n, t["column"], t["blocked"]))
+        log(f"tri_blocked_min: {n}-order eigensolve: by column {t['column']:.1f} ms, blocked {t['blocked']:.1f} ms")
+    best = _BLOCKED_OFF
+    for n, tc, tb in reversed(rows):
+        if tb < tc:
+            best = n
+        else:
+            break
+    eng.set_tunable("tri_blocked_min", best)
+    return best, rows
+
+
 def _device_name(device: int) -> str:
     try:
         import torch
@@ -173,6 +209,7 @@ def autotune(path: str | None = None, device: int = 0, log=print) -> dict:
         lds_n, jt = tune_jacobi_lds(eng, log)
         graphs, gt = tune_graphs(eng, log)
         pblock, pt = tune_pairs_block(eng, log)
+        bmin, bt = tune_blocked(eng, log)
     finally:
         eng.close()
     name = _device_name(device)
@@ -189,9 +226,13 @@ def autotune(path: str | None = None, device: int = 0, log=print) -> dict:
     lines.append(f"eig_graphs {graphs}")
     lines.append("# pairs_block: core Hamiltonian of 2000 waters (ms): " + ", ".join(f"{k}: {v:.1f}" for k, v in pt.items()))
     lines.append(f"pairs_block {pblock}")
+    lines.append("# tri_blocked_min: large eigensolve (ms): order by-column blocked")
+    lines += [f"#   {n} {a:.1f} {b:.1f}" for n, a, b in bt]
+    lines.append(f"tri_blocked_min {bmin}")
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")
-    return {"route_large_ao": route, "tri_max": tri, "jacobi_lds_n": lds_n, "eig_graphs": graphs, "pairs_block": pblock, "path": path}
+    return {"route_large_ao": route, "tri_max": tri, "jacobi_lds_n": lds_n, "eig_graphs": graphs, "pairs_block": pblock,
+            "tri_blocked_min": bmin, "path": path}
 
 
 if __name__ == "__main__":

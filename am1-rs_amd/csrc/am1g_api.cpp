@@ -51,6 +51,7 @@ bool set_tunable(Tunables& t, const std::string& name, double v) {
     else if (name == "serve_large_batch") t.serve_large_batch = std::max(1, (int)v);
     else if (name == "tri_max") t.tri_max = std::min(256, std::max(0, (int)v));
     else if (name == "pairs_block") t.pairs_block = (v == 64.0 || v == 128.0) ? (int)v : 256;
+    else if (name == "tri_blocked_min") t.tri_blocked_min = std::max(3, (int)v);
     else return false;
     return true;
 }
@@ -62,6 +63,7 @@ bool get_tunable(const Tunables& t, const std::string& name, double& v) {
     else if (name == "serve_large_batch") v = t.serve_large_batch;
     else if (name == "tri_max") v = t.tri_max;
     else if (name == "pairs_block") v = t.pairs_block;
+    else if (name == "tri_blocked_min") v = t.tri_blocked_min;
     else return false;
     return true;
 }
@@ -344,7 +346,10 @@ int am1g_set_tunable(am1g_ctx* ctx, const char* name, double value) {
     if (!ctx || !name) return am1g_set_error(ctx, AM1G_INVALID_INPUT, "null argument");
     std::lock_guard<std::mutex> lock(ctx->mu);
     if (!set_tunable(ctx->tun, name, value)) return am1g_set_error(ctx, AM1G_INVALID_INPUT, "unknown tunable %s", name);
-    if (ctx->eig) ctx->eig->use_graphs = ctx->tun.eig_graphs != 0;
+    if (ctx->eig) {
+        ctx->eig->use_graphs = ctx->tun.eig_graphs != 0;
+        ctx->eig->blocked_min = ctx->tun.tri_blocked_min;
+    }
     return AM1G_OK;
 }
 

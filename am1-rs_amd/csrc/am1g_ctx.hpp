@@ -72,6 +72,7 @@ struct Tunables {
     int serve_large_batch = 2;           // a server lane batches queued grid-path requests from this many
     int tri_max = 256;                   // batched kernel: tridiagonal method up to this order, Jacobi above
     int pairs_block = 256;               // threads per workgroup of the pair-integral kernel
+    int tri_blocked_min = 1 << 30;       // large eigensolver: tridiagonalization in panels from this order
 };
 
 struct am1g_ctx {

@@ -236,6 +236,267 @@ __global__ __launch_bounds__(kNextT) void k_tridiag_next_wide(int n, int c, doub
     }
 }
 
+// ---------------------------------------------------------------- blocked tridiagonalization
+// The same reduction in panels of kPanel columns (the dsytrd/dlatrd scheme; rows stand for
+// columns since A is symmetric). Within a panel the trailing matrix stays as it was at the panel
+// start (A_ps): the current matrix is A_ps - V W^T - W V^T over the panel's columns so far, with
+// V the reflectors and W the vectors w_q = p_q - alpha_q v_q (column-major n x kPanel, global
+// row indices). Column c's reflector is v_c = e_{c+1} + scale_c r, r the current row c past c+1.
+// Per column two grid kernels and no other synchronization:
+//   k_btri_symv  (one wavefront per trailing row x) finishes the sums the previous k_btri_row
+//                left per workgroup (every workgroup the same way, so all agree): |r|^2, the
+//                reflector, and (workgroup 0) s_V = V^T v_c, s_W = W^T v_c; y_x = (A_ps v_c)_x from
+//                the unscaled r (the row loads need none of those sums), and v_x y_x per workgroup;
+//   k_btri_row   (one thread per index x) alpha_c = tau_c^2 / 2 (v^T A_ps v - 2 s_V . s_W) from
+//                those partials, p_x = tau_c (y_x - V_x s_W - W_x s_V), w_x = p_x - alpha_c v_x,
+//                the current row c+1 (x >= c+1), and per workgroup |r|^2, V^T r, W^T r for the
+//                next column.
+// After each panel one rank-2k update of the trailing matrix (two matrix-core GEMMs). The
+// trailing matrix is read once per column instead of read and written, and every cross-workgroup
+// sum is a fixed-order sum of per-workgroup partials, so results do not depend on scheduling.
+constexpr int kPanel = 32;
+constexpr int kRowPart = 2 * kPanel + 2;  // per k_btri_row workgroup: V^T r, W^T r, |r|^2
+constexpr int kRowWg = 128;
+
+// Scalars of column c written by workgroup 0 of k_btri_symv: tau_c, scale_c, s_V, s_W.
+enum BtriScal { kBsTau = 0, kBsScale = 1, kBsV = 8, kBsW = 8 + kPanel, kBsLen = 8 + 2 * kPanel };
+
+// Reflector of row c from |r|^2, summed over the previous k_btri_row's workgroups in a fixed
+// order by every wavefront that needs it (no barrier): returns tau, sets scale and beta.
+__device__ __forceinline__ double btri_reflector(int c, int nb_prev, const double* rbuf, const double* rpart,
+                                                 double& scale, double& beta) {
+    const int lane = threadIdx.x & 63;
+    double ss = 0.0;
+    for (int b = lane; b < nb_prev; b += 64) ss += rpart[(size_t)b * kRowPart + 2 * kPanel];
+    for (int off = 32; off > 0; off >>= 1) ss += __shfl_xor(ss, off, 64);
+    const double ax = rbuf[c + 1];
+    double tt;
+    if (ss == 0.0) {
+        beta = ax;
+        tt = 0.0;
+        scale = 0.0;
+    } else {
+        beta = -copysign(sqrt(ax * ax + ss), ax);
+        tt = (beta - ax) / beta;
+        scale = 1.0 / (ax - beta);
+    }
+    return tt;
+}
+
+// One wavefront per trailing row x in (c, n), four per workgroup. jj = c's column in its panel.
+__global__ __launch_bounds__(256) void k_btri_symv(int n, int c, int jj, int nb_prev, double* A, const double* rbuf,
+                                                   const double* rpart, double* V, const double* W, double* ybuf,
+                                                   double* vpart, double* scal, double* d, double* e, double* tau) {
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int x = c + 1 + blockIdx.x * 4 + w;
+    __shared__ double red[4];
+    // the row's sums first: they need neither the reflector's scale nor the panel sums
+    double yr = 0.0, ax = 0.0;
+    if (x < n) {
+        const double* row = A + (size_t)x * n;
+        int j = c + 2 + lane;
+        for (; j + 15 * 64 < n; j += 16 * 64) {
+            double a[16], r[16];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                a[u] = row[j + 64 * u];
+                r[u] = rbuf[j + 64 * u];
+            }
+#pragma unroll
+            for (int u = 0; u < 16; ++u) yr = fma(a[u], r[u], yr);
+        }
+        for (; j < n; j += 64) yr = fma(row[j], rbuf[j], yr);
+        ax = row[c + 1];
+    }
+    double scale, beta;
+    const double tc = btri_reflector(c, nb_prev, rbuf, rpart, scale, beta);
+    if (blockIdx.x == 0 && w == 0) {
+        // s_V = V^T v_c, s_W = W^T v_c of v_c = e_{c+1} + scale r (the partials hold V^T r and
+        // W^T r from row c+2 on), lane q < kPanel for V and kPanel + q for W
+        double acc = 0.0;
+        for (int b = 0; b < nb_prev; ++b) acc += rpart[(size_t)b * kRowPart + lane];
+        const int q = lane < kPanel ? lane : lane - kPanel;
+        const double* X = lane < kPanel ? V : W;
+        const double sq = q < jj ? X[(size_t)q * n + c + 1] + scale * acc : 0.0;
+        scal[(lane < kPanel ? kBsV : kBsW) + q] = sq;
+        if (lane == 0) {
+            scal[kBsTau] = tc;
+            scal[kBsScale] = scale;
+            d[c] = rbuf[c];
+            e[c] = beta;
+            tau[c] = tc;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) yr += __shfl_xor(yr, off, 64);
+    double vy = 0.0;
+    if (x < n) {
+        const double y = scale * yr + ax;
+        const double vx = x == c + 1 ? 1.0 : scale * rbuf[x];
+        if (lane == 0) {
+            ybuf[x] = y;
+            V[(size_t)jj * n + x] = vx;
+            if (x >= c + 2) A[(size_t)c * n + x] = vx;
+        }
+        vy = vx * y;
+    }
+    if (lane == 0) red[w] = vy;
+    __syncthreads();
+    if (t == 0) vpart[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// One thread per index x in [c+1, n) of row c+1 (first: c = -1, row 0 of A as it is). Every
+// wavefront forms alpha_c and w_c at row c+1 itself (fixed-order sums, so all agree).
+__global__ __launch_bounds__(kRowWg) void k_btri_row(int n, int c, int jj, int first, int nb_sym, const double* A,
+                                                     const double* ybuf, const double* vpart, const double* scal,
+                                                     const double* V, double* W, double* rbuf, double* rpart) {
+    constexpr int kWaves = kRowWg / 64;
+    const int r = c + 1;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    __shared__ double red[kWaves];
+    __shared__ double wsum[kWaves][2 * kPanel];
+    const int x = r + blockIdx.x * kRowWg + t;
+    // every load first, without branches on the panel column
+    double vq[kPanel], wq[kPanel];
+#pragma unroll
+    for (int q = 0; q < kPanel; ++q) {
+        vq[q] = 0.0;
+        wq[q] = 0.0;
+    }
+    double rx = 0.0, yx = 0.0;
+    const bool in = x < n;
+    if (in) {
+        rx = A[(size_t)r * n + x];
+        if (!first) {
+            yx = ybuf[x];
+#pragma unroll
+            for (int q = 0; q < kPanel; ++q) {
+                vq[q] = V[(size_t)q * n + x];
+                wq[q] = W[(size_t)q * n + x];
+            }
+        }
+    }
+    double alpha = 0.0, tc = 0.0, w_r = 0.0, vrl = 0.0, wrl = 0.0, svl = 0.0, swl = 0.0;
+    if (!first) {
+        // lane q < kPanel holds column q's s_V, s_W and row c+1's V, W entries
+        const int q = lane & (kPanel - 1);
+        svl = q < jj ? scal[kBsV + q] : 0.0;
+        swl = q < jj ? scal[kBsW + q] : 0.0;
+        vrl = q < jj ? V[(size_t)q * n + r] : (q == jj ? 1.0 : 0.0);
+        wrl = q < jj ? W[(size_t)q * n + r] : 0.0;
+        tc = scal[kBsTau];
+        const double yr = ybuf[r];
+        double acc = 0.0;
+        int b = lane;
+        for (; b + 7 * 64 < nb_sym; b += 8 * 64) {
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = vpart[b + 64 * u];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) acc += v[u];
+        }
+        for (; b < nb_sym; b += 64) acc += vpart[b];
+        double svw = lane < kPanel ? svl * swl : 0.0;
+        double cqr = lane < kPanel ? vrl * swl + wrl * svl : 0.0;
+        for (int off = 32; off > 0; off >>= 1) {
+            acc += __shfl_xor(acc, off, 64);
+            svw += __shfl_xor(svw, off, 64);
+            cqr += __shfl_xor(cqr, off, 64);
+        }
+        alpha = 0.5 * tc * tc * (acc - 2.0 * svw);
+        w_r = tc * (yr - cqr) - alpha;  // w_c at row c+1, where v_c = 1
+    }
+#pragma unroll
+    for (int q = 0; q < kPanel; ++q) {
+        vq[q] = q <= jj ? vq[q] : 0.0;
+        wq[q] = q < jj ? wq[q] : 0.0;
+    }
+    double ss = 0.0;
+    if (!first) {
+        // every lane takes part (the shuffles read lanes 0..kPanel-1); lanes past n hold zeros
+        double cq = 0.0;
+#pragma unroll
+        for (int q = 0; q < kPanel; ++q) {
+            const double svq = __shfl(svl, q, 64), swq = __shfl(swl, q, 64);
+            cq += vq[q] * swq + wq[q] * svq;
+        }
+        double vx = 0.0;
+#pragma unroll
+        for (int q = 0; q < kPanel; ++q)
+            if (q == jj) vx = vq[q];
+        const double wx = x == r ? w_r : tc * (yx - cq) - alpha * vx;
+        if (in) W[(size_t)jj * n + x] = wx;
+        double rc = wx + w_r * vx;
+#pragma unroll
+        for (int q = 0; q < kPanel; ++q) {
+            const double vrq = __shfl(vrl, q, 64), wrq = __shfl(wrl, q, 64);
+            if (q < jj) rc += vrq * wq[q] + wrq * vq[q];
+            if (q == jj) wq[q] = wx;
+        }
+        rx -= rc;
+    }
+    if (in) rbuf[x] = rx;
+    if (in && x >= r + 2) {
+        ss = rx * rx;
+#pragma unroll
+        for (int q = 0; q < kPanel; ++q) {
+            vq[q] *= rx;
+            wq[q] *= rx;
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < kPanel; ++q) {
+            vq[q] = 0.0;
+            wq[q] = 0.0;
+        }
+    }
+    // per-workgroup partials [V^T r | W^T r] (64 entries, recursive halving: lane l ends with
+    // entry l), then across the wavefronts; and |r|^2
+#pragma unroll
+    for (int k = 0; k < kPanel; ++k) {
+        const bool hi = (lane & 32) != 0;
+        const double recv = __shfl_xor(hi ? vq[k] : wq[k], 32, 64);
+        vq[k] = (hi ? wq[k] : vq[k]) + recv;
+    }
+#pragma unroll
+    for (int step = 16; step >= 1; step >>= 1) {
+#pragma unroll
+        for (int k = 0; k < step; ++k) {
+            const bool hi = (lane & step) != 0;
+            const double recv = __shfl_xor(hi ? vq[k] : vq[k + step], step, 64);
+            vq[k] = (hi ? vq[k + step] : vq[k]) + recv;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) ss += __shfl_xor(ss, off, 64);
+    wsum[w][lane] = vq[0];
+    if (lane == 0) red[w] = ss;
+    __syncthreads();
+    double* bp = rpart + (size_t)blockIdx.x * kRowPart;
+    if (t < 2 * kPanel) {
+        double v = 0.0;
+        for (int k = 0; k < kWaves; ++k) v += wsum[k][t];
+        bp[t] = v;
+    }
+    if (t == 2 * kPanel) {
+        double v = 0.0;
+        for (int k = 0; k < kWaves; ++k) v += red[k];
+        bp[t] = v;
+    }
+    if (t == 2 * kPanel + 1) bp[t] = 0.0;
+}
+
+// After the last column (c = n-3): the last two diagonal entries and the last off-diagonal one,
+// from row n-2 (rbuf) and the panel's updates of A[n-1][n-1].
+__global__ __launch_bounds__(64) void k_btri_tail(int n, int jj, const double* A, const double* rbuf, const double* V,
+                                                  const double* W, double* d, double* e, double* tau) {
+    if (threadIdx.x != 0) return;
+    d[n - 2] = rbuf[n - 2];
+    e[n - 2] = rbuf[n - 1];
+    tau[n - 2] = 0.0;
+    double dl = A[(size_t)(n - 1) * n + n - 1];
+    for (int q = 0; q <= jj; ++q) dl -= 2.0 * V[(size_t)q * n + n - 1] * W[(size_t)q * n + n - 1];
+    d[n - 1] = dl;
+}
+
 // ---------------------------------------------------------------- divide and conquer
 // Leaves: the tridiagonal block [s, s + m) with the split-modified diagonal, diagonalized by the
 // workgroup Jacobi solver; eigenvectors go to the columns of Q (column-major, ld n) in ascending
@@ -499,9 +760,56 @@ hipError_t eig_large(EigWork& w, int n, double* A, double* evals, double* Ct, hi
     double* p = w.pbuf.as<double>();
     double* Q = w.Q.as<double>();
 
-    // ---- 1. tridiagonalization: 2 n dependent kernels; captured once per order and buffer set
+    // ---- 1. tridiagonalization: 2-3 n dependent kernels; captured once per order and buffer set
     // as a hipGraph and replayed (kernels only, so the graph holds no memset or copy nodes)
+    const bool blocked = n >= std::max(w.blocked_min, 8);
+    double *bV = nullptr, *bW = nullptr, *bybuf = nullptr, *bvpart = nullptr, *brow = nullptr, *brpart = nullptr,
+           *bscal = nullptr;
+    if (blocked) {
+        EIG_TRY(w.bV.reserve(sizeof(double) * (size_t)n * kPanel));
+        EIG_TRY(w.bW.reserve(sizeof(double) * (size_t)n * kPanel));
+        EIG_TRY(w.bybuf.reserve(sizeof(double) * n));
+        EIG_TRY(w.bvpart.reserve(sizeof(double) * ((size_t)n / 4 + 1)));
+        EIG_TRY(w.brow.reserve(sizeof(double) * n));
+        EIG_TRY(w.brpart.reserve(sizeof(double) * ((size_t)n / kRowWg + 1) * kRowPart));
+        EIG_TRY(w.bscal.reserve(sizeof(double) * kBsLen));
+        bV = w.bV.as<double>();
+        bW = w.bW.as<double>();
+        bybuf = w.bybuf.as<double>();
+        bvpart = w.bvpart.as<double>();
+        brow = w.brow.as<double>();
+        brpart = w.brpart.as<double>();
+        bscal = w.bscal.as<double>();
+    }
+    auto blocked_launches = [&]() {
+        int nb_prev = (int)blocks(n, kRowWg);
+        hipLaunchKernelGGL(k_btri_row, dim3(nb_prev), dim3(kRowWg), 0, st, n, -1, 0, 1, 0, A, bybuf, bvpart, bscal, bV,
+                           bW, brow, brpart);
+        int jj = 0;
+        for (int c = 0; c + 2 < n; ++c) {
+            jj = c % kPanel;
+            const int m = n - c - 1;
+            const int nb_sym = (int)blocks(m, 4);
+            hipLaunchKernelGGL(k_btri_symv, dim3(nb_sym), dim3(256), 0, st, n, c, jj, nb_prev, A, brow, brpart, bV, bW,
+                               bybuf, bvpart, bscal, d, e, tau);
+            nb_prev = (int)blocks(m, kRowWg);
+            hipLaunchKernelGGL(k_btri_row, dim3(nb_prev), dim3(kRowWg), 0, st, n, c, jj, 0, nb_sym, A, bybuf, bvpart,
+                               bscal, bV, bW, brow, brpart);
+            if (jj == kPanel - 1 && c + 3 < n) {
+                // panel done: A[k1:, k1:] -= V W^T + W V^T (column-major view of the symmetric block)
+                const int k1 = c + 1, mt = n - k1;
+                double* At = A + (size_t)k1 * n + k1;
+                (void)gemm_f64(st, false, true, mt, mt, kPanel, -1.0, bV + k1, n, bW + k1, n, 1.0, At, n);
+                (void)gemm_f64(st, false, true, mt, mt, kPanel, -1.0, bW + k1, n, bV + k1, n, 1.0, At, n);
+            }
+        }
+        hipLaunchKernelGGL(k_btri_tail, dim3(1), dim3(64), 0, st, n, jj, A, brow, bV, bW, d, e, tau);
+    };
     auto tridiag_launches = [&]() {
+        if (blocked) {
+            blocked_launches();
+            return;
+        }
         hipLaunchKernelGGL(k_reflect0, dim3(1), dim3(256), 0, st, n, A, d, e, tau, v);
         for (int c = 0; c + 2 < n; ++c) {
             hipLaunchKernelGGL(k_tridiag_rows, dim3(blocks(n - c - 1, 4)), dim3(256), 0, st, n, c, A, tau, v, p);
@@ -516,10 +824,11 @@ hipError_t eig_large(EigWork& w, int n, double* A, double* evals, double* Ct, hi
     } else if (!w.use_graphs || n < 64) {
         tridiag_launches();
     } else {
-        const void* key[6] = {A, d, e, tau, v, p};
+        const void* key[16] = {A, d, e, tau, v, p, bV, bW, bybuf, bvpart, brow, brpart, bscal, nullptr, nullptr,
+                               nullptr};
         TridiagGraph* hit = nullptr;
         for (auto& g : w.graphs)
-            if (g.n == n && std::equal(key, key + 6, g.key)) hit = &g;
+            if (g.n == n && std::equal(key, key + 16, g.key)) hit = &g;
         if (!hit) {
             hipGraph_t graph = nullptr;
             EIG_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -536,7 +845,7 @@ hipError_t eig_large(EigWork& w, int n, double* A, double* evals, double* Ct, hi
             }
             TridiagGraph g;
             g.n = n;
-            std::copy(key, key + 6, g.key);
+            std::copy(key, key + 16, g.key);
             g.exec = exec;
             w.graphs.push_back(g);
             hit = &w.graphs.back();

@@ -32,7 +32,7 @@ struct EigBuf {
 // buffers, replayed as one graph launch.
 struct TridiagGraph {
     int n = 0;
-    const void* key[6] = {};  // A, d, e, tau, v, p
+    const void* key[16] = {};  // every device buffer the captured launches read or write
     hipGraphExec_t exec = nullptr;
 };
 
@@ -44,8 +44,10 @@ struct EigWork {
     EigBuf V, Tm, X, Y, G;                    // back-transformation blocks
     EigBuf rot, idx, flt, lam;                // merge tables
     EigBuf gw;                                // split-K partial products of the GEMMs
+    EigBuf bV, bW, bybuf, bvpart, brow, brpart, bscal;  // blocked tridiagonalization
     std::vector<double> h_d, h_e, h_lam;
     bool use_graphs = true;                   // replay the tridiagonalization as a hipGraph
+    int blocked_min = 1 << 30;                // orders from which the tridiagonalization runs in panels
     std::vector<TridiagGraph> graphs;         // at most kMaxGraphs, oldest dropped first
     static constexpr size_t kMaxGraphs = 4;
     ~EigWork() {

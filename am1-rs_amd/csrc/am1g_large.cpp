@@ -25,6 +25,7 @@ struct LargeBuffers {
 
 int am1g_large_setup(am1g_ctx* ctx) {
     if (!ctx->eig) ctx->eig = new EigWork();
+    ctx->eig->blocked_min = ctx->tun.tri_blocked_min;
     ctx->eig->use_graphs = ctx->tun.eig_graphs != 0;
     if (!ctx->large) ctx->large = new LargeBuffers();
     return AM1G_OK;
@@ -249,7 +250,10 @@ static int large_uhf(am1g_ctx* ctx, const am1g_request& rq, const MolDesc& md, c
         // a test 100x tighter; should that one stall (a near-degenerate open shell can hold the
         // error on a plateau), ten iterations after the p_tol test is met end the run as well.
         const bool conv_loose = de < md.e_tol && (dsum < md.p_tol || err < 1.0e-7);
-        const bool conv_tight = de < md.e_tol && (dsum < 1.0e-2 * md.p_tol || err < 1.0e-7);
+        // a density test averaged over n^2 entries weakens with n: at hundreds of basis functions the
+        // mixed run also needs the commutator below 1e-6 before it stops (orbital energies within
+        // ~1e-7 eV of the fixed point)
+        const bool conv_tight = de < md.e_tol && ((dsum < 1.0e-2 * md.p_tol && err < 1.0e-6) || err < 1.0e-7);
         if (conv_loose && fp64_iters >= 2) ++loose_iters;
         if (fp64_iters >= 2 && (md.precision == 1 ? (conv_tight || loose_iters > 10) : conv_loose)) {
             converged = 1;
@@ -646,7 +650,8 @@ int am1g_large_scf(am1g_ctx* ctx, const am1g_request& rq, const Prepared& prep, 
         // a test 100x tighter; should that one stall (a near-degenerate open shell can hold the
         // error on a plateau), ten iterations after the p_tol test is met end the run as well.
         const bool conv_loose = de < md.e_tol && (dp < md.p_tol || err < 1.0e-7);
-        const bool conv_tight = de < md.e_tol && (dp < 1.0e-2 * md.p_tol || err < 1.0e-7);
+        // see the UHF loop: the mixed run also needs the commutator below 1e-6
+        const bool conv_tight = de < md.e_tol && ((dp < 1.0e-2 * md.p_tol && err < 1.0e-6) || err < 1.0e-7);
         if (conv_loose && fp64_iters >= 2) ++loose_iters;
         if (fp64_iters >= 2 && (md.precision == 1 ? (conv_tight || loose_iters > 10) : conv_loose)) {
             converged = 1;

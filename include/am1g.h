@@ -248,6 +248,10 @@ int am1g_matmul(am1g_ctx* ctx, int trans_a, int trans_b, int m, int n, int k, do
  *                      default 2
  *   pairs_block        threads per workgroup of the pair-integral kernel (64, 128 or 256);
  *                      default 256
+ *   tri_blocked_min    the large eigensolver reduces orders from this one to tridiagonal form
+ *                      in panels of 32 columns (the trailing matrix read once per column, a
+ *                      matrix-core rank-64 update per panel) instead of one column at a time;
+ *                      default 2^30 (off)
  * No reference counterpart. AM1G_INVALID_INPUT for an unknown name. */
 int am1g_set_tunable(am1g_ctx* ctx, const char* name, double value);
 int am1g_get_tunable(const am1g_ctx* ctx, const char* name, double* value);

@@ -79,3 +79,28 @@ def test_non_finite_input_is_a_linear_algebra_error(engine):
     a[1, 2] = a[2, 1] = np.nan
     with pytest.raises(ValueError, match="LinearAlgebra"):
         engine.eigh(a)
+
+
+@pytest.mark.parametrize("n", [8, 63, 257, 600, 1100])
+def test_blocked_tridiagonalization(n):
+    """The panel-blocked tridiagonalization (tunable ``tri_blocked_min``) on its own engine:
+    random matrices, a spectrum with exact 4-fold degeneracy (heavy deflation) and a diagonal
+    matrix, against numpy, and against the column-at-a-time reduction of the same matrix."""
+    import am1g
+
+    eng = am1g.Engine(0)
+    try:
+        eng.set_tunable("tri_blocked_min", 8)
+        rng = np.random.default_rng(1000 + n)
+        b = rng.normal(size=(n, n))
+        a = 0.5 * (b + b.T)
+        w, _ = _check(eng, a)
+        q, _ = np.linalg.qr(rng.normal(size=(n, n)))
+        lam = np.repeat(np.arange(n // 4 + 1, dtype=float), 4)[:n]
+        _check(eng, (q * lam) @ q.T)
+        _check(eng, np.diag(rng.normal(size=n)))
+        eng.set_tunable("tri_blocked_min", 1 << 30)
+        w2, _ = eng.eigh(a)
+        assert np.abs(w - w2).max() <= 1e-11 * max(1.0, np.abs(w).max())
+    finally:
+        eng.close()
