@@ -137,6 +137,7 @@ pub mod raw {
         pub max_batch: c_int,
         pub memory_budget_mb: c_double,
         pub max_wait_ms: c_double,
+        pub continuous: c_int,
     }
 
     #[repr(C)]

@@ -107,6 +107,7 @@ class ServeOptions(ctypes.Structure):
         ("max_batch", ctypes.c_int),
         ("memory_budget_mb", ctypes.c_double),
         ("max_wait_ms", ctypes.c_double),
+        ("continuous", ctypes.c_int),
     ]
 
 
@@ -590,7 +591,10 @@ class Server:
     completion)."""
 
     def __init__(self, device: int = 0, lanes: int | None = None, max_batch: int | None = None,
-                 memory_budget_mb: float | None = None, max_wait_ms: float | None = None):
+                 memory_budget_mb: float | None = None, max_wait_ms: float | None = None,
+                 continuous: bool | None = None):
+        """``continuous`` (default True): one persistent service launch takes requests as they
+        arrive; False: lanes of batched launches (``lanes``, ``max_batch``, ``max_wait_ms``)."""
         self._L = load()
         op = ServeOptions()
         self._L.am1g_serve_defaults(ctypes.byref(op))
@@ -602,6 +606,8 @@ class Server:
             op.memory_budget_mb = memory_budget_mb
         if max_wait_ms is not None:
             op.max_wait_ms = max_wait_ms
+        if continuous is not None:
+            op.continuous = int(bool(continuous))
         self._sv = ctypes.c_void_p()
         st = self._L.am1g_serve_create(int(device), ctypes.byref(op), ctypes.byref(self._sv))
         if st != 0:

@@ -690,26 +690,39 @@ __global__ __launch_bounds__(256) void k_build_v(int n, int k0, int nb, const do
 }
 
 // Compact WY factor T (nb x nb upper triangular, column-major) from G = Vb^T Vb (forward,
-// columnwise): T[j][j] = tau_j, T[0:j, j] = -tau_j T[0:j, 0:j] G[0:j, j]. T and G live in LDS.
+// columnwise), by recursive doubling: T_jj = tau_j, and two adjacent diagonal blocks A, B of a
+// finished level combine as T_AB = -T_AA G_AB T_BB (the product (I - V_A T_AA V_A^T)(I - V_B T_BB
+// V_B^T) = I - [V_A V_B] [[T_AA, T_AB], [0, T_BB]] [V_A V_B]^T). log2(nb) levels of two small
+// products each instead of nb dependent column steps.
 __global__ __launch_bounds__(256) void k_larft(int nb, int k0, const double* tau, const double* G, double* T) {
-    __shared__ double sT[kNb * kNb], sG[kNb * kNb];
+    __shared__ double sT[kNb * kNb], sG[kNb * kNb], sX[kNb * kNb];
     const int t = threadIdx.x;
     for (int idx = t; idx < nb * nb; idx += 256) {
-        sT[idx] = 0.0;
+        const int i = idx % nb, j = idx / nb;
+        sT[idx] = i == j ? tau[k0 + j] : 0.0;
         sG[idx] = G[idx];
     }
     __syncthreads();
-    for (int j = 0; j < nb; ++j) {
-        const double tj = tau[k0 + j];
-        double val = 0.0;
-        if (t < j) {
+    for (int sz = 1; sz < nb; sz *= 2) {
+        // X_AB = G_AB T_BB for every pair (A = [i0, i0+sz), B = [i0+sz, i0+2sz) cut at nb)
+        for (int idx = t; idx < nb * nb; idx += 256) {
+            const int a = idx % nb, b = idx / nb;
+            const int i0 = a / (2 * sz) * (2 * sz), b0 = i0 + sz;
+            if (a >= b0 || b < b0 || b >= min(b0 + sz, nb)) continue;
             double acc = 0.0;
-            for (int l = t; l < j; ++l) acc += sT[t + l * nb] * sG[l + j * nb];
-            val = -tj * acc;
+            for (int l = b0; l <= b; ++l) acc += sG[a + l * nb] * sT[l + b * nb];
+            sX[idx] = acc;
         }
         __syncthreads();
-        if (t < j) sT[t + j * nb] = val;
-        if (t == 0) sT[j + j * nb] = tj;
+        // T_AB = -T_AA X_AB
+        for (int idx = t; idx < nb * nb; idx += 256) {
+            const int a = idx % nb, b = idx / nb;
+            const int i0 = a / (2 * sz) * (2 * sz), b0 = i0 + sz;
+            if (a >= b0 || b < b0 || b >= min(b0 + sz, nb)) continue;
+            double acc = 0.0;
+            for (int l = a; l < b0; ++l) acc += sT[a + l * nb] * sX[l + b * nb];
+            sT[idx] = -acc;
+        }
         __syncthreads();
     }
     for (int idx = t; idx < nb * nb; idx += 256) T[idx] = sT[idx];

@@ -21,6 +21,25 @@ constexpr int kRouteLargeAo = 192;
 // Dynamic LDS of the per-molecule kernel (the static part is the molecule tables).
 constexpr int kLdsBytes = 19200 * 8;
 
+// Continuous admission (the server's service launch): one request of the host-pinned ring,
+// written by the host before it publishes the request's ticket. The pool offsets in md point
+// into the service launch's device pools.
+struct SvcSlot {
+    MolDesc md;
+    int z[kMaxAtoms];
+    int aoff[kMaxAtoms];
+    int acls[kMaxAtoms];
+    int alist[kMaxAtoms];
+    double xyz[3 * kMaxAtoms];
+};
+// One request's results, written by the service launch into the host-pinned ring.
+struct SvcOut {
+    MolResult res;
+    double charges[kMaxAtoms];
+    double mo[kMaxBatchAo];
+    double mob[kMaxBatchAo];
+};
+
 struct BatchArgs {
     const MolDesc* mols;
     int n_mols;
@@ -53,6 +72,23 @@ struct BatchArgs {
     double* charges_h;
     double* mo_e_h;
     double* mo_eb_h;
+    // Service launch (all null otherwise): workgroups take requests from the host-pinned ring as
+    // the host publishes them (ticket t in slot t % svc_cap) and compute each request's pair
+    // integrals themselves; they leave once the host sets *svc_stop, or once the launch has been
+    // idle for svc_idle_us (a deadline all workgroups share, so they leave together). The atom
+    // tables (z, xyz, aoff, acls, alist) are then written by the kernel.
+    const SvcSlot* svc_in;
+    SvcOut* svc_out;
+    long long* svc_done;             // per slot: ticket + 1 once that ticket's results are out
+    const long long* svc_published;  // tickets below this one are written
+    const int* svc_stop;
+    long long* svc_next;             // device: the next ticket to take
+    int* svc_busy;                   // device: workgroups working on a request now
+    unsigned long long* svc_last;    // device: device clock of the last start, take or finish
+    int* svc_alive;                  // host-pinned: workgroups of service launches that have not left
+    int svc_cap;
+    int svc_idle_us;                 // workgroups leave together once nothing has started, been taken or
+                                     // finished for this long and none is busy
 };
 
 hipError_t launch_scf_batch(const BatchArgs& a, int precision, int grid, size_t lds_bytes, hipStream_t stream);

@@ -86,6 +86,100 @@ __device__ __forceinline__ BlockRef block_of(const MolShared& ms, const double* 
 // block into the integral pool, its resonance block 1/2 (beta_mu + beta_nu) S_mu_nu into both
 // triangles of H_core (each off-diagonal atom block belongs to exactly one pair), and its
 // core-core repulsion (hamiltonian.rs:281-344, repulsion.rs:144-167).
+// One atom pair k_all (molecule-local: heavy-heavy, heavy-H, H-H in class order) of molecule md:
+// writes its integral block and resonance block, returns its core-core term.
+__device__ __forceinline__ double pair_one(const BatchArgs& g, const MolDesc& md, int64_t k_all) {
+    const DevElem* el = g.elems + md.method * kMaxZ;
+    const double* xyz = g.xyz + md.atom_off * 3;
+    const int* z = g.z + md.atom_off;
+    const int* alist = g.alist + md.atom_off;
+    const int* aoff = g.aoff + md.atom_off;
+    const int nh = md.n_heavy, nH = md.n_h, nao = md.nao;
+    const int64_t n_hh = (int64_t)nh * (nh - 1) / 2, n_hH = (int64_t)nh * nH;
+    double* H = g.mats + md.mat_off + kH * (size_t)nao * nao;
+    double* wpool = g.wpool + md.w_off;
+    int a, b;
+    if (k_all < n_hh) {
+        int i, j;
+        tri_decode(k_all, nh, i, j);
+        a = alist[i];
+        b = alist[j];
+    } else if (k_all < n_hh + n_hH) {
+        int64_t k = k_all - n_hh;
+        a = alist[(int)(k / nH)];
+        b = alist[nh + (int)(k % nH)];
+    } else {
+        int i, j;
+        tri_decode(k_all - n_hh - n_hH, nH, i, j);
+        a = alist[nh + i];
+        b = alist[nh + j];
+    }
+    const DevElem& ea = el[z[a]];
+    const DevElem& eb = el[z[b]];
+    double d[3] = {xyz[3 * b] - xyz[3 * a], xyz[3 * b + 1] - xyz[3 * a + 1], xyz[3 * b + 2] - xyz[3 * a + 2]};
+    double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    const int oa = aoff[a], ob = aoff[b];
+    if (md.far_cut > 0.0 && r > md.far_cut) {
+        // beyond the far-field cutoff the pair is out of the pair list (neighbors.rs:90-101):
+        // no integrals, no resonance, no core-core term; its net charges interact through
+        // the far-field potential instead
+        if (k_all < n_hh) {
+            double* dst = wpool + k_all * 100;
+            for (int q = 0; q < 100; ++q) dst[q] = 0.0;
+            for (int i2 = 0; i2 < 4; ++i2)
+                for (int j2 = 0; j2 < 4; ++j2) {
+                    H[(size_t)(oa + i2) * nao + ob + j2] = 0.0;
+                    H[(size_t)(ob + j2) * nao + oa + i2] = 0.0;
+                }
+        } else if (k_all < n_hh + n_hH) {
+            double* dst = wpool + n_hh * 100 + (k_all - n_hh) * 10;
+            for (int q = 0; q < 10; ++q) dst[q] = 0.0;
+            for (int i2 = 0; i2 < 4; ++i2) {
+                H[(size_t)(oa + i2) * nao + ob] = 0.0;
+                H[(size_t)ob * nao + oa + i2] = 0.0;
+            }
+        } else {
+            wpool[n_hh * 100 + n_hH * 10 + (k_all - n_hh - n_hH)] = 0.0;
+            H[(size_t)oa * nao + ob] = 0.0;
+            H[(size_t)ob * nao + oa] = 0.0;
+        }
+        return 0.0;
+    }
+    double S[16];
+    overlap_block(ea, eb, d, S);
+    if (k_all < n_hh) {
+        heavy_heavy_block(ea, eb, d, wpool + k_all * 100);
+#pragma unroll
+        for (int i2 = 0; i2 < 4; ++i2)
+#pragma unroll
+            for (int j2 = 0; j2 < 4; ++j2) {
+                double v = 0.5 * ((i2 == 0 ? ea.bs : ea.bp) + (j2 == 0 ? eb.bs : eb.bp)) * S[i2 * 4 + j2];
+                H[(size_t)(oa + i2) * nao + ob + j2] = v;
+                H[(size_t)(ob + j2) * nao + oa + i2] = v;
+            }
+    } else if (k_all < n_hh + n_hH) {
+        double w[10];
+        heavy_h_block(ea, eb, d, w);
+        double* dst = wpool + n_hh * 100 + (k_all - n_hh) * 10;
+#pragma unroll
+        for (int q = 0; q < 10; ++q) dst[q] = w[q];
+#pragma unroll
+        for (int i2 = 0; i2 < 4; ++i2) {
+            double v = 0.5 * ((i2 == 0 ? ea.bs : ea.bp) + eb.bs) * S[i2 * 4];
+            H[(size_t)(oa + i2) * nao + ob] = v;
+            H[(size_t)ob * nao + oa + i2] = v;
+        }
+    } else {
+        wpool[n_hh * 100 + n_hH * 10 + (k_all - n_hh - n_hH)] = hh_integral(ea, eb, r);
+        double v = 0.5 * (ea.bs + eb.bs) * S[0];
+        H[(size_t)oa * nao + ob] = v;
+        H[(size_t)ob * nao + oa] = v;
+    }
+    // the core-core term is symmetric in the pair; evaluate it in index order (i < j)
+    const int lo_a = min(a, b), hi_a = max(a, b);
+    return core_core_pair(el[z[lo_a]], el[z[hi_a]], z[lo_a], z[hi_a], r);
+}
+
 __global__ __launch_bounds__(256) void k_pairs(BatchArgs g) {
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < g.n_pairs;
          p += (int64_t)gridDim.x * blockDim.x) {
@@ -96,105 +190,24 @@ __global__ __launch_bounds__(256) void k_pairs(BatchArgs g) {
             if (g.pair_off[mid] <= p) lo = mid;
             else hi = mid - 1;
         }
-        const MolDesc md = g.mols[lo];
-        const int64_t k_all = p - g.pair_off[lo];
-        const DevElem* el = g.elems + md.method * kMaxZ;
-        const double* xyz = g.xyz + md.atom_off * 3;
-        const int* z = g.z + md.atom_off;
-        const int* alist = g.alist + md.atom_off;
-        const int* aoff = g.aoff + md.atom_off;
-        const int nh = md.n_heavy, nH = md.n_h, nao = md.nao;
-        const int64_t n_hh = (int64_t)nh * (nh - 1) / 2, n_hH = (int64_t)nh * nH;
-        double* H = g.mats + md.mat_off + kH * (size_t)nao * nao;
-        double* wpool = g.wpool + md.w_off;
-        int a, b;
-        if (k_all < n_hh) {
-            int i, j;
-            tri_decode(k_all, nh, i, j);
-            a = alist[i];
-            b = alist[j];
-        } else if (k_all < n_hh + n_hH) {
-            int64_t k = k_all - n_hh;
-            a = alist[(int)(k / nH)];
-            b = alist[nh + (int)(k % nH)];
-        } else {
-            int i, j;
-            tri_decode(k_all - n_hh - n_hH, nH, i, j);
-            a = alist[nh + i];
-            b = alist[nh + j];
-        }
-        const DevElem& ea = el[z[a]];
-        const DevElem& eb = el[z[b]];
-        double d[3] = {xyz[3 * b] - xyz[3 * a], xyz[3 * b + 1] - xyz[3 * a + 1], xyz[3 * b + 2] - xyz[3 * a + 2]};
-        double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-        const int oa = aoff[a], ob = aoff[b];
-        if (md.far_cut > 0.0 && r > md.far_cut) {
-            // beyond the far-field cutoff the pair is out of the pair list (neighbors.rs:90-101):
-            // no integrals, no resonance, no core-core term; its net charges interact through
-            // the far-field potential instead
-            if (k_all < n_hh) {
-                double* dst = wpool + k_all * 100;
-                for (int q = 0; q < 100; ++q) dst[q] = 0.0;
-                for (int i2 = 0; i2 < 4; ++i2)
-                    for (int j2 = 0; j2 < 4; ++j2) {
-                        H[(size_t)(oa + i2) * nao + ob + j2] = 0.0;
-                        H[(size_t)(ob + j2) * nao + oa + i2] = 0.0;
-                    }
-            } else if (k_all < n_hh + n_hH) {
-                double* dst = wpool + n_hh * 100 + (k_all - n_hh) * 10;
-                for (int q = 0; q < 10; ++q) dst[q] = 0.0;
-                for (int i2 = 0; i2 < 4; ++i2) {
-                    H[(size_t)(oa + i2) * nao + ob] = 0.0;
-                    H[(size_t)ob * nao + oa + i2] = 0.0;
-                }
-            } else {
-                wpool[n_hh * 100 + n_hH * 10 + (k_all - n_hh - n_hH)] = 0.0;
-                H[(size_t)oa * nao + ob] = 0.0;
-                H[(size_t)ob * nao + oa] = 0.0;
-            }
-            g.ecore_pair[p] = 0.0;
-            continue;
-        }
-        double S[16];
-        overlap_block(ea, eb, d, S);
-        if (k_all < n_hh) {
-            heavy_heavy_block(ea, eb, d, wpool + k_all * 100);
-#pragma unroll
-            for (int i2 = 0; i2 < 4; ++i2)
-#pragma unroll
-                for (int j2 = 0; j2 < 4; ++j2) {
-                    double v = 0.5 * ((i2 == 0 ? ea.bs : ea.bp) + (j2 == 0 ? eb.bs : eb.bp)) * S[i2 * 4 + j2];
-                    H[(size_t)(oa + i2) * nao + ob + j2] = v;
-                    H[(size_t)(ob + j2) * nao + oa + i2] = v;
-                }
-        } else if (k_all < n_hh + n_hH) {
-            double w[10];
-            heavy_h_block(ea, eb, d, w);
-            double* dst = wpool + n_hh * 100 + (k_all - n_hh) * 10;
-#pragma unroll
-            for (int q = 0; q < 10; ++q) dst[q] = w[q];
-#pragma unroll
-            for (int i2 = 0; i2 < 4; ++i2) {
-                double v = 0.5 * ((i2 == 0 ? ea.bs : ea.bp) + eb.bs) * S[i2 * 4];
-                H[(size_t)(oa + i2) * nao + ob] = v;
-                H[(size_t)ob * nao + oa + i2] = v;
-            }
-        } else {
-            wpool[n_hh * 100 + n_hH * 10 + (k_all - n_hh - n_hH)] = hh_integral(ea, eb, r);
-            double v = 0.5 * (ea.bs + eb.bs) * S[0];
-            H[(size_t)oa * nao + ob] = v;
-            H[(size_t)ob * nao + oa] = v;
-        }
-        // the core-core term is symmetric in the pair; evaluate it in index order (i < j)
-        const int lo_a = min(a, b), hi_a = max(a, b);
-        g.ecore_pair[p] = core_core_pair(el[z[lo_a]], el[z[hi_a]], z[lo_a], z[hi_a], r);
+        g.ecore_pair[p] = pair_one(g, g.mols[lo], p - g.pair_off[lo]);
     }
+}
+
+// Service launch: every pair of one molecule by its own workgroup. Returns the core-core sum,
+// accumulated per thread in the order of mol_setup's sum over k_pairs' results, then reduced the
+// same way, so it equals the batched launch's value bit for bit.
+__device__ __noinline__ double pairs_wg(const BatchArgs& g, const MolDesc& md, WgScratch& s) {
+    const int64_t np = (int64_t)md.nat * (md.nat - 1) / 2;
+    double ecore = 0.0;
+    for (int64_t k = threadIdx.x; k < np; k += kWg) ecore += pair_one(g, md, k);
+    return wg_sum(ecore, s);
 }
 
 // Per-molecule setup inside the SCF kernel: atom tables into LDS, H_core diagonal blocks
 // U + sum over partners of (-Z_y)(mu nu|s_y s_y) (hamiltonian.rs:229-332), core-core sum.
 __device__ double mol_setup(const MolDesc& md, int mi, const BatchArgs& g, MolShared& ms, WgScratch& s, double* H,
-                            const double* wpool) {
+                            const double* wpool, bool own_pairs, double ecore_own) {
     const int t = threadIdx.x;
     const int nat = md.nat, nao = md.nao;
     const DevElem* el = g.elems + md.method * kMaxZ;
@@ -241,6 +254,7 @@ __device__ double mol_setup(const MolDesc& md, int mi, const BatchArgs& g, MolSh
         H[(size_t)(o + mu) * nao + o + nu] = u + acc;
         H[(size_t)(o + nu) * nao + o + mu] = u + acc;
     }
+    if (own_pairs) return ecore_own;  // the workgroup computed its pairs itself (service launch)
     double ecore = 0.0;
     const int64_t p0 = g.pair_off[mi], p1 = g.pair_off[mi + 1];
     for (int64_t p = p0 + t; p < p1; p += kWg) ecore += g.ecore_pair[p];
@@ -1301,27 +1315,106 @@ __device__ __noinline__ void scf_uhf(const MolDesc& md, const BatchArgs& g, MolS
     phase_mark(pc, 5);
 }
 
+// Service launch: the next published ticket for this workgroup, or -1 once the host sets the stop
+// flag, or once the launch has been idle for svc_idle_us: no workgroup busy and nothing started,
+// taken or finished since (one deadline for all workgroups, so the launch empties at once rather
+// than thinning out while requests still come). Thread 0 only. Every wait is bounded: the
+// compare-and-swap fails only when another workgroup took a ticket, and the busy workgroups finish
+// their requests (each SCF is bounded by max_scf).
+__device__ long long svc_take(const BatchArgs& g) {
+    const uint64_t limit = (uint64_t)g.svc_idle_us * 100u;  // 100 MHz device clock
+    for (;;) {
+        if (__hip_atomic_load(g.svc_stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)) return -1;
+        const long long pub = __hip_atomic_load(g.svc_published, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+        long long cur = __hip_atomic_load(g.svc_next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        while (cur < pub) {
+            if (__hip_atomic_compare_exchange_strong(g.svc_next, &cur, cur + 1, __ATOMIC_ACQ_REL, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT)) {
+                __hip_atomic_fetch_max(g.svc_last, (unsigned long long)__builtin_amdgcn_s_memrealtime(),
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                return cur;
+            }
+        }
+        const unsigned long long last = __hip_atomic_load(g.svc_last, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned long long now = __builtin_amdgcn_s_memrealtime();
+        if (now > last && now - last > limit &&
+            __hip_atomic_load(g.svc_busy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+            return -1;
+        // about 14 us between polls of the pinned host flags
+        __builtin_amdgcn_s_sleep(127);
+        __builtin_amdgcn_s_sleep(127);
+        __builtin_amdgcn_s_sleep(127);
+        __builtin_amdgcn_s_sleep(127);
+    }
+}
+
 template <int PREC>
 __global__ __launch_bounds__(kWg) void k_scf_batch(BatchArgs g) {
     double* lds = lds_dyn;
     __shared__ MolShared ms;
     __shared__ WgScratch s;
     __shared__ int next;
+    __shared__ long long ticket;
+    __shared__ MolDesc smd;
+    const bool svc = g.svc_in != nullptr;
+    if (svc && threadIdx.x == 0) {
+        __hip_atomic_fetch_add(g.svc_alive, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_fetch_max(g.svc_last, (unsigned long long)__builtin_amdgcn_s_memrealtime(), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+    }
     for (;;) {
-        if (threadIdx.x == 0) next = atomicAdd(g.queue, 1);
-        __syncthreads();
-        const int slot = next;
-        __syncthreads();
-        if (slot >= g.n_mols) break;
-        const int mi = g.order ? g.order[slot] : slot;
-        const MolDesc md = g.mols[mi];
+        int mi;
+        MolDesc md;
+        double ecore_own = 0.0;
+        if (svc) {
+            if (threadIdx.x == 0) {
+                ticket = svc_take(g);
+                if (ticket >= 0) {
+                    __hip_atomic_fetch_add(g.svc_busy, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    smd = g.svc_in[ticket % g.svc_cap].md;
+                }
+            }
+            __syncthreads();
+            const long long tk = ticket;
+            if (tk < 0) break;
+            mi = (int)(tk % g.svc_cap);
+            md = smd;
+            __syncthreads();
+            // Pool regions are reused by later requests on other XCDs: drop this XCD's cached
+            // lines of them before use (the previous user wrote its lines back when it finished).
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            const SvcSlot& sl = g.svc_in[mi];
+            int* z = const_cast<int*>(g.z) + md.atom_off;
+            int* aoff = const_cast<int*>(g.aoff) + md.atom_off;
+            int* acls = const_cast<int*>(g.acls) + md.atom_off;
+            int* alist = const_cast<int*>(g.alist) + md.atom_off;
+            double* xyz = const_cast<double*>(g.xyz) + md.atom_off * 3;
+            for (int a = threadIdx.x; a < md.nat; a += kWg) {
+                z[a] = sl.z[a];
+                aoff[a] = sl.aoff[a];
+                acls[a] = sl.acls[a];
+                alist[a] = sl.alist[a];
+            }
+            for (int k = threadIdx.x; k < 3 * md.nat; k += kWg) xyz[k] = sl.xyz[k];
+            __syncthreads();
+            ecore_own = pairs_wg(g, md, s);
+            __syncthreads();
+        } else {
+            if (threadIdx.x == 0) next = atomicAdd(g.queue, 1);
+            __syncthreads();
+            const int slot = next;
+            __syncthreads();
+            if (slot >= g.n_mols) break;
+            mi = g.order ? g.order[slot] : slot;
+            md = g.mols[mi];
+        }
         MolResult res;
         PhaseClock pc;
         res.t_start = __builtin_amdgcn_s_memrealtime();
         pc.last = res.t_start;
         for (int k = 0; k < kPhases; ++k) pc.acc[k] = 0;
         double* M = g.mats + md.mat_off;
-        double ecore = mol_setup(md, mi, g, ms, s, M + kH * (size_t)md.nao * md.nao, g.wpool + md.w_off);
+        double ecore = mol_setup(md, mi, g, ms, s, M + kH * (size_t)md.nao * md.nao, g.wpool + md.w_off, svc, ecore_own);
         __syncthreads();
         phase_mark(pc, 0);
         if (md.uhf) scf_uhf<PREC>(md, g, ms, s, lds, ecore, res, pc);
@@ -1332,7 +1425,26 @@ __global__ __launch_bounds__(kWg) void k_scf_batch(BatchArgs g) {
             g.results[mi] = res;
         }
         __syncthreads();
-        if (g.done) {
+        if (svc) {
+            // results into the host-pinned ring, then the slot's done flag (system scope, after the
+            // data and after this XCD's cached pool lines are written back)
+            SvcOut& out = g.svc_out[mi];
+            for (int a = threadIdx.x; a < md.nat; a += kWg) out.charges[a] = g.charges[md.atom_off + a];
+            for (int k = threadIdx.x; k < md.nao; k += kWg) {
+                out.mo[k] = g.mo_e[md.ao_off + k];
+                if (md.uhf) out.mob[k] = g.mo_e_beta[md.ao_off + k];
+            }
+            if (threadIdx.x == 0) out.res = res;
+            __threadfence_system();
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                __hip_atomic_store(g.svc_done + mi, ticket + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+                __hip_atomic_fetch_max(g.svc_last, (unsigned long long)__builtin_amdgcn_s_memrealtime(),
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_fetch_add(g.svc_busy, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            __syncthreads();
+        } else if (g.done) {
             // publish this molecule to the host while the launch goes on
             for (int a = threadIdx.x; a < md.nat; a += kWg)
                 g.charges_h[md.atom_off + a] = g.charges[md.atom_off + a];
@@ -1346,6 +1458,7 @@ __global__ __launch_bounds__(kWg) void k_scf_batch(BatchArgs g) {
             if (threadIdx.x == 0) __hip_atomic_store(g.done + mi, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
         }
     }
+    if (svc && threadIdx.x == 0) __hip_atomic_fetch_add(g.svc_alive, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 template __global__ void k_scf_batch<0>(BatchArgs);

@@ -69,8 +69,12 @@ def verify(results, refs):
     counts only when it converged and its total energy, every Mulliken charge and the HOMO/LUMO
     (both spin channels) are within tolerance. Returns (ok mask, largest errors)."""
     ok = np.zeros(len(results), dtype=bool)
-    worst = {"dE_ev": 0.0, "dq_e": 0.0, "dorb_ev": 0.0}
+    worst = {"dE_ev": 0.0, "dq_e": 0.0, "dorb_ev": 0.0, "not_converged": 0, "oracle_not_converged": 0}
     for i, (r, o) in enumerate(zip(results, refs)):
+        if o["status"] != 0:
+            worst["oracle_not_converged"] += 1
+        if r["status"] != 0 or not r["converged"]:
+            worst["not_converged"] += 1
         if r["status"] != 0 or not r["converged"] or o["status"] != 0:
             continue
         de = abs(r["total_ev"] - o["total_ev"])
@@ -92,13 +96,13 @@ def verify(results, refs):
     return ok, worst
 
 
-def serve_stream(am1g, stream, rate, lanes, max_batch, precision):
+def serve_stream(am1g, stream, rate, lanes, max_batch, precision, continuous=True):
     """Submit the requests at their Poisson arrival times to a continuous-batching server and
     wait for all of them: throughput over the whole run and per-request latency percentiles.
     A request that asks for AM1-BCC charges gets them from its own SCF when it is an AM1 request;
     an RM1 request that asks for them also submits the AM1 SCF the charges need (the corrections
     are fitted to AM1 Mulliken charges), and finishes when both have."""
-    sv = am1g.Server(0, lanes=lanes, max_batch=max_batch)
+    sv = am1g.Server(0, lanes=lanes, max_batch=max_batch, continuous=continuous)
     tickets = []
     t0 = time.perf_counter()
     for req in stream:
@@ -133,6 +137,7 @@ def serve_stream(am1g, stream, rate, lanes, max_batch, precision):
     return {"offered_per_s": rate, "requests": len(stream), "ok": ok, "scfs_per_s": round(ok / elapsed, 2),
             "scfs_run": n_scf, "elapsed_s": elapsed, "_results": res,
             "p50_ms": percentile(lat, 50), "p95_ms": percentile(lat, 95), "p99_ms": percentile(lat, 99),
+            "mode": "service launch" if continuous else f"{lanes} lanes of batched launches",
             "lanes": lanes, "max_batch": max_batch, "precision": precision, "launches": stats["launches"],
             "uhf": sum(1 for r in res if r["unrestricted"]),
             "rm1": sum(1 for q in stream if q.method == "rm1"),
@@ -372,11 +377,14 @@ def main():
         mid_rate = 200.0
         mid = G.serving_stream(args.serve_requests, mid_rate)
         serving = {
-            "plain": serve_stream(am1g, slow, slow_rate, lanes=1, max_batch=1, precision="fp64"),
+            "plain": serve_stream(am1g, slow, slow_rate, lanes=1, max_batch=1, precision="fp64", continuous=False),
             "batched_mixed_same_load": serve_stream(am1g, slow, slow_rate, lanes=4, max_batch=512,
                                                     precision="mixed"),
             "batched_mixed_200": serve_stream(am1g, mid, mid_rate, lanes=4, max_batch=512, precision="mixed"),
             "batched_mixed": serve_stream(am1g, fast, args.serve_rate, lanes=4, max_batch=512, precision="mixed"),
+            "batched_fp64": serve_stream(am1g, fast, args.serve_rate, lanes=4, max_batch=512, precision="fp64"),
+            "lanes_mixed": serve_stream(am1g, fast, args.serve_rate, lanes=4, max_batch=512, precision="mixed",
+                                        continuous=False),
             "note": "AM1-BCC requests get the corrections on the host after their AM1 SCF (an RM1 request "
                     "that asks for them also runs the AM1 SCF); latency = submit to completion on the host "
                     "clock; max_abs_dbcc_e compares the served BCC charges of AM1 requests with the "
@@ -389,7 +397,8 @@ def main():
                 for q in longest]
         osv = {"fp64": O.scf_many(sreq, threads=threads), "mixed": O.scf_many(sreq, threads=threads, p_tol=1e-9)}
         log(f"serving oracle: 2 x {len(longest)} SCFs in {time.perf_counter() - t:.1f}s")
-        for key in ("plain", "batched_mixed_same_load", "batched_mixed_200", "batched_mixed"):
+        for key in ("plain", "batched_mixed_same_load", "batched_mixed_200", "batched_mixed", "batched_fp64",
+                    "lanes_mixed"):
             leg = serving[key]
             rs = leg.pop("_results")
             okv, wv = verify(rs, osv[leg["precision"]][: len(rs)])
@@ -404,6 +413,8 @@ def main():
             leg["max_abs_dE_ev"] = wv["dE_ev"]
             leg["max_abs_dq_e"] = wv["dq_e"]
             leg["max_abs_dorb_ev"] = wv["dorb_ev"]
+            leg["not_converged"] = wv["not_converged"]
+            leg["oracle_not_converged"] = wv["oracle_not_converged"]
         log(f"serving at {mid_rate:g}/s: {serving['batched_mixed_200']['verified_scfs_per_s']} verified SCFs/s "
             f"p50 {serving['batched_mixed_200']['p50_ms']:.1f} ms p99 {serving['batched_mixed_200']['p99_ms']:.1f} ms")
         log(f"serving: plain {serving['plain']['verified_scfs_per_s']} SCFs/s p99 {serving['plain']['p99_ms']:.1f} ms; "

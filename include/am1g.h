@@ -189,6 +189,11 @@ typedef struct {
     double memory_budget_mb;  /* device memory the running launches may hold (default 32768) */
     double max_wait_ms;       /* batching window: a free lane waits this long after the first queued
                                  request for more to arrive before launching (default 0) */
+    int continuous;           /* 1 (default): one persistent service launch takes requests as they
+                                 arrive (a request joins the running launch as soon as a workgroup is
+                                 free); lanes and max_batch then apply only to requests beyond one
+                                 workgroup's reach, which run alone on the grid-level path.
+                                 0: lanes of batched launches as described above */
 } am1g_serve_options;
 
 typedef struct {

@@ -1,5 +1,7 @@
 """Continuous-batching server (``am1g_serve_*``): every result is the same alone and in the mix,
-requests that arrive while others run are admitted, cancellation and the memory budget work."""
+requests that arrive while others run are admitted, cancellation and the memory budget work, in
+both modes: the persistent service launch (``continuous``, the default), where a request joins the
+running launch, and lanes of batched launches."""
 import time
 
 import numpy as np
@@ -33,7 +35,7 @@ def _mix(seed=7, n_chain=24, n_water=8):
 
 def test_results_in_the_mix_equal_results_alone(engine):
     reqs = _mix()
-    sv = am1g.Server(0, lanes=3, max_batch=8)
+    sv = am1g.Server(0, lanes=3, max_batch=8, continuous=False)
     try:
         tickets = [sv.submit(z, x, **kw) for z, x, kw in reqs]
         got = [sv.poll(t, wait_ms=-1) for t in tickets]
@@ -57,7 +59,7 @@ def test_results_in_the_mix_equal_results_alone(engine):
 
 def test_requests_arriving_during_a_launch_are_admitted(engine):
     mols = G.chain_batch(400, seed=3)
-    sv = am1g.Server(0, lanes=2, max_batch=256)
+    sv = am1g.Server(0, lanes=2, max_batch=256, continuous=False)
     try:
         first = [sv.submit(m.z, m.xyz_bohr) for m in mols[:300]]
         time.sleep(0.005)
@@ -76,7 +78,7 @@ def test_requests_arriving_during_a_launch_are_admitted(engine):
 
 def test_cancel_and_unknown_ticket():
     mols = G.chain_batch(64, seed=5)
-    sv = am1g.Server(0, lanes=1, max_batch=4)
+    sv = am1g.Server(0, lanes=1, max_batch=4, continuous=False)
     try:
         ts = [sv.submit(m.z, m.xyz_bohr) for m in mols]
         sv.cancel(ts[-1])
@@ -92,7 +94,7 @@ def test_cancel_and_unknown_ticket():
 
 def test_memory_budget_bounds_admission():
     mols = G.chain_batch(48, seed=9)
-    sv = am1g.Server(0, lanes=4, max_batch=512, memory_budget_mb=2.0)
+    sv = am1g.Server(0, lanes=4, max_batch=512, memory_budget_mb=2.0, continuous=False)
     try:
         ts = [sv.submit(m.z, m.xyz_bohr) for m in mols]
         res = [sv.poll(t, wait_ms=-1) for t in ts]
@@ -114,7 +116,7 @@ def test_results_stream_out_before_the_launch_ends():
     rng = np.random.default_rng(12)
     big = G.small_water_cluster(21, rng)
     small = G.chain_batch(40, seed=13)
-    sv = am1g.Server(0, lanes=1, max_batch=64, max_wait_ms=500.0)
+    sv = am1g.Server(0, lanes=1, max_batch=64, max_wait_ms=500.0, continuous=False)
     try:
         tb = sv.submit(big.z, big.xyz_bohr, e_tol=0.0, p_tol=0.0, max_scf=200)
         ts = [sv.submit(m.z, m.xyz_bohr) for m in small]
@@ -137,7 +139,7 @@ def test_large_requests_run_alone_on_the_grid_path(engine):
     rng = np.random.default_rng(21)
     bigs = [G.small_water_cluster(k, rng) for k in (34, 40)]
     small = G.chain_batch(30, seed=22)
-    sv = am1g.Server(0, lanes=2, max_batch=16)
+    sv = am1g.Server(0, lanes=2, max_batch=16, continuous=False)
     try:
         tb = [sv.submit(bigs[0].z, bigs[0].xyz_bohr)]
         ts = [sv.submit(m.z, m.xyz_bohr) for m in small[:15]]
@@ -171,7 +173,7 @@ def test_many_large_requests_share_one_launch(engine):
     batched kernel, whichever ran it)."""
     rng = np.random.default_rng(31)
     bigs = [G.small_water_cluster(int(k), rng) for k in rng.integers(33, 42, size=12)]
-    sv = am1g.Server(0, lanes=2, max_batch=64)
+    sv = am1g.Server(0, lanes=2, max_batch=64, continuous=False)
     try:
         ts = [sv.submit(m.z, m.xyz_bohr) for m in bigs]
         rs = [sv.poll(t, wait_ms=-1) for t in ts]
@@ -187,3 +189,96 @@ def test_many_large_requests_share_one_launch(engine):
         grid = engine.scf(m.z, m.xyz_bohr, full=False)
         batched = engine.scf_batch([(m.z, m.xyz_bohr)])[0]
         assert r["total_ev"] in (grid["total_ev"], batched["total_ev"])
+
+
+# ------------------------------------------------------------------ persistent service launch
+def test_service_results_equal_results_alone(engine):
+    """Chains, water clusters, UHF radicals and RM1 requests through the service launch: each
+    result is bit for bit the molecule's batched result alone (same arithmetic, pair integrals
+    computed by the request's own workgroup), and within the contract of the oracle."""
+    reqs = _mix(seed=17)
+    sv = am1g.Server(0)
+    try:
+        tickets = [sv.submit(z, x, **kw) for z, x, kw in reqs]
+        got = [sv.poll(t, wait_ms=-1) for t in tickets]
+        stats = sv.stats()
+    finally:
+        sv.close()
+    assert stats["completed"] == len(reqs)
+    for (z, x, kw), r in zip(reqs, got):
+        alone = engine.scf_batch([(z, x, kw)])[0]
+        assert r["status"] == alone["status"] == 0
+        assert r["total_ev"] == alone["total_ev"]
+        assert np.array_equal(r["charges"], alone["charges"])
+        assert r["iterations"] == alone["iterations"]
+        assert r["homo_ev"] == alone["homo_ev"] and r["lumo_ev"] == alone["lumo_ev"]
+    for i in (0, 3, len(reqs) - 1):
+        z, x, kw = reqs[i]
+        o = O.scf(z, x, full=False, **kw)
+        assert abs(got[i]["total_ev"] - o["total_ev"]) <= 1e-6
+        assert np.abs(got[i]["charges"] - o["charges"]).max() <= 1e-6
+
+
+def test_service_request_joins_the_running_launch():
+    """A 21-water cluster held for exactly 200 iterations keeps the service launch running; chains
+    submitted after it started run in that same launch (one launch in all) and finish first."""
+    rng = np.random.default_rng(12)
+    big = G.small_water_cluster(21, rng)
+    small = G.chain_batch(40, seed=13)
+    sv = am1g.Server(0)
+    try:
+        tb = sv.submit(big.z, big.xyz_bohr, e_tol=0.0, p_tol=0.0, max_scf=200)
+        t0 = time.time()
+        while sv.stats()["running"] < 1 and time.time() - t0 < 10.0:
+            time.sleep(0.001)
+        time.sleep(0.01)  # the cluster is on the device now
+        ts = [sv.submit(m.z, m.xyz_bohr) for m in small]
+        rs = [sv.poll(t, wait_ms=-1) for t in ts]
+        rb = sv.poll(tb, wait_ms=-1)
+        stats = sv.stats()
+    finally:
+        sv.close()
+    assert rb["status"] == 4 and rb["iterations"] == 200  # ScfNotConverged
+    assert all(r["status"] == 0 for r in rs)
+    assert stats["launches"] == 1, stats
+    assert max(r["latency_ms"] for r in rs) < rb["latency_ms"]
+
+
+def test_service_memory_budget_and_cancel(engine):
+    """A 2 MB budget admits a few requests at a time (the rest wait for memory), a cancelled
+    queued request is dropped, every other result is the one the molecule gets alone, and the
+    device memory held never exceeds the budget."""
+    mols = G.chain_batch(48, seed=9)
+    sv = am1g.Server(0, memory_budget_mb=2.0)
+    try:
+        ts = [sv.submit(m.z, m.xyz_bohr) for m in mols]
+        sv.cancel(ts[-1])
+        res = [sv.poll(t, wait_ms=-1) for t in ts]
+        stats = sv.stats()
+    finally:
+        sv.close()
+    assert res[-1]["status"] == am1g.CANCELLED
+    assert all(r["status"] == 0 for r in res[:-1])
+    assert stats["peak_admitted_mb"] <= 2.0
+    alone = engine.scf_batch([(m.z, m.xyz_bohr) for m in mols[:-1]])
+    for a, b in zip(res[:-1], alone):
+        assert a["total_ev"] == b["total_ev"]
+
+
+def test_service_beyond_one_workgroup_runs_on_the_grid_lane(engine):
+    """A 50-water cluster (300 basis functions) is beyond the service launch's reach: it runs on
+    the grid-level path and gives what ``Engine.scf`` gives, while chains go through the service."""
+    rng = np.random.default_rng(41)
+    big = G.small_water_cluster(50, rng)
+    small = G.chain_batch(12, seed=42)
+    sv = am1g.Server(0)
+    try:
+        tb = sv.submit(big.z, big.xyz_bohr)
+        ts = [sv.submit(m.z, m.xyz_bohr) for m in small]
+        rb = sv.poll(tb, wait_ms=-1)
+        rs = [sv.poll(t, wait_ms=-1) for t in ts]
+    finally:
+        sv.close()
+    assert rb["status"] == 0 and all(r["status"] == 0 for r in rs)
+    alone = engine.scf(big.z, big.xyz_bohr, full=False)
+    assert rb["total_ev"] == alone["total_ev"] and np.array_equal(rb["charges"], alone["charges"])
