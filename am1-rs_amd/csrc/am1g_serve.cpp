@@ -138,6 +138,7 @@ struct Service {
     size_t slot_off[kCap], slot_bytes[kCap];
     MolDesc slot_md[kCap];
     bool running[2] = {false, false};  // a service launch is on ctx->stream / extra
+    bool paused = false;               // the grid lane holds the device: no new launch
     Clock::time_point started[2];
     BatchArgs args;
     ~Service() {
@@ -409,6 +410,7 @@ struct am1g_server {
                 const hipError_t e = hipStreamQuery(streams[q]);
                 if (e == hipSuccess) {
                     v.running[q] = false;
+                    cv_work.notify_all();  // a paused grid lane waits for this
                 } else if (e != hipErrorNotReady) {
                     err = std::string("HIP error in the service launch: ") + hipGetErrorString(e);
                     fail_all_running();
@@ -421,7 +423,7 @@ struct am1g_server {
             // (a launch's workgroups count themselves in as they start: look after a few ms)
             const bool thinned = n_running == 1 && 2 * alive < v.ctx->cus &&
                                  Clock::now() - v.started[1 - q] > std::chrono::milliseconds(5);
-            if (pending && (n_running == 0 || thinned)) {
+            if (pending && !v.paused && (n_running == 0 || thinned)) {
                 const hipError_t e = launch_scf_batch(v.args, 1, v.ctx->cus, (size_t)kLdsBytes, streams[q]);
                 if (e != hipSuccess) {
                     err = std::string("service launch failed: ") + hipGetErrorString(e);
@@ -494,7 +496,24 @@ struct am1g_server {
                     }
                 large_mode = h.batchable && n_large >= ctx->tun.serve_large_batch;
             }
-            if (svc || (jobs.at(head).large && !large_mode)) {
+            if (svc) {
+                // A grid-path request needs the whole device, which the service launch holds while it
+                // has work: its workgroups leave after their current requests (the shared stop flag),
+                // no new launch starts, and this lane runs every queued grid-path request before
+                // handing the device back.
+                Service& v = *svc;
+                v.paused = true;
+                __atomic_store_n(v.stop.as<int>(), 1, __ATOMIC_RELEASE);
+                cv_work.wait(lk, [&] { return stop || (!v.running[0] && !v.running[1]); });
+                if (!stop)
+                    for (long long q = head_of(); q >= 0; q = head_of()) run_large(lk, ctx, q);
+                if (stop) return;  // the device stop flag stays set: the server is closing
+                __atomic_store_n(v.stop.as<int>(), 0, __ATOMIC_RELEASE);
+                v.paused = false;
+                cv_work.notify_all();
+                continue;
+            }
+            if (jobs.at(head).large && !large_mode) {
                 Job& j = jobs.at(head);
                 const bool alone = admitted == 0;
                 if (!alone && admitted + j.bytes > budget) {

@@ -267,18 +267,31 @@ def test_service_memory_budget_and_cancel(engine):
 
 def test_service_beyond_one_workgroup_runs_on_the_grid_lane(engine):
     """A 50-water cluster (300 basis functions) is beyond the service launch's reach: it runs on
-    the grid-level path and gives what ``Engine.scf`` gives, while chains go through the service."""
+    the grid-level path and gives what ``Engine.scf`` gives, while chains go through the service.
+    It is submitted while a long request keeps the service launch busy and chains keep arriving:
+    the service launch pauses after its running requests, so the cluster does not wait for the
+    stream of chains to end."""
     rng = np.random.default_rng(41)
     big = G.small_water_cluster(50, rng)
-    small = G.chain_batch(12, seed=42)
+    hold = G.small_water_cluster(21, rng)
+    small = G.chain_batch(60, seed=42)
     sv = am1g.Server(0)
     try:
+        th = sv.submit(hold.z, hold.xyz_bohr, e_tol=0.0, p_tol=0.0, max_scf=200)
+        ts = [sv.submit(m.z, m.xyz_bohr) for m in small[:20]]
         tb = sv.submit(big.z, big.xyz_bohr)
-        ts = [sv.submit(m.z, m.xyz_bohr) for m in small]
+        ts += [sv.submit(m.z, m.xyz_bohr) for m in small[20:]]
         rb = sv.poll(tb, wait_ms=-1)
         rs = [sv.poll(t, wait_ms=-1) for t in ts]
+        rh = sv.poll(th, wait_ms=-1)
+        stats = sv.stats()
     finally:
         sv.close()
     assert rb["status"] == 0 and all(r["status"] == 0 for r in rs)
+    assert rh["status"] == 4 and rh["iterations"] == 200
+    assert stats["completed"] == 62
     alone = engine.scf(big.z, big.xyz_bohr, full=False)
     assert rb["total_ev"] == alone["total_ev"] and np.array_equal(rb["charges"], alone["charges"])
+    batched = engine.scf_batch([(m.z, m.xyz_bohr) for m in small])
+    for a, b in zip(rs, batched):
+        assert a["total_ev"] == b["total_ev"]
