@@ -21,6 +21,9 @@ Measures on the current GPU, through the C ABI, and writes the table the library
   trailing matrix, then a one-workgroup reflector step); orders 600, 1200 and 3072 are timed
   both ways, and the blocked form is kept from the smallest order from which it wins at every
   larger order measured (otherwise off).
+* ``batch_own_pairs``: whether a batched launch computes each molecule's pair integrals in the
+  molecule's own workgroup (one launch) or runs the pair-integral kernel over the whole batch
+  first; timed on 1024 chains of the benchmark workload.
 * ``pairs_block``: the workgroup size of the pair-integral kernel (64, 128 or 256 threads);
   timed by the core-Hamiltonian build of 2000 waters (18 M atom pairs), which that kernel
   dominates.
@@ -139,6 +142,23 @@ def tune_graphs(eng, log):
     return best, times
 
 
+def tune_own_pairs(eng, log):
+    ms = [(m.z, m.xyz_bohr) for m in G.chain_batch(1024, seed=1)]
+    times = {}
+    for own in (0, 1):
+        eng.set_tunable("batch_own_pairs", own)
+        eng.scf_batch(ms)
+        ts = []
+        for _ in range(3):
+            eng.scf_batch(ms)
+            ts.append(eng.last_kernel_ms())
+        times[own] = min(ts)
+        log(f"batch_own_pairs {own}: 1024 chains {times[own]:.2f} ms")
+    best = min(times, key=times.get)
+    eng.set_tunable("batch_own_pairs", best)
+    return best, times
+
+
 _BLOCKED_OFF = 1 << 30
 
 
@@ -202,6 +222,7 @@ def autotune(path: str | None = None, device: int = 0, log=print) -> dict:
     beside the library). Returns the chosen values."""
     path = path or os.path.join(os.path.dirname(am1g.LIB_PATH), "tuned.txt")
     os.environ["AM1G_TUNE_FILE"] = ""  # measure from the defaults, not from an earlier table
+    name = _device_name(device)  # before the library touches the device (torch's query fails after)
     eng = am1g.Engine(device)
     try:
         tri, tt = tune_tri_max(eng, log)
@@ -210,9 +231,9 @@ def autotune(path: str | None = None, device: int = 0, log=print) -> dict:
         graphs, gt = tune_graphs(eng, log)
         pblock, pt = tune_pairs_block(eng, log)
         bmin, bt = tune_blocked(eng, log)
+        own, ot = tune_own_pairs(eng, log)
     finally:
         eng.close()
-    name = _device_name(device)
     lines = [f"# libam1g tunables measured by am1g.tune on {name}, {time.strftime('%Y-%m-%d')}",
              "# route_large_ao: one molecule, best of 3 (ms): AOs workgroup grid"]
     lines += [f"#   {nao} {a:.2f} {b:.2f}" for nao, a, b in rows]
@@ -229,10 +250,12 @@ def autotune(path: str | None = None, device: int = 0, log=print) -> dict:
     lines.append("# tri_blocked_min: large eigensolve (ms): order by-column blocked")
     lines += [f"#   {n} {a:.1f} {b:.1f}" for n, a, b in bt]
     lines.append(f"tri_blocked_min {bmin}")
+    lines.append("# batch_own_pairs: 1024 chains (ms): " + ", ".join(f"{k}: {v:.2f}" for k, v in ot.items()))
+    lines.append(f"batch_own_pairs {own}")
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")
     return {"route_large_ao": route, "tri_max": tri, "jacobi_lds_n": lds_n, "eig_graphs": graphs, "pairs_block": pblock,
-            "tri_blocked_min": bmin, "path": path}
+            "tri_blocked_min": bmin, "batch_own_pairs": own, "path": path}
 
 
 if __name__ == "__main__":

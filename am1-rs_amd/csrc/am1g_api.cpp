@@ -52,6 +52,7 @@ bool set_tunable(Tunables& t, const std::string& name, double v) {
     else if (name == "tri_max") t.tri_max = std::min(256, std::max(0, (int)v));
     else if (name == "pairs_block") t.pairs_block = (v == 64.0 || v == 128.0) ? (int)v : 256;
     else if (name == "tri_blocked_min") t.tri_blocked_min = std::max(3, (int)v);
+    else if (name == "batch_own_pairs") t.batch_own_pairs = v != 0.0;
     else return false;
     return true;
 }
@@ -64,6 +65,7 @@ bool get_tunable(const Tunables& t, const std::string& name, double& v) {
     else if (name == "tri_max") v = t.tri_max;
     else if (name == "pairs_block") v = t.pairs_block;
     else if (name == "tri_blocked_min") v = t.tri_blocked_min;
+    else if (name == "batch_own_pairs") v = t.batch_own_pairs;
     else return false;
     return true;
 }
@@ -562,7 +564,8 @@ static int batch_impl(am1g_ctx* ctx, int n, const am1g_request* rqs, am1g_result
         a.mo_eb_h = ctx->h_mob.as<double>();
     }
     HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
-    HIP_TRY(ctx, launch_pairs(a, st, ctx->tun.pairs_block));
+    a.own_pairs = ctx->tun.batch_own_pairs;
+    if (!a.own_pairs) HIP_TRY(ctx, launch_pairs(a, st, ctx->tun.pairs_block));
     HIP_TRY(ctx, launch_scf_batch(a, precision, grid, lds, st));
     HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
     if (streaming) {

@@ -73,6 +73,7 @@ struct Tunables {
     int tri_max = 256;                   // batched kernel: tridiagonal method up to this order, Jacobi above
     int pairs_block = 256;               // threads per workgroup of the pair-integral kernel
     int tri_blocked_min = 1 << 30;       // large eigensolver: tridiagonalization in panels from this order
+    int batch_own_pairs = 1;             // batched launch: workgroups compute their own pair integrals
 };
 
 struct am1g_ctx {

@@ -64,6 +64,7 @@ struct BatchArgs {
     const int64_t* pair_off;  // [n_mols + 1] first global pair index of each molecule
     int64_t n_pairs;
     double* ecore_pair;  // core-core repulsion of each pair, summed per molecule in fixed order
+    int own_pairs;       // 1: each workgroup computes its molecule's pairs (no k_pairs launch)
     // Streaming (optional, all null otherwise): as each molecule finishes, its result, charges and
     // orbital energies are copied into host-visible memory and done[m] is set with system scope,
     // so the host can hand results out while the launch still runs.

@@ -1407,6 +1407,7 @@ __global__ __launch_bounds__(kWg) void k_scf_batch(BatchArgs g) {
             if (slot >= g.n_mols) break;
             mi = g.order ? g.order[slot] : slot;
             md = g.mols[mi];
+            if (g.own_pairs) ecore_own = pairs_wg(g, md, s);  // instead of a k_pairs launch before
         }
         MolResult res;
         PhaseClock pc;
@@ -1414,7 +1415,8 @@ __global__ __launch_bounds__(kWg) void k_scf_batch(BatchArgs g) {
         pc.last = res.t_start;
         for (int k = 0; k < kPhases; ++k) pc.acc[k] = 0;
         double* M = g.mats + md.mat_off;
-        double ecore = mol_setup(md, mi, g, ms, s, M + kH * (size_t)md.nao * md.nao, g.wpool + md.w_off, svc, ecore_own);
+        double ecore = mol_setup(md, mi, g, ms, s, M + kH * (size_t)md.nao * md.nao, g.wpool + md.w_off,
+                                 svc || g.own_pairs, ecore_own);
         __syncthreads();
         phase_mark(pc, 0);
         if (md.uhf) scf_uhf<PREC>(md, g, ms, s, lds, ecore, res, pc);

@@ -253,6 +253,9 @@ int am1g_matmul(am1g_ctx* ctx, int trans_a, int trans_b, int m, int n, int k, do
  *                      default 2
  *   pairs_block        threads per workgroup of the pair-integral kernel (64, 128 or 256);
  *                      default 256
+ *   batch_own_pairs    1: in a batched launch each workgroup computes its molecule's pair
+ *                      integrals before its SCF (one launch); 0: a k_pairs launch over every pair
+ *                      of the batch first; the same bits either way; default 1
  *   tri_blocked_min    the large eigensolver reduces orders from this one to tridiagonal form
  *                      in panels of 32 columns (the trailing matrix read once per column, a
  *                      matrix-core rank-64 update per panel) instead of one column at a time;
