@@ -418,6 +418,8 @@ static int batch_impl(am1g_ctx* ctx, int n, const am1g_request* rqs, am1g_result
     for (int j = 0; j < m; ++j) {
         const am1g_request& rq = rqs[ok_idx[j]];
         MolDesc md = prep[ok_idx[j]].md;
+        const am1g_result& rj = results[ok_idx[j]];
+        md.all_vecs = (rj.mo_coeff || rj.mo_coeff_beta) ? 1 : 0;
         md.atom_off = atom_off;
         md.ao_off = ao_off;
         md.mat_off = mat_off;

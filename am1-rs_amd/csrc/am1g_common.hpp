@@ -48,6 +48,9 @@ struct MolDesc {
     int method, uhf, accel, max_scf;
     int precision;   // 0 fp64, 1 mixed
     int n_heavy, n_h, use_diis;
+    int all_vecs;    // 1: every eigenvector in each iteration (the caller takes the coefficients);
+                     // 0: the occupied ones, all the density needs
+    int pad0;
     double e_tol, p_tol, adiis_switch;
     double field[3];    // uniform external field, eV per (e*Bohr); zero for none
     double field_nuc;   // its nuclear energy -F . sum_a Z_a R_a (eV)

@@ -524,11 +524,16 @@ __device__ void mol_density(int nao, int nocc, double w, const double* Ct, doubl
 // Larger orders, and the rare matrix whose close eigenvalues that method cannot separate: A =
 // Ct F Ct^T (warm start in the previous orbitals, or F itself on the first pass), Jacobi, sort
 // ascending. Ct is replaced by the sorted eigenvectors (transposed) and eps by the sorted
-// eigenvalues. tol_abs: Jacobi's rotation threshold; bracket: half-width around the previous
-// eigenvalues for a warm eigenvalue search (tridiagonal method). Returns the Jacobi sweep count (0 for the tridiagonal method), or -1.
+// eigenvalues. warm: Ct holds the previous orbitals, all of them (the Jacobi warm start).
+// tol_abs: Jacobi's rotation threshold; bracket: half-width around the previous eigenvalues in
+// eps for a warm eigenvalue search (tridiagonal method; 0 = none). nvec: the tridiagonal method
+// forms only the nvec lowest eigenvectors (rows nvec.. of Ct are left as they were; every
+// eigenvalue is still computed), and partial tells whether it did (nvec < nao). Returns the
+// Jacobi sweep count (0 for the tridiagonal method), or -1.
 template <typename T>
 __device__ int mol_eigen(int nao, const double* Fuse, double* Ct, double* T0, double* T1, double* lds, int lds_n,
-                         int tri_max, bool warm, double* eps, WgScratch& s, double tol_abs, double bracket) {
+                         int tri_max, bool warm, double* eps, WgScratch& s, double tol_abs, double bracket, int nvec,
+                         bool& partial) {
     const int t = threadIdx.x;
     const int n = nao;
     const size_t nn = (size_t)n * n;
@@ -539,11 +544,17 @@ __device__ int mol_eigen(int nao, const double* Fuse, double* Ct, double* T0, do
     const bool v_lds = a_lds && v_rows == n;
     if (n <= tri_max && n <= kTriMax && n <= lds_n && tri_lds_doubles(n) + n <= kLdsDoubles) {
         // direct method (tridiagonalization); it declines only for unresolved close eigenvalues
-        if (wg_eigh_tri(n, Fuse, Ct, eps, T1, lds, kLdsDoubles, s, warm ? bracket : 0.0) == 0) return 0;
+        if (wg_eigh_tri(n, Fuse, Ct, eps, T1, lds, kLdsDoubles, s, bracket, nvec) == 0) {
+            partial = nvec < n;
+            return 0;
+        }
     } else if (n > kTriMax && n <= tri_max && n <= kTriBig) {
         // the same method with the matrix in global memory (T0: the matrix, then the
         // eigenvectors of T; T1: the reflectors)
-        if (wg_eigh_tri_g(n, Fuse, Ct, eps, T0, T1, lds, kLdsDoubles, s, warm ? bracket : 0.0) == 0) return 0;
+        if (wg_eigh_tri_g(n, Fuse, Ct, eps, T0, T1, lds, kLdsDoubles, s, bracket) == 0) {
+            partial = false;
+            return 0;
+        }
     }
     const double* src = Fuse;
     if (warm) {
@@ -612,6 +623,7 @@ __device__ int mol_eigen(int nao, const double* Fuse, double* Ct, double* T0, do
         for (size_t idx = t; idx < nn; idx += kWg) Ct[idx] = T0[idx];
         __syncthreads();
     }
+    partial = false;
     return sweeps;
 }
 
@@ -656,6 +668,194 @@ __device__ __forceinline__ DiisView diis_view(double* hist, int nao, int channel
     v.f = base;
     v.e = base + kDiisDepth * v.len_s;
     return v;
+}
+
+// Uniform values into scalar registers (a device function receives its arguments in vector
+// registers, and values read from LDS are vector values too).
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+template <typename T>
+__device__ __forceinline__ T* uni(T* p) {
+    const uint64_t v = (uint64_t)p;
+    const uint64_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+    const uint64_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+    return (T*)((hi << 32) | lo);
+}
+
+// Walk of a packed upper triangle of order n by wavefronts: wavefront w takes rows w, w + kWaves,
+// ... and each row in chunks of 64 columns (lane l on the chunk's column l); the rounds below take
+// kTriSlots chunks at a time and issue all of their loads before any arithmetic, so a wavefront
+// waits for memory once per round. With the diagonal (DIAG), row i holds columns i..n-1 at packed
+// offset i n - i (i - 1) / 2; without it (the strict triangle) columns i+1..n-1 at offset
+// i (n - 1) - i (i - 1) / 2. The cursor (row, chunk start) is uniform across the wavefront.
+constexpr int kTriSlots = 4;
+template <bool DIAG>
+__device__ __forceinline__ int tri_row_len(int i, int n) { return DIAG ? n - i : n - 1 - i; }
+template <bool DIAG>
+__device__ __forceinline__ void tri_cursor_next(int& ri, int& ci, int n) {
+    if (ri < n - (DIAG ? 0 : 1)) {
+        ci += 64;
+        if (ci >= tri_row_len<DIAG>(ri, n)) {
+            ri += kWaves;
+            ci = 0;
+        }
+    }
+}
+// Slot entry: packed offset and (row, column); an idle slot (past the end) points at entry (0, 0)
+// of the triangle ((0, 1) without the diagonal), so its loads stay in bounds.
+template <bool DIAG>
+__device__ __forceinline__ bool tri_slot(int ri, int ci, int n, int& pk, int& i, int& j) {
+    const int off = ci + (threadIdx.x & 63);
+    const bool ok = ri < n - (DIAG ? 0 : 1) && off < tri_row_len<DIAG>(ri, n);
+    i = ok ? ri : 0;
+    const int o = ok ? off : 0;
+    j = i + o + (DIAG ? 0 : 1);
+    pk = (DIAG ? i * n : i * (n - 1)) - i * (i - 1) / 2 + o;
+    return ok;
+}
+
+// RHF DIIS push (the history of scf.rs:560-606): writes the new entry (the packed error
+// e_ij = T_ij - T_ji of T = FP over the strict upper triangle, and the packed F) into history slot
+// (head + newest) % kDiisDepth and leaves, for each of the 2 kDiisDepth + 2 sums below, one partial
+// per wavefront in buf[q * kWaves + w] (the caller adds the kWaves partials in order):
+// q < D: <e_new, e_q> (CDIIS Gram row; q = newest gives ||e||^2); with need_d, D + q:
+// <D_new, F_q> and 2 D: <D_new, H>; 2 D + 1: E_elec = 1/2 sum P (H + F). Sums over the packed
+// triangle count off-diagonal entries once, so diagonal ones enter at half weight (the caller
+// doubles). History slot q is (head + q) % D for q < newest; the ones past the history read the
+// oldest slot and are discarded (in the first iteration that is the new entry's own slot).
+__device__ __noinline__ void diis_push_rhf(int nao_, const double* T0_, const double* F_, const double* H_,
+                                           const double* P_, double* he_base_, double* hf_base_, int len_a_,
+                                           int len_s_, int head_, int newest_, int need_d_, double* buf) {
+    constexpr int D = kDiisDepth, S = kTriSlots;
+    const int nao = uni(nao_), head = uni(head_), newest = uni(newest_), len_a = uni(len_a_), len_s = uni(len_s_);
+    const bool need_d = uni(need_d_) != 0;
+    const double* T0 = uni(T0_);
+    const double* F = uni(F_);
+    const double* H = uni(H_);
+    const double* P = uni(P_);
+    double* he_base = uni(he_base_);
+    double* hf_base = uni(hf_base_);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const double* ep[D];
+    const double* fp[D];
+#pragma unroll
+    for (int q = 0; q < D; ++q) {
+        const int r = (head + (q < newest ? q : 0)) % D;
+        ep[q] = he_base + (int64_t)r * len_a;
+        fp[q] = hf_base + (int64_t)r * len_s;
+    }
+    const int rn = (head + newest) % D;
+    double* __restrict__ he = he_base + (int64_t)rn * len_a;
+    double* __restrict__ hf = hf_base + (int64_t)rn * len_s;
+    double part[2 * D + 2];
+#pragma unroll
+    for (int q = 0; q < 2 * D + 2; ++q) part[q] = 0.0;
+    for (int ri = w, ci = 0; ri < nao - 1;) {
+        int pk[S];
+        bool ok[S];
+        double ta[S], tb[S], hv[S][D];
+#pragma unroll
+        for (int u = 0; u < S; ++u) {
+            int i, j;
+            ok[u] = tri_slot<false>(ri, ci, nao, pk[u], i, j);
+            tri_cursor_next<false>(ri, ci, nao);
+            ta[u] = T0[i * nao + j];
+            tb[u] = T0[j * nao + i];
+#pragma unroll
+            for (int q = 0; q < D; ++q) hv[u][q] = ep[q][pk[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < S; ++u) {
+            if (ok[u]) {
+                const double e = ta[u] - tb[u];
+                he[pk[u]] = e;
+#pragma unroll
+                for (int q = 0; q < D; ++q) part[q] = fma(e, q < newest ? hv[u][q] : (q == newest ? e : 0.0), part[q]);
+            }
+        }
+    }
+    for (int ri = w, ci = 0; ri < nao;) {
+        int pk[S];
+        bool ok[S], dg[S];
+        double fv[S], hv[S], pv[S], qv[S][D];
+#pragma unroll
+        for (int u = 0; u < S; ++u) {
+            int i, j;
+            ok[u] = tri_slot<true>(ri, ci, nao, pk[u], i, j);
+            tri_cursor_next<true>(ri, ci, nao);
+            dg[u] = i == j;
+            fv[u] = F[i * nao + j];
+            hv[u] = H[i * nao + j];
+            pv[u] = P[i * nao + j];
+            if (need_d) {
+#pragma unroll
+                for (int q = 0; q < D; ++q) qv[u][q] = fp[q][pk[u]];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < S; ++u) {
+            if (ok[u]) {
+                const double f = fv[u], h = hv[u];
+                const double pw = dg[u] ? 0.5 * pv[u] : pv[u];
+                hf[pk[u]] = f;
+                part[2 * D + 1] = fma(pw, h + f, part[2 * D + 1]);
+                if (need_d) {
+                    part[2 * D] = fma(pw, h, part[2 * D]);
+#pragma unroll
+                    for (int q = 0; q < D; ++q)
+                        part[D + q] = fma(pw, q < newest ? qv[u][q] : (q == newest ? f : 0.0), part[D + q]);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 2 * D + 2; ++q) {
+        const double x = wave_sum(part[q]);
+        if (lane == 0) buf[q * kWaves + w] = x;
+    }
+    __syncthreads();
+}
+
+// F <- sum_{q < m} c_q F_q (scf.rs:627-653) from the packed history (slots ring[q]), written to
+// both triangles of the dense F; returns 1.0 on a thread that produced a non-finite entry.
+__device__ __noinline__ double diis_combine(int nao_, double* F_, const double* hf_base_, int len_s_,
+                                            const int* ring, const double* coef, int m_) {
+    constexpr int D = kDiisDepth, S = kTriSlots;
+    const int nao = uni(nao_), len_s = uni(len_s_), m = uni(m_);
+    double* F = uni(F_);
+    const double* hf_base = uni(hf_base_);
+    const int w = threadIdx.x >> 6;
+    const double* fq[D];
+    double cq[D];
+#pragma unroll
+    for (int q = 0; q < D; ++q) {
+        fq[q] = hf_base + (int64_t)uni(ring[q < m ? q : m - 1]) * len_s;
+        cq[q] = q < m ? coef[q] : 0.0;
+    }
+    double bad = 0.0;
+    for (int ri = w, ci = 0; ri < nao;) {
+        int pk[S], ii[S], jj[S];
+        bool ok[S];
+        double v[S][D];
+#pragma unroll
+        for (int u = 0; u < S; ++u) {
+            ok[u] = tri_slot<true>(ri, ci, nao, pk[u], ii[u], jj[u]);
+            tri_cursor_next<true>(ri, ci, nao);
+#pragma unroll
+            for (int q = 0; q < D; ++q) v[u][q] = fq[q][pk[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < S; ++u) {
+            double acc = 0.0;
+#pragma unroll
+            for (int q = 0; q < D; ++q) acc = fma(cq[q], q < m ? v[u][q] : 0.0, acc);
+            if (ok[u]) {
+                F[ii[u] * nao + jj[u]] = acc;
+                F[jj[u] * nao + ii[u]] = acc;
+                if (!isfinite(acc)) bad = 1.0;
+            }
+        }
+    }
+    return bad;
 }
 
 // P <- Pn over n entries; returns this thread's part of sum (Pn - P)^2. Eight entries per thread
@@ -769,58 +969,14 @@ __device__ __noinline__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolS
         const int head = full ? (ms.head + 1) % kDiisDepth : ms.head;
         const int m = full ? kDiisDepth : ms.hcount + 1;
         const int newest = m - 1;
-        const double* ep[kDiisDepth];
-        const double* fp[kDiisDepth];
-#pragma unroll
-        for (int i = 0; i < kDiisDepth; ++i) {  // past the history: the oldest slot (read, unused)
-            const int64_t r = (head + (i < newest ? i : 0)) % kDiisDepth;
-            ep[i] = dv.e + r * dv.len_a;
-            fp[i] = dv.f + r * dv.len_s;
-        }
-        double* __restrict__ he = dv.e + (int64_t)((head + newest) % kDiisDepth) * dv.len_a;
-        double* __restrict__ hf = dv.f + (int64_t)((head + newest) % kDiisDepth) * dv.len_s;
         // Two passes over the packed triangles push the new entry and form, in one reduction:
-        // part[q] = <e_new, e_q> (CDIIS Gram row; q = newest gives ||e||^2), and for A-DIIS
-        // part[8 + q] = <D_new, F_q>, part[16] = <D_new, H>; part[17] = E_elec = 1/2 sum P (H + F).
-        // A matrix product <X, Y> over the packed upper triangle counts off-diagonal entries
-        // twice, so diagonal ones enter at half weight and the sum is doubled. The remaining
-        // A-DIIS entries need no density history: F_j = H + G(D_j) with G linear and symmetric
-        // (<A, G(B)> = <B, G(A)>), so <D_j, F_new> = h_j + <D_new, F_j> - h_new.
-        double part[2 * kDiisDepth + 2];
-#pragma unroll
-        for (int q = 0; q < 2 * kDiisDepth + 2; ++q) part[q] = 0.0;
-#pragma unroll 4
-        for (int64_t pk = t; pk < dv.len_a; pk += kWg) {
-            int i, j;
-            stri_rowcol(pk, nao, i, j);
-            const double e = T0[(size_t)i * nao + j] - T0[(size_t)j * nao + i];
-            he[pk] = e;
-            // every slot pointer is valid, so all loads issue without branches; the ones past
-            // the history are discarded by the selects
-#pragma unroll
-            for (int q = 0; q < kDiisDepth; ++q) {
-                const double hq = ep[q][pk];
-                part[q] = fma(e, q < newest ? hq : (q == newest ? e : 0.0), part[q]);
-            }
-        }
-#pragma unroll 4
-        for (int64_t pk = t; pk < dv.len_s; pk += kWg) {
-            int i, j;
-            tri_rowcol(pk, nao, i, j);
-            const size_t ij = (size_t)i * nao + j;
-            const double f = F[ij], h = H[ij];
-            const double pw = i == j ? 0.5 * P[ij] : P[ij];
-            hf[pk] = f;
-            part[2 * kDiisDepth + 1] = fma(pw, h + f, part[2 * kDiisDepth + 1]);
-            if (need_d) {
-                part[2 * kDiisDepth] = fma(pw, h, part[2 * kDiisDepth]);
-#pragma unroll
-                for (int q = 0; q < kDiisDepth; ++q) {
-                    const double fq = fp[q][pk];
-                    part[kDiisDepth + q] = fma(pw, q < newest ? fq : (q == newest ? f : 0.0), part[kDiisDepth + q]);
-                }
-            }
-        }
+        // <e_new, e_q> (CDIIS Gram row; q = newest gives ||e||^2), and for A-DIIS <D_new, F_q>,
+        // <D_new, H>; and E_elec = 1/2 sum P (H + F). A matrix product <X, Y> over the packed upper
+        // triangle counts off-diagonal entries twice, so diagonal ones enter at half weight and the
+        // sum is doubled. The remaining A-DIIS entries need no density history: F_j = H + G(D_j)
+        // with G linear and symmetric (<A, G(B)> = <B, G(A)>), so <D_j, F_new> = h_j + <D_new, F_j>
+        // - h_new.
+        diis_push_rhf(nao, T0, F, H, P, dv.e, dv.f, (int)dv.len_a, (int)dv.len_s, head, newest, need_d, lds);
         // The cached Gram entries move one step when the oldest entry is dropped: wavefront 0
         // reads them here, before the reduction's barriers, and writes them after, lane l
         // moving entry (l / 8, l % 8) from (l / 8 + 1, l % 8 + 1). Only stores follow the
@@ -832,8 +988,16 @@ __device__ __noinline__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolS
         double hsh[kDiisDepth];  // <D_j, H> in the new logical order
 #pragma unroll
         for (int j = 0; j < kDiisDepth; ++j) hsh[j] = full ? (j + 1 < kDiisDepth ? ms.hd[j + 1] : 0.0) : ms.hd[j];
+        // the totals of the push's partial sums (LDS is free outside the eigensolver)
         double tot[2 * kDiisDepth + 2];
-        wg_sum_many(part, lds, tot);  // LDS is free outside the eigensolver
+#pragma unroll
+        for (int q = 0; q < 2 * kDiisDepth + 2; ++q) {
+            double x = 0.0;
+#pragma unroll
+            for (int w = 0; w < kWaves; ++w) x += lds[q * kWaves + w];
+            tot[q] = x;
+        }
+        __syncthreads();
         double bnn = 0.0;
 #pragma unroll
         for (int q = 0; q < kDiisDepth; ++q)
@@ -884,44 +1048,10 @@ __device__ __noinline__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolS
         if (ms.flags[0]) {
             // F <- sum_i c_i F_i (unpacked) in place: F itself is history slot `newest` and is
             // rebuilt next pass; a non-finite combination puts that slot back
-            double bad = 0.0;
-            const double* fq[kDiisDepth];
-            double cq[kDiisDepth];
-#pragma unroll
-            for (int q = 0; q < kDiisDepth; ++q) {
-                fq[q] = dv.f + (int64_t)ms.ring[q < m ? q : m - 1] * dv.len_s;
-                cq[q] = q < m ? ms.coef[q] : 0.0;
-            }
-            // four entries per thread per round, every load before any store (F does not alias
-            // the history, but the compiler cannot know it); slots past the history repeat the
-            // newest and are discarded by the selects
-            constexpr int U = 4;
-            for (int64_t base = t; base < dv.len_s; base += (int64_t)U * kWg) {
-                double v[U][kDiisDepth];
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const int64_t pk = min(base + (int64_t)u * kWg, dv.len_s - 1);
-#pragma unroll
-                    for (int q = 0; q < kDiisDepth; ++q) v[u][q] = fq[q][pk];
-                }
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const int64_t pk = base + (int64_t)u * kWg;
-                    double acc = 0.0;
-#pragma unroll
-                    for (int q = 0; q < kDiisDepth; ++q) acc = fma(cq[q], q < m ? v[u][q] : 0.0, acc);
-                    if (pk < dv.len_s) {
-                        int i, j;
-                        tri_rowcol(pk, nao, i, j);
-                        F[(size_t)i * nao + j] = acc;
-                        F[(size_t)j * nao + i] = acc;
-                        if (!isfinite(acc)) bad = 1.0;
-                    }
-                }
-            }
+            double bad = diis_combine(nao, F, dv.f, (int)dv.len_s, ms.ring, ms.coef, m);
             bad = wg_max(bad, s);
             if (bad != 0.0) {
-                const double* fn = fq[m - 1];
+                const double* fn = dv.f + (int64_t)ms.ring[m - 1] * dv.len_s;
                 for (int64_t pk = t; pk < dv.len_s; pk += kWg) {
                     int i, j;
                     tri_rowcol(pk, nao, i, j);
@@ -933,11 +1063,14 @@ __device__ __noinline__ void scf_rhf(const MolDesc& md, const BatchArgs& g, MolS
         const double* Fuse = F;
         __syncthreads();
         phase_mark(pc, 2);
+        // the occupied orbitals are all the density needs; every one when the caller takes the
+        // coefficients
+        bool partial = false;
         int sweeps = mol_eigen<double>(nao, Fuse, Ct, T0, T1, lds, g.lds_n, g.tri_max, have_c, eps, s, 1.0e-7 * err,
-                                       100.0 * err);
+                                       iter > 0 ? 100.0 * err : 0.0, md.all_vecs ? nao : nocc, partial);
         phase_mark(pc, 3);
         total_sweeps += sweeps;
-        have_c = true;
+        have_c = !partial;
         if (sweeps < 0) {
             status = kLinearAlgebra;
             break;
@@ -1220,13 +1353,17 @@ __device__ __noinline__ void scf_uhf(const MolDesc& md, const BatchArgs& g, MolS
             break;
         }
         phase_mark(pc, 2);
+        bool full_c = true;
         for (int c = 0; c < 2; ++c) {
+            bool partial = false;
             int sw = mol_eigen<double>(nao, Fuse[c], Cs[c], T0, T1, lds, g.lds_n, g.tri_max, have_c, eps[c], s,
-                                      1.0e-7 * err, 100.0 * err);
+                                      1.0e-7 * err, iter > 0 ? 100.0 * err : 0.0,
+                                      md.all_vecs ? nao : (c == 0 ? md.n_alpha : md.n_beta), partial);
             if (sw < 0) status = kLinearAlgebra;
             else total_sweeps += sw;
+            full_c = full_c && !partial;
         }
-        have_c = true;
+        have_c = full_c;
         phase_mark(pc, 3);
         if (status != kOk) break;
         double dsum[2];
@@ -2093,8 +2230,9 @@ __global__ __launch_bounds__(kWg) void k_dc_eigen(DcArgs d, const double* F, int
         Fs[(size_t)j * n + i] = v;
     }
     __syncthreads();
+    bool partial = false;
     const int sw = mol_eigen<double>(n, Fs, Ct, d.t0 + d.sub_mat[sub], d.t1 + d.sub_mat[sub], lds_dyn, d.lds_n, d.tri_max,
-                                     d.warm != 0, d.eps + d.sub_ao[sub], s, d.tol_abs, 0.0);
+                                     d.warm != 0, d.eps + d.sub_ao[sub], s, d.tol_abs, 0.0, n, partial);
     const unsigned char* core = d.ao_core + d.sub_ao[sub];
     for (int i = t / 16; i < n; i += kWg / 16) {
         double acc = 0.0;

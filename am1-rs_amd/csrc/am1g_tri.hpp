@@ -513,10 +513,13 @@ __device__ __forceinline__ double tri_spectrum(int n, double* __restrict__ d, do
 // 1e-6 eV). bracket > 0: eps holds
 // the eigenvalues of a nearby matrix (the previous SCF iteration's), and the search for
 // eigenvalue j starts from [eps_j - bracket, eps_j + bracket] when Sturm counts confirm that it
-// contains exactly that one.
+// contains exactly that one. nvec: only the eigenvectors of the nvec lowest eigenvalues are formed
+// and written (rows nvec.. of Ct are left as they were); the others cost no back transformation.
+// An eigenvector is computed the same way whatever nvec is, so the rows written do not depend on it.
 __device__ __noinline__ int wg_eigh_tri(int n, const double* __restrict__ F, double* __restrict__ Ct,
                                         double* __restrict__ eps_out, double* __restrict__ Vg,
-                                        double* __restrict__ lds, int lds_cap, WgScratch& s, double bracket) {
+                                        double* __restrict__ lds, int lds_cap, WgScratch& s, double bracket,
+                                        int nvec) {
     const int t = threadIdx.x, lane = t & 63;
     const int ld = tri_ld(n), sv = tri_vec_stride(n);
     double* A = lds;               // eigenvectors of T (step 4), reflectors (step 5)
@@ -530,6 +533,8 @@ __device__ __noinline__ int wg_eigh_tri(int n, const double* __restrict__ F, dou
     int* bs = (int*)(pub + 2 * sv);        // block start of each row
     int* be = bs + n;                      // block end (exclusive)
     int* rk = (int*)vb;                    // rank of each eigenvalue (vb is free after step 1)
+    int* inv = (int*)pb;                   // the eigenvalue of each rank (pb is free after step 1)
+    int* need = inv + kTriMax;             // whether eigenvector j of T is formed
     double2* de = (double2*)(lds + tri_de_offset(n));  // (d_i, e_{i-1}^2)
     // Register blocking: a group of eight threads owns rows 2g and 2g + 1 (g = t / 8); thread q
     // of the group owns columns 2q + 16j + {0, 1}, j = 0..7, of both rows, in zr[b][2j + h].
@@ -568,83 +573,93 @@ __device__ __noinline__ int wg_eigh_tri(int n, const double* __restrict__ F, dou
     double vown[16];    // the previous reflector on the owned columns
 #pragma unroll
     for (int u = 0; u < 16; ++u) vown[u] = 0.0;
+    const int wlast = (n - 1) >> 4;  // the wavefront that owns row n - 1 (active in every step)
     for (int k = 0; k + 2 < n; ++k) {
-        const double* vo = vb + (par ^ 1) * sv;
-        const double* po = pb + (par ^ 1) * sv;
-        double* vn = vb + par * sv;
-        double* pn = pb + par * sv;
-        const double* rowk = pub + (k & 1) * sv;
-        // K = (tau/2) p.v of the previous reflector (its support starts at row k)
-        double pv = 0.0;
-        for (int i = k + lane; i < n; i += 64) pv = fma(po[i], vo[i], pv);
-        const double Ko = 0.5 * tauo * wave_sum(pv);
-        // column k (= row k) after the previous update: x_r = A[k][r] - v_r w_k - w_r v_k for
-        // r >= k, rows lane and 64 + lane
-        const double vk = vo[k], wk = fma(-Ko, vk, po[k]);
-        const int ra = lane, rb = 64 + lane;
-        double xa = 0.0, xb = 0.0;
-        if (ra >= k && ra < n) {
-            const double v = vo[ra];
-            xa = fma(-v, wk, fma(-fma(-Ko, v, po[ra]), vk, rowk[ra]));
-        }
-        if (rb >= k && rb < n) {
-            const double v = vo[rb];
-            xb = fma(-v, wk, fma(-fma(-Ko, v, po[rb]), vk, rowk[rb]));
-        }
-        // the diagonal entry and the reflector's leading entry, by the same formula
-        const double dk = fma(-vk, wk, fma(-fma(-Ko, vk, po[k]), vk, rowk[k]));
-        const double v1 = vo[k + 1];
-        const double x0 = fma(-v1, wk, fma(-fma(-Ko, v1, po[k + 1]), vk, rowk[k + 1]));
-        double sig = ra >= k + 2 ? xa * xa : 0.0;
-        if (rb >= k + 2) sig = fma(xb, xb, sig);
-        sig = wave_sum(sig);
-        // beta = -sign(x0) |x|, tau = (beta - x0) / beta = 1 + |x0| / |x|, v = x / (x0 - beta);
-        // 1/|x| and the last reciprocal from refined hardware estimates (a few ulps)
-        double tk = 0.0, beta = x0, scale = 0.0;
-        if (sig > 0.0) {
-            const double nx2 = fma(x0, x0, sig);
-            const double rn = (nx2 > 1.0e-280 && nx2 < 1.0e280) ? fast_rsqrt(nx2) : 1.0 / sqrt(nx2);
-            beta = -copysign(nx2 * rn, x0);
-            tk = fma(fabs(x0), rn, 1.0);
-            const double den = x0 - beta;
-            scale = (fabs(den) > 1.0e-280 && fabs(den) < 1.0e280) ? fast_rcp(den) : 1.0 / den;
-        }
-        const double va = ra == k + 1 ? 1.0 : (ra >= k + 2 ? xa * scale : 0.0);
-        const double vbv = rb == k + 1 ? 1.0 : (rb >= k + 2 ? xb * scale : 0.0);
-        // every wavefront writes the same reflector (zero above its support), so each reads back
-        // what it wrote; rows <= k of the new product are cleared the same way
-        if (ra < n) {
-            vn[ra] = va;
-            if (ra <= k) pn[ra] = 0.0;
-        }
-        if (rb < n) {
-            vn[rb] = vbv;
-            if (rb <= k) pn[rb] = 0.0;
-        }
-        if (t < 64) {
-            if (ra > k && ra < n) Vg[(size_t)k * n + ra] = va;
-            if (rb > k && rb < n) Vg[(size_t)k * n + rb] = vbv;
-            if (lane == 0) {
-                d[k] = dk;
-                e[k] = beta;
-                tau[k] = tk;
-            }
-        }
-        // owned rows k+1..: previous update (zero on the columns it does not touch, since the
-        // previous v and p vanish there) and this reflector's product p_r = tau sum_c a_rc v_c
-        const int r0 = 2 * g8, r1 = r0 + 1;
-        const bool on0 = r0 > k && r0 < n, on1 = r1 > k && r1 < n;
-        const double vr0 = vo[r0 < n ? r0 : 0], wr0 = fma(-Ko, vr0, po[r0 < n ? r0 : 0]);
-        const double vr1 = vo[r1 < n ? r1 : 0], wr1 = fma(-Ko, vr1, po[r1 < n ? r1 : 0]);
-        double s0 = 0.0, s1 = 0.0;
         // Rows and columns below k are finished: there the previous v and p vanish, so the update
         // leaves them unchanged and the new reflector adds nothing to the product. A wavefront
-        // whose rows are all <= k skips the work (row k is not read again), and so does every
-        // quarter of the columns that lies below k (both tests are uniform across a wavefront).
+        // whose rows are all <= k has nothing left to do (row k is not read again) and skips the
+        // step, reflector included, which leaves the SIMD's issue slots to the others; so does
+        // every quarter of the columns that lies below k (both tests are uniform across a
+        // wavefront).
         if (16 * (t >> 6) + 15 > k && 16 * (t >> 6) < n) {
-            // columns 2 q8 + 16 j: the pairs are 16-byte aligned
-            auto fused = [&](int j, double2 p2, double2 u2, double& acc0, double& acc1) {
-                const double w0 = fma(-Ko, vown[2 * j], p2.x), w1 = fma(-Ko, vown[2 * j + 1], p2.y);
+            const double* vo = vb + (par ^ 1) * sv;
+            const double* po = pb + (par ^ 1) * sv;
+            double* vn = vb + par * sv;
+            double* pn = pb + par * sv;
+            const double* rowk = pub + (k & 1) * sv;
+            const int ra = lane, rb = 64 + lane;
+            const int r0 = 2 * g8, r1 = r0 + 1;
+            // The step's LDS reads that do not depend on its reflector, issued together ahead of
+            // the arithmetic. The vectors hold kTriMax entries (zero beyond n), so the reads
+            // need no bounds tests; entries of the published row beyond n are masked.
+            const int i0 = min(k + lane, sv - 1), i1 = min(k + 64 + lane, sv - 1);
+            const double pv0 = po[i0], vv0 = vo[i0], pv1 = po[i1], vv1 = vo[i1];
+            const double vra = vo[ra], pra = po[ra], ara = rowk[ra];
+            const double vrb = vo[rb], prb = po[rb], arb = rowk[rb];
+            const double vk = vo[k], pk = po[k], akk = rowk[k];
+            const double v1 = vo[k + 1], p1 = po[k + 1], ak1 = rowk[k + 1];
+            const double vr0 = vo[r0], pr0 = po[r0], vr1 = vo[r1], pr1 = po[r1];
+            double2 p2[8];  // the previous product on the owned columns (16-byte aligned pairs)
+#pragma unroll
+            for (int j = 0; j < 8; ++j) p2[j] = *(const double2*)(po + 2 * q8 + 16 * j);
+            // K = (tau/2) p.v of the previous reflector (its support starts at row k)
+            double pv = k + lane < n ? fma(pv0, vv0, 0.0) : 0.0;
+            if (k + 64 + lane < n) pv = fma(pv1, vv1, pv);
+            const double Ko = 0.5 * tauo * wave_sum(pv);
+            // column k (= row k) after the previous update: x_r = A[k][r] - v_r w_k - w_r v_k
+            // for r >= k, rows lane and 64 + lane
+            const double wk = fma(-Ko, vk, pk);
+            const double xa = (ra >= k && ra < n) ? fma(-vra, wk, fma(-fma(-Ko, vra, pra), vk, ara)) : 0.0;
+            const double xb = (rb >= k && rb < n) ? fma(-vrb, wk, fma(-fma(-Ko, vrb, prb), vk, arb)) : 0.0;
+            // the diagonal entry and the reflector's leading entry, by the same formula
+            const double dk = fma(-vk, wk, fma(-fma(-Ko, vk, pk), vk, akk));
+            const double x0 = fma(-v1, wk, fma(-fma(-Ko, v1, p1), vk, ak1));
+            double sig = ra >= k + 2 ? xa * xa : 0.0;
+            if (rb >= k + 2) sig = fma(xb, xb, sig);
+            sig = wave_sum(sig);
+            // beta = -sign(x0) |x|, tau = (beta - x0) / beta = 1 + |x0| / |x|, v = x / (x0 - beta);
+            // 1/|x| and the last reciprocal from refined hardware estimates (a few ulps)
+            double tk = 0.0, beta = x0, scale = 0.0;
+            if (sig > 0.0) {
+                const double nx2 = fma(x0, x0, sig);
+                const double rn = (nx2 > 1.0e-280 && nx2 < 1.0e280) ? fast_rsqrt(nx2) : 1.0 / sqrt(nx2);
+                beta = -copysign(nx2 * rn, x0);
+                tk = fma(fabs(x0), rn, 1.0);
+                const double den = x0 - beta;
+                scale = (fabs(den) > 1.0e-280 && fabs(den) < 1.0e280) ? fast_rcp(den) : 1.0 / den;
+            }
+            const double va = ra == k + 1 ? 1.0 : (ra >= k + 2 ? xa * scale : 0.0);
+            const double vbv = rb == k + 1 ? 1.0 : (rb >= k + 2 ? xb * scale : 0.0);
+            // every active wavefront writes the same reflector (zero above its support), so each
+            // reads back what it wrote; rows <= k of the new product are cleared the same way
+            if (ra < n) {
+                vn[ra] = va;
+                if (ra <= k) pn[ra] = 0.0;
+            }
+            if (rb < n) {
+                vn[rb] = vbv;
+                if (rb <= k) pn[rb] = 0.0;
+            }
+            if ((t >> 6) == wlast) {
+                if (ra > k && ra < n) Vg[(size_t)k * n + ra] = va;
+                if (rb > k && rb < n) Vg[(size_t)k * n + rb] = vbv;
+                if (lane == 0) {
+                    d[k] = dk;
+                    e[k] = beta;
+                    tau[k] = tk;
+                }
+            }
+            // owned rows k+1..: previous update (zero on the columns it does not touch, since
+            // the previous v and p vanish there) and this reflector's product
+            // p_r = tau sum_c a_rc v_c
+            const bool on0 = r0 > k && r0 < n, on1 = r1 > k && r1 < n;
+            const double wr0 = fma(-Ko, vr0, pr0), wr1 = fma(-Ko, vr1, pr1);
+            double2 u2[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) u2[j] = *(const double2*)(vn + 2 * q8 + 16 * j);
+            auto fused = [&](int j, double& acc0, double& acc1) {
+                const double2 p = p2[j], u = u2[j];
+                const double w0 = fma(-Ko, vown[2 * j], p.x), w1 = fma(-Ko, vown[2 * j + 1], p.y);
                 const double a00 = fma(-vr0, w0, fma(-wr0, vown[2 * j], zr[0][2 * j]));
                 const double a01 = fma(-vr0, w1, fma(-wr0, vown[2 * j + 1], zr[0][2 * j + 1]));
                 const double a10 = fma(-vr1, w0, fma(-wr1, vown[2 * j], zr[1][2 * j]));
@@ -653,32 +668,18 @@ __device__ __noinline__ int wg_eigh_tri(int n, const double* __restrict__ F, dou
                 zr[0][2 * j + 1] = a01;
                 zr[1][2 * j] = a10;
                 zr[1][2 * j + 1] = a11;
-                acc0 = fma(a00, u2.x, fma(a01, u2.y, acc0));
-                acc1 = fma(a10, u2.x, fma(a11, u2.y, acc1));
-                vown[2 * j] = u2.x;  // this reflector is the next step's previous one
-                vown[2 * j + 1] = u2.y;
+                acc0 = fma(a00, u.x, fma(a01, u.y, acc0));
+                acc1 = fma(a10, u.x, fma(a11, u.y, acc1));
+                vown[2 * j] = u.x;  // this reflector is the next step's previous one
+                vown[2 * j + 1] = u.y;
             };
-            // each half's loads are issued together, ahead of its quarters' arithmetic (one LDS
-            // round trip per half); odd column blocks accumulate separately (shorter chains)
-            double u0 = 0.0, u1 = 0.0;
+            // odd column blocks accumulate separately (shorter chains)
+            double s0 = 0.0, s1 = 0.0, u0 = 0.0, u1 = 0.0;
 #pragma unroll
-            for (int hf = 0; hf < 2; ++hf) {
-                if (64 * hf + 64 > k && 64 * hf < n) {
-                    double2 p2[4], u2[4];
-#pragma unroll
-                    for (int jj = 0; jj < 4; ++jj) {
-                        const int c = 2 * q8 + 16 * (4 * hf + jj);
-                        p2[jj] = *(const double2*)(po + c);
-                        u2[jj] = *(const double2*)(vn + c);
-                    }
-#pragma unroll
-                    for (int qq = 0; qq < 2; ++qq) {
-                        const int jq = 2 * hf + qq;
-                        if (32 * jq + 32 > k && 32 * jq < n) {
-                            fused(2 * jq, p2[2 * qq], u2[2 * qq], s0, s1);
-                            fused(2 * jq + 1, p2[2 * qq + 1], u2[2 * qq + 1], u0, u1);
-                        }
-                    }
+            for (int jq = 0; jq < 4; ++jq) {
+                if (32 * jq + 32 > k && 32 * jq < n) {
+                    fused(2 * jq, s0, s1);
+                    fused(2 * jq + 1, u0, u1);
                 }
             }
             s0 += u0;
@@ -689,22 +690,22 @@ __device__ __noinline__ int wg_eigh_tri(int n, const double* __restrict__ F, dou
             s1 += dpp_move<0x4E>(s1);
             s0 += dpp_move<0x141>(s0);
             s1 += dpp_move<0x141>(s1);
-        }
-        if (q8 == 0) {
-            if (on0) pn[r0] = tk * s0;
-            if (on1) pn[r1] = tk * s1;
-        }
-        if (r0 == k + 1 || r1 == k + 1) {  // publish row k+1 for the next step
-            const int b = r1 == k + 1;
-            double* nxt = pub + ((k + 1) & 1) * sv;
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                const int c = 2 * q8 + 16 * j;
-                *(double2*)(nxt + c) = b ? make_double2(zr[1][2 * j], zr[1][2 * j + 1])
-                                         : make_double2(zr[0][2 * j], zr[0][2 * j + 1]);
+            if (q8 == 0) {
+                if (on0) pn[r0] = tk * s0;
+                if (on1) pn[r1] = tk * s1;
             }
+            if (r0 == k + 1 || r1 == k + 1) {  // publish row k+1 for the next step
+                const int b = r1 == k + 1;
+                double* nxt = pub + ((k + 1) & 1) * sv;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const int c = 2 * q8 + 16 * j;
+                    *(double2*)(nxt + c) = b ? make_double2(zr[1][2 * j], zr[1][2 * j + 1])
+                                             : make_double2(zr[0][2 * j], zr[0][2 * j + 1]);
+                }
+            }
+            tauo = tk;
         }
-        tauo = tk;
         par ^= 1;
         __syncthreads();
     }
@@ -751,8 +752,31 @@ __device__ __noinline__ int wg_eigh_tri(int n, const double* __restrict__ F, dou
     const double nrm = tri_spectrum(n, d, e, lam, bs, be, de, (int*)A, n * ld >= kWg, eps_out, bracket, s);
     const double pivmin = 1.0e-280 * fmax(1.0, nrm * nrm);
 
-    // ---- 4. eigenvectors of T (row j of the LDS matrix)
+    // ---- ranks (ascending, stable) of the eigenvalues, and the eigenvectors of T to form: those
+    // of the nvec lowest eigenvalues, and every member of a run of close eigenvalues (inside one
+    // block) that holds one of them, since Gram-Schmidt below treats a run as a whole
+    for (int i = t / 16; i < n; i += kWg / 16) {
+        const double ei = lam[i];
+        int rank = 0;
+        for (int j = t % 16; j < n; j += 16) rank += (lam[j] < ei) || (lam[j] == ei && j < i);
+        for (int off = 8; off > 0; off >>= 1) rank += __shfl_xor(rank, off, 16);
+        if (t % 16 == 0) {
+            rk[i] = rank;
+            inv[rank] = i;
+        }
+    }
+    __syncthreads();
+    const double gap = 1.0e-6 * nrm;
     if (t < n) {
+        bool nd = rk[t] < nvec;
+        for (int a = t; a > bs[t] && lam[a] - lam[a - 1] < gap; --a) nd = nd || rk[a - 1] < nvec;
+        for (int a = t + 1; a < be[t] && lam[a] - lam[a - 1] < gap; ++a) nd = nd || rk[a] < nvec;
+        need[t] = nd ? 1 : 0;
+    }
+    __syncthreads();
+
+    // ---- 4. eigenvectors of T (row j of the LDS matrix)
+    if (t < n && need[t]) {
         const int j = t;
         double* z = A + j * ld;
         const int b0 = bs[j], b1 = be[j];
@@ -765,7 +789,7 @@ __device__ __noinline__ int wg_eigh_tri(int n, const double* __restrict__ F, dou
             // position inside a run of close eigenvalues: the q-th member takes the q-th best
             // twist, so the run's vectors are independent before Gram-Schmidt
             int q = 0;
-            while (q < 3 && j - q - 1 >= b0 && lam[j - q] - lam[j - q - 1] < 1.0e-6 * nrm) ++q;
+            while (q < 3 && j - q - 1 >= b0 && lam[j - q] - lam[j - q - 1] < gap) ++q;
             ok = tri_vector(z, de, e, b0, b1, lam[j], q, pivmin);
         }
         if (!ok) s.bcast[2] = 1.0;
@@ -773,14 +797,13 @@ __device__ __noinline__ int wg_eigh_tri(int n, const double* __restrict__ F, dou
     __syncthreads();
 
     // ---- close eigenvalues inside one block: Gram-Schmidt over each run (one wavefront; rare)
-    if (t + 1 < n && bs[t + 1] == bs[t] && lam[t + 1] - lam[t] < 1.0e-6 * nrm) s.bcast[3] = 1.0;
+    if (t + 1 < n && need[t] && bs[t + 1] == bs[t] && lam[t + 1] - lam[t] < gap) s.bcast[3] = 1.0;
     __syncthreads();
     if (s.bcast[3] != 0.0 && t < 64) {
-        const double gap = 1.0e-6 * nrm;
         for (int j0 = 0; j0 < n;) {
             int j1 = j0 + 1;
             while (j1 < n && bs[j1] == bs[j0] && lam[j1] - lam[j1 - 1] < gap) ++j1;
-            if (j1 - j0 > 1) {
+            if (j1 - j0 > 1 && need[j0]) {
                 const int b0 = bs[j0], b1 = be[j0];
                 for (int a = j0 + 1; a < j1; ++a) {
                     double* za = A + a * ld;
@@ -798,8 +821,8 @@ __device__ __noinline__ int wg_eigh_tri(int n, const double* __restrict__ F, dou
                         if (lane == 0) s.bcast[2] = 1.0;
                         ss = 1.0;
                     }
-                    const double inv = 1.0 / sqrt(ss);
-                    for (int i = b0 + lane; i < b1; i += 64) za[i] *= inv;
+                    const double rs = 1.0 / sqrt(ss);
+                    for (int i = b0 + lane; i < b1; i += 64) za[i] *= rs;
                 }
             }
             j0 = j1;
@@ -810,31 +833,23 @@ __device__ __noinline__ int wg_eigh_tri(int n, const double* __restrict__ F, dou
     __syncthreads();
     if (failed) return -1;
 
-    // ---- 5. ranks (ascending, stable), eigenvalues out
-    for (int i = t / 16; i < n; i += kWg / 16) {
-        const double ei = lam[i];
-        int rank = 0;
-        for (int j = t % 16; j < n; j += 16) rank += (lam[j] < ei) || (lam[j] == ei && j < i);
-        for (int off = 8; off > 0; off >>= 1) rank += __shfl_xor(rank, off, 16);
-        if (t % 16 == 0) {
-            eps_out[rank] = ei;
-            rk[i] = rank;
-        }
-    }
-    __syncthreads();
+    // ---- 5. eigenvalues out (ascending)
+    for (int i = t; i < n; i += kWg) eps_out[rk[i]] = lam[i];
     // back transformation x^T = z^T H_{n-3} ... H_0: rows 2g, 2g + 1 of the eigenvectors of T
     // into the owners' registers, then all reflectors into the freed LDS area (even stride sr;
     // (n - 2) sr <= n ld)
     const int sr = (n + 1) & ~1;
+    // (group g takes the eigenvectors of ranks 2g and 2g + 1, below nvec)
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
         const int r = 2 * g8 + b;
+        const int jr = r < nvec ? inv[r] : 0;
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 const int c = 2 * q8 + 16 * j + h;
-                zr[b][2 * j + h] = (r < n && c < n) ? A[r * ld + c] : 0.0;
+                zr[b][2 * j + h] = (r < nvec && c < n) ? A[jr * ld + c] : 0.0;
             }
         }
     }
@@ -845,7 +860,7 @@ __device__ __noinline__ int wg_eigh_tri(int n, const double* __restrict__ F, dou
         A[idx] = (c > k && c < n) ? Vg[(size_t)k * n + c] : 0.0;
     }
     __syncthreads();
-    if (2 * g8 < n) {
+    if (2 * g8 < nvec) {
         for (int k = n - 3; k >= 0; --k) {
             // tau = 0 (a column that needed no reflection) makes the step a no-op
             const double tk = tau[k];
@@ -900,8 +915,8 @@ __device__ __noinline__ int wg_eigh_tri(int n, const double* __restrict__ F, dou
 #pragma unroll
         for (int b = 0; b < 2; ++b) {
             const int r = 2 * g8 + b;
-            if (r >= n) continue;
-            double* dst = Ct + (size_t)rk[r] * n;
+            if (r >= nvec) continue;
+            double* dst = Ct + (size_t)r * n;
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
                 const int c = 2 * q8 + 16 * j;
