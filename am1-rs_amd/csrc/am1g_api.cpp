@@ -53,6 +53,7 @@ bool set_tunable(Tunables& t, const std::string& name, double v) {
     else if (name == "pairs_block") t.pairs_block = (v == 64.0 || v == 128.0) ? (int)v : 256;
     else if (name == "tri_blocked_min") t.tri_blocked_min = std::max(3, (int)v);
     else if (name == "batch_own_pairs") t.batch_own_pairs = v != 0.0;
+    else if (name == "sp2") t.sp2 = v != 0.0;
     else return false;
     return true;
 }
@@ -66,6 +67,7 @@ bool get_tunable(const Tunables& t, const std::string& name, double& v) {
     else if (name == "pairs_block") v = t.pairs_block;
     else if (name == "tri_blocked_min") v = t.tri_blocked_min;
     else if (name == "batch_own_pairs") v = t.batch_own_pairs;
+    else if (name == "sp2") v = t.sp2;
     else return false;
     return true;
 }

@@ -74,6 +74,7 @@ struct Tunables {
     int pairs_block = 256;               // threads per workgroup of the pair-integral kernel
     int tri_blocked_min = 1 << 30;       // large eigensolver: tridiagonalization in panels from this order
     int batch_own_pairs = 1;             // batched launch: workgroups compute their own pair integrals
+    int sp2 = 1;                         // large-system SCF: densities by SP2 purification (0: eigenvectors)
 };
 
 struct am1g_ctx {

@@ -49,6 +49,73 @@ double ms_since(Clock::time_point& t) {
     return d;
 }
 
+// Density of the nocc lowest orbitals of the symmetric n x n matrix F (the extrapolated Fock
+// matrix of one channel) by second-order spectral projection (SP2; A. M. N. Niklasson, Phys. Rev.
+// B 66, 155115 (2002)), without eigenvectors: X_0 = (e_max I - F) / (e_max - e_min) from the
+// Gershgorin bounds of F, then X <- X^2 when that brings tr X closer to nocc and X <- 2X - X^2
+// otherwise. Each step is one n x n x n product on the matrix cores. The idempotency error
+// tr X - tr X^2 falls quadratically once the eigenvalues of X have left the middle of [0, 1]; the
+// run stops when it no longer falls below its value two steps back (it has reached rounding) or
+// at zero. The result w X (w = 2 for a closed shell, 1 for one spin) is left in *out (X or X2).
+// T = float runs the products in FP32 (the FP32 phase of mixed precision). ok is false when the
+// trace did not reach nocc (a degenerate Fermi level, or no convergence in kMaxSteps): the
+// caller diagonalizes instead. rows: 3n doubles of device scratch; dscal: one device double.
+template <typename T>
+hipError_t sp2_density(hipStream_t s, int n, const double* F, int nocc, double w, T* X, T* X2, double* rows,
+                       double* dscal, T** out, bool& ok, int& steps) {
+    constexpr int kMaxSteps = 100;
+    ok = false;
+    steps = 0;
+    *out = X;
+    hipError_t e;
+    std::vector<double> hr(3 * (size_t)n);
+    if ((e = launch_gersh(n, F, rows, s)) != hipSuccess) return e;
+    if ((e = hipMemcpyAsync(hr.data(), rows, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+    if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+    double emin = INFINITY, emax = -INFINITY, trf = 0.0;
+    for (int i = 0; i < n; ++i) {
+        emin = std::min(emin, hr[3 * (size_t)i]);
+        emax = std::max(emax, hr[3 * (size_t)i + 1]);
+        trf += hr[3 * (size_t)i + 2];
+    }
+    if (!(emax > emin) || !std::isfinite(emax - emin)) return hipSuccess;
+    const double scale = 1.0 / (emax - emin);
+    if ((e = launch_sp2_init<T>(n, F, emax, scale, X, s)) != hipSuccess) return e;
+    double trx = scale * ((double)n * emax - trf);
+    double e1 = INFINITY, e2 = INFINITY;  // idempotency errors one and two steps back
+    bool stopped = false;
+    for (int k = 0; k < kMaxSteps && !stopped; ++k) {
+        if constexpr (sizeof(T) == 4)
+            e = gemm_f32(s, false, false, n, n, n, 1.0f, X, n, X, n, 0.0f, X2, n);
+        else
+            e = gemm_f64(s, false, false, n, n, n, 1.0, X, n, X, n, 0.0, X2, n);
+        if (e != hipSuccess) return e;
+        double trx2 = 0.0;
+        if ((e = launch_trace<T>(n, X2, dscal, s)) != hipSuccess) return e;
+        if ((e = hipMemcpyAsync(&trx2, dscal, sizeof(double), hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+        if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+        if (!std::isfinite(trx2)) return hipSuccess;
+        const double idem = trx - trx2;
+        if (std::fabs(trx2 - nocc) < std::fabs(2.0 * trx - trx2 - nocc)) {
+            trx = trx2;
+        } else {
+            if ((e = launch_sp2_combine<T>((int64_t)n * n, X, X2, s)) != hipSuccess) return e;
+            trx = 2.0 * trx - trx2;
+        }
+        std::swap(X, X2);
+        ++steps;
+        // the error of an eigenvalue above 0.618 grows under squaring, so the stall test applies
+        // only once the error is small
+        stopped = idem <= 0.0 || (k >= 2 && idem >= e2 && idem < 1.0e-3 * n);
+        e2 = e1;
+        e1 = idem;
+    }
+    ok = stopped && std::fabs(trx - nocc) < (sizeof(T) == 4 ? 1.0e-1 : 1.0e-4);
+    if (ok && (e = launch_scale<T>((int64_t)n * n, w, X, s)) != hipSuccess) return e;
+    *out = X;
+    return hipSuccess;
+}
+
 }  // namespace
 
 // Unrestricted SCF of one large molecule (uhf_loop, scf.rs:719-858): F_s = H + J(Pa + Pb) - K(P_s),
@@ -122,6 +189,8 @@ static int large_uhf(am1g_ctx* ctx, const am1g_request& rq, const MolDesc& md, c
     const double one = 1.0, zero = 0.0;
     const float onef = 1.0f, zerof = 0.0f;
     std::string eig_err;
+    const double* last_f[2] = {nullptr, nullptr};  // Fock matrices whose SP2 densities the loop ended with
+    int sp2_steps = 0;
     auto fock_energy = [&]() -> int {
         HIP_TRY(ctx, launch_add((int64_t)nn, P[0], P[1], Pt, s));
         HIP_TRY(ctx, launch_fock(sa, Pt, P[0], 1.0, F[0], s, partial, scal + 1));
@@ -194,7 +263,28 @@ static int large_uhf(am1g_ctx* ctx, const am1g_request& rq, const MolDesc& md, c
             break;
         }
         phase[2] += ms_since(t_ph);
+        // each channel's new density by SP2 purification of Fuse[c] (kept for the orbital energies
+        // at the end), or from its eigenvectors when that declines or is switched off
+        bool by_sp2[2] = {false, false};
         for (int c = 0; c < 2; ++c) {
+            if (!ctx->tun.sp2 || nocc[c] <= 0 || nocc[c] >= n) continue;
+            int steps = 0;
+            if (low) {
+                float* X = nullptr;
+                HIP_TRY(ctx, sp2_density<float>(s, n, Fuse[c], nocc[c], 1.0, L.f32a.as<float>(), L.f32b.as<float>(),
+                                                partial, scal + 48, &X, by_sp2[c], steps));
+                if (by_sp2[c]) HIP_TRY(ctx, launch_density_update(n, X, true, P[c], partial, scal + 21 + c, s));
+            } else {
+                double* X = nullptr;
+                HIP_TRY(ctx, sp2_density<double>(s, n, Fuse[c], nocc[c], 1.0, T0, Pn, partial, scal + 48, &X,
+                                                 by_sp2[c], steps));
+                if (by_sp2[c]) HIP_TRY(ctx, launch_density_update(n, X, false, P[c], partial, scal + 21 + c, s));
+            }
+            sp2_steps += steps;
+        }
+        for (int c = 0; c < 2; ++c) {
+            last_f[c] = by_sp2[c] ? Fuse[c] : nullptr;
+            if (by_sp2[c]) continue;
             hipError_t e = eig_large(*ctx->eig, n, const_cast<double*>(Fuse[c]), eps[c], Ct[c], s, eig_err);
             if (e != hipSuccess) {
                 if (eig_err.empty())
@@ -208,6 +298,12 @@ static int large_uhf(am1g_ctx* ctx, const am1g_request& rq, const MolDesc& md, c
         if (low) ++fp32_iters;
         double dsum = 0.0;
         for (int c = 0; c < 2; ++c) {
+            if (by_sp2[c]) {
+                HIP_TRY(ctx, hipMemcpyAsync(h_scal + 21 + c, scal + 21 + c, sizeof(double), hipMemcpyDeviceToHost, s));
+                HIP_TRY(ctx, hipStreamSynchronize(s));
+                dsum += std::sqrt(h_scal[21 + c] / (double)std::max<size_t>(1, nn));
+                continue;
+            }
             if (nocc[c] > 0 && low) {
                 float* Cf = L.f32a.as<float>();
                 float* Pf = L.f32b.as<float>();
@@ -260,6 +356,19 @@ static int large_uhf(am1g_ctx* ctx, const am1g_request& rq, const MolDesc& md, c
             break;
         }
     }
+    // orbital energies and orbitals of each channel's last extrapolated Fock matrix, when its
+    // density came from SP2 (the eigensolve of the loop's last iteration otherwise)
+    for (int c = 0; c < 2 && status == AM1G_OK; ++c) {
+        if (!last_f[c]) continue;
+        hipError_t e = eig_large(*ctx->eig, n, const_cast<double*>(last_f[c]), eps[c], Ct[c], s, eig_err);
+        if (e != hipSuccess) {
+            if (eig_err.empty())
+                return res->status = am1g_set_error(ctx, AM1G_DEVICE_ERROR, "HIP error %s in the eigensolver",
+                                                    hipGetErrorString(e));
+            status = AM1G_LINEAR_ALGEBRA;
+        }
+    }
+    phase[3] += ms_since(t_ph);
     // ---- final energy, populations, dipole, both channels' orbitals (scf.rs:327-468)
     if ((st = fock_energy()) != AM1G_OK) return res->status = st;
     HIP_TRY(ctx, launch_atom_props(sa, Pt, L.props.as<double>(), s));
@@ -322,7 +431,7 @@ static int large_uhf(am1g_ctx* ctx, const am1g_request& rq, const MolDesc& md, c
     }
     phase[5] += ms_since(t_ph);
     for (int k = 0; k < 6; ++k) res->phase_ms[k] = phase[k];
-    res->eigen_sweeps = 0;
+    res->eigen_sweeps = sp2_steps;
     if (status == AM1G_LINEAR_ALGEBRA) am1g_set_error(ctx, status, "%s", eig_err.c_str());
     else if (status == AM1G_SCF_DIVERGED) am1g_set_error(ctx, status, "SCF diverged after %d iterations", iterations);
     else if (r.status == AM1G_SCF_NOT_CONVERGED)
@@ -517,6 +626,8 @@ int am1g_large_scf(am1g_ctx* ctx, const am1g_request& rq, const Prepared& prep, 
     const double one = 1.0, zero = 0.0, two = 2.0;
     const float zerof = 0.0f, twof = 2.0f;
     std::string eig_err;
+    double* last_f = nullptr;  // the Fock matrix whose SP2 density the loop ended with
+    int sp2_steps = 0;
     for (int iter = 0; iter < md.max_scf; ++iter) {
         iterations = iter + 1;
         HIP_TRY(ctx, launch_fock(sa, P, P, 0.5, F, s, partial, scal + 0));
@@ -594,33 +705,58 @@ int am1g_large_scf(am1g_ctx* ctx, const am1g_request& rq, const Prepared& prep, 
             if (bad == 0.0) Fuse = T1;
         }
         phase[2] += ms_since(t_ph);
-        // eigensolve (destroys Fuse; F is rebuilt by the next Fock pass)
-        hipError_t e = eig_large(*ctx->eig, n, Fuse, eps, Ct, s, eig_err);
-        if (e != hipSuccess) {
-            if (!eig_err.empty()) {
-                status = AM1G_LINEAR_ALGEBRA;
-                break;
-            }
-            return res->status = am1g_set_error(ctx, AM1G_DEVICE_ERROR, "HIP error %s in the eigensolver",
-                                                hipGetErrorString(e));
-        }
-        phase[3] += ms_since(t_ph);
-        // density 2 C_occ C_occ^T: rows 0..nocc-1 of Ct are the occupied orbitals (column-major n x nocc)
-        if (nocc > 0) {
+        // New density: by SP2 purification of Fuse, which stays intact for the orbital energies at
+        // the end, or from the eigenvectors when that declines or is switched off.
+        bool by_sp2 = false;
+        if (ctx->tun.sp2 && nocc > 0 && nocc < n) {
+            int steps = 0;
             if (low) {
-                ++fp32_iters;
-                float* Cf = L.f32a.as<float>();
-                float* Pf = L.f32b.as<float>();
-                HIP_TRY(ctx, launch_to_f32((int64_t)n * nocc, Ct, Cf, s));
-                HIP_TRY(ctx, gemm_f32(s, false, true, n, n, nocc, twof, Cf, n, Cf, n, zerof, Pf, n));
-                HIP_TRY(ctx, launch_density_update(n, Pf, true, P, partial, scal + 27, s));
+                float* X = nullptr;
+                HIP_TRY(ctx, sp2_density<float>(s, n, Fuse, nocc, 2.0, L.f32a.as<float>(), L.f32b.as<float>(), partial,
+                                                scal + 48, &X, by_sp2, steps));
+                phase[3] += ms_since(t_ph);
+                if (by_sp2) {
+                    ++fp32_iters;
+                    HIP_TRY(ctx, launch_density_update(n, X, true, P, partial, scal + 27, s));
+                }
             } else {
-                HIP_TRY(ctx, gemm_f64(s, false, true, n, n, nocc, two, Ct, n, Ct, n, zero, Pn, n));
+                double* X = nullptr;
+                HIP_TRY(ctx, sp2_density<double>(s, n, Fuse, nocc, 2.0, T0, Pn, partial, scal + 48, &X, by_sp2, steps));
+                phase[3] += ms_since(t_ph);
+                if (by_sp2) HIP_TRY(ctx, launch_density_update(n, X, false, P, partial, scal + 27, s));
+            }
+            sp2_steps += steps;
+        }
+        last_f = by_sp2 ? Fuse : nullptr;
+        if (!by_sp2) {
+            // eigensolve (destroys Fuse; F is rebuilt by the next Fock pass)
+            hipError_t e = eig_large(*ctx->eig, n, Fuse, eps, Ct, s, eig_err);
+            if (e != hipSuccess) {
+                if (!eig_err.empty()) {
+                    status = AM1G_LINEAR_ALGEBRA;
+                    break;
+                }
+                return res->status = am1g_set_error(ctx, AM1G_DEVICE_ERROR, "HIP error %s in the eigensolver",
+                                                    hipGetErrorString(e));
+            }
+            phase[3] += ms_since(t_ph);
+            // density 2 C_occ C_occ^T: rows 0..nocc-1 of Ct are the occupied orbitals (column-major n x nocc)
+            if (nocc > 0) {
+                if (low) {
+                    ++fp32_iters;
+                    float* Cf = L.f32a.as<float>();
+                    float* Pf = L.f32b.as<float>();
+                    HIP_TRY(ctx, launch_to_f32((int64_t)n * nocc, Ct, Cf, s));
+                    HIP_TRY(ctx, gemm_f32(s, false, true, n, n, nocc, twof, Cf, n, Cf, n, zerof, Pf, n));
+                    HIP_TRY(ctx, launch_density_update(n, Pf, true, P, partial, scal + 27, s));
+                } else {
+                    HIP_TRY(ctx, gemm_f64(s, false, true, n, n, nocc, two, Ct, n, Ct, n, zero, Pn, n));
+                    HIP_TRY(ctx, launch_density_update(n, Pn, false, P, partial, scal + 27, s));
+                }
+            } else {
+                HIP_TRY(ctx, hipMemsetAsync(Pn, 0, sizeof(double) * nn, s));
                 HIP_TRY(ctx, launch_density_update(n, Pn, false, P, partial, scal + 27, s));
             }
-        } else {
-            HIP_TRY(ctx, hipMemsetAsync(Pn, 0, sizeof(double) * nn, s));
-            HIP_TRY(ctx, launch_density_update(n, Pn, false, P, partial, scal + 27, s));
         }
         double dd = 0.0;
         HIP_TRY(ctx, hipMemcpyAsync(&dd, scal + 27, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -657,6 +793,18 @@ int am1g_large_scf(am1g_ctx* ctx, const am1g_request& rq, const Prepared& prep, 
             converged = 1;
             break;
         }
+    }
+    // orbital energies and orbitals of the last extrapolated Fock matrix, when its density came
+    // from SP2 (the eigensolve of the loop's last iteration otherwise)
+    if (last_f && status == AM1G_OK) {
+        hipError_t e = eig_large(*ctx->eig, n, last_f, eps, Ct, s, eig_err);
+        if (e != hipSuccess) {
+            if (eig_err.empty())
+                return res->status = am1g_set_error(ctx, AM1G_DEVICE_ERROR, "HIP error %s in the eigensolver",
+                                                    hipGetErrorString(e));
+            status = AM1G_LINEAR_ALGEBRA;
+        }
+        phase[3] += ms_since(t_ph);
     }
     // ---- final energy, populations, dipole (scf.rs:327-468)
     HIP_TRY(ctx, launch_fock(sa, P, P, 0.5, F, s, partial, scal + 0));
@@ -708,7 +856,7 @@ int am1g_large_scf(am1g_ctx* ctx, const am1g_request& rq, const Prepared& prep, 
     }
     phase[5] += ms_since(t_ph);
     for (int k = 0; k < 6; ++k) res->phase_ms[k] = phase[k];
-    res->eigen_sweeps = 0;
+    res->eigen_sweeps = sp2_steps;
     if (status == AM1G_LINEAR_ALGEBRA) am1g_set_error(ctx, status, "%s", eig_err.c_str());
     else if (status == AM1G_SCF_DIVERGED) am1g_set_error(ctx, status, "SCF diverged after %d iterations", iterations);
     else if (r.status == AM1G_SCF_NOT_CONVERGED)

@@ -149,6 +149,61 @@ __global__ __launch_bounds__(256) void k_to_f32(int64_t len, const double* x, fl
         y[k] = (float)x[k];
 }
 
+// ---- Density matrix by second-order spectral projection (SP2; A. M. N. Niklasson, Phys. Rev. B
+// 66, 155115 (2002)): the projector on the nocc lowest eigenvectors of F from a sequence of
+// squarings X <- X^2 or X <- 2X - X^2 of X_0 = (e_max I - F) / (e_max - e_min), no eigenvectors.
+
+// Gershgorin interval of each row of the symmetric F (one wavefront per row): out[3i] = F_ii -
+// sum_{j != i} |F_ij|, out[3i + 1] = F_ii + sum_{j != i} |F_ij|, out[3i + 2] = F_ii.
+__global__ __launch_bounds__(256) void k_gersh(int n, const double* F, double* out) {
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (i >= n) return;
+    const double* row = F + (size_t)i * n;
+    double r = 0.0;
+    for (int j = lane; j < n; j += 64) r += j == i ? 0.0 : fabs(row[j]);
+    for (int off = 32; off > 0; off >>= 1) r += __shfl_xor(r, off, 64);
+    if (lane == 0) {
+        const double d = row[i];
+        out[3 * i] = d - r;
+        out[3 * i + 1] = d + r;
+        out[3 * i + 2] = d;
+    }
+}
+
+// X = scale (e_max I - F)
+template <typename T>
+__global__ __launch_bounds__(256) void k_sp2_init(int n, const double* F, double emax, double scale, T* X) {
+    const int i = blockIdx.y;
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j < n) X[(size_t)i * n + j] = (T)(scale * ((i == j ? emax : 0.0) - F[(size_t)i * n + j]));
+}
+
+// out[0] = trace of X, summed in FP64 in a fixed order (one block)
+template <typename T>
+__global__ __launch_bounds__(256) void k_trace(int n, const T* X, double* out) {
+    __shared__ double sh[4];
+    double v = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) v += (double)X[(size_t)i * n + i];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) out[0] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+
+// X2 <- 2 X - X2
+template <typename T>
+__global__ __launch_bounds__(256) void k_sp2_combine(int64_t len, const T* X, T* X2) {
+    for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < len; k += (int64_t)gridDim.x * 256)
+        X2[k] = (T)2 * X[k] - X2[k];
+}
+
+// X <- w X
+template <typename T>
+__global__ __launch_bounds__(256) void k_scale(int64_t len, double w, T* X) {
+    for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < len; k += (int64_t)gridDim.x * 256)
+        X[k] = (T)(w * (double)X[k]);
+}
+
 // out[a] = (population, P[s][px], P[s][py], P[s][pz]) of atom a.
 __global__ __launch_bounds__(256) void k_atom_props(int nat, int n, const int* z, const int* aoff, const DevElem* el,
                                                     const double* P, double* out) {
@@ -223,6 +278,45 @@ hipError_t launch_to_f32(int64_t len, const double* x, float* y, hipStream_t st)
     hipLaunchKernelGGL(k_to_f32, dim3((unsigned)std::min<int64_t>(nblk(len, 256), 65536)), dim3(256), 0, st, len, x, y);
     return hipGetLastError();
 }
+
+hipError_t launch_gersh(int n, const double* F, double* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_gersh, dim3(nblk(n, 4)), dim3(256), 0, st, n, F, out);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_sp2_init(int n, const double* F, double emax, double scale, T* X, hipStream_t st) {
+    hipLaunchKernelGGL(k_sp2_init<T>, dim3(nblk(n, 256), n), dim3(256), 0, st, n, F, emax, scale, X);
+    return hipGetLastError();
+}
+template hipError_t launch_sp2_init<double>(int, const double*, double, double, double*, hipStream_t);
+template hipError_t launch_sp2_init<float>(int, const double*, double, double, float*, hipStream_t);
+
+template <typename T>
+hipError_t launch_trace(int n, const T* X, double* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_trace<T>, dim3(1), dim3(256), 0, st, n, X, out);
+    return hipGetLastError();
+}
+template hipError_t launch_trace<double>(int, const double*, double*, hipStream_t);
+template hipError_t launch_trace<float>(int, const float*, double*, hipStream_t);
+
+template <typename T>
+hipError_t launch_sp2_combine(int64_t len, const T* X, T* X2, hipStream_t st) {
+    hipLaunchKernelGGL(k_sp2_combine<T>, dim3((unsigned)std::min<int64_t>(nblk(len, 256), 65536)), dim3(256), 0, st,
+                       len, X, X2);
+    return hipGetLastError();
+}
+template hipError_t launch_sp2_combine<double>(int64_t, const double*, double*, hipStream_t);
+template hipError_t launch_sp2_combine<float>(int64_t, const float*, float*, hipStream_t);
+
+template <typename T>
+hipError_t launch_scale(int64_t len, double w, T* X, hipStream_t st) {
+    hipLaunchKernelGGL(k_scale<T>, dim3((unsigned)std::min<int64_t>(nblk(len, 256), 65536)), dim3(256), 0, st, len, w,
+                       X);
+    return hipGetLastError();
+}
+template hipError_t launch_scale<double>(int64_t, double, double*, hipStream_t);
+template hipError_t launch_scale<float>(int64_t, double, float*, hipStream_t);
 
 hipError_t launch_atom_props(const SysArgs& a, const double* P, double* out, hipStream_t st) {
     if (a.md.nat > 0)

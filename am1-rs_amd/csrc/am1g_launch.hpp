@@ -197,6 +197,17 @@ hipError_t launch_combine(int n, const double* hf, const int* ring, const double
                           double* partial, double* out, hipStream_t st);
 hipError_t launch_density_update(int n, const void* Pn, bool f32, double* P, double* partial, double* out,
                                  hipStream_t st);
+// SP2 density purification (large-system SCF): Gershgorin rows (3 doubles per row: lower bound,
+// upper bound, diagonal), X_0 = scale (e_max I - F), the trace of X, X2 <- 2X - X2, X <- w X.
+hipError_t launch_gersh(int n, const double* F, double* out, hipStream_t st);
+template <typename T>
+hipError_t launch_sp2_init(int n, const double* F, double emax, double scale, T* X, hipStream_t st);
+template <typename T>
+hipError_t launch_trace(int n, const T* X, double* out, hipStream_t st);
+template <typename T>
+hipError_t launch_sp2_combine(int64_t len, const T* X, T* X2, hipStream_t st);
+template <typename T>
+hipError_t launch_scale(int64_t len, double w, T* X, hipStream_t st);
 hipError_t launch_to_f32(int64_t len, const double* x, float* y, hipStream_t st);
 hipError_t launch_add(int64_t len, const double* x, const double* y, double* z, hipStream_t st);
 hipError_t launch_atom_props(const SysArgs& a, const double* P, double* out, hipStream_t st);

@@ -91,7 +91,8 @@ typedef struct {
     double phase_ms[6];       /* device time per phase: setup (integrals, H_core), Fock + energy,
                                  commutator + DIIS, eigensolve, density + convergence, final
                                  energy and properties (am1-rs AM1_TIMING phases, timing.rs:34-85) */
-    int eigen_sweeps;         /* Jacobi sweeps summed over all iterations (diagnostic) */
+    int eigen_sweeps;         /* Jacobi sweeps summed over all iterations; on the grid-level path
+                                 the SP2 purification steps (diagnostic) */
     int bcc_warnings;         /* AM1-BCC: perception or parameter warnings (am1g_bcc_charges) */
     double* charges;          /* [n_atoms] Mulliken charges */
     double* mo_energies;      /* [nao] (alpha for UHF) */
@@ -260,6 +261,10 @@ int am1g_matmul(am1g_ctx* ctx, int trans_a, int trans_b, int m, int n, int k, do
  *                      in panels of 32 columns (the trailing matrix read once per column, a
  *                      matrix-core rank-64 update per panel) instead of one column at a time;
  *                      default 2^30 (off)
+ *   sp2                1: the grid-level SCF forms each iteration's density by SP2 purification
+ *                      of the extrapolated Fock matrix (a chain of n x n x n products on the
+ *                      matrix cores) and diagonalizes once, after the loop, for the orbital
+ *                      energies and orbitals; 0: it diagonalizes in every iteration; default 1
  * No reference counterpart. AM1G_INVALID_INPUT for an unknown name. */
 int am1g_set_tunable(am1g_ctx* ctx, const char* name, double value);
 int am1g_get_tunable(const am1g_ctx* ctx, const char* name, double* value);

@@ -132,3 +132,57 @@ def test_water512_full_size_against_oracle_fixture(engine):
     assert np.abs(rm["charges"] - gf["charges"]).max() <= 1e-6
     assert abs(rm["homo_ev"] - float(gf["homo_ev"])) <= 1e-6
     assert abs(rm["lumo_ev"] - float(gf["lumo_ev"])) <= 1e-6
+
+
+def test_sp2_densities_match_diagonalization(engine):
+    """The grid-level SCF forms each iteration's density by SP2 purification of the extrapolated
+    Fock matrix (tunable sp2, on by default; eigen_sweeps counts the purification steps) and
+    diagonalizes once, after the loop. The same SCF diagonalizing in every iteration reaches the
+    same fixed point: 72 waters (432 basis functions), RHF and a UHF doublet (one more hydrogen
+    far away), energies, charges, orbital energies and densities."""
+    from am1g import geometry as G
+
+    mol = G.water_cluster((6, 6, 2), seed=1)
+    radical = (np.concatenate([mol.z, [1]]), np.concatenate([mol.xyz_bohr, [[40.0, 40.0, 40.0]]]))
+    for z, x, kw in ((mol.z, mol.xyz_bohr, {}), (radical[0], radical[1], {"multiplicity": 2})):
+        try:
+            engine.set_tunable("sp2", 0)
+            a = engine.scf(z, x, **kw)
+            engine.set_tunable("sp2", 1)
+            b = engine.scf(z, x, **kw)
+        finally:
+            engine.set_tunable("sp2", 1)
+        assert a["status"] == 0 and b["status"] == 0
+        assert a["eigen_sweeps"] == 0 and b["eigen_sweeps"] > 0
+        assert abs(a["iterations"] - b["iterations"]) <= 1
+        assert abs(a["total_ev"] - b["total_ev"]) <= 1e-8
+        assert np.abs(a["charges"] - b["charges"]).max() <= 1e-9
+        assert abs(a["homo_ev"] - b["homo_ev"]) <= 1e-8 and abs(a["lumo_ev"] - b["lumo_ev"]) <= 1e-8
+        assert np.abs(a["mo_energies"] - b["mo_energies"]).max() <= 1e-8
+        assert np.abs(a["density"] - b["density"]).max() <= 1e-9
+        if kw:
+            assert abs(a["homo_beta_ev"] - b["homo_beta_ev"]) <= 1e-8
+            assert np.abs(a["mo_energies_beta"] - b["mo_energies_beta"]).max() <= 1e-8
+
+
+def test_sp2_declines_a_degenerate_fermi_level(engine, forced):
+    """Two waters 50 Angstrom apart with two electrons removed (a closed shell): the Fermi level
+    falls inside the pair of degenerate HOMOs, so the purification cannot reach the electron
+    count, and the iteration diagonalizes instead; after one iteration the result equals the
+    diagonalizing run's bit for bit."""
+    z, x = M.parse(M.WATER)
+    z2 = np.concatenate([z, z])
+    x2 = np.concatenate([x, x + np.array([50.0 * M.ANGSTROM_TO_BOHR, 0.0, 0.0])])
+    out = []
+    try:
+        for sp2 in (0, 1):
+            engine.set_tunable("sp2", sp2)
+            out.append(engine.scf(z2, x2, charge=2.0, max_scf=1))
+    finally:
+        engine.set_tunable("sp2", 1)
+    a, b = out
+    assert a["iterations"] == b["iterations"] == 1
+    assert a["total_ev"] == b["total_ev"]
+    assert np.array_equal(a["charges"], b["charges"])
+    assert np.array_equal(a["mo_energies"], b["mo_energies"])
+    assert np.array_equal(a["density"], b["density"])
