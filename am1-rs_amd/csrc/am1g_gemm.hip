@@ -45,22 +45,34 @@ struct Mfma<float> {
 };
 
 // Tile grid over C plus, in blockIdx.y, the split of K. `part` (split-K only) receives the raw
-// partial product of each split, [split][n][m] column-major with leading dimension m.
+// partial product of each split, [split][n][m] column-major with leading dimension m. sym: the
+// product is known to be symmetric (m == n); only the tiles on and above the diagonal are formed,
+// and each off-diagonal one is also stored transposed below it.
 template <typename T, bool TA, bool TB>
 __global__ __launch_bounds__(kThreads, 2) void k_gemm(int m, int n, int k, int k_chunk, T alpha,
                                                       const T* __restrict__ A, int lda, const T* __restrict__ B,
                                                       int ldb, T beta, T* __restrict__ C, int ldc,
-                                                      T* __restrict__ part, int tiles_m, int tiles_n) {
+                                                      T* __restrict__ part, int tiles_m, int tiles_n, int sym) {
     typedef typename Mfma<T>::V V;
     __shared__ T As[2][kBK][kBM + kPad];  // op(A) panel, [k][i]
     __shared__ T Bs[2][kBK][kBN + kPad];  // op(B) panel, [k][j]
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     // Consecutive workgroup ids go to the eight XCDs in turn; when the tile count allows, each
     // XCD takes a contiguous run of tiles (one column block of C: the same B panel in its L2).
-    const int ntiles = tiles_m * tiles_n;
+    const int ntiles = sym ? tiles_n * (tiles_n + 1) / 2 : tiles_m * tiles_n;
     int b = blockIdx.x;
     if ((ntiles & 7) == 0) b = (b & 7) * (ntiles >> 3) + (b >> 3);
-    const int i0 = (b % tiles_m) * kBM, j0 = (b / tiles_m) * kBN;
+    int ti = b % tiles_m, tj = b / tiles_m;
+    if (sym) {
+        // upper-triangle tile b -> (ti, tj), ti <= tj, column by column: column tj starts at
+        // tj (tj + 1) / 2 (a single-precision estimate, corrected in integers)
+        tj = (int)((sqrtf(8.0f * (float)b + 1.0f) - 1.0f) * 0.5f);
+        while (tj * (tj + 1) / 2 > b) --tj;
+        while ((tj + 1) * (tj + 2) / 2 <= b) ++tj;
+        ti = b - tj * (tj + 1) / 2;
+    }
+    const int i0 = ti * kBM, j0 = tj * kBN;
+    const bool mirror = sym && ti != tj;
     const int kb = blockIdx.y * k_chunk, ke = min(k, kb + k_chunk);
 
     T ra[4], rb[4];
@@ -148,6 +160,10 @@ __global__ __launch_bounds__(kThreads, 2) void k_gemm(int m, int n, int k, int k
                 } else {
                     T* c = C + (size_t)gj * ldc + gi;
                     *c = beta != T(0) ? alpha * v + beta * *c : alpha * v;
+                    if (mirror) {
+                        T* c2 = C + (size_t)gi * ldc + gj;
+                        *c2 = beta != T(0) ? alpha * v + beta * *c2 : alpha * v;
+                    }
                 }
             }
 }
@@ -179,21 +195,23 @@ void plan_split(int m, int n, int k, int& split, int& chunk) {
 
 template <typename T>
 hipError_t gemm_t(hipStream_t st, bool ta, bool tb, int m, int n, int k, T alpha, const T* A, int lda, const T* B,
-                  int ldb, T beta, T* C, int ldc, T* work, size_t work_elems) {
+                  int ldb, T beta, T* C, int ldc, T* work, size_t work_elems, bool sym = false) {
     if (m <= 0 || n <= 0) return hipSuccess;
+    if (sym && m != n) return hipErrorInvalidValue;
     const int tiles_m = (m + kBM - 1) / kBM, tiles_n = (n + kBN - 1) / kBN;
     int split = 1, chunk = k;
-    if (k > 0) plan_split(m, n, k, split, chunk);
+    if (k > 0 && !sym) plan_split(m, n, k, split, chunk);
     if (split > 1 && (!work || work_elems < (size_t)split * m * n)) {
         split = 1;
         chunk = k;
     }
     if (k <= 0) chunk = 0;
     T* part = split > 1 ? work : nullptr;
-    const dim3 grid((unsigned)(tiles_m * tiles_n), (unsigned)split);
+    const int ntiles = sym ? tiles_n * (tiles_n + 1) / 2 : tiles_m * tiles_n;
+    const dim3 grid((unsigned)ntiles, (unsigned)split);
 #define AM1G_GEMM_LAUNCH(TA_, TB_)                                                                                \
     hipLaunchKernelGGL((k_gemm<T, TA_, TB_>), grid, dim3(kThreads), 0, st, m, n, k, chunk, alpha, A, lda, B, ldb, \
-                       beta, C, ldc, part, tiles_m, tiles_n)
+                       beta, C, ldc, part, tiles_m, tiles_n, sym ? 1 : 0)
     if (ta && tb) AM1G_GEMM_LAUNCH(true, true);
     else if (ta) AM1G_GEMM_LAUNCH(true, false);
     else if (tb) AM1G_GEMM_LAUNCH(false, true);
@@ -226,6 +244,16 @@ hipError_t gemm_f32(hipStream_t st, bool trans_a, bool trans_b, int m, int n, in
                     int lda, const float* B, int ldb, float beta, float* C, int ldc, float* work,
                     size_t work_elems) {
     return gemm_t<float>(st, trans_a, trans_b, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, work, work_elems);
+}
+
+hipError_t gemm_sym_f64(hipStream_t st, bool trans_a, bool trans_b, int n, int k, double alpha, const double* A,
+                        int lda, const double* B, int ldb, double beta, double* C, int ldc) {
+    return gemm_t<double>(st, trans_a, trans_b, n, n, k, alpha, A, lda, B, ldb, beta, C, ldc, nullptr, 0, true);
+}
+
+hipError_t gemm_sym_f32(hipStream_t st, bool trans_a, bool trans_b, int n, int k, float alpha, const float* A,
+                        int lda, const float* B, int ldb, float beta, float* C, int ldc) {
+    return gemm_t<float>(st, trans_a, trans_b, n, n, k, alpha, A, lda, B, ldb, beta, C, ldc, nullptr, 0, true);
 }
 
 }  // namespace am1g

@@ -24,4 +24,13 @@ hipError_t gemm_f32(hipStream_t st, bool trans_a, bool trans_b, int m, int n, in
                     int lda, const float* B, int ldb, float beta, float* C, int ldc, float* work = nullptr,
                     size_t work_elems = 0);
 
+// The same for a product known to be symmetric (n x n; e.g. X X with X symmetric, or A A^T):
+// only the 64 x 64 tiles on and above the diagonal are computed (about half the work), and each
+// off-diagonal tile is stored at both places. Off-diagonal pairs are then exactly symmetric;
+// inside a diagonal tile the two triangles are computed separately.
+hipError_t gemm_sym_f64(hipStream_t st, bool trans_a, bool trans_b, int n, int k, double alpha, const double* A,
+                        int lda, const double* B, int ldb, double beta, double* C, int ldc);
+hipError_t gemm_sym_f32(hipStream_t st, bool trans_a, bool trans_b, int n, int k, float alpha, const float* A,
+                        int lda, const float* B, int ldb, float beta, float* C, int ldc);
+
 }  // namespace am1g

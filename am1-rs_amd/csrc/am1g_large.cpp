@@ -85,10 +85,11 @@ hipError_t sp2_density(hipStream_t s, int n, const double* F, int nocc, double w
     double e1 = INFINITY, e2 = INFINITY;  // idempotency errors one and two steps back
     bool stopped = false;
     for (int k = 0; k < kMaxSteps && !stopped; ++k) {
+        // X^2 of the symmetric X: the upper tiles only, mirrored
         if constexpr (sizeof(T) == 4)
-            e = gemm_f32(s, false, false, n, n, n, 1.0f, X, n, X, n, 0.0f, X2, n);
+            e = gemm_sym_f32(s, false, false, n, n, 1.0f, X, n, X, n, 0.0f, X2, n);
         else
-            e = gemm_f64(s, false, false, n, n, n, 1.0, X, n, X, n, 0.0, X2, n);
+            e = gemm_sym_f64(s, false, false, n, n, 1.0, X, n, X, n, 0.0, X2, n);
         if (e != hipSuccess) return e;
         double trx2 = 0.0;
         if ((e = launch_trace<T>(n, X2, dscal, s)) != hipSuccess) return e;
