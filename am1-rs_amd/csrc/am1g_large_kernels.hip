@@ -96,17 +96,17 @@ __global__ __launch_bounds__(256) void k_dots_df(int n, const double* hf, const 
     block_partials<2 * kDiisDepth>(v, partial);
 }
 
+// out[q] = sum over the nparts block partials of column q, one block per column (launched with
+// `width` blocks), each summed in a fixed order.
 __global__ __launch_bounds__(256) void k_reduce_cols(int64_t nparts, int width, const double* partial, double* out) {
     __shared__ double sh[4];
-    for (int q = 0; q < width; ++q) {
-        double v = 0.0;
-        for (int64_t i = threadIdx.x; i < nparts; i += 256) v += partial[i * width + q];
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-        __syncthreads();
-        if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
-        __syncthreads();
-        if (threadIdx.x == 0) out[q] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
-    }
+    const int q = blockIdx.x;
+    double v = 0.0;
+    for (int64_t i = threadIdx.x; i < nparts; i += 256) v += partial[i * width + q];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) out[q] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
 }
 
 // Fout = sum_q c_q F_q (unpacked); block partials count non-finite entries.
@@ -238,13 +238,13 @@ hipError_t launch_diis_dots(int n, const double* hf, const double* he, const dou
     const int64_t len_s = (int64_t)n * (n + 1) / 2, len_a = (int64_t)n * (n - 1) / 2;
     const unsigned ge = 1024;
     hipLaunchKernelGGL(k_dots_e, dim3(ge), dim3(256), 0, st, len_a, he, len_a, ring, m, newest, partial);
-    hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, st, (int64_t)ge, kDiisDepth, partial, out);
+    hipLaunchKernelGGL(k_reduce_cols, dim3(kDiisDepth), dim3(256), 0, st, (int64_t)ge, kDiisDepth, partial, out);
     if (hd) {
         const dim3 grid(nblk(n, 256), n);
         double* p2 = partial + (int64_t)ge * kDiisDepth;
         hipLaunchKernelGGL(k_dots_df, grid, dim3(256), 0, st, n, hf, hd, len_s, ring, m, newest, p2);
-        hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, st, (int64_t)grid.x * grid.y, 2 * kDiisDepth, p2,
-                           out + kDiisDepth);
+        hipLaunchKernelGGL(k_reduce_cols, dim3(2 * kDiisDepth), dim3(256), 0, st, (int64_t)grid.x * grid.y,
+                           2 * kDiisDepth, p2, out + kDiisDepth);
     }
     return hipGetLastError();
 }
