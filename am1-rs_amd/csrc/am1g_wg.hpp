@@ -261,33 +261,38 @@ __device__ __noinline__ void wg_gemm_mfma(int M, int N, int K, double alpha, con
     V acc[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) acc[u] = V{0, 0, 0, 0};
+    // Panel staging: each thread takes a fixed position along the operand's contiguous index and
+    // steps over the other (one division per panel instead of one per element, so the loads of
+    // a thread issue back to back).
     for (int k0 = 0; k0 < K; k0 += kp) {
         const int kc = min(kp, K - k0), kc4 = (kc + 3) & ~3;
-#pragma unroll 8
-        for (int idx = t; idx < kc4 * Mp; idx += kWg) {
-            int kk, i;
-            if (TA) {
-                kk = idx / Mp;
-                i = idx - kk * Mp;
-            } else {
-                i = idx / kc4;
-                kk = idx - i * kc4;
+        {
+            // op(A) panel [kk][i]: contiguous along i (TA) or along kk
+            const int fast = TA ? Mp : kc4, slow = TA ? kc4 : Mp;
+            const int f = t % fast, s0 = t / fast, step = kWg / fast;
+            if (s0 < step) {
+#pragma unroll 4
+                for (int sl = s0; sl < slow; sl += step) {
+                    const int kk = TA ? sl : f, i = TA ? f : sl;
+                    As[kk * Mp + i] = (kk < kc && i < M)
+                                          ? (T)(TA ? A[(size_t)(k0 + kk) * lda + i] : A[(size_t)i * lda + k0 + kk])
+                                          : (T)0;
+                }
             }
-            As[kk * Mp + i] = (kk < kc && i < M) ? (T)(TA ? A[(size_t)(k0 + kk) * lda + i] : A[(size_t)i * lda + k0 + kk])
-                                               : (T)0;
         }
-#pragma unroll 8
-        for (int idx = t; idx < kc4 * Np; idx += kWg) {
-            int kk, j;
-            if (TB) {
-                j = idx / kc4;
-                kk = idx - j * kc4;
-            } else {
-                kk = idx / Np;
-                j = idx - kk * Np;
+        {
+            // op(B) panel [kk][j]: contiguous along kk (TB) or along j
+            const int fast = TB ? kc4 : Np, slow = TB ? Np : kc4;
+            const int f = t % fast, s0 = t / fast, step = kWg / fast;
+            if (s0 < step) {
+#pragma unroll 4
+                for (int sl = s0; sl < slow; sl += step) {
+                    const int kk = TB ? f : sl, j = TB ? sl : f;
+                    Bs[kk * Np + j] = (kk < kc && j < N)
+                                          ? (T)(TB ? B[(size_t)j * ldb + k0 + kk] : B[(size_t)(k0 + kk) * ldb + j])
+                                          : (T)0;
+                }
             }
-            Bs[kk * Np + j] = (kk < kc && j < N) ? (T)(TB ? B[(size_t)j * ldb + k0 + kk] : B[(size_t)(k0 + kk) * ldb + j])
-                                               : (T)0;
         }
         __syncthreads();
         for (int kk = 0; kk < kc4; kk += 4) {
