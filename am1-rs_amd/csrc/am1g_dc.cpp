@@ -265,7 +265,7 @@ int am1g_dc(am1g_ctx* ctx, const am1g_request* rq, const am1g_dc_options* op, am
     HIP_TRY(ctx, B.scal.reserve(sizeof(double) * 64));
     double* scal = B.scal.as<double>();
     auto t_start = std::chrono::steady_clock::now();
-    HIP_TRY(ctx, launch_pairs(ba, s, ctx->tun.pairs_block));
+    HIP_TRY(ctx, launch_pairs_one(ba, md, s));
     HIP_TRY(ctx, launch_hcore_diag(sa, s));
     HIP_TRY(ctx, launch_sum(npairs, ba.ecore_pair, B.partial.as<double>(), scal + 20, s));
 

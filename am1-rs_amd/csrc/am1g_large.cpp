@@ -535,7 +535,7 @@ static int large_core(am1g_ctx* ctx, const am1g_request& rq, const MolDesc& md, 
     const int64_t n_part = std::max<int64_t>({large_partials(n), 4096, 2 * fock_energy_parts(md)});
     HIP_TRY(ctx, L.partial.reserve(sizeof(double) * n_part));
     HIP_TRY(ctx, L.scal.reserve(sizeof(double) * 64));
-    HIP_TRY(ctx, launch_pairs(ba, s, ctx->tun.pairs_block));
+    HIP_TRY(ctx, launch_pairs_one(ba, md, s));
     HIP_TRY(ctx, launch_hcore_diag(sa, s));
     HIP_TRY(ctx, launch_sum(npairs, ba.ecore_pair, L.partial.as<double>(), L.scal.as<double>() + 30, s));
     return AM1G_OK;

@@ -108,6 +108,9 @@ hipError_t launch_zero_ints(int* p, int n, hipStream_t stream);
 int eigen_tri_max();
 // block: threads per workgroup of k_pairs (64, 128 or 256; the context tunable "pairs_block")
 hipError_t launch_pairs(const BatchArgs& a, hipStream_t stream, int block = 256);
+// The pairs of one molecule (a.n_mols == 1, md = its descriptor) on the whole device: heavy-heavy
+// and heavy-H blocks staged through LDS and written 64 pairs at a time, hydrogen pairs directly.
+hipError_t launch_pairs_one(const BatchArgs& a, const MolDesc& md, hipStream_t stream);
 void build_element_tables(DevElem* tab);
 
 }  // namespace am1g

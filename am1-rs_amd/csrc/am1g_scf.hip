@@ -13,6 +13,7 @@
 
 #include <mutex>
 
+#include <algorithm>
 #include <cfloat>
 #include <cstdlib>
 
@@ -86,9 +87,26 @@ __device__ __forceinline__ BlockRef block_of(const MolShared& ms, const double* 
 // block into the integral pool, its resonance block 1/2 (beta_mu + beta_nu) S_mu_nu into both
 // triangles of H_core (each off-diagonal atom block belongs to exactly one pair), and its
 // core-core repulsion (hamiltonian.rs:281-344, repulsion.rs:144-167).
+// A resonance entry (i, j) of H_core (i and j on different atoms): both places, or with UPPER the
+// one above the diagonal only.
+template <bool UPPER>
+__device__ __forceinline__ void res_store(double* H, int nao, int i, int j, double v) {
+    if (UPPER) {
+        H[(size_t)min(i, j) * nao + max(i, j)] = v;
+    } else {
+        H[(size_t)i * nao + j] = v;
+        H[(size_t)j * nao + i] = v;
+    }
+}
+
 // One atom pair k_all (molecule-local: heavy-heavy, heavy-H, H-H in class order) of molecule md:
-// writes its integral block and resonance block, returns its core-core term.
-__device__ __forceinline__ double pair_one(const BatchArgs& g, const MolDesc& md, int64_t k_all) {
+// writes its integral block (100, 10 or 1 entries) to w (the pool, or a staging row in LDS), its
+// resonance block into H_core (both triangles; with UPPER only the entries above the diagonal,
+// for a later mirror pass), and returns its core-core term. CLS >= 0: the pair is known to be of
+// that class (0 heavy-heavy, 1 heavy-H, 2 H-H), so only that class's code is compiled in (the
+// light classes then need far fewer registers).
+template <bool UPPER, int CLS = -1>
+__device__ __forceinline__ double pair_one_to(const BatchArgs& g, const MolDesc& md, int64_t k_all, double* w) {
     const DevElem* el = g.elems + md.method * kMaxZ;
     const double* xyz = g.xyz + md.atom_off * 3;
     const int* z = g.z + md.atom_off;
@@ -97,14 +115,14 @@ __device__ __forceinline__ double pair_one(const BatchArgs& g, const MolDesc& md
     const int nh = md.n_heavy, nH = md.n_h, nao = md.nao;
     const int64_t n_hh = (int64_t)nh * (nh - 1) / 2, n_hH = (int64_t)nh * nH;
     double* H = g.mats + md.mat_off + kH * (size_t)nao * nao;
-    double* wpool = g.wpool + md.w_off;
+    const int cls = CLS >= 0 ? CLS : (k_all < n_hh ? 0 : (k_all < n_hh + n_hH ? 1 : 2));
     int a, b;
-    if (k_all < n_hh) {
+    if (cls == 0) {
         int i, j;
         tri_decode(k_all, nh, i, j);
         a = alist[i];
         b = alist[j];
-    } else if (k_all < n_hh + n_hH) {
+    } else if (cls == 1) {
         int64_t k = k_all - n_hh;
         a = alist[(int)(k / nH)];
         b = alist[nh + (int)(k % nH)];
@@ -123,61 +141,102 @@ __device__ __forceinline__ double pair_one(const BatchArgs& g, const MolDesc& md
         // beyond the far-field cutoff the pair is out of the pair list (neighbors.rs:90-101):
         // no integrals, no resonance, no core-core term; its net charges interact through
         // the far-field potential instead
-        if (k_all < n_hh) {
-            double* dst = wpool + k_all * 100;
-            for (int q = 0; q < 100; ++q) dst[q] = 0.0;
+        if (cls == 0) {
+            for (int q = 0; q < 100; ++q) w[q] = 0.0;
             for (int i2 = 0; i2 < 4; ++i2)
-                for (int j2 = 0; j2 < 4; ++j2) {
-                    H[(size_t)(oa + i2) * nao + ob + j2] = 0.0;
-                    H[(size_t)(ob + j2) * nao + oa + i2] = 0.0;
-                }
-        } else if (k_all < n_hh + n_hH) {
-            double* dst = wpool + n_hh * 100 + (k_all - n_hh) * 10;
-            for (int q = 0; q < 10; ++q) dst[q] = 0.0;
-            for (int i2 = 0; i2 < 4; ++i2) {
-                H[(size_t)(oa + i2) * nao + ob] = 0.0;
-                H[(size_t)ob * nao + oa + i2] = 0.0;
-            }
+                for (int j2 = 0; j2 < 4; ++j2) res_store<UPPER>(H, nao, oa + i2, ob + j2, 0.0);
+        } else if (cls == 1) {
+            for (int q = 0; q < 10; ++q) w[q] = 0.0;
+            for (int i2 = 0; i2 < 4; ++i2) res_store<UPPER>(H, nao, oa + i2, ob, 0.0);
         } else {
-            wpool[n_hh * 100 + n_hH * 10 + (k_all - n_hh - n_hH)] = 0.0;
-            H[(size_t)oa * nao + ob] = 0.0;
-            H[(size_t)ob * nao + oa] = 0.0;
+            w[0] = 0.0;
+            res_store<UPPER>(H, nao, oa, ob, 0.0);
         }
         return 0.0;
     }
     double S[16];
     overlap_block(ea, eb, d, S);
-    if (k_all < n_hh) {
-        heavy_heavy_block(ea, eb, d, wpool + k_all * 100);
+    if (cls == 0) {
+        heavy_heavy_block(ea, eb, d, w);
 #pragma unroll
         for (int i2 = 0; i2 < 4; ++i2)
 #pragma unroll
             for (int j2 = 0; j2 < 4; ++j2) {
                 double v = 0.5 * ((i2 == 0 ? ea.bs : ea.bp) + (j2 == 0 ? eb.bs : eb.bp)) * S[i2 * 4 + j2];
-                H[(size_t)(oa + i2) * nao + ob + j2] = v;
-                H[(size_t)(ob + j2) * nao + oa + i2] = v;
+                res_store<UPPER>(H, nao, oa + i2, ob + j2, v);
             }
-    } else if (k_all < n_hh + n_hH) {
-        double w[10];
-        heavy_h_block(ea, eb, d, w);
-        double* dst = wpool + n_hh * 100 + (k_all - n_hh) * 10;
+    } else if (cls == 1) {
+        double wh[10];
+        heavy_h_block(ea, eb, d, wh);
 #pragma unroll
-        for (int q = 0; q < 10; ++q) dst[q] = w[q];
+        for (int q = 0; q < 10; ++q) w[q] = wh[q];
 #pragma unroll
         for (int i2 = 0; i2 < 4; ++i2) {
             double v = 0.5 * ((i2 == 0 ? ea.bs : ea.bp) + eb.bs) * S[i2 * 4];
-            H[(size_t)(oa + i2) * nao + ob] = v;
-            H[(size_t)ob * nao + oa + i2] = v;
+            res_store<UPPER>(H, nao, oa + i2, ob, v);
         }
     } else {
-        wpool[n_hh * 100 + n_hH * 10 + (k_all - n_hh - n_hH)] = hh_integral(ea, eb, r);
+        w[0] = hh_integral(ea, eb, r);
         double v = 0.5 * (ea.bs + eb.bs) * S[0];
-        H[(size_t)oa * nao + ob] = v;
-        H[(size_t)ob * nao + oa] = v;
+        res_store<UPPER>(H, nao, oa, ob, v);
     }
     // the core-core term is symmetric in the pair; evaluate it in index order (i < j)
     const int lo_a = min(a, b), hi_a = max(a, b);
     return core_core_pair(el[z[lo_a]], el[z[hi_a]], z[lo_a], z[hi_a], r);
+}
+
+// Where pair k_all's integral block starts in the molecule's pool (its class offset plus its index).
+__device__ __forceinline__ double* pair_block(const BatchArgs& g, const MolDesc& md, int64_t k_all) {
+    const int nh = md.n_heavy, nH = md.n_h;
+    const int64_t n_hh = (int64_t)nh * (nh - 1) / 2, n_hH = (int64_t)nh * nH;
+    double* wpool = g.wpool + md.w_off;
+    return k_all < n_hh ? wpool + k_all * 100
+                        : (k_all < n_hh + n_hH ? wpool + n_hh * 100 + (k_all - n_hh) * 10
+                                               : wpool + n_hh * 100 + n_hH * 10 + (k_all - n_hh - n_hH));
+}
+
+// The pair's integral block straight into the pool, its resonance block at both places.
+__device__ __forceinline__ double pair_one(const BatchArgs& g, const MolDesc& md, int64_t k_all) {
+    return pair_one_to<false>(g, md, k_all, pair_block(g, md, k_all));
+}
+
+__global__ void k_sym_lower(int n, double* F);
+
+// One molecule's pairs of one class (0 heavy-heavy, 1 heavy-H) on the whole device, 64
+// consecutive pairs per one-wavefront workgroup: each lane forms its pair's block in LDS, then
+// the wavefront writes the 64 blocks, which are contiguous in the pool, with consecutive lanes on
+// consecutive doubles (one lane per pair wrote 800- or 80-byte blocks 64 apart, a cache line
+// per lane and store). The resonance entries go above the diagonal, the core-core terms as before.
+template <int CLS>
+__global__ __launch_bounds__(64) void k_pairs_cls(BatchArgs g) {
+    constexpr int W = CLS == 0 ? 100 : 10, LD = W + 1;  // odd row stride: lanes on distinct banks
+    __shared__ double st[64 * LD];
+    const MolDesc& md = g.mols[0];
+    const int nh = md.n_heavy, nH = md.n_h;
+    const int64_t n_hh = (int64_t)nh * (nh - 1) / 2, n_hH = (int64_t)nh * nH;
+    const int64_t n_cls = CLS == 0 ? n_hh : n_hH, first = CLS == 0 ? 0 : n_hh;
+    double* pool = g.wpool + md.w_off + (CLS == 0 ? 0 : n_hh * 100);
+    const int lane = threadIdx.x;
+    for (int64_t k0 = (int64_t)blockIdx.x * 64; k0 < n_cls; k0 += (int64_t)gridDim.x * 64) {
+        const int64_t k = k0 + lane;
+        if (k < n_cls) {
+            g.ecore_pair[first + k] = pair_one_to<true, CLS>(g, md, first + k, st + lane * LD);
+        }
+        __syncthreads();
+        const int cnt = (int)min((int64_t)64, n_cls - k0);
+        double* dst = pool + k0 * W;
+        for (int e = lane; e < cnt * W; e += 64) dst[e] = st[(e / W) * LD + e % W];
+        __syncthreads();
+    }
+}
+
+// One molecule's pairs [from, to) of class CLS (one thread each), resonance above the diagonal.
+template <int CLS>
+__global__ __launch_bounds__(256) void k_pairs_range(BatchArgs g, int64_t from, int64_t to) {
+    const MolDesc& md = g.mols[0];
+    for (int64_t p = from + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < to;
+         p += (int64_t)gridDim.x * blockDim.x)
+        g.ecore_pair[p] = pair_one_to<true, CLS>(g, md, p, pair_block(g, md, p));
 }
 
 __global__ __launch_bounds__(256) void k_pairs(BatchArgs g) {
@@ -1606,6 +1665,26 @@ template __global__ void k_scf_batch<1>(BatchArgs);
 }  // namespace am1g
 
 namespace am1g {
+hipError_t launch_pairs_one(const BatchArgs& a, const MolDesc& md, hipStream_t stream) {
+    const int64_t n_hh = (int64_t)md.n_heavy * (md.n_heavy - 1) / 2, n_hH = (int64_t)md.n_heavy * md.n_h;
+    auto blocks = [](int64_t n, int per) { return dim3((unsigned)std::min<int64_t>((n + per - 1) / per, 65536)); };
+    // one kernel per class, each compiled for its class alone; the heavy-H blocks (80 bytes) are
+    // staged through LDS so that their stores coalesce (measured: staging the 800-byte
+    // heavy-heavy blocks as well cost more in occupancy than it saved)
+    if (n_hh > 0) hipLaunchKernelGGL(k_pairs_range<0>, blocks(n_hh, 256), dim3(256), 0, stream, a, (int64_t)0, n_hh);
+    if (n_hH > 0) hipLaunchKernelGGL(k_pairs_cls<1>, blocks(n_hH, 64), dim3(64), 0, stream, a);
+    if (a.n_pairs > n_hh + n_hH)
+        hipLaunchKernelGGL(k_pairs_range<2>, blocks(a.n_pairs - n_hh - n_hH, 256), dim3(256), 0, stream, a,
+                           n_hh + n_hH, a.n_pairs);
+    // the resonance blocks were written above the diagonal only: mirror them (a tiled transpose,
+    // coalesced on both sides, instead of a scattered store per entry)
+    const int nt = (md.nao + 31) / 32;
+    if (md.nao > 1)
+        hipLaunchKernelGGL(k_sym_lower, dim3((unsigned)(nt * (nt + 1) / 2)), dim3(256), 0, stream, md.nao,
+                           a.mats + md.mat_off + kH * (size_t)md.nao * md.nao);
+    return hipGetLastError();
+}
+
 hipError_t launch_pairs(const BatchArgs& a, hipStream_t stream, int block) {
     if (a.n_pairs == 0) return hipSuccess;
     block = block == 64 || block == 128 ? block : 256;  // the kernel's grid-stride loop takes any
