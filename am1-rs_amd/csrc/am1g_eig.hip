@@ -754,8 +754,36 @@ void split(int s, int m, std::vector<Node>& merges, std::vector<int>& ls, std::v
 
 }  // namespace
 
+namespace {
+// Number of eigenvalues of the symmetric tridiagonal T = (d, e) below x: the sign changes of the
+// LDL^T pivots q_i = (d_i - x) - e_{i-1}^2 / q_{i-1}, a pivot below pivmin in size taken as
+// -pivmin (as in LAPACK's dstebz).
+int sturm_count(const std::vector<double>& d, const std::vector<double>& e, int n, double x, double pivmin) {
+    int c = 0;
+    double q = d[0] - x;
+    if (std::fabs(q) < pivmin) q = -pivmin;
+    if (q < 0.0) ++c;
+    for (int i = 1; i < n; ++i) {
+        q = (d[i] - x) - e[i - 1] * e[i - 1] / q;
+        if (std::fabs(q) < pivmin) q = -pivmin;
+        if (q < 0.0) ++c;
+    }
+    return c;
+}
+}  // namespace
+
 hipError_t eig_large(EigWork& w, int n, double* A, double* evals, double* Ct, hipStream_t st,
                      std::string& err) {
+    return eig_large_impl(w, n, A, evals, Ct, st, err, -1, -1, nullptr);
+}
+
+hipError_t eig_large_values(EigWork& w, int n, double* A, int k_lo, int k_hi, double* vals, hipStream_t st,
+                            std::string& err) {
+    return eig_large_impl(w, n, A, nullptr, nullptr, st, err, k_lo, k_hi, vals);
+}
+
+hipError_t eig_large_impl(EigWork& w, int n, double* A, double* evals, double* Ct, hipStream_t st,
+                          std::string& err, int k_lo, int k_hi, double* vals) {
     err.clear();
     if (n < 1) return hipSuccess;
     const size_t nn = (size_t)n * n;
@@ -876,6 +904,28 @@ hipError_t eig_large(EigWork& w, int n, double* A, double* evals, double* Ct, hi
             err = "symmetric eigensolver: non-finite matrix";
             return hipErrorUnknown;
         }
+    if (vals) {
+        // only the eigenvalues k_lo..k_hi: bisection on Sturm counts of T to rounding level
+        double lo = INFINITY, hi = -INFINITY, tnorm = 0.0;
+        for (int i = 0; i < n; ++i) {
+            const double r = (i > 0 ? std::fabs(w.h_e[i - 1]) : 0.0) + (i + 1 < n ? std::fabs(w.h_e[i]) : 0.0);
+            lo = std::min(lo, w.h_d[i] - r);
+            hi = std::max(hi, w.h_d[i] + r);
+            tnorm = std::max(tnorm, std::fabs(w.h_d[i]) + r);
+        }
+        const double pivmin = 1.0e-280 * std::max(1.0, tnorm * tnorm);
+        for (int k = k_lo; k <= k_hi; ++k) {
+            double a = lo - 1.0e-12 * (tnorm + 1.0), b = hi + 1.0e-12 * (tnorm + 1.0);
+            for (int it = 0; it < 200; ++it) {
+                const double m = 0.5 * (a + b);
+                if (!(m > a && m < b)) break;  // the interval is down to adjacent doubles
+                if (sturm_count(w.h_d, w.h_e, n, m, pivmin) > k) b = m;
+                else a = m;
+            }
+            vals[k - k_lo] = 0.5 * (a + b);
+        }
+        return hipSuccess;
+    }
 
     // ---- 2. tridiagonal divide and conquer
     std::vector<Node> merges;

@@ -65,4 +65,13 @@ namespace am1g {
 hipError_t eig_large(EigWork& w, int n, double* A, double* evals, double* Ct, hipStream_t st,
                      std::string& err);
 
+// Only the eigenvalues of ascending indices k_lo..k_hi of the symmetric n x n A (row-major;
+// overwritten), into vals[0..k_hi - k_lo] on the host: the same tridiagonalization, then bisection
+// on Sturm counts of the tridiagonal to rounding level instead of the divide and conquer and the
+// back-transformation.
+hipError_t eig_large_values(EigWork& w, int n, double* A, int k_lo, int k_hi, double* vals, hipStream_t st,
+                            std::string& err);
+hipError_t eig_large_impl(EigWork& w, int n, double* A, double* evals, double* Ct, hipStream_t st,
+                          std::string& err, int k_lo, int k_hi, double* vals);
+
 }  // namespace am1g

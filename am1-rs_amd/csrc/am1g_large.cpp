@@ -359,9 +359,19 @@ static int large_uhf(am1g_ctx* ctx, const am1g_request& rq, const MolDesc& md, c
     }
     // orbital energies and orbitals of each channel's last extrapolated Fock matrix, when its
     // density came from SP2 (the eigensolve of the loop's last iteration otherwise)
+    const bool frontier_only = !res->mo_energies && !res->mo_energies_beta && !res->mo_coeff && !res->mo_coeff_beta;
+    double frontier[2][2] = {{NAN, NAN}, {NAN, NAN}};  // per channel: eigenvalues nocc - 1 and nocc
     for (int c = 0; c < 2 && status == AM1G_OK; ++c) {
         if (!last_f[c]) continue;
-        hipError_t e = eig_large(*ctx->eig, n, const_cast<double*>(last_f[c]), eps[c], Ct[c], s, eig_err);
+        hipError_t e;
+        if (frontier_only) {
+            const int k_lo = std::max(nocc[c] - 1, 0), k_hi = std::min(nocc[c], n - 1);
+            double v[2] = {NAN, NAN};
+            e = eig_large_values(*ctx->eig, n, const_cast<double*>(last_f[c]), k_lo, k_hi, v, s, eig_err);
+            for (int k = k_lo; k <= k_hi; ++k) frontier[c][k - (nocc[c] - 1)] = v[k - k_lo];
+        } else {
+            e = eig_large(*ctx->eig, n, const_cast<double*>(last_f[c]), eps[c], Ct[c], s, eig_err);
+        }
         if (e != hipSuccess) {
             if (eig_err.empty())
                 return res->status = am1g_set_error(ctx, AM1G_DEVICE_ERROR, "HIP error %s in the eigensolver",
@@ -403,10 +413,11 @@ static int large_uhf(am1g_ctx* ctx, const am1g_request& rq, const MolDesc& md, c
     r.core_ev = ecore;
     r.total_ev = electronic + ecore + md.field_nuc;
     r.heat_of_formation_kcal = (electronic + ecore + md.field_nuc - eiso + eheat) * kEvToKcal;
-    r.homo_ev = na >= 1 ? heps[na - 1] : NAN;
-    r.lumo_ev = na < n ? heps[na] : NAN;
-    r.homo_beta_ev = nb >= 1 ? hepsb[nb - 1] : NAN;
-    r.lumo_beta_ev = nb < n ? hepsb[nb] : NAN;
+    const bool fa = frontier_only && last_f[0], fb = frontier_only && last_f[1];
+    r.homo_ev = na >= 1 ? (fa ? frontier[0][0] : heps[na - 1]) : NAN;
+    r.lumo_ev = na < n ? (fa ? frontier[0][1] : heps[na]) : NAN;
+    r.homo_beta_ev = nb >= 1 ? (fb ? frontier[1][0] : hepsb[nb - 1]) : NAN;
+    r.lumo_beta_ev = nb < n ? (fb ? frontier[1][1] : hepsb[nb]) : NAN;
     for (int k = 0; k < 3; ++k) r.dipole[k] = dip[k] * kAuDipoleToDebye;
     r.residual = residual;
     r.fp32_iterations = fp32_iters;
@@ -796,9 +807,20 @@ int am1g_large_scf(am1g_ctx* ctx, const am1g_request& rq, const Prepared& prep, 
         }
     }
     // orbital energies and orbitals of the last extrapolated Fock matrix, when its density came
-    // from SP2 (the eigensolve of the loop's last iteration otherwise)
+    // from SP2 (the eigensolve of the loop's last iteration otherwise); when the caller takes
+    // neither orbital energies nor orbitals, only HOMO and LUMO, by bisection on its tridiagonal form
+    const bool frontier_only = !res->mo_energies && !res->mo_coeff;
+    double frontier[2] = {NAN, NAN};  // eigenvalues nocc - 1 and nocc (frontier_only)
     if (last_f && status == AM1G_OK) {
-        hipError_t e = eig_large(*ctx->eig, n, last_f, eps, Ct, s, eig_err);
+        hipError_t e;
+        if (frontier_only) {
+            const int k_lo = std::max(nocc - 1, 0), k_hi = std::min(nocc, n - 1);
+            double v[2] = {NAN, NAN};
+            e = eig_large_values(*ctx->eig, n, last_f, k_lo, k_hi, v, s, eig_err);
+            for (int k = k_lo; k <= k_hi; ++k) frontier[k - (nocc - 1)] = v[k - k_lo];
+        } else {
+            e = eig_large(*ctx->eig, n, last_f, eps, Ct, s, eig_err);
+        }
         if (e != hipSuccess) {
             if (eig_err.empty())
                 return res->status = am1g_set_error(ctx, AM1G_DEVICE_ERROR, "HIP error %s in the eigensolver",
@@ -807,6 +829,7 @@ int am1g_large_scf(am1g_ctx* ctx, const am1g_request& rq, const Prepared& prep, 
         }
         phase[3] += ms_since(t_ph);
     }
+    const bool use_frontier = last_f && frontier_only;
     // ---- final energy, populations, dipole (scf.rs:327-468)
     HIP_TRY(ctx, launch_fock(sa, P, P, 0.5, F, s, partial, scal + 0));
     HIP_TRY(ctx, launch_atom_props(sa, P, L.props.as<double>(), s));
@@ -839,8 +862,8 @@ int am1g_large_scf(am1g_ctx* ctx, const am1g_request& rq, const Prepared& prep, 
     r.core_ev = ecore;
     r.total_ev = electronic + ecore + md.field_nuc;
     r.heat_of_formation_kcal = (electronic + ecore + md.field_nuc - eiso + eheat) * kEvToKcal;
-    r.homo_ev = nocc >= 1 ? heps[nocc - 1] : NAN;
-    r.lumo_ev = nocc < n ? heps[nocc] : NAN;
+    r.homo_ev = nocc >= 1 ? (use_frontier ? frontier[0] : heps[nocc - 1]) : NAN;
+    r.lumo_ev = nocc < n ? (use_frontier ? frontier[1] : heps[nocc]) : NAN;
     r.homo_beta_ev = NAN;
     r.lumo_beta_ev = NAN;
     for (int k = 0; k < 3; ++k) r.dipole[k] = dip[k] * kAuDipoleToDebye;

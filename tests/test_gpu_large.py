@@ -186,3 +186,21 @@ def test_sp2_declines_a_degenerate_fermi_level(engine, forced):
     assert np.array_equal(a["charges"], b["charges"])
     assert np.array_equal(a["mo_energies"], b["mo_energies"])
     assert np.array_equal(a["density"], b["density"])
+
+
+def test_frontier_orbital_energies_by_bisection(engine):
+    """When the caller takes neither orbital energies nor orbitals, the grid-level SCF finds HOMO
+    and LUMO by bisection on the tridiagonal form of its last Fock matrix instead of the full
+    eigensolve; they equal the full eigensolve's (RHF and a UHF doublet, 72 waters)."""
+    from am1g import geometry as G
+
+    mol = G.water_cluster((6, 6, 2), seed=1)
+    radical = (np.concatenate([mol.z, [1]]), np.concatenate([mol.xyz_bohr, [[40.0, 40.0, 40.0]]]))
+    for z, x, kw in ((mol.z, mol.xyz_bohr, {}), (radical[0], radical[1], {"multiplicity": 2})):
+        a = engine.scf(z, x, full=True, **kw)
+        b = engine.scf(z, x, full=False, **kw)
+        assert a["status"] == 0 and b["status"] == 0
+        assert a["total_ev"] == b["total_ev"]
+        keys = ("homo_ev", "lumo_ev") + (("homo_beta_ev", "lumo_beta_ev") if kw else ())
+        for k in keys:
+            assert abs(a[k] - b[k]) <= 1e-10, (k, a[k], b[k])
