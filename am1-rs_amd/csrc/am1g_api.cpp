@@ -54,6 +54,7 @@ bool set_tunable(Tunables& t, const std::string& name, double v) {
     else if (name == "tri_blocked_min") t.tri_blocked_min = std::max(3, (int)v);
     else if (name == "batch_own_pairs") t.batch_own_pairs = v != 0.0;
     else if (name == "sp2") t.sp2 = v != 0.0;
+    else if (name == "batch_small_wg") t.batch_small_wg = v != 0.0;
     else return false;
     return true;
 }
@@ -68,6 +69,7 @@ bool get_tunable(const Tunables& t, const std::string& name, double& v) {
     else if (name == "tri_blocked_min") v = t.tri_blocked_min;
     else if (name == "batch_own_pairs") v = t.batch_own_pairs;
     else if (name == "sp2") v = t.sp2;
+    else if (name == "batch_small_wg") v = t.batch_small_wg;
     else return false;
     return true;
 }
@@ -570,7 +572,30 @@ static int batch_impl(am1g_ctx* ctx, int n, const am1g_request* rqs, am1g_result
     HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
     a.own_pairs = ctx->tun.batch_own_pairs;
     if (!a.own_pairs) HIP_TRY(ctx, launch_pairs(a, st, ctx->tun.pairs_block));
-    HIP_TRY(ctx, launch_scf_batch(a, precision, grid, lds, st));
+    // Molecules of at most kSmallWgAo basis functions (the tail of the largest-first order) run
+    // two per CU in the 256-thread build when there are enough of them to fill the device twice;
+    // the rest (all of a smaller batch) in the 512-thread one. The split depends on the batch, and
+    // the two builds sum in different orders (results within rounding of each other), so it is
+    // made only for batches that large: a molecule alone, in a small batch or in the server gets
+    // the 512-thread build's bits.
+    int n_small = 0;
+    if (ctx->tun.batch_small_wg)
+        for (int j = 0; j < m; ++j) n_small += descs[j].nao <= kSmallWgAo ? 1 : 0;
+    if (n_small >= 2 * ctx->cus) {
+        const int n_big = m - n_small;
+        if (n_big > 0) {
+            BatchArgs big = a;
+            big.n_mols = n_big;
+            HIP_TRY(ctx, launch_scf_batch(big, precision, std::min(n_big, ctx->cus), lds, st));
+        }
+        BatchArgs small = a;
+        small.order = a.order + n_big;
+        small.n_mols = n_small;
+        small.queue = a.queue + 1;
+        HIP_TRY(ctx, launch_scf_batch_small(small, precision, std::min(n_small, 2 * ctx->cus), st));
+    } else {
+        HIP_TRY(ctx, launch_scf_batch(a, precision, grid, lds, st));
+    }
     HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
     if (streaming) {
         // hand out each molecule as its done flag appears, while the launch runs

@@ -75,6 +75,7 @@ struct Tunables {
     int tri_blocked_min = 1 << 30;       // large eigensolver: tridiagonalization in panels from this order
     int batch_own_pairs = 1;             // batched launch: workgroups compute their own pair integrals
     int sp2 = 1;                         // large-system SCF: densities by SP2 purification (0: eigenvectors)
+    int batch_small_wg = 1;              // large batches: molecules <= 64 AOs two per CU (256-thread build)
 };
 
 struct am1g_ctx {

@@ -93,6 +93,10 @@ struct BatchArgs {
 };
 
 hipError_t launch_scf_batch(const BatchArgs& a, int precision, int grid, size_t lds_bytes, hipStream_t stream);
+// The same kernel built for 256-thread workgroups (two per CU) and molecules of at most
+// kSmallWgAo basis functions (am1g_scf_small.hip); grid: workgroups (two per CU fill the device).
+constexpr int kSmallWgAo = 64;
+hipError_t launch_scf_batch_small(const BatchArgs& a, int precision, int grid, hipStream_t stream);
 // Size of the per-device scratch array of the allocation-free batched entry point.
 constexpr size_t kScratchBytes = (size_t)64 << 20;
 constexpr int kMaxDevices = 64;

@@ -566,7 +566,12 @@ __device__ __noinline__ void mol_fock(const MolDesc& md, const DevElem* el, cons
 // LDS of one workgroup: 150 KiB of dynamic space (the static part holds the molecule tables).
 // The eigensolver keeps A (packed lower triangle) there and, while it fits, the eigenvector
 // matrix too; the GEMMs use the whole region as panel scratch when the eigensolver is idle.
-constexpr int kLdsDoubles = 19200;
+// Dynamic LDS of the batched kernel in doubles: 150 KiB beside the static part (one workgroup per
+// CU); the small-molecule build takes 68 KiB (two per CU).
+#ifndef AM1G_LDS_DOUBLES
+#define AM1G_LDS_DOUBLES 19200
+#endif
+constexpr int kLdsDoubles = AM1G_LDS_DOUBLES;
 constexpr int kALds = 192;  // largest order diagonalized with A in LDS (n(n+1)/2 <= kLdsDoubles)
 
 // P = w * C_occ C_occ^T from transposed eigenvectors Ct (row k = MO k): P = w Ct_occ^T Ct_occ.
@@ -1545,7 +1550,9 @@ __device__ long long svc_take(const BatchArgs& g) {
 }
 
 template <int PREC>
-__global__ __launch_bounds__(kWg) void k_scf_batch(BatchArgs g) {
+// Two wavefronts per SIMD: one 512-thread workgroup per CU, or two 256-thread ones (the
+// small-molecule build), each wavefront within 256 registers.
+__global__ __launch_bounds__(kWg, 2) void k_scf_batch(BatchArgs g) {
     double* lds = lds_dyn;
     __shared__ MolShared ms;
     __shared__ WgScratch s;
@@ -1665,6 +1672,7 @@ template __global__ void k_scf_batch<1>(BatchArgs);
 }  // namespace am1g
 
 namespace am1g {
+#ifndef AM1G_WG_VARIANT  // the small-molecule build needs only the batched kernel and its launch
 hipError_t launch_pairs_one(const BatchArgs& a, const MolDesc& md, hipStream_t stream) {
     const int64_t n_hh = (int64_t)md.n_heavy * (md.n_heavy - 1) / 2, n_hH = (int64_t)md.n_heavy * md.n_h;
     auto blocks = [](int64_t n, int per) { return dim3((unsigned)std::min<int64_t>((n + per - 1) / per, 65536)); };
@@ -1743,15 +1751,18 @@ int eigen_tri_max() {
     return (v && v[0] == '1') ? 0 : kTriBig;
 }
 
+#endif  // AM1G_WG_VARIANT
+
 hipError_t launch_scf_batch(const BatchArgs& a, int precision, int grid, size_t lds_bytes, hipStream_t stream) {
     static bool attr_done = false;
     if (!attr_done) {
         // allow the dynamic LDS region to take what the static part leaves of the 160 KiB
+        const int dyn = kLdsDoubles * (int)sizeof(double);
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_scf_batch<0>),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes);
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
         if (e != hipSuccess) return e;
         e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_scf_batch<1>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes);
+                                hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
         if (e != hipSuccess) return e;
         attr_done = true;
     }
@@ -1762,6 +1773,8 @@ hipError_t launch_scf_batch(const BatchArgs& a, int precision, int grid, size_t 
     return hipGetLastError();
 }
 }  // namespace am1g
+
+#ifndef AM1G_WG_VARIANT
 
 // =============================================================================================
 // Grid-level kernels for one large system: core Hamiltonian diagonal blocks, Fock build and
@@ -2860,3 +2873,4 @@ hipError_t launch_dc_update(int64_t npat, const int64_t* pat, int nao, const dou
 }
 
 }  // namespace am1g
+#endif  // AM1G_WG_VARIANT

@@ -9,7 +9,12 @@
 
 namespace am1g {
 
-constexpr int kWg = 512;  // threads per workgroup for the per-molecule SCF (8 wavefronts)
+// Threads per workgroup of the per-molecule SCF: 512 (8 wavefronts); the small-molecule build of
+// the batched kernel compiles the same code with 256 (two molecules per CU).
+#ifndef AM1G_WG_THREADS
+#define AM1G_WG_THREADS 512
+#endif
+constexpr int kWg = AM1G_WG_THREADS;
 constexpr int kWaves = kWg / 64;
 
 struct WgScratch {
@@ -299,7 +304,7 @@ __device__ __noinline__ void wg_gemm_mfma(int M, int N, int K, double alpha, con
             const int ka = (kk + (lane >> 4));
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
-                const int tile = w + 8 * u;
+                const int tile = w + kWaves * u;
                 if (tile < ntile) {
                     const int ti = tile / TN, tj = tile - ti * TN;
                     const T a = As[ka * Mp + ti * 16 + (lane & 15)];
@@ -312,7 +317,7 @@ __device__ __noinline__ void wg_gemm_mfma(int M, int N, int K, double alpha, con
     }
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
-        const int tile = w + 8 * u;
+        const int tile = w + kWaves * u;
         if (tile < ntile) {
             const int ti = tile / TN, tj = tile - ti * TN;
             const int col = tj * 16 + (lane & 15);
@@ -325,13 +330,14 @@ __device__ __noinline__ void wg_gemm_mfma(int M, int N, int K, double alpha, con
     }
 }
 
-// Dispatch: matrix cores when the output has at most 8 x 8 tiles of 16, LDS-staged register
-// tiles when it fits two 4 x 4 tiles per thread, register/global otherwise.
+// Dispatch: matrix cores when the output has at most eight 16 x 16 tiles per wavefront (8 x 8
+// tiles with 8 wavefronts), LDS-staged register tiles when it fits two 4 x 4 tiles per thread
+// (orders up to 128 with 4 wavefronts), register/global otherwise.
 template <typename T, bool TA, bool TB>
 __device__ __forceinline__ void wg_gemm_any(int M, int N, int K, double alpha, const double* A, int lda,
                                             const double* B, int ldb, double* C, int ldc, double* sm,
                                             int sm_doubles) {
-    if (M <= 128 && N <= 128)
+    if (M <= 128 && N <= 128 && ((M + 15) / 16) * ((N + 15) / 16) <= 8 * kWaves)
         wg_gemm_mfma<T, TA, TB>(M, N, K, alpha, A, lda, B, ldb, C, ldc, sm, sm_doubles);
     else if (M <= 128 && N <= 128 && ((M + 3) / 4) * ((N + 3) / 4) <= 2 * kWg)
         wg_gemm_staged<T, TA, TB>(M, N, K, alpha, A, lda, B, ldb, C, ldc, sm, sm_doubles);

@@ -261,6 +261,11 @@ int am1g_matmul(am1g_ctx* ctx, int trans_a, int trans_b, int m, int n, int k, do
  *                      in panels of 32 columns (the trailing matrix read once per column, a
  *                      matrix-core rank-64 update per panel) instead of one column at a time;
  *                      default 2^30 (off)
+ *   batch_small_wg     1: in a batch with at least two molecules of at most 64 basis functions
+ *                      per CU, those run two per CU in a 256-thread build of the batched kernel
+ *                      (its own launch after the larger molecules'); their results then differ
+ *                      from the same molecule's alone in the last bits (another summation order);
+ *                      0: every molecule in the 512-thread build; default 1
  *   sp2                1: the grid-level SCF forms each iteration's density by SP2 purification
  *                      of the extrapolated Fock matrix (a chain of n x n x n products on the
  *                      matrix cores) and diagonalizes once, after the loop, for the orbital

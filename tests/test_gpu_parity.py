@@ -101,6 +101,33 @@ def test_water_cluster_32(engine):
     _compare(rb, o)
 
 
+def test_large_batch_runs_small_molecules_two_per_cu(engine):
+    """A batch with at least two molecules of at most 64 basis functions per CU (MI355X: 256
+    CUs) runs those in the 256-thread build of the batched kernel, two per CU. Its results agree
+    with the 512-thread build's (tunable batch_small_wg = 0) to rounding, differ from them in the
+    last bits (so the split took place), and match the oracle to the contract."""
+    from am1g import geometry as G
+
+    mols = [m for m in G.chain_batch(2000, seed=5) if m.nao <= 64][:640]
+    assert len(mols) == 640
+    ms = [(m.z, m.xyz_bohr) for m in mols]
+    try:
+        engine.set_tunable("batch_small_wg", 0)
+        a = engine.scf_batch(ms)
+        engine.set_tunable("batch_small_wg", 1)
+        b = engine.scf_batch(ms)
+    finally:
+        engine.set_tunable("batch_small_wg", 1)
+    assert all(x["status"] == 0 for x in a + b)
+    de = max(abs(x["total_ev"] - y["total_ev"]) for x, y in zip(a, b))
+    dq = max(np.abs(x["charges"] - y["charges"]).max() for x, y in zip(a, b))
+    assert de <= 1e-9 and dq <= 1e-9
+    assert any(x["total_ev"] != y["total_ev"] or not np.array_equal(x["charges"], y["charges"])
+               for x, y in zip(a, b))
+    for i in range(0, 640, 40):
+        _compare(b[i], O.scf(*ms[i], full=False))
+
+
 def test_batch_matches_oracle_and_is_batch_invariant(engine):
     from am1g import geometry as G
 
