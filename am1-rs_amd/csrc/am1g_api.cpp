@@ -305,6 +305,8 @@ int am1g_create(int device, am1g_ctx** out) {
     am1g_load_tunables(ctx);
     HIP_TRY(ctx, hipEventCreate(&ctx->ev0));
     HIP_TRY(ctx, hipEventCreate(&ctx->ev1));
+    HIP_TRY(ctx, hipEventCreateWithFlags(&ctx->ev2, hipEventDisableTiming));
+    HIP_TRY(ctx, hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking));
     build_element_tables(ctx->host_tab);
     HIP_TRY(ctx, ctx->elems.reserve(sizeof(ctx->host_tab)));
     HIP_TRY(ctx, hipMemcpy(ctx->elems.p, ctx->host_tab, sizeof(ctx->host_tab), hipMemcpyHostToDevice));
@@ -320,6 +322,8 @@ void am1g_destroy(am1g_ctx* ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->ev2) (void)hipEventDestroy(ctx->ev2);
+    if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -582,6 +586,9 @@ static int batch_impl(am1g_ctx* ctx, int n, const am1g_request* rqs, am1g_result
     if (ctx->tun.batch_small_wg)
         for (int j = 0; j < m; ++j) n_small += descs[j].nao <= kSmallWgAo ? 1 : 0;
     if (n_small >= 2 * ctx->cus) {
+        // The small-molecule launch runs on a second stream after everything queued so far
+        // (ev0): its workgroups take each CU that a workgroup of the larger molecules' launch
+        // leaves, so the tail of that launch fills with small molecules.
         const int n_big = m - n_small;
         if (n_big > 0) {
             BatchArgs big = a;
@@ -592,7 +599,10 @@ static int batch_impl(am1g_ctx* ctx, int n, const am1g_request* rqs, am1g_result
         small.order = a.order + n_big;
         small.n_mols = n_small;
         small.queue = a.queue + 1;
-        HIP_TRY(ctx, launch_scf_batch_small(small, precision, std::min(n_small, 2 * ctx->cus), st));
+        HIP_TRY(ctx, hipStreamWaitEvent(ctx->stream2, ctx->ev0, 0));
+        HIP_TRY(ctx, launch_scf_batch_small(small, precision, std::min(n_small, 2 * ctx->cus), ctx->stream2));
+        HIP_TRY(ctx, hipEventRecord(ctx->ev2, ctx->stream2));
+        HIP_TRY(ctx, hipStreamWaitEvent(st, ctx->ev2, 0));
     } else {
         HIP_TRY(ctx, launch_scf_batch(a, precision, grid, lds, st));
     }

@@ -84,7 +84,8 @@ struct am1g_ctx {
     std::string tune_source;  // the table the tunables came from (empty: defaults)
     int cus = 256;
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipStream_t stream2 = nullptr;  // the small-molecule launch of a split batch, beside `stream`
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
     double last_ms = 0.0;
     std::mutex mu;
     DevElem host_tab[kMethods * kMaxZ];
